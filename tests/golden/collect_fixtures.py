@@ -1,0 +1,58 @@
+"""Copy the reference's Newton-Raphson power-flow validation cases into tests/golden/power_flow/.
+
+Run once in the build container (the reference tree is not present on the GPU box):
+    python tests/golden/collect_fixtures.py /root/reference
+A case is taken when its params.json lists newton_raphson and every component it uses is one this
+build supports. The data files (input, update_batch, *_output*.json, params.json) are copied unchanged
+together with their .license sidecars (MPL-2.0).
+"""
+
+import json
+import shutil
+import sys
+from pathlib import Path
+
+SUPPORTED = {"node", "line", "transformer", "shunt", "source", "sym_gen", "asym_gen", "sym_load", "asym_load"}
+FILES = ["input.json", "update_batch.json", "sym_output.json", "asym_output.json", "sym_output_batch.json",
+         "asym_output_batch.json", "params.json"]
+
+
+def components(path):
+    d = json.load(open(path))["data"]
+    comps = set()
+    for sc in d if isinstance(d, list) else [d]:
+        comps |= set(sc)
+    return comps
+
+
+def main(ref_root):
+    src_root = Path(ref_root) / "tests" / "data" / "power_flow"
+    dst_root = Path(__file__).resolve().parent / "power_flow"
+    taken = []
+    for params in sorted(src_root.rglob("params.json")):
+        case = params.parent
+        p = json.load(open(params))
+        methods = p.get("calculation_method")
+        methods = methods if isinstance(methods, list) else [methods]
+        if "newton_raphson" not in methods or "raises" in p:
+            continue
+        comps = set()
+        for f in FILES[:-1]:
+            if (case / f).exists():
+                comps |= components(case / f)
+        if not comps <= SUPPORTED:
+            continue
+        rel = case.relative_to(src_root)
+        dst = dst_root / rel
+        dst.mkdir(parents=True, exist_ok=True)
+        for f in FILES:
+            if (case / f).exists():
+                shutil.copy(case / f, dst / f)
+                if (case / (f + ".license")).exists():
+                    shutil.copy(case / (f + ".license"), dst / (f + ".license"))
+        taken.append(str(rel))
+    print("\n".join(taken))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
