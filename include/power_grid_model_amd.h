@@ -1,0 +1,256 @@
+/*
+ * power_grid_model_amd.h -- C ABI of the MI355X batched Newton-Raphson power-flow library.
+ *
+ * The PGM_* entry points are the drop-in boundary: same names, arguments, ownership and error
+ * behaviour as the reference's C API (power_grid_model_c/power_grid_model_c/include/power_grid_model_c/
+ * *.h), for the power-flow path. Each declaration cites the reference declaration it replaces.
+ * Only plain C types cross this boundary.
+ *
+ * Scope: calculation_type = power flow, calculation_method = default or Newton-Raphson, symmetric or
+ * asymmetric, components node, line, transformer, shunt, source, sym_gen, asym_gen, sym_load,
+ * asym_load. Other requests fail with PGM_regular_error and a message naming what is unsupported.
+ *
+ * PGM_amd_* entry points have no counterpart in the reference: they expose the device-resident batch
+ * (for callers that keep inputs in HBM across calls), precision selection and per-scenario iteration
+ * counts.
+ */
+#ifndef POWER_GRID_MODEL_AMD_H
+#define POWER_GRID_MODEL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+#define PGM_AMD_NOEXCEPT noexcept
+extern "C" {
+#else
+#define PGM_AMD_NOEXCEPT
+#endif
+
+#define PGM_AMD_API __attribute__((visibility("default")))
+
+/* basics.h:47-75 */
+typedef int64_t PGM_Idx;
+typedef int32_t PGM_ID;
+typedef struct PGM_PowerGridModel PGM_PowerGridModel;
+typedef struct PGM_Handle PGM_Handle;
+typedef struct PGM_Options PGM_Options;
+typedef struct PGM_MetaAttribute PGM_MetaAttribute;
+typedef struct PGM_MetaComponent PGM_MetaComponent;
+typedef struct PGM_MetaDataset PGM_MetaDataset;
+typedef struct PGM_ConstDataset PGM_ConstDataset;
+typedef struct PGM_MutableDataset PGM_MutableDataset;
+typedef struct PGM_WritableDataset PGM_WritableDataset;
+typedef struct PGM_DatasetInfo PGM_DatasetInfo;
+
+/* basics.h:80-140 */
+enum PGM_CalculationType { PGM_power_flow = 0, PGM_state_estimation = 1, PGM_short_circuit = 2 };
+enum PGM_CalculationMethod {
+    PGM_default_method = -128,
+    PGM_linear = 0,
+    PGM_newton_raphson = 1,
+    PGM_iterative_linear = 2,
+    PGM_iterative_current = 3,
+    PGM_linear_current = 4,
+    PGM_iec60909 = 5
+};
+enum PGM_SymmetryType { PGM_asymmetric = 0, PGM_symmetric = 1 };
+enum PGM_ErrorCode { PGM_no_error = 0, PGM_regular_error = 1, PGM_batch_error = 2, PGM_serialization_error = 3 };
+enum PGM_CType { PGM_int32 = 0, PGM_int8 = 1, PGM_double = 2, PGM_double3 = 3 };
+
+/* handle.h:27-100 */
+PGM_AMD_API PGM_Handle* PGM_create_handle(void) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_destroy_handle(PGM_Handle* handle) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Idx PGM_error_code(PGM_Handle const* handle) PGM_AMD_NOEXCEPT;
+PGM_AMD_API char const* PGM_error_message(PGM_Handle const* handle) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Idx PGM_n_failed_scenarios(PGM_Handle const* handle) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Idx const* PGM_failed_scenarios(PGM_Handle const* handle) PGM_AMD_NOEXCEPT;
+PGM_AMD_API char const** PGM_batch_errors(PGM_Handle const* handle) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_clear_error(PGM_Handle* handle) PGM_AMD_NOEXCEPT;
+PGM_AMD_API char const* PGM_version(void) PGM_AMD_NOEXCEPT;
+
+/* options.h:25-120 */
+PGM_AMD_API PGM_Options* PGM_create_options(PGM_Handle* handle) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_destroy_options(PGM_Options* opt) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_set_calculation_type(PGM_Handle* handle, PGM_Options* opt, PGM_Idx type) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_set_calculation_method(PGM_Handle* handle, PGM_Options* opt,
+                                            PGM_Idx method) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_set_symmetric(PGM_Handle* handle, PGM_Options* opt, PGM_Idx sym) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_set_err_tol(PGM_Handle* handle, PGM_Options* opt, double err_tol) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_set_max_iter(PGM_Handle* handle, PGM_Options* opt, PGM_Idx max_iter) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_set_threading(PGM_Handle* handle, PGM_Options* opt, PGM_Idx threading) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_set_short_circuit_voltage_scaling(PGM_Handle* handle, PGM_Options* opt,
+                                                       PGM_Idx short_circuit_voltage_scaling) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_set_tap_changing_strategy(PGM_Handle* handle, PGM_Options* opt,
+                                               PGM_Idx tap_changing_strategy) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_set_experimental_features(PGM_Handle* handle, PGM_Options* opt,
+                                               PGM_Idx experimental_features) PGM_AMD_NOEXCEPT;
+
+/* meta_data.h:32-190 */
+PGM_AMD_API PGM_Idx PGM_meta_n_datasets(PGM_Handle* handle) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_MetaDataset const* PGM_meta_get_dataset_by_idx(PGM_Handle* handle, PGM_Idx idx) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_MetaDataset const* PGM_meta_get_dataset_by_name(PGM_Handle* handle,
+                                                                 char const* dataset) PGM_AMD_NOEXCEPT;
+PGM_AMD_API char const* PGM_meta_dataset_name(PGM_Handle* handle, PGM_MetaDataset const* dataset) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Idx PGM_meta_n_components(PGM_Handle* handle, PGM_MetaDataset const* dataset) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_MetaComponent const* PGM_meta_get_component_by_idx(PGM_Handle* handle,
+                                                                   PGM_MetaDataset const* dataset,
+                                                                   PGM_Idx idx) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_MetaComponent const* PGM_meta_get_component_by_name(PGM_Handle* handle, char const* dataset,
+                                                                    char const* component) PGM_AMD_NOEXCEPT;
+PGM_AMD_API char const* PGM_meta_component_name(PGM_Handle* handle,
+                                                PGM_MetaComponent const* component) PGM_AMD_NOEXCEPT;
+PGM_AMD_API size_t PGM_meta_component_size(PGM_Handle* handle, PGM_MetaComponent const* component) PGM_AMD_NOEXCEPT;
+PGM_AMD_API size_t PGM_meta_component_alignment(PGM_Handle* handle,
+                                                PGM_MetaComponent const* component) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Idx PGM_meta_n_attributes(PGM_Handle* handle, PGM_MetaComponent const* component) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_MetaAttribute const* PGM_meta_get_attribute_by_idx(PGM_Handle* handle,
+                                                                   PGM_MetaComponent const* component,
+                                                                   PGM_Idx idx) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_MetaAttribute const* PGM_meta_get_attribute_by_name(PGM_Handle* handle, char const* dataset,
+                                                                    char const* component,
+                                                                    char const* attribute) PGM_AMD_NOEXCEPT;
+PGM_AMD_API char const* PGM_meta_attribute_name(PGM_Handle* handle,
+                                                PGM_MetaAttribute const* attribute) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Idx PGM_meta_attribute_ctype(PGM_Handle* handle, PGM_MetaAttribute const* attribute) PGM_AMD_NOEXCEPT;
+PGM_AMD_API size_t PGM_meta_attribute_offset(PGM_Handle* handle, PGM_MetaAttribute const* attribute) PGM_AMD_NOEXCEPT;
+PGM_AMD_API int PGM_is_little_endian(PGM_Handle* handle) PGM_AMD_NOEXCEPT;
+
+/* buffer.h:24-118 */
+PGM_AMD_API void* PGM_create_buffer(PGM_Handle* handle, PGM_MetaComponent const* component,
+                                    PGM_Idx size) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_destroy_buffer(void* ptr) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_buffer_set_nan(PGM_Handle* handle, PGM_MetaComponent const* component, void* ptr,
+                                    PGM_Idx buffer_offset, PGM_Idx size) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_buffer_set_value(PGM_Handle* handle, PGM_MetaAttribute const* attribute, void* buffer_ptr,
+                                      void const* src_ptr, PGM_Idx buffer_offset, PGM_Idx size,
+                                      PGM_Idx src_stride) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_buffer_get_value(PGM_Handle* handle, PGM_MetaAttribute const* attribute,
+                                      void const* buffer_ptr, void* dest_ptr, PGM_Idx buffer_offset, PGM_Idx size,
+                                      PGM_Idx dest_stride) PGM_AMD_NOEXCEPT;
+
+/* dataset.h:27-333 */
+PGM_AMD_API char const* PGM_dataset_info_name(PGM_Handle* handle, PGM_DatasetInfo const* info) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Idx PGM_dataset_info_is_batch(PGM_Handle* handle, PGM_DatasetInfo const* info) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Idx PGM_dataset_info_batch_size(PGM_Handle* handle, PGM_DatasetInfo const* info) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Idx PGM_dataset_info_n_components(PGM_Handle* handle,
+                                                  PGM_DatasetInfo const* info) PGM_AMD_NOEXCEPT;
+PGM_AMD_API char const* PGM_dataset_info_component_name(PGM_Handle* handle, PGM_DatasetInfo const* info,
+                                                        PGM_Idx component_idx) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Idx PGM_dataset_info_elements_per_scenario(PGM_Handle* handle, PGM_DatasetInfo const* info,
+                                                           PGM_Idx component_idx) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Idx PGM_dataset_info_total_elements(PGM_Handle* handle, PGM_DatasetInfo const* info,
+                                                    PGM_Idx component_idx) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Idx PGM_dataset_info_has_attribute_indications(PGM_Handle* handle, PGM_DatasetInfo const* info,
+                                                               PGM_Idx component_idx) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Idx PGM_dataset_info_n_attribute_indications(PGM_Handle* handle, PGM_DatasetInfo const* info,
+                                                             PGM_Idx component_idx) PGM_AMD_NOEXCEPT;
+PGM_AMD_API char const* PGM_dataset_info_attribute_name(PGM_Handle* handle, PGM_DatasetInfo const* info,
+                                                        PGM_Idx component_idx,
+                                                        PGM_Idx attribute_idx) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_ConstDataset* PGM_create_dataset_const(PGM_Handle* handle, char const* dataset, PGM_Idx is_batch,
+                                                       PGM_Idx batch_size) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_ConstDataset* PGM_create_dataset_const_from_mutable(
+    PGM_Handle* handle, PGM_MutableDataset const* mutable_dataset) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_destroy_dataset_const(PGM_ConstDataset* dataset) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_dataset_const_add_buffer(PGM_Handle* handle, PGM_ConstDataset* dataset, char const* component,
+                                              PGM_Idx elements_per_scenario, PGM_Idx total_elements,
+                                              PGM_Idx const* indptr, void const* data) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_dataset_const_add_attribute_buffer(PGM_Handle* handle, PGM_ConstDataset* dataset,
+                                                        char const* component, char const* attribute,
+                                                        void const* data) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_DatasetInfo const* PGM_dataset_const_get_info(PGM_Handle* handle,
+                                                              PGM_ConstDataset const* dataset) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_dataset_const_set_next_cartesian_product_dimension(
+    PGM_Handle* handle, PGM_ConstDataset* dataset, PGM_ConstDataset const* next_dataset) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_MutableDataset* PGM_create_dataset_mutable(PGM_Handle* handle, char const* dataset,
+                                                           PGM_Idx is_batch, PGM_Idx batch_size) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_destroy_dataset_mutable(PGM_MutableDataset* dataset) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_dataset_mutable_add_buffer(PGM_Handle* handle, PGM_MutableDataset* dataset,
+                                                char const* component, PGM_Idx elements_per_scenario,
+                                                PGM_Idx total_elements, PGM_Idx const* indptr,
+                                                void* data) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_dataset_mutable_add_attribute_buffer(PGM_Handle* handle, PGM_MutableDataset* dataset,
+                                                          char const* component, char const* attribute,
+                                                          void* data) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_DatasetInfo const* PGM_dataset_mutable_get_info(PGM_Handle* handle,
+                                                                PGM_MutableDataset const* dataset) PGM_AMD_NOEXCEPT;
+
+/* model.h:33-130 */
+PGM_AMD_API PGM_PowerGridModel* PGM_create_model(PGM_Handle* handle, double system_frequency,
+                                                 PGM_ConstDataset const* input_dataset) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_update_model(PGM_Handle* handle, PGM_PowerGridModel* model,
+                                  PGM_ConstDataset const* update_dataset) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_PowerGridModel* PGM_copy_model(PGM_Handle* handle, PGM_PowerGridModel const* model) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_get_indexer(PGM_Handle* handle, PGM_PowerGridModel const* model, char const* component,
+                                 PGM_Idx size, PGM_ID const* ids, PGM_Idx* indexer) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_calculate(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_Options const* opt,
+                               PGM_MutableDataset const* output_dataset,
+                               PGM_ConstDataset const* batch_dataset) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_destroy_model(PGM_PowerGridModel* model) PGM_AMD_NOEXCEPT;
+
+/* ---------- extensions (no counterpart in the reference) ---------- */
+
+/* Factorisation precision of the Newton-Raphson linear solves: 0 = FP64 (default, plain path),
+ * 1 = FP32 factor and triangular solves with FP64 mismatch and state update (mixed path). */
+PGM_AMD_API void PGM_amd_set_factor_precision(PGM_Handle* handle, PGM_Options* opt,
+                                              PGM_Idx precision) PGM_AMD_NOEXCEPT;
+
+/* Device-resident batch: build the batch (topology grouping, plans, device upload of every
+ * scenario's update data) once, then run it any number of times with the inputs already in HBM.
+ * PGM_calculate is build + run + fetch + destroy. Returns NULL on error (message in handle). */
+typedef struct PGM_AmdBatch PGM_AmdBatch;
+PGM_AMD_API PGM_AmdBatch* PGM_amd_batch_create(PGM_Handle* handle, PGM_PowerGridModel const* model,
+                                               PGM_Options const* opt,
+                                               PGM_ConstDataset const* batch_dataset) PGM_AMD_NOEXCEPT;
+/* Run every scenario to convergence on the device; returns when the device work is complete. */
+PGM_AMD_API void PGM_amd_batch_run(PGM_Handle* handle, PGM_AmdBatch* batch) PGM_AMD_NOEXCEPT;
+/* Copy results into the caller's output dataset and report per-scenario errors like PGM_calculate. */
+PGM_AMD_API void PGM_amd_batch_fetch(PGM_Handle* handle, PGM_AmdBatch* batch,
+                                     PGM_MutableDataset const* output_dataset) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_amd_batch_destroy(PGM_AmdBatch* batch) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Idx PGM_amd_batch_n_scenarios(PGM_AmdBatch const* batch) PGM_AMD_NOEXCEPT;
+/* Per-scenario Newton-Raphson iteration count of the last run (max over the scenario's math models;
+ * 0 for failed scenarios). iterations must hold n_scenarios entries. */
+PGM_AMD_API void PGM_amd_batch_iterations(PGM_AmdBatch const* batch, PGM_Idx* iterations) PGM_AMD_NOEXCEPT;
+/* Statistics of the last run: out[0] = device time of the NR kernels in ms (HIP events),
+ * out[1] = sum over scenarios of iterations, out[2] = number of bus blocks factorised per scenario
+ * (sum over groups), out[3] = LU entries per scenario, out[4] = number of topology groups,
+ * out[5] = average duration in ms of one forward-sweep launch, out[6] = number of forward-sweep
+ * launches, out[7] = buses per scenario. */
+PGM_AMD_API void PGM_amd_batch_stats(PGM_AmdBatch const* batch, double* out) PGM_AMD_NOEXCEPT;
+
+/* Host-only inspection of the product's topology for a model (no GPU needed): per node the math
+ * model index (-1 if not energised) and the bus position; fills receives (model, bus_a, bus_b) triples.
+ * Returns the number of fill-ins, or -1 if max_fill is too small. */
+PGM_AMD_API PGM_Idx PGM_amd_inspect_topology(PGM_Handle* handle, PGM_PowerGridModel const* model,
+                                             PGM_Idx* node_group, PGM_Idx* node_pos, PGM_Idx* fills,
+                                             PGM_Idx max_fill) PGM_AMD_NOEXCEPT;
+
+/* Host-only statistics and validity check of the device factorisation schedule of a model's base
+ * topology (no GPU needed). out[0] buses, out[1] LU entries, out[2] levels, out[3] tasks, out[4] Schur
+ * updates, out[5] critical path in pivots (sum over levels of the longest task), out[6] 1 when every
+ * dependency of every pivot is scheduled before it, else 0. */
+PGM_AMD_API void PGM_amd_plan_stats(PGM_Handle* handle, PGM_PowerGridModel const* model, PGM_Idx symmetric,
+                                    PGM_Idx target_parallel_tasks, PGM_Idx* out) PGM_AMD_NOEXCEPT;
+
+/* Host-only minimum-degree ordering on an adjacency map given as sorted keys + CSR neighbour lists
+ * (same contract as sparse_ordering.hpp:183 minimum_degree_ordering). Returns the number of fills or -1. */
+PGM_AMD_API PGM_Idx PGM_amd_minimum_degree_ordering(PGM_Idx n_keys, PGM_Idx const* keys, PGM_Idx const* indptr,
+                                                    PGM_Idx const* neighbours, PGM_Idx* alpha_out,
+                                                    PGM_Idx* n_alpha_out, PGM_Idx* fills_out,
+                                                    PGM_Idx max_fills) PGM_AMD_NOEXCEPT;
+
+/* GPU block-sparse LU factorise + solve of `batch` independent systems sharing one pattern (test hook
+ * for the device LU). nb in {2, 6}; data: batch x nnz blocks (column-major nb*nb), rhs/x: batch x n x nb.
+ * precision 0 = FP64, 1 = FP32. status[batch]: 0 ok, 1 singular. Returns 0, or -1 on a launch error. */
+PGM_AMD_API int PGM_amd_block_lu_solve(PGM_Idx nb, PGM_Idx n, PGM_Idx const* row_indptr,
+                                       PGM_Idx const* col_indices, PGM_Idx const* diag_lu, PGM_Idx batch,
+                                       double const* data, double const* rhs, double* x, int32_t* status,
+                                       PGM_Idx precision) PGM_AMD_NOEXCEPT;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,252 @@
+// capi_model.cpp -- model and calculation entry points of the C ABI (model.h of the reference,
+// power_grid_model_c/power_grid_model_c/src/model.cpp:48-359), plus the PGM_amd_* extensions.
+#include "engine.hpp"
+
+#include "../../include/power_grid_model_amd.h"
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+using namespace pgm_amd;
+
+namespace {
+void record_error(PGM_Handle* h, char const* msg) {
+    if (!h) return;
+    h->err_code = PGM_regular_error;
+    h->err_msg = msg;
+}
+template <class F> void guarded(PGM_Handle* h, F&& f) {
+    if (h) h->clear();
+    try {
+        f();
+    } catch (std::exception const& e) {
+        record_error(h, e.what());
+    } catch (...) {
+        record_error(h, "Unknown error!\n");
+    }
+}
+}  // namespace
+
+extern "C" {
+
+PGM_PowerGridModel* PGM_create_model(PGM_Handle* handle, double system_frequency,
+                                     PGM_ConstDataset const* input_dataset) noexcept {
+    PGM_PowerGridModel* out = nullptr;
+    guarded(handle, [&] {
+        if (input_dataset->meta->name != "input") throw PowerGridError("input dataset must be of type input\n");
+        if (input_dataset->is_batch) throw PowerGridError("input dataset cannot be a batch\n");
+        auto* m = new PGM_PowerGridModel{};
+        try {
+            m->model = build_model(*input_dataset, system_frequency);
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        out = m;
+    });
+    return out;
+}
+
+void PGM_update_model(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_ConstDataset const* update) noexcept {
+    guarded(handle, [&] {
+        if (update->meta->name != "update") throw PowerGridError("update dataset must be of type update\n");
+        Model& m = model->model;
+        ComponentState st = m.base;
+        std::vector<char> rec;
+        for (auto const& buf : update->buffers) {
+            int comp = -1;
+            for (int c = 0; c < N_COMP; ++c)
+                if (buf.meta->name == comp_names[c]) comp = c;
+            rec.resize(buf.meta->size);
+            for (Idx i = buf.begin(0); i < buf.end(0); ++i) {
+                buf.read_record(i, rec.data());
+                ID id;
+                std::memcpy(&id, rec.data(), 4);
+                Idx const idx = id == na_IntID ? i - buf.begin(0) : find_component(m, comp, id);
+                bool structural = false;
+                apply_update_record(m, st, comp, idx, rec.data(), structural);
+            }
+        }
+        m.base = st;
+    });
+}
+
+PGM_PowerGridModel* PGM_copy_model(PGM_Handle* handle, PGM_PowerGridModel const* model) noexcept {
+    PGM_PowerGridModel* out = nullptr;
+    guarded(handle, [&] { out = new PGM_PowerGridModel{*model}; });
+    return out;
+}
+
+void PGM_get_indexer(PGM_Handle* handle, PGM_PowerGridModel const* model, char const* component, PGM_Idx size,
+                     PGM_ID const* ids, PGM_Idx* indexer) noexcept {
+    guarded(handle, [&] {
+        int comp = -1;
+        for (int c = 0; c < N_COMP; ++c)
+            if (std::strcmp(component, comp_names[c]) == 0) comp = c;
+        if (comp < 0) throw PowerGridError(std::string("unknown component ") + component);
+        for (PGM_Idx i = 0; i < size; ++i) indexer[i] = find_component(model->model, comp, ids[i]);
+    });
+}
+
+void PGM_calculate(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_Options const* opt,
+                   PGM_MutableDataset const* output_dataset, PGM_ConstDataset const* batch_dataset) noexcept {
+    guarded(handle, [&] {
+        if (batch_dataset && (!batch_dataset->is_batch || !output_dataset->is_batch))
+            throw PowerGridError(
+                "If batch_dataset is provided. Both batch_dataset and output_dataset should be a batch!\n");
+        if (batch_dataset && batch_dataset->batch_size == 0) return;
+        auto batch = create_batch(model->model, *opt, batch_dataset);
+        batch->run();
+        batch->fetch(*output_dataset, handle);
+    });
+}
+
+void PGM_destroy_model(PGM_PowerGridModel* model) noexcept { delete model; }
+
+PGM_AmdBatch* PGM_amd_batch_create(PGM_Handle* handle, PGM_PowerGridModel const* model, PGM_Options const* opt,
+                                   PGM_ConstDataset const* batch_dataset) noexcept {
+    PGM_AmdBatch* out = nullptr;
+    guarded(handle, [&] {
+        auto b = create_batch(model->model, *opt, batch_dataset);
+        out = new PGM_AmdBatch{std::move(b)};
+    });
+    return out;
+}
+void PGM_amd_batch_run(PGM_Handle* handle, PGM_AmdBatch* batch) noexcept {
+    guarded(handle, [&] { batch->b->run(); });
+}
+void PGM_amd_batch_fetch(PGM_Handle* handle, PGM_AmdBatch* batch, PGM_MutableDataset const* output) noexcept {
+    guarded(handle, [&] { batch->b->fetch(*output, handle); });
+}
+void PGM_amd_batch_destroy(PGM_AmdBatch* batch) noexcept { delete batch; }
+PGM_Idx PGM_amd_batch_n_scenarios(PGM_AmdBatch const* batch) noexcept { return batch->b->n_scenarios; }
+void PGM_amd_batch_iterations(PGM_AmdBatch const* batch, PGM_Idx* iterations) noexcept {
+    std::copy(batch->b->iterations.begin(), batch->b->iterations.end(), iterations);
+}
+void PGM_amd_batch_stats(PGM_AmdBatch const* batch, double* out) noexcept {
+    auto const& b = *batch->b;
+    double it_sum = 0;
+    for (auto x : b.iterations) it_sum += double(x);
+    double buses = 0, entries = 0;
+    batch_plan_sizes(b, buses, entries);
+    out[0] = b.kernel_ms;
+    out[1] = it_sum;
+    out[2] = buses;
+    out[3] = entries;
+    out[4] = double(b.groups.size());
+    out[5] = b.fwd_launch_ms;
+    out[6] = double(b.fwd_launches);
+    out[7] = buses;
+}
+
+PGM_Idx PGM_amd_inspect_topology(PGM_Handle* handle, PGM_PowerGridModel const* model, PGM_Idx* node_group,
+                                 PGM_Idx* node_pos, PGM_Idx* fills, PGM_Idx max_fill) noexcept {
+    PGM_Idx result = -1;
+    guarded(handle, [&] {
+        Model const& m = model->model;
+        Topology const t = build_topology(m, m.base);
+        for (Idx i = 0; i < m.n_node(); ++i) {
+            node_group[i] = t.node_group[i];
+            node_pos[i] = t.node_pos[i];
+        }
+        PGM_Idx k = 0;
+        for (size_t g = 0; g < t.models.size(); ++g)
+            for (auto const& f : t.models[g].fill_in) {
+                if (k >= max_fill) {
+                    result = -1;
+                    return;
+                }
+                fills[3 * k] = PGM_Idx(g);
+                fills[3 * k + 1] = f[0];
+                fills[3 * k + 2] = f[1];
+                ++k;
+            }
+        result = k;
+    });
+    return result;
+}
+
+void PGM_amd_plan_stats(PGM_Handle* handle, PGM_PowerGridModel const* model, PGM_Idx symmetric,
+                        PGM_Idx target_parallel_tasks, PGM_Idx* out) noexcept {
+    guarded(handle, [&] {
+        Model const& m = model->model;
+        Topology const t = build_topology(m, m.base);
+        std::vector<Idx> all(t.models.size());
+        for (size_t k = 0; k < all.size(); ++k) all[k] = Idx(k);
+        Plan const p = build_plan(m, m.base, t, all, symmetric != 0, int(target_parallel_tasks));
+        // position of every pivot: (level, task, index within the task)
+        std::vector<int> level(p.n, -1), task(p.n, -1), pos(p.n, -1);
+        Idx crit = 0;
+        for (size_t l = 0; l + 1 < p.level_ptr.size(); ++l) {
+            Idx longest = 0;
+            for (int q = p.level_ptr[l]; q < p.level_ptr[l + 1]; ++q) {
+                int const tk = p.level_task[q];
+                longest = std::max<Idx>(longest, p.task_ptr[tk + 1] - p.task_ptr[tk]);
+                for (int i = p.task_ptr[tk]; i < p.task_ptr[tk + 1]; ++i) {
+                    level[p.task_pivot[i]] = int(l);
+                    task[p.task_pivot[i]] = tk;
+                    pos[p.task_pivot[i]] = i;
+                }
+            }
+            crit += longest;
+        }
+        auto before = [&](int a, int b) {  // pivot a is complete before pivot b starts
+            if (level[a] < 0 || level[b] < 0) return false;
+            if (level[a] < level[b]) return true;
+            return task[a] == task[b] && pos[a] < pos[b];
+        };
+        bool valid = true;
+        auto row_of = [&](int e) {
+            return int(std::upper_bound(p.row_ptr.begin(), p.row_ptr.end(), e) - p.row_ptr.begin()) - 1;
+        };
+        for (Idx r = 0; r < p.n; ++r) {
+            for (int k = p.row_ptr[r]; k < p.diag[r]; ++k) valid = valid && before(p.col[k], int(r));
+            for (int q = p.schur_ptr[r]; q < p.schur_ptr[r + 1]; ++q) {
+                int const le = p.schur[q][1], ue = p.schur[q][2];
+                int const k = row_of(ue);  // the U entry (k, j) lives in row k, its pivot
+                valid = valid && before(k, int(r)) && p.col[le] == k;
+            }
+        }
+        out[0] = p.n;
+        out[1] = p.E;
+        out[2] = Idx(p.level_ptr.size()) - 1;
+        out[3] = Idx(p.task_mm.size());
+        out[4] = Idx(p.schur.size());
+        out[5] = crit;
+        out[6] = valid ? 1 : 0;
+    });
+}
+
+PGM_Idx PGM_amd_minimum_degree_ordering(PGM_Idx n_keys, PGM_Idx const* keys, PGM_Idx const* indptr,
+                                        PGM_Idx const* neighbours, PGM_Idx* alpha_out, PGM_Idx* n_alpha_out,
+                                        PGM_Idx* fills_out, PGM_Idx max_fills) noexcept {
+    try {
+        std::vector<Idx> k(keys, keys + n_keys);
+        std::vector<std::vector<Idx>> adj(n_keys);
+        for (PGM_Idx i = 0; i < n_keys; ++i) adj[i].assign(neighbours + indptr[i], neighbours + indptr[i + 1]);
+        auto [alpha, fl] = minimum_degree(k, adj);
+        std::copy(alpha.begin(), alpha.end(), alpha_out);
+        *n_alpha_out = PGM_Idx(alpha.size());
+        if (PGM_Idx(fl.size()) > max_fills) return -1;
+        for (size_t i = 0; i < fl.size(); ++i) {
+            fills_out[2 * i] = fl[i].first;
+            fills_out[2 * i + 1] = fl[i].second;
+        }
+        return PGM_Idx(fl.size());
+    } catch (...) {
+        return -1;
+    }
+}
+
+int PGM_amd_block_lu_solve(PGM_Idx nb, PGM_Idx n, PGM_Idx const* row_indptr, PGM_Idx const* col_indices,
+                           PGM_Idx const* diag_lu, PGM_Idx batch, double const* data, double const* rhs, double* x,
+                           int32_t* status, PGM_Idx precision) noexcept {
+    try {
+        return device_block_lu(nb, n, row_indptr, col_indices, diag_lu, batch, data, rhs, x, status, precision == 1);
+    } catch (...) {
+        return -1;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,1034 @@
+// kernels.hip -- gfx950 kernels of the batched Newton-Raphson power flow.
+//
+// One wave = 64 scenarios of one group running the same task (a connected piece of the elimination
+// tree). A lane walks the task's pivots in increasing order (forward sweep) or decreasing order
+// (backward sweep); every per-scenario access is a coalesced 64-lane load/store because all
+// per-scenario arrays are scenario-interleaved. The pattern, the admittances and the schedule are
+// the same for every lane and are read through the scalar cache.
+//
+// Math restated from the reference (paths relative to power_grid_model_c/power_grid_model/include/
+// power_grid_model/math_solver/):
+//   linear initial guess     newton_raphson_pf_solver.hpp:255-303, 706-762, 856-870
+//   Jacobian and mismatch    newton_raphson_pf_solver.hpp:462-547, 764-852
+//   full-pivot block LU      sparse_lu_solver.hpp:85-165 (first maximum of |a|^2 in column-major
+//                            order, SparseMatrixError when |u_ii| < eps * max pivot or not normal)
+//   sparse LU + solves       sparse_lu_solver.hpp:346-495, 769-827 (right-looking there; here the
+//                            same updates are pulled by the pivot that owns each entry, in the same
+//                            increasing elimination order)
+//   voltage update           newton_raphson_pf_solver.hpp:325-349
+//   results                  y_bus.hpp:482-546, common_solver_functions.hpp:80-160, 383-446
+#include "kernels.hpp"
+#include "records.hpp"
+
+#include <cfloat>
+
+namespace pgm_amd {
+
+namespace {
+
+constexpr int kWave = 64;
+
+struct cplx {
+    double re, im;
+};
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ cplx cconj(cplx a) { return {a.re, -a.im}; }
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
+    double const d = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+__device__ __forceinline__ double cabs_(cplx a) { return sqrt(a.re * a.re + a.im * a.im); }
+
+template <class FT> __device__ __forceinline__ bool is_normal_(FT x) {
+    FT const ax = x < FT(0) ? -x : x;
+    FT const mn = sizeof(FT) == 8 ? FT(DBL_MIN) : FT(FLT_MIN);
+    FT const mx = sizeof(FT) == 8 ? FT(DBL_MAX) : FT(FLT_MAX);
+    return ax >= mn && ax <= mx;  // false for 0, subnormal, inf, nan
+}
+
+// ---------------- dense NB x NB blocks, column-major a[c * NB + r] ----------------
+template <int NB, class T> __device__ __forceinline__ void swap_rows(T (&a)[NB * NB], int t, int r_sel) {
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+        if (r <= t) continue;
+        bool const sw = r_sel == r;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            T const x = a[c * NB + t], y = a[c * NB + r];
+            a[c * NB + t] = sw ? y : x;
+            a[c * NB + r] = sw ? x : y;
+        }
+    }
+}
+template <int NB, class T> __device__ __forceinline__ void swap_cols(T (&a)[NB * NB], int t, int c_sel) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        if (c <= t) continue;
+        bool const sw = c_sel == c;
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            T const x = a[t * NB + r], y = a[c * NB + r];
+            a[t * NB + r] = sw ? y : x;
+            a[c * NB + r] = sw ? x : y;
+        }
+    }
+}
+__device__ __forceinline__ int perm_row(uint32_t code, int t) { return int((code >> (3 * t)) & 7u); }
+__device__ __forceinline__ int perm_col(uint32_t code, int t) { return int((code >> (15 + 3 * t)) & 7u); }
+
+// full-pivot LU in place: P a Q = L U; returns false when the reference would raise SparseMatrixError
+template <int NB, class FT> __device__ __forceinline__ bool factor_block(FT (&a)[NB * NB], uint32_t& code) {
+    FT max_pivot = FT(0);
+    bool zero = false;
+    code = 0;
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+        FT best = FT(-1);
+        int rb = t, cb = t;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            if (c < t) continue;
+#pragma unroll
+            for (int r = 0; r < NB; ++r) {
+                if (r < t) continue;
+                FT const v = a[c * NB + r] * a[c * NB + r];
+                bool const gt = v > best;
+                best = gt ? v : best;
+                rb = gt ? r : rb;
+                cb = gt ? c : cb;
+            }
+        }
+        zero = zero || best == FT(0);
+        max_pivot = sqrt(best) > max_pivot ? sqrt(best) : max_pivot;
+        if (t < NB - 1) {
+            code |= uint32_t(rb) << (3 * t);
+            code |= uint32_t(cb) << (15 + 3 * t);
+            swap_rows<NB, FT>(a, t, rb);
+            swap_cols<NB, FT>(a, t, cb);
+            FT const piv = a[t * NB + t];
+#pragma unroll
+            for (int r = 0; r < NB; ++r)
+                if (r > t) a[t * NB + r] /= piv;
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                if (c <= t) continue;
+#pragma unroll
+                for (int r = 0; r < NB; ++r)
+                    if (r > t) a[c * NB + r] -= a[t * NB + r] * a[c * NB + t];
+            }
+        }
+    }
+    FT const eps = sizeof(FT) == 8 ? FT(DBL_EPSILON) : FT(FLT_EPSILON);
+    FT const thr = eps * max_pivot;
+    bool ok = !zero;
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+        FT const d = a[t * NB + t];
+        FT const ad = d < FT(0) ? -d : d;
+        ok = ok && !(ad < thr) && is_normal_(d);
+    }
+    return ok;
+}
+template <int NB, class T> __device__ __forceinline__ void apply_p_rows(T (&m)[NB * NB], uint32_t code) {
+#pragma unroll
+    for (int t = 0; t < NB - 1; ++t) swap_rows<NB, T>(m, t, perm_row(code, t));
+}
+template <int NB, class T> __device__ __forceinline__ void apply_q_cols(T (&m)[NB * NB], uint32_t code) {
+#pragma unroll
+    for (int t = 0; t < NB - 1; ++t) swap_cols<NB, T>(m, t, perm_col(code, t));
+}
+template <int NB> __device__ __forceinline__ void apply_p_vec(double (&v)[NB], uint32_t code) {
+#pragma unroll
+    for (int t = 0; t < NB - 1; ++t) {
+        int const r = perm_row(code, t);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            if (k <= t) continue;
+            bool const sw = r == k;
+            double const x = v[t], y = v[k];
+            v[t] = sw ? y : x;
+            v[k] = sw ? x : y;
+        }
+    }
+}
+template <int NB> __device__ __forceinline__ void apply_q_vec(double (&v)[NB], uint32_t code) {
+#pragma unroll
+    for (int t = NB - 2; t >= 0; --t) {
+        int const c = perm_col(code, t);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            if (k <= t) continue;
+            bool const sw = c == k;
+            double const x = v[t], y = v[k];
+            v[t] = sw ? y : x;
+            v[k] = sw ? x : y;
+        }
+    }
+}
+// m := L^-1 m, L unit lower in lu
+template <int NB, class FT> __device__ __forceinline__ void lower_left(FT const (&lu)[NB * NB], FT (&m)[NB * NB]) {
+#pragma unroll
+    for (int r = 1; r < NB; ++r)
+#pragma unroll
+        for (int k = 0; k < r; ++k) {
+            FT const l = lu[k * NB + r];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) m[c * NB + r] -= l * m[c * NB + k];
+        }
+}
+// m := m U^-1
+template <int NB, class FT> __device__ __forceinline__ void upper_right(FT const (&lu)[NB * NB], FT (&m)[NB * NB]) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+#pragma unroll
+        for (int k = 0; k < c; ++k) {
+            FT const u = lu[c * NB + k];
+#pragma unroll
+            for (int r = 0; r < NB; ++r) m[c * NB + r] -= u * m[k * NB + r];
+        }
+        FT const d = lu[c * NB + c];
+#pragma unroll
+        for (int r = 0; r < NB; ++r) m[c * NB + r] /= d;
+    }
+}
+template <int NB, class FT> __device__ __forceinline__ void lower_vec(FT const (&lu)[NB * NB], double (&v)[NB]) {
+#pragma unroll
+    for (int r = 1; r < NB; ++r)
+#pragma unroll
+        for (int k = 0; k < r; ++k) v[r] -= double(lu[k * NB + r]) * v[k];
+}
+template <int NB, class FT> __device__ __forceinline__ void upper_vec(FT const (&lu)[NB * NB], double (&v)[NB]) {
+#pragma unroll
+    for (int r = NB - 1; r >= 0; --r) {
+#pragma unroll
+        for (int k = NB - 1; k > r; --k) v[r] -= double(lu[k * NB + r]) * v[k];
+        v[r] /= double(lu[r * NB + r]);
+    }
+}
+
+template <int NB, class FT> struct Blocks {
+    FT* base;
+    size_t sp;
+    int s;
+    __device__ __forceinline__ FT* ptr(int e, int k) const { return base + ((size_t(e) * (NB * NB) + k) * sp + s); }
+    __device__ __forceinline__ void load(int e, FT (&m)[NB * NB]) const {
+#pragma unroll
+        for (int k = 0; k < NB * NB; ++k) m[k] = *ptr(e, k);
+    }
+    __device__ __forceinline__ void store(int e, FT const (&m)[NB * NB]) const {
+#pragma unroll
+        for (int k = 0; k < NB * NB; ++k) *ptr(e, k) = m[k];
+    }
+    __device__ __forceinline__ void store_zero(int e) const {
+#pragma unroll
+        for (int k = 0; k < NB * NB; ++k) *ptr(e, k) = FT(0);
+    }
+};
+
+// HNML block of entry (i, j): pf = (u_i outer conj u_j) .* conj(Y_ij); H = L = Im pf, N = -M = Re pf
+template <int P, class T>
+__device__ __forceinline__ void hnml(double const* y, cplx const (&ui)[P], cplx const (&uj)[P], T (&b)[4 * P * P],
+                                     double (&prow)[P], double (&qrow)[P]) {
+    constexpr int NB = 2 * P;
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+        prow[r] = 0.0;
+        qrow[r] = 0.0;
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+            cplx const yc = {y[(r * P + c) * 2], -y[(r * P + c) * 2 + 1]};
+            cplx const pf = cmul(cmul(ui[r], cconj(uj[c])), yc);
+            b[c * NB + r] = T(pf.im);
+            b[(c + P) * NB + r] = T(pf.re);
+            b[c * NB + r + P] = T(-pf.re);
+            b[(c + P) * NB + r + P] = T(pf.im);
+            prow[r] += pf.re;
+            qrow[r] += pf.im;
+        }
+    }
+}
+// [[G, -B], [B, G]] of an admittance tensor
+template <int P, class T> __device__ __forceinline__ void linear_block(double const* y, T (&b)[4 * P * P]) {
+    constexpr int NB = 2 * P;
+#pragma unroll
+    for (int r = 0; r < P; ++r)
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+            double const g = y[(r * P + c) * 2], bb = y[(r * P + c) * 2 + 1];
+            b[c * NB + r] = T(g);
+            b[(c + P) * NB + r] = T(-bb);
+            b[c * NB + r + P] = T(bb);
+            b[(c + P) * NB + r + P] = T(g);
+        }
+}
+// admittance tensor of a source: y1 (sym) or (2 y1 + y0) / 3 on, (y0 - y1) / 3 off the diagonal
+template <int P> __device__ __forceinline__ void source_y(cplx y1, cplx y0, double (&y)[P * P * 2]) {
+#pragma unroll
+    for (int r = 0; r < P; ++r)
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+            cplx v;
+            if (P == 1)
+                v = y1;
+            else if (r == c)
+                v = {(2.0 * y1.re + y0.re) / 3.0, (2.0 * y1.im + y0.im) / 3.0};
+            else
+                v = {(y0.re - y1.re) / 3.0, (y0.im - y1.im) / 3.0};
+            y[(r * P + c) * 2] = v.re;
+            y[(r * P + c) * 2 + 1] = v.im;
+        }
+}
+template <int P> __device__ __forceinline__ void balanced(cplx x, cplx (&v)[P]) {
+    v[0] = x;
+    if (P == 3) {
+        cplx const a2 = {-0.5, -0.86602540378443864676};
+        cplx const a = {-0.5, 0.86602540378443864676};
+        v[1 % P] = cmul(x, a2);
+        v[2 % P] = cmul(x, a);
+    }
+}
+
+template <int P> __device__ __forceinline__ void load_u(DevGroup const& g, int bus, int s, cplx (&u)[P]) {
+    size_t const sp = g.S_pad;
+#pragma unroll
+    for (int ph = 0; ph < P; ++ph) {
+        u[ph].re = g.u[((size_t(bus) * P + ph) * 2) * sp + s];
+        u[ph].im = g.u[((size_t(bus) * P + ph) * 2 + 1) * sp + s];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// forward sweep: assemble row / column of each pivot, pull the Schur updates, factor the diagonal
+// block, finish the U row and L column, forward-substitute
+// ---------------------------------------------------------------------------------------------------
+template <int P, class FT, int MODE>
+__global__ __launch_bounds__(kWave) void forward_kernel(DevGroup g, int const* __restrict__ tasks, int nblk) {
+    constexpr int NB = 2 * P;
+    constexpr int NB2 = NB * NB;
+    constexpr int PP = P * P;
+    int const task = tasks[blockIdx.x / nblk];
+    int const s = int(blockIdx.x % nblk) * kWave + int(threadIdx.x);
+    size_t const sp = g.S_pad;
+    int const mm = g.task_mm[task];
+    if (s >= g.S) return;
+    if (g.done[size_t(mm) * sp + s] != ACTIVE) return;
+    Blocks<NB, FT> const lu{static_cast<FT*>(g.lu), sp, s};
+
+    for (int ti = g.task_ptr[task]; ti < g.task_ptr[task + 1]; ++ti) {
+        int const p = g.task_pivot[ti];
+        int const rb = g.row_ptr[p], re = g.row_ptr[p + 1], dg = g.diag[p];
+        double D[NB2];
+        double rhs[NB];
+#pragma unroll
+        for (int k = 0; k < NB2; ++k) D[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < NB; ++k) rhs[k] = 0.0;
+
+        if constexpr (MODE == MODE_RAW) {
+#pragma unroll
+            for (int k = 0; k < NB2; ++k) D[k] = g.raw_a[(size_t(dg) * NB2 + k) * sp + s];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) rhs[k] = g.raw_b[(size_t(p) * NB + k) * sp + s];
+            for (int k = dg + 1; k < re; ++k) {
+                FT a[NB2];
+#pragma unroll
+                for (int q = 0; q < NB2; ++q) a[q] = FT(g.raw_a[(size_t(k) * NB2 + q) * sp + s]);
+                lu.store(k, a);
+                int const e = g.transpose[k];
+#pragma unroll
+                for (int q = 0; q < NB2; ++q) a[q] = FT(g.raw_a[(size_t(e) * NB2 + q) * sp + s]);
+                lu.store(e, a);
+            }
+        } else if constexpr (MODE == MODE_INIT) {
+            // admittance matrix in the real domain plus load admittances and sources
+            for (int k = dg; k < re; ++k) {
+                FT a[NB2];
+                if (g.fill[k]) {
+#pragma unroll
+                    for (int q = 0; q < NB2; ++q) a[q] = FT(0);
+                } else {
+                    linear_block<P, FT>(g.ydata + size_t(k) * PP * 2, a);
+                }
+                if (k == dg) {
+#pragma unroll
+                    for (int q = 0; q < NB2; ++q) D[q] = double(a[q]);
+                } else {
+                    lu.store(k, a);
+                    int const e = g.transpose[k];
+                    if (g.fill[e]) {
+                        lu.store_zero(e);
+                    } else {
+                        linear_block<P, FT>(g.ydata + size_t(e) * PP * 2, a);
+                        lu.store(e, a);
+                    }
+                }
+            }
+            for (int l = g.load_ptr[p]; l < g.load_ptr[p + 1]; ++l) {
+#pragma unroll
+                for (int ph = 0; ph < P; ++ph) {
+                    double const sr = g.s_inj[((size_t(l) * P + ph) * 2) * sp + s];
+                    double const si = g.s_inj[((size_t(l) * P + ph) * 2 + 1) * sp + s];
+                    // y_load = -conj(s)
+                    double const yr = -sr, yi = si;
+                    D[(ph + P) * NB + ph] += -yi;
+                    D[ph * NB + ph] += yr;
+                    D[(ph + P) * NB + ph + P] += yr;
+                    D[ph * NB + ph + P] += yi;
+                }
+            }
+            for (int q = g.src_ptr[p]; q < g.src_ptr[p + 1]; ++q) {
+                cplx const y1 = {g.y_src[(size_t(q) * 4) * sp + s], g.y_src[(size_t(q) * 4 + 1) * sp + s]};
+                cplx const y0 = {g.y_src[(size_t(q) * 4 + 2) * sp + s], g.y_src[(size_t(q) * 4 + 3) * sp + s]};
+                cplx const ur = {g.u_src[(size_t(q) * 2) * sp + s], g.u_src[(size_t(q) * 2 + 1) * sp + s]};
+                double ys[PP * 2];
+                source_y<P>(y1, y0, ys);
+                cplx ub[P];
+                balanced<P>(ur, ub);
+#pragma unroll
+                for (int r = 0; r < P; ++r) {
+                    cplx acc = {0.0, 0.0};
+#pragma unroll
+                    for (int c = 0; c < P; ++c) {
+                        double const gg = ys[(r * P + c) * 2], bb = ys[(r * P + c) * 2 + 1];
+                        D[(c + P) * NB + r] -= bb;
+                        D[c * NB + r] += gg;
+                        D[(c + P) * NB + r + P] += gg;
+                        D[c * NB + r + P] += bb;
+                        acc = cadd(acc, cmul({gg, bb}, ub[c]));
+                    }
+                    rhs[r] += acc.re;
+                    rhs[r + P] += acc.im;
+                }
+            }
+        } else {
+            // Newton-Raphson Jacobian row p, column p and power mismatch
+            cplx up[P];
+            load_u<P>(g, p, s, up);
+            double pcal[P], qcal[P];
+#pragma unroll
+            for (int ph = 0; ph < P; ++ph) pcal[ph] = qcal[ph] = 0.0;
+            for (int k = rb; k < re; ++k) {
+                int const j = g.col[k];
+                if (g.fill[k]) {
+                    if (j > p) {
+                        lu.store_zero(k);
+                        lu.store_zero(g.transpose[k]);
+                    }
+                    continue;
+                }
+                cplx uj[P];
+                if (j == p) {
+#pragma unroll
+                    for (int ph = 0; ph < P; ++ph) uj[ph] = up[ph];
+                } else {
+                    load_u<P>(g, j, s, uj);
+                }
+                double pr[P], qr[P];
+                if (j == p) {
+                    hnml<P, double>(g.ydata + size_t(k) * PP * 2, up, uj, D, pr, qr);
+                } else {
+                    FT a[NB2];
+                    hnml<P, FT>(g.ydata + size_t(k) * PP * 2, up, uj, a, pr, qr);
+                    if (j > p) lu.store(k, a);
+                }
+#pragma unroll
+                for (int ph = 0; ph < P; ++ph) {
+                    pcal[ph] += pr[ph];
+                    qcal[ph] += qr[ph];
+                }
+                if (j > p) {
+                    int const e = g.transpose[k];
+                    if (g.fill[e]) {
+                        lu.store_zero(e);
+                    } else {
+                        FT a[NB2];
+                        double pr2[P], qr2[P];
+                        hnml<P, FT>(g.ydata + size_t(e) * PP * 2, uj, up, a, pr2, qr2);
+                        lu.store(e, a);
+                    }
+                }
+            }
+#pragma unroll
+            for (int ph = 0; ph < P; ++ph) {
+                rhs[ph] = -pcal[ph];
+                rhs[ph + P] = -qcal[ph];
+                D[ph * NB + ph] += -qcal[ph];
+                D[(ph + P) * NB + ph] += pcal[ph];
+                D[ph * NB + ph + P] += pcal[ph];
+                D[(ph + P) * NB + ph + P] += qcal[ph];
+            }
+            // loads: constant power / impedance / current (newton_raphson_pf_solver.hpp:764-822)
+            for (int l = g.load_ptr[p]; l < g.load_ptr[p + 1]; ++l) {
+                int const type = g.load_type[l];
+#pragma unroll
+                for (int ph = 0; ph < P; ++ph) {
+                    double const sr = g.s_inj[((size_t(l) * P + ph) * 2) * sp + s];
+                    double const si = g.s_inj[((size_t(l) * P + ph) * 2 + 1) * sp + s];
+                    double const v = g.xv[((size_t(p) * P + ph) * 2 + 1) * sp + s];
+                    double const f = type == 0 ? 1.0 : (type == 1 ? v * v : v);
+                    rhs[ph] += sr * f;
+                    rhs[ph + P] += si * f;
+                    double const df = type == 0 ? 0.0 : (type == 1 ? 2.0 * v * v : v);
+                    D[(ph + P) * NB + ph] += -sr * df;
+                    D[(ph + P) * NB + ph + P] += -si * df;
+                }
+            }
+            // sources: two-bus equivalent with y_ref (newton_raphson_pf_solver.hpp:824-852)
+            for (int q = g.src_ptr[p]; q < g.src_ptr[p + 1]; ++q) {
+                cplx const y1 = {g.y_src[(size_t(q) * 4) * sp + s], g.y_src[(size_t(q) * 4 + 1) * sp + s]};
+                cplx const y0 = {g.y_src[(size_t(q) * 4 + 2) * sp + s], g.y_src[(size_t(q) * 4 + 3) * sp + s]};
+                cplx const ur = {g.u_src[(size_t(q) * 2) * sp + s], g.u_src[(size_t(q) * 2 + 1) * sp + s]};
+                double ys[PP * 2], yn[PP * 2];
+                source_y<P>(y1, y0, ys);
+#pragma unroll
+                for (int k = 0; k < PP * 2; ++k) yn[k] = -ys[k];
+                cplx ub[P];
+                balanced<P>(ur, ub);
+                double bmm[NB2], bms[NB2];
+                double p1[P], q1[P], p2[P], q2[P];
+                hnml<P, double>(ys, up, up, bmm, p1, q1);
+                hnml<P, double>(yn, up, ub, bms, p2, q2);
+#pragma unroll
+                for (int r = 0; r < P; ++r) {
+                    double const pc = p1[r] + p2[r];
+                    double const qc = q1[r] + q2[r];
+                    bmm[r * NB + r] += -qc;
+                    bmm[(r + P) * NB + r] += pc;
+                    bmm[r * NB + r + P] += pc;
+                    bmm[(r + P) * NB + r + P] += qc;
+                    rhs[r] -= pc;
+                    rhs[r + P] -= qc;
+                }
+#pragma unroll
+                for (int k = 0; k < NB2; ++k) D[k] += bmm[k];
+            }
+        }
+
+        // Schur updates owned by pivot p, pulled from the already factorised descendants
+        FT Df[NB2];
+#pragma unroll
+        for (int k = 0; k < NB2; ++k) Df[k] = FT(D[k]);
+        for (int q = g.schur_ptr[p]; q < g.schur_ptr[p + 1]; ++q) {
+            int const tgt = g.schur[3 * q], le = g.schur[3 * q + 1], ue = g.schur[3 * q + 2];
+            FT L[NB2], U[NB2];
+            lu.load(le, L);
+            lu.load(ue, U);
+            if (tgt == dg) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c)
+#pragma unroll
+                    for (int r = 0; r < NB; ++r) {
+                        FT acc = FT(0);
+#pragma unroll
+                        for (int k = 0; k < NB; ++k) acc += L[k * NB + r] * U[c * NB + k];
+                        Df[c * NB + r] -= acc;
+                    }
+            } else {
+                FT A[NB2];
+                lu.load(tgt, A);
+#pragma unroll
+                for (int c = 0; c < NB; ++c)
+#pragma unroll
+                    for (int r = 0; r < NB; ++r) {
+                        FT acc = FT(0);
+#pragma unroll
+                        for (int k = 0; k < NB; ++k) acc += L[k * NB + r] * U[c * NB + k];
+                        A[c * NB + r] -= acc;
+                    }
+                lu.store(tgt, A);
+            }
+        }
+
+        uint32_t code;
+        bool const ok = factor_block<NB, FT>(Df, code);
+        if (!ok) {
+            g.done[size_t(mm) * sp + s] = FAILED;
+            g.err[size_t(mm) * sp + s] = ERR_SPARSE;
+            return;
+        }
+        lu.store(dg, Df);
+        g.perm[size_t(p) * sp + s] = code;
+        // U row: U_pj = L^-1 P A_pj ; L column: L_ip = A_ip Q U^-1
+        for (int k = dg + 1; k < re; ++k) {
+            FT A[NB2];
+            lu.load(k, A);
+            apply_p_rows<NB, FT>(A, code);
+            lower_left<NB, FT>(Df, A);
+            lu.store(k, A);
+            int const e = g.transpose[k];
+            lu.load(e, A);
+            apply_q_cols<NB, FT>(A, code);
+            upper_right<NB, FT>(Df, A);
+            lu.store(e, A);
+        }
+        // forward substitution: y_p = L_pp^-1 P (b_p - sum_k L_pk y_k)
+        for (int k = rb; k < dg; ++k) {
+            int const j = g.col[k];
+            FT L[NB2];
+            lu.load(k, L);
+            double yj[NB];
+#pragma unroll
+            for (int r = 0; r < NB; ++r) yj[r] = g.y[(size_t(j) * NB + r) * sp + s];
+#pragma unroll
+            for (int r = 0; r < NB; ++r) {
+                double acc = 0.0;
+#pragma unroll
+                for (int c = 0; c < NB; ++c) acc += double(L[c * NB + r]) * yj[c];
+                rhs[r] -= acc;
+            }
+        }
+        apply_p_vec<NB>(rhs, code);
+        lower_vec<NB, FT>(Df, rhs);
+#pragma unroll
+        for (int r = 0; r < NB; ++r) g.y[(size_t(p) * NB + r) * sp + s] = rhs[r];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// backward sweep: x_p = Q U_pp^-1 (y_p - sum_j U_pj x_j), then the state update of bus p
+// ---------------------------------------------------------------------------------------------------
+template <int P, class FT, int MODE>
+__global__ __launch_bounds__(kWave) void backward_kernel(DevGroup g, int const* __restrict__ tasks, int nblk) {
+    constexpr int NB = 2 * P;
+    constexpr int NB2 = NB * NB;
+    int const task = tasks[blockIdx.x / nblk];
+    int const s = int(blockIdx.x % nblk) * kWave + int(threadIdx.x);
+    size_t const sp = g.S_pad;
+    int const mm = g.task_mm[task];
+    if (s >= g.S) return;
+    if (g.done[size_t(mm) * sp + s] != ACTIVE) return;
+    Blocks<NB, FT> const lu{static_cast<FT*>(g.lu), sp, s};
+    double dev = 0.0;
+    for (int ti = g.task_ptr[task + 1] - 1; ti >= g.task_ptr[task]; --ti) {
+        int const p = g.task_pivot[ti];
+        int const re = g.row_ptr[p + 1], dg = g.diag[p];
+        double v[NB];
+#pragma unroll
+        for (int r = 0; r < NB; ++r) v[r] = g.y[(size_t(p) * NB + r) * sp + s];
+        for (int k = dg + 1; k < re; ++k) {
+            int const j = g.col[k];
+            FT U[NB2];
+            lu.load(k, U);
+            double xj[NB];
+#pragma unroll
+            for (int r = 0; r < NB; ++r) xj[r] = g.y[(size_t(j) * NB + r) * sp + s];
+#pragma unroll
+            for (int r = 0; r < NB; ++r) {
+                double acc = 0.0;
+#pragma unroll
+                for (int c = 0; c < NB; ++c) acc += double(U[c * NB + r]) * xj[c];
+                v[r] -= acc;
+            }
+        }
+        FT Df[NB2];
+        lu.load(dg, Df);
+        upper_vec<NB, FT>(Df, v);
+        apply_q_vec<NB>(v, g.perm[size_t(p) * sp + s]);
+#pragma unroll
+        for (int r = 0; r < NB; ++r) g.y[(size_t(p) * NB + r) * sp + s] = v[r];
+        if constexpr (MODE == MODE_INIT) {
+#pragma unroll
+            for (int ph = 0; ph < P; ++ph) {
+                double const ur = v[ph], ui = v[ph + P];
+                size_t const b = (size_t(p) * P + ph) * 2;
+                g.u[b * sp + s] = ur;
+                g.u[(b + 1) * sp + s] = ui;
+                g.xv[b * sp + s] = atan2(ui, ur);
+                g.xv[(b + 1) * sp + s] = sqrt(ur * ur + ui * ui);
+            }
+        } else if constexpr (MODE == MODE_NR) {
+            double dbus = 0.0;
+#pragma unroll
+            for (int ph = 0; ph < P; ++ph) {
+                size_t const b = (size_t(p) * P + ph) * 2;
+                double th = g.xv[b * sp + s] + v[ph];
+                double vm = g.xv[(b + 1) * sp + s];
+                vm += vm * v[ph + P];
+                double sn, cs;
+                sincos(th, &sn, &cs);
+                double const nr = vm * cs, ni = vm * sn;
+                double const dr = nr - g.u[b * sp + s], di = ni - g.u[(b + 1) * sp + s];
+                double const d = sqrt(dr * dr + di * di);
+                dbus = ph == 0 ? d : (d > dbus ? d : dbus);
+                g.xv[b * sp + s] = th;
+                g.xv[(b + 1) * sp + s] = vm;
+                g.u[b * sp + s] = nr;
+                g.u[(b + 1) * sp + s] = ni;
+            }
+            dev = dbus > dev ? dbus : dev;
+        }
+    }
+    if constexpr (MODE == MODE_NR) {
+        atomicMax(&g.devmax[size_t(mm) * sp + s], static_cast<unsigned long long>(__double_as_longlong(dev)));
+    }
+}
+
+__global__ void init_state_kernel(DevGroup g) {
+    size_t const i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(g.n_mm) * g.S_pad) return;
+    g.done[i] = ACTIVE;
+    g.err[i] = ERR_NONE;
+    g.iters[i] = 0;
+    g.devmax[i] = 0ull;
+    g.lastdev[i] = 0.0;
+}
+
+__global__ void converge_kernel(DevGroup g, int it, int max_iter, double tol, int* active) {
+    size_t const i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(g.n_mm) * g.S_pad) return;
+    int const s = int(i % g.S_pad);
+    if (s >= g.S || g.done[i] != ACTIVE) return;
+    double const md = __longlong_as_double(static_cast<long long>(g.devmax[i]));
+    g.devmax[i] = 0ull;
+    g.iters[i] = it;
+    if (!(md > tol)) {
+        g.done[i] = CONVERGED;
+    } else if (it >= max_iter) {
+        g.done[i] = FAILED;
+        g.err[i] = ERR_DIVERGE;
+        g.lastdev[i] = md;
+    } else {
+        atomicAdd(active, 1);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// results per bus: load powers and currents, bus injection and source results, shunt flows
+// ---------------------------------------------------------------------------------------------------
+template <int P> __global__ void bus_results_kernel(DevOutput o) {
+    size_t const idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    size_t const sp = o.S_pad;
+    if (idx >= size_t(o.n) * sp) return;
+    int const b = int(idx / sp);
+    int const s = int(idx % sp);
+    if (s >= o.S || !o.ok[s]) return;
+    auto U = [&](int bus, int ph) {
+        return cplx{o.u[((size_t(bus) * P + ph) * 2) * sp + s], o.u[((size_t(bus) * P + ph) * 2 + 1) * sp + s]};
+    };
+    cplx ub[P];
+#pragma unroll
+    for (int ph = 0; ph < P; ++ph) ub[ph] = U(b, ph);
+    cplx iload[P], acc_s[P];
+#pragma unroll
+    for (int ph = 0; ph < P; ++ph) iload[ph] = acc_s[ph] = {0.0, 0.0};
+    for (int l = o.load_ptr[b]; l < o.load_ptr[b + 1]; ++l) {
+        int const type = o.load_type[l];
+#pragma unroll
+        for (int ph = 0; ph < P; ++ph) {
+            cplx sv = {o.s_inj[((size_t(l) * P + ph) * 2) * sp + s], o.s_inj[((size_t(l) * P + ph) * 2 + 1) * sp + s]};
+            double const a2 = ub[ph].re * ub[ph].re + ub[ph].im * ub[ph].im;
+            double const f = type == 0 ? 1.0 : (type == 1 ? a2 : sqrt(a2));
+            sv = {sv.re * f, sv.im * f};
+            cplx const iv = cconj(cdiv(sv, ub[ph]));
+            size_t const base = (size_t(l) * 2) * P;
+            o.load_si[((base + ph) * 2) * sp + s] = sv.re;
+            o.load_si[((base + ph) * 2 + 1) * sp + s] = sv.im;
+            o.load_si[((base + P + ph) * 2) * sp + s] = iv.re;
+            o.load_si[((base + P + ph) * 2 + 1) * sp + s] = iv.im;
+            iload[ph] = cadd(iload[ph], iv);
+            acc_s[ph] = cadd(acc_s[ph], sv);
+        }
+    }
+    int const s0 = o.src_ptr[b], s1 = o.src_ptr[b + 1];
+    if (s1 > s0) {
+        // bus injection u * conj(sum_j Y_bj u_j)
+        cplx inj[P];
+#pragma unroll
+        for (int ph = 0; ph < P; ++ph) inj[ph] = {0.0, 0.0};
+        for (int k = o.row_ptr[b]; k < o.row_ptr[b + 1]; ++k) {
+            if (o.fill[k]) continue;
+            int const j = o.col[k];
+            double const* y = o.ydata + size_t(k) * P * P * 2;
+#pragma unroll
+            for (int r = 0; r < P; ++r)
+#pragma unroll
+                for (int c = 0; c < P; ++c)
+                    inj[r] = cadd(inj[r], cmul({y[(r * P + c) * 2], y[(r * P + c) * 2 + 1]}, U(j, c)));
+        }
+        cplx it_[P];
+#pragma unroll
+        for (int ph = 0; ph < P; ++ph) {
+            cplx const sb = cmul(cconj(inj[ph]), ub[ph]);
+            it_[ph] = csub(cconj(cdiv(sb, ub[ph])), iload[ph]);
+        }
+        auto put_src = [&](int q, cplx const (&iv)[P]) {
+#pragma unroll
+            for (int ph = 0; ph < P; ++ph) {
+                cplx const sv = cmul(ub[ph], cconj(iv[ph]));
+                size_t const base = (size_t(q) * 2) * P;
+                o.src_si[((base + ph) * 2) * sp + s] = sv.re;
+                o.src_si[((base + ph) * 2 + 1) * sp + s] = sv.im;
+                o.src_si[((base + P + ph) * 2) * sp + s] = iv[ph].re;
+                o.src_si[((base + P + ph) * 2 + 1) * sp + s] = iv[ph].im;
+                acc_s[ph] = cadd(acc_s[ph], sv);
+            }
+        };
+        auto ysrc = [&](int q, int k) {
+            return cplx{o.y_src[(size_t(q) * 4 + 2 * k) * sp + s], o.y_src[(size_t(q) * 4 + 2 * k + 1) * sp + s]};
+        };
+        auto usrc = [&](int q) { return cplx{o.u_src[(size_t(q) * 2) * sp + s], o.u_src[(size_t(q) * 2 + 1) * sp + s]}; };
+        if (s1 - s0 == 1) {
+            put_src(s0, it_);
+        } else if (P == 1) {
+            cplx yt = {0, 0}, irt = {0, 0};
+            for (int q = s0; q < s1; ++q) yt = cadd(yt, ysrc(q, 0));
+            cplx const zt = cdiv({1.0, 0.0}, yt);
+            for (int q = s0; q < s1; ++q) irt = cadd(irt, cmul(usrc(q), ysrc(q, 0)));
+            for (int q = s0; q < s1; ++q) {
+                cplx const ratio = cmul(ysrc(q, 0), zt);
+                cplx const lhs = cmul(ratio, csub(cmul(usrc(q), yt), irt));
+                cplx iv[P];
+                iv[0] = cadd(lhs, cmul(ratio, it_[0]));
+                put_src(q, iv);
+            }
+        } else {
+            cplx const a = {-0.5, 0.86602540378443864676}, a2 = {-0.5, -0.86602540378443864676};
+            cplx yt0 = {0, 0}, yt1 = {0, 0}, ir1 = {0, 0};
+            for (int q = s0; q < s1; ++q) {
+                yt0 = cadd(yt0, ysrc(q, 1));
+                yt1 = cadd(yt1, ysrc(q, 0));
+                ir1 = cadd(ir1, cmul(usrc(q), ysrc(q, 0)));
+            }
+            // to sequence components: (1/3) [[1,1,1],[1,a,a2],[1,a2,a]]
+            cplx const i0 = {(it_[0].re + it_[1 % P].re + it_[2 % P].re) / 3.0,
+                             (it_[0].im + it_[1 % P].im + it_[2 % P].im) / 3.0};
+            cplx const i1t = cadd(cadd(it_[0], cmul(a, it_[1 % P])), cmul(a2, it_[2 % P]));
+            cplx const i2t = cadd(cadd(it_[0], cmul(a2, it_[1 % P])), cmul(a, it_[2 % P]));
+            cplx const i1 = {i1t.re / 3.0, i1t.im / 3.0}, i2 = {i2t.re / 3.0, i2t.im / 3.0};
+            for (int q = s0; q < s1; ++q) {
+                cplx const r1 = cdiv(ysrc(q, 0), yt1);
+                cplx const lhs = cmul(r1, csub(cmul(usrc(q), yt1), ir1));
+                cplx const v0 = cmul(cdiv(ysrc(q, 1), yt0), i0);
+                cplx const v1 = cadd(lhs, cmul(r1, i1));
+                cplx const v2 = cmul(cdiv(ysrc(q, 0), yt1), i2);
+                // back to phases: [[1,1,1],[1,a2,a],[1,a,a2]]
+                cplx iv[P];
+                iv[0] = cadd(cadd(v0, v1), v2);
+                iv[1 % P] = cadd(cadd(v0, cmul(a2, v1)), cmul(a, v2));
+                iv[2 % P] = cadd(cadd(v0, cmul(a, v1)), cmul(a2, v2));
+                put_src(q, iv);
+            }
+        }
+    }
+#pragma unroll
+    for (int ph = 0; ph < P; ++ph) {
+        o.bus_inj[((size_t(b) * P + ph) * 2) * sp + s] = acc_s[ph].re;
+        o.bus_inj[((size_t(b) * P + ph) * 2 + 1) * sp + s] = acc_s[ph].im;
+    }
+    for (int q = o.shunt_ptr[b]; q < o.shunt_ptr[b + 1]; ++q) {
+        double const* y = o.shunt_y + size_t(q) * P * P * 2;
+#pragma unroll
+        for (int r = 0; r < P; ++r) {
+            cplx iy = {0.0, 0.0};
+#pragma unroll
+            for (int c = 0; c < P; ++c) iy = cadd(iy, cmul({y[(r * P + c) * 2], y[(r * P + c) * 2 + 1]}, ub[c]));
+            cplx const iv = {-iy.re, -iy.im};
+            cplx const sv = cmul(ub[r], cconj(iv));
+            size_t const base = (size_t(q) * 2) * P;
+            o.shunt_si[((base + r) * 2) * sp + s] = sv.re;
+            o.shunt_si[((base + r) * 2 + 1) * sp + s] = sv.im;
+            o.shunt_si[((base + P + r) * 2) * sp + s] = iv.re;
+            o.shunt_si[((base + P + r) * 2 + 1) * sp + s] = iv.im;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// record writers (node.hpp, branch.hpp, appliance.hpp get_output)
+// ---------------------------------------------------------------------------------------------------
+template <int P> __global__ void node_records_kernel(DevOutput o, NodeRec r, NodeOutput<P>* out) {
+    size_t const idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    size_t const sp = o.S_pad;
+    if (idx >= size_t(r.n_comp) * sp) return;
+    int const i = int(idx / sp), s = int(idx % sp);
+    if (s >= o.S || !o.ok[s]) return;
+    NodeOutput<P> rec{};
+    rec.id = r.id[i];
+    int const bus = r.bus[i];
+    double const base_power = P == 1 ? 1e6 : 1e6 / 3.0;
+    double const u_scale = P == 1 ? 1.0 : 0.57735026918962576451;
+    if (bus >= 0) {
+        rec.energized = 1;
+#pragma unroll
+        for (int ph = 0; ph < P; ++ph) {
+            double const ur = o.u[((size_t(bus) * P + ph) * 2) * sp + s];
+            double const ui = o.u[((size_t(bus) * P + ph) * 2 + 1) * sp + s];
+            rec.u_pu[ph] = sqrt(ur * ur + ui * ui);
+            rec.u[ph] = u_scale * r.u_rated[i] * rec.u_pu[ph];
+            rec.u_angle[ph] = atan2(ui, ur);
+            rec.p[ph] = base_power * o.bus_inj[((size_t(bus) * P + ph) * 2) * sp + s];
+            rec.q[ph] = base_power * o.bus_inj[((size_t(bus) * P + ph) * 2 + 1) * sp + s];
+        }
+    }
+    out[size_t(o.scenario_index[s]) * r.n_comp + i] = rec;
+}
+
+template <int P> __global__ void branch_records_kernel(DevOutput o, BranchRec r, BranchOutput<P>* out) {
+    size_t const idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    size_t const sp = o.S_pad;
+    if (idx >= size_t(r.n_comp) * sp) return;
+    int const i = int(idx / sp), s = int(idx % sp);
+    if (s >= o.S || !o.ok[s]) return;
+    BranchOutput<P> rec{};
+    rec.id = r.id[i];
+    int const m = r.math[i];
+    double const base_power = P == 1 ? 1e6 : 1e6 / 3.0;
+    if (m >= 0) {
+        rec.energized = 1;
+        int const f = r.from_bus[m], t = r.to_bus[m];
+        cplx uf[P], ut[P];
+#pragma unroll
+        for (int ph = 0; ph < P; ++ph) {
+            uf[ph] = f >= 0 ? cplx{o.u[((size_t(f) * P + ph) * 2) * sp + s], o.u[((size_t(f) * P + ph) * 2 + 1) * sp + s]}
+                            : cplx{0.0, 0.0};
+            ut[ph] = t >= 0 ? cplx{o.u[((size_t(t) * P + ph) * 2) * sp + s], o.u[((size_t(t) * P + ph) * 2 + 1) * sp + s]}
+                            : cplx{0.0, 0.0};
+        }
+        double const* y = r.branch_y + size_t(m) * 4 * P * P * 2;
+        auto Y = [&](int w, int rr, int cc) { return cplx{y[((w * P + rr) * P + cc) * 2], y[((w * P + rr) * P + cc) * 2 + 1]}; };
+        double sum_sf = 0, sum_st = 0, max_i = 0;
+#pragma unroll
+        for (int ph = 0; ph < P; ++ph) {
+            cplx ifr = {0, 0}, ito = {0, 0};
+#pragma unroll
+            for (int c = 0; c < P; ++c) {
+                ifr = cadd(ifr, cadd(cmul(Y(0, ph, c), uf[c]), cmul(Y(1, ph, c), ut[c])));
+                ito = cadd(ito, cadd(cmul(Y(2, ph, c), uf[c]), cmul(Y(3, ph, c), ut[c])));
+            }
+            cplx const sf = cmul(uf[ph], cconj(ifr));
+            cplx const st = cmul(ut[ph], cconj(ito));
+            rec.p_from[ph] = base_power * sf.re;
+            rec.q_from[ph] = base_power * sf.im;
+            rec.i_from[ph] = r.base_i_from[i] * cabs_(ifr);
+            rec.s_from[ph] = base_power * cabs_(sf);
+            rec.p_to[ph] = base_power * st.re;
+            rec.q_to[ph] = base_power * st.im;
+            rec.i_to[ph] = r.base_i_to[i] * cabs_(ito);
+            rec.s_to[ph] = base_power * cabs_(st);
+            sum_sf += rec.s_from[ph];
+            sum_st += rec.s_to[ph];
+            double const mi = rec.i_from[ph] > rec.i_to[ph] ? rec.i_from[ph] : rec.i_to[ph];
+            max_i = ph == 0 ? mi : (mi > max_i ? mi : max_i);
+        }
+        rec.loading = r.is_line ? max_i / r.rating[i] : (sum_sf > sum_st ? sum_sf : sum_st) / r.rating[i];
+    }
+    out[size_t(o.scenario_index[s]) * r.n_comp + i] = rec;
+}
+
+template <int P> __global__ void appliance_records_kernel(DevOutput o, ApplRec r, ApplianceOutput<P>* out) {
+    size_t const idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    size_t const sp = o.S_pad;
+    if (idx >= size_t(r.n_comp) * sp) return;
+    int const i = int(idx / sp), s = int(idx % sp);
+    if (s >= o.S || !o.ok[s]) return;
+    ApplianceOutput<P> rec{};
+    rec.id = r.id[i];
+    int const m = r.math[i];
+    double const base_power = P == 1 ? 1e6 : 1e6 / 3.0;
+    if (m >= 0) {
+        rec.energized = r.status ? r.status[size_t(i) * sp + s] : r.status_const[i];
+        size_t const base = (size_t(m) * 2) * P;
+#pragma unroll
+        for (int ph = 0; ph < P; ++ph) {
+            cplx const sv = {r.si[((base + ph) * 2) * sp + s], r.si[((base + ph) * 2 + 1) * sp + s]};
+            cplx const iv = {r.si[((base + P + ph) * 2) * sp + s], r.si[((base + P + ph) * 2 + 1) * sp + s]};
+            rec.p[ph] = base_power * sv.re * r.direction;
+            rec.q[ph] = base_power * sv.im * r.direction;
+            rec.s[ph] = base_power * cabs_(sv);
+            rec.i[ph] = r.base_i[i] * cabs_(iv);
+            rec.pf[ph] = rec.s[ph] < 1e-8 ? 0.0 : rec.p[ph] / rec.s[ph];
+        }
+    }
+    out[size_t(o.scenario_index[s]) * r.n_comp + i] = rec;
+}
+
+template <int P, class FT> void launch_fwd(DevGroup const& g, int mode, int const* tasks, int n_tasks, hipStream_t st) {
+    int const nblk = (g.S + kWave - 1) / kWave;
+    dim3 const grid(unsigned(n_tasks) * unsigned(nblk));
+    if (mode == MODE_INIT)
+        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_INIT>), grid, dim3(kWave), 0, st, g, tasks, nblk);
+    else if (mode == MODE_NR)
+        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_NR>), grid, dim3(kWave), 0, st, g, tasks, nblk);
+    else
+        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_RAW>), grid, dim3(kWave), 0, st, g, tasks, nblk);
+}
+template <int P, class FT> void launch_bwd(DevGroup const& g, int mode, int const* tasks, int n_tasks, hipStream_t st) {
+    int const nblk = (g.S + kWave - 1) / kWave;
+    dim3 const grid(unsigned(n_tasks) * unsigned(nblk));
+    if (mode == MODE_INIT)
+        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_INIT>), grid, dim3(kWave), 0, st, g, tasks, nblk);
+    else if (mode == MODE_NR)
+        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_NR>), grid, dim3(kWave), 0, st, g, tasks, nblk);
+    else
+        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_RAW>), grid, dim3(kWave), 0, st, g, tasks, nblk);
+}
+
+}  // namespace
+
+void launch_forward(DevGroup const& g, int mode, int const* tasks, int n_tasks, hipStream_t st) {
+    if (n_tasks <= 0 || g.S <= 0) return;
+    if (g.P == 1)
+        g.fp32 ? launch_fwd<1, float>(g, mode, tasks, n_tasks, st) : launch_fwd<1, double>(g, mode, tasks, n_tasks, st);
+    else
+        g.fp32 ? launch_fwd<3, float>(g, mode, tasks, n_tasks, st) : launch_fwd<3, double>(g, mode, tasks, n_tasks, st);
+}
+void launch_backward(DevGroup const& g, int mode, int const* tasks, int n_tasks, hipStream_t st) {
+    if (n_tasks <= 0 || g.S <= 0) return;
+    if (g.P == 1)
+        g.fp32 ? launch_bwd<1, float>(g, mode, tasks, n_tasks, st) : launch_bwd<1, double>(g, mode, tasks, n_tasks, st);
+    else
+        g.fp32 ? launch_bwd<3, float>(g, mode, tasks, n_tasks, st) : launch_bwd<3, double>(g, mode, tasks, n_tasks, st);
+}
+void launch_converge(DevGroup const& g, int it, int max_iter, double err_tol, int* active, hipStream_t st) {
+    size_t const n = size_t(g.n_mm) * g.S_pad;
+    if (n == 0) return;
+    hipLaunchKernelGGL(converge_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, g, it, max_iter, err_tol,
+                       active);
+}
+void launch_init_state(DevGroup const& g, hipStream_t st) {
+    size_t const n = size_t(g.n_mm) * g.S_pad;
+    if (n == 0) return;
+    hipLaunchKernelGGL(init_state_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, g);
+}
+void launch_bus_results(DevOutput const& o, hipStream_t st) {
+    size_t const n = size_t(o.n) * o.S_pad;
+    if (n == 0) return;
+    dim3 const grid(unsigned((n + 255) / 256));
+    if (o.P == 1)
+        hipLaunchKernelGGL(bus_results_kernel<1>, grid, dim3(256), 0, st, o);
+    else
+        hipLaunchKernelGGL(bus_results_kernel<3>, grid, dim3(256), 0, st, o);
+}
+void launch_node_records(DevOutput const& o, NodeRec const& r, void* out, hipStream_t st) {
+    size_t const n = size_t(r.n_comp) * o.S_pad;
+    if (n == 0) return;
+    dim3 const grid(unsigned((n + 255) / 256));
+    if (o.P == 1)
+        hipLaunchKernelGGL(node_records_kernel<1>, grid, dim3(256), 0, st, o, r, static_cast<NodeOutput<1>*>(out));
+    else
+        hipLaunchKernelGGL(node_records_kernel<3>, grid, dim3(256), 0, st, o, r, static_cast<NodeOutput<3>*>(out));
+}
+void launch_branch_records(DevOutput const& o, BranchRec const& r, void* out, hipStream_t st) {
+    size_t const n = size_t(r.n_comp) * o.S_pad;
+    if (n == 0) return;
+    dim3 const grid(unsigned((n + 255) / 256));
+    if (o.P == 1)
+        hipLaunchKernelGGL(branch_records_kernel<1>, grid, dim3(256), 0, st, o, r, static_cast<BranchOutput<1>*>(out));
+    else
+        hipLaunchKernelGGL(branch_records_kernel<3>, grid, dim3(256), 0, st, o, r, static_cast<BranchOutput<3>*>(out));
+}
+void launch_appliance_records(DevOutput const& o, ApplRec const& r, void* out, hipStream_t st) {
+    size_t const n = size_t(r.n_comp) * o.S_pad;
+    if (n == 0) return;
+    dim3 const grid(unsigned((n + 255) / 256));
+    if (o.P == 1)
+        hipLaunchKernelGGL(appliance_records_kernel<1>, grid, dim3(256), 0, st, o, r,
+                           static_cast<ApplianceOutput<1>*>(out));
+    else
+        hipLaunchKernelGGL(appliance_records_kernel<3>, grid, dim3(256), 0, st, o, r,
+                           static_cast<ApplianceOutput<3>*>(out));
+}
+
+}  // namespace pgm_amd

@@ -1,0 +1,140 @@
+// kernels.hpp -- device data of one group (one topology, a set of scenarios) and the launchers of
+// the Newton-Raphson kernels in kernels.hip.
+//
+// Layout in HBM: every per-scenario array is scenario-interleaved, index (item * S_pad + s), so the 64
+// lanes of a wave (64 consecutive scenarios) read and write contiguous 512-byte (FP64) lines. Arrays
+// shared by all scenarios of the group (pattern, admittances, schedule) are read with uniform loads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pgm_amd {
+
+enum SweepMode : int { MODE_INIT = 0, MODE_NR = 1, MODE_RAW = 2 };
+enum DoneState : int { ACTIVE = 0, CONVERGED = 1, FAILED = 2 };
+enum ErrCode : int { ERR_NONE = 0, ERR_SPARSE = 1, ERR_DIVERGE = 2 };
+
+struct DevGroup {
+    int P = 1;         // phases
+    int n = 0;         // buses
+    int E = 0;         // LU entries
+    int n_mm = 0;      // math models
+    int S = 0;         // scenarios in the group
+    int S_pad = 0;     // S rounded up to 64
+    int fp32 = 0;      // factor storage precision
+    // structure (shared)
+    int const* row_ptr = nullptr;
+    int const* col = nullptr;
+    int const* diag = nullptr;
+    int const* transpose = nullptr;
+    int8_t const* fill = nullptr;
+    double const* ydata = nullptr;  // E * P*P * 2
+    int const* schur_ptr = nullptr;
+    int const* schur = nullptr;     // 3 ints per update
+    int const* task_ptr = nullptr;
+    int const* task_pivot = nullptr;
+    int const* task_mm = nullptr;
+    int const* load_ptr = nullptr;
+    int8_t const* load_type = nullptr;
+    int const* src_ptr = nullptr;
+    // per scenario
+    double* u = nullptr;        // [n][P][2][S_pad]
+    double* xv = nullptr;       // [n][P][2][S_pad] (theta, V)
+    double* y = nullptr;        // [n][2P][S_pad]
+    void* lu = nullptr;         // [E][4P^2][S_pad] double or float
+    uint32_t* perm = nullptr;   // [n][S_pad]
+    double const* s_inj = nullptr;  // [n_load][P][2][S_pad]
+    double const* u_src = nullptr;  // [n_src][2][S_pad]
+    double const* y_src = nullptr;  // [n_src][4][S_pad] (y1 re, im, y0 re, im)
+    int* done = nullptr;            // [n_mm][S_pad]
+    int* err = nullptr;             // [n_mm][S_pad]
+    int* iters = nullptr;           // [n_mm][S_pad]
+    unsigned long long* devmax = nullptr;  // [n_mm][S_pad], bits of a non-negative double
+    double* lastdev = nullptr;      // [n_mm][S_pad]
+    // raw block LU (test hook): input blocks and rhs
+    double const* raw_a = nullptr;  // [E][4P^2][S_pad]
+    double const* raw_b = nullptr;  // [n][2P][S_pad]
+};
+
+// one sweep over the tasks of one level; tasks = level task ids (device), n_tasks of them
+void launch_forward(DevGroup const& g, int mode, int const* tasks, int n_tasks, hipStream_t st);
+void launch_backward(DevGroup const& g, int mode, int const* tasks, int n_tasks, hipStream_t st);
+// per (math model, scenario): fold the deviation of iteration `it`, mark converged / diverged,
+// count the still active pairs into *active
+void launch_converge(DevGroup const& g, int it, int max_iter, double err_tol, int* active, hipStream_t st);
+void launch_init_state(DevGroup const& g, hipStream_t st);
+
+// outputs of a group, written as the caller's records at out[scenario_index[s] * n_comp + i]
+struct DevOutput {
+    int P = 1;
+    int S = 0, S_pad = 0;
+    int const* scenario_index = nullptr;  // global scenario of each lane
+    int const* ok = nullptr;              // per lane: 1 when every math model converged
+    // buses / Y rows (Y entries = non-fill LU entries)
+    int n = 0;
+    int const* row_ptr = nullptr;
+    int const* col = nullptr;
+    int8_t const* fill = nullptr;
+    double const* ydata = nullptr;
+    double const* u = nullptr;
+    // loads (math order)
+    int n_load = 0;
+    int const* load_bus = nullptr;
+    int8_t const* load_type = nullptr;
+    double const* s_inj = nullptr;
+    double* load_si = nullptr;  // [n_load][2 (s, i)][P][2][S_pad]
+    // sources (math order)
+    int n_src = 0;
+    int const* src_bus = nullptr;
+    int const* src_ptr = nullptr;  // per bus
+    int const* load_ptr = nullptr; // per bus
+    int const* shunt_ptr = nullptr; // per bus
+    double const* u_src = nullptr;
+    double const* y_src = nullptr;
+    double* src_si = nullptr;   // [n_src][2][P][2][S_pad]
+    double* bus_inj = nullptr;  // [n][P][2][S_pad] appliance injection sum per bus
+    // shunts
+    int n_shunt = 0;
+    int const* shunt_bus = nullptr;
+    double const* shunt_y = nullptr;  // P*P*2 per shunt
+    double* shunt_si = nullptr;       // [n_shunt][2][P][2][S_pad]
+};
+void launch_bus_results(DevOutput const& o, hipStream_t st);
+
+// record writers: one thread per (lane, component); map gives the math index of every component (-1
+// when it is not energised in this group)
+struct NodeRec {
+    int n_comp;
+    int const* id;
+    double const* u_rated;
+    int const* bus;
+};
+struct BranchRec {
+    int n_comp;
+    int const* id;
+    int const* math;           // math branch or -1
+    int const* from_bus;       // per math branch
+    int const* to_bus;
+    double const* branch_y;    // per math branch 4 * P*P * 2
+    double const* base_i_from;
+    double const* base_i_to;
+    double const* rating;      // line: i_n, transformer: sn
+    int is_line;
+};
+struct ApplRec {
+    int n_comp;
+    int const* id;
+    int const* math;      // math index into si array or -1
+    double const* base_i;
+    double direction;
+    int8_t const* status; // per component per lane [n_comp][S_pad], or nullptr (use status_const)
+    int8_t const* status_const;
+    double const* si;     // [n_math][2][P][2][S_pad]
+};
+void launch_node_records(DevOutput const& o, NodeRec const& r, void* out, hipStream_t st);
+void launch_branch_records(DevOutput const& o, BranchRec const& r, void* out, hipStream_t st);
+void launch_appliance_records(DevOutput const& o, ApplRec const& r, void* out, hipStream_t st);
+
+}  // namespace pgm_amd

@@ -1,0 +1,136 @@
+// records.hpp -- row-based dataset records of the power-flow path.
+//
+// The layouts are the reference's dataset layouts (code_generation/data/attribute_classes/
+// {input,update,output}.json; structs in power_grid_model_c/power_grid_model/include/power_grid_model/
+// auxiliary/{input,update,output}.hpp). They have to be byte-identical because callers hand the same
+// numpy / C buffers to either library; the C compiler's natural alignment produces the same offsets.
+#pragma once
+
+#include <cstdint>
+#include <limits>
+
+namespace pgm_amd {
+
+using ID = int32_t;
+using IntS = int8_t;
+using Idx = int64_t;
+
+constexpr IntS na_IntS = std::numeric_limits<IntS>::min();
+constexpr ID na_IntID = std::numeric_limits<ID>::min();
+
+// ---------- input ----------
+struct NodeInput {
+    ID id;
+    double u_rated;
+};
+struct LineInput {
+    ID id, from_node, to_node;
+    IntS from_status, to_status;
+    double r1, x1, c1, tan1, r0, x0, c0, tan0, i_n;
+};
+struct TransformerInput {
+    ID id, from_node, to_node;
+    IntS from_status, to_status;
+    double u1, u2, sn, uk, pk, i0, p0, i0_zero_sequence, p0_zero_sequence;
+    IntS winding_from, winding_to, clock, tap_side, tap_pos, tap_min, tap_max, tap_nom;
+    double tap_size, uk_min, uk_max, pk_min, pk_max;
+    double r_grounding_from, x_grounding_from, r_grounding_to, x_grounding_to;
+};
+struct ShuntInput {
+    ID id, node;
+    IntS status;
+    double g1, b1, g0, b0;
+};
+struct SourceInput {
+    ID id, node;
+    IntS status;
+    double u_ref, u_ref_angle, sk, rx_ratio, z01_ratio;
+};
+struct SymLoadGenInput {
+    ID id, node;
+    IntS status, type;
+    double p_specified, q_specified;
+};
+struct AsymLoadGenInput {
+    ID id, node;
+    IntS status, type;
+    double p_specified[3], q_specified[3];
+};
+
+// ---------- update ----------
+struct BranchUpdate {
+    ID id;
+    IntS from_status, to_status;
+};
+struct TransformerUpdate {
+    ID id;
+    IntS from_status, to_status, tap_pos;
+};
+struct ShuntUpdate {
+    ID id;
+    IntS status;
+    double g1, b1, g0, b0;
+};
+struct SourceUpdate {
+    ID id;
+    IntS status;
+    double u_ref, u_ref_angle, sk, rx_ratio, z01_ratio;
+};
+struct SymLoadGenUpdate {
+    ID id;
+    IntS status;
+    double p_specified, q_specified;
+};
+struct AsymLoadGenUpdate {
+    ID id;
+    IntS status;
+    double p_specified[3], q_specified[3];
+};
+
+// ---------- output ----------
+template <int P> struct NodeOutput {
+    ID id;
+    IntS energized;
+    double u_pu[P], u[P], u_angle[P], p[P], q[P];
+};
+template <int P> struct BranchOutput {
+    ID id;
+    IntS energized;
+    double loading;
+    double p_from[P], q_from[P], i_from[P], s_from[P], p_to[P], q_to[P], i_to[P], s_to[P];
+};
+template <int P> struct ApplianceOutput {
+    ID id;
+    IntS energized;
+    double p[P], q[P], i[P], s[P], pf[P];
+};
+
+static_assert(sizeof(NodeInput) == 16);
+static_assert(sizeof(LineInput) == 88);
+static_assert(sizeof(TransformerInput) == 168);
+static_assert(sizeof(SourceInput) == 56);
+static_assert(sizeof(SymLoadGenInput) == 32);
+static_assert(sizeof(AsymLoadGenInput) == 64);
+static_assert(sizeof(SymLoadGenUpdate) == 24);
+static_assert(sizeof(AsymLoadGenUpdate) == 56);
+static_assert(sizeof(NodeOutput<1>) == 48 && sizeof(NodeOutput<3>) == 128);
+static_assert(sizeof(BranchOutput<1>) == 80 && sizeof(BranchOutput<3>) == 208);
+static_assert(sizeof(ApplianceOutput<1>) == 48 && sizeof(ApplianceOutput<3>) == 128);
+
+// component slots, in the reference container order (all_components.hpp:37-40)
+enum Comp : int {
+    C_NODE = 0,
+    C_LINE,
+    C_TRANSFORMER,
+    C_SHUNT,
+    C_SOURCE,
+    C_SYM_GEN,
+    C_ASYM_GEN,
+    C_SYM_LOAD,
+    C_ASYM_LOAD,
+    N_COMP
+};
+inline char const* const comp_names[N_COMP] = {"node",   "line",    "transformer", "shunt",    "source",
+                                               "sym_gen", "asym_gen", "sym_load",    "asym_load"};
+
+}  // namespace pgm_amd

@@ -1,0 +1,43 @@
+"""power_grid_model_amd -- batched Newton-Raphson power flow on AMD MI355X (gfx950).
+
+Drop-in for the power-flow path of power-grid-model: the same input / update / output datasets and the
+same PowerGridModel.calculate_power_flow(update_data=...) call, computed by hand-written HIP kernels.
+"""
+
+from ._lib import LIB_PATH, lib
+from .errors import (
+    ConflictID,
+    ConflictVoltage,
+    DatasetError,
+    IDNotFound,
+    IDWrongType,
+    InvalidCalculationMethod,
+    InvalidTransformerClock,
+    IterationDiverge,
+    PowerGridBatchError,
+    PowerGridError,
+    SparseMatrixError,
+)
+from .meta import initialize_array, power_grid_meta_data
+from .model import (
+    BranchSide,
+    CalculationMethod,
+    CalculationType,
+    DeviceBatch,
+    FactorPrecision,
+    LoadGenType,
+    PowerGridModel,
+    WindingType,
+    block_lu_solve,
+    inspect_topology,
+    minimum_degree_ordering,
+    plan_stats,
+)
+
+__all__ = [
+    "LIB_PATH", "lib", "ConflictID", "ConflictVoltage", "DatasetError", "IDNotFound", "IDWrongType",
+    "InvalidCalculationMethod", "InvalidTransformerClock", "IterationDiverge", "PowerGridBatchError",
+    "PowerGridError", "SparseMatrixError", "initialize_array", "power_grid_meta_data", "BranchSide",
+    "CalculationMethod", "CalculationType", "DeviceBatch", "FactorPrecision", "LoadGenType", "PowerGridModel",
+    "WindingType", "block_lu_solve", "inspect_topology", "minimum_degree_ordering", "plan_stats",
+]

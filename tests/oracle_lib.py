@@ -220,3 +220,24 @@ def nr_math_model(p, n_bus, branch_bus, branch_param, shunt_bus, shunt_param, so
                                 err_tol, max_iter, _ptr(u), _ptr(bf), _ptr(sf), _ptr(lf), _ptr(srf), _ptr(inj))
     return it, {"u": u, "branch": bf, "shunt": sf[:nsh], "load": lf[:nl], "source": srf[:nsrc],
                 "bus_injection": inj}
+
+
+def uniform_update(update: dict, input_data: dict, scenarios=None) -> tuple[dict, int]:
+    """Convert a batch update in any of the package's forms (uniform 2D arrays or sparse {"indptr", "data"})
+    into the oracle's uniform form, padding scenarios with no-op rows (ids of untouched components,
+    null attributes). Optionally keep only the listed scenarios."""
+    from pgm_formats import batch_to_uniform
+
+    n = None
+    for v in update.values():
+        n = (len(v["indptr"]) - 1) if isinstance(v, dict) else v.shape[0]
+        break
+    sel = list(range(n)) if scenarios is None else list(scenarios)
+    per_scn = [dict() for _ in sel]
+    for comp, v in update.items():
+        for k, s in enumerate(sel):
+            if isinstance(v, dict):
+                per_scn[k][comp] = v["data"][v["indptr"][s]:v["indptr"][s + 1]]
+            else:
+                per_scn[k][comp] = v[s]
+    return batch_to_uniform(per_scn, input_data), len(sel)

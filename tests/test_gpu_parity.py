@@ -1,0 +1,210 @@
+"""GPU parity: the HIP path against the reference's fixtures and against the CPU oracle.
+
+Bars (north star): bit-exact integer work (bus order, fill-ins, per-scenario iteration counts of the
+FP64 path, failure sets); voltage magnitude within 1e-8 pu, angle within 1e-8 rad, branch / load p, q
+within 1e-6 relative. Flows that are numerically zero (loop lines carrying ~1e-4 W in a 10 MW grid) get
+an absolute floor of 1e-9 times the largest |value| of that attribute, the FP64 cancellation level.
+"""
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from validation_utils import compare_result, list_cases, load_case
+
+pgm = pytest.importorskip("power_grid_model_amd")
+from power_grid_model_amd import DeviceBatch, PowerGridModel, synthetic  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+U_TOL = 1e-8
+ANGLE_TOL = 1e-8
+PQ_RTOL = 1e-6
+
+
+# ------------------------------------------------------------------------------------------------
+# device block LU against the reference's known answers (test_sparse_lu_solver.cpp:70-250)
+# ------------------------------------------------------------------------------------------------
+FULL3 = dict(row_indptr=[0, 3, 6, 9], col_indices=[0, 1, 2, 0, 1, 2, 0, 1, 2], diag_lu=[0, 4, 8])
+BLOCKS_FILL = [[[0, 1], [100, 0]], [[1, 2], [7, -1]], [[3, 4], [5, 6]], [[1, 2], [-3, 4]], [[0, 200], [3, 1]],
+               [[0, 0], [0, 0]], [[5, 6], [-7, 8]], [[0, 0], [0, 0]], [[1, 0], [0, 100]]]
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 1e-4)])
+def test_device_block_lu_known_answers(precision, tol):
+    singular = [list(b) for b in BLOCKS_FILL]
+    singular[0] = [[0, 0], [100, 0]]
+    blocks = np.array([BLOCKS_FILL, singular] * 40, dtype=float)
+    rhs = np.array([[[38, 356], [-389, 2], [44, 611]]] * 80, dtype=float)
+    status, x = pgm.block_lu_solve(2, blocks=blocks, rhs=rhs, precision=precision, **FULL3)
+    assert (status[0::2] == 0).all() and (status[1::2] == 1).all()
+    np.testing.assert_allclose(x[0::2], np.broadcast_to([[3, 4], [-1, -2], [5, 6]], x[0::2].shape), atol=tol)
+
+
+def test_device_block_lu_meshed_and_6x6():
+    blocks = [[[20, 1], [2, 21]], [[1, 0], [2, -1]], [[-1, 2], [0, 1]], [[0, 1], [-2, 1]], [[22, -1], [1, 23]],
+              [[2, 1], [-1, 0]], [[0, 0], [0, 0]], [[1, -2], [0, 1]], [[24, 2], [-1, 25]], [[-2, 1], [1, 2]],
+              [[1, 1], [-1, 2]], [[0, 0], [0, 0]], [[0, -1], [2, 1]], [[26, -2], [1, 27]]]
+    pat = dict(row_indptr=[0, 3, 7, 10, 14], col_indices=[0, 1, 3, 0, 1, 2, 3, 1, 2, 3, 0, 1, 2, 3],
+               diag_lu=[0, 4, 8, 13])
+    status, x = pgm.block_lu_solve(2, blocks=np.array([blocks]), rhs=np.array([[[21, 40], [-17, 64], [82, -47],
+                                                                                [55, 38]]]), **pat)
+    assert status[0] == 0
+    np.testing.assert_allclose(x[0].ravel(), [1, 2, -1, 3, 4, -2, 2, 1], atol=1e-12)
+    # 6x6 blocks: compare with the oracle's block LU and a dense solve on random systems
+    rng = np.random.default_rng(11)
+    pattern = {0: [0, 1, 3], 1: [0, 1, 2, 3], 2: [1, 2, 3], 3: [0, 1, 2, 3]}
+    ip, ci, dg = [0], [], []
+    for r in range(4):
+        ci += pattern[r]
+        dg.append(ip[-1] + pattern[r].index(r))
+        ip.append(len(ci))
+    nsys = 70
+    bl = rng.normal(size=(nsys, len(ci), 6, 6))
+    for k, c in enumerate(ci):
+        r = next(rr for rr in range(4) if ip[rr] <= k < ip[rr + 1])
+        if r == c:
+            bl[:, k] += 20 * np.eye(6)
+    rhs = rng.normal(size=(nsys, 4, 6))
+    status, x = pgm.block_lu_solve(6, ip, ci, dg, bl, rhs)
+    assert (status == 0).all()
+    for s in (0, 33, 69):
+        rc, xo = ol.block_lu_solve(6, ip, ci, dg, bl[s], rhs[s])
+        np.testing.assert_allclose(x[s], xo, rtol=1e-10, atol=1e-12)
+
+
+# ------------------------------------------------------------------------------------------------
+# reference validation cases through PowerGridModel on the GPU
+# ------------------------------------------------------------------------------------------------
+CASES = list_cases()
+
+
+@pytest.mark.parametrize("name,sym,batch", CASES, ids=[f"{c[0]}-{'sym' if c[1] else 'asym'}"
+                                                       f"{'-batch' if c[2] else ''}" for c in CASES])
+def test_validation_case_gpu(name, sym, batch):
+    params, input_data, update, n_scenarios, expected = load_case(name, sym, batch)
+    model = PowerGridModel(input_data)
+    out = model.calculate_power_flow(symmetric=sym, update_data=update)
+    if not batch:
+        out = {k: v.reshape(1, *v.shape) for k, v in out.items()}
+    for s, exp in enumerate(expected):
+        compare_result(out, exp, params["rtol"], params["atol"], s)
+
+
+# ------------------------------------------------------------------------------------------------
+# synthetic workloads against the oracle
+# ------------------------------------------------------------------------------------------------
+def _subset(case, scenarios):
+    upd = {}
+    for comp, v in case.update.items():
+        if isinstance(v, dict):
+            ind = v["indptr"]
+            rows = [v["data"][ind[s]:ind[s + 1]] for s in scenarios]
+            indptr = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int64)
+            upd[comp] = {"indptr": indptr, "data": np.concatenate(rows) if rows else v["data"][:0]}
+        else:
+            upd[comp] = v[scenarios]
+    return upd
+
+
+def _compare(out, ref, sym, scenarios_ok):
+    node, rnode = out["node"][scenarios_ok], ref["node"][scenarios_ok]
+    assert (node["energized"] == rnode["energized"]).all()
+    np.testing.assert_allclose(node["u_pu"], rnode["u_pu"], atol=U_TOL, rtol=0)
+    en = rnode["energized"] == 1
+    da = np.angle(np.exp(1j * (node["u_angle"] - rnode["u_angle"])))
+    assert np.max(np.abs(np.where(en[..., None] if not sym else en, da, 0))) < ANGLE_TOL
+    for comp in ("line", "transformer", "sym_load", "asym_load", "source"):
+        if comp not in ref:
+            continue
+        a, r = out[comp][scenarios_ok], ref[comp][scenarios_ok]
+        fields = ("p_from", "q_from", "p_to", "q_to") if comp in ("line", "transformer") else ("p", "q")
+        for f in fields:
+            floor = 1e-9 * max(np.max(np.abs(r[f])), 1.0)
+            np.testing.assert_allclose(a[f], r[f], rtol=PQ_RTOL, atol=floor, err_msg=f"{comp}.{f}")
+        assert (a["energized"] == r["energized"]).all()
+
+
+WORKLOADS = [
+    ("radial_200", lambda: synthetic.radial_200(), list(range(100))),
+    ("meshed_1000", lambda: synthetic.meshed_1000(n_scenarios=300), list(range(300))),
+    ("asym_500", lambda: synthetic.asym_500(n_scenarios=130), list(range(130))),
+    ("feeders_64", lambda: synthetic.feeders_64(n_scenarios=6, n_feeder=16), list(range(6))),
+]
+
+
+@pytest.mark.parametrize("name,make,scen", WORKLOADS, ids=[w[0] for w in WORKLOADS])
+def test_workload_fp64_parity(name, make, scen):
+    case = make()
+    upd = _subset(case, scen)
+    model = PowerGridModel(case.input)
+    batch = DeviceBatch(model, upd, symmetric=case.symmetric)
+    batch.run()
+    out = batch.fetch()
+    it_gpu = batch.iterations()
+    oupd, k = ol.uniform_update(upd, case.input)
+    ref, it_ref, st_ref, _ = ol.power_flow(case.input, oupd, k, symmetric=case.symmetric, threading=0)
+    assert (st_ref == 0).all()
+    assert (it_gpu == it_ref).all(), (it_gpu, it_ref)
+    _compare(out, ref, case.symmetric, slice(None))
+
+
+def test_year_20k_sample_with_switching():
+    case = synthetic.year_20k()
+    ind = case.update["line"]["indptr"]
+    sw = np.flatnonzero(np.diff(ind) > 0)
+    scen = list(range(0, 40)) + list(sw[:24])
+    upd = _subset(case, scen)
+    model = PowerGridModel(case.input)
+    batch = DeviceBatch(model, upd)
+    batch.run()
+    out = batch.fetch()
+    oupd, k = ol.uniform_update(upd, case.input)
+    ref, it_ref, st_ref, _ = ol.power_flow(case.input, oupd, k, threading=0)
+    assert (st_ref == 0).all()
+    assert (batch.iterations() == it_ref).all()
+    _compare(out, ref, True, slice(None))
+
+
+@pytest.mark.parametrize("name,make,scen", WORKLOADS[:3], ids=[w[0] for w in WORKLOADS[:3]])
+def test_workload_mixed_precision(name, make, scen):
+    case = make()
+    upd = _subset(case, scen)
+    model = PowerGridModel(case.input)
+    batch = DeviceBatch(model, upd, symmetric=case.symmetric, factor_precision="fp32")
+    batch.run()
+    out = batch.fetch()
+    oupd, k = ol.uniform_update(upd, case.input)
+    ref, it_ref, st_ref, _ = ol.power_flow(case.input, oupd, k, symmetric=case.symmetric, threading=0)
+    assert (st_ref == 0).all()
+    # inexact Newton: the same 1e-8 pu stopping rule, at most two extra iterations
+    assert (batch.iterations() <= it_ref + 2).all()
+    _compare(out, ref, case.symmetric, slice(None))
+
+
+def test_error_cases_match_oracle():
+    case = synthetic.radial_200(n_scenarios=8)
+    upd = {k: v.copy() for k, v in case.update.items()}
+    upd["sym_load"]["p_specified"][3] *= 1e4  # far beyond loadability: no convergence
+    model = PowerGridModel(case.input)
+    out = model.calculate_power_flow(update_data=upd, continue_on_batch_error=True)
+    err = model.batch_error
+    ref, it_ref, st_ref, msg_ref = ol.power_flow(case.input, upd, 8)
+    assert list(err.failed_scenarios) == list(np.flatnonzero(st_ref))
+    for s, m in zip(err.failed_scenarios, err.error_messages):
+        assert m.split("Max deviation")[0] == msg_ref[s].split("Max deviation")[0]
+    ok = np.flatnonzero(st_ref == 0)
+    np.testing.assert_allclose(out["node"]["u_pu"][ok], ref["node"]["u_pu"][ok], atol=U_TOL)
+
+
+def test_singular_single_calculation_raises():
+    case = synthetic.radial_200(n_scenarios=1, n_node=5)
+    inp = {k: v.copy() for k, v in case.input.items()}
+    inp["line"]["r1"] = np.inf  # zero admittance everywhere: singular initial system
+    inp["line"]["x1"] = np.inf
+    inp["line"]["c1"] = 0.0
+    model = PowerGridModel(inp)
+    with pytest.raises(pgm.SparseMatrixError):
+        model.calculate_power_flow()
+    ref, _, st, msg = ol.power_flow(inp)
+    assert st[0] == 1
