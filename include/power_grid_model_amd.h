@@ -217,7 +217,10 @@ PGM_AMD_API void PGM_amd_batch_iterations(PGM_AmdBatch const* batch, PGM_Idx* it
  * out[1] = sum over scenarios of iterations, out[2] = number of bus blocks factorised per scenario
  * (sum over groups), out[3] = LU entries per scenario, out[4] = number of topology groups,
  * out[5] = average duration in ms of one forward-sweep launch, out[6] = number of forward-sweep
- * launches, out[7] = buses per scenario. */
+ * launches, out[7] = buses per scenario, out[8] = algorithmic bytes of the timed forward-sweep
+ * launches (factor written and read once, 2 E b, plus 32 P bytes per bus, for the scenarios still
+ * iterating), out[9] = their summed duration in ms (HIP events), out[10] = number of timed launches.
+ * out must hold 11 entries. */
 PGM_AMD_API void PGM_amd_batch_stats(PGM_AmdBatch const* batch, double* out) PGM_AMD_NOEXCEPT;
 
 /* Host-only inspection of the product's topology for a model (no GPU needed): per node the math

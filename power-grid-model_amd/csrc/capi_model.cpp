@@ -138,6 +138,9 @@ void PGM_amd_batch_stats(PGM_AmdBatch const* batch, double* out) noexcept {
     out[5] = b.fwd_launch_ms;
     out[6] = double(b.fwd_launches);
     out[7] = buses;
+    out[8] = b.fwd_bytes;
+    out[9] = b.fwd_total_ms;
+    out[10] = double(b.fwd_timed);
 }
 
 PGM_Idx PGM_amd_inspect_topology(PGM_Handle* handle, PGM_PowerGridModel const* model, PGM_Idx* node_group,

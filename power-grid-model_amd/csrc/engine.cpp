@@ -15,6 +15,7 @@
 #include <map>
 #include <numeric>
 #include <sstream>
+#include <thread>
 
 namespace pgm_amd {
 
@@ -25,16 +26,25 @@ namespace pgm_amd {
             throw PowerGridError(std::string("HIP error in ") + #expr + ": " + hipGetErrorString(e_) + "\n");    \
     } while (0)
 
-void DeviceBuffer::alloc(size_t bytes) {
-    release();
-    if (bytes == 0) return;
-    PGM_HIP_CHECK(hipMalloc(&ptr_, bytes));
-    bytes_ = bytes;
+Arena::~Arena() {
+    for (void* c : chunks_) (void)hipFree(c);
 }
-void DeviceBuffer::release() {
-    if (ptr_) (void)hipFree(ptr_);
-    ptr_ = nullptr;
-    bytes_ = 0;
+void* Arena::take(size_t bytes) {
+    bytes = (bytes + 255) / 256 * 256;
+    if (bytes == 0) return nullptr;
+    if (bytes > left_) {
+        size_t const chunk = std::max<size_t>(bytes, size_t(256) << 20);
+        void* p = nullptr;
+        PGM_HIP_CHECK(hipMalloc(&p, chunk));
+        chunks_.push_back(p);
+        cur_ = static_cast<char*>(p);
+        left_ = chunk;
+        reserved_ += chunk;
+    }
+    void* r = cur_;
+    cur_ += bytes;
+    left_ -= bytes;
+    return r;
 }
 
 std::string format_double(double x) {
@@ -62,11 +72,21 @@ void ensure_device() {
         throw PowerGridError("No HIP device available: this library runs the power flow on an MI355X (gfx950) GPU\n");
 }
 
-// blocking copy: the host vectors are often temporaries
-template <class T> void upload(DeviceBuffer& d, std::vector<T> const& h, hipStream_t st) {
-    (void)st;
-    d.alloc(h.size() * sizeof(T));
-    if (!h.empty()) PGM_HIP_CHECK(hipMemcpy(d.get(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+template <class T> T* dev_alloc(Arena& a, size_t count) { return static_cast<T*>(a.take(count * sizeof(T))); }
+// blocking copy into the arena (the host vectors are often temporaries); nullptr for an empty vector
+template <class T> T* put(Arena& a, std::vector<T> const& h) {
+    if (h.empty()) return nullptr;
+    T* d = dev_alloc<T>(a, h.size());
+    PGM_HIP_CHECK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+std::vector<double> complex_to_doubles(std::vector<cd> const& v) {
+    std::vector<double> r(v.size() * 2);
+    for (size_t i = 0; i < v.size(); ++i) {
+        r[2 * i] = v[i].real();
+        r[2 * i + 1] = v[i].imag();
+    }
+    return r;
 }
 
 bool set_status(IntS& tgt, IntS v) {
@@ -170,26 +190,14 @@ struct Group {
     std::vector<int> scenarios;
     int S = 0, S_pad = 0;
     int n_load = 0, n_src = 0, n_shunt = 0;
-    // device: structure
-    DeviceBuffer row_ptr, col, diag, transpose, fill, ydata, schur_ptr, schur, task_ptr, task_pivot, task_mm,
-        level_task, load_ptr, load_type, src_ptr, shunt_ptr, load_bus, src_bus, shunt_bus, shunt_y;
-    // device: per scenario
-    DeviceBuffer u, xv, y, lu, perm, s_inj, u_src, y_src, done, err, iters, devmax, lastdev, active;
-    DeviceBuffer scenario_index, ok, load_status, load_si, src_si, shunt_si, bus_inj;
-    // device: record maps
-    DeviceBuffer node_id, node_urated, node_bus;
-    DeviceBuffer br_id[2], br_math[2], br_bif[2], br_bit[2], br_rating[2], br_from, br_to, br_y;
-    DeviceBuffer ap_id[N_COMP], ap_math[N_COMP], ap_base_i[N_COMP], ap_status_const[N_COMP];
+    long long pair_off = 0, lane_off = 0;
     DevGroup dg;
     DevOutput dout;
-    std::vector<int> level_ptr;
-    int* h_active = nullptr;  // pinned
-    Group() = default;
-    Group(Group const&) = delete;
-    Group& operator=(Group const&) = delete;
-    ~Group() {
-        if (h_active) (void)hipHostFree(h_active);
-    }
+    NodeRec node_rec{};
+    BranchRec branch_rec[2]{};
+    ApplRec appl_rec[N_COMP]{};
+    int8_t* load_status = nullptr;
+    std::vector<double> level_entries, level_pivots;  // LU entries of the rows / pivots per level
 };
 
 namespace {
@@ -218,7 +226,275 @@ std::string structural_key(Model const& m, ComponentState const& st) {
     return k.str();
 }
 
-// the records of component `comp` of scenario s, as (index within the component, record pointer)
+
+// structure and per-scenario device arrays of one group (records maps included)
+void upload_group(Batch& b, Group& g) {
+    Model const& m = b.model;
+    Arena& A = b.arena;
+    Plan const& p = g.plan;
+    int const P = p.P, NB = 2 * P;
+    size_t const sp = size_t(g.S_pad);
+    size_t const n = size_t(p.n), E = size_t(p.E);
+    DevGroup& d = g.dg;
+    d.P = P;
+    d.n = int(n);
+    d.E = int(E);
+    d.n_mm = int(p.n_mm);
+    d.S = g.S;
+    d.S_pad = g.S_pad;
+    d.fp32 = b.fp32 ? 1 : 0;
+    d.row_ptr = put(A, p.row_ptr);
+    d.col = put(A, p.col);
+    d.diag = put(A, p.diag);
+    d.transpose = put(A, p.transpose);
+    d.fill = put(A, p.fill);
+    d.ydata = put(A, complex_to_doubles(p.ydata));
+    d.schur_ptr = put(A, p.schur_ptr);
+    std::vector<int> sch(p.schur.size() * 3);
+    for (size_t i = 0; i < p.schur.size(); ++i)
+        for (int k = 0; k < 3; ++k) sch[3 * i + k] = p.schur[i][k];
+    d.schur = put(A, sch);
+    d.task_ptr = put(A, p.task_ptr);
+    d.task_pivot = put(A, p.task_pivot);
+    d.task_mm = put(A, p.task_mm);
+    d.load_ptr = put(A, p.load_ptr);
+    d.load_type = put(A, p.load_type);
+    d.src_ptr = put(A, p.src_ptr);
+    g.n_load = int(p.load_comp.size());
+    g.n_src = int(p.src_comp.size());
+    g.n_shunt = int(p.shunt_comp.size());
+    size_t const fsz = b.fp32 ? sizeof(float) : sizeof(double);
+    d.u = dev_alloc<double>(A, n * P * 2 * sp);
+    d.xv = dev_alloc<double>(A, n * P * 2 * sp);
+    d.y = dev_alloc<double>(A, n * NB * sp);
+    d.lu = A.take(E * NB * NB * sp * fsz);
+    d.perm = dev_alloc<uint32_t>(A, n * sp);
+    d.done = b.pairs.done + g.pair_off;
+    d.err = b.pairs.err + g.pair_off;
+    d.iters = b.pairs.iters + g.pair_off;
+    d.devmax = b.pairs.devmax + g.pair_off;
+    d.lastdev = b.pairs.lastdev + g.pair_off;
+    for (size_t l = 0; l + 1 < p.level_ptr.size(); ++l) {
+        double ent = 0, piv = 0;
+        for (int q = p.level_ptr[l]; q < p.level_ptr[l + 1]; ++q) {
+            int const tk = p.level_task[q];
+            for (int i = p.task_ptr[tk]; i < p.task_ptr[tk + 1]; ++i) {
+                int const pv = p.task_pivot[i];
+                ent += p.row_ptr[pv + 1] - p.row_ptr[pv];
+                piv += 1;
+            }
+        }
+        g.level_entries.push_back(ent);
+        g.level_pivots.push_back(piv);
+    }
+
+    DevOutput& o = g.dout;
+    o.P = P;
+    o.S = g.S;
+    o.S_pad = g.S_pad;
+    std::vector<int> si(g.scenarios.begin(), g.scenarios.end());
+    si.resize(g.S_pad, 0);
+    o.scenario_index = put(A, si);
+    o.ok = b.d_ok + g.lane_off;
+    o.n = int(n);
+    o.row_ptr = d.row_ptr;
+    o.col = d.col;
+    o.fill = d.fill;
+    o.ydata = d.ydata;
+    o.u = d.u;
+    o.n_load = g.n_load;
+    o.load_type = d.load_type;
+    o.load_si = dev_alloc<double>(A, size_t(g.n_load) * 2 * P * 2 * sp);
+    o.n_src = g.n_src;
+    o.src_ptr = d.src_ptr;
+    o.load_ptr = d.load_ptr;
+    o.shunt_ptr = put(A, p.shunt_ptr);
+    o.src_si = dev_alloc<double>(A, size_t(g.n_src) * 2 * P * 2 * sp);
+    o.bus_inj = dev_alloc<double>(A, n * P * 2 * sp);
+    o.n_shunt = g.n_shunt;
+    o.shunt_y = put(A, complex_to_doubles(p.shunt_y));
+    o.shunt_si = dev_alloc<double>(A, size_t(g.n_shunt) * 2 * P * 2 * sp);
+
+    // record maps (the plan holds every math model of the topology, in order)
+    Topology const& t = g.topo;
+    {
+        std::vector<int> ids, bus;
+        std::vector<double> ur;
+        for (Idx i = 0; i < m.n_node(); ++i) {
+            ids.push_back(m.node[i].id);
+            ur.push_back(m.node[i].u_rated);
+            Idx const gg = t.node_group[i];
+            bus.push_back(gg < 0 ? -1 : p.mm_bus_begin[gg] + int(t.node_pos[i]));
+        }
+        g.node_rec = NodeRec{int(m.n_node()), put(A, ids), put(A, ur), put(A, bus)};
+    }
+    {
+        std::vector<int> mm_branch_begin(t.models.size() + 1, 0);
+        for (size_t k = 0; k < t.models.size(); ++k)
+            mm_branch_begin[k + 1] = mm_branch_begin[k] + int(t.models[k].branch.size());
+        int* br_from = put(A, p.branch_from);
+        int* br_to = put(A, p.branch_to);
+        double* br_y = put(A, complex_to_doubles(p.branch_y));
+        for (int kind = 0; kind < 2; ++kind) {
+            Idx const first = kind == 0 ? 0 : m.n_line();
+            Idx const cnt = kind == 0 ? m.n_line() : Idx(m.trafo.size());
+            std::vector<int> ids, math;
+            std::vector<double> bif, bit, rating;
+            for (Idx k = 0; k < cnt; ++k) {
+                Idx const bidx = first + k;
+                ids.push_back(kind == 0 ? m.line[k].id : m.trafo[k].id);
+                Idx const gg = t.branch_group[bidx];
+                math.push_back(gg < 0 ? -1 : mm_branch_begin[gg] + int(t.branch_pos[bidx]));
+                if (kind == 0) {
+                    bif.push_back(m.lstat[k].base_i);
+                    bit.push_back(m.lstat[k].base_i);
+                    rating.push_back(m.lstat[k].i_n);
+                } else {
+                    bif.push_back(m.tstat[k].base_i_from);
+                    bit.push_back(m.tstat[k].base_i_to);
+                    rating.push_back(m.tstat[k].sn);
+                }
+            }
+            g.branch_rec[kind] = BranchRec{int(cnt),       put(A, ids), put(A, math),   br_from, br_to, br_y,
+                                           put(A, bif), put(A, bit), put(A, rating), kind == 0 ? 1 : 0};
+        }
+    }
+    {
+        std::vector<int> lg_math(m.n_lg(), -1), src_math(m.source.size(), -1), sh_math(m.shunt.size(), -1);
+        for (size_t k = 0; k < p.load_comp.size(); ++k) lg_math[p.load_comp[k]] = int(k);
+        for (size_t k = 0; k < p.src_comp.size(); ++k) src_math[p.src_comp[k]] = int(k);
+        for (size_t k = 0; k < p.shunt_comp.size(); ++k) sh_math[p.shunt_comp[k]] = int(k);
+        auto make = [&](int comp, Idx cnt, auto id_of, auto math_of, auto node_of, auto stat_of, double dir,
+                        int8_t const* status_lanes, double const* si) {
+            std::vector<int> ids, math;
+            std::vector<double> bi;
+            std::vector<int8_t> stt;
+            for (Idx k = 0; k < cnt; ++k) {
+                ids.push_back(id_of(k));
+                math.push_back(math_of(k));
+                bi.push_back(kBaseP3 / m.node[node_of(k)].u_rated / kSqrt3);
+                stt.push_back(stat_of(k));
+            }
+            g.appl_rec[comp] = ApplRec{int(cnt), put(A, ids), put(A, math), put(A, bi), dir, status_lanes,
+                                       put(A, stt), si};
+        };
+        make(
+            C_SHUNT, Idx(m.shunt.size()), [&](Idx k) { return m.shunt[k].id; }, [&](Idx k) { return sh_math[k]; },
+            [&](Idx k) { return m.sh_node[k]; }, [&](Idx k) { return int8_t(g.st.sh_status[k] ? 1 : 0); }, -1.0,
+            nullptr, o.shunt_si);
+        make(
+            C_SOURCE, Idx(m.source.size()), [&](Idx k) { return m.source[k].id; },
+            [&](Idx k) { return src_math[k]; }, [&](Idx k) { return m.src_node[k]; },
+            [&](Idx k) { return int8_t(g.st.src_status[k] ? 1 : 0); }, 1.0, nullptr, o.src_si);
+        int const lg_comps[4] = {C_SYM_GEN, C_ASYM_GEN, C_SYM_LOAD, C_ASYM_LOAD};
+        for (int kk = 0; kk < 4; ++kk) {
+            Idx const off = m.lg_offset[kk];
+            make(
+                lg_comps[kk], m.lg_count[kk], [&](Idx k) { return m.lg_id[off + k]; },
+                [&](Idx k) { return lg_math[off + k]; }, [&](Idx k) { return m.lg_node[off + k]; },
+                [&](Idx) { return int8_t(0); }, kk < 2 ? 1.0 : -1.0, g.load_status + size_t(off) * sp, o.load_si);
+        }
+    }
+}
+
+// per-scenario injections, source voltages and admittances of one group
+template <class RowFn> void fill_scenario_data(Batch& b, Group& g, bool has_update, RowFn&& for_each_row) {
+    Model const& m = b.model;
+    int const P = b.P;
+    bool const sym = b.symmetric;
+    Plan const& p = g.plan;
+    size_t const sp = size_t(g.S_pad);
+    std::vector<double> s_inj(size_t(g.n_load) * P * 2 * sp, 0.0);
+    std::vector<int8_t> lstat(size_t(m.n_lg()) * sp, 0);
+    std::vector<double> u_src(size_t(g.n_src) * 2 * sp, 0.0), y_src(size_t(g.n_src) * 4 * sp, 0.0);
+    std::vector<int> lg_math(m.n_lg(), -1), src_math(m.source.size(), -1);
+    for (size_t k = 0; k < p.load_comp.size(); ++k) lg_math[p.load_comp[k]] = int(k);
+    for (size_t k = 0; k < p.src_comp.size(); ++k) src_math[p.src_comp[k]] = int(k);
+    auto put_load = [&](ComponentState const& st, Idx l, int lane) {
+        lstat[size_t(l) * sp + lane] = st.lg_status[l] ? 1 : 0;
+        int const k = lg_math[l];
+        if (k < 0) return;
+        auto const s3 = load_power(m, st, l, sym);
+        for (int ph = 0; ph < P; ++ph) {
+            s_inj[((size_t(k) * P + ph) * 2) * sp + lane] = s3[ph].real();
+            s_inj[((size_t(k) * P + ph) * 2 + 1) * sp + lane] = s3[ph].imag();
+        }
+    };
+    auto put_src = [&](ComponentState const& st, Idx sidx, int lane) {
+        int const k = src_math[sidx];
+        if (k < 0) return;
+        cd y1, y0;
+        source_param(st, sidx, y1, y0);
+        cd const ur = st.src_uref[sidx] * std::exp(cd(0.0, st.src_uang[sidx]));
+        u_src[(size_t(k) * 2) * sp + lane] = ur.real();
+        u_src[(size_t(k) * 2 + 1) * sp + lane] = ur.imag();
+        y_src[(size_t(k) * 4) * sp + lane] = y1.real();
+        y_src[(size_t(k) * 4 + 1) * sp + lane] = y1.imag();
+        y_src[(size_t(k) * 4 + 2) * sp + lane] = y0.real();
+        y_src[(size_t(k) * 4 + 3) * sp + lane] = y0.imag();
+    };
+    // base values in every lane
+    for (Idx l = 0; l < m.n_lg(); ++l) {
+        put_load(m.base, l, 0);
+        for (int lane = 1; lane < g.S; ++lane) {
+            lstat[size_t(l) * sp + lane] = lstat[size_t(l) * sp];
+            int const k = lg_math[l];
+            if (k < 0) continue;
+            for (int ph = 0; ph < P; ++ph)
+                for (int c = 0; c < 2; ++c)
+                    s_inj[((size_t(k) * P + ph) * 2 + c) * sp + lane] = s_inj[((size_t(k) * P + ph) * 2 + c) * sp];
+        }
+    }
+    for (Idx si = 0; si < Idx(m.source.size()); ++si)
+        for (int lane = 0; lane < g.S; ++lane) put_src(g.st, si, lane);
+    if (has_update) {
+        ComponentState work = m.base;
+        std::vector<std::pair<int, Idx>> touched;
+        for (int lane = 0; lane < g.S; ++lane) {
+            Idx const s = g.scenarios[lane];
+            touched.clear();
+            bool structural = false;
+            for_each_row(s, [&](int comp, Idx idx, void const* rec) {
+                if (comp == C_LINE || comp == C_TRANSFORMER || comp == C_SHUNT) return;
+                apply_update_record(m, work, comp, idx, rec, structural);
+                touched.emplace_back(comp, idx);
+            });
+            for (auto const& [comp, idx] : touched) {
+                if (comp == C_SOURCE) {
+                    put_src(work, idx, lane);
+                } else {
+                    int const kk = comp == C_SYM_GEN ? 0 : comp == C_ASYM_GEN ? 1 : comp == C_SYM_LOAD ? 2 : 3;
+                    put_load(work, m.lg_offset[kk] + idx, lane);
+                }
+            }
+            for (auto const& [comp, idx] : touched) {  // restore for the next lane
+                if (comp == C_SOURCE) {
+                    work.src_status[idx] = m.base.src_status[idx];
+                    work.src_uref[idx] = m.base.src_uref[idx];
+                    work.src_uang[idx] = m.base.src_uang[idx];
+                    work.src_sk[idx] = m.base.src_sk[idx];
+                    work.src_rx[idx] = m.base.src_rx[idx];
+                    work.src_z01[idx] = m.base.src_z01[idx];
+                } else {
+                    int const kk = comp == C_SYM_GEN ? 0 : comp == C_ASYM_GEN ? 1 : comp == C_SYM_LOAD ? 2 : 3;
+                    Idx const l = m.lg_offset[kk] + idx;
+                    work.lg_status[l] = m.base.lg_status[l];
+                    work.lg_p[l] = m.base.lg_p[l];
+                    work.lg_q[l] = m.base.lg_q[l];
+                }
+            }
+        }
+    }
+    Arena& A = b.arena;
+    g.dg.s_inj = put(A, s_inj);
+    g.dg.u_src = put(A, u_src);
+    g.dg.y_src = put(A, y_src);
+    g.dout.s_inj = g.dg.s_inj;
+    g.dout.u_src = g.dg.u_src;
+    g.dout.y_src = g.dg.y_src;
+    PGM_HIP_CHECK(hipMemcpy(g.load_status, lstat.data(), lstat.size(), hipMemcpyHostToDevice));
+}
+
 struct UpdateReader {
     ComponentBuffer const* buf = nullptr;
     std::vector<char> tmp;
@@ -231,257 +507,11 @@ struct UpdateReader {
 Batch::~Batch() {
     groups.clear();
     for (auto e : event_pool) (void)hipEventDestroy(e);
+    if (h_active) (void)hipHostFree(h_active);
     if (ev_start) (void)hipEventDestroy(ev_start);
     if (ev_stop) (void)hipEventDestroy(ev_stop);
-    if (ev_fwd0) (void)hipEventDestroy(ev_fwd0);
-    if (ev_fwd1) (void)hipEventDestroy(ev_fwd1);
     if (stream) (void)hipStreamDestroy(stream);
 }
-
-namespace {
-
-void upload_group(Batch& b, Group& g, Model const& m) {
-    Plan const& p = g.plan;
-    hipStream_t const st = b.stream;
-    int const P = p.P, NB = 2 * P;
-    g.S = int(g.scenarios.size());
-    g.S_pad = (g.S + 63) / 64 * 64;
-    size_t const sp = size_t(g.S_pad);
-    upload(g.row_ptr, p.row_ptr, st);
-    upload(g.col, p.col, st);
-    upload(g.diag, p.diag, st);
-    upload(g.transpose, p.transpose, st);
-    upload(g.fill, p.fill, st);
-    std::vector<double> yd(p.ydata.size() * 2);
-    for (size_t i = 0; i < p.ydata.size(); ++i) {
-        yd[2 * i] = p.ydata[i].real();
-        yd[2 * i + 1] = p.ydata[i].imag();
-    }
-    upload(g.ydata, yd, st);
-    upload(g.schur_ptr, p.schur_ptr, st);
-    std::vector<int> sch(p.schur.size() * 3);
-    for (size_t i = 0; i < p.schur.size(); ++i)
-        for (int k = 0; k < 3; ++k) sch[3 * i + k] = p.schur[i][k];
-    upload(g.schur, sch, st);
-    upload(g.task_ptr, p.task_ptr, st);
-    upload(g.task_pivot, p.task_pivot, st);
-    upload(g.task_mm, p.task_mm, st);
-    upload(g.level_task, p.level_task, st);
-    g.level_ptr = p.level_ptr;
-    upload(g.load_ptr, p.load_ptr, st);
-    upload(g.load_type, p.load_type, st);
-    upload(g.src_ptr, p.src_ptr, st);
-    upload(g.shunt_ptr, p.shunt_ptr, st);
-    g.n_load = int(p.load_comp.size());
-    g.n_src = int(p.src_comp.size());
-    g.n_shunt = int(p.shunt_comp.size());
-    std::vector<double> shy(p.shunt_y.size() * 2);
-    for (size_t i = 0; i < p.shunt_y.size(); ++i) {
-        shy[2 * i] = p.shunt_y[i].real();
-        shy[2 * i + 1] = p.shunt_y[i].imag();
-    }
-    upload(g.shunt_y, shy, st);
-
-    size_t const n = size_t(p.n), E = size_t(p.E);
-    size_t const fsz = b.fp32 ? sizeof(float) : sizeof(double);
-    g.u.alloc(n * P * 2 * sp * 8);
-    g.xv.alloc(n * P * 2 * sp * 8);
-    g.y.alloc(n * NB * sp * 8);
-    g.lu.alloc(E * NB * NB * sp * fsz);
-    g.perm.alloc(n * sp * 4);
-    g.done.alloc(size_t(p.n_mm) * sp * 4);
-    g.err.alloc(size_t(p.n_mm) * sp * 4);
-    g.iters.alloc(size_t(p.n_mm) * sp * 4);
-    g.devmax.alloc(size_t(p.n_mm) * sp * 8);
-    g.lastdev.alloc(size_t(p.n_mm) * sp * 8);
-    g.active.alloc(4);
-    g.load_si.alloc(size_t(g.n_load) * 2 * P * 2 * sp * 8);
-    g.src_si.alloc(size_t(g.n_src) * 2 * P * 2 * sp * 8);
-    g.shunt_si.alloc(size_t(g.n_shunt) * 2 * P * 2 * sp * 8);
-    g.bus_inj.alloc(n * P * 2 * sp * 8);
-    PGM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&g.h_active), sizeof(int)));
-    std::vector<int> si(g.scenarios.begin(), g.scenarios.end());
-    si.resize(g.S_pad, 0);
-    upload(g.scenario_index, si, st);
-    g.ok.alloc(sp * 4);
-
-    DevGroup& d = g.dg;
-    d.P = P;
-    d.n = int(n);
-    d.E = int(E);
-    d.n_mm = int(p.n_mm);
-    d.S = g.S;
-    d.S_pad = g.S_pad;
-    d.fp32 = b.fp32 ? 1 : 0;
-    d.row_ptr = g.row_ptr.as<int>();
-    d.col = g.col.as<int>();
-    d.diag = g.diag.as<int>();
-    d.transpose = g.transpose.as<int>();
-    d.fill = g.fill.as<int8_t>();
-    d.ydata = g.ydata.as<double>();
-    d.schur_ptr = g.schur_ptr.as<int>();
-    d.schur = g.schur.as<int>();
-    d.task_ptr = g.task_ptr.as<int>();
-    d.task_pivot = g.task_pivot.as<int>();
-    d.task_mm = g.task_mm.as<int>();
-    d.load_ptr = g.load_ptr.as<int>();
-    d.load_type = g.load_type.as<int8_t>();
-    d.src_ptr = g.src_ptr.as<int>();
-    d.u = g.u.as<double>();
-    d.xv = g.xv.as<double>();
-    d.y = g.y.as<double>();
-    d.lu = g.lu.get();
-    d.perm = g.perm.as<uint32_t>();
-    d.s_inj = g.s_inj.as<double>();
-    d.u_src = g.u_src.as<double>();
-    d.y_src = g.y_src.as<double>();
-    d.done = g.done.as<int>();
-    d.err = g.err.as<int>();
-    d.iters = g.iters.as<int>();
-    d.devmax = g.devmax.as<unsigned long long>();
-    d.lastdev = g.lastdev.as<double>();
-
-    DevOutput& o = g.dout;
-    o.P = P;
-    o.S = g.S;
-    o.S_pad = g.S_pad;
-    o.scenario_index = g.scenario_index.as<int>();
-    o.ok = g.ok.as<int>();
-    o.n = int(n);
-    o.row_ptr = d.row_ptr;
-    o.col = d.col;
-    o.fill = d.fill;
-    o.ydata = d.ydata;
-    o.u = d.u;
-    o.n_load = g.n_load;
-    o.load_type = d.load_type;
-    o.s_inj = d.s_inj;
-    o.load_si = g.load_si.as<double>();
-    o.n_src = g.n_src;
-    o.src_ptr = d.src_ptr;
-    o.load_ptr = d.load_ptr;
-    o.shunt_ptr = g.shunt_ptr.as<int>();
-    o.u_src = d.u_src;
-    o.y_src = d.y_src;
-    o.src_si = g.src_si.as<double>();
-    o.bus_inj = g.bus_inj.as<double>();
-    o.n_shunt = g.n_shunt;
-    o.shunt_y = g.shunt_y.as<double>();
-    o.shunt_si = g.shunt_si.as<double>();
-
-    // record maps
-    Topology const& t = g.topo;
-    std::vector<int> mm_rank(t.models.size(), -1);  // math model -> position in the plan
-    // the plan holds all models of the topology in order
-    for (size_t k = 0; k < t.models.size(); ++k) mm_rank[k] = int(k);
-    {
-        std::vector<int> ids, bus;
-        std::vector<double> ur;
-        for (Idx i = 0; i < m.n_node(); ++i) {
-            ids.push_back(m.node[i].id);
-            ur.push_back(m.node[i].u_rated);
-            Idx const gg = t.node_group[i];
-            bus.push_back(gg < 0 ? -1 : p.mm_bus_begin[mm_rank[gg]] + int(t.node_pos[i]));
-        }
-        upload(g.node_id, ids, st);
-        upload(g.node_urated, ur, st);
-        upload(g.node_bus, bus, st);
-    }
-    {
-        // math branch index in plan order: models concatenated
-        std::vector<int> mm_branch_begin(t.models.size() + 1, 0);
-        for (size_t k = 0; k < t.models.size(); ++k)
-            mm_branch_begin[k + 1] = mm_branch_begin[k] + int(t.models[k].branch.size());
-        for (int kind = 0; kind < 2; ++kind) {
-            Idx const first = kind == 0 ? 0 : m.n_line();
-            Idx const cnt = kind == 0 ? m.n_line() : Idx(m.trafo.size());
-            std::vector<int> ids, math;
-            std::vector<double> bif, bit, rating;
-            for (Idx k = 0; k < cnt; ++k) {
-                Idx const bidx = first + k;
-                ids.push_back(kind == 0 ? m.line[k].id : m.trafo[k].id);
-                Idx const gg = t.branch_group[bidx];
-                math.push_back(gg < 0 ? -1 : mm_branch_begin[mm_rank[gg]] + int(t.branch_pos[bidx]));
-                if (kind == 0) {
-                    bif.push_back(m.lstat[k].base_i);
-                    bit.push_back(m.lstat[k].base_i);
-                    rating.push_back(m.lstat[k].i_n);
-                } else {
-                    bif.push_back(m.tstat[k].base_i_from);
-                    bit.push_back(m.tstat[k].base_i_to);
-                    rating.push_back(m.tstat[k].sn);
-                }
-            }
-            upload(g.br_id[kind], ids, st);
-            upload(g.br_math[kind], math, st);
-            upload(g.br_bif[kind], bif, st);
-            upload(g.br_bit[kind], bit, st);
-            upload(g.br_rating[kind], rating, st);
-        }
-        upload(g.br_from, p.branch_from, st);
-        upload(g.br_to, p.branch_to, st);
-        std::vector<double> by(p.branch_y.size() * 2);
-        for (size_t i = 0; i < p.branch_y.size(); ++i) {
-            by[2 * i] = p.branch_y[i].real();
-            by[2 * i + 1] = p.branch_y[i].imag();
-        }
-        upload(g.br_y, by, st);
-    }
-    {
-        // appliance maps: index into the plan's load / source / shunt arrays
-        std::vector<int> lg_math(m.n_lg(), -1), src_math(m.source.size(), -1), sh_math(m.shunt.size(), -1);
-        for (size_t k = 0; k < p.load_comp.size(); ++k) lg_math[p.load_comp[k]] = int(k);
-        for (size_t k = 0; k < p.src_comp.size(); ++k) src_math[p.src_comp[k]] = int(k);
-        for (size_t k = 0; k < p.shunt_comp.size(); ++k) sh_math[p.shunt_comp[k]] = int(k);
-        auto set = [&](int comp, std::vector<int> const& ids, std::vector<int> const& math,
-                       std::vector<double> const& base_i, std::vector<int8_t> const& status) {
-            upload(g.ap_id[comp], ids, st);
-            upload(g.ap_math[comp], math, st);
-            upload(g.ap_base_i[comp], base_i, st);
-            upload(g.ap_status_const[comp], status, st);
-        };
-        {
-            std::vector<int> ids, math;
-            std::vector<double> bi;
-            std::vector<int8_t> stt;
-            for (size_t k = 0; k < m.shunt.size(); ++k) {
-                ids.push_back(m.shunt[k].id);
-                math.push_back(sh_math[k]);
-                bi.push_back(kBaseP3 / m.node[m.sh_node[k]].u_rated / kSqrt3);
-                stt.push_back(g.st.sh_status[k] ? 1 : 0);
-            }
-            set(C_SHUNT, ids, math, bi, stt);
-        }
-        {
-            std::vector<int> ids, math;
-            std::vector<double> bi;
-            std::vector<int8_t> stt;
-            for (size_t k = 0; k < m.source.size(); ++k) {
-                ids.push_back(m.source[k].id);
-                math.push_back(src_math[k]);
-                bi.push_back(kBaseP3 / m.node[m.src_node[k]].u_rated / kSqrt3);
-                stt.push_back(g.st.src_status[k] ? 1 : 0);
-            }
-            set(C_SOURCE, ids, math, bi, stt);
-        }
-        int const lg_comps[4] = {C_SYM_GEN, C_ASYM_GEN, C_SYM_LOAD, C_ASYM_LOAD};
-        for (int kk = 0; kk < 4; ++kk) {
-            std::vector<int> ids, math;
-            std::vector<double> bi;
-            std::vector<int8_t> stt;
-            for (Idx k = 0; k < m.lg_count[kk]; ++k) {
-                Idx const l = m.lg_offset[kk] + k;
-                ids.push_back(m.lg_id[l]);
-                math.push_back(lg_math[l]);
-                bi.push_back(kBaseP3 / m.node[m.lg_node[l]].u_rated / kSqrt3);
-                stt.push_back(0);
-            }
-            set(lg_comps[kk], ids, math, bi, stt);
-        }
-    }
-}
-
-}  // namespace
 
 std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, PGM_ConstDataset const* update) {
     if (opt.calculation_type != PGM_power_flow)
@@ -490,26 +520,25 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         throw PowerGridError("The calculation method is invalid for this calculation!");
     if (update && update->next) throw PowerGridError("Cartesian product batches are not supported by this build\n");
     ensure_device();
-    auto b = std::make_unique<Batch>();
-    b->model = model;
-    Model const& m = b->model;
-    b->symmetric = opt.symmetric != 0;
-    b->P = b->symmetric ? 1 : 3;
-    b->err_tol = opt.err_tol;
-    b->max_iter = opt.max_iter;
-    b->fp32 = opt.factor_precision == 1;
-    b->is_batch = update != nullptr;
-    b->n_scenarios = update ? update->batch_size : 1;
-    PGM_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    PGM_HIP_CHECK(hipEventCreate(&b->ev_start));
-    PGM_HIP_CHECK(hipEventCreate(&b->ev_stop));
-    PGM_HIP_CHECK(hipEventCreate(&b->ev_fwd0));
-    PGM_HIP_CHECK(hipEventCreate(&b->ev_fwd1));
-    Idx const S = b->n_scenarios;
-    bool const sym = b->symmetric;
-    int const P = b->P;
+    auto bp = std::make_unique<Batch>();
+    Batch& b = *bp;
+    b.model = model;
+    Model const& m = b.model;
+    b.symmetric = opt.symmetric != 0;
+    b.P = b.symmetric ? 1 : 3;
+    b.err_tol = opt.err_tol;
+    b.max_iter = opt.max_iter;
+    b.fp32 = opt.factor_precision == 1;
+    b.is_batch = update != nullptr;
+    b.n_scenarios = update ? update->batch_size : 1;
+    PGM_HIP_CHECK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+    PGM_HIP_CHECK(hipEventCreate(&b.ev_start));
+    PGM_HIP_CHECK(hipEventCreate(&b.ev_stop));
+    PGM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b.h_active), sizeof(int)));
+    Idx const S = b.n_scenarios;
+    bool const sym = b.symmetric;
 
-    // ---- update readers ----
+    // ---- update readers (update.hpp:60-200: rows by id, or by position when every id is NA) ----
     std::vector<UpdateReader> readers;
     if (update) {
         for (auto const& buf : update->buffers) {
@@ -521,7 +550,6 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             r.buf = &buf;
             r.comp = comp;
             r.tmp.resize(buf.meta->size);
-            // ids: all NA means positional update (update.hpp:60-90)
             bool all_na = true, any_na = false;
             for (Idx i = 0; i < buf.total_elements; ++i) {
                 buf.read_record(i, r.tmp.data());
@@ -551,156 +579,161 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             }
         }
     };
+    bool const has_structural_update = std::any_of(readers.begin(), readers.end(), [](UpdateReader const& r) {
+        return r.comp == C_LINE || r.comp == C_TRANSFORMER || r.comp == C_SHUNT || r.comp == C_SOURCE;
+    });
 
-    // ---- group scenarios by structural state ----
+    // ---- group scenarios by their structural state ----
     std::map<std::string, int> key_to_group;
-    std::vector<int> scn_group(S), scn_lane(S);
+    std::vector<int> scn_group(S, 0);
     std::vector<std::string> scn_error(S);
+    auto group_of = [&](std::string const& key, ComponentState const& st) {
+        auto it = key_to_group.find(key);
+        if (it != key_to_group.end()) return it->second;
+        int const gi = int(b.groups.size());
+        key_to_group.emplace(key, gi);
+        auto g = std::make_unique<Group>();
+        g->st = st;
+        b.groups.push_back(std::move(g));
+        return gi;
+    };
     for (Idx s = 0; s < S; ++s) {
-        ComponentState st = m.base;
-        bool structural = false;
         try {
-            for_each_row(s, [&](int comp, Idx idx, void const* rec) {
-                if (comp == C_LINE || comp == C_TRANSFORMER || comp == C_SHUNT || comp == C_SOURCE)
-                    apply_update_record(m, st, comp, idx, rec, structural);
-            });
+            if (!has_structural_update) {
+                // still resolve every id, so that an unknown id fails its scenario
+                for_each_row(s, [](int, Idx, void const*) {});
+                scn_group[s] = group_of(std::string{}, m.base);
+            } else {
+                ComponentState st = m.base;
+                bool structural = false;
+                for_each_row(s, [&](int comp, Idx idx, void const* rec) {
+                    if (comp == C_LINE || comp == C_TRANSFORMER || comp == C_SHUNT || comp == C_SOURCE)
+                        apply_update_record(m, st, comp, idx, rec, structural);
+                });
+                scn_group[s] = group_of(structural ? structural_key(m, st) : std::string{}, structural ? st : m.base);
+            }
         } catch (PowerGridError const& e) {
             if (!update) throw;
             scn_error[s] = e.what();
             scn_group[s] = -1;
             continue;
         }
-        std::string const key = structural ? structural_key(m, st) : std::string{};
-        auto it = key_to_group.find(key);
-        int gi;
-        if (it == key_to_group.end()) {
-            gi = int(b->groups.size());
-            key_to_group.emplace(key, gi);
-            auto g = std::make_unique<Group>();
-            g->st = st;
-            b->groups.push_back(std::move(g));
-        } else {
-            gi = it->second;
-        }
-        scn_group[s] = gi;
-        scn_lane[s] = int(b->groups[gi]->scenarios.size());
-        b->groups[gi]->scenarios.push_back(int(s));
-    }
-    if (!update && !scn_error[0].empty()) throw PowerGridError(scn_error[0]);
-
-    // ---- per group: topology, plan, upload ----
-    for (auto& gp : b->groups) {
-        Group& g = *gp;
-        g.topo = build_topology(m, g.st);
-        std::vector<Idx> all(g.topo.models.size());
-        std::iota(all.begin(), all.end(), Idx{0});
-        int const nblk = int((g.scenarios.size() + 63) / 64);
-        int const target = std::clamp(4096 / std::max(nblk, 1), 4, 256);
-        g.plan = build_plan(m, g.st, g.topo, all, sym, target);
-        upload_group(*b, g, m);
+        b.groups[scn_group[s]]->scenarios.push_back(int(s));
     }
 
-    // ---- per-scenario injections, source voltages and admittances ----
-    for (auto& gp : b->groups) {
+    // ---- per group: topology and plan, built in parallel on the host ----
+    {
+        size_t const G = b.groups.size();
+        unsigned const hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::string> errs(G);
+        auto work = [&](size_t t) {
+            for (size_t gi = t; gi < G; gi += hw) {
+                try {
+                    Group& g = *b.groups[gi];
+                    g.topo = build_topology(m, g.st);
+                    std::vector<Idx> all(g.topo.models.size());
+                    std::iota(all.begin(), all.end(), Idx{0});
+                    int const nblk = int((g.scenarios.size() + 63) / 64);
+                    int const target = std::clamp(4096 / std::max(nblk, 1), 4, 256);
+                    g.plan = build_plan(m, g.st, g.topo, all, sym, target);
+                } catch (std::exception const& e) {
+                    errs[gi] = e.what();
+                }
+            }
+        };
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < hw; ++t) th.emplace_back(work, t);
+        for (auto& x : th) x.join();
+        for (auto const& e : errs)
+            if (!e.empty()) throw PowerGridError(e);
+    }
+
+    // ---- batch-wide iteration state and lanes ----
+    long long pairs = 0, lanes = 0;
+    for (auto& gp : b.groups) {
         Group& g = *gp;
-        Plan const& p = g.plan;
-        size_t const sp = size_t(g.S_pad);
-        std::vector<double> s_inj(size_t(g.n_load) * P * 2 * sp, 0.0);
-        std::vector<int8_t> lstat(size_t(m.n_lg()) * sp, 0);
-        std::vector<double> u_src(size_t(g.n_src) * 2 * sp, 0.0), y_src(size_t(g.n_src) * 4 * sp, 0.0);
-        std::vector<int> lg_math(m.n_lg(), -1), src_math(m.source.size(), -1);
-        for (size_t k = 0; k < p.load_comp.size(); ++k) lg_math[p.load_comp[k]] = int(k);
-        for (size_t k = 0; k < p.src_comp.size(); ++k) src_math[p.src_comp[k]] = int(k);
-        // base values broadcast to every lane
-        auto put_load = [&](ComponentState const& st, Idx l, int lane) {
-            lstat[size_t(l) * sp + lane] = st.lg_status[l] ? 1 : 0;
-            int const k = lg_math[l];
-            if (k < 0) return;
-            auto const s3 = load_power(m, st, l, sym);
-            for (int ph = 0; ph < P; ++ph) {
-                s_inj[((size_t(k) * P + ph) * 2) * sp + lane] = s3[ph].real();
-                s_inj[((size_t(k) * P + ph) * 2 + 1) * sp + lane] = s3[ph].imag();
-            }
-        };
-        auto put_src = [&](ComponentState const& st, Idx sidx, int lane) {
-            int const k = src_math[sidx];
-            if (k < 0) return;
-            cd y1, y0;
-            source_param(st, sidx, y1, y0);
-            cd const ur = st.src_uref[sidx] * std::exp(cd(0.0, st.src_uang[sidx]));
-            u_src[(size_t(k) * 2) * sp + lane] = ur.real();
-            u_src[(size_t(k) * 2 + 1) * sp + lane] = ur.imag();
-            y_src[(size_t(k) * 4) * sp + lane] = y1.real();
-            y_src[(size_t(k) * 4 + 1) * sp + lane] = y1.imag();
-            y_src[(size_t(k) * 4 + 2) * sp + lane] = y0.real();
-            y_src[(size_t(k) * 4 + 3) * sp + lane] = y0.imag();
-        };
-        for (int lane = 0; lane < g.S; ++lane) {
-            Idx const s = g.scenarios[lane];
-            (void)s;
+        g.S = int(g.scenarios.size());
+        g.S_pad = g.S > 32 ? (g.S + 63) / 64 * 64 : g.S;
+        g.pair_off = pairs;
+        g.lane_off = lanes;
+        pairs += (long long)(g.plan.n_mm) * g.S_pad;
+        lanes += g.S_pad;
+        b.real_pairs += (long long)(g.plan.n_mm) * g.S;
+        b.total_lanes += g.S;
+    }
+    b.pairs.n = pairs;
+    b.pairs.done = dev_alloc<int>(b.arena, size_t(pairs));
+    b.pairs.err = dev_alloc<int>(b.arena, size_t(pairs));
+    b.pairs.iters = dev_alloc<int>(b.arena, size_t(pairs));
+    b.pairs.devmax = dev_alloc<unsigned long long>(b.arena, size_t(pairs));
+    b.pairs.lastdev = dev_alloc<double>(b.arena, size_t(pairs));
+    b.d_active = dev_alloc<int>(b.arena, 1);
+    b.d_ok = dev_alloc<int>(b.arena, size_t(std::max<long long>(lanes, 1)));
+    int8_t* lstat_all = dev_alloc<int8_t>(b.arena, size_t(std::max<long long>(lanes * m.n_lg(), 1)));
+    {
+        long long off = 0;
+        for (auto& gp : b.groups) {
+            gp->load_status = lstat_all + off;
+            off += (long long)(gp->S_pad) * m.n_lg();
         }
-        // fill base for every lane, then apply each scenario's own load / source rows
-        for (Idx l = 0; l < m.n_lg(); ++l) {
-            put_load(m.base, l, 0);
-            for (int lane = 1; lane < g.S; ++lane) {
-                lstat[size_t(l) * sp + lane] = lstat[size_t(l) * sp];
-                int const k = lg_math[l];
-                if (k < 0) continue;
-                for (int ph = 0; ph < P; ++ph)
-                    for (int c = 0; c < 2; ++c)
-                        s_inj[((size_t(k) * P + ph) * 2 + c) * sp + lane] = s_inj[((size_t(k) * P + ph) * 2 + c) * sp];
-            }
+    }
+    for (auto& gp : b.groups) {
+        upload_group(b, *gp);
+        fill_scenario_data(b, *gp, update != nullptr, for_each_row);
+    }
+
+    // ---- device tables and work lists ----
+    size_t const G = b.groups.size();
+    {
+        std::vector<DevGroup> dg;
+        std::vector<DevOutput> dout;
+        std::vector<NodeRec> nr;
+        std::vector<BranchRec> br[2];
+        std::vector<ApplRec> ar[N_COMP];
+        for (auto& gp : b.groups) {
+            dg.push_back(gp->dg);
+            dout.push_back(gp->dout);
+            nr.push_back(gp->node_rec);
+            for (int k = 0; k < 2; ++k) br[k].push_back(gp->branch_rec[k]);
+            for (int c = 0; c < N_COMP; ++c) ar[c].push_back(gp->appl_rec[c]);
         }
-        for (Idx si = 0; si < Idx(m.source.size()); ++si)
-            for (int lane = 0; lane < g.S; ++lane) put_src(g.st, si, lane);
-        if (update) {
-            ComponentState work = m.base;
-            for (int lane = 0; lane < g.S; ++lane) {
-                Idx const s = g.scenarios[lane];
-                std::vector<std::pair<int, Idx>> touched;
-                bool structural = false;
-                for_each_row(s, [&](int comp, Idx idx, void const* rec) {
-                    if (comp == C_LINE || comp == C_TRANSFORMER || comp == C_SHUNT) return;
-                    apply_update_record(m, work, comp, idx, rec, structural);
-                    touched.emplace_back(comp, idx);
-                });
-                for (auto const& [comp, idx] : touched) {
-                    if (comp == C_SOURCE) {
-                        put_src(work, idx, lane);
-                    } else {
-                        int const kk = comp == C_SYM_GEN ? 0 : comp == C_ASYM_GEN ? 1 : comp == C_SYM_LOAD ? 2 : 3;
-                        put_load(work, m.lg_offset[kk] + idx, lane);
-                    }
-                }
-                // restore the touched entries for the next lane
-                for (auto const& [comp, idx] : touched) {
-                    if (comp == C_SOURCE) {
-                        work.src_status[idx] = m.base.src_status[idx];
-                        work.src_uref[idx] = m.base.src_uref[idx];
-                        work.src_uang[idx] = m.base.src_uang[idx];
-                        work.src_sk[idx] = m.base.src_sk[idx];
-                        work.src_rx[idx] = m.base.src_rx[idx];
-                        work.src_z01[idx] = m.base.src_z01[idx];
-                    } else {
-                        int const kk = comp == C_SYM_GEN ? 0 : comp == C_ASYM_GEN ? 1 : comp == C_SYM_LOAD ? 2 : 3;
-                        Idx const l = m.lg_offset[kk] + idx;
-                        work.lg_status[l] = m.base.lg_status[l];
-                        work.lg_p[l] = m.base.lg_p[l];
-                        work.lg_q[l] = m.base.lg_q[l];
-                    }
-                }
+        b.d_groups = put(b.arena, dg);
+        b.d_outputs = put(b.arena, dout);
+        b.d_node_recs = put(b.arena, nr);
+        for (int k = 0; k < 2; ++k) b.d_branch_recs[k] = put(b.arena, br[k]);
+        for (int c = 0; c < N_COMP; ++c)
+            if (c == C_SHUNT || c == C_SOURCE || c >= C_SYM_GEN) b.d_appl_recs[c] = put(b.arena, ar[c]);
+        int max_levels = 0;
+        for (auto& gp : b.groups) max_levels = std::max(max_levels, int(gp->plan.level_ptr.size()) - 1);
+        double const fsz = b.fp32 ? 4.0 : 8.0;
+        for (int l = 0; l < max_levels; ++l) {
+            std::vector<WorkItem> items;
+            double bytes = 0;
+            for (size_t gi = 0; gi < G; ++gi) {
+                Group const& g = *b.groups[gi];
+                Plan const& p = g.plan;
+                if (l + 1 >= int(p.level_ptr.size()) || g.S == 0) continue;
+                int const nblk = (g.S + 63) / 64;
+                for (int q = p.level_ptr[l]; q < p.level_ptr[l + 1]; ++q)
+                    for (int blk = 0; blk < nblk; ++blk) items.push_back(WorkItem{int(gi), p.level_task[q], blk});
+                // factor written once and read once by the forward substitution (2 E b), voltages read
+                // and the forward vector written (32 P bytes per bus)
+                bytes += (2.0 * g.level_entries[l] * fsz * 4 * b.P * b.P + 32.0 * b.P * g.level_pivots[l]) * g.S;
             }
+            b.d_items.push_back(put(b.arena, items));
+            b.n_items.push_back(int(items.size()));
+            b.level_bytes_full.push_back(bytes);
         }
-        upload(g.s_inj, s_inj, b->stream);
-        upload(g.u_src, u_src, b->stream);
-        upload(g.y_src, y_src, b->stream);
-        upload(g.load_status, lstat, b->stream);
-        g.dg.s_inj = g.s_inj.as<double>();
-        g.dg.u_src = g.u_src.as<double>();
-        g.dg.y_src = g.y_src.as<double>();
-        g.dout.s_inj = g.dg.s_inj;
-        g.dout.u_src = g.dg.u_src;
-        g.dout.y_src = g.dg.y_src;
+        std::vector<long long> bus_off{0};
+        for (auto& gp : b.groups) bus_off.push_back(bus_off.back() + (long long)(gp->plan.n) * gp->S_pad);
+        b.bus_total = bus_off.back();
+        b.d_bus_off = put(b.arena, bus_off);
+        for (int c = 0; c < N_COMP; ++c) {
+            std::vector<long long> off{0};
+            for (auto& gp : b.groups) off.push_back(off.back() + (long long)(m.count(c)) * gp->S_pad);
+            b.rec_total[c] = off.back();
+            b.d_rec_off[c] = put(b.arena, off);
+        }
     }
 
     // ---- output record buffers ----
@@ -710,26 +743,25 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
                           : (c == C_LINE || c == C_TRANSFORMER)
                               ? (sym ? sizeof(BranchOutput<1>) : sizeof(BranchOutput<3>))
                               : (sym ? sizeof(ApplianceOutput<1>) : sizeof(ApplianceOutput<3>));
-        b->rec_size[c] = rs;
-        if (cnt > 0) b->out[c].alloc(size_t(S) * size_t(cnt) * rs);
+        b.rec_size[c] = rs;
+        if (cnt > 0) b.out[c] = b.arena.take(size_t(S) * size_t(cnt) * rs);
     }
-    b->status.assign(S, 0);
-    b->iterations.assign(S, 0);
-    b->messages.assign(S, "");
+    b.status.assign(S, 0);
+    b.iterations.assign(S, 0);
+    b.messages.assign(S, "");
     for (Idx s = 0; s < S; ++s)
         if (scn_group[s] < 0) {
-            b->status[s] = 3;
-            b->messages[s] = scn_error[s];
+            b.status[s] = 3;
+            b.messages[s] = scn_error[s];
         }
-    PGM_HIP_CHECK(hipStreamSynchronize(b->stream));
-    return b;
+    PGM_HIP_CHECK(hipDeviceSynchronize());
+    return bp;
 }
 
 void Batch::run() {
     hipStream_t const st = stream;
-    PGM_HIP_CHECK(hipEventRecord(ev_start, st));
     fwd_launches = 0;
-    double fwd_ms = 0.0;
+    fwd_bytes = 0.0;
     std::vector<hipEvent_t> fwd_events;
     auto event_from_pool = [&](size_t i) {
         while (event_pool.size() <= i) {
@@ -739,63 +771,57 @@ void Batch::run() {
         }
         return event_pool[i];
     };
-    for (auto& gp : groups) {
-        Group& g = *gp;
-        if (g.S == 0) continue;
-        DevGroup const& d = g.dg;
-        int const L = int(g.level_ptr.size()) - 1;
-        int const* lt = g.level_task.as<int>();
-        auto level = [&](int l) { return std::make_pair(lt + g.level_ptr[l], g.level_ptr[l + 1] - g.level_ptr[l]); };
-        launch_init_state(d, st);
-        for (int l = 0; l < L; ++l) launch_forward(d, MODE_INIT, level(l).first, level(l).second, st);
-        for (int l = L - 1; l >= 0; --l) launch_backward(d, MODE_INIT, level(l).first, level(l).second, st);
-        for (int it = 1; it <= int(max_iter); ++it) {
-            for (int l = 0; l < L; ++l) {
-                bool const timed = fwd_events.size() < 2 * kMaxTimedLaunches;
-                if (timed) {
-                    fwd_events.push_back(event_from_pool(fwd_events.size()));
-                    PGM_HIP_CHECK(hipEventRecord(fwd_events.back(), st));
-                }
-                launch_forward(d, MODE_NR, level(l).first, level(l).second, st);
-                if (timed) {
-                    fwd_events.push_back(event_from_pool(fwd_events.size()));
-                    PGM_HIP_CHECK(hipEventRecord(fwd_events.back(), st));
-                }
-                ++fwd_launches;
-            }
-            for (int l = L - 1; l >= 0; --l) launch_backward(d, MODE_NR, level(l).first, level(l).second, st);
-            *g.h_active = 0;
-            PGM_HIP_CHECK(hipMemcpyAsync(g.active.get(), g.h_active, sizeof(int), hipMemcpyHostToDevice, st));
-            launch_converge(d, it, int(max_iter), err_tol, g.active.as<int>(), st);
-            PGM_HIP_CHECK(hipMemcpyAsync(g.h_active, g.active.get(), sizeof(int), hipMemcpyDeviceToHost, st));
-            PGM_HIP_CHECK(hipStreamSynchronize(st));
-            if (*g.h_active == 0) break;
+    int const L = int(n_items.size());
+    PGM_HIP_CHECK(hipEventRecord(ev_start, st));
+    launch_init_state(pairs, st);
+    for (int l = 0; l < L; ++l) launch_forward(d_groups, d_items[l], n_items[l], P, fp32, MODE_INIT, st);
+    for (int l = L - 1; l >= 0; --l) launch_backward(d_groups, d_items[l], n_items[l], P, fp32, MODE_INIT, st);
+    double active_frac = 1.0;
+    for (int it = 1; it <= int(max_iter); ++it) {
+        for (int l = 0; l < L; ++l) {
+            fwd_events.push_back(event_from_pool(fwd_events.size()));
+            PGM_HIP_CHECK(hipEventRecord(fwd_events.back(), st));
+            launch_forward(d_groups, d_items[l], n_items[l], P, fp32, MODE_NR, st);
+            fwd_events.push_back(event_from_pool(fwd_events.size()));
+            PGM_HIP_CHECK(hipEventRecord(fwd_events.back(), st));
+            fwd_bytes += level_bytes_full[l] * active_frac;
+            ++fwd_launches;
         }
-        PGM_HIP_CHECK(hipGetLastError());
+        for (int l = L - 1; l >= 0; --l) launch_backward(d_groups, d_items[l], n_items[l], P, fp32, MODE_NR, st);
+        *h_active = 0;
+        PGM_HIP_CHECK(hipMemcpyAsync(d_active, h_active, sizeof(int), hipMemcpyHostToDevice, st));
+        launch_converge(pairs, it, int(max_iter), err_tol, d_active, st);
+        PGM_HIP_CHECK(hipMemcpyAsync(h_active, d_active, sizeof(int), hipMemcpyDeviceToHost, st));
+        PGM_HIP_CHECK(hipStreamSynchronize(st));
+        // lanes still iterating, for the byte count of the next iteration's launches
+        active_frac = real_pairs > 0 ? double(*h_active) / double(real_pairs) : 0.0;
+        if (*h_active == 0) break;
     }
+    PGM_HIP_CHECK(hipGetLastError());
     // per-scenario status
+    size_t const np = size_t(pairs.n);
+    std::vector<int> done(np), err(np), its(np);
+    std::vector<double> ldev(np);
+    if (np > 0) {
+        PGM_HIP_CHECK(hipMemcpyAsync(err.data(), pairs.err, np * 4, hipMemcpyDeviceToHost, st));
+        PGM_HIP_CHECK(hipMemcpyAsync(its.data(), pairs.iters, np * 4, hipMemcpyDeviceToHost, st));
+        PGM_HIP_CHECK(hipMemcpyAsync(ldev.data(), pairs.lastdev, np * 8, hipMemcpyDeviceToHost, st));
+        PGM_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    long long lanes = 0;
+    for (auto& gp : groups) lanes += gp->S_pad;
+    std::vector<int> ok(size_t(std::max<long long>(lanes, 1)), 0);
     for (auto& gp : groups) {
         Group& g = *gp;
-        if (g.S == 0) continue;
         size_t const sp = size_t(g.S_pad);
-        size_t const nmm = size_t(g.dg.n_mm);
-        std::vector<int> done(nmm * sp), err(nmm * sp), its(nmm * sp);
-        std::vector<double> ldev(nmm * sp);
-        if (nmm > 0) {
-            PGM_HIP_CHECK(hipMemcpyAsync(done.data(), g.done.get(), done.size() * 4, hipMemcpyDeviceToHost, st));
-            PGM_HIP_CHECK(hipMemcpyAsync(err.data(), g.err.get(), err.size() * 4, hipMemcpyDeviceToHost, st));
-            PGM_HIP_CHECK(hipMemcpyAsync(its.data(), g.iters.get(), its.size() * 4, hipMemcpyDeviceToHost, st));
-            PGM_HIP_CHECK(hipMemcpyAsync(ldev.data(), g.lastdev.get(), ldev.size() * 8, hipMemcpyDeviceToHost, st));
-            PGM_HIP_CHECK(hipStreamSynchronize(st));
-        }
-        std::vector<int> ok(sp, 0);
+        size_t const nmm = size_t(g.plan.n_mm);
         for (int lane = 0; lane < g.S; ++lane) {
             Idx const s = g.scenarios[lane];
             int stt = 0;
             Idx it_max = 0;
             std::string msg;
             for (size_t k = 0; k < nmm; ++k) {
-                size_t const i = k * sp + lane;
+                size_t const i = size_t(g.pair_off) + k * sp + lane;
                 if (stt == 0 && err[i] == ERR_SPARSE) {
                     stt = 1;
                     msg = SparseMatrixErrorMessage;
@@ -810,50 +836,25 @@ void Batch::run() {
             status[s] = stt;
             iterations[s] = stt == 0 ? it_max : 0;
             messages[s] = msg;
-            ok[lane] = stt == 0 ? 1 : 0;
+            ok[size_t(g.lane_off) + lane] = stt == 0 ? 1 : 0;
         }
-        PGM_HIP_CHECK(hipMemcpy(g.ok.get(), ok.data(), sp * 4, hipMemcpyHostToDevice));
-        // results and records
-        launch_bus_results(g.dout, st);
-        NodeRec nr{int(model.n_node()), g.node_id.as<int>(), g.node_urated.as<double>(), g.node_bus.as<int>()};
-        launch_node_records(g.dout, nr, out[C_NODE].get(), st);
-        for (int kind = 0; kind < 2; ++kind) {
-            int const comp = kind == 0 ? C_LINE : C_TRANSFORMER;
-            BranchRec br{int(model.count(comp)), g.br_id[kind].as<int>(), g.br_math[kind].as<int>(),
-                         g.br_from.as<int>(), g.br_to.as<int>(), g.br_y.as<double>(), g.br_bif[kind].as<double>(),
-                         g.br_bit[kind].as<double>(), g.br_rating[kind].as<double>(), kind == 0 ? 1 : 0};
-            launch_branch_records(g.dout, br, out[comp].get(), st);
-        }
-        {
-            ApplRec ar{int(model.count(C_SHUNT)), g.ap_id[C_SHUNT].as<int>(), g.ap_math[C_SHUNT].as<int>(),
-                       g.ap_base_i[C_SHUNT].as<double>(), -1.0, nullptr, g.ap_status_const[C_SHUNT].as<int8_t>(),
-                       g.shunt_si.as<double>()};
-            launch_appliance_records(g.dout, ar, out[C_SHUNT].get(), st);
-            ApplRec as{int(model.count(C_SOURCE)), g.ap_id[C_SOURCE].as<int>(), g.ap_math[C_SOURCE].as<int>(),
-                       g.ap_base_i[C_SOURCE].as<double>(), 1.0, nullptr, g.ap_status_const[C_SOURCE].as<int8_t>(),
-                       g.src_si.as<double>()};
-            launch_appliance_records(g.dout, as, out[C_SOURCE].get(), st);
-        }
-        int const lg_comps[4] = {C_SYM_GEN, C_ASYM_GEN, C_SYM_LOAD, C_ASYM_LOAD};
-        for (int kk = 0; kk < 4; ++kk) {
-            int const comp = lg_comps[kk];
-            ApplRec al{int(model.count(comp)),
-                       g.ap_id[comp].as<int>(),
-                       g.ap_math[comp].as<int>(),
-                       g.ap_base_i[comp].as<double>(),
-                       (comp == C_SYM_GEN || comp == C_ASYM_GEN) ? 1.0 : -1.0,
-                       g.load_status.as<int8_t>() + size_t(model.lg_offset[kk]) * g.S_pad,
-                       nullptr,
-                       g.load_si.as<double>()};
-            launch_appliance_records(g.dout, al, out[comp].get(), st);
-        }
-        PGM_HIP_CHECK(hipGetLastError());
     }
+    PGM_HIP_CHECK(hipMemcpy(d_ok, ok.data(), ok.size() * 4, hipMemcpyHostToDevice));
+    int const G = int(groups.size());
+    launch_bus_results(P, d_outputs, d_bus_off, G, bus_total, st);
+    launch_node_records(P, d_outputs, d_node_recs, d_rec_off[C_NODE], G, rec_total[C_NODE], out[C_NODE], st);
+    launch_branch_records(P, d_outputs, d_branch_recs[0], d_rec_off[C_LINE], G, rec_total[C_LINE], out[C_LINE], st);
+    launch_branch_records(P, d_outputs, d_branch_recs[1], d_rec_off[C_TRANSFORMER], G, rec_total[C_TRANSFORMER],
+                          out[C_TRANSFORMER], st);
+    for (int c : {int(C_SHUNT), int(C_SOURCE), int(C_SYM_GEN), int(C_ASYM_GEN), int(C_SYM_LOAD), int(C_ASYM_LOAD)})
+        launch_appliance_records(P, d_outputs, d_appl_recs[c], d_rec_off[c], G, rec_total[c], out[c], st);
+    PGM_HIP_CHECK(hipGetLastError());
     PGM_HIP_CHECK(hipEventRecord(ev_stop, st));
     PGM_HIP_CHECK(hipEventSynchronize(ev_stop));
     float ms = 0;
     PGM_HIP_CHECK(hipEventElapsedTime(&ms, ev_start, ev_stop));
     kernel_ms = ms;
+    double fwd_ms = 0.0;
     for (size_t i = 0; i + 1 < fwd_events.size(); i += 2) {
         float e = 0;
         PGM_HIP_CHECK(hipEventElapsedTime(&e, fwd_events[i], fwd_events[i + 1]));
@@ -861,6 +862,7 @@ void Batch::run() {
     }
     fwd_timed = Idx(fwd_events.size() / 2);
     fwd_launch_ms = fwd_timed > 0 ? fwd_ms / double(fwd_timed) : 0.0;
+    fwd_total_ms = fwd_ms;
     ran = true;
 }
 
@@ -884,11 +886,11 @@ void Batch::fetch(PGM_MutableDataset const& output, PGM_Handle* handle) {
         bool const direct = !buf.columnar() && buf.elements_per_scenario == cnt;
         bool const all_ok = std::all_of(status.begin(), status.end(), [](int x) { return x == 0; });
         if (direct && all_ok) {
-            PGM_HIP_CHECK(hipMemcpy(buf.data, out[comp].get(), size_t(n_scenarios) * cnt * rs, hipMemcpyDeviceToHost));
+            PGM_HIP_CHECK(hipMemcpy(buf.data, out[comp], size_t(n_scenarios) * cnt * rs, hipMemcpyDeviceToHost));
             continue;
         }
         host.resize(size_t(n_scenarios) * cnt * rs);
-        PGM_HIP_CHECK(hipMemcpy(host.data(), out[comp].get(), host.size(), hipMemcpyDeviceToHost));
+        PGM_HIP_CHECK(hipMemcpy(host.data(), out[comp], host.size(), hipMemcpyDeviceToHost));
         for (Idx s = 0; s < n_scenarios; ++s) {
             if (status[s] != 0) continue;
             Idx const b0 = buf.begin(s), b1 = buf.end(s);
@@ -954,7 +956,7 @@ int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices
             int const c = p.col[k];
             auto const b0 = p.col.begin() + p.row_ptr[c], b1 = p.col.begin() + p.row_ptr[c + 1];
             auto const it = std::lower_bound(b0, b1, int(r));
-            if (it == b1 || *it != int(r)) return -1;  // pattern must be symmetric
+            if (it == b1 || *it != int(r)) return -1;  // the pattern must be symmetric
             p.transpose[k] = int(it - p.col.begin());
         }
     build_schedule(p, 16);
@@ -964,41 +966,9 @@ int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices
         hipStream_t s;
         ~StreamGuard() { (void)hipStreamDestroy(s); }
     } guard{st};
-    int const S = int(batch), S_pad = (S + 63) / 64 * 64;
+    Arena A;
+    int const S = int(batch), S_pad = S > 32 ? (S + 63) / 64 * 64 : S;
     size_t const sp = size_t(S_pad);
-    DeviceBuffer row_ptr, col, diag, transpose, fill, schur_ptr, schur, task_ptr, task_pivot, task_mm, level_task;
-    upload(row_ptr, p.row_ptr, st);
-    upload(col, p.col, st);
-    upload(diag, p.diag, st);
-    upload(transpose, p.transpose, st);
-    upload(fill, p.fill, st);
-    upload(schur_ptr, p.schur_ptr, st);
-    std::vector<int> sch(p.schur.size() * 3);
-    for (size_t i = 0; i < p.schur.size(); ++i)
-        for (int k = 0; k < 3; ++k) sch[3 * i + k] = p.schur[i][k];
-    upload(schur, sch, st);
-    upload(task_ptr, p.task_ptr, st);
-    upload(task_pivot, p.task_pivot, st);
-    upload(task_mm, p.task_mm, st);
-    upload(level_task, p.level_task, st);
-    std::vector<double> a(size_t(p.E) * NB2 * sp, 0.0), bv(size_t(n) * NB * sp, 0.0);
-    for (int s = 0; s < S; ++s) {
-        for (Idx e = 0; e < p.E; ++e)
-            for (int k = 0; k < NB2; ++k) a[(size_t(e) * NB2 + k) * sp + s] = data[(size_t(s) * p.E + e) * NB2 + k];
-        for (Idx r = 0; r < n; ++r)
-            for (int k = 0; k < NB; ++k) bv[(size_t(r) * NB + k) * sp + s] = rhs[(size_t(s) * n + r) * NB + k];
-    }
-    DeviceBuffer da, db, y, lu, perm, done, err, iters, devmax, lastdev;
-    upload(da, a, st);
-    upload(db, bv, st);
-    y.alloc(size_t(n) * NB * sp * 8);
-    lu.alloc(size_t(p.E) * NB2 * sp * (fp32 ? 4 : 8));
-    perm.alloc(size_t(n) * sp * 4);
-    done.alloc(sp * 4);
-    err.alloc(sp * 4);
-    iters.alloc(sp * 4);
-    devmax.alloc(sp * 8);
-    lastdev.alloc(sp * 8);
     DevGroup d;
     d.P = P;
     d.n = int(n);
@@ -1007,37 +977,63 @@ int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices
     d.S = S;
     d.S_pad = S_pad;
     d.fp32 = fp32 ? 1 : 0;
-    d.row_ptr = row_ptr.as<int>();
-    d.col = col.as<int>();
-    d.diag = diag.as<int>();
-    d.transpose = transpose.as<int>();
-    d.fill = fill.as<int8_t>();
-    d.schur_ptr = schur_ptr.as<int>();
-    d.schur = schur.as<int>();
-    d.task_ptr = task_ptr.as<int>();
-    d.task_pivot = task_pivot.as<int>();
-    d.task_mm = task_mm.as<int>();
-    d.y = y.as<double>();
-    d.lu = lu.get();
-    d.perm = perm.as<uint32_t>();
-    d.done = done.as<int>();
-    d.err = err.as<int>();
-    d.iters = iters.as<int>();
-    d.devmax = devmax.as<unsigned long long>();
-    d.lastdev = lastdev.as<double>();
-    d.raw_a = da.as<double>();
-    d.raw_b = db.as<double>();
-    launch_init_state(d, st);
-    int const L = int(p.level_ptr.size()) - 1;
-    int const* lt = level_task.as<int>();
-    for (int l = 0; l < L; ++l) launch_forward(d, MODE_RAW, lt + p.level_ptr[l], p.level_ptr[l + 1] - p.level_ptr[l], st);
-    for (int l = L - 1; l >= 0; --l)
-        launch_backward(d, MODE_RAW, lt + p.level_ptr[l], p.level_ptr[l + 1] - p.level_ptr[l], st);
+    d.row_ptr = put(A, p.row_ptr);
+    d.col = put(A, p.col);
+    d.diag = put(A, p.diag);
+    d.transpose = put(A, p.transpose);
+    d.fill = put(A, p.fill);
+    d.schur_ptr = put(A, p.schur_ptr);
+    std::vector<int> sch(p.schur.size() * 3);
+    for (size_t i = 0; i < p.schur.size(); ++i)
+        for (int k = 0; k < 3; ++k) sch[3 * i + k] = p.schur[i][k];
+    d.schur = put(A, sch);
+    d.task_ptr = put(A, p.task_ptr);
+    d.task_pivot = put(A, p.task_pivot);
+    d.task_mm = put(A, p.task_mm);
+    std::vector<double> a(size_t(p.E) * NB2 * sp, 0.0), bv(size_t(n) * NB * sp, 0.0);
+    for (int s = 0; s < S; ++s) {
+        for (Idx e = 0; e < p.E; ++e)
+            for (int k = 0; k < NB2; ++k) a[(size_t(e) * NB2 + k) * sp + s] = data[(size_t(s) * p.E + e) * NB2 + k];
+        for (Idx r = 0; r < n; ++r)
+            for (int k = 0; k < NB; ++k) bv[(size_t(r) * NB + k) * sp + s] = rhs[(size_t(s) * n + r) * NB + k];
+    }
+    d.raw_a = put(A, a);
+    d.raw_b = put(A, bv);
+    d.y = dev_alloc<double>(A, size_t(n) * NB * sp);
+    d.lu = A.take(size_t(p.E) * NB2 * sp * (fp32 ? 4 : 8));
+    d.perm = dev_alloc<uint32_t>(A, size_t(n) * sp);
+    PairState ps;
+    ps.n = (long long)sp;
+    ps.done = dev_alloc<int>(A, sp);
+    ps.err = dev_alloc<int>(A, sp);
+    ps.iters = dev_alloc<int>(A, sp);
+    ps.devmax = dev_alloc<unsigned long long>(A, sp);
+    ps.lastdev = dev_alloc<double>(A, sp);
+    d.done = ps.done;
+    d.err = ps.err;
+    d.iters = ps.iters;
+    d.devmax = ps.devmax;
+    d.lastdev = ps.lastdev;
+    DevGroup* dg = put(A, std::vector<DevGroup>{d});
+    int const nblk = (S + 63) / 64;
+    std::vector<WorkItem*> items;
+    std::vector<int> counts;
+    for (size_t l = 0; l + 1 < p.level_ptr.size(); ++l) {
+        std::vector<WorkItem> it;
+        for (int q = p.level_ptr[l]; q < p.level_ptr[l + 1]; ++q)
+            for (int blk = 0; blk < nblk; ++blk) it.push_back(WorkItem{0, p.level_task[q], blk});
+        items.push_back(put(A, it));
+        counts.push_back(int(it.size()));
+    }
+    launch_init_state(ps, st);
+    int const L = int(items.size());
+    for (int l = 0; l < L; ++l) launch_forward(dg, items[l], counts[l], P, fp32, MODE_RAW, st);
+    for (int l = L - 1; l >= 0; --l) launch_backward(dg, items[l], counts[l], P, fp32, MODE_RAW, st);
     PGM_HIP_CHECK(hipGetLastError());
     std::vector<double> yh(bv.size());
     std::vector<int> eh(sp);
-    PGM_HIP_CHECK(hipMemcpyAsync(yh.data(), y.get(), yh.size() * 8, hipMemcpyDeviceToHost, st));
-    PGM_HIP_CHECK(hipMemcpyAsync(eh.data(), err.get(), eh.size() * 4, hipMemcpyDeviceToHost, st));
+    PGM_HIP_CHECK(hipMemcpyAsync(yh.data(), d.y, yh.size() * 8, hipMemcpyDeviceToHost, st));
+    PGM_HIP_CHECK(hipMemcpyAsync(eh.data(), ps.err, eh.size() * 4, hipMemcpyDeviceToHost, st));
     PGM_HIP_CHECK(hipStreamSynchronize(st));
     for (int s = 0; s < S; ++s) {
         status[s] = eh[s] == ERR_SPARSE ? 1 : 0;

@@ -16,30 +16,26 @@ struct PGM_PowerGridModel {
 
 namespace pgm_amd {
 
-class DeviceBuffer {
+// bump allocator over large device chunks: one batch makes thousands of small arrays (one set per
+// topology group) and frees them together
+class Arena {
   public:
-    DeviceBuffer() = default;
-    explicit DeviceBuffer(size_t bytes) { alloc(bytes); }
-    ~DeviceBuffer() { release(); }
-    DeviceBuffer(DeviceBuffer const&) = delete;
-    DeviceBuffer& operator=(DeviceBuffer const&) = delete;
-    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) {
-        o.ptr_ = nullptr;
-        o.bytes_ = 0;
-    }
-    void alloc(size_t bytes);
-    void release();
-    template <class T> T* as() const { return static_cast<T*>(ptr_); }
-    void* get() const { return ptr_; }
-    size_t bytes() const { return bytes_; }
+    Arena() = default;
+    Arena(Arena const&) = delete;
+    Arena& operator=(Arena const&) = delete;
+    ~Arena();
+    void* take(size_t bytes);
+    size_t reserved() const { return reserved_; }
 
   private:
-    void* ptr_ = nullptr;
-    size_t bytes_ = 0;
+    std::vector<void*> chunks_;
+    char* cur_ = nullptr;
+    size_t left_ = 0;
+    size_t reserved_ = 0;
 };
 
-// apply one update record of component `comp` to a state; returns true if it changed the topology
-// (branch or source status) or the admittances (tap, shunt) -- these define the group of a scenario
+// apply one update record of component `comp` to a state; `structural` is set when it changes a branch
+// or source status, a tap or a shunt -- the state that defines the topology group of a scenario
 bool apply_update_record(Model const& m, ComponentState& st, int comp, Idx index, void const* record,
                          bool& structural);
 
@@ -55,21 +51,43 @@ struct Batch {
     bool is_batch = false;
     Idx n_scenarios = 1;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_fwd0 = nullptr, ev_fwd1 = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    Arena arena;
     std::vector<std::unique_ptr<Group>> groups;
+    // device tables over the groups
+    DevGroup* d_groups = nullptr;
+    DevOutput* d_outputs = nullptr;
+    NodeRec* d_node_recs = nullptr;
+    BranchRec* d_branch_recs[2] = {nullptr, nullptr};
+    ApplRec* d_appl_recs[N_COMP] = {};
+    // work items per level (forward in increasing, backward in decreasing level order)
+    std::vector<WorkItem*> d_items;
+    std::vector<int> n_items;
+    std::vector<double> level_bytes_full;  // algorithmic forward bytes per level, every lane iterating
+    PairState pairs;
+    long long real_pairs = 0;
+    long long total_lanes = 0;
+    int* d_active = nullptr;
+    int* h_active = nullptr;  // pinned
+    int* d_ok = nullptr;      // per lane of every group
+    long long* d_bus_off = nullptr;
+    long long bus_total = 0;
+    long long* d_rec_off[N_COMP] = {};
+    long long rec_total[N_COMP] = {};
     // output records on the device, per component slot: n_scenarios * count records
-    DeviceBuffer out[N_COMP];
+    void* out[N_COMP] = {};
     size_t rec_size[N_COMP] = {};
     // results of the last run
-    std::vector<int> status;          // 0 ok, 1 sparse, 2 diverge
+    std::vector<int> status;  // 0 ok, 1 sparse, 2 diverge, 3 update error
     std::vector<Idx> iterations;
     std::vector<std::string> messages;
     double kernel_ms = 0.0;
     double fwd_launch_ms = 0.0;
     Idx fwd_launches = 0;
     Idx fwd_timed = 0;
+    double fwd_bytes = 0.0;     // algorithmic bytes of the timed forward-sweep launches
+    double fwd_total_ms = 0.0;  // their summed durations
     std::vector<hipEvent_t> event_pool;
-    static constexpr size_t kMaxTimedLaunches = 4096;
     bool ran = false;
 
     Batch() = default;

@@ -304,12 +304,15 @@ template <int P> __device__ __forceinline__ void load_u(DevGroup const& g, int b
 // block, finish the U row and L column, forward-substitute
 // ---------------------------------------------------------------------------------------------------
 template <int P, class FT, int MODE>
-__global__ __launch_bounds__(kWave) void forward_kernel(DevGroup g, int const* __restrict__ tasks, int nblk) {
+__global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restrict__ groups,
+                                                        WorkItem const* __restrict__ items) {
     constexpr int NB = 2 * P;
     constexpr int NB2 = NB * NB;
     constexpr int PP = P * P;
-    int const task = tasks[blockIdx.x / nblk];
-    int const s = int(blockIdx.x % nblk) * kWave + int(threadIdx.x);
+    WorkItem const wi = items[blockIdx.x];
+    DevGroup const g = groups[wi.group];
+    int const task = wi.task;
+    int const s = wi.block * kWave + int(threadIdx.x);
     size_t const sp = g.S_pad;
     int const mm = g.task_mm[task];
     if (s >= g.S) return;
@@ -590,11 +593,14 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup g, int const* _
 // backward sweep: x_p = Q U_pp^-1 (y_p - sum_j U_pj x_j), then the state update of bus p
 // ---------------------------------------------------------------------------------------------------
 template <int P, class FT, int MODE>
-__global__ __launch_bounds__(kWave) void backward_kernel(DevGroup g, int const* __restrict__ tasks, int nblk) {
+__global__ __launch_bounds__(kWave) void backward_kernel(DevGroup const* __restrict__ groups,
+                                                         WorkItem const* __restrict__ items) {
     constexpr int NB = 2 * P;
     constexpr int NB2 = NB * NB;
-    int const task = tasks[blockIdx.x / nblk];
-    int const s = int(blockIdx.x % nblk) * kWave + int(threadIdx.x);
+    WorkItem const wi = items[blockIdx.x];
+    DevGroup const g = groups[wi.group];
+    int const task = wi.task;
+    int const s = wi.block * kWave + int(threadIdx.x);
     size_t const sp = g.S_pad;
     int const mm = g.task_mm[task];
     if (s >= g.S) return;
@@ -665,42 +671,61 @@ __global__ __launch_bounds__(kWave) void backward_kernel(DevGroup g, int const* 
     }
 }
 
-__global__ void init_state_kernel(DevGroup g) {
+__global__ void init_state_kernel(PairState ps) {
     size_t const i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= size_t(g.n_mm) * g.S_pad) return;
-    g.done[i] = ACTIVE;
-    g.err[i] = ERR_NONE;
-    g.iters[i] = 0;
-    g.devmax[i] = 0ull;
-    g.lastdev[i] = 0.0;
+    if (i >= size_t(ps.n)) return;
+    ps.done[i] = ACTIVE;
+    ps.err[i] = ERR_NONE;
+    ps.iters[i] = 0;
+    ps.devmax[i] = 0ull;
+    ps.lastdev[i] = 0.0;
 }
 
-__global__ void converge_kernel(DevGroup g, int it, int max_iter, double tol, int* active) {
+// pairs of padding lanes are never touched by the sweeps: their deviation stays 0 and they retire
+// at the first check
+__global__ void converge_kernel(PairState ps, int it, int max_iter, double tol, int* active) {
     size_t const i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= size_t(g.n_mm) * g.S_pad) return;
-    int const s = int(i % g.S_pad);
-    if (s >= g.S || g.done[i] != ACTIVE) return;
-    double const md = __longlong_as_double(static_cast<long long>(g.devmax[i]));
-    g.devmax[i] = 0ull;
-    g.iters[i] = it;
+    if (i >= size_t(ps.n)) return;
+    if (ps.done[i] != ACTIVE) return;
+    double const md = __longlong_as_double(static_cast<long long>(ps.devmax[i]));
+    ps.devmax[i] = 0ull;
+    ps.iters[i] = it;
     if (!(md > tol)) {
-        g.done[i] = CONVERGED;
+        ps.done[i] = CONVERGED;
     } else if (it >= max_iter) {
-        g.done[i] = FAILED;
-        g.err[i] = ERR_DIVERGE;
-        g.lastdev[i] = md;
+        ps.done[i] = FAILED;
+        ps.err[i] = ERR_DIVERGE;
+        ps.lastdev[i] = md;
     } else {
         atomicAdd(active, 1);
     }
 }
 
+// group of a flat thread index: offsets[g] <= idx < offsets[g + 1]
+__device__ __forceinline__ int find_group(long long const* __restrict__ offsets, int n_groups, long long idx) {
+    int lo = 0, hi = n_groups;
+    while (hi - lo > 1) {
+        int const mid = (lo + hi) >> 1;
+        if (offsets[mid] <= idx)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 // ---------------------------------------------------------------------------------------------------
 // results per bus: load powers and currents, bus injection and source results, shunt flows
 // ---------------------------------------------------------------------------------------------------
-template <int P> __global__ void bus_results_kernel(DevOutput o) {
-    size_t const idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+template <int P>
+__global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long const* __restrict__ offsets,
+                                   int n_groups) {
+    long long const gidx = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gidx >= offsets[n_groups]) return;
+    int const gr = find_group(offsets, n_groups, gidx);
+    DevOutput const& o = outs[gr];
+    size_t const idx = size_t(gidx - offsets[gr]);
     size_t const sp = o.S_pad;
-    if (idx >= size_t(o.n) * sp) return;
     int const b = int(idx / sp);
     int const s = int(idx % sp);
     if (s >= o.S || !o.ok[s]) return;
@@ -838,10 +863,16 @@ template <int P> __global__ void bus_results_kernel(DevOutput o) {
 // ---------------------------------------------------------------------------------------------------
 // record writers (node.hpp, branch.hpp, appliance.hpp get_output)
 // ---------------------------------------------------------------------------------------------------
-template <int P> __global__ void node_records_kernel(DevOutput o, NodeRec r, NodeOutput<P>* out) {
-    size_t const idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+template <int P>
+__global__ void node_records_kernel(DevOutput const* __restrict__ outs, NodeRec const* __restrict__ recs,
+                        long long const* __restrict__ offsets, int n_groups, NodeOutput<P>* out) {
+    long long const gidx = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gidx >= offsets[n_groups]) return;
+    int const gr = find_group(offsets, n_groups, gidx);
+    DevOutput const& o = outs[gr];
+    NodeRec const& r = recs[gr];
+    size_t const idx = size_t(gidx - offsets[gr]);
     size_t const sp = o.S_pad;
-    if (idx >= size_t(r.n_comp) * sp) return;
     int const i = int(idx / sp), s = int(idx % sp);
     if (s >= o.S || !o.ok[s]) return;
     NodeOutput<P> rec{};
@@ -865,10 +896,16 @@ template <int P> __global__ void node_records_kernel(DevOutput o, NodeRec r, Nod
     out[size_t(o.scenario_index[s]) * r.n_comp + i] = rec;
 }
 
-template <int P> __global__ void branch_records_kernel(DevOutput o, BranchRec r, BranchOutput<P>* out) {
-    size_t const idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+template <int P>
+__global__ void branch_records_kernel(DevOutput const* __restrict__ outs, BranchRec const* __restrict__ recs,
+                        long long const* __restrict__ offsets, int n_groups, BranchOutput<P>* out) {
+    long long const gidx = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gidx >= offsets[n_groups]) return;
+    int const gr = find_group(offsets, n_groups, gidx);
+    DevOutput const& o = outs[gr];
+    BranchRec const& r = recs[gr];
+    size_t const idx = size_t(gidx - offsets[gr]);
     size_t const sp = o.S_pad;
-    if (idx >= size_t(r.n_comp) * sp) return;
     int const i = int(idx / sp), s = int(idx % sp);
     if (s >= o.S || !o.ok[s]) return;
     BranchOutput<P> rec{};
@@ -917,10 +954,16 @@ template <int P> __global__ void branch_records_kernel(DevOutput o, BranchRec r,
     out[size_t(o.scenario_index[s]) * r.n_comp + i] = rec;
 }
 
-template <int P> __global__ void appliance_records_kernel(DevOutput o, ApplRec r, ApplianceOutput<P>* out) {
-    size_t const idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+template <int P>
+__global__ void appliance_records_kernel(DevOutput const* __restrict__ outs, ApplRec const* __restrict__ recs,
+                        long long const* __restrict__ offsets, int n_groups, ApplianceOutput<P>* out) {
+    long long const gidx = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gidx >= offsets[n_groups]) return;
+    int const gr = find_group(offsets, n_groups, gidx);
+    DevOutput const& o = outs[gr];
+    ApplRec const& r = recs[gr];
+    size_t const idx = size_t(gidx - offsets[gr]);
     size_t const sp = o.S_pad;
-    if (idx >= size_t(r.n_comp) * sp) return;
     int const i = int(idx / sp), s = int(idx % sp);
     if (s >= o.S || !o.ok[s]) return;
     ApplianceOutput<P> rec{};
@@ -944,90 +987,94 @@ template <int P> __global__ void appliance_records_kernel(DevOutput o, ApplRec r
     out[size_t(o.scenario_index[s]) * r.n_comp + i] = rec;
 }
 
-template <int P, class FT> void launch_fwd(DevGroup const& g, int mode, int const* tasks, int n_tasks, hipStream_t st) {
-    int const nblk = (g.S + kWave - 1) / kWave;
-    dim3 const grid(unsigned(n_tasks) * unsigned(nblk));
+template <int P, class FT>
+void launch_fwd(DevGroup const* groups, WorkItem const* items, int n_items, int mode, hipStream_t st) {
+    dim3 const grid{unsigned(n_items)};
     if (mode == MODE_INIT)
-        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_INIT>), grid, dim3(kWave), 0, st, g, tasks, nblk);
+        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_INIT>), grid, dim3(kWave), 0, st, groups, items);
     else if (mode == MODE_NR)
-        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_NR>), grid, dim3(kWave), 0, st, g, tasks, nblk);
+        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_NR>), grid, dim3(kWave), 0, st, groups, items);
     else
-        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_RAW>), grid, dim3(kWave), 0, st, g, tasks, nblk);
+        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_RAW>), grid, dim3(kWave), 0, st, groups, items);
 }
-template <int P, class FT> void launch_bwd(DevGroup const& g, int mode, int const* tasks, int n_tasks, hipStream_t st) {
-    int const nblk = (g.S + kWave - 1) / kWave;
-    dim3 const grid(unsigned(n_tasks) * unsigned(nblk));
+template <int P, class FT>
+void launch_bwd(DevGroup const* groups, WorkItem const* items, int n_items, int mode, hipStream_t st) {
+    dim3 const grid{unsigned(n_items)};
     if (mode == MODE_INIT)
-        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_INIT>), grid, dim3(kWave), 0, st, g, tasks, nblk);
+        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_INIT>), grid, dim3(kWave), 0, st, groups, items);
     else if (mode == MODE_NR)
-        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_NR>), grid, dim3(kWave), 0, st, g, tasks, nblk);
+        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_NR>), grid, dim3(kWave), 0, st, groups, items);
     else
-        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_RAW>), grid, dim3(kWave), 0, st, g, tasks, nblk);
+        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_RAW>), grid, dim3(kWave), 0, st, groups, items);
 }
 
 }  // namespace
 
-void launch_forward(DevGroup const& g, int mode, int const* tasks, int n_tasks, hipStream_t st) {
-    if (n_tasks <= 0 || g.S <= 0) return;
-    if (g.P == 1)
-        g.fp32 ? launch_fwd<1, float>(g, mode, tasks, n_tasks, st) : launch_fwd<1, double>(g, mode, tasks, n_tasks, st);
+void launch_forward(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
+                    hipStream_t st) {
+    if (n_items <= 0) return;
+    if (P == 1)
+        fp32 ? launch_fwd<1, float>(groups, items, n_items, mode, st) : launch_fwd<1, double>(groups, items, n_items, mode, st);
     else
-        g.fp32 ? launch_fwd<3, float>(g, mode, tasks, n_tasks, st) : launch_fwd<3, double>(g, mode, tasks, n_tasks, st);
+        fp32 ? launch_fwd<3, float>(groups, items, n_items, mode, st) : launch_fwd<3, double>(groups, items, n_items, mode, st);
 }
-void launch_backward(DevGroup const& g, int mode, int const* tasks, int n_tasks, hipStream_t st) {
-    if (n_tasks <= 0 || g.S <= 0) return;
-    if (g.P == 1)
-        g.fp32 ? launch_bwd<1, float>(g, mode, tasks, n_tasks, st) : launch_bwd<1, double>(g, mode, tasks, n_tasks, st);
+void launch_backward(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
+                     hipStream_t st) {
+    if (n_items <= 0) return;
+    if (P == 1)
+        fp32 ? launch_bwd<1, float>(groups, items, n_items, mode, st) : launch_bwd<1, double>(groups, items, n_items, mode, st);
     else
-        g.fp32 ? launch_bwd<3, float>(g, mode, tasks, n_tasks, st) : launch_bwd<3, double>(g, mode, tasks, n_tasks, st);
+        fp32 ? launch_bwd<3, float>(groups, items, n_items, mode, st) : launch_bwd<3, double>(groups, items, n_items, mode, st);
 }
-void launch_converge(DevGroup const& g, int it, int max_iter, double err_tol, int* active, hipStream_t st) {
-    size_t const n = size_t(g.n_mm) * g.S_pad;
-    if (n == 0) return;
-    hipLaunchKernelGGL(converge_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, g, it, max_iter, err_tol,
-                       active);
+void launch_converge(PairState const& ps, int it, int max_iter, double err_tol, int* active, hipStream_t st) {
+    if (ps.n == 0) return;
+    hipLaunchKernelGGL(converge_kernel, dim3(unsigned((ps.n + 255) / 256)), dim3(256), 0, st, ps, it, max_iter,
+                       err_tol, active);
 }
-void launch_init_state(DevGroup const& g, hipStream_t st) {
-    size_t const n = size_t(g.n_mm) * g.S_pad;
-    if (n == 0) return;
-    hipLaunchKernelGGL(init_state_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, g);
+void launch_init_state(PairState const& ps, hipStream_t st) {
+    if (ps.n == 0) return;
+    hipLaunchKernelGGL(init_state_kernel, dim3(unsigned((ps.n + 255) / 256)), dim3(256), 0, st, ps);
 }
-void launch_bus_results(DevOutput const& o, hipStream_t st) {
-    size_t const n = size_t(o.n) * o.S_pad;
-    if (n == 0) return;
-    dim3 const grid(unsigned((n + 255) / 256));
-    if (o.P == 1)
-        hipLaunchKernelGGL(bus_results_kernel<1>, grid, dim3(256), 0, st, o);
+void launch_bus_results(int P, DevOutput const* outs, long long const* offsets, int n_groups, long long total,
+                        hipStream_t st) {
+    if (total == 0) return;
+    dim3 const grid(unsigned((total + 255) / 256));
+    if (P == 1)
+        hipLaunchKernelGGL(bus_results_kernel<1>, grid, dim3(256), 0, st, outs, offsets, n_groups);
     else
-        hipLaunchKernelGGL(bus_results_kernel<3>, grid, dim3(256), 0, st, o);
+        hipLaunchKernelGGL(bus_results_kernel<3>, grid, dim3(256), 0, st, outs, offsets, n_groups);
 }
-void launch_node_records(DevOutput const& o, NodeRec const& r, void* out, hipStream_t st) {
-    size_t const n = size_t(r.n_comp) * o.S_pad;
-    if (n == 0) return;
-    dim3 const grid(unsigned((n + 255) / 256));
-    if (o.P == 1)
-        hipLaunchKernelGGL(node_records_kernel<1>, grid, dim3(256), 0, st, o, r, static_cast<NodeOutput<1>*>(out));
+void launch_node_records(int P, DevOutput const* outs, NodeRec const* recs, long long const* offsets, int n_groups,
+                         long long total, void* out, hipStream_t st) {
+    if (total == 0) return;
+    dim3 const grid(unsigned((total + 255) / 256));
+    if (P == 1)
+        hipLaunchKernelGGL(node_records_kernel<1>, grid, dim3(256), 0, st, outs, recs, offsets, n_groups,
+                           static_cast<NodeOutput<1>*>(out));
     else
-        hipLaunchKernelGGL(node_records_kernel<3>, grid, dim3(256), 0, st, o, r, static_cast<NodeOutput<3>*>(out));
+        hipLaunchKernelGGL(node_records_kernel<3>, grid, dim3(256), 0, st, outs, recs, offsets, n_groups,
+                           static_cast<NodeOutput<3>*>(out));
 }
-void launch_branch_records(DevOutput const& o, BranchRec const& r, void* out, hipStream_t st) {
-    size_t const n = size_t(r.n_comp) * o.S_pad;
-    if (n == 0) return;
-    dim3 const grid(unsigned((n + 255) / 256));
-    if (o.P == 1)
-        hipLaunchKernelGGL(branch_records_kernel<1>, grid, dim3(256), 0, st, o, r, static_cast<BranchOutput<1>*>(out));
+void launch_branch_records(int P, DevOutput const* outs, BranchRec const* recs, long long const* offsets,
+                           int n_groups, long long total, void* out, hipStream_t st) {
+    if (total == 0) return;
+    dim3 const grid(unsigned((total + 255) / 256));
+    if (P == 1)
+        hipLaunchKernelGGL(branch_records_kernel<1>, grid, dim3(256), 0, st, outs, recs, offsets, n_groups,
+                           static_cast<BranchOutput<1>*>(out));
     else
-        hipLaunchKernelGGL(branch_records_kernel<3>, grid, dim3(256), 0, st, o, r, static_cast<BranchOutput<3>*>(out));
+        hipLaunchKernelGGL(branch_records_kernel<3>, grid, dim3(256), 0, st, outs, recs, offsets, n_groups,
+                           static_cast<BranchOutput<3>*>(out));
 }
-void launch_appliance_records(DevOutput const& o, ApplRec const& r, void* out, hipStream_t st) {
-    size_t const n = size_t(r.n_comp) * o.S_pad;
-    if (n == 0) return;
-    dim3 const grid(unsigned((n + 255) / 256));
-    if (o.P == 1)
-        hipLaunchKernelGGL(appliance_records_kernel<1>, grid, dim3(256), 0, st, o, r,
+void launch_appliance_records(int P, DevOutput const* outs, ApplRec const* recs, long long const* offsets,
+                              int n_groups, long long total, void* out, hipStream_t st) {
+    if (total == 0) return;
+    dim3 const grid(unsigned((total + 255) / 256));
+    if (P == 1)
+        hipLaunchKernelGGL(appliance_records_kernel<1>, grid, dim3(256), 0, st, outs, recs, offsets, n_groups,
                            static_cast<ApplianceOutput<1>*>(out));
     else
-        hipLaunchKernelGGL(appliance_records_kernel<3>, grid, dim3(256), 0, st, o, r,
+        hipLaunchKernelGGL(appliance_records_kernel<3>, grid, dim3(256), 0, st, outs, recs, offsets, n_groups,
                            static_cast<ApplianceOutput<3>*>(out));
 }
 

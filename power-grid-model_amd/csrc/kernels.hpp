@@ -58,13 +58,28 @@ struct DevGroup {
     double const* raw_b = nullptr;  // [n][2P][S_pad]
 };
 
-// one sweep over the tasks of one level; tasks = level task ids (device), n_tasks of them
-void launch_forward(DevGroup const& g, int mode, int const* tasks, int n_tasks, hipStream_t st);
-void launch_backward(DevGroup const& g, int mode, int const* tasks, int n_tasks, hipStream_t st);
-// per (math model, scenario): fold the deviation of iteration `it`, mark converged / diverged,
-// count the still active pairs into *active
-void launch_converge(DevGroup const& g, int it, int max_iter, double err_tol, int* active, hipStream_t st);
-void launch_init_state(DevGroup const& g, hipStream_t st);
+// one wave of work: (group, task, block of 64 scenarios)
+struct WorkItem {
+    int group, task, block;
+};
+// per (group, math model, scenario) iteration state, contiguous over the whole batch
+struct PairState {
+    long long n = 0;
+    int* done = nullptr;
+    int* err = nullptr;
+    int* iters = nullptr;
+    unsigned long long* devmax = nullptr;  // bits of a non-negative double
+    double* lastdev = nullptr;
+};
+
+// one sweep over the work items of one level (groups: device table of DevGroup)
+void launch_forward(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
+                    hipStream_t st);
+void launch_backward(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
+                     hipStream_t st);
+// fold the deviation of iteration `it`, mark converged / diverged, count the still active pairs
+void launch_converge(PairState const& ps, int it, int max_iter, double err_tol, int* active, hipStream_t st);
+void launch_init_state(PairState const& ps, hipStream_t st);
 
 // outputs of a group, written as the caller's records at out[scenario_index[s] * n_comp + i]
 struct DevOutput {
@@ -101,7 +116,6 @@ struct DevOutput {
     double const* shunt_y = nullptr;  // P*P*2 per shunt
     double* shunt_si = nullptr;       // [n_shunt][2][P][2][S_pad]
 };
-void launch_bus_results(DevOutput const& o, hipStream_t st);
 
 // record writers: one thread per (lane, component); map gives the math index of every component (-1
 // when it is not energised in this group)
@@ -133,8 +147,14 @@ struct ApplRec {
     int8_t const* status_const;
     double const* si;     // [n_math][2][P][2][S_pad]
 };
-void launch_node_records(DevOutput const& o, NodeRec const& r, void* out, hipStream_t st);
-void launch_branch_records(DevOutput const& o, BranchRec const& r, void* out, hipStream_t st);
-void launch_appliance_records(DevOutput const& o, ApplRec const& r, void* out, hipStream_t st);
+// flat launches over all groups: offsets[g] = first flat index of group g (n_groups + 1 entries)
+void launch_bus_results(int P, DevOutput const* outs, long long const* offsets, int n_groups, long long total,
+                        hipStream_t st);
+void launch_node_records(int P, DevOutput const* outs, NodeRec const* recs, long long const* offsets, int n_groups,
+                         long long total, void* out, hipStream_t st);
+void launch_branch_records(int P, DevOutput const* outs, BranchRec const* recs, long long const* offsets,
+                           int n_groups, long long total, void* out, hipStream_t st);
+void launch_appliance_records(int P, DevOutput const* outs, ApplRec const* recs, long long const* offsets,
+                              int n_groups, long long total, void* out, hipStream_t st);
 
 }  // namespace pgm_amd

@@ -333,10 +333,10 @@ class DeviceBatch:
         return out
 
     def stats(self) -> dict:
-        out = np.zeros(8, dtype=np.float64)
+        out = np.zeros(11, dtype=np.float64)
         lib.PGM_amd_batch_stats(self._ptr, out.ctypes.data)
         keys = ["kernel_ms", "iterations_sum", "buses", "lu_entries", "groups", "fwd_launch_ms", "fwd_launches",
-                "buses_per_scenario"]
+                "buses_per_scenario", "fwd_bytes", "fwd_total_ms", "fwd_timed"]
         return dict(zip(keys, out.tolist()))
 
 
