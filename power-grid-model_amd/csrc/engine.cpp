@@ -198,20 +198,53 @@ struct Group {
     ApplRec appl_rec[N_COMP]{};
     int8_t* load_status = nullptr;
     std::vector<double> level_entries, level_pivots;  // LU entries of the rows / pivots per level
+    // union group: scenarios of several small groups that differ only in branch side statuses. The
+    // structure is the one of the union state (a side is closed when it is closed in any member); each
+    // member keeps its own statuses for its lanes [lane_begin, lane_end)
+    bool is_union = false;
+    bool packed = false;  // at most 32 scenarios: packed kernels, several tasks per wave
+    struct Member {
+        int lane_begin = 0, lane_end = 0;
+        std::vector<IntS> line_fs, line_ts, tr_fs, tr_ts;
+    };
+    std::vector<Member> members;
 };
 
 namespace {
 
-std::string structural_key(Model const& m, ComponentState const& st) {
+// groups with fewer scenarios than this whose branches are only opened against the base state are
+// solved as one union group on the base structure (contingency-style batches: one line out per scenario)
+constexpr int kUnionBelow = 32;
+// groups of at most this many scenarios run on the packed kernels (64 / S tasks of a level per wave)
+constexpr int kPackedMax = 32;
+// target number of parallel tasks of a packed group's schedule
+constexpr int kPackedTarget = 64;
+
+// every branch side of st is open where the base state has it open
+bool opens_only(Model const& m, ComponentState const& st) {
+    auto le = [](std::vector<IntS> const& a, std::vector<IntS> const& b) {
+        for (size_t i = 0; i < a.size(); ++i)
+            if (a[i] != 0 && b[i] == 0) return false;
+        return true;
+    };
+    return le(st.line_fs, m.base.line_fs) && le(st.line_ts, m.base.line_ts) && le(st.tr_fs, m.base.tr_fs) &&
+           le(st.tr_ts, m.base.tr_ts);
+}
+
+std::string structural_key(Model const& m, ComponentState const& st, bool branch_status = true) {
     // the components whose state differs from the base define the group
     std::ostringstream k;
     auto const& b = m.base;
-    for (Idx i = 0; i < m.n_line(); ++i)
-        if (st.line_fs[i] != b.line_fs[i] || st.line_ts[i] != b.line_ts[i])
-            k << 'l' << i << ':' << int(st.line_fs[i]) << int(st.line_ts[i]) << ';';
-    for (Idx i = 0; i < Idx(m.trafo.size()); ++i)
-        if (st.tr_fs[i] != b.tr_fs[i] || st.tr_ts[i] != b.tr_ts[i] || st.tr_tap[i] != b.tr_tap[i])
-            k << 't' << i << ':' << int(st.tr_fs[i]) << int(st.tr_ts[i]) << ',' << int(st.tr_tap[i]) << ';';
+    if (branch_status)
+        for (Idx i = 0; i < m.n_line(); ++i)
+            if (st.line_fs[i] != b.line_fs[i] || st.line_ts[i] != b.line_ts[i])
+                k << 'l' << i << ':' << int(st.line_fs[i]) << int(st.line_ts[i]) << ';';
+    for (Idx i = 0; i < Idx(m.trafo.size()); ++i) {
+        bool const sd = branch_status && (st.tr_fs[i] != b.tr_fs[i] || st.tr_ts[i] != b.tr_ts[i]);
+        if (sd || st.tr_tap[i] != b.tr_tap[i])
+            k << 't' << i << ':' << (branch_status ? int(st.tr_fs[i]) : 0) << (branch_status ? int(st.tr_ts[i]) : 0)
+              << ',' << int(st.tr_tap[i]) << ';';
+    }
     for (Idx i = 0; i < Idx(m.shunt.size()); ++i) {
         double const a[4] = {st.sh_g1[i], st.sh_b1[i], st.sh_g0[i], st.sh_b0[i]};
         double const c[4] = {b.sh_g1[i], b.sh_b1[i], b.sh_g0[i], b.sh_b0[i]};
@@ -257,6 +290,7 @@ void upload_group(Batch& b, Group& g) {
     d.task_ptr = put(A, p.task_ptr);
     d.task_pivot = put(A, p.task_pivot);
     d.task_mm = put(A, p.task_mm);
+    d.level_task = put(A, p.level_task);
     d.load_ptr = put(A, p.load_ptr);
     d.load_type = put(A, p.load_type);
     d.src_ptr = put(A, p.src_ptr);
@@ -355,8 +389,9 @@ void upload_group(Batch& b, Group& g) {
                     rating.push_back(m.tstat[k].sn);
                 }
             }
-            g.branch_rec[kind] = BranchRec{int(cnt),       put(A, ids), put(A, math),   br_from, br_to, br_y,
-                                           put(A, bif), put(A, bit), put(A, rating), kind == 0 ? 1 : 0};
+            g.branch_rec[kind] = BranchRec{int(cnt),    put(A, ids), put(A, math), br_from,        br_to,
+                                           br_y,        nullptr,     nullptr,      put(A, bif),    put(A, bit),
+                                           put(A, rating), kind == 0 ? 1 : 0};
         }
     }
     {
@@ -366,17 +401,19 @@ void upload_group(Batch& b, Group& g) {
         for (size_t k = 0; k < p.shunt_comp.size(); ++k) sh_math[p.shunt_comp[k]] = int(k);
         auto make = [&](int comp, Idx cnt, auto id_of, auto math_of, auto node_of, auto stat_of, double dir,
                         int8_t const* status_lanes, double const* si) {
-            std::vector<int> ids, math;
+            std::vector<int> ids, math, bus;
             std::vector<double> bi;
             std::vector<int8_t> stt;
             for (Idx k = 0; k < cnt; ++k) {
                 ids.push_back(id_of(k));
                 math.push_back(math_of(k));
-                bi.push_back(kBaseP3 / m.node[node_of(k)].u_rated / kSqrt3);
+                Idx const node = node_of(k), gg = t.node_group[node];
+                bus.push_back(gg < 0 ? -1 : p.mm_bus_begin[gg] + int(t.node_pos[node]));
+                bi.push_back(kBaseP3 / m.node[node].u_rated / kSqrt3);
                 stt.push_back(stat_of(k));
             }
-            g.appl_rec[comp] = ApplRec{int(cnt), put(A, ids), put(A, math), put(A, bi), dir, status_lanes,
-                                       put(A, stt), si};
+            g.appl_rec[comp] = ApplRec{int(cnt), put(A, ids),  put(A, math), put(A, bus), put(A, bi), dir,
+                                       status_lanes, put(A, stt), si};
         };
         make(
             C_SHUNT, Idx(m.shunt.size()), [&](Idx k) { return m.shunt[k].id; }, [&](Idx k) { return sh_math[k]; },
@@ -394,6 +431,167 @@ void upload_group(Batch& b, Group& g) {
                 [&](Idx k) { return lg_math[off + k]; }, [&](Idx k) { return m.lg_node[off + k]; },
                 [&](Idx) { return int8_t(0); }, kk < 2 ? 1.0 : -1.0, g.load_status + size_t(off) * sp, o.load_si);
         }
+    }
+}
+
+// per-lane data of a union group: side statuses of every math branch, buses without a source, the
+// admittances of every status combination and the Y values of the entries a member's switched
+// branches touch, summed in the same order as the shared ones (branches in math order, then shunts)
+void upload_union_lanes(Batch& b, Group& g) {
+    Model const& m = b.model;
+    Arena& A = b.arena;
+    Plan const& p = g.plan;
+    int const PP = p.P * p.P, Q = 2 * PP;
+    size_t const sp = size_t(g.S_pad);
+    Idx const n = p.n, nbr = Idx(p.branch_comp.size()), nl = m.n_line();
+    auto side = [&](std::vector<IntS> const& lf, std::vector<IntS> const& lt, std::vector<IntS> const& tf,
+                    std::vector<IntS> const& tt, Idx c) {
+        return (c < nl ? (lf[c] != 0) : (tf[c - nl] != 0)) | ((c < nl ? (lt[c] != 0) : (tt[c - nl] != 0)) << 1);
+    };
+    // admittances of the four status combinations of every math branch
+    std::vector<double> yvar(size_t(nbr) * 16 * PP * 2);
+    {
+        ComponentState work = g.st;
+        for (Idx mb = 0; mb < nbr; ++mb) {
+            Idx const c = p.branch_comp[mb];
+            for (int v = 0; v < 4; ++v) {
+                IntS const fs = v & 1, ts = (v >> 1) & 1;
+                if (c < nl) {
+                    work.line_fs[c] = fs;
+                    work.line_ts[c] = ts;
+                } else {
+                    work.tr_fs[c - nl] = fs;
+                    work.tr_ts[c - nl] = ts;
+                }
+                auto const y = branch_param(m, work, c, b.symmetric);
+                for (int w = 0; w < 4; ++w)
+                    for (int q = 0; q < PP; ++q) {
+                        size_t const o = ((size_t(mb) * 4 + v) * 4 * PP + size_t(w) * PP + q) * 2;
+                        yvar[o] = y.v[w].m[q].real();
+                        yvar[o + 1] = y.v[w].m[q].imag();
+                    }
+            }
+            if (c < nl) {
+                work.line_fs[c] = g.st.line_fs[c];
+                work.line_ts[c] = g.st.line_ts[c];
+            } else {
+                work.tr_fs[c - nl] = g.st.tr_fs[c - nl];
+                work.tr_ts[c - nl] = g.st.tr_ts[c - nl];
+            }
+        }
+    }
+    // math branches incident to each bus, in math order
+    std::vector<std::vector<int>> inc(n);
+    for (Idx mb = 0; mb < nbr; ++mb) {
+        int const f = p.branch_from[mb], t = p.branch_to[mb];
+        if (f >= 0) inc[f].push_back(int(mb));
+        if (t >= 0 && t != f) inc[t].push_back(int(mb));
+    }
+    auto find = [&](int r, int c) {
+        auto const b0 = p.col.begin() + p.row_ptr[r], b1 = p.col.begin() + p.row_ptr[r + 1];
+        auto const it = std::lower_bound(b0, b1, c);
+        if (it == b1 || *it != c) throw PowerGridError("union group: LU pattern entry missing\n");
+        return int(it - p.col.begin());
+    };
+    std::vector<uint8_t> dead(size_t(n) * sp, 0), lstat(size_t(nbr) * sp, 0);
+    std::vector<int> patch_lane, patch_entry;
+    std::vector<double> patch_val;
+    std::vector<uint8_t> bits(nbr), reach(n);
+    std::vector<int> stack;
+    for (auto const& mem : g.members) {
+        for (Idx mb = 0; mb < nbr; ++mb)
+            bits[mb] = uint8_t(side(mem.line_fs, mem.line_ts, mem.tr_fs, mem.tr_ts, p.branch_comp[mb]));
+        // buses reached from a source over branches closed on both sides
+        std::fill(reach.begin(), reach.end(), 0);
+        stack.clear();
+        for (Idx r = 0; r < n; ++r)
+            if (p.src_ptr[r + 1] > p.src_ptr[r]) {
+                reach[r] = 1;
+                stack.push_back(int(r));
+            }
+        while (!stack.empty()) {
+            int const r = stack.back();
+            stack.pop_back();
+            for (int mb : inc[r]) {
+                if (bits[mb] != 3) continue;
+                int const f = p.branch_from[mb], t = p.branch_to[mb];
+                if (f < 0 || t < 0) continue;
+                int const o = f == r ? t : f;
+                if (!reach[o]) {
+                    reach[o] = 1;
+                    stack.push_back(o);
+                }
+            }
+        }
+        // entries touched by branches whose statuses differ from the union state
+        std::vector<std::pair<int, int>> touched;
+        for (Idx mb = 0; mb < nbr; ++mb) {
+            Idx const c = p.branch_comp[mb];
+            if (bits[mb] == side(g.st.line_fs, g.st.line_ts, g.st.tr_fs, g.st.tr_ts, c)) continue;
+            int const f = p.branch_from[mb], t = p.branch_to[mb];
+            if (f >= 0) touched.emplace_back(f, f);
+            if (t >= 0) touched.emplace_back(t, t);
+            if (f >= 0 && t >= 0) {
+                touched.emplace_back(f, t);
+                touched.emplace_back(t, f);
+            }
+        }
+        std::sort(touched.begin(), touched.end());
+        touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+        std::vector<cd> val(PP);
+        for (auto const& [r, c] : touched) {
+            std::fill(val.begin(), val.end(), cd{});
+            for (int mb : inc[r]) {
+                int const f = p.branch_from[mb], t = p.branch_to[mb];
+                int w = -1;
+                if (r == c) {
+                    if (f == r) w = 0;
+                    else if (t == r) w = 3;
+                } else if (f == r && t == c) {
+                    w = 1;
+                } else if (t == r && f == c) {
+                    w = 2;
+                }
+                if (w < 0) continue;
+                double const* y = yvar.data() + ((size_t(mb) * 4 + bits[mb]) * 4 * PP + size_t(w) * PP) * 2;
+                for (int q = 0; q < PP; ++q) val[q] += cd(y[2 * q], y[2 * q + 1]);
+            }
+            if (r == c)
+                for (int q = p.shunt_ptr[r]; q < p.shunt_ptr[r + 1]; ++q)
+                    for (int k = 0; k < PP; ++k) val[k] += p.shunt_y[size_t(q) * PP + k];
+            int const e = find(r, c);
+            for (int lane = mem.lane_begin; lane < mem.lane_end; ++lane) {
+                patch_lane.push_back(lane);
+                patch_entry.push_back(e);
+                for (int k = 0; k < PP; ++k) {
+                    patch_val.push_back(val[k].real());
+                    patch_val.push_back(val[k].imag());
+                }
+            }
+        }
+        for (int lane = mem.lane_begin; lane < mem.lane_end; ++lane) {
+            for (Idx r = 0; r < n; ++r) dead[size_t(r) * sp + lane] = reach[r] ? 0 : 1;
+            for (Idx mb = 0; mb < nbr; ++mb) lstat[size_t(mb) * sp + lane] = bits[mb];
+        }
+    }
+    double* ylane = dev_alloc<double>(A, size_t(p.E) * Q * sp);
+    int* d_pl = put(A, patch_lane);
+    int* d_pe = put(A, patch_entry);
+    double* d_pv = put(A, patch_val);
+    launch_expand_ylane(g.dg.ydata, int(p.E), PP, int(sp), ylane, int(patch_lane.size()), d_pl, d_pe, d_pv,
+                        b.stream);
+    PGM_HIP_CHECK(hipGetLastError());
+    PGM_HIP_CHECK(hipStreamSynchronize(b.stream));
+    uint8_t* d_dead = put(A, dead);
+    uint8_t* d_lstat = put(A, lstat);
+    double* d_yvar = put(A, yvar);
+    g.dg.ylane = ylane;
+    g.dg.dead = d_dead;
+    g.dout.ylane = ylane;
+    g.dout.dead = d_dead;
+    for (int k = 0; k < 2; ++k) {
+        g.branch_rec[k].lane_status = d_lstat;
+        g.branch_rec[k].branch_y_var = d_yvar;
     }
 }
 
@@ -510,6 +708,9 @@ Batch::~Batch() {
     if (h_active) (void)hipHostFree(h_active);
     if (ev_start) (void)hipEventDestroy(ev_start);
     if (ev_stop) (void)hipEventDestroy(ev_stop);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (stream_u) (void)hipStreamDestroy(stream_u);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -532,6 +733,9 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     b.is_batch = update != nullptr;
     b.n_scenarios = update ? update->batch_size : 1;
     PGM_HIP_CHECK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+    PGM_HIP_CHECK(hipStreamCreateWithFlags(&b.stream_u, hipStreamNonBlocking));
+    PGM_HIP_CHECK(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
+    PGM_HIP_CHECK(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
     PGM_HIP_CHECK(hipEventCreate(&b.ev_start));
     PGM_HIP_CHECK(hipEventCreate(&b.ev_stop));
     PGM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b.h_active), sizeof(int)));
@@ -621,6 +825,46 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         b.groups[scn_group[s]]->scenarios.push_back(int(s));
     }
 
+    // ---- small groups that only open branches against the base state: one union group on the base
+    // structure per state of the other components ----
+    {
+        std::map<std::string, std::vector<int>> by_key;
+        for (size_t gi = 0; gi < b.groups.size(); ++gi)
+            if (b.groups[gi]->scenarios.size() < size_t(kUnionBelow) && opens_only(m, b.groups[gi]->st))
+                by_key[structural_key(m, b.groups[gi]->st, false)].push_back(int(gi));
+        std::vector<char> merged(b.groups.size(), 0);
+        std::vector<std::unique_ptr<Group>> unions;
+        for (auto& [key, list] : by_key) {
+            if (list.size() < 2) continue;
+            auto u = std::make_unique<Group>();
+            u->is_union = true;
+            u->st = b.groups[list[0]]->st;
+            u->st.line_fs = m.base.line_fs;
+            u->st.line_ts = m.base.line_ts;
+            u->st.tr_fs = m.base.tr_fs;
+            u->st.tr_ts = m.base.tr_ts;
+            for (int gi : list) {
+                Group& g = *b.groups[gi];
+                Group::Member mem;
+                mem.lane_begin = int(u->scenarios.size());
+                mem.lane_end = mem.lane_begin + int(g.scenarios.size());
+                mem.line_fs = g.st.line_fs;
+                mem.line_ts = g.st.line_ts;
+                mem.tr_fs = g.st.tr_fs;
+                mem.tr_ts = g.st.tr_ts;
+                u->members.push_back(std::move(mem));
+                u->scenarios.insert(u->scenarios.end(), g.scenarios.begin(), g.scenarios.end());
+                merged[gi] = 1;
+            }
+            unions.push_back(std::move(u));
+        }
+        std::vector<std::unique_ptr<Group>> kept;
+        for (size_t gi = 0; gi < b.groups.size(); ++gi)
+            if (!merged[gi]) kept.push_back(std::move(b.groups[gi]));
+        for (auto& u : unions) kept.push_back(std::move(u));
+        b.groups = std::move(kept);
+    }
+
     // ---- per group: topology and plan, built in parallel on the host ----
     {
         size_t const G = b.groups.size();
@@ -634,7 +878,8 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
                     std::vector<Idx> all(g.topo.models.size());
                     std::iota(all.begin(), all.end(), Idx{0});
                     int const nblk = int((g.scenarios.size() + 63) / 64);
-                    int const target = std::clamp(4096 / std::max(nblk, 1), 4, 256);
+                    g.packed = !g.is_union && g.scenarios.size() <= size_t(kPackedMax);
+                    int const target = g.packed ? kPackedTarget : std::clamp(4096 / std::max(nblk, 1), 4, 256);
                     g.plan = build_plan(m, g.st, g.topo, all, sym, target);
                 } catch (std::exception const& e) {
                     errs[gi] = e.what();
@@ -679,6 +924,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     }
     for (auto& gp : b.groups) {
         upload_group(b, *gp);
+        if (gp->is_union) upload_union_lanes(b, *gp);
         fill_scenario_data(b, *gp, update != nullptr, for_each_row);
     }
 
@@ -703,16 +949,46 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         for (int k = 0; k < 2; ++k) b.d_branch_recs[k] = put(b.arena, br[k]);
         for (int c = 0; c < N_COMP; ++c)
             if (c == C_SHUNT || c == C_SOURCE || c >= C_SYM_GEN) b.d_appl_recs[c] = put(b.arena, ar[c]);
-        int max_levels = 0;
-        for (auto& gp : b.groups) max_levels = std::max(max_levels, int(gp->plan.level_ptr.size()) - 1);
+        int max_levels = 0, max_levels_u = 0, max_levels_p = 0;
+        for (auto& gp : b.groups) {
+            int& ml = gp->is_union ? max_levels_u : gp->packed ? max_levels_p : max_levels;
+            ml = std::max(ml, int(gp->plan.level_ptr.size()) - 1);
+        }
         double const fsz = b.fp32 ? 4.0 : 8.0;
+        // union and packed groups run on their own stream with their own kernels
+        for (int l = 0; l < max_levels_u; ++l) {
+            std::vector<WorkItem> items;
+            for (size_t gi = 0; gi < G; ++gi) {
+                Group const& g = *b.groups[gi];
+                Plan const& p = g.plan;
+                if (!g.is_union || l + 1 >= int(p.level_ptr.size()) || g.S == 0) continue;
+                int const nblk = (g.S + 63) / 64;
+                for (int q = p.level_ptr[l]; q < p.level_ptr[l + 1]; ++q)
+                    for (int blk = 0; blk < nblk; ++blk) items.push_back(WorkItem{int(gi), p.level_task[q], blk});
+            }
+            b.d_items_u.push_back(put(b.arena, items));
+            b.n_items_u.push_back(int(items.size()));
+        }
+        for (int l = 0; l < max_levels_p; ++l) {
+            std::vector<WorkItem> items;
+            for (size_t gi = 0; gi < G; ++gi) {
+                Group const& g = *b.groups[gi];
+                Plan const& p = g.plan;
+                if (!g.packed || l + 1 >= int(p.level_ptr.size()) || g.S == 0) continue;
+                int const per_wave = kWaveLanes / g.S;
+                for (int q = p.level_ptr[l]; q < p.level_ptr[l + 1]; q += per_wave)
+                    items.push_back(WorkItem{int(gi), q, std::min(per_wave, p.level_ptr[l + 1] - q)});
+            }
+            b.d_items_p.push_back(put(b.arena, items));
+            b.n_items_p.push_back(int(items.size()));
+        }
         for (int l = 0; l < max_levels; ++l) {
             std::vector<WorkItem> items;
             double bytes = 0;
             for (size_t gi = 0; gi < G; ++gi) {
                 Group const& g = *b.groups[gi];
                 Plan const& p = g.plan;
-                if (l + 1 >= int(p.level_ptr.size()) || g.S == 0) continue;
+                if (g.is_union || g.packed || l + 1 >= int(p.level_ptr.size()) || g.S == 0) continue;
                 int const nblk = (g.S + 63) / 64;
                 for (int q = p.level_ptr[l]; q < p.level_ptr[l + 1]; ++q)
                     for (int blk = 0; blk < nblk; ++blk) items.push_back(WorkItem{int(gi), p.level_task[q], blk});
@@ -771,13 +1047,40 @@ void Batch::run() {
         }
         return event_pool[i];
     };
-    int const L = int(n_items.size());
+    int const L = int(n_items.size()), LU = int(n_items_u.size()), LP = int(n_items_p.size());
+    hipStream_t const su = stream_u;
+    // the union groups' sweeps run on a second stream, overlapping the main groups' sweeps; both
+    // join before each convergence check
+    auto fork = [&] {
+        if (LU + LP == 0) return;
+        PGM_HIP_CHECK(hipEventRecord(ev_fork, st));
+        PGM_HIP_CHECK(hipStreamWaitEvent(su, ev_fork, 0));
+    };
+    auto join = [&] {
+        if (LU + LP == 0) return;
+        PGM_HIP_CHECK(hipEventRecord(ev_join, su));
+        PGM_HIP_CHECK(hipStreamWaitEvent(st, ev_join, 0));
+    };
+    auto union_sweeps = [&](int mode) {
+        // packed and union groups interleaved level by level on the second stream
+        for (int l = 0; l < std::max(LU, LP); ++l) {
+            if (l < LP) launch_forward(d_groups, d_items_p[l], n_items_p[l], P, fp32, mode, su, KIND_PACKED);
+            if (l < LU) launch_forward(d_groups, d_items_u[l], n_items_u[l], P, fp32, mode, su, KIND_UNION);
+        }
+        for (int l = std::max(LU, LP) - 1; l >= 0; --l) {
+            if (l < LP) launch_backward(d_groups, d_items_p[l], n_items_p[l], P, fp32, mode, su, KIND_PACKED);
+            if (l < LU) launch_backward(d_groups, d_items_u[l], n_items_u[l], P, fp32, mode, su, KIND_UNION);
+        }
+    };
     PGM_HIP_CHECK(hipEventRecord(ev_start, st));
     launch_init_state(pairs, st);
+    fork();
+    union_sweeps(MODE_INIT);
     for (int l = 0; l < L; ++l) launch_forward(d_groups, d_items[l], n_items[l], P, fp32, MODE_INIT, st);
     for (int l = L - 1; l >= 0; --l) launch_backward(d_groups, d_items[l], n_items[l], P, fp32, MODE_INIT, st);
     double active_frac = 1.0;
     for (int it = 1; it <= int(max_iter); ++it) {
+        union_sweeps(MODE_NR);
         for (int l = 0; l < L; ++l) {
             fwd_events.push_back(event_from_pool(fwd_events.size()));
             PGM_HIP_CHECK(hipEventRecord(fwd_events.back(), st));
@@ -788,9 +1091,11 @@ void Batch::run() {
             ++fwd_launches;
         }
         for (int l = L - 1; l >= 0; --l) launch_backward(d_groups, d_items[l], n_items[l], P, fp32, MODE_NR, st);
+        join();
         *h_active = 0;
         PGM_HIP_CHECK(hipMemcpyAsync(d_active, h_active, sizeof(int), hipMemcpyHostToDevice, st));
         launch_converge(pairs, it, int(max_iter), err_tol, d_active, st);
+        fork();
         PGM_HIP_CHECK(hipMemcpyAsync(h_active, d_active, sizeof(int), hipMemcpyDeviceToHost, st));
         PGM_HIP_CHECK(hipStreamSynchronize(st));
         // lanes still iterating, for the byte count of the next iteration's launches

@@ -51,7 +51,9 @@ struct Batch {
     bool is_batch = false;
     Idx n_scenarios = 1;
     hipStream_t stream = nullptr;
+    hipStream_t stream_u = nullptr;  // union groups
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     Arena arena;
     std::vector<std::unique_ptr<Group>> groups;
     // device tables over the groups
@@ -64,6 +66,10 @@ struct Batch {
     std::vector<WorkItem*> d_items;
     std::vector<int> n_items;
     std::vector<double> level_bytes_full;  // algorithmic forward bytes per level, every lane iterating
+    std::vector<WorkItem*> d_items_u;      // union groups, per level
+    std::vector<int> n_items_u;
+    std::vector<WorkItem*> d_items_p;      // packed groups, per level
+    std::vector<int> n_items_p;
     PairState pairs;
     long long real_pairs = 0;
     long long total_lanes = 0;

@@ -299,29 +299,63 @@ template <int P> __device__ __forceinline__ void load_u(DevGroup const& g, int b
     }
 }
 
+// Y values of LU entry k for lane s: shared by the group, or per lane in union groups
+template <int P, bool UNI>
+__device__ __forceinline__ void load_y(DevGroup const& g, int k, int s, double (&y)[P * P * 2]) {
+    constexpr int Q = P * P * 2;
+    if constexpr (UNI) {
+        size_t const sp = g.S_pad;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) y[q] = g.ylane[(size_t(k) * Q + q) * sp + s];
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) y[q] = g.ydata[size_t(k) * Q + q];
+    }
+}
+
+// task and lane of a thread. Plain and union groups: one task, 64 consecutive scenarios per wave.
+// Packed groups (at most 32 scenarios): a wave holds 64 / S tasks of one level, S lanes each
+// (item.task = first index into the level's task list, item.block = number of tasks)
+template <int KIND>
+__device__ __forceinline__ bool lane_work(DevGroup const& g, WorkItem const& wi, int& task, int& s) {
+    if constexpr (KIND == KIND_PACKED) {
+        int const t = int(threadIdx.x) / g.S;
+        s = int(threadIdx.x) - t * g.S;
+        if (t >= wi.block) return false;
+        task = g.level_task[wi.task + t];
+        return true;
+    } else {
+        task = wi.task;
+        s = wi.block * kWave + int(threadIdx.x);
+        return s < g.S;
+    }
+}
+
 // ---------------------------------------------------------------------------------------------------
 // forward sweep: assemble row / column of each pivot, pull the Schur updates, factor the diagonal
 // block, finish the U row and L column, forward-substitute
 // ---------------------------------------------------------------------------------------------------
-template <int P, class FT, int MODE>
+template <int P, class FT, int MODE, int KIND>
 __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restrict__ groups,
                                                         WorkItem const* __restrict__ items) {
     constexpr int NB = 2 * P;
     constexpr int NB2 = NB * NB;
     constexpr int PP = P * P;
+    constexpr bool UNI = KIND == KIND_UNION;
     WorkItem const wi = items[blockIdx.x];
     DevGroup const g = groups[wi.group];
-    int const task = wi.task;
-    int const s = wi.block * kWave + int(threadIdx.x);
+    int task, s;
+    if (!lane_work<KIND>(g, wi, task, s)) return;
     size_t const sp = g.S_pad;
     int const mm = g.task_mm[task];
-    if (s >= g.S) return;
     if (g.done[size_t(mm) * sp + s] != ACTIVE) return;
     Blocks<NB, FT> const lu{static_cast<FT*>(g.lu), sp, s};
 
     for (int ti = g.task_ptr[task]; ti < g.task_ptr[task + 1]; ++ti) {
         int const p = g.task_pivot[ti];
         int const rb = g.row_ptr[p], re = g.row_ptr[p + 1], dg = g.diag[p];
+        // a bus without a source in this lane (union groups): identity row, zero row and column
+        bool const dead_p = UNI && g.dead[size_t(p) * sp + s] != 0;
         double D[NB2];
         double rhs[NB];
 #pragma unroll
@@ -348,11 +382,13 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
             // admittance matrix in the real domain plus load admittances and sources
             for (int k = dg; k < re; ++k) {
                 FT a[NB2];
-                if (g.fill[k]) {
+                if (g.fill[k] || dead_p) {
 #pragma unroll
                     for (int q = 0; q < NB2; ++q) a[q] = FT(0);
                 } else {
-                    linear_block<P, FT>(g.ydata + size_t(k) * PP * 2, a);
+                    double yk[PP * 2];
+                    load_y<P, UNI>(g, k, s, yk);
+                    linear_block<P, FT>(yk, a);
                 }
                 if (k == dg) {
 #pragma unroll
@@ -360,15 +396,21 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                 } else {
                     lu.store(k, a);
                     int const e = g.transpose[k];
-                    if (g.fill[e]) {
+                    if (g.fill[e] || dead_p) {
                         lu.store_zero(e);
                     } else {
-                        linear_block<P, FT>(g.ydata + size_t(e) * PP * 2, a);
+                        double ye[PP * 2];
+                        load_y<P, UNI>(g, e, s, ye);
+                        linear_block<P, FT>(ye, a);
                         lu.store(e, a);
                     }
                 }
             }
-            for (int l = g.load_ptr[p]; l < g.load_ptr[p + 1]; ++l) {
+            if (dead_p) {
+#pragma unroll
+                for (int q = 0; q < NB2; ++q) D[q] = (q % (NB + 1) == 0) ? 1.0 : 0.0;
+            }
+            for (int l = g.load_ptr[p]; l < (dead_p ? g.load_ptr[p] : g.load_ptr[p + 1]); ++l) {
 #pragma unroll
                 for (int ph = 0; ph < P; ++ph) {
                     double const sr = g.s_inj[((size_t(l) * P + ph) * 2) * sp + s];
@@ -381,7 +423,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                     D[ph * NB + ph + P] += yi;
                 }
             }
-            for (int q = g.src_ptr[p]; q < g.src_ptr[p + 1]; ++q) {
+            for (int q = g.src_ptr[p]; q < (dead_p ? g.src_ptr[p] : g.src_ptr[p + 1]); ++q) {
                 cplx const y1 = {g.y_src[(size_t(q) * 4) * sp + s], g.y_src[(size_t(q) * 4 + 1) * sp + s]};
                 cplx const y0 = {g.y_src[(size_t(q) * 4 + 2) * sp + s], g.y_src[(size_t(q) * 4 + 3) * sp + s]};
                 cplx const ur = {g.u_src[(size_t(q) * 2) * sp + s], g.u_src[(size_t(q) * 2 + 1) * sp + s]};
@@ -405,6 +447,13 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                     rhs[r + P] += acc.im;
                 }
             }
+        } else if (dead_p) {
+            for (int k = dg + 1; k < re; ++k) {
+                lu.store_zero(k);
+                lu.store_zero(g.transpose[k]);
+            }
+#pragma unroll
+            for (int q = 0; q < NB2; ++q) D[q] = (q % (NB + 1) == 0) ? 1.0 : 0.0;
         } else {
             // Newton-Raphson Jacobian row p, column p and power mismatch
             cplx up[P];
@@ -429,11 +478,13 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                     load_u<P>(g, j, s, uj);
                 }
                 double pr[P], qr[P];
+                double yk[PP * 2];
+                load_y<P, UNI>(g, k, s, yk);
                 if (j == p) {
-                    hnml<P, double>(g.ydata + size_t(k) * PP * 2, up, uj, D, pr, qr);
+                    hnml<P, double>(yk, up, uj, D, pr, qr);
                 } else {
                     FT a[NB2];
-                    hnml<P, FT>(g.ydata + size_t(k) * PP * 2, up, uj, a, pr, qr);
+                    hnml<P, FT>(yk, up, uj, a, pr, qr);
                     if (j > p) lu.store(k, a);
                 }
 #pragma unroll
@@ -448,7 +499,9 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                     } else {
                         FT a[NB2];
                         double pr2[P], qr2[P];
-                        hnml<P, FT>(g.ydata + size_t(e) * PP * 2, uj, up, a, pr2, qr2);
+                        double ye[PP * 2];
+                        load_y<P, UNI>(g, e, s, ye);
+                        hnml<P, FT>(ye, uj, up, a, pr2, qr2);
                         lu.store(e, a);
                     }
                 }
@@ -592,18 +645,17 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
 // ---------------------------------------------------------------------------------------------------
 // backward sweep: x_p = Q U_pp^-1 (y_p - sum_j U_pj x_j), then the state update of bus p
 // ---------------------------------------------------------------------------------------------------
-template <int P, class FT, int MODE>
+template <int P, class FT, int MODE, int KIND>
 __global__ __launch_bounds__(kWave) void backward_kernel(DevGroup const* __restrict__ groups,
                                                          WorkItem const* __restrict__ items) {
     constexpr int NB = 2 * P;
     constexpr int NB2 = NB * NB;
     WorkItem const wi = items[blockIdx.x];
     DevGroup const g = groups[wi.group];
-    int const task = wi.task;
-    int const s = wi.block * kWave + int(threadIdx.x);
+    int task, s;
+    if (!lane_work<KIND>(g, wi, task, s)) return;
     size_t const sp = g.S_pad;
     int const mm = g.task_mm[task];
-    if (s >= g.S) return;
     if (g.done[size_t(mm) * sp + s] != ACTIVE) return;
     Blocks<NB, FT> const lu{static_cast<FT*>(g.lu), sp, s};
     double dev = 0.0;
@@ -729,6 +781,15 @@ __global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long
     int const b = int(idx / sp);
     int const s = int(idx % sp);
     if (s >= o.S || !o.ok[s]) return;
+    if (o.dead && o.dead[size_t(b) * sp + s]) {
+        // not energised in this lane: the appliances of the bus carry nothing
+        for (int l = o.load_ptr[b]; l < o.load_ptr[b + 1]; ++l)
+            for (int q = 0; q < 4 * P; ++q) o.load_si[(size_t(l) * 4 * P + q) * sp + s] = 0.0;
+        for (int q = o.shunt_ptr[b]; q < o.shunt_ptr[b + 1]; ++q)
+            for (int c = 0; c < 4 * P; ++c) o.shunt_si[(size_t(q) * 4 * P + c) * sp + s] = 0.0;
+        for (int c = 0; c < 2 * P; ++c) o.bus_inj[(size_t(b) * 2 * P + c) * sp + s] = 0.0;
+        return;
+    }
     auto U = [&](int bus, int ph) {
         return cplx{o.u[((size_t(bus) * P + ph) * 2) * sp + s], o.u[((size_t(bus) * P + ph) * 2 + 1) * sp + s]};
     };
@@ -765,7 +826,10 @@ __global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long
         for (int k = o.row_ptr[b]; k < o.row_ptr[b + 1]; ++k) {
             if (o.fill[k]) continue;
             int const j = o.col[k];
-            double const* y = o.ydata + size_t(k) * P * P * 2;
+            double y[P * P * 2];
+#pragma unroll
+            for (int q = 0; q < P * P * 2; ++q)
+                y[q] = o.ylane ? o.ylane[(size_t(k) * P * P * 2 + q) * sp + s] : o.ydata[size_t(k) * P * P * 2 + q];
 #pragma unroll
             for (int r = 0; r < P; ++r)
 #pragma unroll
@@ -880,7 +944,7 @@ __global__ void node_records_kernel(DevOutput const* __restrict__ outs, NodeRec 
     int const bus = r.bus[i];
     double const base_power = P == 1 ? 1e6 : 1e6 / 3.0;
     double const u_scale = P == 1 ? 1.0 : 0.57735026918962576451;
-    if (bus >= 0) {
+    if (bus >= 0 && !(o.dead && o.dead[size_t(bus) * sp + s])) {
         rec.energized = 1;
 #pragma unroll
         for (int ph = 0; ph < P; ++ph) {
@@ -912,9 +976,20 @@ __global__ void branch_records_kernel(DevOutput const* __restrict__ outs, Branch
     rec.id = r.id[i];
     int const m = r.math[i];
     double const base_power = P == 1 ? 1e6 : 1e6 / 3.0;
-    if (m >= 0) {
+    int f = m >= 0 ? r.from_bus[m] : -1, t = m >= 0 ? r.to_bus[m] : -1;
+    double const* y = m >= 0 ? r.branch_y + size_t(m) * 4 * P * P * 2 : nullptr;
+    bool energized = m >= 0;
+    if (m >= 0 && r.lane_status) {
+        // union group: the side statuses of this lane pick the admittance; energised when a closed
+        // side meets a live bus
+        int const stt = r.lane_status[size_t(m) * sp + s];
+        if (!(stt & 1)) f = -1;
+        if (!(stt & 2)) t = -1;
+        energized = (f >= 0 && !o.dead[size_t(f) * sp + s]) || (t >= 0 && !o.dead[size_t(t) * sp + s]);
+        y = r.branch_y_var + (size_t(m) * 4 + stt) * 4 * P * P * 2;
+    }
+    if (energized) {
         rec.energized = 1;
-        int const f = r.from_bus[m], t = r.to_bus[m];
         cplx uf[P], ut[P];
 #pragma unroll
         for (int ph = 0; ph < P; ++ph) {
@@ -923,7 +998,6 @@ __global__ void branch_records_kernel(DevOutput const* __restrict__ outs, Branch
             ut[ph] = t >= 0 ? cplx{o.u[((size_t(t) * P + ph) * 2) * sp + s], o.u[((size_t(t) * P + ph) * 2 + 1) * sp + s]}
                             : cplx{0.0, 0.0};
         }
-        double const* y = r.branch_y + size_t(m) * 4 * P * P * 2;
         auto Y = [&](int w, int rr, int cc) { return cplx{y[((w * P + rr) * P + cc) * 2], y[((w * P + rr) * P + cc) * 2 + 1]}; };
         double sum_sf = 0, sum_st = 0, max_i = 0;
 #pragma unroll
@@ -970,7 +1044,8 @@ __global__ void appliance_records_kernel(DevOutput const* __restrict__ outs, App
     rec.id = r.id[i];
     int const m = r.math[i];
     double const base_power = P == 1 ? 1e6 : 1e6 / 3.0;
-    if (m >= 0) {
+    bool const dead = o.dead && r.bus[i] >= 0 && o.dead[size_t(r.bus[i]) * sp + s];
+    if (m >= 0 && !dead) {
         rec.energized = r.status ? r.status[size_t(i) * sp + s] : r.status_const[i];
         size_t const base = (size_t(m) * 2) * P;
 #pragma unroll
@@ -987,44 +1062,93 @@ __global__ void appliance_records_kernel(DevOutput const* __restrict__ outs, App
     out[size_t(o.scenario_index[s]) * r.n_comp + i] = rec;
 }
 
-template <int P, class FT>
+template <int P, class FT, int KIND>
 void launch_fwd(DevGroup const* groups, WorkItem const* items, int n_items, int mode, hipStream_t st) {
     dim3 const grid{unsigned(n_items)};
     if (mode == MODE_INIT)
-        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_INIT>), grid, dim3(kWave), 0, st, groups, items);
+        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_INIT, KIND>), grid, dim3(kWave), 0, st, groups, items);
     else if (mode == MODE_NR)
-        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_NR>), grid, dim3(kWave), 0, st, groups, items);
-    else
-        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_RAW>), grid, dim3(kWave), 0, st, groups, items);
+        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_NR, KIND>), grid, dim3(kWave), 0, st, groups, items);
+    else if constexpr (KIND == KIND_PLAIN)
+        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_RAW, KIND_PLAIN>), grid, dim3(kWave), 0, st, groups, items);
 }
-template <int P, class FT>
+template <int P, class FT, int KIND>
 void launch_bwd(DevGroup const* groups, WorkItem const* items, int n_items, int mode, hipStream_t st) {
     dim3 const grid{unsigned(n_items)};
     if (mode == MODE_INIT)
-        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_INIT>), grid, dim3(kWave), 0, st, groups, items);
+        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_INIT, KIND>), grid, dim3(kWave), 0, st, groups, items);
     else if (mode == MODE_NR)
-        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_NR>), grid, dim3(kWave), 0, st, groups, items);
+        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_NR, KIND>), grid, dim3(kWave), 0, st, groups, items);
+    else if constexpr (KIND == KIND_PLAIN)
+        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_RAW, KIND_PLAIN>), grid, dim3(kWave), 0, st, groups, items);
+}
+template <bool FWD, int KIND>
+void launch_sweep(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
+                  hipStream_t st) {
+    if (P == 1) {
+        if constexpr (FWD)
+            fp32 ? launch_fwd<1, float, KIND>(groups, items, n_items, mode, st)
+                 : launch_fwd<1, double, KIND>(groups, items, n_items, mode, st);
+        else
+            fp32 ? launch_bwd<1, float, KIND>(groups, items, n_items, mode, st)
+                 : launch_bwd<1, double, KIND>(groups, items, n_items, mode, st);
+    } else {
+        if constexpr (FWD)
+            fp32 ? launch_fwd<3, float, KIND>(groups, items, n_items, mode, st)
+                 : launch_fwd<3, double, KIND>(groups, items, n_items, mode, st);
+        else
+            fp32 ? launch_bwd<3, float, KIND>(groups, items, n_items, mode, st)
+                 : launch_bwd<3, double, KIND>(groups, items, n_items, mode, st);
+    }
+}
+template <bool FWD>
+void launch_kind(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
+                 hipStream_t st, int kind) {
+    if (n_items <= 0) return;
+    if (kind == KIND_UNION)
+        launch_sweep<FWD, KIND_UNION>(groups, items, n_items, P, fp32, mode, st);
+    else if (kind == KIND_PACKED)
+        launch_sweep<FWD, KIND_PACKED>(groups, items, n_items, P, fp32, mode, st);
     else
-        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_RAW>), grid, dim3(kWave), 0, st, groups, items);
+        launch_sweep<FWD, KIND_PLAIN>(groups, items, n_items, P, fp32, mode, st);
+}
+
+__global__ void broadcast_y_kernel(double const* __restrict__ ydata, long long n_val, int S_pad,
+                                   double* __restrict__ ylane) {
+    long long const i = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_val * S_pad) return;
+    ylane[i] = ydata[i / S_pad];
+}
+__global__ void patch_y_kernel(int n_patch, int Q, int S_pad, int const* __restrict__ lane,
+                               int const* __restrict__ entry, double const* __restrict__ val,
+                               double* __restrict__ ylane) {
+    long long const i = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= (long long)(n_patch) * Q) return;
+    int const q = int(i / Q), c = int(i % Q);
+    ylane[(size_t(entry[q]) * Q + c) * S_pad + lane[q]] = val[i];
 }
 
 }  // namespace
 
 void launch_forward(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
-                    hipStream_t st) {
-    if (n_items <= 0) return;
-    if (P == 1)
-        fp32 ? launch_fwd<1, float>(groups, items, n_items, mode, st) : launch_fwd<1, double>(groups, items, n_items, mode, st);
-    else
-        fp32 ? launch_fwd<3, float>(groups, items, n_items, mode, st) : launch_fwd<3, double>(groups, items, n_items, mode, st);
+                    hipStream_t st, int kind) {
+    launch_kind<true>(groups, items, n_items, P, fp32, mode, st, kind);
 }
 void launch_backward(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
-                     hipStream_t st) {
-    if (n_items <= 0) return;
-    if (P == 1)
-        fp32 ? launch_bwd<1, float>(groups, items, n_items, mode, st) : launch_bwd<1, double>(groups, items, n_items, mode, st);
-    else
-        fp32 ? launch_bwd<3, float>(groups, items, n_items, mode, st) : launch_bwd<3, double>(groups, items, n_items, mode, st);
+                     hipStream_t st, int kind) {
+    launch_kind<false>(groups, items, n_items, P, fp32, mode, st, kind);
+}
+void launch_expand_ylane(double const* ydata, int E, int PP, int S_pad, double* ylane, int n_patch,
+                         int const* patch_lane, int const* patch_entry, double const* patch_val, hipStream_t st) {
+    long long const n_val = (long long)(E) * PP * 2;
+    long long const total = n_val * S_pad;
+    if (total > 0)
+        hipLaunchKernelGGL(broadcast_y_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, ydata, n_val,
+                           S_pad, ylane);
+    long long const np = (long long)(n_patch) * PP * 2;
+    if (np > 0)
+        hipLaunchKernelGGL(patch_y_kernel, dim3(unsigned((np + 255) / 256)), dim3(256), 0, st, n_patch, PP * 2,
+                           S_pad, patch_lane, patch_entry, patch_val, ylane);
 }
 void launch_converge(PairState const& ps, int it, int max_iter, double err_tol, int* active, hipStream_t st) {
     if (ps.n == 0) return;

@@ -12,6 +12,7 @@
 
 namespace pgm_amd {
 
+constexpr int kWaveLanes = 64;  // lanes of a gfx950 wavefront: one scenario (or task) each
 enum SweepMode : int { MODE_INIT = 0, MODE_NR = 1, MODE_RAW = 2 };
 enum DoneState : int { ACTIVE = 0, CONVERGED = 1, FAILED = 2 };
 enum ErrCode : int { ERR_NONE = 0, ERR_SPARSE = 1, ERR_DIVERGE = 2 };
@@ -31,11 +32,17 @@ struct DevGroup {
     int const* transpose = nullptr;
     int8_t const* fill = nullptr;
     double const* ydata = nullptr;  // E * P*P * 2
+    // union groups (scenarios that differ only in branch statuses share one superset structure): Y
+    // values per lane [E][P*P*2][S_pad] instead of ydata, and buses without a source in a lane
+    // [n][S_pad] (solved as identity rows, reported as not energised)
+    double const* ylane = nullptr;
+    uint8_t const* dead = nullptr;
     int const* schur_ptr = nullptr;
     int const* schur = nullptr;     // 3 ints per update
     int const* task_ptr = nullptr;
     int const* task_pivot = nullptr;
     int const* task_mm = nullptr;
+    int const* level_task = nullptr;  // tasks listed level by level
     int const* load_ptr = nullptr;
     int8_t const* load_type = nullptr;
     int const* src_ptr = nullptr;
@@ -58,7 +65,13 @@ struct DevGroup {
     double const* raw_b = nullptr;  // [n][2P][S_pad]
 };
 
-// one wave of work: (group, task, block of 64 scenarios)
+// union groups: Y values of every lane from the shared values plus per-lane patches of the entries
+// that switched branches touch (patch q: lane, entry and P*P*2 values)
+void launch_expand_ylane(double const* ydata, int E, int PP, int S_pad, double* ylane, int n_patch,
+                         int const* patch_lane, int const* patch_entry, double const* patch_val, hipStream_t st);
+
+// one wave of work: (group, task, block of 64 scenarios); packed groups: (group, first index into the
+// level's task list, number of tasks)
 struct WorkItem {
     int group, task, block;
 };
@@ -73,10 +86,13 @@ struct PairState {
 };
 
 // one sweep over the work items of one level (groups: device table of DevGroup)
+// kernel kinds: plain groups (one task and 64 scenarios per wave), union groups (per-lane Y values and
+// dead-bus rows) and packed groups (at most 32 scenarios: several tasks of one level per wave)
+enum : int { KIND_PLAIN = 0, KIND_UNION = 1, KIND_PACKED = 2 };
 void launch_forward(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
-                    hipStream_t st);
+                    hipStream_t st, int kind = KIND_PLAIN);
 void launch_backward(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
-                     hipStream_t st);
+                     hipStream_t st, int kind = KIND_PLAIN);
 // fold the deviation of iteration `it`, mark converged / diverged, count the still active pairs
 void launch_converge(PairState const& ps, int it, int max_iter, double err_tol, int* active, hipStream_t st);
 void launch_init_state(PairState const& ps, hipStream_t st);
@@ -93,6 +109,8 @@ struct DevOutput {
     int const* col = nullptr;
     int8_t const* fill = nullptr;
     double const* ydata = nullptr;
+    double const* ylane = nullptr;   // union groups, as in DevGroup
+    uint8_t const* dead = nullptr;   // union groups, as in DevGroup
     double const* u = nullptr;
     // loads (math order)
     int n_load = 0;
@@ -132,6 +150,10 @@ struct BranchRec {
     int const* from_bus;       // per math branch
     int const* to_bus;
     double const* branch_y;    // per math branch 4 * P*P * 2
+    // union groups: per lane and math branch the side statuses (bit 0 from, bit 1 to), and per math
+    // branch the admittances of the four status combinations [4][4 * P*P * 2]; nullptr otherwise
+    uint8_t const* lane_status;
+    double const* branch_y_var;
     double const* base_i_from;
     double const* base_i_to;
     double const* rating;      // line: i_n, transformer: sn
@@ -141,6 +163,7 @@ struct ApplRec {
     int n_comp;
     int const* id;
     int const* math;      // math index into si array or -1
+    int const* bus;       // bus of the appliance or -1 (union groups check it against the dead mask)
     double const* base_i;
     double direction;
     int8_t const* status; // per component per lane [n_comp][S_pad], or nullptr (use status_const)

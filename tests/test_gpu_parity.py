@@ -166,6 +166,107 @@ def test_year_20k_sample_with_switching():
     _compare(out, ref, True, slice(None))
 
 
+def _switching_case(n_scenarios=96, seed=5):
+    """meshed_1000 with a second source and four random line status rows per scenario (either side, so
+    one-sided open lines, islands without a source and grids split between the two sources all occur):
+    every scenario has its own topology, so the batch runs as one union group."""
+    from power_grid_model_amd import initialize_array
+
+    case = synthetic.meshed_1000(n_scenarios=n_scenarios)
+    inp = {k: v.copy() for k, v in case.input.items()}
+    src2 = inp["source"].copy()
+    src2["id"] = 30_000_001
+    src2["node"] = 700
+    src2["u_ref"] = 1.03
+    inp["source"] = np.concatenate([inp["source"], src2])
+    rng = np.random.default_rng(seed)
+    n_line = len(inp["line"])
+    upd_line = initialize_array("update", "line", (n_scenarios, 4))
+    for s in range(n_scenarios):
+        idx = rng.choice(n_line, 4, replace=False)
+        upd_line["id"][s] = inp["line"]["id"][idx]
+        upd_line["from_status"][s] = rng.integers(0, 2, 4)
+        upd_line["to_status"][s] = rng.integers(0, 2, 4)
+    upd = {"sym_load": case.update["sym_load"], "line": upd_line}
+    return inp, upd
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_union_group_switching(precision):
+    inp, upd = _switching_case()
+    model = PowerGridModel(inp)
+    batch = DeviceBatch(model, upd, factor_precision=precision)
+    batch.run()
+    out = batch.fetch()
+    assert batch.stats()["groups"] == 1  # every scenario in the one union group
+    ref, it_ref, st_ref, _ = ol.power_flow(inp, upd, len(upd["line"]), threading=0)
+    assert (st_ref == 0).all()
+    dead = (ref["node"]["energized"] == 0).sum(axis=1)
+    assert dead.max() > 0  # some scenarios have islands without a source
+    if precision == "fp64":
+        assert (batch.iterations() == it_ref).all()
+    else:
+        assert (batch.iterations() <= it_ref + 2).all()
+    _compare(out, ref, True, slice(None))
+    for comp in ("sym_load", "source"):
+        assert (out[comp]["energized"] == ref[comp]["energized"]).all()
+        for f in ("i", "s", "pf"):
+            floor = 1e-9 * max(np.max(np.abs(ref[comp][f])), 1.0)
+            np.testing.assert_allclose(out[comp][f], ref[comp][f], rtol=PQ_RTOL, atol=floor, err_msg=f"{comp}.{f}")
+    for f in ("i_from", "i_to", "s_from", "s_to", "loading"):
+        floor = 1e-9 * max(np.max(np.abs(ref["line"][f])), 1.0)
+        np.testing.assert_allclose(out["line"][f], ref["line"][f], rtol=PQ_RTOL, atol=floor, err_msg=f"line.{f}")
+    for f in ("p", "q"):
+        floor = 1e-9 * max(np.max(np.abs(ref["node"][f])), 1.0)
+        np.testing.assert_allclose(out["node"][f], ref["node"][f], rtol=PQ_RTOL, atol=floor, err_msg=f"node.{f}")
+
+
+def _tie_case():
+    """meshed_1000 plus 8 normally open ties; switching pattern k closes tie k and opens one line, and is
+    repeated by 1, 2, 3, 5, 7, 11, 1 and 33 scenarios: packed groups of one to eleven scenarios and one
+    plain group."""
+    from power_grid_model_amd import initialize_array
+
+    case = synthetic.meshed_1000(n_scenarios=63)
+    inp = {k: v.copy() for k, v in case.input.items()}
+    rng = np.random.default_rng(9)
+    ties = inp["line"][:8].copy()
+    ties["id"] = 11_000_000 + np.arange(8)
+    ties["from_node"] = rng.integers(100, 1000, 8)
+    ties["to_node"] = rng.integers(100, 1000, 8)
+    ties["from_status"] = 0
+    ties["to_status"] = 0
+    inp["line"] = np.concatenate([inp["line"], ties])
+    reps = [1, 2, 3, 5, 7, 11, 1, 33]
+    pattern = np.repeat(np.arange(8), reps)
+    opened = rng.choice(len(case.input["line"]), 8, replace=False)
+    upd_line = initialize_array("update", "line", (len(pattern), 2))
+    upd_line["id"][:, 0] = ties["id"][pattern]
+    upd_line["from_status"][:, 0] = 1
+    upd_line["to_status"][:, 0] = 1
+    upd_line["id"][:, 1] = inp["line"]["id"][opened[pattern]]
+    upd_line["from_status"][:, 1] = 0
+    upd_line["to_status"][:, 1] = 0
+    return inp, {"sym_load": case.update["sym_load"][: len(pattern)], "line": upd_line}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_packed_groups(precision):
+    inp, upd = _tie_case()
+    model = PowerGridModel(inp)
+    batch = DeviceBatch(model, upd, factor_precision=precision)
+    batch.run()
+    out = batch.fetch()
+    assert batch.stats()["groups"] == 8
+    ref, it_ref, st_ref, _ = ol.power_flow(inp, upd, len(upd["line"]), threading=0)
+    assert (st_ref == 0).all()
+    if precision == "fp64":
+        assert (batch.iterations() == it_ref).all()
+    else:
+        assert (batch.iterations() <= it_ref + 2).all()
+    _compare(out, ref, True, slice(None))
+
+
 @pytest.mark.parametrize("name,make,scen", WORKLOADS[:3], ids=[w[0] for w in WORKLOADS[:3]])
 def test_workload_mixed_precision(name, make, scen):
     case = make()
