@@ -61,6 +61,18 @@ def cpu_baseline(case, sample_scenarios, threads):
     return done / t_total, t_total
 
 
+def _leading(update, n):
+    """The first n scenarios of an update dataset (uniform arrays or sparse {indptr, data})."""
+    out = {}
+    for comp, v in update.items():
+        if isinstance(v, dict):
+            ind = v["indptr"][: n + 1]
+            out[comp] = {"indptr": ind.copy(), "data": v["data"][: ind[-1]]}
+        else:
+            out[comp] = v[:n]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +84,10 @@ def main():
     ap.add_argument("--no-fp64-leg", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--e2e-fraction", type=float, default=0.2,
+                    help="share of the scenarios timed through calculate_power_flow from host buffers (0: skip)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run a 256 MiB device-to-device torch copy (the reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): scenarios shard across GPUs without any "
@@ -79,6 +95,16 @@ def main():
         sys.exit(2)
 
     from power_grid_model_amd import DeviceBatch, PowerGridModel, synthetic
+
+    if args.copy_probe:
+        import torch
+
+        x = torch.empty(64 << 20, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log("[bench] copy probe: 268435456 bytes read and written by one torch copy kernel")
+        del x, y
 
     make = synthetic.CONFIGS[args.workload]
     t0 = time.perf_counter()
@@ -126,6 +152,21 @@ def main():
         fp64_value = case.n_scenarios / float(np.median(t64))
         del b64
 
+    # PCIe-inclusive rate: calculate_power_flow from host buffers (host topology and plans, upload, run,
+    # node / line / load records back to the host) on a leading sample of the scenarios
+    e2e = None
+    if args.e2e_fraction > 0:
+        n_e2e = max(1, int(case.n_scenarios * args.e2e_fraction))
+        upd = _leading(case.update, n_e2e)
+        comps = [c for c in ("node", "line", "sym_load", "asym_load") if c in case.input]
+        t0 = time.perf_counter()
+        model.calculate_power_flow(symmetric=case.symmetric, update_data=upd, factor_precision=args.precision,
+                                   output_component_types=comps)
+        secs = time.perf_counter() - t0
+        e2e = {"value": n_e2e / secs, "unit": "scenarios/s", "scenarios": n_e2e, "seconds": secs,
+               "outputs": comps}
+        log(f"[bench] calculate_power_flow from host buffers: {n_e2e} scenarios in {secs:.2f}s")
+
     cpu = None
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
@@ -166,6 +207,7 @@ def main():
                      "bytes_per_launch": fwd_bytes_per_launch, "avg_launch_ms": st["fwd_launch_ms"]},
         "step_roofline": {"algorithmic_bytes": step_bytes, "achieved_gbs": step_bytes / t_step / 1e9,
                           "frac": step_bytes / t_step / 1e9 / HBM_PEAK_GBS},
+        "pcie_inclusive": e2e,
         "cpu_baseline": cpu,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
     }

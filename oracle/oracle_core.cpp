@@ -985,7 +985,13 @@ template <int P> struct NRSolver {
         return b;
     }
 
-    MathOutput<P> run(MathInput<P> const& in, double err_tol, Idx max_iter) {
+    // method: 0 Newton-Raphson, 1 linear (math_solver.hpp:43-64: a model whose load_gens are all const_y
+    // always runs linear). Linear: the initial-guess solve is the answer (linear_pf_solver.hpp), and the
+    // results treat every load as const_y
+    MathOutput<P> run(MathInput<P> const& in, double err_tol, Idx max_iter, int method = 0) {
+        bool const all_const_y =
+            std::all_of(mm.load_type.begin(), mm.load_type.end(), [](int8_t t) { return t == 1; });
+        bool const linear = method == 1 || all_const_y;
         Idx const n = mm.n_bus;
         Idx const nnz_lu = ys.row_indptr_lu.back();
         std::vector<Blk<NB>> jac(nnz_lu);
@@ -1046,6 +1052,11 @@ template <int P> struct NRSolver {
                 vmag[b][ph] = std::abs(out.u[b].v[ph]);
                 theta[b][ph] = std::arg(out.u[b].v[ph]);
             }
+        if (linear) {
+            out.iterations = 1;
+            calculate_result(in, out, true);
+            return out;
+        }
 
         // iterations (iterative_pf_solver.hpp:55-74)
         double max_dev = std::numeric_limits<double>::infinity();
@@ -1158,7 +1169,7 @@ template <int P> struct NRSolver {
     }
 
     // calculate_pf_result (common_solver_functions.hpp:414-446), y_bus.hpp:482-546
-    void calculate_result(MathInput<P> const& in, MathOutput<P>& out) const {
+    void calculate_result(MathInput<P> const& in, MathOutput<P>& out, bool const_y = false) const {
         Idx const n = mm.n_bus;
         Idx const nbr = Idx(mm.branch_bus.size());
         out.br_if.resize(nbr);
@@ -1213,7 +1224,7 @@ template <int P> struct NRSolver {
                     cd const s = in.s_inj[l].v[ph];
                     cd const u = out.u[b].v[ph];
                     cd sl;
-                    switch (mm.load_type[l]) {
+                    switch (const_y ? int8_t(1) : mm.load_type[l]) {
                     case 0: sl = s; break;
                     case 1: sl = s * std::norm(u); break;
                     default: sl = s * std::abs(u); break;
@@ -1918,7 +1929,7 @@ template <int P> void write_outputs(Model const& m, State const& st, TopoCache c
 }
 
 template <int P> Idx solve_scenario(Model const& m, State const& st, TopoCache const& tc, double err_tol,
-                                    Idx max_iter, OrMutableDataset const* out, Idx scn) {
+                                    Idx max_iter, OrMutableDataset const* out, Idx scn, int method) {
     Idx const nmm = Idx(tc.topo.bus_nodes.size());
     std::vector<MathOutput<P>> res;
     Idx max_it = 0;
@@ -1928,7 +1939,7 @@ template <int P> Idx solve_scenario(Model const& m, State const& st, TopoCache c
         for (Idx l : tc.mm_load[g]) in.s_inj.push_back(load_s<P>(m, st, l));
         for (Idx s : tc.mm_source[g]) in.u_source.push_back(st.src_uref[s] * std::exp(cd(0.0, st.src_uang[s])));
         NRSolver<P> solver(mm);
-        res.push_back(solver.run(in, err_tol, max_iter));
+        res.push_back(solver.run(in, err_tol, max_iter, method));
         max_it = std::max(max_it, res.back().iterations);
     }
     if (out) write_outputs<P>(m, st, tc, res, *out, scn);
@@ -1952,6 +1963,14 @@ extern "C" int or_power_flow(OrConstDataset const* input, double system_frequenc
                              int64_t n_scenarios, int symmetric, double err_tol, int64_t max_iter, int64_t threading,
                              OrMutableDataset const* output, int64_t* iterations, int32_t* status, char* messages,
                              int64_t msg_stride) {
+    return or_power_flow_method(input, system_frequency, update, n_scenarios, symmetric, err_tol, max_iter,
+                                threading, 0, output, iterations, status, messages, msg_stride);
+}
+
+extern "C" int or_power_flow_method(OrConstDataset const* input, double system_frequency,
+                                    OrConstDataset const* update, int64_t n_scenarios, int symmetric, double err_tol,
+                                    int64_t max_iter, int64_t threading, int method, OrMutableDataset const* output,
+                                    int64_t* iterations, int32_t* status, char* messages, int64_t msg_stride) {
     Model model;
     try {
         model = build_model(*input, system_frequency);
@@ -1987,8 +2006,8 @@ extern "C" int or_power_flow(OrConstDataset const* input, double system_frequenc
                     tc = &own;
                     cache_valid = !topo_changed;
                 }
-                Idx const it = symmetric ? solve_scenario<1>(model, st, *tc, err_tol, max_iter, output, s)
-                                         : solve_scenario<3>(model, st, *tc, err_tol, max_iter, output, s);
+                Idx const it = symmetric ? solve_scenario<1>(model, st, *tc, err_tol, max_iter, output, s, method)
+                                         : solve_scenario<3>(model, st, *tc, err_tol, max_iter, output, s, method);
                 if (iterations) iterations[s] = it;
                 if (status) status[s] = 0;
                 write_msg(messages, msg_stride, s, "");

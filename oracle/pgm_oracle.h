@@ -87,6 +87,12 @@ int or_power_flow(OrConstDataset const* input, double system_frequency, OrConstD
                   OrMutableDataset const* output, int64_t* iterations, int32_t* status, char* messages,
                   int64_t msg_stride);
 
+/* or_power_flow with a calculation method: 0 Newton-Raphson, 1 linear (linear_pf_solver.hpp). */
+int or_power_flow_method(OrConstDataset const* input, double system_frequency, OrConstDataset const* update,
+                         int64_t n_scenarios, int symmetric, double err_tol, int64_t max_iter, int64_t threading,
+                         int method, OrMutableDataset const* output, int64_t* iterations, int32_t* status,
+                         char* messages, int64_t msg_stride);
+
 /* Minimum-degree ordering (sparse_ordering.hpp:183-219). The adjacency map is given as sorted keys with
  * CSR neighbour lists. alpha_out has room for n_vertices (all vertices), fills_out for 2*max_fills.
  * Returns the number of fills, or -1 if max_fills is too small. n_alpha_out receives the order length. */

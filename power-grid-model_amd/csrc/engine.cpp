@@ -14,6 +14,7 @@
 #include <cstring>
 #include <map>
 #include <numeric>
+#include <cstdlib>
 #include <sstream>
 #include <thread>
 
@@ -217,8 +218,15 @@ namespace {
 constexpr int kUnionBelow = 32;
 // groups of at most this many scenarios run on the packed kernels (64 / S tasks of a level per wave)
 constexpr int kPackedMax = 32;
-// target number of parallel tasks of a packed group's schedule
-constexpr int kPackedTarget = 64;
+// target number of parallel tasks of a packed group's schedule (PGM_AMD_PACKED_TARGET overrides)
+int packed_target() {
+    static int const t = [] {
+        char const* v = std::getenv("PGM_AMD_PACKED_TARGET");
+        int const x = v ? std::atoi(v) : 0;
+        return x > 0 ? x : 64;
+    }();
+    return t;
+}
 
 // every branch side of st is open where the base state has it open
 bool opens_only(Model const& m, ComponentState const& st) {
@@ -291,6 +299,7 @@ void upload_group(Batch& b, Group& g) {
     d.task_pivot = put(A, p.task_pivot);
     d.task_mm = put(A, p.task_mm);
     d.level_task = put(A, p.level_task);
+    d.keep = put(A, p.keep);
     d.load_ptr = put(A, p.load_ptr);
     d.load_type = put(A, p.load_type);
     d.src_ptr = put(A, p.src_ptr);
@@ -329,6 +338,7 @@ void upload_group(Batch& b, Group& g) {
     std::vector<int> si(g.scenarios.begin(), g.scenarios.end());
     si.resize(g.S_pad, 0);
     o.scenario_index = put(A, si);
+    o.linear = b.linear ? 1 : 0;
     o.ok = b.d_ok + g.lane_off;
     o.n = int(n);
     o.row_ptr = d.row_ptr;
@@ -717,7 +727,8 @@ Batch::~Batch() {
 std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, PGM_ConstDataset const* update) {
     if (opt.calculation_type != PGM_power_flow)
         throw PowerGridError("The calculation type is not supported by this build: only power flow is implemented\n");
-    if (opt.calculation_method != PGM_default_method && opt.calculation_method != PGM_newton_raphson)
+    if (opt.calculation_method != PGM_default_method && opt.calculation_method != PGM_newton_raphson &&
+        opt.calculation_method != PGM_linear)
         throw PowerGridError("The calculation method is invalid for this calculation!");
     if (update && update->next) throw PowerGridError("Cartesian product batches are not supported by this build\n");
     ensure_device();
@@ -730,6 +741,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     b.err_tol = opt.err_tol;
     b.max_iter = opt.max_iter;
     b.fp32 = opt.factor_precision == 1;
+    b.linear = opt.calculation_method == PGM_linear;
     b.is_batch = update != nullptr;
     b.n_scenarios = update ? update->batch_size : 1;
     PGM_HIP_CHECK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
@@ -879,7 +891,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
                     std::iota(all.begin(), all.end(), Idx{0});
                     int const nblk = int((g.scenarios.size() + 63) / 64);
                     g.packed = !g.is_union && g.scenarios.size() <= size_t(kPackedMax);
-                    int const target = g.packed ? kPackedTarget : std::clamp(4096 / std::max(nblk, 1), 4, 256);
+                    int const target = g.packed ? packed_target() : std::clamp(4096 / std::max(nblk, 1), 4, 256);
                     g.plan = build_plan(m, g.st, g.topo, all, sym, target);
                 } catch (std::exception const& e) {
                     errs[gi] = e.what();
@@ -912,6 +924,26 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     b.pairs.iters = dev_alloc<int>(b.arena, size_t(pairs));
     b.pairs.devmax = dev_alloc<unsigned long long>(b.arena, size_t(pairs));
     b.pairs.lastdev = dev_alloc<double>(b.arena, size_t(pairs));
+    // pairs solved by the linear method: every pair for calculation_method = linear, else the math
+    // models whose load_gens are all const_y (math_solver.hpp:43-64)
+    {
+        std::vector<uint8_t> lin(size_t(pairs), b.linear ? 1 : 0);
+        bool any = b.linear;
+        if (!b.linear)
+            for (auto& gp : b.groups) {
+                Plan const& p = gp->plan;
+                std::vector<char> all_y(size_t(p.n_mm), 1);
+                for (Idx bus = 0; bus < p.n; ++bus)
+                    for (int l = p.load_ptr[bus]; l < p.load_ptr[bus + 1]; ++l)
+                        if (p.load_type[l] != 1) all_y[p.bus_mm[bus]] = 0;
+                for (Idx k = 0; k < p.n_mm; ++k)
+                    if (all_y[k]) {
+                        any = true;
+                        std::fill_n(lin.begin() + gp->pair_off + k * gp->S_pad, gp->S_pad, uint8_t(1));
+                    }
+            }
+        if (any) b.d_pair_linear = put(b.arena, lin);
+    }
     b.d_active = dev_alloc<int>(b.arena, 1);
     b.d_ok = dev_alloc<int>(b.arena, size_t(std::max<long long>(lanes, 1)));
     int8_t* lstat_all = dev_alloc<int8_t>(b.arena, size_t(std::max<long long>(lanes * m.n_lg(), 1)));
@@ -1079,7 +1111,13 @@ void Batch::run() {
     for (int l = 0; l < L; ++l) launch_forward(d_groups, d_items[l], n_items[l], P, fp32, MODE_INIT, st);
     for (int l = L - 1; l >= 0; --l) launch_backward(d_groups, d_items[l], n_items[l], P, fp32, MODE_INIT, st);
     double active_frac = 1.0;
-    for (int it = 1; it <= int(max_iter); ++it) {
+    if (d_pair_linear) {
+        // the linear solve is the answer of these pairs: no Newton-Raphson iteration
+        join();
+        launch_retire_linear(pairs, d_pair_linear, st);
+        fork();
+    }
+    for (int it = 1; it <= (linear ? 0 : int(max_iter)); ++it) {
         union_sweeps(MODE_NR);
         for (int l = 0; l < L; ++l) {
             fwd_events.push_back(event_from_pool(fwd_events.size()));
@@ -1295,6 +1333,7 @@ int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices
     d.task_ptr = put(A, p.task_ptr);
     d.task_pivot = put(A, p.task_pivot);
     d.task_mm = put(A, p.task_mm);
+    d.keep = put(A, p.keep);
     std::vector<double> a(size_t(p.E) * NB2 * sp, 0.0), bv(size_t(n) * NB * sp, 0.0);
     for (int s = 0; s < S; ++s) {
         for (Idx e = 0; e < p.E; ++e)

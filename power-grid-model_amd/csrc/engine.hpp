@@ -48,6 +48,7 @@ struct Batch {
     double err_tol = 1e-8;
     Idx max_iter = 20;
     bool fp32 = false;
+    bool linear = false;  // calculation_method = linear
     bool is_batch = false;
     Idx n_scenarios = 1;
     hipStream_t stream = nullptr;
@@ -73,6 +74,7 @@ struct Batch {
     PairState pairs;
     long long real_pairs = 0;
     long long total_lanes = 0;
+    uint8_t* d_pair_linear = nullptr;  // per pair: solved by the linear method (nullptr: none)
     int* d_active = nullptr;
     int* h_active = nullptr;  // pinned
     int* d_ok = nullptr;      // per lane of every group

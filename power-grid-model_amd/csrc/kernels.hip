@@ -356,6 +356,44 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
         int const rb = g.row_ptr[p], re = g.row_ptr[p + 1], dg = g.diag[p];
         // a bus without a source in this lane (union groups): identity row, zero row and column
         bool const dead_p = UNI && g.dead[size_t(p) * sp + s] != 0;
+        // rows with one U entry and no off-diagonal Schur target (every bus of a radial feeder) keep
+        // their raw U entry and its transpose in registers until the row is finished
+        bool const keep = P == 1 && g.keep[p] != 0;
+        FT ku[NB2], kl[NB2];
+#pragma unroll
+        for (int q = 0; q < NB2; ++q) ku[q] = kl[q] = FT(0);
+        auto put_u = [&](int k, FT const (&a)[NB2]) {
+            if (keep) {
+#pragma unroll
+                for (int q = 0; q < NB2; ++q) ku[q] = a[q];
+            } else {
+                lu.store(k, a);
+            }
+        };
+        auto put_l = [&](int e, FT const (&a)[NB2]) {
+            if (keep) {
+#pragma unroll
+                for (int q = 0; q < NB2; ++q) kl[q] = a[q];
+            } else {
+                lu.store(e, a);
+            }
+        };
+        auto zero_u = [&](int k) {
+            if (keep) {
+#pragma unroll
+                for (int q = 0; q < NB2; ++q) ku[q] = FT(0);
+            } else {
+                lu.store_zero(k);
+            }
+        };
+        auto zero_l = [&](int e) {
+            if (keep) {
+#pragma unroll
+                for (int q = 0; q < NB2; ++q) kl[q] = FT(0);
+            } else {
+                lu.store_zero(e);
+            }
+        };
         double D[NB2];
         double rhs[NB];
 #pragma unroll
@@ -372,11 +410,11 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                 FT a[NB2];
 #pragma unroll
                 for (int q = 0; q < NB2; ++q) a[q] = FT(g.raw_a[(size_t(k) * NB2 + q) * sp + s]);
-                lu.store(k, a);
+                put_u(k, a);
                 int const e = g.transpose[k];
 #pragma unroll
                 for (int q = 0; q < NB2; ++q) a[q] = FT(g.raw_a[(size_t(e) * NB2 + q) * sp + s]);
-                lu.store(e, a);
+                put_l(e, a);
             }
         } else if constexpr (MODE == MODE_INIT) {
             // admittance matrix in the real domain plus load admittances and sources
@@ -394,15 +432,15 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
 #pragma unroll
                     for (int q = 0; q < NB2; ++q) D[q] = double(a[q]);
                 } else {
-                    lu.store(k, a);
+                    put_u(k, a);
                     int const e = g.transpose[k];
                     if (g.fill[e] || dead_p) {
-                        lu.store_zero(e);
+                        zero_l(e);
                     } else {
                         double ye[PP * 2];
                         load_y<P, UNI>(g, e, s, ye);
                         linear_block<P, FT>(ye, a);
-                        lu.store(e, a);
+                        put_l(e, a);
                     }
                 }
             }
@@ -449,8 +487,8 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
             }
         } else if (dead_p) {
             for (int k = dg + 1; k < re; ++k) {
-                lu.store_zero(k);
-                lu.store_zero(g.transpose[k]);
+                zero_u(k);
+                zero_l(g.transpose[k]);
             }
 #pragma unroll
             for (int q = 0; q < NB2; ++q) D[q] = (q % (NB + 1) == 0) ? 1.0 : 0.0;
@@ -465,8 +503,8 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                 int const j = g.col[k];
                 if (g.fill[k]) {
                     if (j > p) {
-                        lu.store_zero(k);
-                        lu.store_zero(g.transpose[k]);
+                        zero_u(k);
+                        zero_l(g.transpose[k]);
                     }
                     continue;
                 }
@@ -485,7 +523,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                 } else {
                     FT a[NB2];
                     hnml<P, FT>(yk, up, uj, a, pr, qr);
-                    if (j > p) lu.store(k, a);
+                    if (j > p) put_u(k, a);
                 }
 #pragma unroll
                 for (int ph = 0; ph < P; ++ph) {
@@ -495,14 +533,14 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                 if (j > p) {
                     int const e = g.transpose[k];
                     if (g.fill[e]) {
-                        lu.store_zero(e);
+                        zero_l(e);
                     } else {
                         FT a[NB2];
                         double pr2[P], qr2[P];
                         double ye[PP * 2];
                         load_y<P, UNI>(g, e, s, ye);
                         hnml<P, FT>(ye, uj, up, a, pr2, qr2);
-                        lu.store(e, a);
+                        put_l(e, a);
                     }
                 }
             }
@@ -562,7 +600,9 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
             }
         }
 
-        // Schur updates owned by pivot p, pulled from the already factorised descendants
+        // Schur updates owned by pivot p, pulled from the already factorised descendants. The diagonal
+        // ones come one per L entry (p, k) of the row, k ascending, and also carry the forward
+        // substitution b_p -= L_pk y_k
         FT Df[NB2];
 #pragma unroll
         for (int k = 0; k < NB2; ++k) Df[k] = FT(D[k]);
@@ -581,6 +621,17 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                         for (int k = 0; k < NB; ++k) acc += L[k * NB + r] * U[c * NB + k];
                         Df[c * NB + r] -= acc;
                     }
+                int const j = g.col[le];
+                double yj[NB];
+#pragma unroll
+                for (int r = 0; r < NB; ++r) yj[r] = g.y[(size_t(j) * NB + r) * sp + s];
+#pragma unroll
+                for (int r = 0; r < NB; ++r) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) acc += double(L[c * NB + r]) * yj[c];
+                    rhs[r] -= acc;
+                }
             } else {
                 FT A[NB2];
                 lu.load(tgt, A);
@@ -609,32 +660,27 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
         // U row: U_pj = L^-1 P A_pj ; L column: L_ip = A_ip Q U^-1
         for (int k = dg + 1; k < re; ++k) {
             FT A[NB2];
-            lu.load(k, A);
+            if (keep) {
+#pragma unroll
+                for (int q = 0; q < NB2; ++q) A[q] = ku[q];
+            } else {
+                lu.load(k, A);
+            }
             apply_p_rows<NB, FT>(A, code);
             lower_left<NB, FT>(Df, A);
             lu.store(k, A);
             int const e = g.transpose[k];
-            lu.load(e, A);
+            if (keep) {
+#pragma unroll
+                for (int q = 0; q < NB2; ++q) A[q] = kl[q];
+            } else {
+                lu.load(e, A);
+            }
             apply_q_cols<NB, FT>(A, code);
             upper_right<NB, FT>(Df, A);
             lu.store(e, A);
         }
         // forward substitution: y_p = L_pp^-1 P (b_p - sum_k L_pk y_k)
-        for (int k = rb; k < dg; ++k) {
-            int const j = g.col[k];
-            FT L[NB2];
-            lu.load(k, L);
-            double yj[NB];
-#pragma unroll
-            for (int r = 0; r < NB; ++r) yj[r] = g.y[(size_t(j) * NB + r) * sp + s];
-#pragma unroll
-            for (int r = 0; r < NB; ++r) {
-                double acc = 0.0;
-#pragma unroll
-                for (int c = 0; c < NB; ++c) acc += double(L[c * NB + r]) * yj[c];
-                rhs[r] -= acc;
-            }
-        }
         apply_p_vec<NB>(rhs, code);
         lower_vec<NB, FT>(Df, rhs);
 #pragma unroll
@@ -733,6 +779,13 @@ __global__ void init_state_kernel(PairState ps) {
     ps.lastdev[i] = 0.0;
 }
 
+__global__ void retire_linear_kernel(PairState ps, uint8_t const* __restrict__ pair_linear) {
+    size_t const i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(ps.n) || !pair_linear[i] || ps.done[i] != ACTIVE) return;
+    ps.done[i] = CONVERGED;
+    ps.iters[i] = 1;
+}
+
 // pairs of padding lanes are never touched by the sweeps: their deviation stays 0 and they retire
 // at the first check
 __global__ void converge_kernel(PairState ps, int it, int max_iter, double tol, int* active) {
@@ -800,7 +853,7 @@ __global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long
 #pragma unroll
     for (int ph = 0; ph < P; ++ph) iload[ph] = acc_s[ph] = {0.0, 0.0};
     for (int l = o.load_ptr[b]; l < o.load_ptr[b + 1]; ++l) {
-        int const type = o.load_type[l];
+        int const type = o.linear ? 1 : o.load_type[l];
 #pragma unroll
         for (int ph = 0; ph < P; ++ph) {
             cplx sv = {o.s_inj[((size_t(l) * P + ph) * 2) * sp + s], o.s_inj[((size_t(l) * P + ph) * 2 + 1) * sp + s]};
@@ -937,7 +990,8 @@ __global__ void node_records_kernel(DevOutput const* __restrict__ outs, NodeRec 
     NodeRec const& r = recs[gr];
     size_t const idx = size_t(gidx - offsets[gr]);
     size_t const sp = o.S_pad;
-    int const i = int(idx / sp), s = int(idx % sp);
+    // scenario-major: consecutive threads write consecutive records of one scenario
+    int const s = int(idx / size_t(r.n_comp)), i = int(idx % size_t(r.n_comp));
     if (s >= o.S || !o.ok[s]) return;
     NodeOutput<P> rec{};
     rec.id = r.id[i];
@@ -970,7 +1024,8 @@ __global__ void branch_records_kernel(DevOutput const* __restrict__ outs, Branch
     BranchRec const& r = recs[gr];
     size_t const idx = size_t(gidx - offsets[gr]);
     size_t const sp = o.S_pad;
-    int const i = int(idx / sp), s = int(idx % sp);
+    // scenario-major: consecutive threads write consecutive records of one scenario
+    int const s = int(idx / size_t(r.n_comp)), i = int(idx % size_t(r.n_comp));
     if (s >= o.S || !o.ok[s]) return;
     BranchOutput<P> rec{};
     rec.id = r.id[i];
@@ -1038,7 +1093,8 @@ __global__ void appliance_records_kernel(DevOutput const* __restrict__ outs, App
     ApplRec const& r = recs[gr];
     size_t const idx = size_t(gidx - offsets[gr]);
     size_t const sp = o.S_pad;
-    int const i = int(idx / sp), s = int(idx % sp);
+    // scenario-major: consecutive threads write consecutive records of one scenario
+    int const s = int(idx / size_t(r.n_comp)), i = int(idx % size_t(r.n_comp));
     if (s >= o.S || !o.ok[s]) return;
     ApplianceOutput<P> rec{};
     rec.id = r.id[i];
@@ -1154,6 +1210,10 @@ void launch_converge(PairState const& ps, int it, int max_iter, double err_tol, 
     if (ps.n == 0) return;
     hipLaunchKernelGGL(converge_kernel, dim3(unsigned((ps.n + 255) / 256)), dim3(256), 0, st, ps, it, max_iter,
                        err_tol, active);
+}
+void launch_retire_linear(PairState const& ps, uint8_t const* pair_linear, hipStream_t st) {
+    if (ps.n == 0) return;
+    hipLaunchKernelGGL(retire_linear_kernel, dim3(unsigned((ps.n + 255) / 256)), dim3(256), 0, st, ps, pair_linear);
 }
 void launch_init_state(PairState const& ps, hipStream_t st) {
     if (ps.n == 0) return;

@@ -43,6 +43,7 @@ struct DevGroup {
     int const* task_pivot = nullptr;
     int const* task_mm = nullptr;
     int const* level_task = nullptr;  // tasks listed level by level
+    int8_t const* keep = nullptr;     // per pivot: one U entry, no off-diagonal Schur target
     int const* load_ptr = nullptr;
     int8_t const* load_type = nullptr;
     int const* src_ptr = nullptr;
@@ -96,6 +97,8 @@ void launch_backward(DevGroup const* groups, WorkItem const* items, int n_items,
 // fold the deviation of iteration `it`, mark converged / diverged, count the still active pairs
 void launch_converge(PairState const& ps, int it, int max_iter, double err_tol, int* active, hipStream_t st);
 void launch_init_state(PairState const& ps, hipStream_t st);
+// pairs flagged linear that are still active after the linear solve: converged after one solve
+void launch_retire_linear(PairState const& ps, uint8_t const* pair_linear, hipStream_t st);
 
 // outputs of a group, written as the caller's records at out[scenario_index[s] * n_comp + i]
 struct DevOutput {
@@ -103,6 +106,7 @@ struct DevOutput {
     int S = 0, S_pad = 0;
     int const* scenario_index = nullptr;  // global scenario of each lane
     int const* ok = nullptr;              // per lane: 1 when every math model converged
+    int linear = 0;                       // linear method: load results as const_y (linear_pf_solver.hpp)
     // buses / Y rows (Y entries = non-fill LU entries)
     int n = 0;
     int const* row_ptr = nullptr;

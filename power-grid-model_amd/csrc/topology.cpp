@@ -357,6 +357,12 @@ void build_schedule(Plan& p, int target_parallel_tasks) {
     for (Idx r = 0; r < n; ++r) p.schur_ptr[r + 1] = p.schur_ptr[r] + int(own[r].size());
     p.schur.reserve(p.schur_ptr[n]);
     for (Idx r = 0; r < n; ++r) p.schur.insert(p.schur.end(), own[r].begin(), own[r].end());
+    p.keep.assign(n, 0);
+    for (Idx r = 0; r < n; ++r) {
+        bool k = p.row_ptr[r + 1] - p.diag[r] - 1 == 1;
+        for (auto const& u : own[r]) k = k && u[0] == p.diag[r];
+        p.keep[r] = k ? 1 : 0;
+    }
 
     // tasks over the elimination tree: parent(p) = first column right of the diagonal in row p
     std::vector<int> parent(n, -1);
@@ -401,6 +407,7 @@ void build_schedule(Plan& p, int target_parallel_tasks) {
             bool const small = W[r] <= T || !has_child;
             if (last ? !parent_rem : (small && (!parent_rem || W[parent[r]] > T))) roots.push_back(int(r));
         }
+        std::vector<std::pair<double, int>> level_tasks;  // (weight, task)
         for (int root : roots) {
             // collect the subtree of root among remaining pivots, in increasing order
             std::vector<int> members{root};
@@ -408,16 +415,23 @@ void build_schedule(Plan& p, int target_parallel_tasks) {
                 for (int c : children[members[q]])
                     if (remaining[c]) members.push_back(c);
             std::sort(members.begin(), members.end());
+            double tw = 0;
             for (int v : members) {
                 p.task_pivot.push_back(v);
                 remaining[v] = 0;
                 level_of[v] = lvl;
+                tw += w[v];
             }
             n_rem -= Idx(members.size());
             p.task_ptr.push_back(int(p.task_pivot.size()));
             p.task_mm.push_back(p.bus_mm[root]);
-            p.level_task.push_back(int(p.task_mm.size()) - 1);
+            level_tasks.emplace_back(tw, int(p.task_mm.size()) - 1);
         }
+        // heaviest tasks first: waves start longest-first, and packed waves (several tasks of a level
+        // per wave) hold tasks of similar length
+        std::stable_sort(level_tasks.begin(), level_tasks.end(),
+                         [](auto const& a, auto const& b) { return a.first > b.first; });
+        for (auto const& lt : level_tasks) p.level_task.push_back(lt.second);
         p.level_ptr.push_back(int(p.level_task.size()));
         if (roots.empty()) throw std::logic_error("task decomposition made no progress");
     }

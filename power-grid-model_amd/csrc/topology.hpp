@@ -52,6 +52,8 @@ struct Plan {
     // Schur updates in pull form, per pivot: (target, l, u) entry triples, k ascending per target
     std::vector<int> schur_ptr;
     std::vector<std::array<int, 3>> schur;
+    // per pivot: the row has exactly one U entry and every Schur update it owns targets its diagonal
+    std::vector<int8_t> keep;
     // tasks: disjoint sets of pivots; a task only depends on tasks of lower level
     std::vector<int> task_ptr, task_pivot, task_mm;
     std::vector<int> level_ptr, level_task;  // tasks per level

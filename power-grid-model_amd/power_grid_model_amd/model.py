@@ -295,11 +295,12 @@ class DeviceBatch:
 
     def __init__(self, model: PowerGridModel, update_data: dict | None, *, symmetric: bool = True,
                  error_tolerance: float = 1e-8, max_iterations: int = 20,
-                 factor_precision: FactorPrecision | str = FactorPrecision.fp64):
+                 factor_precision: FactorPrecision | str = FactorPrecision.fp64,
+                 calculation_method: CalculationMethod | str = CalculationMethod.newton_raphson):
         self._model = model
         self._handle = Handle()
         self.symmetric = symmetric
-        self._opt = model._options(symmetric, error_tolerance, max_iterations, CalculationMethod.newton_raphson, -1,
+        self._opt = model._options(symmetric, error_tolerance, max_iterations, calculation_method, -1,
                                    factor_precision)
         self._upd = _make_const_dataset(self._handle, update_data, "update", True) if update_data else None
         self._ptr = lib.PGM_amd_batch_create(self._handle.ptr, model._ptr, self._opt.ptr,

@@ -43,6 +43,9 @@ def lib():
         _lib.or_power_flow.restype = C.c_int
         _lib.or_power_flow.argtypes = [C.POINTER(_Const), C.c_double, C.POINTER(_Const), _i64, C.c_int,
                                        C.c_double, _i64, _i64, C.POINTER(_Mut), _p, _p, _p, _i64]
+        _lib.or_power_flow_method.restype = C.c_int
+        _lib.or_power_flow_method.argtypes = [C.POINTER(_Const), C.c_double, C.POINTER(_Const), _i64, C.c_int,
+                                              C.c_double, _i64, _i64, C.c_int, C.POINTER(_Mut), _p, _p, _p, _i64]
         _lib.or_minimum_degree_ordering.restype = _i64
         _lib.or_minimum_degree_ordering.argtypes = [_i64, _p, _p, _p, _p, _p, _p, _i64]
         _lib.or_build_topology.restype = _i64
@@ -61,8 +64,12 @@ def _ptr(a):
     return None if a is None else a.ctypes.data
 
 
+METHODS = {"newton_raphson": 0, "linear": 1}
+
+
 def power_flow(input_data: dict, update: dict | None = None, n_scenarios: int = 1, symmetric: bool = True,
-               err_tol: float = 1e-8, max_iter: int = 20, threading: int = -1, system_frequency: float = 50.0):
+               err_tol: float = 1e-8, max_iter: int = 20, threading: int = -1, system_frequency: float = 50.0,
+               method: str = "newton_raphson"):
     """Run the oracle. update: {component: array of shape (n_scenarios, rows_per_scenario)}.
 
     Returns (output dict {component: array (n_scenarios, n)}, iterations, status, messages)."""
@@ -101,9 +108,9 @@ def power_flow(input_data: dict, update: dict | None = None, n_scenarios: int = 
     status = np.zeros(n_scenarios, dtype=np.int32)
     stride = 512
     msgs = np.zeros(n_scenarios * stride, dtype=np.uint8)
-    rc = lib().or_power_flow(C.byref(cin), system_frequency, C.byref(cup) if cup is not None else None,
-                             n_scenarios, int(symmetric), err_tol, max_iter, threading, C.byref(cout),
-                             _ptr(iters), _ptr(status), _ptr(msgs), stride)
+    rc = lib().or_power_flow_method(C.byref(cin), system_frequency, C.byref(cup) if cup is not None else None,
+                                    n_scenarios, int(symmetric), err_tol, max_iter, threading, METHODS[method],
+                                    C.byref(cout), _ptr(iters), _ptr(status), _ptr(msgs), stride)
     messages = [bytes(msgs[i * stride:(i + 1) * stride]).split(b"\0")[0].decode() for i in range(n_scenarios)]
     if rc != 0:
         raise RuntimeError(messages[0])

@@ -91,6 +91,21 @@ def test_validation_case_gpu(name, sym, batch):
         compare_result(out, exp, params["rtol"], params["atol"], s)
 
 
+LINEAR_CASES = list_cases("linear")
+
+
+@pytest.mark.parametrize("name,sym,batch", LINEAR_CASES, ids=[f"{c[0]}-{'sym' if c[1] else 'asym'}"
+                                                              f"{'-batch' if c[2] else ''}" for c in LINEAR_CASES])
+def test_validation_case_gpu_linear(name, sym, batch):
+    params, input_data, update, n_scenarios, expected = load_case(name, sym, batch)
+    model = PowerGridModel(input_data)
+    out = model.calculate_power_flow(symmetric=sym, update_data=update, calculation_method="linear")
+    if not batch:
+        out = {k: v.reshape(1, *v.shape) for k, v in out.items()}
+    for s, exp in enumerate(expected):
+        compare_result(out, exp, params["rtol"], params["atol"], s)
+
+
 # ------------------------------------------------------------------------------------------------
 # synthetic workloads against the oracle
 # ------------------------------------------------------------------------------------------------
@@ -281,6 +296,43 @@ def test_workload_mixed_precision(name, make, scen):
     # inexact Newton: the same 1e-8 pu stopping rule, at most two extra iterations
     assert (batch.iterations() <= it_ref + 2).all()
     _compare(out, ref, case.symmetric, slice(None))
+
+
+@pytest.mark.parametrize("name,make,scen", WORKLOADS[:3], ids=[w[0] for w in WORKLOADS[:3]])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_workload_linear_method(name, make, scen, precision):
+    case = make()
+    upd = _subset(case, scen[:64])
+    model = PowerGridModel(case.input)
+    batch = DeviceBatch(model, upd, symmetric=case.symmetric, factor_precision=precision,
+                        calculation_method="linear")
+    batch.run()
+    out = batch.fetch()
+    assert (batch.iterations() == 1).all()
+    oupd, k = ol.uniform_update(upd, case.input)
+    ref, it_ref, st_ref, _ = ol.power_flow(case.input, oupd, k, symmetric=case.symmetric, method="linear")
+    assert (st_ref == 0).all()
+    if precision == "fp64":
+        _compare(out, ref, case.symmetric, slice(None))
+    else:
+        # one FP32 solve, no Newton correction: FP32 accuracy of the factorisation
+        np.testing.assert_allclose(out["node"]["u_pu"], ref["node"]["u_pu"], atol=1e-5, rtol=0)
+
+
+def test_all_const_y_grid_runs_linear():
+    """Newton-Raphson on a grid whose loads are all const_y runs the linear method
+    (math_solver.hpp:43-64): one solve, the oracle's results."""
+    case = synthetic.radial_200(n_scenarios=16)
+    inp = {k: v.copy() for k, v in case.input.items()}
+    inp["sym_load"]["type"] = 1
+    model = PowerGridModel(inp)
+    batch = DeviceBatch(model, case.update)
+    batch.run()
+    out = batch.fetch()
+    ref, it_ref, st_ref, _ = ol.power_flow(inp, case.update, 16)
+    assert (st_ref == 0).all() and (it_ref == 1).all()
+    assert (batch.iterations() == 1).all()
+    _compare(out, ref, True, slice(None))
 
 
 def test_error_cases_match_oracle():

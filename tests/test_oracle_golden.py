@@ -259,10 +259,69 @@ def test_two_bus_quadratic_closed_form():
     assert abs(u0 - 1.0) < 1e-9
 
 
+def _two_node_grid(load_type):
+    from pgm_formats import INPUT, empty
+
+    node = empty(INPUT["node"], 2)
+    node["id"] = [1, 2]
+    node["u_rated"] = 10.5e3
+    line = empty(INPUT["line"], 1)
+    line["id"] = 3
+    line["from_node"] = 1
+    line["to_node"] = 2
+    line["from_status"] = 1
+    line["to_status"] = 1
+    line[["r1", "x1", "c1", "tan1", "r0", "x0", "c0", "tan0", "i_n"]] = (0.5, 1.5, 0.0, 0.0, 1.5, 4.5, 0.0, 0.0, 500.0)
+    src = empty(INPUT["source"], 1)
+    src["id"] = 4
+    src["node"] = 1
+    src["status"] = 1
+    src["u_ref"] = 1.02
+    src["sk"] = 1e9
+    load = empty(INPUT["sym_load"], 1)
+    load["id"] = 5
+    load["node"] = 2
+    load["status"] = 1
+    load["type"] = load_type
+    load["p_specified"] = 2e6
+    load["q_specified"] = 0.6e6
+    return {"node": node, "line": line, "source": src, "sym_load": load}
+
+
+@pytest.mark.parametrize("load_type", [0, 1])
+def test_linear_method_closed_form(load_type):
+    """Linear power flow (linear_pf_solver.hpp): every load is the admittance -conj(s) at 1 pu, one
+    complex solve, load results as const_y. A two-node grid solved with numpy."""
+    grid = _two_node_grid(load_type)
+    out, it, st, msg = ol.power_flow(grid, method="linear")
+    assert st[0] == 0
+    zb = 10.5e3 ** 2 / 1e6
+    y_line = 1.0 / ((0.5 + 1.5j) / zb)
+    z = 1e6 / 1e9  # source impedance in pu, rx = 0.1
+    x1 = z / math.sqrt(1 + 0.1 ** 2)
+    y_src = 1.0 / (0.1 * x1 + 1j * x1)
+    s = -(2e6 + 0.6e6j) / 1e6
+    y_load = -np.conj(s)
+    a = np.array([[y_src + y_line, -y_line], [-y_line, y_line + y_load]])
+    u = np.linalg.solve(a, np.array([y_src * 1.02, 0.0]))
+    np.testing.assert_allclose(out["node"]["u_pu"][0], np.abs(u), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(out["node"]["u_angle"][0], np.angle(u), rtol=0, atol=1e-12)
+    # const_y results: p = p_specified |u|^2 whatever the load type
+    np.testing.assert_allclose(out["sym_load"]["p"][0, 0], 2e6 * abs(u[1]) ** 2, rtol=1e-12)
+    np.testing.assert_allclose(out["sym_load"]["q"][0, 0], 0.6e6 * abs(u[1]) ** 2, rtol=1e-12)
+    # Newton-Raphson on an all-const_y grid runs the linear method too (math_solver.hpp:43-64)
+    nr = ol.power_flow(grid)[0]
+    if load_type == 1:
+        assert nr["node"]["u_pu"].tobytes() == out["node"]["u_pu"].tobytes()
+    else:
+        assert abs(nr["node"]["u_pu"][0, 1] - abs(u[1])) > 1e-6
+
+
 # ------------------------------------------------------------------------------------------------
 # reference validation cases
 # ------------------------------------------------------------------------------------------------
 CASES = list_cases()
+LINEAR_CASES = list_cases("linear")
 
 
 @pytest.mark.parametrize("name,sym,batch", CASES, ids=[f"{c[0]}-{'sym' if c[1] else 'asym'}"
@@ -270,6 +329,16 @@ CASES = list_cases()
 def test_validation_case(name, sym, batch):
     params, input_data, update, n_scenarios, expected = load_case(name, sym, batch)
     out, iters, status, msgs = ol.power_flow(input_data, update, n_scenarios, symmetric=sym, threading=-1)
+    assert (status == 0).all(), msgs
+    for s, exp in enumerate(expected):
+        compare_result(out, exp, params["rtol"], params["atol"], s)
+
+
+@pytest.mark.parametrize("name,sym,batch", LINEAR_CASES, ids=[f"{c[0]}-{'sym' if c[1] else 'asym'}"
+                                                              f"{'-batch' if c[2] else ''}" for c in LINEAR_CASES])
+def test_validation_case_linear(name, sym, batch):
+    params, input_data, update, n_scenarios, expected = load_case(name, sym, batch)
+    out, iters, status, msgs = ol.power_flow(input_data, update, n_scenarios, symmetric=sym, method="linear")
     assert (status == 0).all(), msgs
     for s, exp in enumerate(expected):
         compare_result(out, exp, params["rtol"], params["atol"], s)

@@ -19,10 +19,15 @@ from pgm_formats import batch_to_uniform, load_dataset
 GOLDEN = Path(__file__).resolve().parent / "golden" / "power_flow"
 
 
-def list_cases():
+def list_cases(method: str = "newton_raphson"):
+    """(case, symmetric, batch) of every fixture whose params.json lists the calculation method."""
     cases = []
     for params in sorted(GOLDEN.rglob("params.json")):
         case = params.parent
+        methods = json.load(open(params)).get("calculation_method")
+        methods = methods if isinstance(methods, list) else [methods]
+        if method not in methods:
+            continue
         for sym, stem in ((True, "sym_output"), (False, "asym_output")):
             for batch in (False, True):
                 f = case / (stem + ("_batch" if batch else "") + ".json")
