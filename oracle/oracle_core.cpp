@@ -494,6 +494,7 @@ struct CompBranch {
 };
 struct TopoResult {
     std::vector<Idx> node_group, node_pos;
+    std::vector<double> phase;  // phase shift of every node against its source (topology.hpp dfs)
     std::vector<std::vector<Idx>> bus_nodes;  // per math model: node index per bus
     std::vector<Idx> slack;
     std::vector<bool> radial;
@@ -618,7 +619,7 @@ TopoResult build_topology(Idx n_node, std::vector<CompBranch> const& branches, s
         res.fill_in.push_back(fills);
         ++group;
     }
-    (void)phase;
+    res.phase = phase;
     return res;
 }
 
@@ -916,6 +917,7 @@ template <int NB> struct SparseLU {
 // ------------------------------------------------------------------------------------------------
 template <int P> struct MathModel {
     Idx n_bus = 0;
+    std::vector<double> phase_shift;  // per bus
     std::vector<std::array<Idx, 2>> branch_bus;
     std::vector<BranchParam<P>> branch_param;
     std::vector<Idx> shunt_bus;
@@ -985,13 +987,15 @@ template <int P> struct NRSolver {
         return b;
     }
 
-    // method: 0 Newton-Raphson, 1 linear (math_solver.hpp:43-64: a model whose load_gens are all const_y
-    // always runs linear). Linear: the initial-guess solve is the answer (linear_pf_solver.hpp), and the
-    // results treat every load as const_y
+    // method: 0 Newton-Raphson, 1 linear, 2 iterative current, 3 linear current (math_solver.hpp:43-64,
+    // 142-156: a model whose load_gens are all const_y always runs linear). Linear: the initial-guess
+    // solve is the answer (linear_pf_solver.hpp), and the results treat every load as const_y
     MathOutput<P> run(MathInput<P> const& in, double err_tol, Idx max_iter, int method = 0) {
         bool const all_const_y =
             std::all_of(mm.load_type.begin(), mm.load_type.end(), [](int8_t t) { return t == 1; });
         bool const linear = method == 1 || all_const_y;
+        if (!linear && method == 3) return run_iterative_current(in, std::numeric_limits<double>::infinity(), 1);
+        if (!linear && method == 2) return run_iterative_current(in, err_tol, max_iter);
         Idx const n = mm.n_bus;
         Idx const nnz_lu = ys.row_indptr_lu.back();
         std::vector<Blk<NB>> jac(nnz_lu);
@@ -1161,6 +1165,97 @@ template <int P> struct NRSolver {
                     out.u[b].v[ph] = un;
                 }
                 max_dev = std::max(dev, max_dev);
+            }
+        }
+        out.iterations = num_iter;
+        calculate_result(in, out);
+        return out;
+    }
+
+    // iterative_current_pf_solver.hpp: (Y + y_ref) factorised once; each iteration solves for the
+    // injected currents of the loads (by type) and sources at the last voltages; flat start at the
+    // averaged source voltage with the buses' phase shifts. The complex system is solved in its real
+    // form [[G, -B], [B, G]], which differs from the reference's complex LU by rounding only.
+    MathOutput<P> run_iterative_current(MathInput<P> const& in, double err_tol, Idx max_iter) {
+        Idx const n = mm.n_bus;
+        Idx const nnz_lu = ys.row_indptr_lu.back();
+        std::vector<Blk<NB>> mat(nnz_lu);
+        std::vector<Vec<NB>> rhs(n), x(n);
+        SparseLU<NB> lu(ys.row_indptr_lu, ys.col_indices_lu, ys.diag_lu);
+        for (Idx k = 0; k < nnz_lu; ++k) {
+            mat[k] = Blk<NB>{};
+            Idx const ky = ys.map_lu_y_bus[k];
+            if (ky == -1) continue;
+            auto const& y = ydata[ky];
+            for (int r = 0; r < P; ++r)
+                for (int c = 0; c < P; ++c) {
+                    at<NB>(mat[k], r, c) = y(r, c).real();
+                    at<NB>(mat[k], r, c + P) = -y(r, c).imag();
+                    at<NB>(mat[k], r + P, c) = y(r, c).imag();
+                    at<NB>(mat[k], r + P, c + P) = y(r, c).real();
+                }
+        }
+        for (Idx b = 0; b < n; ++b)
+            for (Idx s : sources_per_bus[b]) {
+                Ten<P> const ysrc = y_ref<P>(mm.source_y1[s], mm.source_y0[s]);
+                auto& blk = mat[ys.diag_lu[b]];
+                for (int r = 0; r < P; ++r)
+                    for (int c = 0; c < P; ++c) {
+                        at<NB>(blk, r, c) += ysrc(r, c).real();
+                        at<NB>(blk, r, c + P) -= ysrc(r, c).imag();
+                        at<NB>(blk, r + P, c) += ysrc(r, c).imag();
+                        at<NB>(blk, r + P, c + P) += ysrc(r, c).real();
+                    }
+            }
+        lu.factorize(mat);
+        MathOutput<P> out;
+        out.u.assign(n, Val<P>{});
+        // make_flat_start
+        cd sum_u_ref = 0.0;
+        for (Idx s = 0; s < Idx(mm.source_bus.size()); ++s)
+            sum_u_ref += in.u_source[s] * std::exp(cd(0.0, -mm.phase_shift[mm.source_bus[s]]));
+        cd const u_ref = sum_u_ref / double(mm.source_bus.size());
+        for (Idx b = 0; b < n; ++b) out.u[b] = balanced<P>(u_ref * std::exp(cd(0.0, mm.phase_shift[b])));
+        double max_dev = std::numeric_limits<double>::infinity();
+        Idx num_iter = 0;
+        while (max_dev > err_tol || num_iter == 0) {
+            if (num_iter++ == max_iter)
+                throw IterationDiverge("Iteration failed to converge after " + std::to_string(max_iter) +
+                                       " iterations! Max deviation: " + fmt_double(max_dev) +
+                                       ", error tolerance: " + fmt_double(err_tol) + ".\n");
+            for (Idx b = 0; b < n; ++b) {
+                Val<P> i_inj{};
+                for (Idx l : loads_per_bus[b])
+                    for (int ph = 0; ph < P; ++ph) {
+                        cd const sv = in.s_inj[l].v[ph];
+                        cd const u = out.u[b].v[ph];
+                        switch (mm.load_type[l]) {
+                        case 0: i_inj.v[ph] += std::conj(sv / u); break;
+                        case 1: i_inj.v[ph] += std::conj(sv) * u; break;
+                        case 2: i_inj.v[ph] += std::conj(sv * std::abs(u) / u); break;
+                        default: throw OtherError("unknown load type");
+                        }
+                    }
+                for (Idx s : sources_per_bus[b]) {
+                    Val<P> const iv =
+                        matvec<P>(y_ref<P>(mm.source_y1[s], mm.source_y0[s]), balanced<P>(in.u_source[s]));
+                    for (int ph = 0; ph < P; ++ph) i_inj.v[ph] += iv.v[ph];
+                }
+                for (int ph = 0; ph < P; ++ph) {
+                    rhs[b][ph] = i_inj.v[ph].real();
+                    rhs[b][ph + P] = i_inj.v[ph].imag();
+                }
+            }
+            lu.solve(mat, rhs, x);
+            max_dev = 0.0;
+            for (Idx b = 0; b < n; ++b) {
+                double dev = 0.0;
+                for (int ph = 0; ph < P; ++ph) {
+                    cd const un(x[b][ph], x[b][ph + P]);
+                    dev = std::max(dev, std::abs(un - out.u[b].v[ph]));
+                    out.u[b].v[ph] = un;
+                }
+                max_dev = std::max(max_dev, dev);
             }
         }
         out.iterations = num_iter;
@@ -1627,6 +1722,7 @@ TopoCache build_topo_cache(Model const& m, State const& st) {
 template <int P> MathModel<P> make_math_model(Model const& m, State const& st, TopoCache const& tc, Idx g) {
     MathModel<P> mm;
     mm.n_bus = Idx(tc.topo.bus_nodes[g].size());
+    for (Idx node : tc.topo.bus_nodes[g]) mm.phase_shift.push_back(tc.topo.phase[node]);
     mm.branch_bus = tc.mm_branch_bus[g];
     mm.fill_in = tc.topo.fill_in[g];
     Idx const nl = Idx(m.line.size());

@@ -87,7 +87,8 @@ int or_power_flow(OrConstDataset const* input, double system_frequency, OrConstD
                   OrMutableDataset const* output, int64_t* iterations, int32_t* status, char* messages,
                   int64_t msg_stride);
 
-/* or_power_flow with a calculation method: 0 Newton-Raphson, 1 linear (linear_pf_solver.hpp). */
+/* or_power_flow with a calculation method: 0 Newton-Raphson, 1 linear (linear_pf_solver.hpp),
+ * 2 iterative current, 3 linear current (iterative_current_pf_solver.hpp). */
 int or_power_flow_method(OrConstDataset const* input, double system_frequency, OrConstDataset const* update,
                          int64_t n_scenarios, int symmetric, double err_tol, int64_t max_iter, int64_t threading,
                          int method, OrMutableDataset const* output, int64_t* iterations, int32_t* status,

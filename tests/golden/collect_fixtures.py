@@ -1,9 +1,9 @@
-"""Copy the reference's Newton-Raphson and linear power-flow validation cases into tests/golden/power_flow/.
+"""Copy the reference's power-flow validation cases into tests/golden/power_flow/.
 
 Run once in the build container (the reference tree is not present on the GPU box):
     python tests/golden/collect_fixtures.py /root/reference
-A case is taken when its params.json lists newton_raphson or linear and every component it uses is one
-this build supports. The data files (input, update_batch, *_output*.json, params.json) are copied unchanged
+A case is taken when its params.json lists a method this build runs (newton_raphson, linear,
+iterative_current, linear_current) and every component it uses is one this build supports. The data files (input, update_batch, *_output*.json, params.json) are copied unchanged
 together with their .license sidecars (MPL-2.0).
 """
 
@@ -34,7 +34,7 @@ def main(ref_root):
         p = json.load(open(params))
         methods = p.get("calculation_method")
         methods = methods if isinstance(methods, list) else [methods]
-        if not ({"newton_raphson", "linear"} & set(methods)) or "raises" in p:
+        if not ({"newton_raphson", "linear", "iterative_current", "linear_current"} & set(methods)) or "raises" in p:
             continue
         comps = set()
         for f in FILES[:-1]:

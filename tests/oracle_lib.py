@@ -64,7 +64,7 @@ def _ptr(a):
     return None if a is None else a.ctypes.data
 
 
-METHODS = {"newton_raphson": 0, "linear": 1}
+METHODS = {"newton_raphson": 0, "linear": 1, "iterative_current": 2, "linear_current": 3}
 
 
 def power_flow(input_data: dict, update: dict | None = None, n_scenarios: int = 1, symmetric: bool = True,

@@ -321,7 +321,6 @@ def test_linear_method_closed_form(load_type):
 # reference validation cases
 # ------------------------------------------------------------------------------------------------
 CASES = list_cases()
-LINEAR_CASES = list_cases("linear")
 
 
 @pytest.mark.parametrize("name,sym,batch", CASES, ids=[f"{c[0]}-{'sym' if c[1] else 'asym'}"
@@ -334,14 +333,31 @@ def test_validation_case(name, sym, batch):
         compare_result(out, exp, params["rtol"], params["atol"], s)
 
 
-@pytest.mark.parametrize("name,sym,batch", LINEAR_CASES, ids=[f"{c[0]}-{'sym' if c[1] else 'asym'}"
-                                                              f"{'-batch' if c[2] else ''}" for c in LINEAR_CASES])
-def test_validation_case_linear(name, sym, batch):
+METHOD_CASES = [(m, *c) for m in ("linear", "iterative_current", "linear_current") for c in list_cases(m)]
+
+
+@pytest.mark.parametrize("method,name,sym,batch", METHOD_CASES,
+                         ids=[f"{c[0]}-{c[1]}-{'sym' if c[2] else 'asym'}{'-batch' if c[3] else ''}"
+                              for c in METHOD_CASES])
+def test_validation_case_other_methods(method, name, sym, batch):
     params, input_data, update, n_scenarios, expected = load_case(name, sym, batch)
-    out, iters, status, msgs = ol.power_flow(input_data, update, n_scenarios, symmetric=sym, method="linear")
+    out, iters, status, msgs = ol.power_flow(input_data, update, n_scenarios, symmetric=sym, method=method)
     assert (status == 0).all(), msgs
     for s, exp in enumerate(expected):
         compare_result(out, exp, params["rtol"], params["atol"], s)
+
+
+def test_iterative_current_reaches_newton_raphson_fixed_point():
+    """Iterative current and Newton-Raphson solve the same equations: at a tight tolerance they agree
+    to it, and linear current (one iteration) does not."""
+    grid = _two_node_grid(0)
+    nr = ol.power_flow(grid, err_tol=1e-12)[0]
+    ic, it, st, _ = ol.power_flow(grid, err_tol=1e-12, max_iter=100, method="iterative_current")
+    assert st[0] == 0 and it[0] > 2
+    np.testing.assert_allclose(ic["node"]["u_pu"], nr["node"]["u_pu"], atol=1e-11)
+    np.testing.assert_allclose(ic["sym_load"]["p"], nr["sym_load"]["p"], rtol=1e-10)
+    lc, it1, _, _ = ol.power_flow(grid, method="linear_current")
+    assert it1[0] == 1 and abs(lc["node"]["u_pu"][0, 1] - nr["node"]["u_pu"][0, 1]) > 1e-9
 
 
 def test_batch_threading_identical():
