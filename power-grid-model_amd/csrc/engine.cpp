@@ -15,6 +15,7 @@
 #include <map>
 #include <numeric>
 #include <cstdlib>
+#include <limits>
 #include <sstream>
 #include <thread>
 
@@ -300,6 +301,15 @@ void upload_group(Batch& b, Group& g) {
     d.task_mm = put(A, p.task_mm);
     d.level_task = put(A, p.level_task);
     d.keep = put(A, p.keep);
+    d.phase = put(A, p.phase);
+    d.bus_mm = put(A, p.bus_mm);
+    d.mm_bus_begin = put(A, p.mm_bus_begin);
+    {
+        std::vector<int> sb(p.src_comp.size());
+        for (Idx bus = 0; bus < p.n; ++bus)
+            for (int q = p.src_ptr[bus]; q < p.src_ptr[bus + 1]; ++q) sb[q] = int(bus);
+        d.src_bus = put(A, sb);
+    }
     d.load_ptr = put(A, p.load_ptr);
     d.load_type = put(A, p.load_type);
     d.src_ptr = put(A, p.src_ptr);
@@ -728,7 +738,8 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     if (opt.calculation_type != PGM_power_flow)
         throw PowerGridError("The calculation type is not supported by this build: only power flow is implemented\n");
     if (opt.calculation_method != PGM_default_method && opt.calculation_method != PGM_newton_raphson &&
-        opt.calculation_method != PGM_linear)
+        opt.calculation_method != PGM_linear && opt.calculation_method != PGM_iterative_current &&
+        opt.calculation_method != PGM_linear_current)
         throw PowerGridError("The calculation method is invalid for this calculation!");
     if (update && update->next) throw PowerGridError("Cartesian product batches are not supported by this build\n");
     ensure_device();
@@ -740,8 +751,15 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     b.P = b.symmetric ? 1 : 3;
     b.err_tol = opt.err_tol;
     b.max_iter = opt.max_iter;
-    b.fp32 = opt.factor_precision == 1;
     b.linear = opt.calculation_method == PGM_linear;
+    b.current = opt.calculation_method == PGM_iterative_current || opt.calculation_method == PGM_linear_current;
+    if (opt.calculation_method == PGM_linear_current) {
+        // one iteration of iterative current (math_solver.hpp:151-156)
+        b.err_tol = std::numeric_limits<double>::infinity();
+        b.max_iter = 1;
+    }
+    // a single linear solve has no correction step to absorb FP32 factor error: FP64 factors
+    b.fp32 = opt.factor_precision == 1 && !b.linear;
     b.is_batch = update != nullptr;
     b.n_scenarios = update ? update->batch_size : 1;
     PGM_HIP_CHECK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
@@ -1107,28 +1125,34 @@ void Batch::run() {
     PGM_HIP_CHECK(hipEventRecord(ev_start, st));
     launch_init_state(pairs, st);
     fork();
-    union_sweeps(MODE_INIT);
-    for (int l = 0; l < L; ++l) launch_forward(d_groups, d_items[l], n_items[l], P, fp32, MODE_INIT, st);
-    for (int l = L - 1; l >= 0; --l) launch_backward(d_groups, d_items[l], n_items[l], P, fp32, MODE_INIT, st);
+    // the linear system: Newton-Raphson's initial guess, and the answer of the linear method and of
+    // the all-const_y math models; iterative current starts flat instead
+    if (!current || d_pair_linear) {
+        union_sweeps(MODE_INIT);
+        for (int l = 0; l < L; ++l) launch_forward(d_groups, d_items[l], n_items[l], P, fp32, MODE_INIT, st);
+        for (int l = L - 1; l >= 0; --l) launch_backward(d_groups, d_items[l], n_items[l], P, fp32, MODE_INIT, st);
+    }
     double active_frac = 1.0;
-    if (d_pair_linear) {
-        // the linear solve is the answer of these pairs: no Newton-Raphson iteration
+    if (d_pair_linear || current) {
         join();
-        launch_retire_linear(pairs, d_pair_linear, st);
+        // the linear solve is the answer of these pairs: no further iteration
+        if (d_pair_linear) launch_retire_linear(pairs, d_pair_linear, st);
+        if (current) launch_ic_start(P, d_groups, d_bus_off, int(groups.size()), bus_total, st);
         fork();
     }
     for (int it = 1; it <= (linear ? 0 : int(max_iter)); ++it) {
-        union_sweeps(MODE_NR);
+        int const mode = current ? (it == 1 ? MODE_ICF : MODE_RESID) : MODE_NR;
+        union_sweeps(mode);
         for (int l = 0; l < L; ++l) {
             fwd_events.push_back(event_from_pool(fwd_events.size()));
             PGM_HIP_CHECK(hipEventRecord(fwd_events.back(), st));
-            launch_forward(d_groups, d_items[l], n_items[l], P, fp32, MODE_NR, st);
+            launch_forward(d_groups, d_items[l], n_items[l], P, fp32, mode, st);
             fwd_events.push_back(event_from_pool(fwd_events.size()));
             PGM_HIP_CHECK(hipEventRecord(fwd_events.back(), st));
-            fwd_bytes += level_bytes_full[l] * active_frac;
+            if (mode == MODE_NR) fwd_bytes += level_bytes_full[l] * active_frac;
             ++fwd_launches;
         }
-        for (int l = L - 1; l >= 0; --l) launch_backward(d_groups, d_items[l], n_items[l], P, fp32, MODE_NR, st);
+        for (int l = L - 1; l >= 0; --l) launch_backward(d_groups, d_items[l], n_items[l], P, fp32, mode, st);
         join();
         *h_active = 0;
         PGM_HIP_CHECK(hipMemcpyAsync(d_active, h_active, sizeof(int), hipMemcpyHostToDevice, st));

@@ -48,7 +48,8 @@ struct Batch {
     double err_tol = 1e-8;
     Idx max_iter = 20;
     bool fp32 = false;
-    bool linear = false;  // calculation_method = linear
+    bool linear = false;   // calculation_method = linear
+    bool current = false;  // iterative current or linear current
     bool is_batch = false;
     Idx n_scenarios = 1;
     hipStream_t stream = nullptr;

@@ -485,6 +485,142 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                     rhs[r + P] += acc.im;
                 }
             }
+        } else if constexpr (MODE == MODE_ICF || MODE == MODE_RESID) {
+            // iterative current in residual form: r_p = I_p(u) - sum_j Y_pj u_j - y_src u_p, with I_p the
+            // injected currents of the loads (by type) and sources at the present voltages
+            // (iterative_current_pf_solver.hpp prepare_matrix_and_rhs); ICF also assembles Y + y_src
+            cplx up[P], r[P];
+            load_u<P>(g, p, s, up);
+#pragma unroll
+            for (int ph = 0; ph < P; ++ph) r[ph] = {0.0, 0.0};
+            if (dead_p) {
+                if constexpr (MODE == MODE_ICF) {
+                    for (int k = dg + 1; k < re; ++k) {
+                        zero_u(k);
+                        zero_l(g.transpose[k]);
+                    }
+#pragma unroll
+                    for (int q = 0; q < NB2; ++q) D[q] = (q % (NB + 1) == 0) ? 1.0 : 0.0;
+                }
+            } else {
+                for (int k = rb; k < re; ++k) {
+                    int const j = g.col[k];
+                    if (g.fill[k]) {
+                        if constexpr (MODE == MODE_ICF) {
+                            if (j > p) {
+                                zero_u(k);
+                                zero_l(g.transpose[k]);
+                            }
+                        }
+                        continue;
+                    }
+                    double yk[PP * 2];
+                    load_y<P, UNI>(g, k, s, yk);
+                    cplx uj[P];
+                    if (j == p) {
+#pragma unroll
+                        for (int ph = 0; ph < P; ++ph) uj[ph] = up[ph];
+                    } else {
+                        load_u<P>(g, j, s, uj);
+                    }
+#pragma unroll
+                    for (int rr = 0; rr < P; ++rr)
+#pragma unroll
+                        for (int c = 0; c < P; ++c)
+                            r[rr] = csub(r[rr], cmul({yk[(rr * P + c) * 2], yk[(rr * P + c) * 2 + 1]}, uj[c]));
+                    if constexpr (MODE == MODE_ICF) {
+                        FT a[NB2];
+                        linear_block<P, FT>(yk, a);
+                        if (j == p) {
+#pragma unroll
+                            for (int q = 0; q < NB2; ++q) D[q] = double(a[q]);
+                        } else if (j > p) {
+                            put_u(k, a);
+                            int const e = g.transpose[k];
+                            if (g.fill[e]) {
+                                zero_l(e);
+                            } else {
+                                double ye[PP * 2];
+                                load_y<P, UNI>(g, e, s, ye);
+                                linear_block<P, FT>(ye, a);
+                                put_l(e, a);
+                            }
+                        }
+                    }
+                }
+                for (int l = g.load_ptr[p]; l < g.load_ptr[p + 1]; ++l) {
+                    int const type = g.load_type[l];
+#pragma unroll
+                    for (int ph = 0; ph < P; ++ph) {
+                        cplx const sv = {g.s_inj[((size_t(l) * P + ph) * 2) * sp + s],
+                                         g.s_inj[((size_t(l) * P + ph) * 2 + 1) * sp + s]};
+                        cplx const u = up[ph];
+                        cplx i;
+                        if (type == 0) {
+                            i = cconj(cdiv(sv, u));
+                        } else if (type == 1) {
+                            i = cmul(cconj(sv), u);
+                        } else {
+                            double const a = sqrt(u.re * u.re + u.im * u.im);
+                            i = cconj(cdiv({sv.re * a, sv.im * a}, u));
+                        }
+                        r[ph] = cadd(r[ph], i);
+                    }
+                }
+                for (int q = g.src_ptr[p]; q < g.src_ptr[p + 1]; ++q) {
+                    cplx const y1 = {g.y_src[(size_t(q) * 4) * sp + s], g.y_src[(size_t(q) * 4 + 1) * sp + s]};
+                    cplx const y0 = {g.y_src[(size_t(q) * 4 + 2) * sp + s], g.y_src[(size_t(q) * 4 + 3) * sp + s]};
+                    cplx const ur = {g.u_src[(size_t(q) * 2) * sp + s], g.u_src[(size_t(q) * 2 + 1) * sp + s]};
+                    double ys[PP * 2];
+                    source_y<P>(y1, y0, ys);
+                    cplx ub[P];
+                    balanced<P>(ur, ub);
+#pragma unroll
+                    for (int rr = 0; rr < P; ++rr)
+#pragma unroll
+                        for (int c = 0; c < P; ++c) {
+                            double const gg = ys[(rr * P + c) * 2], bb = ys[(rr * P + c) * 2 + 1];
+                            r[rr] = cadd(r[rr], cmul({gg, bb}, csub(ub[c], up[c])));
+                            if constexpr (MODE == MODE_ICF) {
+                                D[(c + P) * NB + rr] -= bb;
+                                D[c * NB + rr] += gg;
+                                D[(c + P) * NB + rr + P] += gg;
+                                D[c * NB + rr + P] += bb;
+                            }
+                        }
+                }
+            }
+#pragma unroll
+            for (int ph = 0; ph < P; ++ph) {
+                rhs[ph] = r[ph].re;
+                rhs[ph + P] = r[ph].im;
+            }
+            if constexpr (MODE == MODE_RESID) {
+                // stored factors: y_p = L_pp^-1 P (r_p - sum_k L_pk y_k)
+                for (int k = rb; k < dg; ++k) {
+                    int const j = g.col[k];
+                    FT L[NB2];
+                    lu.load(k, L);
+                    double yj[NB];
+#pragma unroll
+                    for (int rr = 0; rr < NB; ++rr) yj[rr] = g.y[(size_t(j) * NB + rr) * sp + s];
+#pragma unroll
+                    for (int rr = 0; rr < NB; ++rr) {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int c = 0; c < NB; ++c) acc += double(L[c * NB + rr]) * yj[c];
+                        rhs[rr] -= acc;
+                    }
+                }
+                FT Dst[NB2];
+                lu.load(dg, Dst);
+                uint32_t const code = g.perm[size_t(p) * sp + s];
+                apply_p_vec<NB>(rhs, code);
+                lower_vec<NB, FT>(Dst, rhs);
+#pragma unroll
+                for (int rr = 0; rr < NB; ++rr) g.y[(size_t(p) * NB + rr) * sp + s] = rhs[rr];
+                continue;
+            }
         } else if (dead_p) {
             for (int k = dg + 1; k < re; ++k) {
                 zero_u(k);
@@ -742,6 +878,18 @@ __global__ __launch_bounds__(kWave) void backward_kernel(DevGroup const* __restr
                 g.xv[b * sp + s] = atan2(ui, ur);
                 g.xv[(b + 1) * sp + s] = sqrt(ur * ur + ui * ui);
             }
+        } else if constexpr (MODE == MODE_ICF || MODE == MODE_RESID) {
+            // u += x; the deviation is |x| (iterative_current_pf_solver.hpp iterate_unknown)
+            double dbus = 0.0;
+#pragma unroll
+            for (int ph = 0; ph < P; ++ph) {
+                size_t const b = (size_t(p) * P + ph) * 2;
+                g.u[b * sp + s] += v[ph];
+                g.u[(b + 1) * sp + s] += v[ph + P];
+                double const d = sqrt(v[ph] * v[ph] + v[ph + P] * v[ph + P]);
+                dbus = d > dbus ? d : dbus;
+            }
+            dev = dbus > dev ? dbus : dev;
         } else if constexpr (MODE == MODE_NR) {
             double dbus = 0.0;
 #pragma unroll
@@ -764,7 +912,7 @@ __global__ __launch_bounds__(kWave) void backward_kernel(DevGroup const* __restr
             dev = dbus > dev ? dbus : dev;
         }
     }
-    if constexpr (MODE == MODE_NR) {
+    if constexpr (MODE == MODE_NR || MODE == MODE_ICF || MODE == MODE_RESID) {
         atomicMax(&g.devmax[size_t(mm) * sp + s], static_cast<unsigned long long>(__double_as_longlong(dev)));
     }
 }
@@ -817,6 +965,39 @@ __device__ __forceinline__ int find_group(long long const* __restrict__ offsets,
             hi = mid;
     }
     return lo;
+}
+
+template <int P>
+__global__ void ic_start_kernel(DevGroup const* __restrict__ groups, long long const* __restrict__ offsets,
+                                int n_groups) {
+    long long const gidx = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gidx >= offsets[n_groups]) return;
+    int const gr = find_group(offsets, n_groups, gidx);
+    DevGroup const& g = groups[gr];
+    size_t const sp = g.S_pad;
+    size_t const idx = size_t(gidx - offsets[gr]);
+    int const b = int(idx / sp), s = int(idx % sp);
+    if (s >= g.S) return;
+    int const mm = g.bus_mm[b];
+    if (g.done[size_t(mm) * sp + s] != ACTIVE) return;
+    int const q0 = g.src_ptr[g.mm_bus_begin[mm]], q1 = g.src_ptr[g.mm_bus_begin[mm + 1]];
+    cplx sum = {0.0, 0.0};
+    for (int q = q0; q < q1; ++q) {
+        cplx const ur = {g.u_src[(size_t(q) * 2) * sp + s], g.u_src[(size_t(q) * 2 + 1) * sp + s]};
+        double sn, cs;
+        sincos(-g.phase[g.src_bus[q]], &sn, &cs);
+        sum = cadd(sum, cmul(ur, {cs, sn}));
+    }
+    double const inv = 1.0 / double(q1 - q0);
+    double sn, cs;
+    sincos(g.phase[b], &sn, &cs);
+    cplx ub[P];
+    balanced<P>(cmul({sum.re * inv, sum.im * inv}, {cs, sn}), ub);
+#pragma unroll
+    for (int ph = 0; ph < P; ++ph) {
+        g.u[((size_t(b) * P + ph) * 2) * sp + s] = ub[ph].re;
+        g.u[((size_t(b) * P + ph) * 2 + 1) * sp + s] = ub[ph].im;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -1125,6 +1306,10 @@ void launch_fwd(DevGroup const* groups, WorkItem const* items, int n_items, int 
         hipLaunchKernelGGL((forward_kernel<P, FT, MODE_INIT, KIND>), grid, dim3(kWave), 0, st, groups, items);
     else if (mode == MODE_NR)
         hipLaunchKernelGGL((forward_kernel<P, FT, MODE_NR, KIND>), grid, dim3(kWave), 0, st, groups, items);
+    else if (mode == MODE_ICF)
+        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_ICF, KIND>), grid, dim3(kWave), 0, st, groups, items);
+    else if (mode == MODE_RESID)
+        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_RESID, KIND>), grid, dim3(kWave), 0, st, groups, items);
     else if constexpr (KIND == KIND_PLAIN)
         hipLaunchKernelGGL((forward_kernel<P, FT, MODE_RAW, KIND_PLAIN>), grid, dim3(kWave), 0, st, groups, items);
 }
@@ -1135,6 +1320,8 @@ void launch_bwd(DevGroup const* groups, WorkItem const* items, int n_items, int 
         hipLaunchKernelGGL((backward_kernel<P, FT, MODE_INIT, KIND>), grid, dim3(kWave), 0, st, groups, items);
     else if (mode == MODE_NR)
         hipLaunchKernelGGL((backward_kernel<P, FT, MODE_NR, KIND>), grid, dim3(kWave), 0, st, groups, items);
+    else if (mode == MODE_ICF || mode == MODE_RESID)
+        hipLaunchKernelGGL((backward_kernel<P, FT, MODE_RESID, KIND>), grid, dim3(kWave), 0, st, groups, items);
     else if constexpr (KIND == KIND_PLAIN)
         hipLaunchKernelGGL((backward_kernel<P, FT, MODE_RAW, KIND_PLAIN>), grid, dim3(kWave), 0, st, groups, items);
 }
@@ -1210,6 +1397,15 @@ void launch_converge(PairState const& ps, int it, int max_iter, double err_tol, 
     if (ps.n == 0) return;
     hipLaunchKernelGGL(converge_kernel, dim3(unsigned((ps.n + 255) / 256)), dim3(256), 0, st, ps, it, max_iter,
                        err_tol, active);
+}
+void launch_ic_start(int P, DevGroup const* groups, long long const* bus_offsets, int n_groups, long long total,
+                     hipStream_t st) {
+    if (total == 0) return;
+    dim3 const grid(unsigned((total + 255) / 256));
+    if (P == 1)
+        hipLaunchKernelGGL(ic_start_kernel<1>, grid, dim3(256), 0, st, groups, bus_offsets, n_groups);
+    else
+        hipLaunchKernelGGL(ic_start_kernel<3>, grid, dim3(256), 0, st, groups, bus_offsets, n_groups);
 }
 void launch_retire_linear(PairState const& ps, uint8_t const* pair_linear, hipStream_t st) {
     if (ps.n == 0) return;

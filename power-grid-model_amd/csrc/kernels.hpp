@@ -13,7 +13,10 @@
 namespace pgm_amd {
 
 constexpr int kWaveLanes = 64;  // lanes of a gfx950 wavefront: one scenario (or task) each
-enum SweepMode : int { MODE_INIT = 0, MODE_NR = 1, MODE_RAW = 2 };
+// INIT: linear system (NR initial guess, linear method); NR: Newton-Raphson step; RAW: block LU test
+// hook; ICF: iterative current, factorise Y + y_src and solve the current residual; RESID: iterative
+// current with the stored factors (u += x after both)
+enum SweepMode : int { MODE_INIT = 0, MODE_NR = 1, MODE_RAW = 2, MODE_ICF = 3, MODE_RESID = 4 };
 enum DoneState : int { ACTIVE = 0, CONVERGED = 1, FAILED = 2 };
 enum ErrCode : int { ERR_NONE = 0, ERR_SPARSE = 1, ERR_DIVERGE = 2 };
 
@@ -44,6 +47,12 @@ struct DevGroup {
     int const* task_mm = nullptr;
     int const* level_task = nullptr;  // tasks listed level by level
     int8_t const* keep = nullptr;     // per pivot: one U entry, no off-diagonal Schur target
+    // iterative current: phase shift and math model of every bus, first bus of every math model,
+    // bus of every source
+    double const* phase = nullptr;
+    int const* bus_mm = nullptr;
+    int const* mm_bus_begin = nullptr;
+    int const* src_bus = nullptr;
     int const* load_ptr = nullptr;
     int8_t const* load_type = nullptr;
     int const* src_ptr = nullptr;
@@ -97,6 +106,10 @@ void launch_backward(DevGroup const* groups, WorkItem const* items, int n_items,
 // fold the deviation of iteration `it`, mark converged / diverged, count the still active pairs
 void launch_converge(PairState const& ps, int it, int max_iter, double err_tol, int* active, hipStream_t st);
 void launch_init_state(PairState const& ps, hipStream_t st);
+// iterative current flat start: every bus of an active pair at the averaged source voltage of its math
+// model, rotated by the bus's phase shift (iterative_current_pf_solver.hpp make_flat_start)
+void launch_ic_start(int P, DevGroup const* groups, long long const* bus_offsets, int n_groups, long long total,
+                     hipStream_t st);
 // pairs flagged linear that are still active after the linear solve: converged after one solve
 void launch_retire_linear(PairState const& ps, uint8_t const* pair_linear, hipStream_t st);
 

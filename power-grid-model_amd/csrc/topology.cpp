@@ -172,17 +172,23 @@ Topology build_topology(Model const& m, ComponentState const& st) {
         }
     std::partial_sum(deg.begin(), deg.end(), deg.begin());
     std::vector<Idx> target(deg[nn]);
+    // phase shift along an edge: theta_to = theta_from - shift for the branch's from->to direction
+    std::vector<double> edge_shift(deg[nn]);
+    auto shift = [&](Idx b) { return b < nl ? 0.0 : double(m.tstat[b - nl].clock) * kDeg30; };
     {
         std::vector<Idx> fillp(deg.begin(), deg.end() - 1);
         for (Idx b = 0; b < nb; ++b)
             if (bfs(b) && bts(b) && bfrom(b) != bto(b)) {
+                edge_shift[fillp[bfrom(b)]] = -shift(b);
                 target[fillp[bfrom(b)]++] = bto(b);
+                edge_shift[fillp[bto(b)]] = shift(b);
                 target[fillp[bto(b)]++] = bfrom(b);
             }
     }
     Topology t;
     t.node_group.assign(nn, -1);
     t.node_pos.assign(nn, -1);
+    t.node_phase.assign(nn, 0.0);
     std::vector<Idx> parent(nn);
     std::iota(parent.begin(), parent.end(), Idx{0});
     std::vector<char> state(nn, 0);      // 0 new, 1 on the DFS stack, 2 finished
@@ -207,8 +213,10 @@ Topology build_topology(Model const& m, ComponentState const& st) {
                 stack.pop_back();
                 continue;
             }
-            Idx const v = target[cursor[u]++];
+            Idx const e = cursor[u]++;
+            Idx const v = target[e];
             if (state[v] == 0) {
+                t.node_phase[v] = t.node_phase[u] + edge_shift[e];
                 parent[v] = u;
                 state[v] = 1;
                 t.node_group[v] = grp;
@@ -449,6 +457,11 @@ Plan build_plan(Model const& m, ComponentState const& st, Topology const& topo, 
     p.n = p.mm_bus_begin.back();
     Idx const n = p.n;
     p.bus_mm.resize(n);
+    p.phase.resize(n);
+    for (Idx k = 0; k < p.n_mm; ++k) {
+        auto const& mm = topo.models[models[k]];
+        for (Idx i = 0; i < mm.n_bus; ++i) p.phase[p.mm_bus_begin[k] + i] = topo.node_phase[mm.bus_node[i]];
+    }
     for (Idx k = 0; k < p.n_mm; ++k)
         for (int b = p.mm_bus_begin[k]; b < p.mm_bus_begin[k + 1]; ++b) p.bus_mm[b] = int(k);
 

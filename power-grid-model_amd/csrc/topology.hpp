@@ -26,6 +26,7 @@ struct MathModel {
 
 struct Topology {
     std::vector<Idx> node_group, node_pos;
+    std::vector<double> node_phase;  // phase shift against the model's source (transformer clocks)
     std::vector<MathModel> models;
     std::vector<Idx> branch_group, branch_pos;
     std::vector<Idx> lg_group, lg_pos, src_group, src_pos, sh_group, sh_pos;
@@ -47,6 +48,7 @@ struct Plan {
     std::vector<int> row_ptr, col, diag, transpose;
     std::vector<int8_t> fill;  // entry is a fill-in
     std::vector<int> bus_mm;
+    std::vector<double> phase;  // phase shift of every bus
     // Y values per LU entry (P*P complex, row-major), zero for fill-ins
     std::vector<cd> ydata;
     // Schur updates in pull form, per pivot: (target, l, u) entry triples, k ascending per target

@@ -91,15 +91,16 @@ def test_validation_case_gpu(name, sym, batch):
         compare_result(out, exp, params["rtol"], params["atol"], s)
 
 
-LINEAR_CASES = list_cases("linear")
+METHOD_CASES = [(m, *c) for m in ("linear", "iterative_current", "linear_current") for c in list_cases(m)]
 
 
-@pytest.mark.parametrize("name,sym,batch", LINEAR_CASES, ids=[f"{c[0]}-{'sym' if c[1] else 'asym'}"
-                                                              f"{'-batch' if c[2] else ''}" for c in LINEAR_CASES])
-def test_validation_case_gpu_linear(name, sym, batch):
+@pytest.mark.parametrize("method,name,sym,batch", METHOD_CASES,
+                         ids=[f"{c[0]}-{c[1]}-{'sym' if c[2] else 'asym'}{'-batch' if c[3] else ''}"
+                              for c in METHOD_CASES])
+def test_validation_case_gpu_other_methods(method, name, sym, batch):
     params, input_data, update, n_scenarios, expected = load_case(name, sym, batch)
     model = PowerGridModel(input_data)
-    out = model.calculate_power_flow(symmetric=sym, update_data=update, calculation_method="linear")
+    out = model.calculate_power_flow(symmetric=sym, update_data=update, calculation_method=method)
     if not batch:
         out = {k: v.reshape(1, *v.shape) for k, v in out.items()}
     for s, exp in enumerate(expected):
@@ -317,6 +318,31 @@ def test_workload_linear_method(name, make, scen, precision):
     else:
         # one FP32 solve, no Newton correction: FP32 accuracy of the factorisation
         np.testing.assert_allclose(out["node"]["u_pu"], ref["node"]["u_pu"], atol=1e-5, rtol=0)
+
+
+@pytest.mark.parametrize("name,make,scen", WORKLOADS[:3], ids=[w[0] for w in WORKLOADS[:3]])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_workload_iterative_current(name, make, scen, precision):
+    """Iterative current on the GPU runs in residual form, u += F^-1 (I(u) - (Y + y_src) u), with the
+    factors F of Y + y_src computed once: the reference's iteration in exact arithmetic, and the exact
+    fixed point whatever the factor precision."""
+    case = make()
+    upd = _subset(case, scen[:64])
+    model = PowerGridModel(case.input)
+    batch = DeviceBatch(model, upd, symmetric=case.symmetric, factor_precision=precision, max_iterations=100,
+                        calculation_method="iterative_current")
+    batch.run()
+    out = batch.fetch()
+    oupd, k = ol.uniform_update(upd, case.input)
+    ref, it_ref, st_ref, _ = ol.power_flow(case.input, oupd, k, symmetric=case.symmetric, max_iter=100,
+                                           method="iterative_current")
+    assert (st_ref == 0).all()
+    it = batch.iterations()
+    if precision == "fp64":
+        assert (np.abs(it - it_ref) <= 1).all() and np.mean(it == it_ref) > 0.9, (it, it_ref)
+    else:
+        assert (it <= it_ref + 2).all()
+    _compare(out, ref, case.symmetric, slice(None))
 
 
 def test_all_const_y_grid_runs_linear():
