@@ -15,6 +15,14 @@
 //   outputs               component/{node,branch,appliance}.hpp get_output, main_core/output.hpp,
 //                         main_core/topological_node_output.hpp:69-117
 //   batch semantics       job_dispatch.hpp:35-172 (threads stride scenarios; per-scenario errors)
+//   other methods         math_solver.hpp:43-64 and 142-156, linear_pf_solver.hpp,
+//                         iterative_current_pf_solver.hpp (complex systems solved in their real
+//                         [[G, -B], [B, G]] form: equal to the reference's up to rounding)
+//
+// Pinning (tests/test_oracle_golden.py): minimum degree, topology, Y-bus pattern and block LU against
+// the reference's known answers; NR against closed-form grids at 1e-12; every method against the
+// reference's validation fixtures that list it. Parity unpinned: per-scenario iteration counts on large
+// generated grids and bitwise agreement with Eigen's operation order (no reference fixture holds them).
 
 #include "pgm_oracle.h"
 
