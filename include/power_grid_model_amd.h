@@ -176,6 +176,39 @@ PGM_AMD_API void PGM_dataset_mutable_add_attribute_buffer(PGM_Handle* handle, PG
 PGM_AMD_API PGM_DatasetInfo const* PGM_dataset_mutable_get_info(PGM_Handle* handle,
                                                                 PGM_MutableDataset const* dataset) PGM_AMD_NOEXCEPT;
 
+/* dataset.h:241-269: writable datasets only come out of a deserializer, which this build does not
+ * provide; these fail with PGM_regular_error (the reference's Python package binds them at import) */
+PGM_AMD_API PGM_DatasetInfo const* PGM_dataset_writable_get_info(PGM_Handle* handle,
+                                                                 PGM_WritableDataset const* dataset) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_dataset_writable_set_buffer(PGM_Handle* handle, PGM_WritableDataset* dataset,
+                                                 char const* component, PGM_Idx* indptr, void* data) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_dataset_writable_set_attribute_buffer(PGM_Handle* handle, PGM_WritableDataset* dataset,
+                                                           char const* component, char const* attribute,
+                                                           void* data) PGM_AMD_NOEXCEPT;
+
+/* serialization.h:29-114: JSON / msgpack (de)serialization is out of this build's scope; every entry
+ * point fails with PGM_regular_error naming it, so callers that bind the whole API still load */
+typedef struct PGM_Serializer PGM_Serializer;
+typedef struct PGM_Deserializer PGM_Deserializer;
+PGM_AMD_API PGM_Deserializer* PGM_create_deserializer_from_binary_buffer(PGM_Handle* handle, char const* data,
+                                                                         PGM_Idx size,
+                                                                         PGM_Idx serialization_format) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Deserializer* PGM_create_deserializer_from_null_terminated_string(
+    PGM_Handle* handle, char const* data_string, PGM_Idx serialization_format) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_WritableDataset* PGM_deserializer_get_dataset(PGM_Handle* handle,
+                                                              PGM_Deserializer* deserializer) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_deserializer_parse_to_buffer(PGM_Handle* handle, PGM_Deserializer* deserializer) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_destroy_deserializer(PGM_Deserializer* deserializer) PGM_AMD_NOEXCEPT;
+PGM_AMD_API PGM_Serializer* PGM_create_serializer(PGM_Handle* handle, PGM_ConstDataset const* dataset,
+                                                  PGM_Idx serialization_format) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_serializer_get_to_binary_buffer(PGM_Handle* handle, PGM_Serializer* serializer,
+                                                     PGM_Idx use_compact_list, char const** data,
+                                                     PGM_Idx* size) PGM_AMD_NOEXCEPT;
+PGM_AMD_API char const* PGM_serializer_get_to_zero_terminated_string(PGM_Handle* handle, PGM_Serializer* serializer,
+                                                                     PGM_Idx use_compact_list,
+                                                                     PGM_Idx indent) PGM_AMD_NOEXCEPT;
+PGM_AMD_API void PGM_destroy_serializer(PGM_Serializer* serializer) PGM_AMD_NOEXCEPT;
+
 /* model.h:33-130 */
 PGM_AMD_API PGM_PowerGridModel* PGM_create_model(PGM_Handle* handle, double system_frequency,
                                                  PGM_ConstDataset const* input_dataset) PGM_AMD_NOEXCEPT;

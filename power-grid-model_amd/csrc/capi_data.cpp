@@ -606,3 +606,63 @@ PGM_DatasetInfo const* PGM_dataset_mutable_get_info(PGM_Handle*, PGM_MutableData
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// writable datasets and (de)serialization: not provided by this build (dataset.h:241-269,
+// serialization.h:29-114); the reference's Python layer binds them at import, so they exist and fail
+// with a regular error
+// ------------------------------------------------------------------------------------------------
+namespace {
+void unsupported(PGM_Handle* h, char const* what) {
+    if (!h) return;
+    h->clear();
+    h->err_code = PGM_regular_error;
+    h->err_msg = std::string(what) + " is not supported by this build (power flow only)\n";
+}
+}  // namespace
+
+PGM_DatasetInfo const* PGM_dataset_writable_get_info(PGM_Handle* handle, PGM_WritableDataset const*) noexcept {
+    unsupported(handle, "Writable datasets (deserialization)");
+    return nullptr;
+}
+void PGM_dataset_writable_set_buffer(PGM_Handle* handle, PGM_WritableDataset*, char const*, PGM_Idx*, void*) noexcept {
+    unsupported(handle, "Writable datasets (deserialization)");
+}
+void PGM_dataset_writable_set_attribute_buffer(PGM_Handle* handle, PGM_WritableDataset*, char const*, char const*,
+                                               void*) noexcept {
+    unsupported(handle, "Writable datasets (deserialization)");
+}
+PGM_Deserializer* PGM_create_deserializer_from_binary_buffer(PGM_Handle* handle, char const*, PGM_Idx,
+                                                             PGM_Idx) noexcept {
+    unsupported(handle, "Deserialization");
+    return nullptr;
+}
+PGM_Deserializer* PGM_create_deserializer_from_null_terminated_string(PGM_Handle* handle, char const*,
+                                                                      PGM_Idx) noexcept {
+    unsupported(handle, "Deserialization");
+    return nullptr;
+}
+PGM_WritableDataset* PGM_deserializer_get_dataset(PGM_Handle* handle, PGM_Deserializer*) noexcept {
+    unsupported(handle, "Deserialization");
+    return nullptr;
+}
+void PGM_deserializer_parse_to_buffer(PGM_Handle* handle, PGM_Deserializer*) noexcept {
+    unsupported(handle, "Deserialization");
+}
+void PGM_destroy_deserializer(PGM_Deserializer*) noexcept {}
+PGM_Serializer* PGM_create_serializer(PGM_Handle* handle, PGM_ConstDataset const*, PGM_Idx) noexcept {
+    unsupported(handle, "Serialization");
+    return nullptr;
+}
+void PGM_serializer_get_to_binary_buffer(PGM_Handle* handle, PGM_Serializer*, PGM_Idx, char const** data,
+                                         PGM_Idx* size) noexcept {
+    if (data) *data = nullptr;
+    if (size) *size = 0;
+    unsupported(handle, "Serialization");
+}
+char const* PGM_serializer_get_to_zero_terminated_string(PGM_Handle* handle, PGM_Serializer*, PGM_Idx,
+                                                         PGM_Idx) noexcept {
+    unsupported(handle, "Serialization");
+    return nullptr;
+}
+void PGM_destroy_serializer(PGM_Serializer*) noexcept {}

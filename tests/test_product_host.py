@@ -51,6 +51,49 @@ def test_every_declared_symbol_is_exported():
     assert missing == []
 
 
+# every C function the reference's Python layer binds at import
+# (src/power_grid_model/_core/power_grid_core.py, @make_c_binding methods); restated as data
+REFERENCE_PYTHON_BINDINGS = [
+    "version", "create_dataset_const_from_mutable", "create_handle", "error_code", "error_message", "n_failed_scenarios", "failed_scenarios",
+    "batch_errors", "clear_error", "meta_n_datasets", "meta_get_dataset_by_idx", "meta_dataset_name",
+    "meta_n_components", "meta_get_component_by_idx", "meta_component_name", "meta_component_size",
+    "meta_component_alignment", "meta_n_attributes", "meta_get_attribute_by_idx", "meta_attribute_name",
+    "meta_attribute_ctype", "meta_attribute_offset", "is_little_endian", "create_options", "destroy_options",
+    "set_calculation_type", "set_calculation_method", "set_symmetric", "set_err_tol", "set_max_iter",
+    "set_threading", "set_tap_changing_strategy", "set_short_circuit_voltage_scaling",
+    "set_experimental_features", "create_model", "update_model", "copy_model", "get_indexer", "destroy_model",
+    "calculate", "dataset_info_name", "dataset_info_is_batch", "dataset_info_batch_size",
+    "dataset_info_n_components", "dataset_info_component_name", "dataset_info_elements_per_scenario",
+    "dataset_info_total_elements", "dataset_info_has_attribute_indications", "dataset_info_n_attribute_indications",
+    "dataset_info_attribute_name", "create_dataset_const", "destroy_dataset_const", "dataset_const_add_buffer",
+    "dataset_const_add_attribute_buffer", "dataset_const_set_next_cartesian_product_dimension",
+    "dataset_const_get_info", "dataset_writable_get_info", "dataset_writable_set_buffer",
+    "dataset_writable_set_attribute_buffer", "create_dataset_mutable", "destroy_dataset_mutable",
+    "dataset_mutable_add_buffer", "dataset_mutable_add_attribute_buffer", "dataset_mutable_get_info",
+    "create_deserializer_from_binary_buffer", "create_deserializer_from_null_terminated_string",
+    "deserializer_get_dataset", "deserializer_parse_to_buffer", "destroy_deserializer", "create_serializer",
+    "serializer_get_to_binary_buffer", "serializer_get_to_zero_terminated_string", "destroy_serializer",
+]
+
+
+def test_exports_everything_the_reference_python_layer_binds():
+    missing = [n for n in REFERENCE_PYTHON_BINDINGS if not hasattr(lib, "PGM_" + n)]
+    assert missing == []
+
+
+def test_serialization_fails_with_a_regular_error():
+    import ctypes as C
+
+    from power_grid_model_amd._lib import Handle
+
+    h = Handle()
+    f = lib.PGM_create_serializer
+    f.restype = C.c_void_p
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+    assert f(h.ptr, None, 0) is None
+    assert h.code() == 1 and "not supported" in h.message()
+
+
 def test_meta_layouts_match_reference_formats():
     for dataset, comps in pf.DATASETS.items():
         meta = power_grid_meta_data[dataset]
