@@ -23,6 +23,7 @@ from .model import (
     BranchSide,
     CalculationMethod,
     CalculationType,
+    ComponentAttributeFilterOptions,
     DeviceBatch,
     FactorPrecision,
     LoadGenType,
@@ -38,6 +39,6 @@ __all__ = [
     "LIB_PATH", "lib", "ConflictID", "ConflictVoltage", "DatasetError", "IDNotFound", "IDWrongType",
     "InvalidCalculationMethod", "InvalidTransformerClock", "IterationDiverge", "PowerGridBatchError",
     "PowerGridError", "SparseMatrixError", "initialize_array", "power_grid_meta_data", "BranchSide",
-    "CalculationMethod", "CalculationType", "DeviceBatch", "FactorPrecision", "LoadGenType", "PowerGridModel",
+    "CalculationMethod", "CalculationType", "ComponentAttributeFilterOptions", "DeviceBatch", "FactorPrecision", "LoadGenType", "PowerGridModel",
     "WindingType", "block_lu_solve", "inspect_topology", "minimum_degree_ordering", "plan_stats",
 ]

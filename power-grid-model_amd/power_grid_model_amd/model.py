@@ -30,6 +30,13 @@ class CalculationMethod(IntEnum):
     iec60909 = 5
 
 
+class ComponentAttributeFilterOptions(IntEnum):
+    """Columnar output of every attribute of the selected components (enum.py:128-134)."""
+
+    everything = 0
+    relevant = 1
+
+
 class LoadGenType(IntEnum):
     const_power = 0
     const_impedance = 1
@@ -147,10 +154,47 @@ def _make_mutable_dataset(h: Handle, data: dict, dataset_type: str, is_batch: bo
     ds = _Dataset(ptr, lib.PGM_destroy_dataset_mutable, keep)
     for comp, arr in data.items():
         keep.append(arr)
+        if isinstance(arr, dict):
+            # columnar: one buffer per requested attribute (dataset.h add_attribute_buffer)
+            n = next(iter(arr.values())).shape[-1] if arr else 0
+            lib.PGM_dataset_mutable_add_buffer(h.ptr, ptr, comp.encode(), n, n * batch_size, None, None)
+            h.check()
+            for attr, col in arr.items():
+                lib.PGM_dataset_mutable_add_attribute_buffer(h.ptr, ptr, comp.encode(), attr.encode(),
+                                                             col.ctypes.data)
+                h.check()
+            continue
         n = arr.shape[-1]
         lib.PGM_dataset_mutable_add_buffer(h.ptr, ptr, comp.encode(), n, n * batch_size, None, arr.ctypes.data)
         h.check()
     return ds
+
+
+def _process_output_filter(out_type: str, data_filter, available: list[str]) -> dict:
+    """Normalise output_component_types as the reference does (utils.py:456-486): None = every
+    component row-based, a list or set = those components row-based, a filter option = every component
+    columnar, a dict = per component None (row-based), an attribute list or a filter option (columnar)."""
+    if data_filter is None:
+        flt = {c: None for c in available}
+    elif isinstance(data_filter, ComponentAttributeFilterOptions):
+        flt = {c: data_filter for c in available}
+    elif isinstance(data_filter, (list, set)):
+        flt = {c: None for c in data_filter}
+    elif isinstance(data_filter, dict) and all(
+            a is None or isinstance(a, (set, list, ComponentAttributeFilterOptions)) for a in data_filter.values()):
+        flt = dict(data_filter)
+    else:
+        raise ValueError(f"Invalid filter provided: {data_filter}")
+    meta = power_grid_meta_data[out_type]
+    unknown = [c for c in flt if c not in meta]
+    if unknown:
+        raise KeyError(f"You have specified some unknown component types: {unknown}")
+    for c, attrs in flt.items():
+        if isinstance(attrs, (list, set)):
+            bad = [a for a in attrs if a not in meta[c].dtype.names]
+            if bad:
+                raise KeyError(f"You have specified some unknown attributes: {{'{c}': {bad}}}")
+    return flt
 
 
 class PowerGridModel:
@@ -211,16 +255,25 @@ class PowerGridModel:
         return _Dataset(opt, lib.PGM_destroy_options, [])
 
     def _output_arrays(self, output_component_types, symmetric, is_batch, batch_size):
+        """Output buffers as the reference creates them (data_handling.py:146-200): structured arrays for
+        row-based components, {attribute: array} for columnar ones."""
         out_type = "sym_output" if symmetric else "asym_output"
         comps = self.all_component_count
-        if output_component_types is None:
-            selected = list(comps)
-        elif isinstance(output_component_types, dict):
-            selected = [c for c in output_component_types if c in comps]
-        else:
-            selected = [c for c in output_component_types if c in comps]
-        shape = (lambda n: (batch_size, n)) if is_batch else (lambda n: (n,))
-        return out_type, {c: initialize_array(out_type, c, shape(comps[c])) for c in selected}
+        flt = _process_output_filter(out_type, output_component_types, list(comps))
+        out = {}
+        for c, n in comps.items():
+            if c not in flt:
+                continue
+            shape = (batch_size, n) if is_batch else n
+            req = flt[c]
+            dtype = power_grid_meta_data[out_type][c].dtype
+            if req is None:
+                out[c] = initialize_array(out_type, c, shape)
+            elif isinstance(req, ComponentAttributeFilterOptions):
+                out[c] = {a: np.empty(shape, dtype=dtype[a]) for a in dtype.names}
+            else:
+                out[c] = {a: np.empty(shape, dtype=dtype[a]) for a in req}
+        return out_type, out
 
     def calculate_power_flow(
         self,

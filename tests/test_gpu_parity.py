@@ -361,6 +361,27 @@ def test_all_const_y_grid_runs_linear():
     _compare(out, ref, True, slice(None))
 
 
+def test_columnar_attribute_filtered_output():
+    """output_component_types as a dict of attribute lists gives columnar output (data_handling.py
+    create_output_data); values equal the row-based output."""
+    case = synthetic.radial_200(n_scenarios=12)
+    model = PowerGridModel(case.input)
+    rows = model.calculate_power_flow(update_data=case.update)
+    cols = model.calculate_power_flow(update_data=case.update,
+                                      output_component_types={"node": ["u_pu", "u_angle"], "line": ["p_from", "q_to"],
+                                                              "sym_load": None})
+    assert set(cols) == {"node", "line", "sym_load"}
+    assert set(cols["node"]) == {"u_pu", "u_angle"}
+    for comp, attrs in (("node", ["u_pu", "u_angle"]), ("line", ["p_from", "q_to"])):
+        for a in attrs:
+            assert cols[comp][a].shape == rows[comp][a].shape
+            assert cols[comp][a].tobytes() == rows[comp][a].tobytes()
+    assert cols["sym_load"].tobytes() == rows["sym_load"].tobytes()
+    everything = model.calculate_power_flow(update_data=case.update,
+                                            output_component_types=pgm.ComponentAttributeFilterOptions.everything)
+    assert everything["line"]["i_from"].tobytes() == rows["line"]["i_from"].tobytes()
+
+
 def test_error_cases_match_oracle():
     case = synthetic.radial_200(n_scenarios=8)
     upd = {k: v.copy() for k, v in case.update.items()}
