@@ -224,7 +224,7 @@ int packed_target() {
     static int const t = [] {
         char const* v = std::getenv("PGM_AMD_PACKED_TARGET");
         int const x = v ? std::atoi(v) : 0;
-        return x > 0 ? x : 64;
+        return x > 0 ? x : 256;
     }();
     return t;
 }
