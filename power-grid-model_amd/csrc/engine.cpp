@@ -730,7 +730,7 @@ Batch::~Batch() {
     if (ev_stop) (void)hipEventDestroy(ev_stop);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
-    if (stream_u) (void)hipStreamDestroy(stream_u);
+    if (stream_u && stream_u != stream) (void)hipStreamDestroy(stream_u);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -763,7 +763,12 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     b.is_batch = update != nullptr;
     b.n_scenarios = update ? update->batch_size : 1;
     PGM_HIP_CHECK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
-    PGM_HIP_CHECK(hipStreamCreateWithFlags(&b.stream_u, hipStreamNonBlocking));
+    // packed and union groups overlap the plain groups on a second stream, unless PGM_AMD_STREAMS=1
+    // asks for one stream (callers that capture or serialise the work)
+    if (char const* v = std::getenv("PGM_AMD_STREAMS"); v && std::atoi(v) == 1)
+        b.stream_u = b.stream;
+    else
+        PGM_HIP_CHECK(hipStreamCreateWithFlags(&b.stream_u, hipStreamNonBlocking));
     PGM_HIP_CHECK(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
     PGM_HIP_CHECK(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
     PGM_HIP_CHECK(hipEventCreate(&b.ev_start));
@@ -1089,6 +1094,12 @@ void Batch::run() {
     fwd_launches = 0;
     fwd_bytes = 0.0;
     std::vector<hipEvent_t> fwd_events;
+    // event timing of the forward launches (stats); PGM_AMD_EVENTS=0 leaves it out, for tools that
+    // intercept the queue and take only kernel dispatches
+    bool const timing = [] {
+        char const* v = std::getenv("PGM_AMD_EVENTS");
+        return !(v && std::atoi(v) == 0);
+    }();
     auto event_from_pool = [&](size_t i) {
         while (event_pool.size() <= i) {
             hipEvent_t e;
@@ -1102,12 +1113,12 @@ void Batch::run() {
     // the union groups' sweeps run on a second stream, overlapping the main groups' sweeps; both
     // join before each convergence check
     auto fork = [&] {
-        if (LU + LP == 0) return;
+        if (LU + LP == 0 || su == st) return;
         PGM_HIP_CHECK(hipEventRecord(ev_fork, st));
         PGM_HIP_CHECK(hipStreamWaitEvent(su, ev_fork, 0));
     };
     auto join = [&] {
-        if (LU + LP == 0) return;
+        if (LU + LP == 0 || su == st) return;
         PGM_HIP_CHECK(hipEventRecord(ev_join, su));
         PGM_HIP_CHECK(hipStreamWaitEvent(st, ev_join, 0));
     };
@@ -1122,7 +1133,7 @@ void Batch::run() {
             if (l < LU) launch_backward(d_groups, d_items_u[l], n_items_u[l], P, fp32, mode, su, KIND_UNION);
         }
     };
-    PGM_HIP_CHECK(hipEventRecord(ev_start, st));
+    if (timing) PGM_HIP_CHECK(hipEventRecord(ev_start, st));
     launch_init_state(pairs, st);
     fork();
     // the linear system: Newton-Raphson's initial guess, and the answer of the linear method and of
@@ -1144,11 +1155,15 @@ void Batch::run() {
         int const mode = current ? (it == 1 ? MODE_ICF : MODE_RESID) : MODE_NR;
         union_sweeps(mode);
         for (int l = 0; l < L; ++l) {
-            fwd_events.push_back(event_from_pool(fwd_events.size()));
-            PGM_HIP_CHECK(hipEventRecord(fwd_events.back(), st));
+            if (timing) {
+                fwd_events.push_back(event_from_pool(fwd_events.size()));
+                PGM_HIP_CHECK(hipEventRecord(fwd_events.back(), st));
+            }
             launch_forward(d_groups, d_items[l], n_items[l], P, fp32, mode, st);
-            fwd_events.push_back(event_from_pool(fwd_events.size()));
-            PGM_HIP_CHECK(hipEventRecord(fwd_events.back(), st));
+            if (timing) {
+                fwd_events.push_back(event_from_pool(fwd_events.size()));
+                PGM_HIP_CHECK(hipEventRecord(fwd_events.back(), st));
+            }
             if (mode == MODE_NR) fwd_bytes += level_bytes_full[l] * active_frac;
             ++fwd_launches;
         }
@@ -1216,10 +1231,14 @@ void Batch::run() {
     for (int c : {int(C_SHUNT), int(C_SOURCE), int(C_SYM_GEN), int(C_ASYM_GEN), int(C_SYM_LOAD), int(C_ASYM_LOAD)})
         launch_appliance_records(P, d_outputs, d_appl_recs[c], d_rec_off[c], G, rec_total[c], out[c], st);
     PGM_HIP_CHECK(hipGetLastError());
-    PGM_HIP_CHECK(hipEventRecord(ev_stop, st));
-    PGM_HIP_CHECK(hipEventSynchronize(ev_stop));
     float ms = 0;
-    PGM_HIP_CHECK(hipEventElapsedTime(&ms, ev_start, ev_stop));
+    if (timing) {
+        PGM_HIP_CHECK(hipEventRecord(ev_stop, st));
+        PGM_HIP_CHECK(hipEventSynchronize(ev_stop));
+        PGM_HIP_CHECK(hipEventElapsedTime(&ms, ev_start, ev_stop));
+    } else {
+        PGM_HIP_CHECK(hipStreamSynchronize(st));
+    }
     kernel_ms = ms;
     double fwd_ms = 0.0;
     for (size_t i = 0; i + 1 < fwd_events.size(); i += 2) {

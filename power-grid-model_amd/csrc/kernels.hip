@@ -24,7 +24,8 @@
 
 namespace pgm_amd {
 
-namespace {
+// kernels keep external linkage so profilers and the code-object loader can resolve their symbols
+namespace dev {
 
 constexpr int kWave = 64;
 
@@ -1371,7 +1372,9 @@ __global__ void patch_y_kernel(int n_patch, int Q, int S_pad, int const* __restr
     ylane[(size_t(entry[q]) * Q + c) * S_pad + lane[q]] = val[i];
 }
 
-}  // namespace
+}  // namespace dev
+
+using namespace dev;
 
 void launch_forward(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
                     hipStream_t st, int kind) {

@@ -268,7 +268,7 @@ class PowerGridModel:
             req = flt[c]
             dtype = power_grid_meta_data[out_type][c].dtype
             if req is None:
-                out[c] = initialize_array(out_type, c, shape)
+                out[c] = initialize_array(out_type, c, shape, empty=True)
             elif isinstance(req, ComponentAttributeFilterOptions):
                 out[c] = {a: np.empty(shape, dtype=dtype[a]) for a in dtype.names}
             else:

@@ -283,6 +283,22 @@ def test_packed_groups(precision):
     _compare(out, ref, True, slice(None))
 
 
+def test_single_stream_schedule_gives_identical_results(monkeypatch):
+    """PGM_AMD_STREAMS=1 runs packed and union groups on the main stream: same results bit for bit."""
+    inp, upd = _tie_case()
+    model = PowerGridModel(inp)
+    two = DeviceBatch(model, upd)
+    two.run()
+    ref = two.fetch()
+    monkeypatch.setenv("PGM_AMD_STREAMS", "1")
+    one = DeviceBatch(model, upd)
+    one.run()
+    out = one.fetch()
+    for comp in ref:
+        assert out[comp].tobytes() == ref[comp].tobytes(), comp
+    assert (one.iterations() == two.iterations()).all()
+
+
 @pytest.mark.parametrize("name,make,scen", WORKLOADS[:3], ids=[w[0] for w in WORKLOADS[:3]])
 def test_workload_mixed_precision(name, make, scen):
     case = make()
