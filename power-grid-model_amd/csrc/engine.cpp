@@ -1060,8 +1060,19 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         b.bus_total = bus_off.back();
         b.d_bus_off = put(b.arena, bus_off);
         for (int c = 0; c < N_COMP; ++c) {
+            size_t const rs = c == C_NODE ? (b.symmetric ? sizeof(NodeOutput<1>) : sizeof(NodeOutput<3>))
+                              : (c == C_LINE || c == C_TRANSFORMER)
+                                  ? (b.symmetric ? sizeof(BranchOutput<1>) : sizeof(BranchOutput<3>))
+                                  : (b.symmetric ? sizeof(ApplianceOutput<1>) : sizeof(ApplianceOutput<3>));
+            long long const slots = 64LL * record_tile(rs);
             std::vector<long long> off{0};
-            for (auto& gp : b.groups) off.push_back(off.back() + (long long)(m.count(c)) * gp->S_pad);
+            for (auto& gp : b.groups) {
+                // tiles of W lanes x slots / W components, W = min(S_pad, 64)
+                long long const cnt = m.count(c);
+                long long const w = std::min<long long>(gp->S_pad, 64), tn = slots / w;
+                long long const tiles = cnt == 0 ? 0 : (gp->S_pad + w - 1) / w * ((cnt + tn - 1) / tn);
+                off.push_back(off.back() + tiles);
+            }
             b.rec_total[c] = off.back();
             b.d_rec_off[c] = put(b.arena, off);
         }

@@ -1162,20 +1162,10 @@ __global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long
 // ---------------------------------------------------------------------------------------------------
 // record writers (node.hpp, branch.hpp, appliance.hpp get_output)
 // ---------------------------------------------------------------------------------------------------
+// one record for component i of lane s (the lane is valid and converged)
 template <int P>
-__global__ void node_records_kernel(DevOutput const* __restrict__ outs, NodeRec const* __restrict__ recs,
-                        long long const* __restrict__ offsets, int n_groups, NodeOutput<P>* out) {
-    long long const gidx = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gidx >= offsets[n_groups]) return;
-    int const gr = find_group(offsets, n_groups, gidx);
-    DevOutput const& o = outs[gr];
-    NodeRec const& r = recs[gr];
-    size_t const idx = size_t(gidx - offsets[gr]);
+__device__ __forceinline__ void node_record(DevOutput const& o, NodeRec const& r, int i, int s, NodeOutput<P>& rec) {
     size_t const sp = o.S_pad;
-    // scenario-major: consecutive threads write consecutive records of one scenario
-    int const s = int(idx / size_t(r.n_comp)), i = int(idx % size_t(r.n_comp));
-    if (s >= o.S || !o.ok[s]) return;
-    NodeOutput<P> rec{};
     rec.id = r.id[i];
     int const bus = r.bus[i];
     double const base_power = P == 1 ? 1e6 : 1e6 / 3.0;
@@ -1193,23 +1183,11 @@ __global__ void node_records_kernel(DevOutput const* __restrict__ outs, NodeRec 
             rec.q[ph] = base_power * o.bus_inj[((size_t(bus) * P + ph) * 2 + 1) * sp + s];
         }
     }
-    out[size_t(o.scenario_index[s]) * r.n_comp + i] = rec;
 }
 
 template <int P>
-__global__ void branch_records_kernel(DevOutput const* __restrict__ outs, BranchRec const* __restrict__ recs,
-                        long long const* __restrict__ offsets, int n_groups, BranchOutput<P>* out) {
-    long long const gidx = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gidx >= offsets[n_groups]) return;
-    int const gr = find_group(offsets, n_groups, gidx);
-    DevOutput const& o = outs[gr];
-    BranchRec const& r = recs[gr];
-    size_t const idx = size_t(gidx - offsets[gr]);
+__device__ __forceinline__ void branch_record(DevOutput const& o, BranchRec const& r, int i, int s, BranchOutput<P>& rec) {
     size_t const sp = o.S_pad;
-    // scenario-major: consecutive threads write consecutive records of one scenario
-    int const s = int(idx / size_t(r.n_comp)), i = int(idx % size_t(r.n_comp));
-    if (s >= o.S || !o.ok[s]) return;
-    BranchOutput<P> rec{};
     rec.id = r.id[i];
     int const m = r.math[i];
     double const base_power = P == 1 ? 1e6 : 1e6 / 3.0;
@@ -1262,23 +1240,11 @@ __global__ void branch_records_kernel(DevOutput const* __restrict__ outs, Branch
         }
         rec.loading = r.is_line ? max_i / r.rating[i] : (sum_sf > sum_st ? sum_sf : sum_st) / r.rating[i];
     }
-    out[size_t(o.scenario_index[s]) * r.n_comp + i] = rec;
 }
 
 template <int P>
-__global__ void appliance_records_kernel(DevOutput const* __restrict__ outs, ApplRec const* __restrict__ recs,
-                        long long const* __restrict__ offsets, int n_groups, ApplianceOutput<P>* out) {
-    long long const gidx = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gidx >= offsets[n_groups]) return;
-    int const gr = find_group(offsets, n_groups, gidx);
-    DevOutput const& o = outs[gr];
-    ApplRec const& r = recs[gr];
-    size_t const idx = size_t(gidx - offsets[gr]);
+__device__ __forceinline__ void appliance_record(DevOutput const& o, ApplRec const& r, int i, int s, ApplianceOutput<P>& rec) {
     size_t const sp = o.S_pad;
-    // scenario-major: consecutive threads write consecutive records of one scenario
-    int const s = int(idx / size_t(r.n_comp)), i = int(idx % size_t(r.n_comp));
-    if (s >= o.S || !o.ok[s]) return;
-    ApplianceOutput<P> rec{};
     rec.id = r.id[i];
     int const m = r.math[i];
     double const base_power = P == 1 ? 1e6 : 1e6 / 3.0;
@@ -1297,7 +1263,66 @@ __global__ void appliance_records_kernel(DevOutput const* __restrict__ outs, App
             rec.pf[ph] = rec.s[ph] < 1e-8 ? 0.0 : rec.p[ph] / rec.s[ph];
         }
     }
-    out[size_t(o.scenario_index[s]) * r.n_comp + i] = rec;
+}
+
+template <int P>
+__device__ __forceinline__ void make_record(DevOutput const& o, NodeRec const& r, int i, int s, NodeOutput<P>& rec) {
+    node_record<P>(o, r, i, s, rec);
+}
+template <int P>
+__device__ __forceinline__ void make_record(DevOutput const& o, BranchRec const& r, int i, int s, BranchOutput<P>& rec) {
+    branch_record<P>(o, r, i, s, rec);
+}
+template <int P>
+__device__ __forceinline__ void make_record(DevOutput const& o, ApplRec const& r, int i, int s,
+                                            ApplianceOutput<P>& rec) {
+    appliance_record<P>(o, r, i, s, rec);
+}
+
+// A block builds a tile of W lanes x (64 TN / W) components in LDS (W = min(S_pad, 64)), reading the
+// scenario-interleaved state with lane-contiguous (coalesced) loads, then writes each lane's records as
+// one contiguous run of 16-byte stores: both sides of the transpose stay coalesced, for wide groups and
+// for single-scenario ones. Tile t of group g: offsets[g] <= t.
+template <int P, class Rec, class Out>
+__global__ __launch_bounds__(256) void records_kernel(DevOutput const* __restrict__ outs, Rec const* __restrict__ recs,
+                                                      long long const* __restrict__ tile_off, int n_groups,
+                                                      Out* __restrict__ out) {
+    constexpr int SLOTS = kWaveLanes * record_tile(sizeof(Out));
+    constexpr int CH = int(sizeof(Out) / 16);
+    static_assert(sizeof(Out) % 16 == 0, "records are copied in 16-byte chunks");
+    __shared__ __align__(16) Out tile[SLOTS];
+    long long const t = blockIdx.x;
+    int const gr = find_group(tile_off, n_groups, t);
+    DevOutput const& o = outs[gr];
+    Rec const& r = recs[gr];
+    int const W = o.S_pad < kWaveLanes ? o.S_pad : kWaveLanes;
+    int const TNg = SLOTS / W;
+    int const n_itiles = (r.n_comp + TNg - 1) / TNg;
+    long long const local = t - tile_off[gr];
+    int const s0 = int(local / n_itiles) * W, i0 = int(local % n_itiles) * TNg;
+    for (int q = int(threadIdx.x); q < W * TNg; q += int(blockDim.x)) {
+        int const lane = q % W, ii = q / W;
+        int const s = s0 + lane, i = i0 + ii;
+        if (s < o.S && o.ok[s] && i < r.n_comp) {
+            // zero the slot (padding included: the records come out byte for byte the same every run),
+            // then fill the attributes in place
+            Out& slot = tile[lane * TNg + ii];
+            uint4* z = reinterpret_cast<uint4*>(&slot);
+#pragma unroll
+            for (int k = 0; k < CH; ++k) z[k] = uint4{0u, 0u, 0u, 0u};
+            make_record<P>(o, r, i, s, slot);
+        }
+    }
+    __syncthreads();
+    int const n_valid = r.n_comp - i0 < TNg ? r.n_comp - i0 : TNg;
+    for (int c = int(threadIdx.x); c < W * TNg * CH; c += int(blockDim.x)) {
+        int const ln = c / (TNg * CH), k = c % (TNg * CH);
+        int const sl = s0 + ln;
+        if (k >= n_valid * CH || sl >= o.S || !o.ok[sl]) continue;
+        uint4 const* src = reinterpret_cast<uint4 const*>(&tile[ln * TNg]) + k;
+        uint4* dst = reinterpret_cast<uint4*>(out + size_t(o.scenario_index[sl]) * r.n_comp + i0) + k;
+        *dst = *src;
+    }
 }
 
 template <int P, class FT, int KIND>
@@ -1427,38 +1452,35 @@ void launch_bus_results(int P, DevOutput const* outs, long long const* offsets, 
     else
         hipLaunchKernelGGL(bus_results_kernel<3>, grid, dim3(256), 0, st, outs, offsets, n_groups);
 }
-void launch_node_records(int P, DevOutput const* outs, NodeRec const* recs, long long const* offsets, int n_groups,
-                         long long total, void* out, hipStream_t st) {
-    if (total == 0) return;
-    dim3 const grid(unsigned((total + 255) / 256));
+void launch_node_records(int P, DevOutput const* outs, NodeRec const* recs, long long const* tile_off, int n_groups,
+                         long long tiles, void* out, hipStream_t st) {
+    if (tiles == 0) return;
     if (P == 1)
-        hipLaunchKernelGGL(node_records_kernel<1>, grid, dim3(256), 0, st, outs, recs, offsets, n_groups,
-                           static_cast<NodeOutput<1>*>(out));
+        hipLaunchKernelGGL((records_kernel<1, NodeRec, NodeOutput<1>>), dim3(unsigned(tiles)), dim3(256), 0, st, outs,
+                           recs, tile_off, n_groups, static_cast<NodeOutput<1>*>(out));
     else
-        hipLaunchKernelGGL(node_records_kernel<3>, grid, dim3(256), 0, st, outs, recs, offsets, n_groups,
-                           static_cast<NodeOutput<3>*>(out));
+        hipLaunchKernelGGL((records_kernel<3, NodeRec, NodeOutput<3>>), dim3(unsigned(tiles)), dim3(256), 0, st, outs,
+                           recs, tile_off, n_groups, static_cast<NodeOutput<3>*>(out));
 }
-void launch_branch_records(int P, DevOutput const* outs, BranchRec const* recs, long long const* offsets,
-                           int n_groups, long long total, void* out, hipStream_t st) {
-    if (total == 0) return;
-    dim3 const grid(unsigned((total + 255) / 256));
+void launch_branch_records(int P, DevOutput const* outs, BranchRec const* recs, long long const* tile_off,
+                           int n_groups, long long tiles, void* out, hipStream_t st) {
+    if (tiles == 0) return;
     if (P == 1)
-        hipLaunchKernelGGL(branch_records_kernel<1>, grid, dim3(256), 0, st, outs, recs, offsets, n_groups,
-                           static_cast<BranchOutput<1>*>(out));
+        hipLaunchKernelGGL((records_kernel<1, BranchRec, BranchOutput<1>>), dim3(unsigned(tiles)), dim3(256), 0, st,
+                           outs, recs, tile_off, n_groups, static_cast<BranchOutput<1>*>(out));
     else
-        hipLaunchKernelGGL(branch_records_kernel<3>, grid, dim3(256), 0, st, outs, recs, offsets, n_groups,
-                           static_cast<BranchOutput<3>*>(out));
+        hipLaunchKernelGGL((records_kernel<3, BranchRec, BranchOutput<3>>), dim3(unsigned(tiles)), dim3(256), 0, st,
+                           outs, recs, tile_off, n_groups, static_cast<BranchOutput<3>*>(out));
 }
-void launch_appliance_records(int P, DevOutput const* outs, ApplRec const* recs, long long const* offsets,
-                              int n_groups, long long total, void* out, hipStream_t st) {
-    if (total == 0) return;
-    dim3 const grid(unsigned((total + 255) / 256));
+void launch_appliance_records(int P, DevOutput const* outs, ApplRec const* recs, long long const* tile_off,
+                              int n_groups, long long tiles, void* out, hipStream_t st) {
+    if (tiles == 0) return;
     if (P == 1)
-        hipLaunchKernelGGL(appliance_records_kernel<1>, grid, dim3(256), 0, st, outs, recs, offsets, n_groups,
-                           static_cast<ApplianceOutput<1>*>(out));
+        hipLaunchKernelGGL((records_kernel<1, ApplRec, ApplianceOutput<1>>), dim3(unsigned(tiles)), dim3(256), 0, st,
+                           outs, recs, tile_off, n_groups, static_cast<ApplianceOutput<1>*>(out));
     else
-        hipLaunchKernelGGL(appliance_records_kernel<3>, grid, dim3(256), 0, st, outs, recs, offsets, n_groups,
-                           static_cast<ApplianceOutput<3>*>(out));
+        hipLaunchKernelGGL((records_kernel<3, ApplRec, ApplianceOutput<3>>), dim3(unsigned(tiles)), dim3(256), 0, st,
+                           outs, recs, tile_off, n_groups, static_cast<ApplianceOutput<3>*>(out));
 }
 
 }  // namespace pgm_amd
