@@ -61,6 +61,18 @@ def cpu_baseline(case, sample_scenarios, threads):
     return done / t_total, t_total
 
 
+def measured_traffic(workload: str, precision: str):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 --pmc summary of the same
+    workload (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy kernel), or None."""
+    path = ROOT / "profiles" / f"forward_kernel_{workload}_traffic.md"
+    if precision != "fp32" or not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def _leading(update, n):
     """The first n scenarios of an update dataset (uniform arrays or sparse {indptr, data})."""
     out = {}
@@ -87,7 +99,7 @@ def main():
     ap.add_argument("--e2e-fraction", type=float, default=0.2,
                     help="share of the scenarios timed through calculate_power_flow from host buffers (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="first run a 256 MiB device-to-device torch copy (the reference kernel for HBM counters)")
+                    help="first run one plain 64 MiB copy kernel (the calibration kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): scenarios shard across GPUs without any "
@@ -97,14 +109,11 @@ def main():
     from power_grid_model_amd import DeviceBatch, PowerGridModel, synthetic
 
     if args.copy_probe:
-        import torch
+        from power_grid_model_amd._lib import lib as _clib
 
-        x = torch.empty(64 << 20, dtype=torch.float32, device="cuda:0").fill_(1.0)
-        y = torch.empty_like(x)
-        y.copy_(x)
-        torch.cuda.synchronize()
-        log("[bench] copy probe: 268435456 bytes read and written by one torch copy kernel")
-        del x, y
+        if _clib.PGM_amd_copy_probe(16 << 20) != 0:
+            raise RuntimeError("copy probe failed")
+        log("[bench] copy probe: copy_probe_kernel read and wrote 67108864 bytes")
 
     make = synthetic.CONFIGS[args.workload]
     t0 = time.perf_counter()
@@ -202,7 +211,7 @@ def main():
         "fp64_gpu_value": fp64_value,
         "mixed_over_fp64": (value / fp64_value) if fp64_value else None,
         "roofline": {"bound": "hbm", "achieved": fwd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": fwd_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": fwd_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(case.name, args.precision),
                      "kernel": "forward_kernel (assembly + block LU + forward substitution)",
                      "bytes_per_launch": fwd_bytes_per_launch, "avg_launch_ms": st["fwd_launch_ms"]},
         "step_roofline": {"algorithmic_bytes": step_bytes, "achieved_gbs": step_bytes / t_step / 1e9,

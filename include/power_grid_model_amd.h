@@ -285,6 +285,11 @@ PGM_AMD_API int PGM_amd_block_lu_solve(PGM_Idx nb, PGM_Idx n, PGM_Idx const* row
                                        double const* data, double const* rhs, double* x, int32_t* status,
                                        PGM_Idx precision) PGM_AMD_NOEXCEPT;
 
+/* Test hook with no counterpart in the reference: one plain copy kernel over n_floats floats (reads and
+ * writes 4 * n_floats bytes of HBM), the reference kernel that calibrates rocprofv3 FETCH_SIZE /
+ * WRITE_SIZE counters. Allocates and frees its own buffers. Returns 0, or -1 on a HIP error. */
+PGM_AMD_API int PGM_amd_copy_probe(PGM_Idx n_floats) PGM_AMD_NOEXCEPT;
+
 #ifdef __cplusplus
 }
 #endif

@@ -253,3 +253,12 @@ int PGM_amd_block_lu_solve(PGM_Idx nb, PGM_Idx n, PGM_Idx const* row_indptr, PGM
 }
 
 }  // extern "C"
+
+int PGM_amd_copy_probe(PGM_Idx n_floats) noexcept {
+    try {
+        pgm_amd::ensure_device();
+        return pgm_amd::copy_probe(n_floats);
+    } catch (...) {
+        return -1;
+    }
+}

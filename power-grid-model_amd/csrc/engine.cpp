@@ -49,6 +49,13 @@ void* Arena::take(size_t bytes) {
     return r;
 }
 
+void ensure_device() {
+    int n = 0;
+    hipError_t const e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        throw PowerGridError("No HIP device available: this library runs the power flow on an MI355X (gfx950) GPU\n");
+}
+
 std::string format_double(double x) {
     char buf[64];
     for (int prec = 1; prec <= 17; ++prec) {
@@ -67,12 +74,6 @@ char const* const SparseMatrixErrorMessage =
     "It might also mean that you are running into a corner case where PGM cannot resolve yet.\n"
     "See https://github.com/PowerGridModel/power-grid-model/issues/864.";
 
-void ensure_device() {
-    int n = 0;
-    hipError_t const e = hipGetDeviceCount(&n);
-    if (e != hipSuccess || n == 0)
-        throw PowerGridError("No HIP device available: this library runs the power flow on an MI355X (gfx950) GPU\n");
-}
 
 template <class T> T* dev_alloc(Arena& a, size_t count) { return static_cast<T*>(a.take(count * sizeof(T))); }
 // blocking copy into the arena (the host vectors are often temporaries); nullptr for an empty vector

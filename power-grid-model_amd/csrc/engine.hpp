@@ -112,6 +112,8 @@ int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices
                     double const* data, double const* rhs, double* x, int32_t* status, bool fp32);
 
 std::string format_double(double x);
+// throws PowerGridError when no HIP device is visible
+void ensure_device();
 void batch_plan_sizes(Batch const& b, double& buses, double& entries);
 
 }  // namespace pgm_amd

@@ -1397,9 +1397,32 @@ __global__ void patch_y_kernel(int n_patch, int Q, int S_pad, int const* __restr
     ylane[(size_t(entry[q]) * Q + c) * S_pad + lane[q]] = val[i];
 }
 
+__global__ void copy_probe_kernel(float const* __restrict__ a, float* __restrict__ b, long long n) {
+    long long const i = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
 }  // namespace dev
 
 using namespace dev;
+
+int copy_probe(long long n) {
+    if (n <= 0) return -1;
+    float *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, size_t(n) * sizeof(float)) != hipSuccess) return -1;
+    if (hipMalloc(&b, size_t(n) * sizeof(float)) != hipSuccess) {
+        (void)hipFree(a);
+        return -1;
+    }
+    int rc = hipMemset(a, 0, size_t(n) * sizeof(float)) == hipSuccess ? 0 : -1;
+    if (rc == 0) {
+        hipLaunchKernelGGL(copy_probe_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, nullptr, a, b, n);
+        rc = (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess) ? 0 : -1;
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}
 
 void launch_forward(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
                     hipStream_t st, int kind) {

@@ -106,6 +106,8 @@ void launch_backward(DevGroup const* groups, WorkItem const* items, int n_items,
 // fold the deviation of iteration `it`, mark converged / diverged, count the still active pairs
 void launch_converge(PairState const& ps, int it, int max_iter, double err_tol, int* active, hipStream_t st);
 void launch_init_state(PairState const& ps, hipStream_t st);
+// one plain copy kernel over n floats (HBM counter calibration); 0 or -1
+int copy_probe(long long n);
 // iterative current flat start: every bus of an active pair at the averaged source voltage of its math
 // model, rotated by the bus's phase shift (iterative_current_pf_solver.hpp make_flat_start)
 void launch_ic_start(int P, DevGroup const* groups, long long const* bus_offsets, int n_groups, long long total,
