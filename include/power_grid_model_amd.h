@@ -266,9 +266,11 @@ PGM_AMD_API PGM_Idx PGM_amd_inspect_topology(PGM_Handle* handle, PGM_PowerGridMo
 /* Host-only statistics and validity check of the device factorisation schedule of a model's base
  * topology (no GPU needed). out[0] buses, out[1] LU entries, out[2] levels, out[3] tasks, out[4] Schur
  * updates, out[5] critical path in pivots (sum over levels of the longest task), out[6] 1 when every
- * dependency of every pivot is scheduled before it, else 0. */
+ * dependency of every pivot is scheduled before it (and, for packed_lanes > 0, the interleaved layout
+ * of a packed group of that many scenarios is consistent), else 0. */
 PGM_AMD_API void PGM_amd_plan_stats(PGM_Handle* handle, PGM_PowerGridModel const* model, PGM_Idx symmetric,
-                                    PGM_Idx target_parallel_tasks, PGM_Idx* out) PGM_AMD_NOEXCEPT;
+                                    PGM_Idx target_parallel_tasks, PGM_Idx packed_lanes,
+                                    PGM_Idx* out) PGM_AMD_NOEXCEPT;
 
 /* Host-only minimum-degree ordering on an adjacency map given as sorted keys + CSR neighbour lists
  * (same contract as sparse_ordering.hpp:183 minimum_degree_ordering). Returns the number of fills or -1. */

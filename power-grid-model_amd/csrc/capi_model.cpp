@@ -171,7 +171,7 @@ PGM_Idx PGM_amd_inspect_topology(PGM_Handle* handle, PGM_PowerGridModel const* m
 }
 
 void PGM_amd_plan_stats(PGM_Handle* handle, PGM_PowerGridModel const* model, PGM_Idx symmetric,
-                        PGM_Idx target_parallel_tasks, PGM_Idx* out) noexcept {
+                        PGM_Idx target_parallel_tasks, PGM_Idx packed_lanes, PGM_Idx* out) noexcept {
     guarded(handle, [&] {
         Model const& m = model->model;
         Topology const t = build_topology(m, m.base);
@@ -209,6 +209,16 @@ void PGM_amd_plan_stats(PGM_Handle* handle, PGM_PowerGridModel const* model, PGM
                 int const le = p.schur[q][1], ue = p.schur[q][2];
                 int const k = row_of(ue);  // the U entry (k, j) lives in row k, its pivot
                 valid = valid && before(k, int(r)) && p.col[le] == k;
+            }
+        }
+        if (packed_lanes > 0) {
+            // the layout of a packed group of packed_lanes scenarios must stay consistent
+            Plan q = p;
+            try {
+                interleave_plan(q, int(kWaveLanes / std::clamp<Idx>(packed_lanes, 1, kWaveLanes)));
+                check_plan(q);
+            } catch (std::logic_error const&) {
+                valid = false;
             }
         }
         out[0] = p.n;

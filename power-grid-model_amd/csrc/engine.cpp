@@ -287,6 +287,8 @@ void upload_group(Batch& b, Group& g) {
     d.S_pad = g.S_pad;
     d.fp32 = b.fp32 ? 1 : 0;
     d.row_ptr = put(A, p.row_ptr);
+    d.row_end = put(A, p.row_end);
+    d.row_stride = put(A, p.row_stride);
     d.col = put(A, p.col);
     d.diag = put(A, p.diag);
     d.transpose = put(A, p.transpose);
@@ -304,7 +306,8 @@ void upload_group(Batch& b, Group& g) {
     d.keep = put(A, p.keep);
     d.phase = put(A, p.phase);
     d.bus_mm = put(A, p.bus_mm);
-    d.mm_bus_begin = put(A, p.mm_bus_begin);
+    d.mm_src_ptr = put(A, p.mm_src_ptr);
+    d.mm_src = put(A, p.mm_src);
     {
         std::vector<int> sb(p.src_comp.size());
         for (Idx bus = 0; bus < p.n; ++bus)
@@ -334,7 +337,7 @@ void upload_group(Batch& b, Group& g) {
             int const tk = p.level_task[q];
             for (int i = p.task_ptr[tk]; i < p.task_ptr[tk + 1]; ++i) {
                 int const pv = p.task_pivot[i];
-                ent += p.row_ptr[pv + 1] - p.row_ptr[pv];
+                ent += (p.row_end[pv] - p.row_ptr[pv]) / (p.row_stride.empty() ? 1 : p.row_stride[pv]);
                 piv += 1;
             }
         }
@@ -353,6 +356,9 @@ void upload_group(Batch& b, Group& g) {
     o.ok = b.d_ok + g.lane_off;
     o.n = int(n);
     o.row_ptr = d.row_ptr;
+    o.row_end = d.row_end;
+    o.row_stride = d.row_stride;
+    o.bus_mm = d.bus_mm;
     o.col = d.col;
     o.fill = d.fill;
     o.ydata = d.ydata;
@@ -379,7 +385,7 @@ void upload_group(Batch& b, Group& g) {
             ids.push_back(m.node[i].id);
             ur.push_back(m.node[i].u_rated);
             Idx const gg = t.node_group[i];
-            bus.push_back(gg < 0 ? -1 : p.mm_bus_begin[gg] + int(t.node_pos[i]));
+            bus.push_back(gg < 0 ? -1 : p.bus_label[p.mm_bus_begin[gg] + int(t.node_pos[i])]);
         }
         g.node_rec = NodeRec{int(m.n_node()), put(A, ids), put(A, ur), put(A, bus)};
     }
@@ -429,7 +435,7 @@ void upload_group(Batch& b, Group& g) {
                 ids.push_back(id_of(k));
                 math.push_back(math_of(k));
                 Idx const node = node_of(k), gg = t.node_group[node];
-                bus.push_back(gg < 0 ? -1 : p.mm_bus_begin[gg] + int(t.node_pos[node]));
+                bus.push_back(gg < 0 ? -1 : p.bus_label[p.mm_bus_begin[gg] + int(t.node_pos[node])]);
                 bi.push_back(kBaseP3 / m.node[node].u_rated / kSqrt3);
                 stt.push_back(stat_of(k));
             }
@@ -917,6 +923,11 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
                     g.packed = !g.is_union && g.scenarios.size() <= size_t(kPackedMax);
                     int const target = g.packed ? packed_target() : std::clamp(4096 / std::max(nblk, 1), 4, 256);
                     g.plan = build_plan(m, g.st, g.topo, all, sym, target);
+                    if (g.packed) {
+                        // lanes of a packed wave run different tasks: interleave them in memory
+                        interleave_plan(g.plan, kWaveLanes / int(g.scenarios.size()));
+                        check_plan(g.plan);
+                    }
                 } catch (std::exception const& e) {
                     errs[gi] = e.what();
                 }
@@ -1357,6 +1368,7 @@ int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices
             if (it == b1 || *it != int(r)) return -1;  // the pattern must be symmetric
             p.transpose[k] = int(it - p.col.begin());
         }
+    p.row_end.assign(p.row_ptr.begin() + 1, p.row_ptr.end());
     build_schedule(p, 16);
     hipStream_t st;
     PGM_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -1376,6 +1388,7 @@ int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices
     d.S_pad = S_pad;
     d.fp32 = fp32 ? 1 : 0;
     d.row_ptr = put(A, p.row_ptr);
+    d.row_end = put(A, p.row_end);
     d.col = put(A, p.col);
     d.diag = put(A, p.diag);
     d.transpose = put(A, p.transpose);

@@ -354,7 +354,8 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
 
     for (int ti = g.task_ptr[task]; ti < g.task_ptr[task + 1]; ++ti) {
         int const p = g.task_pivot[ti];
-        int const rb = g.row_ptr[p], re = g.row_ptr[p + 1], dg = g.diag[p];
+        int const rb = g.row_ptr[p], re = g.row_end[p], dg = g.diag[p];
+        int const es = g.row_stride ? g.row_stride[p] : 1;
         // a bus without a source in this lane (union groups): identity row, zero row and column
         bool const dead_p = UNI && g.dead[size_t(p) * sp + s] != 0;
         // rows with one U entry and no off-diagonal Schur target (every bus of a radial feeder) keep
@@ -407,7 +408,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
             for (int k = 0; k < NB2; ++k) D[k] = g.raw_a[(size_t(dg) * NB2 + k) * sp + s];
 #pragma unroll
             for (int k = 0; k < NB; ++k) rhs[k] = g.raw_b[(size_t(p) * NB + k) * sp + s];
-            for (int k = dg + 1; k < re; ++k) {
+            for (int k = dg + es; k < re; k += es) {
                 FT a[NB2];
 #pragma unroll
                 for (int q = 0; q < NB2; ++q) a[q] = FT(g.raw_a[(size_t(k) * NB2 + q) * sp + s]);
@@ -419,7 +420,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
             }
         } else if constexpr (MODE == MODE_INIT) {
             // admittance matrix in the real domain plus load admittances and sources
-            for (int k = dg; k < re; ++k) {
+            for (int k = dg; k < re; k += es) {
                 FT a[NB2];
                 if (g.fill[k] || dead_p) {
 #pragma unroll
@@ -496,7 +497,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
             for (int ph = 0; ph < P; ++ph) r[ph] = {0.0, 0.0};
             if (dead_p) {
                 if constexpr (MODE == MODE_ICF) {
-                    for (int k = dg + 1; k < re; ++k) {
+                    for (int k = dg + es; k < re; k += es) {
                         zero_u(k);
                         zero_l(g.transpose[k]);
                     }
@@ -504,7 +505,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                     for (int q = 0; q < NB2; ++q) D[q] = (q % (NB + 1) == 0) ? 1.0 : 0.0;
                 }
             } else {
-                for (int k = rb; k < re; ++k) {
+                for (int k = rb; k < re; k += es) {
                     int const j = g.col[k];
                     if (g.fill[k]) {
                         if constexpr (MODE == MODE_ICF) {
@@ -598,7 +599,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
             }
             if constexpr (MODE == MODE_RESID) {
                 // stored factors: y_p = L_pp^-1 P (r_p - sum_k L_pk y_k)
-                for (int k = rb; k < dg; ++k) {
+                for (int k = rb; k < dg; k += es) {
                     int const j = g.col[k];
                     FT L[NB2];
                     lu.load(k, L);
@@ -623,7 +624,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                 continue;
             }
         } else if (dead_p) {
-            for (int k = dg + 1; k < re; ++k) {
+            for (int k = dg + es; k < re; k += es) {
                 zero_u(k);
                 zero_l(g.transpose[k]);
             }
@@ -636,7 +637,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
             double pcal[P], qcal[P];
 #pragma unroll
             for (int ph = 0; ph < P; ++ph) pcal[ph] = qcal[ph] = 0.0;
-            for (int k = rb; k < re; ++k) {
+            for (int k = rb; k < re; k += es) {
                 int const j = g.col[k];
                 if (g.fill[k]) {
                     if (j > p) {
@@ -795,7 +796,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
         lu.store(dg, Df);
         g.perm[size_t(p) * sp + s] = code;
         // U row: U_pj = L^-1 P A_pj ; L column: L_ip = A_ip Q U^-1
-        for (int k = dg + 1; k < re; ++k) {
+        for (int k = dg + es; k < re; k += es) {
             FT A[NB2];
             if (keep) {
 #pragma unroll
@@ -844,11 +845,12 @@ __global__ __launch_bounds__(kWave) void backward_kernel(DevGroup const* __restr
     double dev = 0.0;
     for (int ti = g.task_ptr[task + 1] - 1; ti >= g.task_ptr[task]; --ti) {
         int const p = g.task_pivot[ti];
-        int const re = g.row_ptr[p + 1], dg = g.diag[p];
+        int const re = g.row_end[p], dg = g.diag[p];
+        int const es = g.row_stride ? g.row_stride[p] : 1;
         double v[NB];
 #pragma unroll
         for (int r = 0; r < NB; ++r) v[r] = g.y[(size_t(p) * NB + r) * sp + s];
-        for (int k = dg + 1; k < re; ++k) {
+        for (int k = dg + es; k < re; k += es) {
             int const j = g.col[k];
             FT U[NB2];
             lu.load(k, U);
@@ -980,10 +982,11 @@ __global__ void ic_start_kernel(DevGroup const* __restrict__ groups, long long c
     int const b = int(idx / sp), s = int(idx % sp);
     if (s >= g.S) return;
     int const mm = g.bus_mm[b];
-    if (g.done[size_t(mm) * sp + s] != ACTIVE) return;
-    int const q0 = g.src_ptr[g.mm_bus_begin[mm]], q1 = g.src_ptr[g.mm_bus_begin[mm + 1]];
+    if (mm < 0 || g.done[size_t(mm) * sp + s] != ACTIVE) return;
+    int const q0 = g.mm_src_ptr[mm], q1 = g.mm_src_ptr[mm + 1];
     cplx sum = {0.0, 0.0};
-    for (int q = q0; q < q1; ++q) {
+    for (int qi = q0; qi < q1; ++qi) {
+        int const q = g.mm_src[qi];
         cplx const ur = {g.u_src[(size_t(q) * 2) * sp + s], g.u_src[(size_t(q) * 2 + 1) * sp + s]};
         double sn, cs;
         sincos(-g.phase[g.src_bus[q]], &sn, &cs);
@@ -1015,7 +1018,7 @@ __global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long
     size_t const sp = o.S_pad;
     int const b = int(idx / sp);
     int const s = int(idx % sp);
-    if (s >= o.S || !o.ok[s]) return;
+    if (s >= o.S || !o.ok[s] || o.bus_mm[b] < 0) return;
     if (o.dead && o.dead[size_t(b) * sp + s]) {
         // not energised in this lane: the appliances of the bus carry nothing
         for (int l = o.load_ptr[b]; l < o.load_ptr[b + 1]; ++l)
@@ -1058,7 +1061,8 @@ __global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long
         cplx inj[P];
 #pragma unroll
         for (int ph = 0; ph < P; ++ph) inj[ph] = {0.0, 0.0};
-        for (int k = o.row_ptr[b]; k < o.row_ptr[b + 1]; ++k) {
+        int const es = o.row_stride ? o.row_stride[b] : 1;
+        for (int k = o.row_ptr[b]; k < o.row_end[b]; k += es) {
             if (o.fill[k]) continue;
             int const j = o.col[k];
             double y[P * P * 2];

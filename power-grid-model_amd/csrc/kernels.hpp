@@ -28,8 +28,11 @@ struct DevGroup {
     int S = 0;         // scenarios in the group
     int S_pad = 0;     // S rounded up to 64
     int fp32 = 0;      // factor storage precision
-    // structure (shared)
+    // structure (shared); row p's entries: row_ptr[p], + stride, ... < row_end[p], stride =
+    // row_stride[p] (nullptr: 1)
     int const* row_ptr = nullptr;
+    int const* row_end = nullptr;
+    int const* row_stride = nullptr;
     int const* col = nullptr;
     int const* diag = nullptr;
     int const* transpose = nullptr;
@@ -51,7 +54,8 @@ struct DevGroup {
     // bus of every source
     double const* phase = nullptr;
     int const* bus_mm = nullptr;
-    int const* mm_bus_begin = nullptr;
+    int const* mm_src_ptr = nullptr;  // sources of every math model: mm_src[mm_src_ptr[k] ...]
+    int const* mm_src = nullptr;
     int const* src_bus = nullptr;
     int const* load_ptr = nullptr;
     int8_t const* load_type = nullptr;
@@ -125,6 +129,9 @@ struct DevOutput {
     // buses / Y rows (Y entries = non-fill LU entries)
     int n = 0;
     int const* row_ptr = nullptr;
+    int const* row_end = nullptr;
+    int const* row_stride = nullptr;  // as in DevGroup
+    int const* bus_mm = nullptr;     // -1 for an unused bus label
     int const* col = nullptr;
     int8_t const* fill = nullptr;
     double const* ydata = nullptr;

@@ -2,7 +2,9 @@
 #include "topology.hpp"
 
 #include <algorithm>
+#include <limits>
 #include <numeric>
+#include <string>
 #include <set>
 #include <stdexcept>
 
@@ -564,8 +566,208 @@ Plan build_plan(Model const& m, ComponentState const& st, Topology const& topo, 
     std::partial_sum(p.load_ptr.begin(), p.load_ptr.end(), p.load_ptr.begin());
     std::partial_sum(p.src_ptr.begin(), p.src_ptr.end(), p.src_ptr.begin());
 
+    p.row_end.assign(p.row_ptr.begin() + 1, p.row_ptr.end());
+    p.bus_label.resize(n);
+    std::iota(p.bus_label.begin(), p.bus_label.end(), 0);
+    p.mm_src_ptr.assign(p.n_mm + 1, 0);
+    for (Idx b = 0; b < n; ++b) p.mm_src_ptr[p.bus_mm[b] + 1] += p.src_ptr[b + 1] - p.src_ptr[b];
+    std::partial_sum(p.mm_src_ptr.begin(), p.mm_src_ptr.end(), p.mm_src_ptr.begin());
+    p.mm_src.resize(p.src_comp.size());
+    {
+        std::vector<int> fillp(p.mm_src_ptr.begin(), p.mm_src_ptr.end() - 1);
+        for (Idx b = 0; b < n; ++b)
+            for (int q = p.src_ptr[b]; q < p.src_ptr[b + 1]; ++q) p.mm_src[fillp[p.bus_mm[b]]++] = q;
+    }
+
     build_schedule(p, target_parallel_tasks);
     return p;
+}
+
+void interleave_plan(Plan& p, int tpw) {
+    tpw = std::max(tpw, 1);
+    Idx const n = p.n;
+    int const PP = p.P * p.P;
+    auto row_stride = [&](int b) { return p.row_stride.empty() ? 1 : p.row_stride[b]; };
+    auto row_len = [&](int b) { return (p.row_end[b] - p.row_ptr[b]) / row_stride(b); };
+    std::vector<int> newbus(n, -1), stride_of(n, 1);
+    std::vector<long long> newent(p.E, -1);
+    long long B = 0, Ent = 0;
+    for (size_t l = 0; l + 1 < p.level_ptr.size(); ++l)
+        for (int q0 = p.level_ptr[l]; q0 < p.level_ptr[l + 1]; q0 += tpw) {
+            int const q1 = std::min(q0 + tpw, p.level_ptr[l + 1]);
+            int K = 0, R = 0;
+            for (int q = q0; q < q1; ++q) {
+                int const tk = p.level_task[q];
+                K = std::max(K, p.task_ptr[tk + 1] - p.task_ptr[tk]);
+                for (int i = p.task_ptr[tk]; i < p.task_ptr[tk + 1]; ++i) R = std::max(R, row_len(p.task_pivot[i]));
+            }
+            // the chunk's stride is its task count: full chunks interleave 64 / S lanes, a one-task chunk
+            // stays contiguous
+            int const nt = q1 - q0;
+            for (int q = q0; q < q1; ++q) {
+                int const t = q - q0, tk = p.level_task[q];
+                for (int k = 0; k < p.task_ptr[tk + 1] - p.task_ptr[tk]; ++k) {
+                    int const piv = p.task_pivot[p.task_ptr[tk] + k];
+                    newbus[piv] = int(B + (long long)(k) * nt + t);
+                    stride_of[piv] = nt;
+                    for (int j = 0; j < row_len(piv); ++j)
+                        newent[p.row_ptr[piv] + j * row_stride(piv)] = Ent + ((long long)(k) * R + j) * nt + t;
+                }
+            }
+            B += (long long)(K) * nt;
+            Ent += (long long)(K) * R * nt;
+        }
+    if (B > std::numeric_limits<int>::max() / 2 || Ent > std::numeric_limits<int>::max() / 2)
+        throw std::logic_error("interleaved plan too large");
+    for (Idx b = 0; b < n; ++b)
+        if (newbus[b] < 0) throw std::logic_error("interleave: bus without a task");
+    int const nL = int(B), EL = int(Ent);
+    std::vector<int> oldbus(nL, -1);
+    for (Idx b = 0; b < n; ++b) oldbus[newbus[b]] = int(b);
+    // buses
+    std::vector<int> row_ptr(nL + 1, 0), row_end(nL, 0), row_str(nL, 1), diag(nL, 0), bus_mm(nL, -1);
+    std::vector<int8_t> keep(nL, 0);
+    std::vector<double> phase(nL, 0.0);
+    for (Idx b = 0; b < n; ++b) {
+        int const L = newbus[b];
+        row_ptr[L] = int(newent[p.row_ptr[b]]);
+        row_end[L] = row_ptr[L] + row_len(int(b)) * stride_of[b];
+        row_str[L] = stride_of[b];
+        diag[L] = int(newent[p.diag[b]]);
+        keep[L] = p.keep[b];
+        bus_mm[L] = p.bus_mm[b];
+        phase[L] = p.phase[b];
+    }
+    row_ptr[nL] = EL;
+    // entries (fill-flagged zero entries for the padding positions)
+    std::vector<int> col(EL, 0), transpose(EL, 0);
+    std::vector<int8_t> fill(EL, 1);
+    std::vector<cd> ydata(size_t(EL) * PP, cd{});
+    for (int e = 0; e < p.E; ++e) {
+        if (newent[e] < 0) continue;  // not an entry of any row (a padding position of the old plan)
+        int const E2 = int(newent[e]);
+        col[E2] = newbus[p.col[e]];
+        transpose[E2] = int(newent[p.transpose[e]]);
+        fill[E2] = p.fill[e];
+        for (int q = 0; q < PP; ++q) ydata[size_t(E2) * PP + q] = p.ydata[size_t(e) * PP + q];
+    }
+    // Schur lists in new bus order
+    std::vector<int> schur_ptr(nL + 1, 0);
+    std::vector<std::array<int, 3>> schur;
+    schur.reserve(p.schur.size());
+    for (int L = 0; L < nL; ++L) {
+        int const b = oldbus[L];
+        if (b >= 0)
+            for (int q = p.schur_ptr[b]; q < p.schur_ptr[b + 1]; ++q)
+                schur.push_back({int(newent[p.schur[q][0]]), int(newent[p.schur[q][1]]), int(newent[p.schur[q][2]])});
+        schur_ptr[L + 1] = int(schur.size());
+    }
+    for (auto& v : p.task_pivot) v = newbus[v];
+    // appliances regrouped in new bus order
+    auto regroup = [&](std::vector<int>& ptr, auto&& move_item) {
+        std::vector<int> nptr(nL + 1, 0);
+        int cnt = 0;
+        for (int L = 0; L < nL; ++L) {
+            int const b = oldbus[L];
+            if (b >= 0)
+                for (int q = ptr[b]; q < ptr[b + 1]; ++q) move_item(q, cnt++);
+            nptr[L + 1] = cnt;
+        }
+        ptr = std::move(nptr);
+    };
+    {
+        auto lt = p.load_type;
+        auto lc = p.load_comp;
+        regroup(p.load_ptr, [&](int q, int i) {
+            p.load_type[i] = lt[q];
+            p.load_comp[i] = lc[q];
+        });
+    }
+    {
+        auto sc = p.src_comp;
+        regroup(p.src_ptr, [&](int q, int i) { p.src_comp[i] = sc[q]; });
+    }
+    {
+        auto hc = p.shunt_comp;
+        auto hy = p.shunt_y;
+        regroup(p.shunt_ptr, [&](int q, int i) {
+            p.shunt_comp[i] = hc[q];
+            for (int k = 0; k < PP; ++k) p.shunt_y[size_t(i) * PP + k] = hy[size_t(q) * PP + k];
+        });
+    }
+    for (auto& v : p.branch_from) v = v >= 0 ? newbus[v] : -1;
+    for (auto& v : p.branch_to) v = v >= 0 ? newbus[v] : -1;
+    for (auto& v : p.bus_label) v = newbus[v];
+    p.row_ptr = std::move(row_ptr);
+    p.row_end = std::move(row_end);
+    p.row_stride = std::move(row_str);
+    p.diag = std::move(diag);
+    p.keep = std::move(keep);
+    p.bus_mm = std::move(bus_mm);
+    p.phase = std::move(phase);
+    p.col = std::move(col);
+    p.transpose = std::move(transpose);
+    p.fill = std::move(fill);
+    p.ydata = std::move(ydata);
+    p.schur_ptr = std::move(schur_ptr);
+    p.schur = std::move(schur);
+    p.n = nL;
+    p.E = EL;
+    // sources of every math model, in the new source order
+    p.mm_src_ptr.assign(p.n_mm + 1, 0);
+    for (int L = 0; L < nL; ++L)
+        if (p.bus_mm[L] >= 0) p.mm_src_ptr[p.bus_mm[L] + 1] += p.src_ptr[L + 1] - p.src_ptr[L];
+    std::partial_sum(p.mm_src_ptr.begin(), p.mm_src_ptr.end(), p.mm_src_ptr.begin());
+    p.mm_src.assign(p.src_comp.size(), 0);
+    std::vector<int> fillp(p.mm_src_ptr.begin(), p.mm_src_ptr.end() - 1);
+    for (int L = 0; L < nL; ++L)
+        for (int q = p.src_ptr[L]; q < p.src_ptr[L + 1]; ++q) p.mm_src[fillp[p.bus_mm[L]]++] = q;
+}
+
+void check_plan(Plan const& p) {
+    auto fail = [](char const* what) { throw std::logic_error(std::string("plan check: ") + what); };
+    auto stride = [&](int r) { return p.row_stride.empty() ? 1 : p.row_stride[r]; };
+    auto in_row = [&](int r, int e) {
+        return e >= p.row_ptr[r] && e < p.row_end[r] && (e - p.row_ptr[r]) % stride(r) == 0;
+    };
+    std::vector<char> pivot(p.n, 0);
+    for (int tk = 0; tk + 1 < int(p.task_ptr.size()); ++tk)
+        for (int i = p.task_ptr[tk]; i < p.task_ptr[tk + 1]; ++i) {
+            int const b = p.task_pivot[i];
+            if (b < 0 || b >= p.n || pivot[b]) fail("task pivots");
+            pivot[b] = 1;
+        }
+    for (int r = 0; r < p.n; ++r) {
+        if (!pivot[r]) {
+            if (p.bus_mm[r] >= 0) fail("bus not in a task");
+            continue;
+        }
+        // L entries (columns eliminated before r) precede the diagonal, U entries follow it in
+        // increasing column order
+        int prev = r, ndiag = 0;
+        for (int k = p.row_ptr[r]; k < p.row_end[r]; k += stride(r)) {
+            int const c = p.col[k];
+            if (k < p.diag[r] && c >= r) fail("L entry after the diagonal column");
+            if (k > p.diag[r]) {
+                if (c <= prev) fail("U part not sorted");
+                prev = c;
+            }
+            if (c == r) {
+                ++ndiag;
+                if (k != p.diag[r]) fail("diagonal");
+            }
+            int const t = p.transpose[k];
+            if (p.transpose[t] != k || p.col[t] != r || !in_row(c, t)) fail("transpose");
+            if (p.fill[t] != p.fill[k]) fail("fill symmetry");
+        }
+        if (ndiag != 1) fail("one diagonal per row");
+        for (int q = p.schur_ptr[r]; q < p.schur_ptr[r + 1]; ++q) {
+            int const tgt = p.schur[q][0], le = p.schur[q][1], ue = p.schur[q][2];
+            int const kk = p.col[le];
+            if (!in_row(kk, ue)) fail("Schur U entry outside its pivot row");
+            if (!(in_row(r, tgt) || p.col[tgt] == r)) fail("Schur target");
+        }
+    }
 }
 
 }  // namespace pgm_amd

@@ -45,10 +45,15 @@ struct Plan {
     Idx E = 0;          // LU entries
     Idx n_mm = 0;
     std::vector<int> mm_bus_begin;  // n_mm + 1
-    std::vector<int> row_ptr, col, diag, transpose;
+    // row p's entries: row_ptr[p], row_ptr[p] + stride, ... < row_end[p], stride = row_stride[p] (empty:
+    // 1, plain CSR)
+    std::vector<int> row_ptr, row_end, row_stride, col, diag, transpose;
     std::vector<int8_t> fill;  // entry is a fill-in
-    std::vector<int> bus_mm;
+    std::vector<int> bus_mm;    // math model of every bus, -1 for an unused (padding) bus label
     std::vector<double> phase;  // phase shift of every bus
+    std::vector<int> mm_src_ptr, mm_src;  // sources of every math model
+    // label of every bus in math-model order (mm_bus_begin[k] + position); identity unless interleaved
+    std::vector<int> bus_label;
     // Y values per LU entry (P*P complex, row-major), zero for fill-ins
     std::vector<cd> ydata;
     // Schur updates in pull form, per pivot: (target, l, u) entry triples, k ascending per target
@@ -76,5 +81,14 @@ void build_schedule(Plan& p, int target_parallel_tasks);
 
 Plan build_plan(Model const& m, ComponentState const& st, Topology const& topo, std::vector<Idx> const& models,
                 bool symmetric, int target_parallel_tasks);
+
+// Relabel buses and LU entries of a plan for waves that run tasks_per_wave tasks of one level side by
+// side (packed groups): in a wave chunk of nt tasks the k-th pivot of task t becomes bus B0 + k nt + t
+// and its row entries E0 + (k R + j) nt + t, so lanes at the same step touch consecutive addresses.
+void interleave_plan(Plan& p, int tasks_per_wave);
+
+// Structural consistency of a plan (rows sorted, one diagonal, transposes paired, Schur triples
+// inside the pivot's row / column, task pivots cover the buses); throws std::logic_error
+void check_plan(Plan const& p);
 
 }  // namespace pgm_amd

@@ -79,7 +79,7 @@ def _load():
         "PGM_amd_batch_iterations": (None, [_p, _p]),
         "PGM_amd_batch_stats": (None, [_p, _p]),
         "PGM_amd_inspect_topology": (Idx, [_p, _p, _p, _p, _p, Idx]),
-        "PGM_amd_plan_stats": (None, [_p, _p, Idx, Idx, _p]),
+        "PGM_amd_plan_stats": (None, [_p, _p, Idx, Idx, Idx, _p]),
         "PGM_amd_minimum_degree_ordering": (Idx, [Idx, _p, _p, _p, _p, _p, _p, Idx]),
         "PGM_amd_block_lu_solve": (C.c_int, [Idx, Idx, _p, _p, _p, Idx, _p, _p, _p, _p, Idx]),
         "PGM_amd_copy_probe": (C.c_int, [Idx]),

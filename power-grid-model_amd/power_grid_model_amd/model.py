@@ -406,11 +406,14 @@ def inspect_topology(model: PowerGridModel, n_node: int):
     return group, pos, [tuple(fills[3 * i:3 * i + 3].tolist()) for i in range(nf)]
 
 
-def plan_stats(model: PowerGridModel, symmetric: bool = True, target_parallel_tasks: int = 64) -> dict:
-    """Host-only statistics of the device factorisation schedule of the model's base topology."""
+def plan_stats(model: PowerGridModel, symmetric: bool = True, target_parallel_tasks: int = 64,
+               packed_lanes: int = 0) -> dict:
+    """Host-only statistics of the device factorisation schedule of the model's base topology;
+    packed_lanes > 0 also checks the interleaved layout of a packed group of that many scenarios."""
     out = np.zeros(7, dtype=np.int64)
     h = model._handle
-    lib.PGM_amd_plan_stats(h.ptr, model._ptr, int(symmetric), target_parallel_tasks, out.ctypes.data)
+    lib.PGM_amd_plan_stats(h.ptr, model._ptr, int(symmetric), target_parallel_tasks, packed_lanes,
+                           out.ctypes.data)
     h.check()
     keys = ["buses", "lu_entries", "levels", "tasks", "schur_updates", "critical_path", "valid"]
     return dict(zip(keys, out.tolist()))

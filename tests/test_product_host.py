@@ -207,9 +207,30 @@ def test_topology_matches_oracle_on_random_grids(seed):
                                   lambda: synthetic.feeders_64(n_scenarios=1, n_feeder=8)],
                          ids=["radial_200", "meshed_1000", "feeders_8"])
 @pytest.mark.parametrize("target", [4, 64, 256])
-def test_schedule_respects_dependencies(make, target):
+@pytest.mark.parametrize("packed_lanes", [0, 1, 3, 32])
+def test_schedule_respects_dependencies(make, target, packed_lanes):
+    """The schedule respects every dependency of the block LU; the interleaved layout of packed groups
+    (1, 3, 32 scenarios) keeps rows, diagonals, transposes and Schur triples consistent."""
     case = make()
-    st = plan_stats(PowerGridModel(case.input), True, target)
+    st = plan_stats(PowerGridModel(case.input), True, target, packed_lanes)
     assert st["valid"] == 1
     assert st["tasks"] >= 1 and st["levels"] >= 1
     assert st["critical_path"] <= st["buses"]
+
+
+def test_interleaved_layout_of_switching_topologies():
+    """Packed layout of year-long switching topologies (ties closed, feeder lines open, cyclic parts)."""
+    case = synthetic.year_20k(n_scenarios=200)
+    ind, data = case.update["line"]["indptr"], case.update["line"]["data"]
+    checked = 0
+    for s in np.flatnonzero(np.diff(ind) > 0)[:4]:
+        inp = {k: v.copy() for k, v in case.input.items()}
+        line = inp["line"]
+        for r in data[ind[s]:ind[s + 1]]:
+            k = np.flatnonzero(line["id"] == r["id"])[0]
+            line["from_status"][k] = r["from_status"]
+            line["to_status"][k] = r["to_status"]
+        st = plan_stats(PowerGridModel(inp), True, 256, 1)
+        assert st["valid"] == 1
+        checked += 1
+    assert checked == 4
