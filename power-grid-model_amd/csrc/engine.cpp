@@ -737,6 +737,7 @@ Batch::~Batch() {
     if (ev_stop) (void)hipEventDestroy(ev_stop);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
+    if (ev_lead) (void)hipEventDestroy(ev_lead);
     if (stream_u && stream_u != stream) (void)hipStreamDestroy(stream_u);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -778,6 +779,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         PGM_HIP_CHECK(hipStreamCreateWithFlags(&b.stream_u, hipStreamNonBlocking));
     PGM_HIP_CHECK(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
     PGM_HIP_CHECK(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
+    PGM_HIP_CHECK(hipEventCreateWithFlags(&b.ev_lead, hipEventDisableTiming));
     PGM_HIP_CHECK(hipEventCreate(&b.ev_start));
     PGM_HIP_CHECK(hipEventCreate(&b.ev_stop));
     PGM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b.h_active), sizeof(int)));
@@ -1145,11 +1147,18 @@ void Batch::run() {
         PGM_HIP_CHECK(hipEventRecord(ev_join, su));
         PGM_HIP_CHECK(hipStreamWaitEvent(st, ev_join, 0));
     };
+    // the second stream's first forward level runs before the main stream starts its sweeps: the
+    // second stream's long upper-level chains then overlap the main stream's wide levels
+    bool const lead = su != st && LU + LP > 0;
     auto union_sweeps = [&](int mode) {
         // packed and union groups interleaved level by level on the second stream
         for (int l = 0; l < std::max(LU, LP); ++l) {
             if (l < LP) launch_forward(d_groups, d_items_p[l], n_items_p[l], P, fp32, mode, su, KIND_PACKED);
             if (l < LU) launch_forward(d_groups, d_items_u[l], n_items_u[l], P, fp32, mode, su, KIND_UNION);
+            if (l == 0 && lead) {
+                PGM_HIP_CHECK(hipEventRecord(ev_lead, su));
+                PGM_HIP_CHECK(hipStreamWaitEvent(st, ev_lead, 0));
+            }
         }
         for (int l = std::max(LU, LP) - 1; l >= 0; --l) {
             if (l < LP) launch_backward(d_groups, d_items_p[l], n_items_p[l], P, fp32, mode, su, KIND_PACKED);

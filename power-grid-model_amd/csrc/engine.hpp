@@ -55,7 +55,7 @@ struct Batch {
     hipStream_t stream = nullptr;
     hipStream_t stream_u = nullptr;  // union groups
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_lead = nullptr;
     Arena arena;
     std::vector<std::unique_ptr<Group>> groups;
     // device tables over the groups
