@@ -351,6 +351,14 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
     int const mm = g.task_mm[task];
     if (g.done[size_t(mm) * sp + s] != ACTIVE) return;
     Blocks<NB, FT> const lu{static_cast<FT*>(g.lu), sp, s};
+    // a keep row whose parent is the next pivot of the task hands its finished U_p,parent,
+    // L_parent,p and y_p to that pivot through LDS (fw_le: label of the forwarded L entry); LDS keeps
+    // the hand-over off the vector registers
+    constexpr bool kFwd = P == 1;
+    __shared__ FT fw_blk[kFwd ? kWave : 1][kFwd ? 2 * NB2 : 1];
+    __shared__ double fw_y[kFwd ? kWave : 1][kFwd ? NB : 1];
+    int const lane = int(threadIdx.x);
+    int fw_le = -1;
 
     for (int ti = g.task_ptr[task]; ti < g.task_ptr[task + 1]; ++ti) {
         int const p = g.task_pivot[ti];
@@ -747,8 +755,17 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
         for (int q = g.schur_ptr[p]; q < g.schur_ptr[p + 1]; ++q) {
             int const tgt = g.schur[3 * q], le = g.schur[3 * q + 1], ue = g.schur[3 * q + 2];
             FT L[NB2], U[NB2];
-            lu.load(le, L);
-            lu.load(ue, U);
+            bool const fwd = kFwd && le == fw_le;
+            if (fwd) {
+#pragma unroll
+                for (int k = 0; k < NB2; ++k) {
+                    L[k] = fw_blk[lane][k];
+                    U[k] = fw_blk[lane][NB2 + k];
+                }
+            } else {
+                lu.load(le, L);
+                lu.load(ue, U);
+            }
             if (tgt == dg) {
 #pragma unroll
                 for (int c = 0; c < NB; ++c)
@@ -759,10 +776,15 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                         for (int k = 0; k < NB; ++k) acc += L[k * NB + r] * U[c * NB + k];
                         Df[c * NB + r] -= acc;
                     }
-                int const j = g.col[le];
                 double yj[NB];
+                if (fwd) {
 #pragma unroll
-                for (int r = 0; r < NB; ++r) yj[r] = g.y[(size_t(j) * NB + r) * sp + s];
+                    for (int r = 0; r < NB; ++r) yj[r] = fw_y[lane][r];
+                } else {
+                    int const j = g.col[le];
+#pragma unroll
+                    for (int r = 0; r < NB; ++r) yj[r] = g.y[(size_t(j) * NB + r) * sp + s];
+                }
 #pragma unroll
                 for (int r = 0; r < NB; ++r) {
                     double acc = 0.0;
@@ -807,6 +829,10 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
             apply_p_rows<NB, FT>(A, code);
             lower_left<NB, FT>(Df, A);
             lu.store(k, A);
+            if (kFwd && keep) {
+#pragma unroll
+                for (int q = 0; q < NB2; ++q) fw_blk[lane][NB2 + q] = A[q];
+            }
             int const e = g.transpose[k];
             if (keep) {
 #pragma unroll
@@ -817,12 +843,27 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
             apply_q_cols<NB, FT>(A, code);
             upper_right<NB, FT>(Df, A);
             lu.store(e, A);
+            if (kFwd && keep) {
+#pragma unroll
+                for (int q = 0; q < NB2; ++q) fw_blk[lane][q] = A[q];
+            }
         }
         // forward substitution: y_p = L_pp^-1 P (b_p - sum_k L_pk y_k)
         apply_p_vec<NB>(rhs, code);
         lower_vec<NB, FT>(Df, rhs);
 #pragma unroll
         for (int r = 0; r < NB; ++r) g.y[(size_t(p) * NB + r) * sp + s] = rhs[r];
+        if constexpr (kFwd) {
+            fw_le = -1;
+            if (keep && ti + 1 < g.task_ptr[task + 1]) {
+                int const k1 = dg + es;  // the row's only U entry
+                if (g.task_pivot[ti + 1] == g.col[k1]) {
+                    fw_le = g.transpose[k1];
+#pragma unroll
+                    for (int r = 0; r < NB; ++r) fw_y[lane][r] = rhs[r];
+                }
+            }
+        }
     }
 }
 
@@ -843,6 +884,11 @@ __global__ __launch_bounds__(kWave) void backward_kernel(DevGroup const* __restr
     if (g.done[size_t(mm) * sp + s] != ACTIVE) return;
     Blocks<NB, FT> const lu{static_cast<FT*>(g.lu), sp, s};
     double dev = 0.0;
+    // x of the pivot solved last, handed to the next pivot through LDS (its parent's value when the
+    // next pivot is that pivot's first child)
+    __shared__ double fx[kWave][NB];
+    int const lane = int(threadIdx.x);
+    int fx_bus = -1;
     for (int ti = g.task_ptr[task + 1] - 1; ti >= g.task_ptr[task]; --ti) {
         int const p = g.task_pivot[ti];
         int const re = g.row_end[p], dg = g.diag[p];
@@ -855,8 +901,13 @@ __global__ __launch_bounds__(kWave) void backward_kernel(DevGroup const* __restr
             FT U[NB2];
             lu.load(k, U);
             double xj[NB];
+            if (j == fx_bus) {
 #pragma unroll
-            for (int r = 0; r < NB; ++r) xj[r] = g.y[(size_t(j) * NB + r) * sp + s];
+                for (int r = 0; r < NB; ++r) xj[r] = fx[lane][r];
+            } else {
+#pragma unroll
+                for (int r = 0; r < NB; ++r) xj[r] = g.y[(size_t(j) * NB + r) * sp + s];
+            }
 #pragma unroll
             for (int r = 0; r < NB; ++r) {
                 double acc = 0.0;
@@ -871,6 +922,9 @@ __global__ __launch_bounds__(kWave) void backward_kernel(DevGroup const* __restr
         apply_q_vec<NB>(v, g.perm[size_t(p) * sp + s]);
 #pragma unroll
         for (int r = 0; r < NB; ++r) g.y[(size_t(p) * NB + r) * sp + s] = v[r];
+        fx_bus = p;
+#pragma unroll
+        for (int r = 0; r < NB; ++r) fx[lane][r] = v[r];
         if constexpr (MODE == MODE_INIT) {
 #pragma unroll
             for (int ph = 0; ph < P; ++ph) {

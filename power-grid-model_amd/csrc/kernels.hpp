@@ -196,9 +196,9 @@ struct ApplRec {
     int8_t const* status_const;
     double const* si;     // [n_math][2][P][2][S_pad]
 };
-// components per record tile (64 lanes x TN records staged in at most 48 KiB of LDS)
+// components per record tile (64 lanes x TN records staged in at most 24 KiB of LDS)
 constexpr int record_tile(size_t record_bytes) {
-    return int(49152 / (64 * record_bytes)) > 0 ? int(49152 / (64 * record_bytes)) : 1;
+    return int(24576 / (64 * record_bytes)) > 0 ? int(24576 / (64 * record_bytes)) : 1;
 }
 // flat launches over all groups: offsets[g] = first flat index of group g (n_groups + 1 entries); the
 // record writers count tiles (ceil(S_pad / W) x ceil(n_comp / (64 record_tile / W)) per group, with
