@@ -29,6 +29,25 @@ namespace dev {
 
 constexpr int kWave = 64;
 
+// Group descriptors, work lists and structure arrays are written by the host before the launches
+// that read them. Reading them through the constant address space lets the compiler use scalar loads
+// where the index is the same on every lane, and a pointer field read from a descriptor this way is a
+// global-memory pointer, so the per-lane data it addresses is reached with global (not flat) accesses.
+// (The host pass only parses kernel bodies; it sees plain loads.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T> using const_ptr = T const __attribute__((address_space(4)))*;
+template <class T, class I> __device__ __forceinline__ T ld(T const* p, I i) { return ((const_ptr<T>)p)[i]; }
+// byte arrays: the aligned 32-bit word holding the byte (scalar loads read whole dwords; every device
+// array starts on a 256-byte boundary and is padded to a multiple of 256 bytes, so the word is inside it)
+template <class I> __device__ __forceinline__ int8_t ld(int8_t const* p, I i) {
+    uintptr_t const a = reinterpret_cast<uintptr_t>(p) + uintptr_t(i);
+    uint32_t const w = ((const_ptr<uint32_t>)(a & ~uintptr_t(3)))[0];
+    return int8_t(w >> ((a & 3) * 8));
+}
+#else
+template <class T, class I> __device__ T ld(T const* p, I i) { return p[i]; }
+#endif
+
 struct cplx {
     double re, im;
 };
@@ -310,7 +329,7 @@ __device__ __forceinline__ void load_y(DevGroup const& g, int k, int s, double (
         for (int q = 0; q < Q; ++q) y[q] = g.ylane[(size_t(k) * Q + q) * sp + s];
     } else {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) y[q] = g.ydata[size_t(k) * Q + q];
+        for (int q = 0; q < Q; ++q) y[q] = ld(g.ydata, size_t(k) * Q + q);
     }
 }
 
@@ -323,7 +342,7 @@ __device__ __forceinline__ bool lane_work(DevGroup const& g, WorkItem const& wi,
         int const t = int(threadIdx.x) / g.S;
         s = int(threadIdx.x) - t * g.S;
         if (t >= wi.block) return false;
-        task = g.level_task[wi.task + t];
+        task = ld(g.level_task, wi.task + t);
         return true;
     } else {
         task = wi.task;
@@ -343,12 +362,12 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
     constexpr int NB2 = NB * NB;
     constexpr int PP = P * P;
     constexpr bool UNI = KIND == KIND_UNION;
-    WorkItem const wi = items[blockIdx.x];
-    DevGroup const g = groups[wi.group];
+    WorkItem const wi = ld(items, blockIdx.x);
+    DevGroup const g = ld(groups, wi.group);
     int task, s;
     if (!lane_work<KIND>(g, wi, task, s)) return;
     size_t const sp = g.S_pad;
-    int const mm = g.task_mm[task];
+    int const mm = ld(g.task_mm, task);
     if (g.done[size_t(mm) * sp + s] != ACTIVE) return;
     Blocks<NB, FT> const lu{static_cast<FT*>(g.lu), sp, s};
     // a keep row whose parent is the next pivot of the task hands its finished U_p,parent,
@@ -360,15 +379,15 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
     int const lane = int(threadIdx.x);
     int fw_le = -1;
 
-    for (int ti = g.task_ptr[task]; ti < g.task_ptr[task + 1]; ++ti) {
-        int const p = g.task_pivot[ti];
-        int const rb = g.row_ptr[p], re = g.row_end[p], dg = g.diag[p];
-        int const es = g.row_stride ? g.row_stride[p] : 1;
+    for (int ti = ld(g.task_ptr, task); ti < ld(g.task_ptr, task + 1); ++ti) {
+        int const p = ld(g.task_pivot, ti);
+        int const rb = ld(g.row_ptr, p), re = ld(g.row_end, p), dg = ld(g.diag, p);
+        int const es = g.row_stride ? ld(g.row_stride, p) : 1;
         // a bus without a source in this lane (union groups): identity row, zero row and column
         bool const dead_p = UNI && g.dead[size_t(p) * sp + s] != 0;
         // rows with one U entry and no off-diagonal Schur target (every bus of a radial feeder) keep
         // their raw U entry and its transpose in registers until the row is finished
-        bool const keep = P == 1 && g.keep[p] != 0;
+        bool const keep = P == 1 && ld(g.keep, p) != 0;
         FT ku[NB2], kl[NB2];
 #pragma unroll
         for (int q = 0; q < NB2; ++q) ku[q] = kl[q] = FT(0);
@@ -421,7 +440,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
 #pragma unroll
                 for (int q = 0; q < NB2; ++q) a[q] = FT(g.raw_a[(size_t(k) * NB2 + q) * sp + s]);
                 put_u(k, a);
-                int const e = g.transpose[k];
+                int const e = ld(g.transpose, k);
 #pragma unroll
                 for (int q = 0; q < NB2; ++q) a[q] = FT(g.raw_a[(size_t(e) * NB2 + q) * sp + s]);
                 put_l(e, a);
@@ -430,7 +449,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
             // admittance matrix in the real domain plus load admittances and sources
             for (int k = dg; k < re; k += es) {
                 FT a[NB2];
-                if (g.fill[k] || dead_p) {
+                if (ld(g.fill, k) || dead_p) {
 #pragma unroll
                     for (int q = 0; q < NB2; ++q) a[q] = FT(0);
                 } else {
@@ -443,8 +462,8 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                     for (int q = 0; q < NB2; ++q) D[q] = double(a[q]);
                 } else {
                     put_u(k, a);
-                    int const e = g.transpose[k];
-                    if (g.fill[e] || dead_p) {
+                    int const e = ld(g.transpose, k);
+                    if (ld(g.fill, e) || dead_p) {
                         zero_l(e);
                     } else {
                         double ye[PP * 2];
@@ -458,7 +477,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
 #pragma unroll
                 for (int q = 0; q < NB2; ++q) D[q] = (q % (NB + 1) == 0) ? 1.0 : 0.0;
             }
-            for (int l = g.load_ptr[p]; l < (dead_p ? g.load_ptr[p] : g.load_ptr[p + 1]); ++l) {
+            for (int l = ld(g.load_ptr, p); l < (dead_p ? ld(g.load_ptr, p) : ld(g.load_ptr, p + 1)); ++l) {
 #pragma unroll
                 for (int ph = 0; ph < P; ++ph) {
                     double const sr = g.s_inj[((size_t(l) * P + ph) * 2) * sp + s];
@@ -471,7 +490,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                     D[ph * NB + ph + P] += yi;
                 }
             }
-            for (int q = g.src_ptr[p]; q < (dead_p ? g.src_ptr[p] : g.src_ptr[p + 1]); ++q) {
+            for (int q = ld(g.src_ptr, p); q < (dead_p ? ld(g.src_ptr, p) : ld(g.src_ptr, p + 1)); ++q) {
                 cplx const y1 = {g.y_src[(size_t(q) * 4) * sp + s], g.y_src[(size_t(q) * 4 + 1) * sp + s]};
                 cplx const y0 = {g.y_src[(size_t(q) * 4 + 2) * sp + s], g.y_src[(size_t(q) * 4 + 3) * sp + s]};
                 cplx const ur = {g.u_src[(size_t(q) * 2) * sp + s], g.u_src[(size_t(q) * 2 + 1) * sp + s]};
@@ -507,19 +526,19 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                 if constexpr (MODE == MODE_ICF) {
                     for (int k = dg + es; k < re; k += es) {
                         zero_u(k);
-                        zero_l(g.transpose[k]);
+                        zero_l(ld(g.transpose, k));
                     }
 #pragma unroll
                     for (int q = 0; q < NB2; ++q) D[q] = (q % (NB + 1) == 0) ? 1.0 : 0.0;
                 }
             } else {
                 for (int k = rb; k < re; k += es) {
-                    int const j = g.col[k];
-                    if (g.fill[k]) {
+                    int const j = ld(g.col, k);
+                    if (ld(g.fill, k)) {
                         if constexpr (MODE == MODE_ICF) {
                             if (j > p) {
                                 zero_u(k);
-                                zero_l(g.transpose[k]);
+                                zero_l(ld(g.transpose, k));
                             }
                         }
                         continue;
@@ -546,8 +565,8 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                             for (int q = 0; q < NB2; ++q) D[q] = double(a[q]);
                         } else if (j > p) {
                             put_u(k, a);
-                            int const e = g.transpose[k];
-                            if (g.fill[e]) {
+                            int const e = ld(g.transpose, k);
+                            if (ld(g.fill, e)) {
                                 zero_l(e);
                             } else {
                                 double ye[PP * 2];
@@ -558,8 +577,8 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                         }
                     }
                 }
-                for (int l = g.load_ptr[p]; l < g.load_ptr[p + 1]; ++l) {
-                    int const type = g.load_type[l];
+                for (int l = ld(g.load_ptr, p); l < ld(g.load_ptr, p + 1); ++l) {
+                    int const type = ld(g.load_type, l);
 #pragma unroll
                     for (int ph = 0; ph < P; ++ph) {
                         cplx const sv = {g.s_inj[((size_t(l) * P + ph) * 2) * sp + s],
@@ -577,7 +596,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                         r[ph] = cadd(r[ph], i);
                     }
                 }
-                for (int q = g.src_ptr[p]; q < g.src_ptr[p + 1]; ++q) {
+                for (int q = ld(g.src_ptr, p); q < ld(g.src_ptr, p + 1); ++q) {
                     cplx const y1 = {g.y_src[(size_t(q) * 4) * sp + s], g.y_src[(size_t(q) * 4 + 1) * sp + s]};
                     cplx const y0 = {g.y_src[(size_t(q) * 4 + 2) * sp + s], g.y_src[(size_t(q) * 4 + 3) * sp + s]};
                     cplx const ur = {g.u_src[(size_t(q) * 2) * sp + s], g.u_src[(size_t(q) * 2 + 1) * sp + s]};
@@ -608,7 +627,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
             if constexpr (MODE == MODE_RESID) {
                 // stored factors: y_p = L_pp^-1 P (r_p - sum_k L_pk y_k)
                 for (int k = rb; k < dg; k += es) {
-                    int const j = g.col[k];
+                    int const j = ld(g.col, k);
                     FT L[NB2];
                     lu.load(k, L);
                     double yj[NB];
@@ -634,7 +653,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
         } else if (dead_p) {
             for (int k = dg + es; k < re; k += es) {
                 zero_u(k);
-                zero_l(g.transpose[k]);
+                zero_l(ld(g.transpose, k));
             }
 #pragma unroll
             for (int q = 0; q < NB2; ++q) D[q] = (q % (NB + 1) == 0) ? 1.0 : 0.0;
@@ -646,11 +665,11 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
 #pragma unroll
             for (int ph = 0; ph < P; ++ph) pcal[ph] = qcal[ph] = 0.0;
             for (int k = rb; k < re; k += es) {
-                int const j = g.col[k];
-                if (g.fill[k]) {
+                int const j = ld(g.col, k);
+                if (ld(g.fill, k)) {
                     if (j > p) {
                         zero_u(k);
-                        zero_l(g.transpose[k]);
+                        zero_l(ld(g.transpose, k));
                     }
                     continue;
                 }
@@ -677,8 +696,8 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                     qcal[ph] += qr[ph];
                 }
                 if (j > p) {
-                    int const e = g.transpose[k];
-                    if (g.fill[e]) {
+                    int const e = ld(g.transpose, k);
+                    if (ld(g.fill, e)) {
                         zero_l(e);
                     } else {
                         FT a[NB2];
@@ -700,8 +719,8 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                 D[(ph + P) * NB + ph + P] += qcal[ph];
             }
             // loads: constant power / impedance / current (newton_raphson_pf_solver.hpp:764-822)
-            for (int l = g.load_ptr[p]; l < g.load_ptr[p + 1]; ++l) {
-                int const type = g.load_type[l];
+            for (int l = ld(g.load_ptr, p); l < ld(g.load_ptr, p + 1); ++l) {
+                int const type = ld(g.load_type, l);
 #pragma unroll
                 for (int ph = 0; ph < P; ++ph) {
                     double const sr = g.s_inj[((size_t(l) * P + ph) * 2) * sp + s];
@@ -716,7 +735,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                 }
             }
             // sources: two-bus equivalent with y_ref (newton_raphson_pf_solver.hpp:824-852)
-            for (int q = g.src_ptr[p]; q < g.src_ptr[p + 1]; ++q) {
+            for (int q = ld(g.src_ptr, p); q < ld(g.src_ptr, p + 1); ++q) {
                 cplx const y1 = {g.y_src[(size_t(q) * 4) * sp + s], g.y_src[(size_t(q) * 4 + 1) * sp + s]};
                 cplx const y0 = {g.y_src[(size_t(q) * 4 + 2) * sp + s], g.y_src[(size_t(q) * 4 + 3) * sp + s]};
                 cplx const ur = {g.u_src[(size_t(q) * 2) * sp + s], g.u_src[(size_t(q) * 2 + 1) * sp + s]};
@@ -752,8 +771,8 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
         FT Df[NB2];
 #pragma unroll
         for (int k = 0; k < NB2; ++k) Df[k] = FT(D[k]);
-        for (int q = g.schur_ptr[p]; q < g.schur_ptr[p + 1]; ++q) {
-            int const tgt = g.schur[3 * q], le = g.schur[3 * q + 1], ue = g.schur[3 * q + 2];
+        for (int q = ld(g.schur_ptr, p); q < ld(g.schur_ptr, p + 1); ++q) {
+            int const tgt = ld(g.schur, 3 * q), le = ld(g.schur, 3 * q + 1), ue = ld(g.schur, 3 * q + 2);
             FT L[NB2], U[NB2];
             bool const fwd = kFwd && le == fw_le;
             if (fwd) {
@@ -781,7 +800,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
 #pragma unroll
                     for (int r = 0; r < NB; ++r) yj[r] = fw_y[lane][r];
                 } else {
-                    int const j = g.col[le];
+                    int const j = ld(g.col, le);
 #pragma unroll
                     for (int r = 0; r < NB; ++r) yj[r] = g.y[(size_t(j) * NB + r) * sp + s];
                 }
@@ -833,7 +852,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
 #pragma unroll
                 for (int q = 0; q < NB2; ++q) fw_blk[lane][NB2 + q] = A[q];
             }
-            int const e = g.transpose[k];
+            int const e = ld(g.transpose, k);
             if (keep) {
 #pragma unroll
                 for (int q = 0; q < NB2; ++q) A[q] = kl[q];
@@ -855,10 +874,10 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
         for (int r = 0; r < NB; ++r) g.y[(size_t(p) * NB + r) * sp + s] = rhs[r];
         if constexpr (kFwd) {
             fw_le = -1;
-            if (keep && ti + 1 < g.task_ptr[task + 1]) {
+            if (keep && ti + 1 < ld(g.task_ptr, task + 1)) {
                 int const k1 = dg + es;  // the row's only U entry
-                if (g.task_pivot[ti + 1] == g.col[k1]) {
-                    fw_le = g.transpose[k1];
+                if (ld(g.task_pivot, ti + 1) == ld(g.col, k1)) {
+                    fw_le = ld(g.transpose, k1);
 #pragma unroll
                     for (int r = 0; r < NB; ++r) fw_y[lane][r] = rhs[r];
                 }
@@ -875,12 +894,12 @@ __global__ __launch_bounds__(kWave) void backward_kernel(DevGroup const* __restr
                                                          WorkItem const* __restrict__ items) {
     constexpr int NB = 2 * P;
     constexpr int NB2 = NB * NB;
-    WorkItem const wi = items[blockIdx.x];
-    DevGroup const g = groups[wi.group];
+    WorkItem const wi = ld(items, blockIdx.x);
+    DevGroup const g = ld(groups, wi.group);
     int task, s;
     if (!lane_work<KIND>(g, wi, task, s)) return;
     size_t const sp = g.S_pad;
-    int const mm = g.task_mm[task];
+    int const mm = ld(g.task_mm, task);
     if (g.done[size_t(mm) * sp + s] != ACTIVE) return;
     Blocks<NB, FT> const lu{static_cast<FT*>(g.lu), sp, s};
     double dev = 0.0;
@@ -889,15 +908,15 @@ __global__ __launch_bounds__(kWave) void backward_kernel(DevGroup const* __restr
     __shared__ double fx[kWave][NB];
     int const lane = int(threadIdx.x);
     int fx_bus = -1;
-    for (int ti = g.task_ptr[task + 1] - 1; ti >= g.task_ptr[task]; --ti) {
-        int const p = g.task_pivot[ti];
-        int const re = g.row_end[p], dg = g.diag[p];
-        int const es = g.row_stride ? g.row_stride[p] : 1;
+    for (int ti = ld(g.task_ptr, task + 1) - 1; ti >= ld(g.task_ptr, task); --ti) {
+        int const p = ld(g.task_pivot, ti);
+        int const re = ld(g.row_end, p), dg = ld(g.diag, p);
+        int const es = g.row_stride ? ld(g.row_stride, p) : 1;
         double v[NB];
 #pragma unroll
         for (int r = 0; r < NB; ++r) v[r] = g.y[(size_t(p) * NB + r) * sp + s];
         for (int k = dg + es; k < re; k += es) {
-            int const j = g.col[k];
+            int const j = ld(g.col, k);
             FT U[NB2];
             lu.load(k, U);
             double xj[NB];
@@ -1030,25 +1049,25 @@ __global__ void ic_start_kernel(DevGroup const* __restrict__ groups, long long c
     long long const gidx = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
     if (gidx >= offsets[n_groups]) return;
     int const gr = find_group(offsets, n_groups, gidx);
-    DevGroup const& g = groups[gr];
+    DevGroup const g = ld(groups, gr);
     size_t const sp = g.S_pad;
     size_t const idx = size_t(gidx - offsets[gr]);
     int const b = int(idx / sp), s = int(idx % sp);
     if (s >= g.S) return;
-    int const mm = g.bus_mm[b];
+    int const mm = ld(g.bus_mm, b);
     if (mm < 0 || g.done[size_t(mm) * sp + s] != ACTIVE) return;
-    int const q0 = g.mm_src_ptr[mm], q1 = g.mm_src_ptr[mm + 1];
+    int const q0 = ld(g.mm_src_ptr, mm), q1 = ld(g.mm_src_ptr, mm + 1);
     cplx sum = {0.0, 0.0};
     for (int qi = q0; qi < q1; ++qi) {
-        int const q = g.mm_src[qi];
+        int const q = ld(g.mm_src, qi);
         cplx const ur = {g.u_src[(size_t(q) * 2) * sp + s], g.u_src[(size_t(q) * 2 + 1) * sp + s]};
         double sn, cs;
-        sincos(-g.phase[g.src_bus[q]], &sn, &cs);
+        sincos(-ld(g.phase, g.src_bus[q]), &sn, &cs);
         sum = cadd(sum, cmul(ur, {cs, sn}));
     }
     double const inv = 1.0 / double(q1 - q0);
     double sn, cs;
-    sincos(g.phase[b], &sn, &cs);
+    sincos(ld(g.phase, b), &sn, &cs);
     cplx ub[P];
     balanced<P>(cmul({sum.re * inv, sum.im * inv}, {cs, sn}), ub);
 #pragma unroll
@@ -1067,17 +1086,17 @@ __global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long
     long long const gidx = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
     if (gidx >= offsets[n_groups]) return;
     int const gr = find_group(offsets, n_groups, gidx);
-    DevOutput const& o = outs[gr];
+    DevOutput const o = ld(outs, gr);
     size_t const idx = size_t(gidx - offsets[gr]);
     size_t const sp = o.S_pad;
     int const b = int(idx / sp);
     int const s = int(idx % sp);
-    if (s >= o.S || !o.ok[s] || o.bus_mm[b] < 0) return;
+    if (s >= o.S || !o.ok[s] || ld(o.bus_mm, b) < 0) return;
     if (o.dead && o.dead[size_t(b) * sp + s]) {
         // not energised in this lane: the appliances of the bus carry nothing
-        for (int l = o.load_ptr[b]; l < o.load_ptr[b + 1]; ++l)
+        for (int l = ld(o.load_ptr, b); l < ld(o.load_ptr, b + 1); ++l)
             for (int q = 0; q < 4 * P; ++q) o.load_si[(size_t(l) * 4 * P + q) * sp + s] = 0.0;
-        for (int q = o.shunt_ptr[b]; q < o.shunt_ptr[b + 1]; ++q)
+        for (int q = ld(o.shunt_ptr, b); q < ld(o.shunt_ptr, b + 1); ++q)
             for (int c = 0; c < 4 * P; ++c) o.shunt_si[(size_t(q) * 4 * P + c) * sp + s] = 0.0;
         for (int c = 0; c < 2 * P; ++c) o.bus_inj[(size_t(b) * 2 * P + c) * sp + s] = 0.0;
         return;
@@ -1091,8 +1110,8 @@ __global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long
     cplx iload[P], acc_s[P];
 #pragma unroll
     for (int ph = 0; ph < P; ++ph) iload[ph] = acc_s[ph] = {0.0, 0.0};
-    for (int l = o.load_ptr[b]; l < o.load_ptr[b + 1]; ++l) {
-        int const type = o.linear ? 1 : o.load_type[l];
+    for (int l = ld(o.load_ptr, b); l < ld(o.load_ptr, b + 1); ++l) {
+        int const type = o.linear ? 1 : ld(o.load_type, l);
 #pragma unroll
         for (int ph = 0; ph < P; ++ph) {
             cplx sv = {o.s_inj[((size_t(l) * P + ph) * 2) * sp + s], o.s_inj[((size_t(l) * P + ph) * 2 + 1) * sp + s]};
@@ -1109,20 +1128,20 @@ __global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long
             acc_s[ph] = cadd(acc_s[ph], sv);
         }
     }
-    int const s0 = o.src_ptr[b], s1 = o.src_ptr[b + 1];
+    int const s0 = ld(o.src_ptr, b), s1 = ld(o.src_ptr, b + 1);
     if (s1 > s0) {
         // bus injection u * conj(sum_j Y_bj u_j)
         cplx inj[P];
 #pragma unroll
         for (int ph = 0; ph < P; ++ph) inj[ph] = {0.0, 0.0};
-        int const es = o.row_stride ? o.row_stride[b] : 1;
-        for (int k = o.row_ptr[b]; k < o.row_end[b]; k += es) {
-            if (o.fill[k]) continue;
-            int const j = o.col[k];
+        int const es = o.row_stride ? ld(o.row_stride, b) : 1;
+        for (int k = ld(o.row_ptr, b); k < ld(o.row_end, b); k += es) {
+            if (ld(o.fill, k)) continue;
+            int const j = ld(o.col, k);
             double y[P * P * 2];
 #pragma unroll
             for (int q = 0; q < P * P * 2; ++q)
-                y[q] = o.ylane ? o.ylane[(size_t(k) * P * P * 2 + q) * sp + s] : o.ydata[size_t(k) * P * P * 2 + q];
+                y[q] = o.ylane ? o.ylane[(size_t(k) * P * P * 2 + q) * sp + s] : ld(o.ydata, size_t(k) * P * P * 2 + q);
 #pragma unroll
             for (int r = 0; r < P; ++r)
 #pragma unroll
@@ -1199,7 +1218,7 @@ __global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long
         o.bus_inj[((size_t(b) * P + ph) * 2) * sp + s] = acc_s[ph].re;
         o.bus_inj[((size_t(b) * P + ph) * 2 + 1) * sp + s] = acc_s[ph].im;
     }
-    for (int q = o.shunt_ptr[b]; q < o.shunt_ptr[b + 1]; ++q) {
+    for (int q = ld(o.shunt_ptr, b); q < ld(o.shunt_ptr, b + 1); ++q) {
         double const* y = o.shunt_y + size_t(q) * P * P * 2;
 #pragma unroll
         for (int r = 0; r < P; ++r) {
@@ -1351,8 +1370,8 @@ __global__ __launch_bounds__(256) void records_kernel(DevOutput const* __restric
     __shared__ __align__(16) Out tile[SLOTS];
     long long const t = blockIdx.x;
     int const gr = find_group(tile_off, n_groups, t);
-    DevOutput const& o = outs[gr];
-    Rec const& r = recs[gr];
+    DevOutput const o = ld(outs, gr);
+    Rec const r = ld(recs, gr);
     int const W = o.S_pad < kWaveLanes ? o.S_pad : kWaveLanes;
     int const TNg = SLOTS / W;
     int const n_itiles = (r.n_comp + TNg - 1) / TNg;
@@ -1378,7 +1397,7 @@ __global__ __launch_bounds__(256) void records_kernel(DevOutput const* __restric
         int const sl = s0 + ln;
         if (k >= n_valid * CH || sl >= o.S || !o.ok[sl]) continue;
         uint4 const* src = reinterpret_cast<uint4 const*>(&tile[ln * TNg]) + k;
-        uint4* dst = reinterpret_cast<uint4*>(out + size_t(o.scenario_index[sl]) * r.n_comp + i0) + k;
+        uint4* dst = reinterpret_cast<uint4*>(out + size_t(ld(o.scenario_index, sl)) * r.n_comp + i0) + k;
         *dst = *src;
     }
 }
