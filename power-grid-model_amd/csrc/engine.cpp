@@ -211,6 +211,10 @@ struct Group {
         std::vector<IntS> line_fs, line_ts, tr_fs, tr_ts;
     };
     std::vector<Member> members;
+    // branches open on both sides in the union state that some member closes (coupled into the
+    // union's math models; the off-pattern entries they add are corrected per lane)
+    std::vector<char> couple;
+    std::vector<int> xlanes;  // lanes with extra entries
 };
 
 namespace {
@@ -239,6 +243,20 @@ bool opens_only(Model const& m, ComponentState const& st) {
     };
     return le(st.line_fs, m.base.line_fs) && le(st.line_ts, m.base.line_ts) && le(st.tr_fs, m.base.tr_fs) &&
            le(st.tr_ts, m.base.tr_ts);
+}
+
+// branches with a side closed in st that the base state has open (-1 when more than limit)
+int closed_against_base(Model const& m, ComponentState const& st, int limit, std::vector<Idx>* list = nullptr) {
+    int cnt = 0;
+    Idx const nl = m.n_line();
+    for (Idx c = 0; c < m.n_branch(); ++c) {
+        bool const f = c < nl ? st.line_fs[c] && !m.base.line_fs[c] : st.tr_fs[c - nl] && !m.base.tr_fs[c - nl];
+        bool const t = c < nl ? st.line_ts[c] && !m.base.line_ts[c] : st.tr_ts[c - nl] && !m.base.tr_ts[c - nl];
+        if (!f && !t) continue;
+        if (++cnt > limit) return -1;
+        if (list) list->push_back(c);
+    }
+    return cnt;
 }
 
 std::string structural_key(Model const& m, ComponentState const& st, bool branch_status = true) {
@@ -507,19 +525,36 @@ void upload_union_lanes(Batch& b, Group& g) {
             }
         }
     }
+    // buses of the nodes of every math branch, whatever its side statuses in the union state
+    Topology const& tp = g.topo;
+    std::vector<int> nfrom(nbr, -1), nto(nbr, -1);
+    auto node_bus = [&](Idx node) {
+        Idx const gg = tp.node_group[node];
+        return gg < 0 ? -1 : p.bus_label[p.mm_bus_begin[gg] + int(tp.node_pos[node])];
+    };
+    for (Idx mb = 0; mb < nbr; ++mb) {
+        Idx const c = p.branch_comp[mb];
+        nfrom[mb] = node_bus(c < nl ? m.line_f[c] : m.tr_f[c - nl]);
+        nto[mb] = node_bus(c < nl ? m.line_t[c] : m.tr_t[c - nl]);
+    }
     // math branches incident to each bus, in math order
     std::vector<std::vector<int>> inc(n);
     for (Idx mb = 0; mb < nbr; ++mb) {
-        int const f = p.branch_from[mb], t = p.branch_to[mb];
+        int const f = nfrom[mb], t = nto[mb];
         if (f >= 0) inc[f].push_back(int(mb));
         if (t >= 0 && t != f) inc[t].push_back(int(mb));
     }
-    auto find = [&](int r, int c) {
+    auto find_entry = [&](int r, int c) {
         auto const b0 = p.col.begin() + p.row_ptr[r], b1 = p.col.begin() + p.row_ptr[r + 1];
         auto const it = std::lower_bound(b0, b1, c);
-        if (it == b1 || *it != c) throw PowerGridError("union group: LU pattern entry missing\n");
-        return int(it - p.col.begin());
+        return it == b1 || *it != c ? -1 : int(it - p.col.begin());
     };
+    // extra (off-pattern) entries of the lanes that close branches against the union structure
+    bool const extras = !g.couple.empty();
+    std::vector<int> xcnt(extras ? sp : 0, 0), xrow(extras ? sp * kExtraMax : 0, 0), xcol(xrow.size(), 0);
+    std::vector<int> xm(xcnt.size(), 0), xb(xrow.size(), 0);
+    std::vector<double> xy(xrow.size() * 2, 0.0);
+    std::vector<int8_t> xbus(extras ? size_t(n) : 0, 0);
     std::vector<uint8_t> dead(size_t(n) * sp, 0), lstat(size_t(nbr) * sp, 0);
     std::vector<int> patch_lane, patch_entry;
     std::vector<double> patch_val;
@@ -541,7 +576,7 @@ void upload_union_lanes(Batch& b, Group& g) {
             stack.pop_back();
             for (int mb : inc[r]) {
                 if (bits[mb] != 3) continue;
-                int const f = p.branch_from[mb], t = p.branch_to[mb];
+                int const f = nfrom[mb], t = nto[mb];
                 if (f < 0 || t < 0) continue;
                 int const o = f == r ? t : f;
                 if (!reach[o]) {
@@ -555,10 +590,10 @@ void upload_union_lanes(Batch& b, Group& g) {
         for (Idx mb = 0; mb < nbr; ++mb) {
             Idx const c = p.branch_comp[mb];
             if (bits[mb] == side(g.st.line_fs, g.st.line_ts, g.st.tr_fs, g.st.tr_ts, c)) continue;
-            int const f = p.branch_from[mb], t = p.branch_to[mb];
+            int const f = nfrom[mb], t = nto[mb];
             if (f >= 0) touched.emplace_back(f, f);
             if (t >= 0) touched.emplace_back(t, t);
-            if (f >= 0 && t >= 0) {
+            if (f >= 0 && t >= 0 && f != t) {
                 touched.emplace_back(f, t);
                 touched.emplace_back(t, f);
             }
@@ -566,10 +601,11 @@ void upload_union_lanes(Batch& b, Group& g) {
         std::sort(touched.begin(), touched.end());
         touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
         std::vector<cd> val(PP);
+        std::vector<std::pair<std::pair<int, int>, cd>> member_extras;
         for (auto const& [r, c] : touched) {
             std::fill(val.begin(), val.end(), cd{});
             for (int mb : inc[r]) {
-                int const f = p.branch_from[mb], t = p.branch_to[mb];
+                int const f = nfrom[mb], t = nto[mb];
                 int w = -1;
                 if (r == c) {
                     if (f == r) w = 0;
@@ -586,7 +622,14 @@ void upload_union_lanes(Batch& b, Group& g) {
             if (r == c)
                 for (int q = p.shunt_ptr[r]; q < p.shunt_ptr[r + 1]; ++q)
                     for (int k = 0; k < PP; ++k) val[k] += p.shunt_y[size_t(q) * PP + k];
-            int const e = find(r, c);
+            int const e = find_entry(r, c);
+            if (e < 0) {
+                // off the pattern: a branch closed in this member that the union structure has open;
+                // between live buses it becomes an extra entry of the member's lanes
+                if (!extras || PP != 1) throw PowerGridError("union group: LU pattern entry missing\n");
+                if (reach[r] && reach[c] && val[0] != cd{}) member_extras.push_back({{r, c}, val[0]});
+                continue;
+            }
             for (int lane = mem.lane_begin; lane < mem.lane_end; ++lane) {
                 patch_lane.push_back(lane);
                 patch_entry.push_back(e);
@@ -596,9 +639,29 @@ void upload_union_lanes(Batch& b, Group& g) {
                 }
             }
         }
+        if (int(member_extras.size()) > kExtraMax) throw PowerGridError("union group: too many extra entries\n");
+        std::vector<int> mbus;
+        for (auto const& [rc, v] : member_extras) {
+            mbus.push_back(rc.first);
+            mbus.push_back(rc.second);
+            xbus[rc.first] = 1;
+        }
+        std::sort(mbus.begin(), mbus.end());
+        mbus.erase(std::unique(mbus.begin(), mbus.end()), mbus.end());
         for (int lane = mem.lane_begin; lane < mem.lane_end; ++lane) {
             for (Idx r = 0; r < n; ++r) dead[size_t(r) * sp + lane] = reach[r] ? 0 : 1;
             for (Idx mb = 0; mb < nbr; ++mb) lstat[size_t(mb) * sp + lane] = bits[mb];
+            if (member_extras.empty()) continue;
+            xcnt[lane] = int(member_extras.size());
+            for (size_t e = 0; e < member_extras.size(); ++e) {
+                xrow[size_t(lane) * kExtraMax + e] = member_extras[e].first.first;
+                xcol[size_t(lane) * kExtraMax + e] = member_extras[e].first.second;
+                xy[(size_t(lane) * kExtraMax + e) * 2] = member_extras[e].second.real();
+                xy[(size_t(lane) * kExtraMax + e) * 2 + 1] = member_extras[e].second.imag();
+            }
+            xm[lane] = int(mbus.size());
+            for (size_t k = 0; k < mbus.size(); ++k) xb[size_t(lane) * kExtraMax + k] = mbus[k];
+            g.xlanes.push_back(lane);
         }
     }
     double* ylane = dev_alloc<double>(A, size_t(p.E) * Q * sp);
@@ -616,10 +679,51 @@ void upload_union_lanes(Batch& b, Group& g) {
     g.dg.dead = d_dead;
     g.dout.ylane = ylane;
     g.dout.dead = d_dead;
+    int* d_nfrom = put(A, nfrom);
+    int* d_nto = put(A, nto);
     for (int k = 0; k < 2; ++k) {
         g.branch_rec[k].lane_status = d_lstat;
         g.branch_rec[k].branch_y_var = d_yvar;
+        // sides are masked per lane by the side statuses
+        g.branch_rec[k].from_bus = d_nfrom;
+        g.branch_rec[k].to_bus = d_nto;
     }
+    if (g.xlanes.empty()) return;
+    // chains from every bus of an extra entry to the root of its elimination tree (radial structure:
+    // the parent of a row is the column of its one U entry)
+    std::vector<int> ch_off(n, -1), ch_len(n, 0), ch_node, ch_ue;
+    int ch_max = 0;
+    for (int lane : g.xlanes)
+        for (int k = 0; k < xm[lane]; ++k) {
+            int const c = xb[size_t(lane) * kExtraMax + k];
+            if (ch_off[c] >= 0) continue;
+            ch_off[c] = int(ch_node.size());
+            int r = c, prev_ue = -1;
+            while (true) {
+                ch_node.push_back(r);
+                ch_ue.push_back(prev_ue);
+                if (p.row_end[r] - p.diag[r] < 2) break;
+                prev_ue = p.diag[r] + 1;
+                r = p.col[prev_ue];
+            }
+            ch_len[c] = int(ch_node.size()) - ch_off[c];
+            ch_max = std::max(ch_max, ch_len[c]);
+        }
+    DevGroup& d = g.dg;
+    d.xbus = put(A, xbus);
+    d.xcnt = put(A, xcnt);
+    d.xrow = put(A, xrow);
+    d.xcol = put(A, xcol);
+    d.xy = put(A, xy);
+    d.xblk = dev_alloc<double>(A, sp * kExtraMax * 4);
+    d.xm = put(A, xm);
+    d.xb = put(A, xb);
+    d.ch_off = put(A, ch_off);
+    d.ch_len = put(A, ch_len);
+    d.ch_node = put(A, ch_node);
+    d.ch_ue = put(A, ch_ue);
+    d.ch_max = ch_max;
+    d.xst = dev_alloc<double>(A, g.xlanes.size() * kExtraMax * size_t(ch_max) * 8);
 }
 
 // per-scenario injections, source voltages and admittances of one group
@@ -869,24 +973,82 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         b.groups[scn_group[s]]->scenarios.push_back(int(s));
     }
 
-    // ---- small groups that only open branches against the base state: one union group on the base
-    // structure per state of the other components ----
+    // ---- small groups that differ from the base state only in branch statuses: one union group on
+    // the base structure per state of the other components. Members that only open branches run on the
+    // shared structure with per-lane admittances. Members that also close branches the base has open
+    // (symmetric Newton-Raphson or linear method, radial base structure, at most kExtraMax / 2 such
+    // branches, no source at their buses) add off-pattern entries that a low-rank correction takes
+    // into account ----
     {
+        bool const closing_ok = sym && !b.current;
         std::map<std::string, std::vector<int>> by_key;
-        for (size_t gi = 0; gi < b.groups.size(); ++gi)
-            if (b.groups[gi]->scenarios.size() < size_t(kUnionBelow) && opens_only(m, b.groups[gi]->st))
-                by_key[structural_key(m, b.groups[gi]->st, false)].push_back(int(gi));
+        for (size_t gi = 0; gi < b.groups.size(); ++gi) {
+            Group const& g = *b.groups[gi];
+            if (g.scenarios.size() >= size_t(kUnionBelow)) continue;
+            if (opens_only(m, g.st) || (closing_ok && closed_against_base(m, g.st, kExtraMax / 2) > 0))
+                by_key[structural_key(m, g.st, false)].push_back(int(gi));
+        }
         std::vector<char> merged(b.groups.size(), 0);
         std::vector<std::unique_ptr<Group>> unions;
-        for (auto& [key, list] : by_key) {
+        for (auto& [key, cand] : by_key) {
+            if (cand.size() < 2) continue;
+            ComponentState ust = b.groups[cand[0]]->st;
+            ust.line_fs = m.base.line_fs;
+            ust.line_ts = m.base.line_ts;
+            ust.tr_fs = m.base.tr_fs;
+            ust.tr_ts = m.base.tr_ts;
+            std::vector<int> list;
+            std::vector<char> couple(size_t(m.n_branch()), 0);
+            std::vector<int> closing;
+            for (int gi : cand) {
+                if (opens_only(m, b.groups[gi]->st))
+                    list.push_back(gi);
+                else
+                    closing.push_back(gi);
+            }
+            if (!closing.empty()) {
+                // the union structure with every closed branch coupled: the closing members' branches
+                // must join two buses of one math model without a source, and the structure must be a
+                // forest (one U entry per row: the correction walks single-parent chains)
+                std::vector<char> all(size_t(m.n_branch()), 0);
+                for (int gi : closing) {
+                    std::vector<Idx> cl;
+                    closed_against_base(m, b.groups[gi]->st, kExtraMax / 2, &cl);
+                    for (Idx c : cl) all[c] = 1;
+                }
+                Topology const t = build_topology(m, ust, &all);
+                std::vector<char> src_node(size_t(m.n_node()), 0);
+                for (Idx k = 0; k < Idx(m.source.size()); ++k) src_node[m.src_node[k]] = 1;
+                std::vector<Idx> models(t.models.size());
+                std::iota(models.begin(), models.end(), Idx{0});
+                Plan const pl = build_plan(m, ust, t, models, sym, 16);
+                bool radial = true;
+                for (Idx r = 0; r < pl.n && radial; ++r) radial = pl.row_end[r] - pl.diag[r] <= 2;
+                Idx const nl = m.n_line();
+                for (int gi : closing) {
+                    if (!radial) break;
+                    std::vector<Idx> cl;
+                    closed_against_base(m, b.groups[gi]->st, kExtraMax / 2, &cl);
+                    bool ok = !cl.empty();
+                    Idx const g0 = ok ? t.node_group[cl[0] < nl ? m.line_f[cl[0]] : m.tr_f[cl[0] - nl]] : -1;
+                    for (Idx c : cl) {
+                        Idx const f = c < nl ? m.line_f[c] : m.tr_f[c - nl];
+                        Idx const to = c < nl ? m.line_t[c] : m.tr_t[c - nl];
+                        // one math model per member (its lanes iterate and converge together)
+                        ok = ok && g0 >= 0 && t.node_group[f] == g0 && t.node_group[to] == g0 && !src_node[f] &&
+                             !src_node[to] && t.branch_group[c] >= 0;
+                    }
+                    if (!ok) continue;
+                    list.push_back(gi);
+                    for (Idx c : cl)
+                        if (t.branch_pos[c] >= 0 && all[c]) couple[c] = 1;
+                }
+            }
             if (list.size() < 2) continue;
             auto u = std::make_unique<Group>();
             u->is_union = true;
-            u->st = b.groups[list[0]]->st;
-            u->st.line_fs = m.base.line_fs;
-            u->st.line_ts = m.base.line_ts;
-            u->st.tr_fs = m.base.tr_fs;
-            u->st.tr_ts = m.base.tr_ts;
+            u->st = std::move(ust);
+            if (std::any_of(couple.begin(), couple.end(), [](char c) { return c != 0; })) u->couple = std::move(couple);
             for (int gi : list) {
                 Group& g = *b.groups[gi];
                 Group::Member mem;
@@ -918,7 +1080,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             for (size_t gi = t; gi < G; gi += hw) {
                 try {
                     Group& g = *b.groups[gi];
-                    g.topo = build_topology(m, g.st);
+                    g.topo = build_topology(m, g.st, g.couple.empty() ? nullptr : &g.couple);
                     std::vector<Idx> all(g.topo.models.size());
                     std::iota(all.begin(), all.end(), Idx{0});
                     int const nblk = int((g.scenarios.size() + 63) / 64);
@@ -1013,6 +1175,12 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             for (int c = 0; c < N_COMP; ++c) ar[c].push_back(gp->appl_rec[c]);
         }
         b.d_groups = put(b.arena, dg);
+        std::vector<ExtraWork> xw;
+        for (size_t gi = 0; gi < b.groups.size(); ++gi)
+            for (size_t k = 0; k < b.groups[gi]->xlanes.size(); ++k)
+                xw.push_back(ExtraWork{int(gi), b.groups[gi]->xlanes[k], int(k)});
+        b.n_xwork = int(xw.size());
+        if (!xw.empty()) b.d_xwork = put(b.arena, xw);
         b.d_outputs = put(b.arena, dout);
         b.d_node_recs = put(b.arena, nr);
         for (int k = 0; k < 2; ++k) b.d_branch_recs[k] = put(b.arena, br[k]);
@@ -1160,6 +1328,9 @@ void Batch::run() {
                 PGM_HIP_CHECK(hipStreamWaitEvent(st, ev_lead, 0));
             }
         }
+        // lanes that close branches against their union structure: Woodbury correction of the
+        // forward vectors before the backward sweep
+        if (n_xwork > 0 && (mode == MODE_INIT || mode == MODE_NR)) launch_lowrank(d_groups, d_xwork, n_xwork, fp32, su);
         for (int l = std::max(LU, LP) - 1; l >= 0; --l) {
             if (l < LP) launch_backward(d_groups, d_items_p[l], n_items_p[l], P, fp32, mode, su, KIND_PACKED);
             if (l < LU) launch_backward(d_groups, d_items_u[l], n_items_u[l], P, fp32, mode, su, KIND_UNION);

@@ -70,6 +70,8 @@ struct Batch {
     std::vector<double> level_bytes_full;  // algorithmic forward bytes per level, every lane iterating
     std::vector<WorkItem*> d_items_u;      // union groups, per level
     std::vector<int> n_items_u;
+    ExtraWork* d_xwork = nullptr;          // union lanes with extra entries (low-rank correction)
+    int n_xwork = 0;
     std::vector<WorkItem*> d_items_p;      // packed groups, per level
     std::vector<int> n_items_p;
     PairState pairs;

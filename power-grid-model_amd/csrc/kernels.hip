@@ -351,6 +351,33 @@ __device__ __forceinline__ bool lane_work(DevGroup const& g, WorkItem const& wi,
     }
 }
 
+// extra (off-pattern) entries of row p in lane s of a union group (branches closed in the lane that
+// the shared structure has open): MODE_INIT keeps the admittance block, MODE_NR the Jacobian block and
+// adds the entry's power terms to the row sums. The blocks enter through the low-rank correction.
+template <int MODE>
+__device__ __forceinline__ void extra_entries(DevGroup const& g, int p, int s, cplx const (&up)[1], double& pcal,
+                                              double& qcal) {
+    int const nx = ld(g.xcnt, s);
+    for (int e = 0; e < nx; ++e) {
+        size_t const q = size_t(s) * kExtraMax + e;
+        if (ld(g.xrow, q) != p) continue;
+        double const y[2] = {ld(g.xy, 2 * q), ld(g.xy, 2 * q + 1)};
+        double blk[4];
+        if constexpr (MODE == MODE_INIT) {
+            linear_block<1, double>(y, blk);
+        } else {
+            cplx uj[1];
+            load_u<1>(g, ld(g.xcol, q), s, uj);
+            double pr[1], qr[1];
+            hnml<1, double>(y, up, uj, blk, pr, qr);
+            pcal += pr[0];
+            qcal += qr[0];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) g.xblk[q * 4 + k] = blk[k];
+    }
+}
+
 // ---------------------------------------------------------------------------------------------------
 // forward sweep: assemble row / column of each pivot, pull the Schur updates, factor the diagonal
 // block, finish the U row and L column, forward-substitute
@@ -512,6 +539,13 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                     }
                     rhs[r] += acc.re;
                     rhs[r + P] += acc.im;
+                }
+            }
+            if constexpr (UNI && P == 1) {
+                if (g.xbus && ld(g.xbus, p) && !dead_p) {
+                    cplx const none[1] = {{0.0, 0.0}};
+                    double unused_p = 0.0, unused_q = 0.0;
+                    extra_entries<MODE_INIT>(g, p, s, none, unused_p, unused_q);
                 }
             }
         } else if constexpr (MODE == MODE_ICF || MODE == MODE_RESID) {
@@ -708,6 +742,9 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                         put_l(e, a);
                     }
                 }
+            }
+            if constexpr (UNI && P == 1) {
+                if (g.xbus && ld(g.xbus, p)) extra_entries<MODE_NR>(g, p, s, up, pcal[0], qcal[0]);
             }
 #pragma unroll
             for (int ph = 0; ph < P; ++ph) {
@@ -990,6 +1027,213 @@ __global__ __launch_bounds__(kWave) void backward_kernel(DevGroup const* __restr
     }
     if constexpr (MODE == MODE_NR || MODE == MODE_ICF || MODE == MODE_RESID) {
         atomicMax(&g.devmax[size_t(mm) * sp + s], static_cast<unsigned long long>(__double_as_longlong(dev)));
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// low-rank correction of a union lane with extra entries. With W the block columns of the lane's m
+// distinct extra buses and M (2m x 2m) the extra blocks, the lane's matrix is J = J' + W M W^T, J' the
+// factorised one. Writing the forward sweep as y = F r and the backward sweep as x = B y (so that
+// J'^-1 = B F), Woodbury gives x = B (y - F W z) with z = (I + M Z)^-1 M W^T B y and Z = W^T B F W.
+// On a radial structure F W and W^T B live on the chains from the buses to the root: S_k, the forward
+// response at chain step k to a unit block at the chain's bus, and T_k, the response of x at the bus
+// to y at step k:
+//   S_0 = F_c, S_k = F_k (-L_{k,k-1} S_{k-1});  T_0 = B_c, T_k = -T_{k-1} U_{k-1,k} B_k
+// (F_k: row permutation and unit-lower solve of the pivot block; B_k: upper solve and column
+// permutation). Then (W^T B y)_c = sum_k T^c_k y_k, Z_cd = sum over the common steps of T^c S^d, and
+// the correction y -= F W z runs along the chains before the backward sweep.
+// One block (one wave) per lane.
+// ---------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void mul22(double const (&a)[4], double const (&b)[4], double (&c)[4]) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 2; ++r) c[j * 2 + r] = a[r] * b[j * 2] + a[2 + r] * b[j * 2 + 1];
+}
+
+template <class FT>
+__global__ __launch_bounds__(kWave) void lowrank_kernel(DevGroup const* __restrict__ groups,
+                                                        ExtraWork const* __restrict__ work) {
+    constexpr int K2 = 2 * kExtraMax;
+    ExtraWork const w = ld(work, blockIdx.x);
+    DevGroup const g = ld(groups, w.group);
+    int const s = w.lane;
+    size_t const sp = g.S_pad;
+    int const m = ld(g.xm, s), nx = ld(g.xcnt, s);
+    int const* xb = g.xb + size_t(s) * kExtraMax;
+    int const mm = ld(g.bus_mm, ld(xb, 0));
+    if (g.done[size_t(mm) * sp + s] != ACTIVE) return;
+    Blocks<2, FT> const lu{static_cast<FT*>(g.lu), sp, s};
+    double* const st = g.xst + size_t(w.slot) * kExtraMax * size_t(g.ch_max) * 8;
+    int const t = int(threadIdx.x);
+    __shared__ double Z[K2][K2], Mx[K2][K2], Km[K2][K2 + 1], w0[K2], z[K2];
+    __shared__ int fail;
+
+    // chains: S and T of every step
+    if (t < m) {
+        int const c = ld(xb, t);
+        int const off = ld(g.ch_off, c), len = ld(g.ch_len, c);
+        double* out = st + size_t(t) * g.ch_max * 8;
+        double S[4] = {1.0, 0.0, 0.0, 1.0}, T[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < len; ++k) {
+            int const i = ld(g.ch_node, off + k);
+            FT Df[4];
+            lu.load(ld(g.diag, i), Df);
+            uint32_t const code = g.perm[size_t(i) * sp + s];
+            double A[4] = {1.0, 0.0, 0.0, 1.0}, R[4] = {1.0, 0.0, 0.0, 1.0};
+            if (k > 0) {
+                int const ue = ld(g.ch_ue, off + k);
+                FT Lf[4], Uf[4];
+                lu.load(ld(g.transpose, ue), Lf);
+                lu.load(ue, Uf);
+                double const Lb[4] = {double(Lf[0]), double(Lf[1]), double(Lf[2]), double(Lf[3])};
+                double const Ub[4] = {double(Uf[0]), double(Uf[1]), double(Uf[2]), double(Uf[3])};
+                mul22(Lb, S, A);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) A[q] = -A[q];
+                mul22(T, Ub, R);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) R[q] = -R[q];
+            }
+            double B[4];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                double v[2] = {A[j * 2], A[j * 2 + 1]};
+                apply_p_vec<2>(v, code);
+                lower_vec<2, FT>(Df, v);
+                S[j * 2] = v[0];
+                S[j * 2 + 1] = v[1];
+                double e[2] = {j == 0 ? 1.0 : 0.0, j == 1 ? 1.0 : 0.0};
+                upper_vec<2, FT>(Df, e);
+                apply_q_vec<2>(e, code);
+                B[j * 2] = e[0];
+                B[j * 2 + 1] = e[1];
+            }
+            mul22(R, B, T);  // k = 0: R = I, T = B_c
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                out[size_t(k) * 8 + q] = S[q];
+                out[size_t(k) * 8 + 4 + q] = T[q];
+            }
+        }
+    }
+    for (int q = t; q < K2 * K2; q += kWave) Mx[q / K2][q % K2] = 0.0;
+    __syncthreads();
+
+    // Z over the common chain steps (one thread per bus pair), W^T B y, M
+    if (t < m * m) {
+        int const c = t / m, d = t % m;
+        int const bc = ld(xb, c), bd = ld(xb, d);
+        int const oc = ld(g.ch_off, bc), od = ld(g.ch_off, bd);
+        int kc = ld(g.ch_len, bc) - 1, kd = ld(g.ch_len, bd) - 1;
+        double const* sc = st + size_t(c) * g.ch_max * 8;
+        double const* sd = st + size_t(d) * g.ch_max * 8;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        while (kc >= 0 && kd >= 0 && ld(g.ch_node, oc + kc) == ld(g.ch_node, od + kd)) {
+            double const Tc[4] = {sc[kc * 8 + 4], sc[kc * 8 + 5], sc[kc * 8 + 6], sc[kc * 8 + 7]};
+            double const Sd[4] = {sd[kd * 8], sd[kd * 8 + 1], sd[kd * 8 + 2], sd[kd * 8 + 3]};
+            double prod[4];
+            mul22(Tc, Sd, prod);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] += prod[q];
+            --kc;
+            --kd;
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 2; ++r) Z[2 * c + r][2 * d + j] = acc[j * 2 + r];
+    }
+    if (t < m) {
+        int const c = ld(xb, t);
+        int const off = ld(g.ch_off, c), len = ld(g.ch_len, c);
+        double const* sc = st + size_t(t) * g.ch_max * 8;
+        double a0 = 0.0, a1 = 0.0;
+        for (int k = 0; k < len; ++k) {
+            int const i = ld(g.ch_node, off + k);
+            double const y0 = g.y[(size_t(i) * 2) * sp + s], y1 = g.y[(size_t(i) * 2 + 1) * sp + s];
+            a0 += sc[k * 8 + 4] * y0 + sc[k * 8 + 6] * y1;
+            a1 += sc[k * 8 + 5] * y0 + sc[k * 8 + 7] * y1;
+        }
+        w0[2 * t] = a0;
+        w0[2 * t + 1] = a1;
+    }
+    if (t < nx) {
+        size_t const q = size_t(s) * kExtraMax + t;
+        int const rb = ld(g.xrow, q), cb = ld(g.xcol, q);
+        int ri = 0, ci = 0;
+        for (int k = 0; k < m; ++k) {
+            int const b = ld(xb, k);
+            ri = b == rb ? k : ri;
+            ci = b == cb ? k : ci;
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 2; ++r) Mx[2 * ri + r][2 * ci + j] = g.xblk[q * 4 + j * 2 + r];
+    }
+    __syncthreads();
+
+    // (I + M Z) z = M w0, Gaussian elimination with partial pivoting
+    if (t == 0) {
+        int const n2 = 2 * m;
+        for (int r = 0; r < n2; ++r) {
+            double rhs = 0.0;
+            for (int k = 0; k < n2; ++k) rhs += Mx[r][k] * w0[k];
+            Km[r][n2] = rhs;
+            for (int c = 0; c < n2; ++c) {
+                double acc = r == c ? 1.0 : 0.0;
+                for (int k = 0; k < n2; ++k) acc += Mx[r][k] * Z[k][c];
+                Km[r][c] = acc;
+            }
+        }
+        int bad = 0;
+        for (int c = 0; c < n2; ++c) {
+            int pr = c;
+            for (int r = c + 1; r < n2; ++r)
+                if (fabs(Km[r][c]) > fabs(Km[pr][c])) pr = r;
+            if (!(fabs(Km[pr][c]) > 0.0) || !isfinite(Km[pr][c])) {
+                bad = 1;
+                break;
+            }
+            if (pr != c)
+                for (int k = c; k <= n2; ++k) {
+                    double const x = Km[c][k];
+                    Km[c][k] = Km[pr][k];
+                    Km[pr][k] = x;
+                }
+            for (int r = c + 1; r < n2; ++r) {
+                double const f = Km[r][c] / Km[c][c];
+                for (int k = c; k <= n2; ++k) Km[r][k] -= f * Km[c][k];
+            }
+        }
+        if (!bad)
+            for (int r = n2 - 1; r >= 0; --r) {
+                double v = Km[r][n2];
+                for (int k = r + 1; k < n2; ++k) v -= Km[r][k] * z[k];
+                z[r] = v / Km[r][r];
+            }
+        fail = bad;
+        if (bad) {
+            g.done[size_t(mm) * sp + s] = FAILED;
+            g.err[size_t(mm) * sp + s] = ERR_SPARSE;
+        }
+    }
+    __syncthreads();
+    if (fail) return;
+
+    // y -= F W z along the chains, one bus after the other (chains share their upper steps)
+    for (int d = 0; d < m; ++d) {
+        int const bd = ld(xb, d);
+        int const off = ld(g.ch_off, bd), len = ld(g.ch_len, bd);
+        double const* sd = st + size_t(d) * g.ch_max * 8;
+        double const z0 = z[2 * d], z1 = z[2 * d + 1];
+        for (int k = t; k < len; k += kWave) {
+            int const i = ld(g.ch_node, off + k);
+            g.y[(size_t(i) * 2) * sp + s] -= sd[k * 8] * z0 + sd[k * 8 + 2] * z1;
+            g.y[(size_t(i) * 2 + 1) * sp + s] -= sd[k * 8 + 1] * z0 + sd[k * 8 + 3] * z1;
+        }
+        __syncthreads();
     }
 }
 
@@ -1551,6 +1795,13 @@ void launch_bus_results(int P, DevOutput const* outs, long long const* offsets, 
         hipLaunchKernelGGL(bus_results_kernel<1>, grid, dim3(256), 0, st, outs, offsets, n_groups);
     else
         hipLaunchKernelGGL(bus_results_kernel<3>, grid, dim3(256), 0, st, outs, offsets, n_groups);
+}
+void launch_lowrank(DevGroup const* groups, ExtraWork const* work, int n_work, bool fp32, hipStream_t st) {
+    if (n_work == 0) return;
+    if (fp32)
+        hipLaunchKernelGGL(lowrank_kernel<float>, dim3(unsigned(n_work)), dim3(kWave), 0, st, groups, work);
+    else
+        hipLaunchKernelGGL(lowrank_kernel<double>, dim3(unsigned(n_work)), dim3(kWave), 0, st, groups, work);
 }
 void launch_node_records(int P, DevOutput const* outs, NodeRec const* recs, long long const* tile_off, int n_groups,
                          long long tiles, void* out, hipStream_t st) {

@@ -77,7 +77,38 @@ struct DevGroup {
     // raw block LU (test hook): input blocks and rhs
     double const* raw_a = nullptr;  // [E][4P^2][S_pad]
     double const* raw_b = nullptr;  // [n][2P][S_pad]
+    // union groups whose lanes close branches that the shared structure has open (symmetric, radial
+    // structure): every such branch adds two off-pattern admittances (a, b) and (b, a) in its lane. The
+    // sweeps factorise the matrix without them; a low-rank correction between the forward and the
+    // backward sweep (Woodbury) makes the solve exact. Per lane: the extra entries (row and column bus,
+    // admittance, and the Jacobian or admittance block the forward sweep computes for it) and the
+    // distinct buses they touch; per bus: whether any lane has an extra entry in that row, and the
+    // chain of pivots from the bus to the root of its elimination tree (node, U entry of the previous
+    // chain node in the node's column)
+    int8_t const* xbus = nullptr;   // [n]
+    int const* xcnt = nullptr;      // [S_pad]
+    int const* xrow = nullptr;      // [S_pad][kExtraMax] bus
+    int const* xcol = nullptr;      // [S_pad][kExtraMax] bus
+    double const* xy = nullptr;     // [S_pad][kExtraMax][2]
+    double* xblk = nullptr;         // [S_pad][kExtraMax][4]
+    int const* xm = nullptr;        // [S_pad] distinct buses
+    int const* xb = nullptr;        // [S_pad][kExtraMax]
+    int const* ch_off = nullptr;    // [n] first chain step of the bus (-1: no chain)
+    int const* ch_len = nullptr;    // [n]
+    int const* ch_node = nullptr;   // chain steps
+    int const* ch_ue = nullptr;     // U entry (previous node, node); -1 at the first step
+    int ch_max = 0;                 // longest chain
+    double* xst = nullptr;          // [lanes with extras][kExtraMax][ch_max][8]: S and T of every step
 };
+// extra (off-pattern) entries per lane of a union group: two per branch closed against the structure
+constexpr int kExtraMax = 8;
+// one block per lane with extra entries: (group, lane, slot of the lane's scratch)
+struct ExtraWork {
+    int group, lane, slot;
+};
+// low-rank correction of the union lanes with extra entries, between the forward and backward sweeps
+// (MODE_INIT: admittance blocks, MODE_NR: Jacobian blocks)
+void launch_lowrank(DevGroup const* groups, ExtraWork const* work, int n_work, bool fp32, hipStream_t st);
 
 // union groups: Y values of every lane from the shared values plus per-lane patches of the entries
 // that switched branches touch (patch q: lane, entry and P*P*2 values)

@@ -155,7 +155,7 @@ minimum_degree(std::vector<Idx> const& keys, std::vector<std::vector<Idx>> const
 // ------------------------------------------------------------------------------------------------
 // topology (topology.hpp:196-368 for the bus order, :370-560 for the coupling of components)
 // ------------------------------------------------------------------------------------------------
-Topology build_topology(Model const& m, ComponentState const& st) {
+Topology build_topology(Model const& m, ComponentState const& st, std::vector<char> const* couple_open) {
     Idx const nn = m.n_node();
     Idx const nb = m.n_branch();
     Idx const nl = m.n_line();
@@ -298,6 +298,8 @@ Topology build_topology(Model const& m, ComponentState const& st) {
             g = t.node_group[f];
         else if (bts(b) && t.node_group[to] != -1)
             g = t.node_group[to];
+        else if (couple_open && (*couple_open)[b] && t.node_group[f] != -1 && t.node_group[f] == t.node_group[to])
+            g = t.node_group[f];  // open on both sides here, closed in some lane of a union group
         if (g < 0) continue;
         auto& mm = t.models[g];
         t.branch_group[b] = g;

@@ -32,7 +32,9 @@ struct Topology {
     std::vector<Idx> lg_group, lg_pos, src_group, src_pos, sh_group, sh_pos;
 };
 
-Topology build_topology(Model const& m, ComponentState const& st);
+// couple_open (per component branch, optional): branches open on both sides in st that still become
+// math branches (both sides -1) of the model holding both their nodes; union groups close them per lane
+Topology build_topology(Model const& m, ComponentState const& st, std::vector<char> const* couple_open = nullptr);
 
 // minimum degree ordering on an adjacency map (keys sorted) -- returns (order, fills)
 std::pair<std::vector<Idx>, std::vector<std::pair<Idx, Idx>>>

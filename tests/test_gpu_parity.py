@@ -237,6 +237,37 @@ def test_union_group_switching(precision):
         np.testing.assert_allclose(out["node"][f], ref["node"][f], rtol=PQ_RTOL, atol=floor, err_msg=f"node.{f}")
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_closing_union_on_radial_base(precision):
+    """Year-profile batch on a small radial base: the switching scenarios close 1-3 normally open ties
+    between feeders and open 1-3 feeder lines (islands re-fed through a tie or left without a source).
+    Every switching scenario has its own meshed topology; they run as one union group on the radial base
+    structure, the closed ties entering through the low-rank correction. Results and FP64 iteration
+    counts equal the oracle's, which solves each scenario on its own topology."""
+    case = synthetic.year_20k(n_scenarios=160, n_feeder=6, feeder_nodes=60, n_ties=12, n_load=200,
+                              switching_fraction=0.5, kappa=0.2)
+    model = PowerGridModel(case.input)
+    batch = DeviceBatch(model, case.update, factor_precision=precision)
+    batch.run()
+    out = batch.fetch()
+    assert batch.stats()["groups"] == 2  # the base group and one union of every switching scenario
+    oupd, k = ol.uniform_update(case.update, case.input)
+    ref, it_ref, st_ref, _ = ol.power_flow(case.input, oupd, k, threading=0)
+    assert (st_ref == 0).all()
+    assert (ref["node"]["energized"] == 0).any()  # islands left without a source occur
+    if precision == "fp64":
+        assert (batch.iterations() == it_ref).all()
+    else:
+        assert (batch.iterations() <= it_ref + 2).all()
+    _compare(out, ref, True, slice(None))
+    for comp in ("sym_load", "source"):
+        assert (out[comp]["energized"] == ref[comp]["energized"]).all()
+    assert (out["line"]["energized"] == ref["line"]["energized"]).all()
+    for f in ("i_from", "i_to", "s_from", "s_to", "loading"):
+        floor = 1e-9 * max(np.max(np.abs(ref["line"][f])), 1.0)
+        np.testing.assert_allclose(out["line"][f], ref["line"][f], rtol=PQ_RTOL, atol=floor, err_msg=f"line.{f}")
+
+
 def _tie_case():
     """meshed_1000 plus 8 normally open ties; switching pattern k closes tie k and opens one line, and is
     repeated by 1, 2, 3, 5, 7, 11, 1 and 33 scenarios: packed groups of one to eleven scenarios and one
