@@ -1,14 +1,18 @@
 """Benchmark: batched Newton-Raphson power flow, scenarios/sec to 1e-8 pu on one MI355X.
 
 A step is one pass of the hot path over one batch: every scenario of the workload solved to
-convergence (linear initial guess, NR iterations, output records of every component) with the inputs
-already resident in HBM (PGM_amd_batch_run). The timed value is the mixed-precision path (FP32 factor
-and triangular solves, FP64 mismatch and state update); the plain FP64 GPU path and the CPU baseline
-(the FP64 CPU restatement, threaded like the reference's batch dispatch over all host cores, on a
-bounded sample) are measured in the same run.
+convergence (linear initial guess, NR iterations, output) with the inputs already resident in HBM
+(PGM_amd_batch_run). The output of a step is the timing output set, columnar: node u_pu and u_angle,
+line p/q at both ends, load p and q (--output records: every component as whole records, also
+reported as records_value). The timed value is the mixed-precision path (FP32 factor and triangular
+solves, FP64 mismatch and state update); the plain FP64 GPU path and the CPU baseline (the FP64 CPU
+restatement, threaded like the reference's batch dispatch over all host cores, on a bounded sample)
+are measured in the same run.
 
     python bench.py [--steps K] [--warmup W] [--workload year_20k] [--precision fp32|fp64]
+                    [--output timing|records]
 """
+
 
 from __future__ import annotations
 
@@ -26,6 +30,10 @@ sys.path.insert(0, str(ROOT / "power-grid-model_amd"))
 sys.path.insert(0, str(ROOT / "tests"))
 
 HBM_PEAK_GBS = 8000.0
+# the timing output set: columnar attributes per component (doubles per phase)
+TIMING_OUTPUT = {"node": ["u_pu", "u_angle"], "line": ["p_from", "q_from", "p_to", "q_to"],
+                 "transformer": ["p_from", "q_from", "p_to", "q_to"], "sym_load": ["p", "q"],
+                 "asym_load": ["p", "q"]}
 
 
 def log(msg):
@@ -98,6 +106,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--e2e-fraction", type=float, default=0.2,
                     help="share of the scenarios timed through calculate_power_flow from host buffers (0: skip)")
+    ap.add_argument("--output", default="timing", choices=["timing", "records"],
+                    help="timing: the columnar timing output set; records: whole records of every component")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain 64 MiB copy kernel (the calibration kernel for HBM counters)")
     args = ap.parse_args()
@@ -121,14 +131,18 @@ def main():
     log(f"[bench] generated {case.name}: {case.n_scenarios} scenarios in {time.perf_counter() - t0:.1f}s")
     model = PowerGridModel(case.input)
 
-    def measure(precision):
+    selection = {c: a for c, a in TIMING_OUTPUT.items() if c in case.input}
+
+    def measure(precision, output=args.output, steps=args.steps):
         t0 = time.perf_counter()
         batch = DeviceBatch(model, case.update, symmetric=case.symmetric, factor_precision=precision)
-        log(f"[bench] {precision}: batch created in {time.perf_counter() - t0:.1f}s")
+        if output == "timing":
+            batch.set_output(selection)
+        log(f"[bench] {precision}: batch created in {time.perf_counter() - t0:.1f}s (output: {output})")
         for _ in range(args.warmup):
             batch.run()
         times = []
-        for _ in range(args.steps):
+        for _ in range(steps):
             t0 = time.perf_counter()
             batch.run()
             times.append(time.perf_counter() - t0)
@@ -146,14 +160,24 @@ def main():
     n_loads = sum(len(case.input.get(c, [])) for c in ("sym_load", "asym_load", "sym_gen", "asym_gen"))
     out_bytes = 0
     for comp, arr in case.input.items():
-        rec = {"node": 48, "line": 80, "transformer": 80}.get(comp, 48) if case.symmetric else \
-            {"node": 128, "line": 208, "transformer": 208}.get(comp, 128)
+        if args.output == "timing":
+            rec = 8 * P * len(selection.get(comp, []))
+        else:
+            rec = {"node": 48, "line": 80, "transformer": 80}.get(comp, 48) if case.symmetric else \
+                {"node": 128, "line": 208, "transformer": 208}.get(comp, 128)
         out_bytes += rec * len(arr) * case.n_scenarios
     step_bytes = algorithmic_bytes_per_step(st, it.astype(np.float64), P, fp32, n_loads, case.n_scenarios,
                                             out_bytes)
     fwd_gbs = st["fwd_bytes"] / (st["fwd_total_ms"] * 1e-3) / 1e9 if st["fwd_total_ms"] > 0 else 0.0
     fwd_bytes_per_launch = st["fwd_bytes"] / max(st["fwd_timed"], 1)
     del batch
+
+    # the same path writing every component as whole records (3.1 MB per year_20k scenario)
+    records_value = None
+    if args.output == "timing":
+        brec, trec, _, _ = measure(args.precision, output="records", steps=max(1, min(args.steps, 5)))
+        records_value = case.n_scenarios / float(np.median(trec))
+        del brec
 
     fp64_value = None
     if fp32 and not args.no_fp64_leg:
@@ -207,7 +231,10 @@ def main():
         "config": {"workload": case.name, "scenarios": case.n_scenarios,
                    "buses": int(st["buses_per_scenario"]), "lu_entries": int(st["lu_entries"]),
                    "symmetric": case.symmetric, "load_scale_kappa": case.kappa, "topology_groups": int(st["groups"]),
-                   "err_tol": 1e-8, "failed_scenarios": failed, "mean_iterations": float(np.mean(it[it > 0]))},
+                   "err_tol": 1e-8, "failed_scenarios": failed, "mean_iterations": float(np.mean(it[it > 0])),
+                   "output": ("columnar " + ", ".join(f"{c}{{{','.join(a)}}}" for c, a in selection.items()))
+                   if args.output == "timing" else "whole records of every component"},
+        "records_value": records_value,
         "fp64_gpu_value": fp64_value,
         "mixed_over_fp64": (value / fp64_value) if fp64_value else None,
         "roofline": {"bound": "hbm", "achieved": fwd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

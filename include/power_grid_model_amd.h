@@ -238,6 +238,15 @@ PGM_AMD_API PGM_AmdBatch* PGM_amd_batch_create(PGM_Handle* handle, PGM_PowerGrid
                                                PGM_ConstDataset const* batch_dataset) PGM_AMD_NOEXCEPT;
 /* Run every scenario to convergence on the device; returns when the device work is complete. */
 PGM_AMD_API void PGM_amd_batch_run(PGM_Handle* handle, PGM_AmdBatch* batch) PGM_AMD_NOEXCEPT;
+/* Select what the following runs produce on the device, from the layout of an output dataset: only
+ * its components are computed, and a component given as attribute buffers (columnar) writes only
+ * those attributes, as device columns. Without this call a batch produces every component as whole
+ * records. The reference filters output the same way through the output dataset it is given
+ * (PGM_calculate, model.h:116-118; columnar buffers: PGM_dataset_mutable_add_attribute_buffer,
+ * dataset.h:312-323); here the filter also removes device work. Fetch then accepts datasets with the
+ * same components and a subset of the selected columns. */
+PGM_AMD_API void PGM_amd_batch_set_output(PGM_Handle* handle, PGM_AmdBatch* batch,
+                                          PGM_MutableDataset const* output) PGM_AMD_NOEXCEPT;
 /* Copy results into the caller's output dataset and report per-scenario errors like PGM_calculate. */
 PGM_AMD_API void PGM_amd_batch_fetch(PGM_Handle* handle, PGM_AmdBatch* batch,
                                      PGM_MutableDataset const* output_dataset) PGM_AMD_NOEXCEPT;

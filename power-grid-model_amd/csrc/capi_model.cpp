@@ -97,6 +97,8 @@ void PGM_calculate(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_Options co
                 "If batch_dataset is provided. Both batch_dataset and output_dataset should be a batch!\n");
         if (batch_dataset && batch_dataset->batch_size == 0) return;
         auto batch = create_batch(model->model, *opt, batch_dataset);
+        // only the requested components (and, for columnar buffers, attributes) are produced
+        batch->set_output(*output_dataset);
         batch->run();
         batch->fetch(*output_dataset, handle);
     });
@@ -115,6 +117,9 @@ PGM_AmdBatch* PGM_amd_batch_create(PGM_Handle* handle, PGM_PowerGridModel const*
 }
 void PGM_amd_batch_run(PGM_Handle* handle, PGM_AmdBatch* batch) noexcept {
     guarded(handle, [&] { batch->b->run(); });
+}
+void PGM_amd_batch_set_output(PGM_Handle* handle, PGM_AmdBatch* batch, PGM_MutableDataset const* output) noexcept {
+    guarded(handle, [&] { batch->b->set_output(*output); });
 }
 void PGM_amd_batch_fetch(PGM_Handle* handle, PGM_AmdBatch* batch, PGM_MutableDataset const* output) noexcept {
     guarded(handle, [&] { batch->b->fetch(*output, handle); });

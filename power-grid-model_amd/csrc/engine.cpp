@@ -833,7 +833,7 @@ struct UpdateReader {
 
 }  // namespace
 
-Batch::~Batch() {
+void Batch::release() {
     groups.clear();
     for (auto e : event_pool) (void)hipEventDestroy(e);
     if (h_active) (void)hipHostFree(h_active);
@@ -1426,13 +1426,33 @@ void Batch::run() {
     }
     PGM_HIP_CHECK(hipMemcpy(d_ok, ok.data(), ok.size() * 4, hipMemcpyHostToDevice));
     int const G = int(groups.size());
-    launch_bus_results(P, d_outputs, d_bus_off, G, bus_total, st);
-    launch_node_records(P, d_outputs, d_node_recs, d_rec_off[C_NODE], G, rec_total[C_NODE], out[C_NODE], st);
-    launch_branch_records(P, d_outputs, d_branch_recs[0], d_rec_off[C_LINE], G, rec_total[C_LINE], out[C_LINE], st);
-    launch_branch_records(P, d_outputs, d_branch_recs[1], d_rec_off[C_TRANSFORMER], G, rec_total[C_TRANSFORMER],
-                          out[C_TRANSFORMER], st);
+    auto column_set = [&](int c) {
+        ColumnSet cs;
+        for (auto const& col : cols[c]) {
+            cs.off[cs.n] = int(col.attr->offset);
+            cs.size[cs.n] = int(col.attr->size);
+            cs.dst[cs.n] = static_cast<char*>(col.dev);
+            ++cs.n;
+        }
+        return cs;
+    };
+    // appliance injections and bus sums: needed by the node and appliance records
+    bool const appl = want[C_SHUNT] || want[C_SOURCE] || want[C_SYM_GEN] || want[C_ASYM_GEN] || want[C_SYM_LOAD] ||
+                      want[C_ASYM_LOAD];
+    if (appl || want[C_NODE]) launch_bus_results(P, d_outputs, d_bus_off, G, bus_total, st);
+    if (want[C_NODE])
+        launch_node_records(P, d_outputs, d_node_recs, d_rec_off[C_NODE], G, rec_total[C_NODE], out[C_NODE],
+                            column_set(C_NODE), st);
+    if (want[C_LINE])
+        launch_branch_records(P, d_outputs, d_branch_recs[0], d_rec_off[C_LINE], G, rec_total[C_LINE], out[C_LINE],
+                              column_set(C_LINE), st);
+    if (want[C_TRANSFORMER])
+        launch_branch_records(P, d_outputs, d_branch_recs[1], d_rec_off[C_TRANSFORMER], G, rec_total[C_TRANSFORMER],
+                              out[C_TRANSFORMER], column_set(C_TRANSFORMER), st);
     for (int c : {int(C_SHUNT), int(C_SOURCE), int(C_SYM_GEN), int(C_ASYM_GEN), int(C_SYM_LOAD), int(C_ASYM_LOAD)})
-        launch_appliance_records(P, d_outputs, d_appl_recs[c], d_rec_off[c], G, rec_total[c], out[c], st);
+        if (want[c])
+            launch_appliance_records(P, d_outputs, d_appl_recs[c], d_rec_off[c], G, rec_total[c], out[c],
+                                     column_set(c), st);
     PGM_HIP_CHECK(hipGetLastError());
     float ms = 0;
     if (timing) {
@@ -1455,25 +1475,97 @@ void Batch::run() {
     ran = true;
 }
 
-void Batch::fetch(PGM_MutableDataset const& output, PGM_Handle* handle) {
-    if (!ran) throw PowerGridError("the batch has not been run\n");
+namespace {
+void check_output_dataset(PGM_MutableDataset const& output, bool symmetric, Idx n_scenarios) {
     bool const want_asym = output.meta->name == "asym_output";
     if (output.meta->name != "sym_output" && output.meta->name != "asym_output")
         throw PowerGridError("output dataset must be sym_output or asym_output\n");
     if (want_asym == symmetric) throw PowerGridError("output dataset symmetry does not match the calculation\n");
     if (output.batch_size != n_scenarios)
         throw PowerGridError("output batch size does not match the number of scenarios\n");
-    std::vector<char> host;
+}
+int comp_slot(std::string const& name) {
+    for (int c = 0; c < N_COMP; ++c)
+        if (name == comp_names[c]) return c;
+    return -1;
+}
+}  // namespace
+
+Batch::~Batch() {
+    for (void* p : col_mem) (void)hipFree(p);
+    release();
+}
+
+void Batch::set_output(PGM_MutableDataset const& output) {
+    check_output_dataset(output, symmetric, n_scenarios);
+    for (void* p : col_mem) PGM_HIP_CHECK(hipFree(p));
+    col_mem.clear();
+    for (int c = 0; c < N_COMP; ++c) {
+        want[c] = false;
+        cols[c].clear();
+    }
     for (auto const& buf : output.buffers) {
-        int comp = -1;
-        for (int c = 0; c < N_COMP; ++c)
-            if (buf.meta->name == comp_names[c]) comp = c;
+        int const comp = comp_slot(buf.meta->name);
+        if (comp < 0) continue;
+        want[comp] = true;
+        if (!buf.columnar()) continue;
+        if (buf.columns.size() > size_t(kMaxColumns)) throw PowerGridError("too many output columns\n");
+        Idx const cnt = model.count(comp);
+        for (auto const& [attr, host] : buf.columns) {
+            void* d = nullptr;
+            size_t const bytes = std::max<size_t>(size_t(n_scenarios) * size_t(cnt) * attr->size, 1);
+            PGM_HIP_CHECK(hipMalloc(&d, bytes));
+            col_mem.push_back(d);
+            cols[comp].push_back(DevColumn{attr, d});
+        }
+    }
+    ran = false;
+}
+
+void Batch::fetch(PGM_MutableDataset const& output, PGM_Handle* handle) {
+    if (!ran) throw PowerGridError("the batch has not been run\n");
+    check_output_dataset(output, symmetric, n_scenarios);
+    std::vector<char> host;
+    bool const all_ok = std::all_of(status.begin(), status.end(), [](int x) { return x == 0; });
+    for (auto const& buf : output.buffers) {
+        int const comp = comp_slot(buf.meta->name);
         if (comp < 0) continue;
         Idx const cnt = model.count(comp);
         if (cnt == 0) continue;
+        if (!want[comp])
+            throw PowerGridError("component " + buf.meta->name + " was not selected for output of this batch\n");
+        if (!cols[comp].empty()) {
+            // device columns: each requested column must have been produced
+            if (!buf.columnar())
+                throw PowerGridError("component " + buf.meta->name + " was selected as columns for this batch\n");
+            for (auto const& [attr, hcol] : buf.columns) {
+                void const* dcol = nullptr;
+                for (auto const& dc : cols[comp])
+                    if (dc.attr == attr) dcol = dc.dev;
+                if (!dcol)
+                    throw PowerGridError("attribute " + attr->name + " of " + buf.meta->name +
+                                         " was not selected for output of this batch\n");
+                size_t const as = attr->size;
+                if (all_ok && buf.elements_per_scenario == cnt) {
+                    PGM_HIP_CHECK(hipMemcpy(hcol, dcol, size_t(n_scenarios) * cnt * as, hipMemcpyDeviceToHost));
+                    continue;
+                }
+                host.resize(size_t(n_scenarios) * cnt * as);
+                PGM_HIP_CHECK(hipMemcpy(host.data(), dcol, host.size(), hipMemcpyDeviceToHost));
+                for (Idx s = 0; s < n_scenarios; ++s) {
+                    if (status[s] != 0) continue;
+                    Idx const b0 = buf.begin(s), b1 = buf.end(s);
+                    if (b1 - b0 != cnt)
+                        throw PowerGridError("output buffer of " + buf.meta->name +
+                                             " must hold every element per scenario\n");
+                    std::memcpy(static_cast<char*>(hcol) + size_t(b0) * as, host.data() + size_t(s) * cnt * as,
+                                size_t(cnt) * as);
+                }
+            }
+            continue;
+        }
         size_t const rs = rec_size[comp];
         bool const direct = !buf.columnar() && buf.elements_per_scenario == cnt;
-        bool const all_ok = std::all_of(status.begin(), status.end(), [](int x) { return x == 0; });
         if (direct && all_ok) {
             PGM_HIP_CHECK(hipMemcpy(buf.data, out[comp], size_t(n_scenarios) * cnt * rs, hipMemcpyDeviceToHost));
             continue;

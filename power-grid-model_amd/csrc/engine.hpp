@@ -88,6 +88,15 @@ struct Batch {
     // output records on the device, per component slot: n_scenarios * count records
     void* out[N_COMP] = {};
     size_t rec_size[N_COMP] = {};
+    // output selection (set_output): the components a run produces; a component given as attribute
+    // columns writes only those columns, each [n_scenarios][count] on the device
+    bool want[N_COMP] = {true, true, true, true, true, true, true, true, true};
+    struct DevColumn {
+        PGM_MetaAttribute const* attr;
+        void* dev;
+    };
+    std::vector<DevColumn> cols[N_COMP];
+    std::vector<void*> col_mem;
     // results of the last run
     std::vector<int> status;  // 0 ok, 1 sparse, 2 diverge, 3 update error
     std::vector<Idx> iterations;
@@ -103,7 +112,9 @@ struct Batch {
 
     Batch() = default;
     ~Batch();
+    void release();
     void run();
+    void set_output(PGM_MutableDataset const& output);
     void fetch(PGM_MutableDataset const& output, PGM_Handle* handle);
 };
 

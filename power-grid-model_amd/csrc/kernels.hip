@@ -1604,10 +1604,12 @@ __device__ __forceinline__ void make_record(DevOutput const& o, ApplRec const& r
 // scenario-interleaved state with lane-contiguous (coalesced) loads, then writes each lane's records as
 // one contiguous run of 16-byte stores: both sides of the transpose stay coalesced, for wide groups and
 // for single-scenario ones. Tile t of group g: offsets[g] <= t.
+// With columns (cols.n > 0) only the listed attributes leave the tile, each into its own column: the
+// lanes of a tile row write consecutive elements of one scenario's column.
 template <int P, class Rec, class Out>
 __global__ __launch_bounds__(256) void records_kernel(DevOutput const* __restrict__ outs, Rec const* __restrict__ recs,
                                                       long long const* __restrict__ tile_off, int n_groups,
-                                                      Out* __restrict__ out) {
+                                                      Out* __restrict__ out, ColumnSet const cols) {
     constexpr int SLOTS = kWaveLanes * record_tile(sizeof(Out));
     constexpr int CH = int(sizeof(Out) / 16);
     static_assert(sizeof(Out) % 16 == 0, "records are copied in 16-byte chunks");
@@ -1636,6 +1638,27 @@ __global__ __launch_bounds__(256) void records_kernel(DevOutput const* __restric
     }
     __syncthreads();
     int const n_valid = r.n_comp - i0 < TNg ? r.n_comp - i0 : TNg;
+    if (cols.n > 0) {
+        for (int a = 0; a < cols.n; ++a) {
+            int const off = cols.off[a], sz = cols.size[a];
+            for (int c = int(threadIdx.x); c < W * TNg; c += int(blockDim.x)) {
+                int const ln = c / TNg, k = c % TNg;
+                int const sl = s0 + ln;
+                if (k >= n_valid || sl >= o.S || !o.ok[sl]) continue;
+                char const* src = reinterpret_cast<char const*>(&tile[ln * TNg + k]) + off;
+                char* dst = cols.dst[a] + (size_t(ld(o.scenario_index, sl)) * r.n_comp + i0 + k) * size_t(sz);
+                if (sz % 8 == 0) {
+                    for (int q = 0; q < sz / 8; ++q)
+                        reinterpret_cast<double*>(dst)[q] = reinterpret_cast<double const*>(src)[q];
+                } else if (sz == 4) {
+                    *reinterpret_cast<int*>(dst) = *reinterpret_cast<int const*>(src);
+                } else {
+                    for (int q = 0; q < sz; ++q) dst[q] = src[q];
+                }
+            }
+        }
+        return;
+    }
     for (int c = int(threadIdx.x); c < W * TNg * CH; c += int(blockDim.x)) {
         int const ln = c / (TNg * CH), k = c % (TNg * CH);
         int const sl = s0 + ln;
@@ -1804,34 +1827,34 @@ void launch_lowrank(DevGroup const* groups, ExtraWork const* work, int n_work, b
         hipLaunchKernelGGL(lowrank_kernel<double>, dim3(unsigned(n_work)), dim3(kWave), 0, st, groups, work);
 }
 void launch_node_records(int P, DevOutput const* outs, NodeRec const* recs, long long const* tile_off, int n_groups,
-                         long long tiles, void* out, hipStream_t st) {
+                         long long tiles, void* out, ColumnSet const& cols, hipStream_t st) {
     if (tiles == 0) return;
     if (P == 1)
         hipLaunchKernelGGL((records_kernel<1, NodeRec, NodeOutput<1>>), dim3(unsigned(tiles)), dim3(256), 0, st, outs,
-                           recs, tile_off, n_groups, static_cast<NodeOutput<1>*>(out));
+                           recs, tile_off, n_groups, static_cast<NodeOutput<1>*>(out), cols);
     else
         hipLaunchKernelGGL((records_kernel<3, NodeRec, NodeOutput<3>>), dim3(unsigned(tiles)), dim3(256), 0, st, outs,
-                           recs, tile_off, n_groups, static_cast<NodeOutput<3>*>(out));
+                           recs, tile_off, n_groups, static_cast<NodeOutput<3>*>(out), cols);
 }
 void launch_branch_records(int P, DevOutput const* outs, BranchRec const* recs, long long const* tile_off,
-                           int n_groups, long long tiles, void* out, hipStream_t st) {
+                           int n_groups, long long tiles, void* out, ColumnSet const& cols, hipStream_t st) {
     if (tiles == 0) return;
     if (P == 1)
         hipLaunchKernelGGL((records_kernel<1, BranchRec, BranchOutput<1>>), dim3(unsigned(tiles)), dim3(256), 0, st,
-                           outs, recs, tile_off, n_groups, static_cast<BranchOutput<1>*>(out));
+                           outs, recs, tile_off, n_groups, static_cast<BranchOutput<1>*>(out), cols);
     else
         hipLaunchKernelGGL((records_kernel<3, BranchRec, BranchOutput<3>>), dim3(unsigned(tiles)), dim3(256), 0, st,
-                           outs, recs, tile_off, n_groups, static_cast<BranchOutput<3>*>(out));
+                           outs, recs, tile_off, n_groups, static_cast<BranchOutput<3>*>(out), cols);
 }
 void launch_appliance_records(int P, DevOutput const* outs, ApplRec const* recs, long long const* tile_off,
-                              int n_groups, long long tiles, void* out, hipStream_t st) {
+                              int n_groups, long long tiles, void* out, ColumnSet const& cols, hipStream_t st) {
     if (tiles == 0) return;
     if (P == 1)
         hipLaunchKernelGGL((records_kernel<1, ApplRec, ApplianceOutput<1>>), dim3(unsigned(tiles)), dim3(256), 0, st,
-                           outs, recs, tile_off, n_groups, static_cast<ApplianceOutput<1>*>(out));
+                           outs, recs, tile_off, n_groups, static_cast<ApplianceOutput<1>*>(out), cols);
     else
         hipLaunchKernelGGL((records_kernel<3, ApplRec, ApplianceOutput<3>>), dim3(unsigned(tiles)), dim3(256), 0, st,
-                           outs, recs, tile_off, n_groups, static_cast<ApplianceOutput<3>*>(out));
+                           outs, recs, tile_off, n_groups, static_cast<ApplianceOutput<3>*>(out), cols);
 }
 
 }  // namespace pgm_amd

@@ -236,11 +236,20 @@ constexpr int record_tile(size_t record_bytes) {
 // W = min(S_pad, 64) lanes per tile)
 void launch_bus_results(int P, DevOutput const* outs, long long const* offsets, int n_groups, long long total,
                         hipStream_t st);
+// attribute columns a record writer produces instead of whole records (columnar output): byte offset
+// and size of the attribute in the record, and the column [n_scenarios][n_comp] (n = 0: whole records)
+constexpr int kMaxColumns = 16;
+struct ColumnSet {
+    int n = 0;
+    int off[kMaxColumns] = {};
+    int size[kMaxColumns] = {};
+    char* dst[kMaxColumns] = {};
+};
 void launch_node_records(int P, DevOutput const* outs, NodeRec const* recs, long long const* offsets, int n_groups,
-                         long long total, void* out, hipStream_t st);
+                         long long total, void* out, ColumnSet const& cols, hipStream_t st);
 void launch_branch_records(int P, DevOutput const* outs, BranchRec const* recs, long long const* offsets,
-                           int n_groups, long long total, void* out, hipStream_t st);
+                           int n_groups, long long total, void* out, ColumnSet const& cols, hipStream_t st);
 void launch_appliance_records(int P, DevOutput const* outs, ApplRec const* recs, long long const* offsets,
-                              int n_groups, long long total, void* out, hipStream_t st);
+                              int n_groups, long long total, void* out, ColumnSet const& cols, hipStream_t st);
 
 }  // namespace pgm_amd

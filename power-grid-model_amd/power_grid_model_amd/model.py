@@ -370,8 +370,23 @@ class DeviceBatch:
         lib.PGM_amd_batch_run(self._handle.ptr, self._ptr)
         self._handle.check()
 
+    def set_output(self, output_component_types=None) -> None:
+        """Select what the following runs produce on the device, with the meaning of calculate_power_flow's
+        output_component_types: only these components are computed, and components given with attribute
+        lists (or as columnar through ComponentAttributeFilterOptions) write only those attributes.
+        fetch() then returns that selection."""
+        is_batch = self._upd is not None
+        out_type, output = self._model._output_arrays(output_component_types, self.symmetric, is_batch,
+                                                      self.n_scenarios)
+        out_ds = _make_mutable_dataset(self._handle, output, out_type, is_batch, self.n_scenarios)
+        lib.PGM_amd_batch_set_output(self._handle.ptr, self._ptr, out_ds.ptr)
+        self._handle.check()
+        self._output_types = output_component_types
+
     def fetch(self, output_component_types=None, continue_on_batch_error: bool = True) -> dict:
         is_batch = self._upd is not None
+        if output_component_types is None:
+            output_component_types = getattr(self, "_output_types", None)
         out_type, output = self._model._output_arrays(output_component_types, self.symmetric, is_batch,
                                                       self.n_scenarios)
         out_ds = _make_mutable_dataset(self._handle, output, out_type, is_batch, self.n_scenarios)

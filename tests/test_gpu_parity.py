@@ -429,6 +429,34 @@ def test_columnar_attribute_filtered_output():
     assert everything["line"]["i_from"].tobytes() == rows["line"]["i_from"].tobytes()
 
 
+def test_device_output_selection():
+    """set_output: only the selected components run, columnar ones write only their attributes on the
+    device; the values equal the whole-record output byte for byte. Fetching a component that was not
+    selected is an error."""
+    case = synthetic.year_20k(n_scenarios=48, n_feeder=4, feeder_nodes=40, n_ties=8, n_load=100,
+                              switching_fraction=0.5, kappa=0.2)
+    model = PowerGridModel(case.input)
+    batch = DeviceBatch(model, case.update, factor_precision="fp32")
+    batch.run()
+    rows = batch.fetch()
+    sel = {"node": ["u_pu", "u_angle", "energized"], "line": ["p_from", "q_from", "p_to", "q_to", "id"],
+           "sym_load": ["p", "q"]}
+    batch.set_output(sel)
+    batch.run()
+    cols = batch.fetch()
+    assert set(cols) == set(sel)
+    for comp, attrs in sel.items():
+        assert set(cols[comp]) == set(attrs)
+        for a in attrs:
+            assert cols[comp][a].tobytes() == rows[comp][a].tobytes(), (comp, a)
+    with pytest.raises(Exception, match="not selected"):
+        batch.fetch(output_component_types=["source"])
+    batch.set_output(["node"])
+    batch.run()
+    only = batch.fetch()
+    assert set(only) == {"node"} and only["node"].tobytes() == rows["node"].tobytes()
+
+
 def test_error_cases_match_oracle():
     case = synthetic.radial_200(n_scenarios=8)
     upd = {k: v.copy() for k, v in case.update.items()}
