@@ -1279,7 +1279,7 @@ __device__ __forceinline__ int find_group(long long const* __restrict__ offsets,
     int lo = 0, hi = n_groups;
     while (hi - lo > 1) {
         int const mid = (lo + hi) >> 1;
-        if (offsets[mid] <= idx)
+        if (ld(offsets, mid) <= idx)
             lo = mid;
         else
             hi = mid;
@@ -1325,16 +1325,8 @@ __global__ void ic_start_kernel(DevGroup const* __restrict__ groups, long long c
 // results per bus: load powers and currents, bus injection and source results, shunt flows
 // ---------------------------------------------------------------------------------------------------
 template <int P>
-__global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long const* __restrict__ offsets,
-                                   int n_groups) {
-    long long const gidx = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gidx >= offsets[n_groups]) return;
-    int const gr = find_group(offsets, n_groups, gidx);
-    DevOutput const o = ld(outs, gr);
-    size_t const idx = size_t(gidx - offsets[gr]);
+__device__ __forceinline__ void bus_result(DevOutput const& o, int b, int s) {
     size_t const sp = o.S_pad;
-    int const b = int(idx / sp);
-    int const s = int(idx % sp);
     if (s >= o.S || !o.ok[s] || ld(o.bus_mm, b) < 0) return;
     if (o.dead && o.dead[size_t(b) * sp + s]) {
         // not energised in this lane: the appliances of the bus carry nothing
@@ -1480,6 +1472,35 @@ __global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long
     }
 }
 
+// one thread per (bus, lane); a wave whose 64 threads are 64 lanes of one bus (groups of 64 lanes
+// and more) takes the bus from wave-uniform values, so the bus's structure is read with scalar loads
+template <int P>
+__global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long const* __restrict__ offsets,
+                                   int n_groups) {
+    long long const total = ld(offsets, n_groups);
+    long long const wbase = (long long)(blockIdx.x) * blockDim.x +
+                            (long long)(__builtin_amdgcn_readfirstlane(int(threadIdx.x) / kWave)) * kWave;
+    if (wbase >= total) return;
+    int const lane = int(threadIdx.x) % kWave;
+    {
+        int const gr = find_group(offsets, n_groups, wbase);
+        DevOutput const o = ld(outs, gr);
+        size_t const sp = o.S_pad;
+        size_t const idx0 = size_t(wbase - ld(offsets, gr));
+        size_t const s0 = idx0 % sp;
+        if (sp % kWave == 0 && s0 + kWave <= sp) {
+            bus_result<P>(o, int(idx0 / sp), int(s0) + lane);
+            return;
+        }
+    }
+    long long const gidx = wbase + lane;
+    if (gidx >= total) return;
+    int const gr = find_group(offsets, n_groups, gidx);
+    DevOutput const o = ld(outs, gr);
+    size_t const idx = size_t(gidx - ld(offsets, gr));
+    bus_result<P>(o, int(idx / o.S_pad), int(idx % o.S_pad));
+}
+
 // ---------------------------------------------------------------------------------------------------
 // record writers (node.hpp, branch.hpp, appliance.hpp get_output)
 // ---------------------------------------------------------------------------------------------------
@@ -1487,18 +1508,18 @@ __global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long
 template <int P>
 __device__ __forceinline__ void node_record(DevOutput const& o, NodeRec const& r, int i, int s, NodeOutput<P>& rec) {
     size_t const sp = o.S_pad;
-    rec.id = r.id[i];
-    int const bus = r.bus[i];
+    rec.id = ld(r.id, i);
+    int const bus = ld(r.bus, i);
     double const base_power = P == 1 ? 1e6 : 1e6 / 3.0;
     double const u_scale = P == 1 ? 1.0 : 0.57735026918962576451;
-    if (bus >= 0 && !(o.dead && o.dead[size_t(bus) * sp + s])) {
+    if (bus >= 0 && !(o.dead && ld(o.dead, size_t(bus) * sp + s))) {
         rec.energized = 1;
 #pragma unroll
         for (int ph = 0; ph < P; ++ph) {
             double const ur = o.u[((size_t(bus) * P + ph) * 2) * sp + s];
             double const ui = o.u[((size_t(bus) * P + ph) * 2 + 1) * sp + s];
             rec.u_pu[ph] = sqrt(ur * ur + ui * ui);
-            rec.u[ph] = u_scale * r.u_rated[i] * rec.u_pu[ph];
+            rec.u[ph] = u_scale * ld(r.u_rated, i) * rec.u_pu[ph];
             rec.u_angle[ph] = atan2(ui, ur);
             rec.p[ph] = base_power * o.bus_inj[((size_t(bus) * P + ph) * 2) * sp + s];
             rec.q[ph] = base_power * o.bus_inj[((size_t(bus) * P + ph) * 2 + 1) * sp + s];
@@ -1509,19 +1530,19 @@ __device__ __forceinline__ void node_record(DevOutput const& o, NodeRec const& r
 template <int P>
 __device__ __forceinline__ void branch_record(DevOutput const& o, BranchRec const& r, int i, int s, BranchOutput<P>& rec) {
     size_t const sp = o.S_pad;
-    rec.id = r.id[i];
-    int const m = r.math[i];
+    rec.id = ld(r.id, i);
+    int const m = ld(r.math, i);
     double const base_power = P == 1 ? 1e6 : 1e6 / 3.0;
-    int f = m >= 0 ? r.from_bus[m] : -1, t = m >= 0 ? r.to_bus[m] : -1;
+    int f = m >= 0 ? ld(r.from_bus, m) : -1, t = m >= 0 ? ld(r.to_bus, m) : -1;
     double const* y = m >= 0 ? r.branch_y + size_t(m) * 4 * P * P * 2 : nullptr;
     bool energized = m >= 0;
     if (m >= 0 && r.lane_status) {
         // union group: the side statuses of this lane pick the admittance; energised when a closed
         // side meets a live bus
-        int const stt = r.lane_status[size_t(m) * sp + s];
+        int const stt = ld(r.lane_status, size_t(m) * sp + s);
         if (!(stt & 1)) f = -1;
         if (!(stt & 2)) t = -1;
-        energized = (f >= 0 && !o.dead[size_t(f) * sp + s]) || (t >= 0 && !o.dead[size_t(t) * sp + s]);
+        energized = (f >= 0 && !ld(o.dead, size_t(f) * sp + s)) || (t >= 0 && !ld(o.dead, size_t(t) * sp + s));
         y = r.branch_y_var + (size_t(m) * 4 + stt) * 4 * P * P * 2;
     }
     if (energized) {
@@ -1534,7 +1555,9 @@ __device__ __forceinline__ void branch_record(DevOutput const& o, BranchRec cons
             ut[ph] = t >= 0 ? cplx{o.u[((size_t(t) * P + ph) * 2) * sp + s], o.u[((size_t(t) * P + ph) * 2 + 1) * sp + s]}
                             : cplx{0.0, 0.0};
         }
-        auto Y = [&](int w, int rr, int cc) { return cplx{y[((w * P + rr) * P + cc) * 2], y[((w * P + rr) * P + cc) * 2 + 1]}; };
+        auto Y = [&](int w, int rr, int cc) {
+            return cplx{ld(y, ((w * P + rr) * P + cc) * 2), ld(y, ((w * P + rr) * P + cc) * 2 + 1)};
+        };
         double sum_sf = 0, sum_st = 0, max_i = 0;
 #pragma unroll
         for (int ph = 0; ph < P; ++ph) {
@@ -1548,30 +1571,31 @@ __device__ __forceinline__ void branch_record(DevOutput const& o, BranchRec cons
             cplx const st = cmul(ut[ph], cconj(ito));
             rec.p_from[ph] = base_power * sf.re;
             rec.q_from[ph] = base_power * sf.im;
-            rec.i_from[ph] = r.base_i_from[i] * cabs_(ifr);
+            rec.i_from[ph] = ld(r.base_i_from, i) * cabs_(ifr);
             rec.s_from[ph] = base_power * cabs_(sf);
             rec.p_to[ph] = base_power * st.re;
             rec.q_to[ph] = base_power * st.im;
-            rec.i_to[ph] = r.base_i_to[i] * cabs_(ito);
+            rec.i_to[ph] = ld(r.base_i_to, i) * cabs_(ito);
             rec.s_to[ph] = base_power * cabs_(st);
             sum_sf += rec.s_from[ph];
             sum_st += rec.s_to[ph];
             double const mi = rec.i_from[ph] > rec.i_to[ph] ? rec.i_from[ph] : rec.i_to[ph];
             max_i = ph == 0 ? mi : (mi > max_i ? mi : max_i);
         }
-        rec.loading = r.is_line ? max_i / r.rating[i] : (sum_sf > sum_st ? sum_sf : sum_st) / r.rating[i];
+        rec.loading = r.is_line ? max_i / ld(r.rating, i) : (sum_sf > sum_st ? sum_sf : sum_st) / ld(r.rating, i);
     }
 }
 
 template <int P>
 __device__ __forceinline__ void appliance_record(DevOutput const& o, ApplRec const& r, int i, int s, ApplianceOutput<P>& rec) {
     size_t const sp = o.S_pad;
-    rec.id = r.id[i];
-    int const m = r.math[i];
+    rec.id = ld(r.id, i);
+    int const m = ld(r.math, i);
     double const base_power = P == 1 ? 1e6 : 1e6 / 3.0;
-    bool const dead = o.dead && r.bus[i] >= 0 && o.dead[size_t(r.bus[i]) * sp + s];
+    int const bus = ld(r.bus, i);
+    bool const dead = o.dead && bus >= 0 && ld(o.dead, size_t(bus) * sp + s);
     if (m >= 0 && !dead) {
-        rec.energized = r.status ? r.status[size_t(i) * sp + s] : r.status_const[i];
+        rec.energized = r.status ? ld(r.status, size_t(i) * sp + s) : ld(r.status_const, i);
         size_t const base = (size_t(m) * 2) * P;
 #pragma unroll
         for (int ph = 0; ph < P; ++ph) {
@@ -1580,7 +1604,7 @@ __device__ __forceinline__ void appliance_record(DevOutput const& o, ApplRec con
             rec.p[ph] = base_power * sv.re * r.direction;
             rec.q[ph] = base_power * sv.im * r.direction;
             rec.s[ph] = base_power * cabs_(sv);
-            rec.i[ph] = r.base_i[i] * cabs_(iv);
+            rec.i[ph] = ld(r.base_i, i) * cabs_(iv);
             rec.pf[ph] = rec.s[ph] < 1e-8 ? 0.0 : rec.p[ph] / rec.s[ph];
         }
     }
@@ -1623,8 +1647,7 @@ __global__ __launch_bounds__(256) void records_kernel(DevOutput const* __restric
     int const n_itiles = (r.n_comp + TNg - 1) / TNg;
     long long const local = t - tile_off[gr];
     int const s0 = int(local / n_itiles) * W, i0 = int(local % n_itiles) * TNg;
-    for (int q = int(threadIdx.x); q < W * TNg; q += int(blockDim.x)) {
-        int const lane = q % W, ii = q / W;
+    auto fill = [&](int lane, int ii) {
         int const s = s0 + lane, i = i0 + ii;
         if (s < o.S && o.ok[s] && i < r.n_comp) {
             // zero the slot (padding included: the records come out byte for byte the same every run),
@@ -1635,6 +1658,13 @@ __global__ __launch_bounds__(256) void records_kernel(DevOutput const* __restric
             for (int k = 0; k < CH; ++k) z[k] = uint4{0u, 0u, 0u, 0u};
             make_record<P>(o, r, i, s, slot);
         }
+    };
+    if (W == kWaveLanes) {
+        // a wave fills one component for 64 lanes: the component's data are wave-uniform (scalar loads)
+        for (int q = int(threadIdx.x); q < W * TNg; q += int(blockDim.x))
+            fill(q & (kWaveLanes - 1), __builtin_amdgcn_readfirstlane(q / kWaveLanes));
+    } else {
+        for (int q = int(threadIdx.x); q < W * TNg; q += int(blockDim.x)) fill(q % W, q / W);
     }
     __syncthreads();
     int const n_valid = r.n_comp - i0 < TNg ? r.n_comp - i0 : TNg;
