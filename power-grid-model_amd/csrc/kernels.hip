@@ -403,8 +403,14 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
     constexpr bool kFwd = P == 1;
     __shared__ FT fw_blk[kFwd ? kWave : 1][kFwd ? 2 * NB2 : 1];
     __shared__ double fw_y[kFwd ? kWave : 1][kFwd ? NB : 1];
+    // Newton-Raphson: the handing row's own voltage and its parent's, which the parent reads again
+    __shared__ double fw_u[kFwd ? kWave : 1][4];
     int const lane = int(threadIdx.x);
     int fw_le = -1;
+    bool fw_uok = false;  // fw_u holds the last row's voltage and its parent's
+    // L_parent,p of a handing row is needed by nobody but the parent, which takes it from LDS
+    // (NR and the linear system; union groups keep it for the low-rank correction's chains)
+    constexpr bool kSkipL = kFwd && (MODE == MODE_NR || MODE == MODE_INIT) && KIND != KIND_UNION;
 
     for (int ti = ld(g.task_ptr, task); ti < ld(g.task_ptr, task + 1); ++ti) {
         int const p = ld(g.task_pivot, ti);
@@ -694,7 +700,12 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
         } else {
             // Newton-Raphson Jacobian row p, column p and power mismatch
             cplx up[P];
-            load_u<P>(g, p, s, up);
+            bool const from_child = kFwd && fw_uok;  // the previous row is a child handing over
+            if (from_child) {
+                up[0] = {fw_u[lane][2], fw_u[lane][3]};
+            } else {
+                load_u<P>(g, p, s, up);
+            }
             double pcal[P], qcal[P];
 #pragma unroll
             for (int ph = 0; ph < P; ++ph) pcal[ph] = qcal[ph] = 0.0;
@@ -711,8 +722,17 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
                 if (j == p) {
 #pragma unroll
                     for (int ph = 0; ph < P; ++ph) uj[ph] = up[ph];
+                } else if (from_child && j == ld(g.col, fw_le)) {
+                    uj[0] = {fw_u[lane][0], fw_u[lane][1]};
                 } else {
                     load_u<P>(g, j, s, uj);
+                }
+                if constexpr (kFwd) {
+                    if (j > p && keep) {
+                        // the row's only U entry: the parent's voltage, kept for the parent
+                        fw_u[lane][2] = uj[0].re;
+                        fw_u[lane][3] = uj[0].im;
+                    }
                 }
                 double pr[P], qr[P];
                 double yk[PP * 2];
@@ -745,6 +765,12 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
             }
             if constexpr (UNI && P == 1) {
                 if (g.xbus && ld(g.xbus, p)) extra_entries<MODE_NR>(g, p, s, up, pcal[0], qcal[0]);
+            }
+            if constexpr (kFwd) {
+                if (keep) {
+                    fw_u[lane][0] = up[0].re;
+                    fw_u[lane][1] = up[0].im;
+                }
             }
 #pragma unroll
             for (int ph = 0; ph < P; ++ph) {
@@ -873,6 +899,9 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
         }
         lu.store(dg, Df);
         g.perm[size_t(p) * sp + s] = code;
+        // a kept row whose parent is the task's next pivot hands its results over through LDS
+        int const t_end = ld(g.task_ptr, task + 1);
+        bool const hand = kFwd && keep && ti + 1 < t_end && ld(g.task_pivot, ti + 1) == ld(g.col, dg + es);
         // U row: U_pj = L^-1 P A_pj ; L column: L_ip = A_ip Q U^-1
         for (int k = dg + es; k < re; k += es) {
             FT A[NB2];
@@ -898,7 +927,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
             }
             apply_q_cols<NB, FT>(A, code);
             upper_right<NB, FT>(Df, A);
-            lu.store(e, A);
+            if (!(kSkipL && hand)) lu.store(e, A);
             if (kFwd && keep) {
 #pragma unroll
                 for (int q = 0; q < NB2; ++q) fw_blk[lane][q] = A[q];
@@ -911,13 +940,14 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
         for (int r = 0; r < NB; ++r) g.y[(size_t(p) * NB + r) * sp + s] = rhs[r];
         if constexpr (kFwd) {
             fw_le = -1;
-            if (keep && ti + 1 < ld(g.task_ptr, task + 1)) {
+            fw_uok = false;
+            if (hand) {
                 int const k1 = dg + es;  // the row's only U entry
-                if (ld(g.task_pivot, ti + 1) == ld(g.col, k1)) {
-                    fw_le = ld(g.transpose, k1);
+                fw_le = ld(g.transpose, k1);
 #pragma unroll
-                    for (int r = 0; r < NB; ++r) fw_y[lane][r] = rhs[r];
-                }
+                for (int r = 0; r < NB; ++r) fw_y[lane][r] = rhs[r];
+                // voltages of the row and its parent (read by this row's U entry)
+                fw_uok = MODE == MODE_NR && !dead_p && !ld(g.fill, k1);
             }
         }
     }
