@@ -186,12 +186,13 @@ def main():
         del b64
 
     # PCIe-inclusive rate: calculate_power_flow from host buffers (host topology and plans, upload, run,
-    # node / line / load records back to the host) on a leading sample of the scenarios
+    # the output set of the step back to the host) on a leading sample of the scenarios
     e2e = None
     if args.e2e_fraction > 0:
         n_e2e = max(1, int(case.n_scenarios * args.e2e_fraction))
         upd = _leading(case.update, n_e2e)
-        comps = [c for c in ("node", "line", "sym_load", "asym_load") if c in case.input]
+        comps = selection if args.output == "timing" else \
+            [c for c in ("node", "line", "sym_load", "asym_load") if c in case.input]
         t0 = time.perf_counter()
         model.calculate_power_flow(symmetric=case.symmetric, update_data=upd, factor_precision=args.precision,
                                    output_component_types=comps)

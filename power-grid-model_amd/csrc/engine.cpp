@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <numeric>
 #include <cstdlib>
@@ -765,25 +766,32 @@ template <class RowFn> void fill_scenario_data(Batch& b, Group& g, bool has_upda
     // base values in every lane
     for (Idx l = 0; l < m.n_lg(); ++l) {
         put_load(m.base, l, 0);
-        for (int lane = 1; lane < g.S; ++lane) {
-            lstat[size_t(l) * sp + lane] = lstat[size_t(l) * sp];
-            int const k = lg_math[l];
-            if (k < 0) continue;
-            for (int ph = 0; ph < P; ++ph)
-                for (int c = 0; c < 2; ++c)
-                    s_inj[((size_t(k) * P + ph) * 2 + c) * sp + lane] = s_inj[((size_t(k) * P + ph) * 2 + c) * sp];
-        }
+        std::fill_n(lstat.begin() + size_t(l) * sp + 1, g.S - 1, lstat[size_t(l) * sp]);
+        int const k = lg_math[l];
+        if (k < 0) continue;
+        for (int ph = 0; ph < P; ++ph)
+            for (int c = 0; c < 2; ++c) {
+                size_t const row = ((size_t(k) * P + ph) * 2 + c) * sp;
+                std::fill_n(s_inj.begin() + row + 1, g.S - 1, s_inj[row]);
+            }
     }
     for (Idx si = 0; si < Idx(m.source.size()); ++si)
         for (int lane = 0; lane < g.S; ++lane) put_src(g.st, si, lane);
     if (has_update) {
+        // lanes are independent: several host threads, each with its own working state
+        unsigned const hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        unsigned const nth = std::max(1u, std::min<unsigned>(hw, unsigned((g.S + 63) / 64)));
+        std::vector<std::exception_ptr> errs(nth);
+        auto worker = [&](unsigned t) {
+        try {
         ComponentState work = m.base;
         std::vector<std::pair<int, Idx>> touched;
-        for (int lane = 0; lane < g.S; ++lane) {
+        std::vector<char> scratch;
+        for (int lane = int(t); lane < g.S; lane += int(nth)) {
             Idx const s = g.scenarios[lane];
             touched.clear();
             bool structural = false;
-            for_each_row(s, [&](int comp, Idx idx, void const* rec) {
+            for_each_row(s, scratch, [&](int comp, Idx idx, void const* rec) {
                 if (comp == C_LINE || comp == C_TRANSFORMER || comp == C_SHUNT) return;
                 apply_update_record(m, work, comp, idx, rec, structural);
                 touched.emplace_back(comp, idx);
@@ -813,6 +821,16 @@ template <class RowFn> void fill_scenario_data(Batch& b, Group& g, bool has_upda
                 }
             }
         }
+        } catch (...) {
+            errs[t] = std::current_exception();
+        }
+        };
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < nth; ++t) th.emplace_back(worker, t);
+        worker(0);
+        for (auto& x : th) x.join();
+        for (auto const& e : errs)
+            if (e) std::rethrow_exception(e);
     }
     Arena& A = b.arena;
     g.dg.s_inj = put(A, s_inj);
@@ -826,10 +844,25 @@ template <class RowFn> void fill_scenario_data(Batch& b, Group& g, bool has_upda
 
 struct UpdateReader {
     ComponentBuffer const* buf = nullptr;
-    std::vector<char> tmp;
     int comp = 0;
     bool all_na_ids = false;
+    // every scenario lists the same ids in the same order: the component indices of scenario 0's rows
+    // serve every scenario (no id lookups per row)
+    bool same_ids = false;
+    std::vector<Idx> idx0;
 };
+
+// id of row i of an update buffer (row-based or columnar; na when the buffer has no id column)
+ID update_row_id(ComponentBuffer const& buf, Idx i) {
+    ID id = na_IntID;
+    if (buf.data) {
+        std::memcpy(&id, static_cast<char const*>(buf.data) + size_t(i) * buf.meta->size, sizeof(ID));
+    } else {
+        for (auto const& [a, col] : buf.columns)
+            if (a->name == "id") std::memcpy(&id, static_cast<char const*>(col) + size_t(i) * sizeof(ID), sizeof(ID));
+    }
+    return id;
+}
 
 }  // namespace
 
@@ -901,12 +934,9 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             UpdateReader r;
             r.buf = &buf;
             r.comp = comp;
-            r.tmp.resize(buf.meta->size);
             bool all_na = true, any_na = false;
             for (Idx i = 0; i < buf.total_elements; ++i) {
-                buf.read_record(i, r.tmp.data());
-                ID id;
-                std::memcpy(&id, r.tmp.data(), 4);
+                ID const id = update_row_id(buf, i);
                 all_na = all_na && id == na_IntID;
                 any_na = any_na || id == na_IntID;
             }
@@ -917,20 +947,46 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             if (r.all_na_ids && (buf.elements_per_scenario < 0 || buf.elements_per_scenario != m.count(comp)))
                 throw PowerGridError("Update data without IDs for component " + buf.meta->name +
                                      " has a different number of elements per scenario then input data!");
+            Idx const eps = buf.elements_per_scenario;
+            if (!r.all_na_ids && eps > 0 && update->batch_size > 0) {
+                bool ok = true;
+                try {
+                    for (Idx i = 0; i < eps; ++i) r.idx0.push_back(find_component(m, comp, update_row_id(buf, i)));
+                } catch (PowerGridError const&) {
+                    ok = false;  // scenario 0 has an unknown id: rows are looked up one by one
+                }
+                for (Idx sc = 1; ok && sc < update->batch_size; ++sc)
+                    for (Idx i = 0; ok && i < eps; ++i)
+                        ok = update_row_id(buf, sc * eps + i) == update_row_id(buf, i);
+                r.same_ids = ok;
+                if (!ok) r.idx0.clear();
+            }
             readers.push_back(std::move(r));
         }
     }
-    auto for_each_row = [&](Idx s, auto&& fn) {
-        for (auto& r : readers) {
-            for (Idx i = r.buf->begin(s); i < r.buf->end(s); ++i) {
-                r.buf->read_record(i, r.tmp.data());
-                ID id;
-                std::memcpy(&id, r.tmp.data(), 4);
-                Idx const idx = r.all_na_ids ? i - r.buf->begin(s) : find_component(m, r.comp, id);
-                fn(r.comp, idx, r.tmp.data());
+    auto for_each_row = [&](Idx s, std::vector<char>& scratch, auto&& fn) {
+        for (auto const& r : readers) {
+            scratch.resize(r.buf->meta->size);
+            Idx const b0 = r.buf->begin(s);
+            for (Idx i = b0; i < r.buf->end(s); ++i) {
+                r.buf->read_record(i, scratch.data());
+                Idx idx;
+                if (r.all_na_ids)
+                    idx = i - b0;
+                else if (r.same_ids)
+                    idx = r.idx0[size_t(i - b0)];
+                else {
+                    ID id;
+                    std::memcpy(&id, scratch.data(), 4);
+                    idx = find_component(m, r.comp, id);
+                }
+                fn(r.comp, idx, scratch.data());
             }
         }
     };
+    // rows whose ids need no per-scenario check (resolved once, or positional)
+    bool const ids_resolved =
+        std::all_of(readers.begin(), readers.end(), [](UpdateReader const& r) { return r.all_na_ids || r.same_ids; });
     bool const has_structural_update = std::any_of(readers.begin(), readers.end(), [](UpdateReader const& r) {
         return r.comp == C_LINE || r.comp == C_TRANSFORMER || r.comp == C_SHUNT || r.comp == C_SOURCE;
     });
@@ -949,16 +1005,17 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         b.groups.push_back(std::move(g));
         return gi;
     };
+    std::vector<char> scratch;
     for (Idx s = 0; s < S; ++s) {
         try {
             if (!has_structural_update) {
                 // still resolve every id, so that an unknown id fails its scenario
-                for_each_row(s, [](int, Idx, void const*) {});
+                if (!ids_resolved) for_each_row(s, scratch, [](int, Idx, void const*) {});
                 scn_group[s] = group_of(std::string{}, m.base);
             } else {
                 ComponentState st = m.base;
                 bool structural = false;
-                for_each_row(s, [&](int comp, Idx idx, void const* rec) {
+                for_each_row(s, scratch, [&](int comp, Idx idx, void const* rec) {
                     if (comp == C_LINE || comp == C_TRANSFORMER || comp == C_SHUNT || comp == C_SOURCE)
                         apply_update_record(m, st, comp, idx, rec, structural);
                 });
