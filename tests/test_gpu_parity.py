@@ -268,6 +268,94 @@ def test_closing_union_on_radial_base(precision):
         np.testing.assert_allclose(out["line"][f], ref["line"][f], rtol=PQ_RTOL, atol=floor, err_msg=f"line.{f}")
 
 
+def _closing_edge_case():
+    """Hand-made switching on a small radial base: one-sided tie closing, four ties (the most a union
+    lane takes), five ties (its own topology group), a tie inside an island that stays dead, a tie that
+    re-feeds an island, two parallel ties between the same buses, and an opening-only scenario."""
+    from power_grid_model_amd import initialize_array
+
+    case = synthetic.year_20k(n_scenarios=16, n_feeder=4, feeder_nodes=40, n_ties=6, n_load=80,
+                              switching_fraction=0.0, kappa=0.2)
+    inp = {k: v.copy() for k, v in case.input.items()}
+    line = inp["line"]
+    ties = line[line["from_status"] == 0]
+    fl = line[line["from_status"] == 1]
+    children = {}
+    for f, t in zip(fl["from_node"], fl["to_node"]):
+        children.setdefault(int(f), []).append(int(t))
+
+    def subtree(a):
+        out, stack = [], [a]
+        while stack:
+            x = stack.pop()
+            out.append(x)
+            stack.extend(children.get(x, []))
+        return out
+
+    # a feeder node with a subtree of at least four nodes, away from the feeder root
+    a = next(int(t) for f, t in zip(fl["from_node"], fl["to_node"])
+             if f >= 2 and len(subtree(int(t))) >= 4 and int(t) not in fl["to_node"][:4])
+    island = subtree(a)
+    into_a = int(fl["id"][np.flatnonzero(fl["to_node"] == a)[0]])
+    other = next(n for n in range(2, 162) if n not in island and (n - 2) // 40 != (a - 2) // 40)
+    extra = initialize_array("input", "line", 3)
+    extra[:] = line[:3]
+    extra["id"] = 90_000_000 + np.arange(3)
+    extra["from_status"] = 0
+    extra["to_status"] = 0
+    extra["from_node"] = [island[1], island[-1], ties["from_node"][0]]
+    extra["to_node"] = [island[2], other, ties["to_node"][0]]  # inside the island; re-feed; parallel to tie 0
+    inp["line"] = np.concatenate([line, extra])
+    tid = list(ties["id"]) + list(extra["id"])
+    on, off, one = (1, 1), (0, 0), (1, 0)
+    rows = [
+        [],                                                          # base
+        [(tid[0], on)],                                              # one tie
+        [(tid[0], one)],                                             # one side of a tie
+        [(tid[k], on) for k in range(4)],                            # four ties
+        [(tid[k], on) for k in range(5)],                            # five ties: own group
+        [(into_a, off), (tid[6], on)],                               # tie inside a dead island
+        [(into_a, off), (tid[7], on)],                               # island re-fed through a tie
+        [(tid[0], on), (tid[8], on)],                                # parallel ties
+        [(into_a, off)],                                             # opening only
+    ]
+    rows = rows + rows[1:4] + rows[5:8]  # repeat some so that the union has members of several kinds
+    k = max(len(r) for r in rows)
+    upd_line = initialize_array("update", "line", (len(rows), k))
+    for s_, r in enumerate(rows):
+        for j in range(k):
+            if j < len(r):
+                upd_line["id"][s_, j] = r[j][0]
+                upd_line["from_status"][s_, j], upd_line["to_status"][s_, j] = r[j][1]
+            else:
+                upd_line["id"][s_, j] = tid[0] if not any(x[0] == tid[0] for x in r) else tid[1]
+    upd = {"sym_load": case.update["sym_load"][: len(rows)], "line": upd_line}
+    return inp, upd
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_closing_union_edge_cases(precision):
+    inp, upd = _closing_edge_case()
+    model = PowerGridModel(inp)
+    batch = DeviceBatch(model, upd, factor_precision=precision)
+    batch.run()
+    out = batch.fetch()
+    assert batch.stats()["groups"] == 2  # one union (the base scenario included), the five-tie scenario
+    ref, it_ref, st_ref, _ = ol.power_flow(inp, upd, len(upd["line"]), threading=0)
+    assert (st_ref == 0).all()
+    assert (ref["node"]["energized"][5] == 0).any()  # the island of scenario 5 stays dead
+    assert (ref["node"]["energized"][6] == 1).all()  # scenario 6 re-feeds it
+    if precision == "fp64":
+        assert (batch.iterations() == it_ref).all(), (batch.iterations(), it_ref)
+    else:
+        assert (batch.iterations() <= it_ref + 2).all()
+    _compare(out, ref, True, slice(None))
+    assert (out["line"]["energized"] == ref["line"]["energized"]).all()
+    for f in ("i_from", "i_to", "s_from", "s_to", "loading"):
+        floor = 1e-9 * max(np.max(np.abs(ref["line"][f])), 1.0)
+        np.testing.assert_allclose(out["line"][f], ref["line"][f], rtol=PQ_RTOL, atol=floor, err_msg=f"line.{f}")
+
+
 def _tie_case():
     """meshed_1000 plus 8 normally open ties; switching pattern k closes tie k and opens one line, and is
     repeated by 1, 2, 3, 5, 7, 11, 1 and 33 scenarios: packed groups of one to eleven scenarios and one
