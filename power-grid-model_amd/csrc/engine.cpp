@@ -728,19 +728,24 @@ void upload_union_lanes(Batch& b, Group& g) {
 }
 
 // per-scenario injections, source voltages and admittances of one group
-template <class RowFn> void fill_scenario_data(Batch& b, Group& g, bool has_update, RowFn&& for_each_row) {
+// device_loads: the load_gen updates (every source in device_srcs) are applied on the device; the host
+// then only handles the other update rows (sources)
+template <class RowFn>
+void fill_scenario_data(Batch& b, Group& g, bool has_update, RowFn&& for_each_row, bool device_loads,
+                        std::vector<LoadUpdateSource> const& device_srcs, LoadTable table) {
     Model const& m = b.model;
     int const P = b.P;
     bool const sym = b.symmetric;
     Plan const& p = g.plan;
     size_t const sp = size_t(g.S_pad);
-    std::vector<double> s_inj(size_t(g.n_load) * P * 2 * sp, 0.0);
-    std::vector<int8_t> lstat(size_t(m.n_lg()) * sp, 0);
+    std::vector<double> s_inj(device_loads ? 0 : size_t(g.n_load) * P * 2 * sp, 0.0);
+    std::vector<int8_t> lstat(device_loads ? 0 : size_t(m.n_lg()) * sp, 0);
     std::vector<double> u_src(size_t(g.n_src) * 2 * sp, 0.0), y_src(size_t(g.n_src) * 4 * sp, 0.0);
     std::vector<int> lg_math(m.n_lg(), -1), src_math(m.source.size(), -1);
     for (size_t k = 0; k < p.load_comp.size(); ++k) lg_math[p.load_comp[k]] = int(k);
     for (size_t k = 0; k < p.src_comp.size(); ++k) src_math[p.src_comp[k]] = int(k);
     auto put_load = [&](ComponentState const& st, Idx l, int lane) {
+        if (device_loads) return;
         lstat[size_t(l) * sp + lane] = st.lg_status[l] ? 1 : 0;
         int const k = lg_math[l];
         if (k < 0) return;
@@ -764,7 +769,7 @@ template <class RowFn> void fill_scenario_data(Batch& b, Group& g, bool has_upda
         y_src[(size_t(k) * 4 + 3) * sp + lane] = y0.imag();
     };
     // base values in every lane
-    for (Idx l = 0; l < m.n_lg(); ++l) {
+    for (Idx l = 0; l < (device_loads ? Idx(0) : m.n_lg()); ++l) {
         put_load(m.base, l, 0);
         std::fill_n(lstat.begin() + size_t(l) * sp + 1, g.S - 1, lstat[size_t(l) * sp]);
         int const k = lg_math[l];
@@ -833,13 +838,40 @@ template <class RowFn> void fill_scenario_data(Batch& b, Group& g, bool has_upda
             if (e) std::rethrow_exception(e);
     }
     Arena& A = b.arena;
-    g.dg.s_inj = put(A, s_inj);
+    if (device_loads) {
+        // base injections of the group's math loads and base statuses, broadcast to every lane, then
+        // each update source scattered into its lanes
+        std::vector<double> base_s(size_t(g.n_load) * P * 2, 0.0);
+        std::vector<int8_t> base_st(size_t(m.n_lg()), 0);
+        for (Idx l = 0; l < m.n_lg(); ++l) {
+            base_st[l] = m.base.lg_status[l] ? 1 : 0;
+            int const k = lg_math[l];
+            if (k < 0) continue;
+            auto const s3 = load_power(m, m.base, l, sym);
+            for (int ph = 0; ph < P; ++ph) {
+                base_s[(size_t(k) * P + ph) * 2] = s3[ph].real();
+                base_s[(size_t(k) * P + ph) * 2 + 1] = s3[ph].imag();
+            }
+        }
+        double* d_sinj = dev_alloc<double>(A, size_t(g.n_load) * P * 2 * sp);
+        double const* d_base_s = put(A, base_s);
+        int8_t const* d_base_st = put(A, base_st);
+        launch_load_broadcast(d_sinj, g.load_status, d_base_s, d_base_st, g.n_load, int(m.n_lg()), P, g.S, g.S_pad,
+                              b.stream);
+        table.math = put(A, lg_math);
+        for (auto const& src : device_srcs)
+            launch_load_scatter(d_sinj, g.load_status, g.S, g.S_pad, g.dout.scenario_index, src, table, b.stream);
+        PGM_HIP_CHECK(hipGetLastError());
+        g.dg.s_inj = d_sinj;
+    } else {
+        g.dg.s_inj = put(A, s_inj);
+        PGM_HIP_CHECK(hipMemcpy(g.load_status, lstat.data(), lstat.size(), hipMemcpyHostToDevice));
+    }
     g.dg.u_src = put(A, u_src);
     g.dg.y_src = put(A, y_src);
     g.dout.s_inj = g.dg.s_inj;
     g.dout.u_src = g.dg.u_src;
     g.dout.y_src = g.dg.y_src;
-    PGM_HIP_CHECK(hipMemcpy(g.load_status, lstat.data(), lstat.size(), hipMemcpyHostToDevice));
 }
 
 struct UpdateReader {
@@ -850,6 +882,9 @@ struct UpdateReader {
     // serve every scenario (no id lookups per row)
     bool same_ids = false;
     std::vector<Idx> idx0;
+    // load_gen updates applied on the device (every scenario lists the same components once each)
+    bool device = false;
+    LoadUpdateSource dsrc;
 };
 
 // id of row i of an update buffer (row-based or columnar; na when the buffer has no id column)
@@ -964,8 +999,11 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             readers.push_back(std::move(r));
         }
     }
-    auto for_each_row = [&](Idx s, std::vector<char>& scratch, auto&& fn) {
+    // host_only: leave out readers applied on the device and readers whose rows need no per-scenario
+    // id check and carry nothing structural
+    auto rows_of = [&](Idx s, std::vector<char>& scratch, bool host_only, auto&& fn) {
         for (auto const& r : readers) {
+            if (host_only && r.device) continue;
             scratch.resize(r.buf->meta->size);
             Idx const b0 = r.buf->begin(s);
             for (Idx i = b0; i < r.buf->end(s); ++i) {
@@ -984,6 +1022,96 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             }
         }
     };
+    auto for_each_row = [&](Idx s, std::vector<char>& scratch, auto&& fn) { rows_of(s, scratch, false, fn); };
+    auto for_each_host_row = [&](Idx s, std::vector<char>& scratch, auto&& fn) { rows_of(s, scratch, true, fn); };
+    // load_gen updates on the device: every load_gen reader lists the same components in every scenario,
+    // each once
+    bool device_loads = false;
+    std::vector<LoadUpdateSource> device_srcs;
+    LoadTable table;
+    if (update && update->batch_size > 0) {
+        bool all_ok = false, any = false;
+        for (auto& r : readers) {
+            bool const lg = r.comp == C_SYM_GEN || r.comp == C_ASYM_GEN || r.comp == C_SYM_LOAD || r.comp == C_ASYM_LOAD;
+            if (!lg) continue;
+            Idx const eps = r.buf->elements_per_scenario;
+            bool ok = eps > 0 && (r.same_ids || r.all_na_ids);
+            if (ok) {
+                std::vector<Idx> idx(static_cast<size_t>(eps));
+                for (Idx e = 0; e < eps; ++e) idx[e] = r.all_na_ids ? e : r.idx0[size_t(e)];
+                std::vector<Idx> sorted = idx;
+                std::sort(sorted.begin(), sorted.end());
+                ok = std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end();
+            }
+            all_ok = any ? (all_ok && ok) : ok;
+            any = true;
+            r.device = ok;
+        }
+        device_loads = any && all_ok;
+        if (!device_loads)
+            for (auto& r : readers) r.device = false;
+    }
+    if (device_loads) {
+        Idx const nlg = m.n_lg();
+        std::vector<int8_t> st(nlg), gen(nlg), asym(nlg);
+        std::vector<double> pp(size_t(nlg) * 3), qq(size_t(nlg) * 3);
+        for (Idx l = 0; l < nlg; ++l) {
+            st[l] = m.base.lg_status[l] ? 1 : 0;
+            gen[l] = m.lg_gen[l] ? 1 : 0;
+            asym[l] = m.lg_asym[l] ? 1 : 0;
+            for (int ph = 0; ph < 3; ++ph) {
+                pp[size_t(l) * 3 + ph] = m.base.lg_p[l][ph];
+                qq[size_t(l) * 3 + ph] = m.base.lg_q[l][ph];
+            }
+        }
+        table.n_lg = int(nlg);
+        table.P = b.P;
+        table.symmetric = sym ? 1 : 0;
+        table.status = put(b.arena, st);
+        table.p = put(b.arena, pp);
+        table.q = put(b.arena, qq);
+        table.gen = put(b.arena, gen);
+        table.asym = put(b.arena, asym);
+        for (auto& r : readers) {
+            if (!r.device) continue;
+            ComponentBuffer const& buf = *r.buf;
+            Idx const eps = buf.elements_per_scenario;
+            int const type = r.comp == C_SYM_GEN ? 0 : r.comp == C_ASYM_GEN ? 1 : r.comp == C_SYM_LOAD ? 2 : 3;
+            std::vector<int> comp(static_cast<size_t>(eps));
+            for (Idx e = 0; e < eps; ++e) comp[e] = int(m.lg_offset[type] + (r.all_na_ids ? e : r.idx0[size_t(e)]));
+            LoadUpdateSource& d = r.dsrc;
+            d.n_elem = int(eps);
+            d.comp = put(b.arena, comp);
+            d.asym = (type == 1 || type == 3) ? 1 : 0;
+            // row-based: the whole row buffer is uploaded once and the fields address into it
+            char* drows = nullptr;
+            if (buf.data) {
+                drows = static_cast<char*>(b.arena.take(size_t(buf.total_elements) * buf.meta->size));
+                PGM_HIP_CHECK(hipMemcpy(drows, buf.data, size_t(buf.total_elements) * buf.meta->size,
+                                        hipMemcpyHostToDevice));
+            }
+            auto field = [&](char const* name, char const*& ptr, long long& stride) {
+                PGM_MetaAttribute const* a = buf.meta->find(name);
+                if (!a) return;
+                if (drows) {
+                    ptr = drows + a->offset;
+                    stride = (long long)(buf.meta->size);
+                    return;
+                }
+                for (auto const& [ca, col] : buf.columns)
+                    if (ca == a) {
+                        char* dcol = static_cast<char*>(b.arena.take(size_t(buf.total_elements) * a->size));
+                        PGM_HIP_CHECK(hipMemcpy(dcol, col, size_t(buf.total_elements) * a->size, hipMemcpyHostToDevice));
+                        ptr = dcol;
+                        stride = (long long)(a->size);
+                    }
+            };
+            field("status", d.status, d.status_stride);
+            field("p_specified", d.p, d.p_stride);
+            field("q_specified", d.q, d.q_stride);
+            device_srcs.push_back(d);
+        }
+    }
     // rows whose ids need no per-scenario check (resolved once, or positional)
     bool const ids_resolved =
         std::all_of(readers.begin(), readers.end(), [](UpdateReader const& r) { return r.all_na_ids || r.same_ids; });
@@ -1015,7 +1143,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             } else {
                 ComponentState st = m.base;
                 bool structural = false;
-                for_each_row(s, scratch, [&](int comp, Idx idx, void const* rec) {
+                for_each_host_row(s, scratch, [&](int comp, Idx idx, void const* rec) {
                     if (comp == C_LINE || comp == C_TRANSFORMER || comp == C_SHUNT || comp == C_SOURCE)
                         apply_update_record(m, st, comp, idx, rec, structural);
                 });
@@ -1213,7 +1341,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     for (auto& gp : b.groups) {
         upload_group(b, *gp);
         if (gp->is_union) upload_union_lanes(b, *gp);
-        fill_scenario_data(b, *gp, update != nullptr, for_each_row);
+        fill_scenario_data(b, *gp, update != nullptr, for_each_host_row, device_loads, device_srcs, table);
     }
 
     // ---- device tables and work lists ----

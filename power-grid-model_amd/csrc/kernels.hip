@@ -1267,6 +1267,89 @@ __global__ __launch_bounds__(kWave) void lowrank_kernel(DevGroup const* __restri
     }
 }
 
+// ---------------------------------------------------------------------------------------------------
+// load_gen updates on the device (update.hpp semantics: a null status or power keeps the base value;
+// s = direction (p + j q) / base_power, load_gen.hpp:121-139; asymmetric components in a symmetric
+// calculation take the mean of the phases)
+// ---------------------------------------------------------------------------------------------------
+__global__ void load_broadcast_kernel(double* __restrict__ s_inj, int8_t* __restrict__ lstat,
+                                      double const* __restrict__ base_s, int8_t const* __restrict__ base_status,
+                                      int rows_s, int n_lg, int S, int S_pad) {
+    long long const i = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+    long long const n_s = (long long)(rows_s) * S_pad;
+    if (i < n_s) {
+        int const row = int(i / S_pad), lane = int(i % S_pad);
+        s_inj[i] = lane < S ? base_s[row] : 0.0;
+    } else if (i < n_s + (long long)(n_lg) * S_pad) {
+        long long const j = i - n_s;
+        int const l = int(j / S_pad), lane = int(j % S_pad);
+        lstat[j] = lane < S ? base_status[l] : int8_t(0);
+    }
+}
+
+__global__ void load_scatter_kernel(double* __restrict__ s_inj, int8_t* __restrict__ lstat, int S, int S_pad,
+                                    int const* __restrict__ scenario_index, LoadUpdateSource const src,
+                                    LoadTable const tab) {
+    long long const i = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= (long long)(S) * src.n_elem) return;
+    int const e = int(i / S), lane = int(i % S);
+    long long const row = (long long)(ld(scenario_index, lane)) * src.n_elem + e;
+    int const l = ld(src.comp, e);
+    bool st = ld(tab.status, l) != 0;
+    double p[3], q[3];
+#pragma unroll
+    for (int ph = 0; ph < 3; ++ph) {
+        p[ph] = ld(tab.p, 3 * l + ph);
+        q[ph] = ld(tab.q, 3 * l + ph);
+    }
+    if (src.status) {
+        int8_t const v = *reinterpret_cast<int8_t const*>(src.status + row * src.status_stride);
+        if (v != int8_t(-128)) st = v != 0;
+    }
+    int const nph = src.asym ? 3 : 1;
+    for (int ph = 0; ph < nph; ++ph) {
+        if (src.p) {
+            double const v = reinterpret_cast<double const*>(src.p + row * src.p_stride)[ph];
+            if (!isnan(v)) p[ph] = v;
+        }
+        if (src.q) {
+            double const v = reinterpret_cast<double const*>(src.q + row * src.q_stride)[ph];
+            if (!isnan(v)) q[ph] = v;
+        }
+    }
+    size_t const sp = S_pad;
+    lstat[size_t(l) * sp + lane] = st ? 1 : 0;
+    int const k = ld(tab.math, l);
+    if (k < 0) return;
+    double re[3] = {0.0, 0.0, 0.0}, im[3] = {0.0, 0.0, 0.0};
+    if (st) {
+        double const dir = ld(tab.gen, l) ? 1.0 : -1.0;
+        if (!ld(tab.asym, l)) {
+            double const r0 = dir * p[0] / 1e6, i0 = dir * q[0] / 1e6;
+            bool const bad = isnan(r0) || isnan(i0);
+#pragma unroll
+            for (int ph = 0; ph < 3; ++ph) {
+                re[ph] = bad ? __builtin_nan("") : r0;
+                im[ph] = bad ? __builtin_nan("") : i0;
+            }
+        } else {
+#pragma unroll
+            for (int ph = 0; ph < 3; ++ph) {
+                re[ph] = dir * p[ph] / (1e6 / 3.0);
+                im[ph] = dir * q[ph] / (1e6 / 3.0);
+            }
+            if (tab.symmetric) {
+                re[0] = (re[0] + re[1] + re[2]) / 3.0;
+                im[0] = (im[0] + im[1] + im[2]) / 3.0;
+            }
+        }
+    }
+    for (int ph = 0; ph < tab.P; ++ph) {
+        s_inj[((size_t(k) * tab.P + ph) * 2) * sp + lane] = re[ph];
+        s_inj[((size_t(k) * tab.P + ph) * 2 + 1) * sp + lane] = im[ph];
+    }
+}
+
 __global__ void init_state_kernel(PairState ps) {
     size_t const i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= size_t(ps.n)) return;
@@ -1878,6 +1961,21 @@ void launch_bus_results(int P, DevOutput const* outs, long long const* offsets, 
         hipLaunchKernelGGL(bus_results_kernel<1>, grid, dim3(256), 0, st, outs, offsets, n_groups);
     else
         hipLaunchKernelGGL(bus_results_kernel<3>, grid, dim3(256), 0, st, outs, offsets, n_groups);
+}
+void launch_load_broadcast(double* s_inj, int8_t* lstat, double const* base_s, int8_t const* base_status, int n_load,
+                           int n_lg, int P, int S, int S_pad, hipStream_t st) {
+    int const rows_s = n_load * P * 2;
+    long long const total = ((long long)(rows_s) + n_lg) * S_pad;
+    if (total == 0) return;
+    hipLaunchKernelGGL(load_broadcast_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, s_inj, lstat,
+                       base_s, base_status, rows_s, n_lg, S, S_pad);
+}
+void launch_load_scatter(double* s_inj, int8_t* lstat, int S, int S_pad, int const* scenario_index,
+                         LoadUpdateSource const& src, LoadTable const& tab, hipStream_t st) {
+    long long const total = (long long)(S) * src.n_elem;
+    if (total == 0) return;
+    hipLaunchKernelGGL(load_scatter_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, s_inj, lstat, S,
+                       S_pad, scenario_index, src, tab);
 }
 void launch_lowrank(DevGroup const* groups, ExtraWork const* work, int n_work, bool fp32, hipStream_t st) {
     if (n_work == 0) return;

@@ -236,6 +236,39 @@ constexpr int record_tile(size_t record_bytes) {
 // W = min(S_pad, 64) lanes per tile)
 void launch_bus_results(int P, DevOutput const* outs, long long const* offsets, int n_groups, long long total,
                         hipStream_t st);
+// load_gen updates applied on the device: one update buffer whose scenarios all list the same
+// components in the same order. Fields are addressed as base + row * stride (row = scenario * n_elem +
+// element); a null field is absent (the base value stays). asym: the record holds three phases.
+struct LoadUpdateSource {
+    int n_elem = 0;
+    int const* comp = nullptr;  // [n_elem] load_gen index (sym_gen, asym_gen, sym_load, asym_load order)
+    int asym = 0;
+    char const* status = nullptr;
+    long long status_stride = 0;
+    char const* p = nullptr;
+    long long p_stride = 0;
+    char const* q = nullptr;
+    long long q_stride = 0;
+};
+// base state of every load_gen and its place in a group
+struct LoadTable {
+    int n_lg = 0;
+    int P = 1;
+    int symmetric = 1;
+    int8_t const* status = nullptr;  // [n_lg]
+    double const* p = nullptr;       // [n_lg][3]
+    double const* q = nullptr;       // [n_lg][3]
+    int8_t const* gen = nullptr;     // [n_lg] 1: generator (direction +1)
+    int8_t const* asym = nullptr;    // [n_lg] 1: asymmetric component
+    int const* math = nullptr;       // [n_lg] math load index in the group, -1 when not in it
+};
+// every lane of a group gets the base injections (lanes >= S: zero) and statuses
+void launch_load_broadcast(double* s_inj, int8_t* lstat, double const* base_s, int8_t const* base_status, int n_load,
+                           int n_lg, int P, int S, int S_pad, hipStream_t st);
+// apply one update source to the lanes of a group (lane -> scenario through scenario_index)
+void launch_load_scatter(double* s_inj, int8_t* lstat, int S, int S_pad, int const* scenario_index,
+                         LoadUpdateSource const& src, LoadTable const& tab, hipStream_t st);
+
 // attribute columns a record writer produces instead of whole records (columnar output): byte offset
 // and size of the attribute in the record, and the column [n_scenarios][n_comp] (n = 0: whole records)
 constexpr int kMaxColumns = 16;
