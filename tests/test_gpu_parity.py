@@ -545,6 +545,36 @@ def test_device_output_selection():
     assert set(only) == {"node"} and only["node"].tobytes() == rows["node"].tobytes()
 
 
+@pytest.mark.parametrize("columnar", [False, True])
+def test_load_update_semantics(columnar):
+    """Load updates applied on the device follow the reference's update semantics: a null status, p or
+    q keeps the input value, status 0 switches the load off; row-based and columnar update buffers,
+    symmetric and asymmetric loads (asymmetric ones also in a symmetric calculation)."""
+    for case, sym in ((synthetic.radial_200(n_scenarios=40), True), (synthetic.asym_500(n_scenarios=40), False),
+                      (synthetic.asym_500(n_scenarios=40), True)):
+        comp = "sym_load" if "sym_load" in case.update else "asym_load"
+        upd = case.update[comp].copy()
+        rng = np.random.default_rng(3)
+        mask = rng.random(upd.shape)
+        upd["status"][mask < 0.1] = np.iinfo(np.int8).min
+        upd["status"][(mask >= 0.1) & (mask < 0.2)] = 0
+        upd["p_specified"][(mask >= 0.2) & (mask < 0.3)] = np.nan
+        upd["q_specified"][(mask >= 0.3) & (mask < 0.4)] = np.nan
+        if columnar:
+            data = {comp: {a: upd[a] for a in ("id", "status", "p_specified", "q_specified")}}
+        else:
+            data = {comp: upd}
+        model = PowerGridModel(case.input)
+        batch = DeviceBatch(model, data, symmetric=sym)
+        batch.run()
+        out = batch.fetch()
+        ref, it_ref, st_ref, _ = ol.power_flow(case.input, {comp: upd}, upd.shape[0], symmetric=sym, threading=0)
+        assert (st_ref == 0).all()
+        assert (batch.iterations() == it_ref).all()
+        _compare(out, ref, sym, slice(None))
+        assert (out[comp]["energized"] == ref[comp]["energized"]).all()
+
+
 def test_error_cases_match_oracle():
     case = synthetic.radial_200(n_scenarios=8)
     upd = {k: v.copy() for k, v in case.update.items()}
