@@ -247,6 +247,16 @@ PGM_AMD_API void PGM_amd_batch_run(PGM_Handle* handle, PGM_AmdBatch* batch) PGM_
  * same components and a subset of the selected columns. */
 PGM_AMD_API void PGM_amd_batch_set_output(PGM_Handle* handle, PGM_AmdBatch* batch,
                                           PGM_MutableDataset const* output) PGM_AMD_NOEXCEPT;
+/* Adjoint gradients of the last run (no counterpart in the reference, which has no derivatives): for
+ * every scenario, L = sum over energised nodes (u_pu - 1)^2 + sum over lines (p_from + p_to) in MW, and
+ * its gradients by every load_gen's p_specified and q_specified (per phase for asymmetric ones) and by
+ * every line's r1 and x1. The Newton-Raphson Jacobian is factorised again at the converged state and
+ * J^T lambda = dL/dx is solved with those factors. Arrays are [n_scenarios][n_load_gen] (load_gens in
+ * the order sym_gen, asym_gen, sym_load, asym_load, input order within each) and
+ * [n_scenarios][n_line]. Symmetric Newton-Raphson batches whose topology groups each hold more than 32
+ * scenarios; failed scenarios get zeros. */
+PGM_AMD_API void PGM_amd_batch_adjoint(PGM_Handle* handle, PGM_AmdBatch* batch, double* grad_p, double* grad_q,
+                                       double* grad_r1, double* grad_x1) PGM_AMD_NOEXCEPT;
 /* Copy results into the caller's output dataset and report per-scenario errors like PGM_calculate. */
 PGM_AMD_API void PGM_amd_batch_fetch(PGM_Handle* handle, PGM_AmdBatch* batch,
                                      PGM_MutableDataset const* output_dataset) PGM_AMD_NOEXCEPT;

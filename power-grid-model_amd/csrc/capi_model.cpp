@@ -118,6 +118,10 @@ PGM_AmdBatch* PGM_amd_batch_create(PGM_Handle* handle, PGM_PowerGridModel const*
 void PGM_amd_batch_run(PGM_Handle* handle, PGM_AmdBatch* batch) noexcept {
     guarded(handle, [&] { batch->b->run(); });
 }
+void PGM_amd_batch_adjoint(PGM_Handle* handle, PGM_AmdBatch* batch, double* grad_p, double* grad_q, double* grad_r1,
+                           double* grad_x1) noexcept {
+    guarded(handle, [&] { batch->b->adjoint(grad_p, grad_q, grad_r1, grad_x1); });
+}
 void PGM_amd_batch_set_output(PGM_Handle* handle, PGM_AmdBatch* batch, PGM_MutableDataset const* output) noexcept {
     guarded(handle, [&] { batch->b->set_output(*output); });
 }

@@ -1660,6 +1660,146 @@ void Batch::run() {
     ran = true;
 }
 
+void Batch::adjoint(double* grad_p, double* grad_q, double* grad_r, double* grad_x) {
+    if (!ran) throw PowerGridError("the batch has not been run\n");
+    if (!symmetric || linear || current)
+        throw PowerGridError("the adjoint is implemented for symmetric Newton-Raphson batches\n");
+    for (auto const& gp : groups)
+        if (gp->is_union || gp->packed)
+            throw PowerGridError("the adjoint needs every topology group to hold more than 32 scenarios\n");
+    Model const& m = model;
+    Idx const nlg = m.n_lg(), nl = m.n_line();
+    size_t const n_grad = size_t(n_scenarios) * size_t(2 * nlg + 2 * nl);
+    if (!d_adj) {
+        // per group: line incidences, node counts, branch admittances and their derivatives by r1, x1,
+        // load and line maps (host, once)
+        std::vector<DevAdjoint> adj;
+        std::vector<long long> off{0};
+        d_grad = dev_alloc<double>(arena, std::max<size_t>(n_grad, 1));
+        for (auto const& gp : groups) {
+            Group const& g = *gp;
+            Plan const& p = g.plan;
+            Topology const& t = g.topo;
+            Idx const n = p.n, nbr = Idx(p.branch_comp.size());
+            DevAdjoint a;
+            a.n_lg = int(nlg);
+            a.n_line = int(nl);
+            std::vector<int> ptr(n + 1, 0), inc;
+            std::vector<std::vector<int>> lists(n);
+            for (Idx mb = 0; mb < nbr; ++mb) {
+                if (p.branch_comp[mb] >= nl) continue;
+                int const f = p.branch_from[mb], to = p.branch_to[mb];
+                if (f >= 0) lists[f].push_back(int(mb));
+                if (to >= 0 && to != f) lists[to].push_back(int(mb));
+            }
+            for (Idx r = 0; r < n; ++r) {
+                ptr[r + 1] = ptr[r] + int(lists[r].size());
+                inc.insert(inc.end(), lists[r].begin(), lists[r].end());
+            }
+            std::vector<int> node_w(n, 0);
+            for (Idx i = 0; i < m.n_node(); ++i) {
+                Idx const gg = t.node_group[i];
+                if (gg >= 0) ++node_w[p.bus_label[p.mm_bus_begin[gg] + int(t.node_pos[i])]];
+            }
+            std::vector<double> dr(size_t(nbr) * 8, 0.0), dx(size_t(nbr) * 8, 0.0);
+            for (Idx mb = 0; mb < nbr; ++mb) {
+                Idx const c = p.branch_comp[mb];
+                if (c >= nl) continue;
+                LineStatic const& ls = m.lstat[c];
+                cd const ys = ls.y1s, ysh = ls.y1sh;
+                double const u1 = m.node[m.line_f[c]].u_rated;
+                double const base_y = (kBaseP3 / u1 / kSqrt3) / (u1 / kSqrt3);
+                // y_series = 1 / ((r1 + j x1) base_y): d/dr1 = -ys^2 base_y, d/dx1 = -j ys^2 base_y
+                cd const d_r = -ys * ys * base_y, d_x = cd(0.0, -1.0) * ys * ys * base_y;
+                bool const fs = g.st.line_fs[c], ts = g.st.line_ts[c];
+                std::array<cd, 4> dy{};  // d(ff, ft, tf, tt) / d y_series (line pi model, tap 1)
+                if (fs && ts) {
+                    dy = {cd(1.0), cd(-1.0), cd(-1.0), cd(1.0)};
+                } else if ((fs || ts) && std::abs(ysh) >= kNumTol) {
+                    cd const den = 1.0 / ys + 2.0 / ysh;
+                    cd const dsh = (1.0 / (ys * ys)) / (den * den);
+                    dy[fs ? 0 : 3] = dsh;
+                }
+                for (int w = 0; w < 4; ++w) {
+                    cd const a_r = dy[w] * d_r, a_x = dy[w] * d_x;
+                    dr[size_t(mb) * 8 + 2 * w] = a_r.real();
+                    dr[size_t(mb) * 8 + 2 * w + 1] = a_r.imag();
+                    dx[size_t(mb) * 8 + 2 * w] = a_x.real();
+                    dx[size_t(mb) * 8 + 2 * w + 1] = a_x.imag();
+                }
+            }
+            std::vector<int> lg_math(nlg, -1), load_bus(p.load_comp.size(), -1), line_math(nl, -1);
+            for (size_t k = 0; k < p.load_comp.size(); ++k) lg_math[p.load_comp[k]] = int(k);
+            for (Idx bus = 0; bus < n; ++bus)
+                for (int l = p.load_ptr[bus]; l < p.load_ptr[bus + 1]; ++l) load_bus[l] = int(bus);
+            std::vector<int> mm_branch_begin(t.models.size() + 1, 0);
+            for (size_t k = 0; k < t.models.size(); ++k)
+                mm_branch_begin[k + 1] = mm_branch_begin[k] + int(t.models[k].branch.size());
+            for (Idx c = 0; c < nl; ++c)
+                if (t.branch_group[c] >= 0) line_math[c] = mm_branch_begin[t.branch_group[c]] + int(t.branch_pos[c]);
+            std::vector<int8_t> gen(nlg);
+            for (Idx l = 0; l < nlg; ++l) gen[l] = m.lg_gen[l] ? 1 : 0;
+            a.bus_line_ptr = put(arena, ptr);
+            a.bus_line = put(arena, inc);
+            a.node_w = put(arena, node_w);
+            a.br_from = put(arena, p.branch_from);
+            a.br_to = put(arena, p.branch_to);
+            a.br_y = put(arena, complex_to_doubles(p.branch_y));
+            a.br_dr = put(arena, dr);
+            a.br_dx = put(arena, dx);
+            a.lg_math = put(arena, lg_math);
+            a.load_bus = put(arena, load_bus);
+            a.line_math = put(arena, line_math);
+            a.lg_gen = put(arena, gen);
+            a.load_status = g.load_status;
+            a.grad_p = d_grad;
+            a.grad_q = d_grad + size_t(n_scenarios) * nlg;
+            a.grad_r = d_grad + size_t(n_scenarios) * 2 * nlg;
+            a.grad_x = d_grad + size_t(n_scenarios) * (2 * nlg + nl);
+            adj.push_back(a);
+            off.push_back(off.back() + (long long)(nlg + nl) * g.S_pad);
+        }
+        d_adj = put(arena, adj);
+        d_adj_off = put(arena, off);
+        adj_total = off.back();
+        // the adjoint always factorises in FP64: a mixed-precision batch gets FP64 factor storage of its
+        // own for it (the gradients then carry no FP32 solve error)
+        if (fp32) {
+            std::vector<DevGroup> dg;
+            for (auto const& gp : groups) {
+                DevGroup d = gp->dg;
+                d.lu = arena.take(size_t(d.E) * 4 * size_t(d.S_pad) * sizeof(double));
+                d.fp32 = 0;
+                dg.push_back(d);
+            }
+            d_groups_adj = put(arena, dg);
+        } else {
+            d_groups_adj = d_groups;
+        }
+    }
+    hipStream_t const st = stream;
+    if (n_grad > 0) PGM_HIP_CHECK(hipMemsetAsync(d_grad, 0, n_grad * sizeof(double), st));
+    int const L = int(n_items.size());
+    DevGroup const* const dga = d_groups_adj;
+    // the Jacobian at the converged state, every block kept
+    for (int l = 0; l < L; ++l) launch_forward(dga, d_items[l], n_items[l], P, false, MODE_NRA, st);
+    launch_adjoint_rhs(dga, d_adj, d_bus_off, int(groups.size()), bus_total, st);
+    for (int l = 0; l < L; ++l) launch_adjoint_sweep(dga, d_items[l], n_items[l], false, true, st);
+    for (int l = L - 1; l >= 0; --l) launch_adjoint_sweep(dga, d_items[l], n_items[l], false, false, st);
+    launch_adjoint_grad(dga, d_outputs, d_adj, d_adj_off, int(groups.size()), adj_total, st);
+    PGM_HIP_CHECK(hipGetLastError());
+    PGM_HIP_CHECK(hipStreamSynchronize(st));
+    size_t const a = size_t(n_scenarios) * nlg, b = size_t(n_scenarios) * nl;
+    if (a > 0) {
+        PGM_HIP_CHECK(hipMemcpy(grad_p, d_grad, a * 8, hipMemcpyDeviceToHost));
+        PGM_HIP_CHECK(hipMemcpy(grad_q, d_grad + a, a * 8, hipMemcpyDeviceToHost));
+    }
+    if (b > 0) {
+        PGM_HIP_CHECK(hipMemcpy(grad_r, d_grad + 2 * a, b * 8, hipMemcpyDeviceToHost));
+        PGM_HIP_CHECK(hipMemcpy(grad_x, d_grad + 2 * a + b, b * 8, hipMemcpyDeviceToHost));
+    }
+}
+
 namespace {
 void check_output_dataset(PGM_MutableDataset const& output, bool symmetric, Idx n_scenarios) {
     bool const want_asym = output.meta->name == "asym_output";

@@ -114,6 +114,15 @@ struct Batch {
     ~Batch();
     void release();
     void run();
+    // adjoint of the last run (symmetric Newton-Raphson, plain groups): per scenario the gradients of
+    // L = sum (u_pu - 1)^2 + sum line losses [MW] by every load_gen's p and q and every line's r1 and x1,
+    // into host arrays [n_scenarios][n_lg] and [n_scenarios][n_line]
+    void adjoint(double* grad_p, double* grad_q, double* grad_r, double* grad_x);
+    DevAdjoint* d_adj = nullptr;
+    DevGroup* d_groups_adj = nullptr;  // the group table with FP64 factor storage (mixed-precision batches)
+    double* d_grad = nullptr;  // grad_p, grad_q, grad_r, grad_x back to back
+    long long* d_adj_off = nullptr;
+    long long adj_total = 0;
     void set_output(PGM_MutableDataset const& output);
     void fetch(PGM_MutableDataset const& output, PGM_Handle* handle);
 };

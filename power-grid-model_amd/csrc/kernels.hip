@@ -48,6 +48,19 @@ template <class I> __device__ __forceinline__ int8_t ld(int8_t const* p, I i) {
 template <class T, class I> __device__ T ld(T const* p, I i) { return p[i]; }
 #endif
 
+// group of a flat thread index: offsets[g] <= idx < offsets[g + 1]
+__device__ __forceinline__ int find_group(long long const* __restrict__ offsets, int n_groups, long long idx) {
+    int lo = 0, hi = n_groups;
+    while (hi - lo > 1) {
+        int const mid = (lo + hi) >> 1;
+        if (ld(offsets, mid) <= idx)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 struct cplx {
     double re, im;
 };
@@ -395,7 +408,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
     if (!lane_work<KIND>(g, wi, task, s)) return;
     size_t const sp = g.S_pad;
     int const mm = ld(g.task_mm, task);
-    if (g.done[size_t(mm) * sp + s] != ACTIVE) return;
+    if (g.done[size_t(mm) * sp + s] != (MODE == MODE_NRA ? CONVERGED : ACTIVE)) return;
     Blocks<NB, FT> const lu{static_cast<FT*>(g.lu), sp, s};
     // a keep row whose parent is the next pivot of the task hands its finished U_p,parent,
     // L_parent,p and y_p to that pivot through LDS (fw_le: label of the forwarded L entry); LDS keeps
@@ -893,8 +906,10 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
         uint32_t code;
         bool const ok = factor_block<NB, FT>(Df, code);
         if (!ok) {
-            g.done[size_t(mm) * sp + s] = FAILED;
-            g.err[size_t(mm) * sp + s] = ERR_SPARSE;
+            if constexpr (MODE != MODE_NRA) {
+                g.done[size_t(mm) * sp + s] = FAILED;
+                g.err[size_t(mm) * sp + s] = ERR_SPARSE;
+            }
             return;
         }
         lu.store(dg, Df);
@@ -1268,6 +1283,210 @@ __global__ __launch_bounds__(kWave) void lowrank_kernel(DevGroup const* __restri
 }
 
 // ---------------------------------------------------------------------------------------------------
+// adjoint: J^T lambda = g with the factors J = L~ U~ of the sweeps (L~: P^T L_pp on the diagonal and the
+// L blocks; U~: U_pp Q^T on the diagonal and the U blocks). U~^T w = g runs in the forward order:
+//   w_p = U_pp^-T Q^T (g_p - sum_k U_kp^T w_k)      over the L entries (p, k) of row p;
+// then L~^T lambda = w in the backward order:
+//   lambda_p = P^T L_pp^-T (w_p - sum_i L_ip^T lambda_i)   over the U entries (p, i) of row p.
+// Symmetric batches, plain groups (one task, 64 scenarios per wave).
+// ---------------------------------------------------------------------------------------------------
+template <int NB> __device__ __forceinline__ void apply_qt_vec(double (&v)[NB], uint32_t code) {
+#pragma unroll
+    for (int t = 0; t < NB - 1; ++t) {
+        int const c = perm_col(code, t);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            if (k <= t) continue;
+            bool const sw = c == k;
+            double const x = v[t], y = v[k];
+            v[t] = sw ? y : x;
+            v[k] = sw ? x : y;
+        }
+    }
+}
+template <int NB> __device__ __forceinline__ void apply_pt_vec(double (&v)[NB], uint32_t code) {
+#pragma unroll
+    for (int t = NB - 2; t >= 0; --t) {
+        int const r = perm_row(code, t);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            if (k <= t) continue;
+            bool const sw = r == k;
+            double const x = v[t], y = v[k];
+            v[t] = sw ? y : x;
+            v[k] = sw ? x : y;
+        }
+    }
+}
+// v := U^-T v, U the upper part of a factored block (lu[c * NB + r], r <= c)
+template <int NB, class FT> __device__ __forceinline__ void upper_t_vec(FT const (&lu)[NB * NB], double (&v)[NB]) {
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+#pragma unroll
+        for (int c = 0; c < r; ++c) v[r] -= double(lu[r * NB + c]) * v[c];
+        v[r] /= double(lu[r * NB + r]);
+    }
+}
+// v := L^-T v, L the unit lower part (lu[c * NB + r], r > c)
+template <int NB, class FT> __device__ __forceinline__ void lower_t_vec(FT const (&lu)[NB * NB], double (&v)[NB]) {
+#pragma unroll
+    for (int r = NB - 1; r >= 0; --r)
+#pragma unroll
+        for (int c = NB - 1; c > r; --c) v[r] -= double(lu[r * NB + c]) * v[c];
+}
+
+template <class FT, bool UT>
+__global__ __launch_bounds__(kWave) void adjoint_sweep_kernel(DevGroup const* __restrict__ groups,
+                                                              WorkItem const* __restrict__ items) {
+    constexpr int NB = 2, NB2 = 4;
+    WorkItem const wi = ld(items, blockIdx.x);
+    DevGroup const g = ld(groups, wi.group);
+    int const task = wi.task;
+    int const s = wi.block * kWave + int(threadIdx.x);
+    if (s >= g.S) return;
+    size_t const sp = g.S_pad;
+    int const mm = ld(g.task_mm, task);
+    if (g.done[size_t(mm) * sp + s] != CONVERGED) return;
+    Blocks<NB, FT> const lu{static_cast<FT*>(g.lu), sp, s};
+    int const t0 = ld(g.task_ptr, task), t1 = ld(g.task_ptr, task + 1);
+    for (int q = 0; q < t1 - t0; ++q) {
+        int const ti = UT ? t0 + q : t1 - 1 - q;
+        int const p = ld(g.task_pivot, ti);
+        int const rb = ld(g.row_ptr, p), re = ld(g.row_end, p), dg = ld(g.diag, p);
+        int const es = g.row_stride ? ld(g.row_stride, p) : 1;
+        double v[NB];
+#pragma unroll
+        for (int r = 0; r < NB; ++r) v[r] = g.y[(size_t(p) * NB + r) * sp + s];
+        // UT: the L entries (p, j), j < p, give U_jp = the block at their transpose; LT: the U entries
+        // (p, j), j > p, give L_jp
+        int const k0 = UT ? rb : dg + es, k1 = UT ? dg : re;
+        for (int k = k0; k < k1; k += es) {
+            int const j = ld(g.col, k);
+            FT B[NB2];
+            lu.load(ld(g.transpose, k), B);
+            double xj[NB];
+#pragma unroll
+            for (int r = 0; r < NB; ++r) xj[r] = g.y[(size_t(j) * NB + r) * sp + s];
+#pragma unroll
+            for (int r = 0; r < NB; ++r) {
+                double acc = 0.0;
+#pragma unroll
+                for (int c = 0; c < NB; ++c) acc += double(B[r * NB + c]) * xj[c];
+                v[r] -= acc;
+            }
+        }
+        FT Df[NB2];
+        lu.load(dg, Df);
+        uint32_t const code = g.perm[size_t(p) * sp + s];
+        if constexpr (UT) {
+            apply_qt_vec<NB>(v, code);
+            upper_t_vec<NB, FT>(Df, v);
+        } else {
+            lower_t_vec<NB, FT>(Df, v);
+            apply_pt_vec<NB>(v, code);
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) g.y[(size_t(p) * NB + r) * sp + s] = v[r];
+    }
+}
+
+// dL/dx at the converged state, x = (theta, ln V) as the Jacobian's columns: the voltage term
+// 2 V (V - 1) per node of the bus, and the losses of the lines at the bus. For a line with admittances
+// ff, ft, tf, tt: loss = Re(ff) V_f^2 + Re(A) + Re(B) + Re(tt) V_t^2 with A = conj(ft) u_f conj(u_t),
+// B = conj(tf) u_t conj(u_f); d/dtheta_f = Im(B) - Im(A) = -d/dtheta_t, d/dlnV_f = 2 Re(ff) V_f^2 + Re(A) + Re(B)
+__global__ void adjoint_rhs_kernel(DevGroup const* __restrict__ groups, DevAdjoint const* __restrict__ adjs,
+                                   long long const* __restrict__ offsets, int n_groups) {
+    long long const gidx = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gidx >= ld(offsets, n_groups)) return;
+    int const gr = find_group(offsets, n_groups, gidx);
+    DevGroup const g = ld(groups, gr);
+    DevAdjoint const a = ld(adjs, gr);
+    size_t const sp = g.S_pad;
+    size_t const idx = size_t(gidx - ld(offsets, gr));
+    int const b = int(idx / sp), s = int(idx % sp);
+    double gth = 0.0, glv = 0.0;
+    int const mm = ld(g.bus_mm, b);
+    if (s < g.S && mm >= 0 && g.done[size_t(mm) * sp + s] == CONVERGED) {
+        auto U = [&](int bus) { return cplx{g.u[(size_t(bus) * 2) * sp + s], g.u[(size_t(bus) * 2 + 1) * sp + s]}; };
+        double const V = g.xv[(size_t(b) * 2 + 1) * sp + s];
+        glv = 2.0 * V * (V - 1.0) * double(ld(a.node_w, b));
+        for (int q = ld(a.bus_line_ptr, b); q < ld(a.bus_line_ptr, b + 1); ++q) {
+            int const mb = ld(a.bus_line, q);
+            int const f = ld(a.br_from, mb), t = ld(a.br_to, mb);
+            auto Y = [&](int w) { return cplx{ld(a.br_y, (mb * 4 + w) * 2), ld(a.br_y, (mb * 4 + w) * 2 + 1)}; };
+            bool const at_f = f == b;
+            cplx const own = Y(at_f ? 0 : 3);
+            glv += 2.0 * own.re * V * V;
+            if (f >= 0 && t >= 0 && f != t) {
+                cplx const uf = U(f), ut = U(t);
+                cplx const A = cmul(cconj(Y(1)), cmul(uf, cconj(ut)));
+                cplx const B = cmul(cconj(Y(2)), cmul(ut, cconj(uf)));
+                glv += A.re + B.re;
+                gth += at_f ? (B.im - A.im) : (A.im - B.im);
+            }
+        }
+    }
+    g.y[(size_t(b) * 2) * sp + s] = gth;
+    g.y[(size_t(b) * 2 + 1) * sp + s] = glv;
+}
+
+// gradients of lane s: loads dL/dp = lambda_P dir f(V) / 1e6 (f = 1, V^2, V for const power, impedance,
+// current), dL/dq likewise with lambda_Q; lines dL/dr = dloss/dr - sum over both ends of
+// (lambda_P Re dS + lambda_Q Im dS), dS_f = u_f conj(dff u_f + dft u_t), dS_t = u_t conj(dtf u_f + dtt u_t)
+__global__ void adjoint_grad_kernel(DevGroup const* __restrict__ groups, DevOutput const* __restrict__ outs,
+                                    DevAdjoint const* __restrict__ adjs, long long const* __restrict__ offsets,
+                                    int n_groups) {
+    long long const gidx = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gidx >= ld(offsets, n_groups)) return;
+    int const gr = find_group(offsets, n_groups, gidx);
+    DevGroup const g = ld(groups, gr);
+    DevOutput const o = ld(outs, gr);
+    DevAdjoint const a = ld(adjs, gr);
+    size_t const sp = g.S_pad;
+    size_t const idx = size_t(gidx - ld(offsets, gr));
+    int const c = int(idx / sp), s = int(idx % sp);
+    if (s >= g.S || !o.ok[s]) return;
+    long long const scn = ld(o.scenario_index, s);
+    auto lam = [&](int bus, int r) { return g.y[(size_t(bus) * 2 + r) * sp + s]; };
+    auto U = [&](int bus) { return cplx{g.u[(size_t(bus) * 2) * sp + s], g.u[(size_t(bus) * 2 + 1) * sp + s]}; };
+    if (c < a.n_lg) {
+        double gp = 0.0, gq = 0.0;
+        int const k = ld(a.lg_math, c);
+        if (k >= 0 && ld(a.load_status, size_t(c) * sp + s)) {
+            int const bus = ld(a.load_bus, k);
+            int const type = ld(g.load_type, k);
+            double const V = g.xv[(size_t(bus) * 2 + 1) * sp + s];
+            double const f = type == 0 ? 1.0 : (type == 1 ? V * V : V);
+            double const dir = ld(a.lg_gen, c) ? 1.0 : -1.0;
+            gp = lam(bus, 0) * dir * f / 1e6;
+            gq = lam(bus, 1) * dir * f / 1e6;
+        }
+        a.grad_p[scn * a.n_lg + c] = gp;
+        a.grad_q[scn * a.n_lg + c] = gq;
+        return;
+    }
+    int const line = c - a.n_lg;
+    double gr_ = 0.0, gx = 0.0;
+    int const mb = ld(a.line_math, line);
+    if (mb >= 0) {
+        int const f = ld(a.br_from, mb), t = ld(a.br_to, mb);
+        cplx const uf = f >= 0 ? U(f) : cplx{0.0, 0.0}, ut = t >= 0 ? U(t) : cplx{0.0, 0.0};
+        for (int which = 0; which < 2; ++which) {
+            double const* d = which == 0 ? a.br_dr : a.br_dx;
+            auto D = [&](int w) { return cplx{ld(d, (mb * 4 + w) * 2), ld(d, (mb * 4 + w) * 2 + 1)}; };
+            cplx const dsf = cmul(uf, cconj(cadd(cmul(D(0), uf), cmul(D(1), ut))));
+            cplx const dst = cmul(ut, cconj(cadd(cmul(D(2), uf), cmul(D(3), ut))));
+            double gsum = dsf.re + dst.re;
+            if (f >= 0) gsum -= lam(f, 0) * dsf.re + lam(f, 1) * dsf.im;
+            if (t >= 0) gsum -= lam(t, 0) * dst.re + lam(t, 1) * dst.im;
+            (which == 0 ? gr_ : gx) = gsum;
+        }
+    }
+    a.grad_r[scn * a.n_line + line] = gr_;
+    a.grad_x[scn * a.n_line + line] = gx;
+}
+
+// ---------------------------------------------------------------------------------------------------
 // load_gen updates on the device (update.hpp semantics: a null status or power keeps the base value;
 // s = direction (p + j q) / base_power, load_gen.hpp:121-139; asymmetric components in a symmetric
 // calculation take the mean of the phases)
@@ -1387,18 +1606,6 @@ __global__ void converge_kernel(PairState ps, int it, int max_iter, double tol, 
     }
 }
 
-// group of a flat thread index: offsets[g] <= idx < offsets[g + 1]
-__device__ __forceinline__ int find_group(long long const* __restrict__ offsets, int n_groups, long long idx) {
-    int lo = 0, hi = n_groups;
-    while (hi - lo > 1) {
-        int const mid = (lo + hi) >> 1;
-        if (ld(offsets, mid) <= idx)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
 
 template <int P>
 __global__ void ic_start_kernel(DevGroup const* __restrict__ groups, long long const* __restrict__ offsets,
@@ -1823,8 +2030,12 @@ void launch_fwd(DevGroup const* groups, WorkItem const* items, int n_items, int 
         hipLaunchKernelGGL((forward_kernel<P, FT, MODE_ICF, KIND>), grid, dim3(kWave), 0, st, groups, items);
     else if (mode == MODE_RESID)
         hipLaunchKernelGGL((forward_kernel<P, FT, MODE_RESID, KIND>), grid, dim3(kWave), 0, st, groups, items);
-    else if constexpr (KIND == KIND_PLAIN)
-        hipLaunchKernelGGL((forward_kernel<P, FT, MODE_RAW, KIND_PLAIN>), grid, dim3(kWave), 0, st, groups, items);
+    else if constexpr (KIND == KIND_PLAIN) {
+        if (mode == MODE_NRA)
+            hipLaunchKernelGGL((forward_kernel<P, FT, MODE_NRA, KIND_PLAIN>), grid, dim3(kWave), 0, st, groups, items);
+        else
+            hipLaunchKernelGGL((forward_kernel<P, FT, MODE_RAW, KIND_PLAIN>), grid, dim3(kWave), 0, st, groups, items);
+    }
 }
 template <int P, class FT, int KIND>
 void launch_bwd(DevGroup const* groups, WorkItem const* items, int n_items, int mode, hipStream_t st) {
@@ -1961,6 +2172,31 @@ void launch_bus_results(int P, DevOutput const* outs, long long const* offsets, 
         hipLaunchKernelGGL(bus_results_kernel<1>, grid, dim3(256), 0, st, outs, offsets, n_groups);
     else
         hipLaunchKernelGGL(bus_results_kernel<3>, grid, dim3(256), 0, st, outs, offsets, n_groups);
+}
+void launch_adjoint_rhs(DevGroup const* groups, DevAdjoint const* adj, long long const* bus_offsets, int n_groups,
+                        long long total, hipStream_t st) {
+    if (total == 0) return;
+    hipLaunchKernelGGL(adjoint_rhs_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, groups, adj,
+                       bus_offsets, n_groups);
+}
+void launch_adjoint_sweep(DevGroup const* groups, WorkItem const* items, int n_items, bool fp32, bool ut_sweep,
+                          hipStream_t st) {
+    if (n_items <= 0) return;
+    dim3 const grid{unsigned(n_items)};
+    if (fp32 && ut_sweep)
+        hipLaunchKernelGGL((adjoint_sweep_kernel<float, true>), grid, dim3(kWave), 0, st, groups, items);
+    else if (fp32)
+        hipLaunchKernelGGL((adjoint_sweep_kernel<float, false>), grid, dim3(kWave), 0, st, groups, items);
+    else if (ut_sweep)
+        hipLaunchKernelGGL((adjoint_sweep_kernel<double, true>), grid, dim3(kWave), 0, st, groups, items);
+    else
+        hipLaunchKernelGGL((adjoint_sweep_kernel<double, false>), grid, dim3(kWave), 0, st, groups, items);
+}
+void launch_adjoint_grad(DevGroup const* groups, DevOutput const* outs, DevAdjoint const* adj,
+                         long long const* offsets, int n_groups, long long total, hipStream_t st) {
+    if (total == 0) return;
+    hipLaunchKernelGGL(adjoint_grad_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, groups, outs, adj,
+                       offsets, n_groups);
 }
 void launch_load_broadcast(double* s_inj, int8_t* lstat, double const* base_s, int8_t const* base_status, int n_load,
                            int n_lg, int P, int S, int S_pad, hipStream_t st) {

@@ -16,7 +16,9 @@ constexpr int kWaveLanes = 64;  // lanes of a gfx950 wavefront: one scenario (or
 // INIT: linear system (NR initial guess, linear method); NR: Newton-Raphson step; RAW: block LU test
 // hook; ICF: iterative current, factorise Y + y_src and solve the current residual; RESID: iterative
 // current with the stored factors (u += x after both)
-enum SweepMode : int { MODE_INIT = 0, MODE_NR = 1, MODE_RAW = 2, MODE_ICF = 3, MODE_RESID = 4 };
+// MODE_NRA: the Newton-Raphson Jacobian assembled and factorised at the converged state of converged
+// lanes, every block kept (the adjoint solves with it)
+enum SweepMode : int { MODE_INIT = 0, MODE_NR = 1, MODE_RAW = 2, MODE_ICF = 3, MODE_RESID = 4, MODE_NRA = 5 };
 enum DoneState : int { ACTIVE = 0, CONVERGED = 1, FAILED = 2 };
 enum ErrCode : int { ERR_NONE = 0, ERR_SPARSE = 1, ERR_DIVERGE = 2 };
 
@@ -236,6 +238,40 @@ constexpr int record_tile(size_t record_bytes) {
 // W = min(S_pad, 64) lanes per tile)
 void launch_bus_results(int P, DevOutput const* outs, long long const* offsets, int n_groups, long long total,
                         hipStream_t st);
+// adjoint of a symmetric Newton-Raphson batch (plain groups): objective per scenario
+//   L = sum over nodes (u_pu - 1)^2 + sum over lines (p_from + p_to) [MW],
+// gradients by every load_gen's p and q [W, var] and every line's r1 and x1 [ohm]
+struct DevAdjoint {
+    int n_lg = 0, n_line = 0;
+    int const* bus_line_ptr = nullptr;  // [n + 1] lines incident to each bus
+    int const* bus_line = nullptr;      // math branch of each incidence
+    int const* node_w = nullptr;        // [n] nodes of each bus
+    int const* br_from = nullptr;       // [n_br] bus of each side (-1: open)
+    int const* br_to = nullptr;
+    double const* br_y = nullptr;       // [n_br][4][2] ff, ft, tf, tt
+    double const* br_dr = nullptr;      // [n_br][4][2] derivatives by r1 (lines; zero otherwise)
+    double const* br_dx = nullptr;      // [n_br][4][2] derivatives by x1
+    int const* lg_math = nullptr;       // [n_lg] math load index, -1 when not in the group
+    int const* load_bus = nullptr;      // [n_load] bus of each math load
+    int const* line_math = nullptr;     // [n_line] math branch of each line, -1 when not in the group
+    int8_t const* lg_gen = nullptr;     // [n_lg] 1: generator (direction +1)
+    int8_t const* load_status = nullptr;  // [n_lg][S_pad] status of every load_gen in every lane
+    double* grad_p = nullptr;           // [n_scenarios][n_lg]
+    double* grad_q = nullptr;
+    double* grad_r = nullptr;           // [n_scenarios][n_line]
+    double* grad_x = nullptr;
+};
+// dL/dx of the converged state into the groups' forward vectors (theta and ln V columns)
+void launch_adjoint_rhs(DevGroup const* groups, DevAdjoint const* adj, long long const* bus_offsets, int n_groups,
+                        long long total, hipStream_t st);
+// J^T lambda = dL/dx with the factors of MODE_NRA: the U^T sweep (forward order), the L^T sweep
+// (backward order); lambda ends in the forward vectors
+void launch_adjoint_sweep(DevGroup const* groups, WorkItem const* items, int n_items, bool fp32, bool ut_sweep,
+                          hipStream_t st);
+// gradients of every load_gen and line of every lane: offsets over (n_lg + n_line) * S_pad per group
+void launch_adjoint_grad(DevGroup const* groups, DevOutput const* outs, DevAdjoint const* adj,
+                         long long const* offsets, int n_groups, long long total, hipStream_t st);
+
 // load_gen updates applied on the device: one update buffer whose scenarios all list the same
 // components in the same order. Fields are addressed as base + row * stride (row = scenario * n_elem +
 // element); a null field is absent (the base value stays). asym: the record holds three phases.

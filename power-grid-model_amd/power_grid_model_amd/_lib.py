@@ -75,6 +75,7 @@ def _load():
         "PGM_amd_batch_run": (None, [_p, _p]),
         "PGM_amd_batch_fetch": (None, [_p, _p, _p]),
         "PGM_amd_batch_set_output": (None, [_p, _p, _p]),
+        "PGM_amd_batch_adjoint": (None, [_p, _p, _p, _p, _p, _p]),
         "PGM_amd_batch_destroy": (None, [_p]),
         "PGM_amd_batch_n_scenarios": (Idx, [_p]),
         "PGM_amd_batch_iterations": (None, [_p, _p]),

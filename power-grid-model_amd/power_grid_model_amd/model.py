@@ -396,6 +396,34 @@ class DeviceBatch:
             raise_regular_error(self._handle.message())
         return output
 
+    def adjoint(self) -> dict:
+        """Adjoint gradients of the last run: for every scenario the objective
+        L = sum over energised nodes (u_pu - 1)^2 + sum over lines (p_from + p_to) [MW] and its gradients
+        by every load_gen's p_specified / q_specified [per W, var] and every line's r1 / x1 [per ohm].
+        Returns {component: {attribute: array (n_scenarios, n_component)}}."""
+        counts = self._model.all_component_count
+        lg = [c for c in ("sym_gen", "asym_gen", "sym_load", "asym_load")]
+        n_lg = sum(counts.get(c, 0) for c in lg)
+        n_line = counts.get("line", 0)
+        S = self.n_scenarios
+        gp = np.zeros((S, n_lg))
+        gq = np.zeros((S, n_lg))
+        gr = np.zeros((S, n_line))
+        gx = np.zeros((S, n_line))
+        lib.PGM_amd_batch_adjoint(self._handle.ptr, self._ptr, gp.ctypes.data, gq.ctypes.data, gr.ctypes.data,
+                                  gx.ctypes.data)
+        self._handle.check()
+        out = {}
+        off = 0
+        for c in lg:
+            n = counts.get(c, 0)
+            if n:
+                out[c] = {"p_specified": gp[:, off:off + n], "q_specified": gq[:, off:off + n]}
+            off += n
+        if n_line:
+            out["line"] = {"r1": gr, "x1": gx}
+        return out
+
     def iterations(self) -> np.ndarray:
         out = np.zeros(self.n_scenarios, dtype=np.int64)
         lib.PGM_amd_batch_iterations(self._ptr, out.ctypes.data)

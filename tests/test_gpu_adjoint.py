@@ -1,0 +1,128 @@
+"""Adjoint gradients of the HIP path against finite differences of the same solver.
+
+Objective per scenario: L = sum over energised nodes (u_pu - 1)^2 + sum over lines (p_from + p_to) in MW.
+Gradients by load p_specified / q_specified and by line r1 / x1 come from one transposed solve with the
+Newton-Raphson Jacobian factorised at the converged state.
+
+The finite differences: every side is solved to 1e-12 pu in FP64. Even so the converged states of a
+200-bus feeder carry rounding of about 1e-11 in L (the CPU oracle and the GPU disagree at that level),
+so a central difference at h = 1e-6 relative has a few percent of noise for these gradients. The test
+therefore uses h = 1e-2 relative with Richardson extrapolation, D = (8 D1(h) - D1(2h)) / 6 over the
+one-sided central differences D1: truncation O(h^4) ~ 1e-8, noise ~ 1e-6 of a typical gradient. The bar
+is 2e-6 relative, with an absolute floor of 1e-4 times the largest gradient of the kind (gradients that
+small are below the differences' noise).
+"""
+
+import numpy as np
+import pytest
+
+pgm = pytest.importorskip("power_grid_model_amd")
+from power_grid_model_amd import DeviceBatch, PowerGridModel, synthetic  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+OUTPUT = {"node": ["u_pu", "energized"], "line": ["p_from", "p_to"]}
+
+
+def _objective(model, update, precision="fp64", tol=1e-12):
+    batch = DeviceBatch(model, update, error_tolerance=tol, max_iterations=30, factor_precision=precision)
+    batch.set_output(OUTPUT)
+    batch.run()
+    out = batch.fetch()
+    u, en = out["node"]["u_pu"], out["node"]["energized"]
+    loss = np.sum(np.where(en == 1, (u - 1.0) ** 2, 0.0), axis=-1)
+    loss = loss + np.sum(out["line"]["p_from"] + out["line"]["p_to"], axis=-1) / 1e6
+    return loss, batch
+
+
+def _case():
+    """radial_200 with a third of the loads const_z and a third const_i (types 1 and 2)."""
+    case = synthetic.radial_200(n_scenarios=40)
+    inp = {k: v.copy() for k, v in case.input.items()}
+    n = len(inp["sym_load"])
+    inp["sym_load"]["type"] = np.arange(n) % 3
+    return inp, case.update
+
+
+H_REL = 1e-2
+
+
+def _richardson(lp, lm, lp2, lm2, h):
+    """(8 (L(+h) - L(-h)) - (L(+2h) - L(-2h))) / (12 h)"""
+    return (8.0 * (lp - lm) - (lp2 - lm2)) / (12.0 * h)
+
+
+def _close(adj, fd, name):
+    floor = 1e-4 * max(np.max(np.abs(adj)), np.max(np.abs(fd)))
+    np.testing.assert_allclose(adj, fd, rtol=2e-6, atol=floor, err_msg=name)
+
+
+def test_adjoint_load_gradients_match_finite_differences():
+    inp, upd = _case()
+    model = PowerGridModel(inp)
+    loss, batch = _objective(model, upd)
+    grads = batch.adjoint()["sym_load"]
+    rng = np.random.default_rng(1)
+    scn = 7
+    loads = rng.choice(len(inp["sym_load"]), 12, replace=False)
+    base = upd["sym_load"][scn]
+    for attr in ("p_specified", "q_specified"):
+        rows = []
+        steps = []
+        for l in loads:
+            h = H_REL * abs(base[attr][l])
+            steps.append(h)
+            for mult in (1.0, -1.0, 2.0, -2.0):
+                r = base.copy()
+                r[attr][l] += mult * h
+                rows.append(r)
+        pert, _ = _objective(model, {"sym_load": np.stack(rows)})
+        fd = _richardson(pert[0::4], pert[1::4], pert[2::4], pert[3::4], np.array(steps))
+        _close(grads[attr][scn, loads], fd, attr)
+
+
+def test_adjoint_line_gradients_match_finite_differences():
+    inp, upd = _case()
+    model = PowerGridModel(inp)
+    _, batch = _objective(model, upd)
+    grads = batch.adjoint()["line"]
+    rng = np.random.default_rng(2)
+    scn = 11
+    lines = rng.choice(len(inp["line"]), 8, replace=False)
+    one = {"sym_load": upd["sym_load"][scn:scn + 1]}
+    for attr, key in (("r1", "r1"), ("x1", "x1")):
+        fd = []
+        for b in lines:
+            h = H_REL * abs(inp["line"][attr][b])
+            vals = []
+            for mult in (1.0, -1.0, 2.0, -2.0):
+                mod = {k: v.copy() for k, v in inp.items()}
+                mod["line"][attr][b] += mult * h
+                val, _ = _objective(PowerGridModel(mod), one)
+                vals.append(val[0])
+            fd.append(_richardson(*vals, h))
+        _close(grads[key][scn, lines], np.array(fd), attr)
+
+
+def test_adjoint_mixed_precision_against_fp64():
+    """The adjoint factorises in FP64 also for a mixed-precision batch: its gradients equal the FP64
+    batch's up to the two converged states' difference (both within 1e-8 pu)."""
+    inp, upd = _case()
+    model = PowerGridModel(inp)
+    _, b64 = _objective(model, upd, "fp64", 1e-8)
+    _, b32 = _objective(model, upd, "fp32", 1e-8)
+    g64, g32 = b64.adjoint(), b32.adjoint()
+    for comp, attrs in (("sym_load", ("p_specified", "q_specified")), ("line", ("r1", "x1"))):
+        for a in attrs:
+            ref = g64[comp][a]
+            np.testing.assert_allclose(g32[comp][a], ref, rtol=1e-5, atol=1e-6 * np.max(np.abs(ref)),
+                                       err_msg=f"{comp}.{a}")
+
+
+def test_adjoint_rejects_unsupported_batches():
+    inp, upd = _case()
+    model = PowerGridModel(inp)
+    batch = DeviceBatch(model, {"sym_load": upd["sym_load"][:8]})  # 8 scenarios: a packed group
+    batch.run()
+    with pytest.raises(Exception, match="more than 32 scenarios"):
+        batch.adjoint()
