@@ -126,3 +126,35 @@ def test_adjoint_rejects_unsupported_batches():
     batch.run()
     with pytest.raises(Exception, match="more than 32 scenarios"):
         batch.adjoint()
+
+
+def test_torch_function_directional_derivative():
+    """PowerFlowLoss.backward (the adjoint) against a Richardson-extrapolated directional difference of the
+    weighted total loss along a random direction in every load p and every line r1 and x1."""
+    torch = pytest.importorskip("torch")
+    from power_grid_model_amd.autograd import PowerFlowLoss
+
+    inp, upd = _case()
+    loads = upd["sym_load"]
+    p0 = torch.tensor(loads["p_specified"], dtype=torch.float64)
+    q0 = torch.tensor(loads["q_specified"], dtype=torch.float64)
+    r0 = torch.tensor(inp["line"]["r1"], dtype=torch.float64)
+    x0 = torch.tensor(inp["line"]["x1"], dtype=torch.float64)
+    p, q, r, x = (t.clone().requires_grad_(True) for t in (p0, q0, r0, x0))
+    w = torch.linspace(0.5, 1.5, p0.shape[0], dtype=torch.float64)
+    loss = PowerFlowLoss.apply(p, q, r, x, inp, "fp32", 1e-8, 20)
+    (loss * w).sum().backward()
+    rng = np.random.default_rng(5)
+    dp = torch.tensor(rng.standard_normal(p0.shape), dtype=torch.float64) * p0 * 1e-2
+    dr = torch.tensor(rng.standard_normal(r0.shape), dtype=torch.float64) * r0 * 1e-2
+    dx = torch.tensor(rng.standard_normal(x0.shape), dtype=torch.float64) * x0 * 1e-2
+
+    def total(t):
+        with torch.no_grad():
+            val = PowerFlowLoss.apply(p0 + t * dp, q0, r0 + t * dr, x0 + t * dx, inp, "fp64", 1e-12, 30)
+        return float((val * w).sum())
+
+    fd = _richardson(total(1.0), total(-1.0), total(2.0), total(-2.0), 1.0)
+    ad = float((p.grad * dp).sum() + (r.grad * dr).sum() + (x.grad * dx).sum())
+    assert abs(ad - fd) <= 2e-6 * abs(fd), (ad, fd)
+    assert q.grad.shape == q0.shape and torch.isfinite(q.grad).all()
