@@ -103,6 +103,7 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-fp64-leg", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--e2e-fraction", type=float, default=0.2,
                     help="share of the scenarios timed through calculate_power_flow from host buffers (0: skip)")
@@ -201,6 +202,29 @@ def main():
                "outputs": comps}
         log(f"[bench] calculate_power_flow from host buffers: {n_e2e} scenarios in {secs:.2f}s")
 
+    # adjoint gradients on the uneven-feeder batch (64 feeders, 128 scenarios): the solve, then one
+    # adjoint pass (re-factorisation in FP64 at the converged state and the transposed sweeps)
+    adjoint = None
+    if not args.no_adjoint:
+        fcase = synthetic.feeders_64()
+        fmodel = PowerGridModel(fcase.input)
+        fb = DeviceBatch(fmodel, fcase.update, factor_precision=args.precision)
+        fb.set_output({"node": ["u_pu"], "line": ["p_from", "p_to"]})
+        fb.run()
+        solve, adj = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fb.run()
+            solve.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            g = fb.adjoint()
+            adj.append(time.perf_counter() - t0)
+        n_par = sum(a.size for comp in g.values() for a in comp.values())
+        adjoint = {"workload": fcase.name, "scenarios": fcase.n_scenarios, "solve_ms": 1e3 * float(np.median(solve)),
+                   "adjoint_ms": 1e3 * float(np.median(adj)), "gradient_entries": int(n_par)}
+        log(f"[bench] adjoint on {fcase.name}: solve {adjoint['solve_ms']:.1f} ms, adjoint {adjoint['adjoint_ms']:.1f} ms")
+        del fb
+
     cpu = None
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
@@ -245,6 +269,7 @@ def main():
         "step_roofline": {"algorithmic_bytes": step_bytes, "achieved_gbs": step_bytes / t_step / 1e9,
                           "frac": step_bytes / t_step / 1e9 / HBM_PEAK_GBS},
         "pcie_inclusive": e2e,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
     }

@@ -158,3 +158,40 @@ def test_torch_function_directional_derivative():
     ad = float((p.grad * dp).sum() + (r.grad * dr).sum() + (x.grad * dx).sum())
     assert abs(ad - fd) <= 2e-6 * abs(fd), (ad, fd)
     assert q.grad.shape == q0.shape and torch.isfinite(q.grad).all()
+
+
+def test_adjoint_on_meshed_grid_with_fill_ins():
+    """meshed_1000 (loops, fill-ins in the factors): load and line gradients against finite differences."""
+    case = synthetic.meshed_1000(n_scenarios=40)
+    inp, upd = case.input, case.update
+    model = PowerGridModel(inp)
+    _, batch = _objective(model, upd)
+    grads = batch.adjoint()
+    rng = np.random.default_rng(4)
+    scn = 3
+    loads = rng.choice(len(inp["sym_load"]), 6, replace=False)
+    base = upd["sym_load"][scn]
+    rows, steps = [], []
+    for l in loads:
+        h = H_REL * abs(base["p_specified"][l])
+        steps.append(h)
+        for mult in (1.0, -1.0, 2.0, -2.0):
+            r = base.copy()
+            r["p_specified"][l] += mult * h
+            rows.append(r)
+    pert, _ = _objective(model, {"sym_load": np.stack(rows)})
+    fd = _richardson(pert[0::4], pert[1::4], pert[2::4], pert[3::4], np.array(steps))
+    _close(grads["sym_load"]["p_specified"][scn, loads], fd, "p_specified")
+    one = {"sym_load": upd["sym_load"][scn:scn + 1]}
+    lines = rng.choice(len(inp["line"]), 4, replace=False)
+    fd = []
+    for b in lines:
+        h = H_REL * abs(inp["line"]["x1"][b])
+        vals = []
+        for mult in (1.0, -1.0, 2.0, -2.0):
+            mod = {k: v.copy() for k, v in inp.items()}
+            mod["line"]["x1"][b] += mult * h
+            val, _ = _objective(PowerGridModel(mod), one)
+            vals.append(val[0])
+        fd.append(_richardson(*vals, h))
+    _close(grads["line"]["x1"][scn, lines], np.array(fd), "x1")
