@@ -235,6 +235,19 @@ int packed_target() {
     return t;
 }
 
+// waves a plain group's schedule aims at per level (tasks x 64-scenario blocks); PGM_AMD_PLAIN_WAVES
+// overrides
+int plain_waves() {
+    static int const w = [] {
+        char const* v = std::getenv("PGM_AMD_PLAIN_WAVES");
+        int const x = v ? std::atoi(v) : 0;
+        // about two rounds of resident waves over the chip at level 0: shorter tasks shorten the
+        // dependent pivot chains more than the extra levels cost (year_20k: 71.1 ms at 4096, 66.0 ms at 12288)
+        return x > 0 ? x : 12288;
+    }();
+    return w;
+}
+
 // every branch side of st is open where the base state has it open
 bool opens_only(Model const& m, ComponentState const& st) {
     auto le = [](std::vector<IntS> const& a, std::vector<IntS> const& b) {
@@ -1270,7 +1283,8 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
                     std::iota(all.begin(), all.end(), Idx{0});
                     int const nblk = int((g.scenarios.size() + 63) / 64);
                     g.packed = !g.is_union && g.scenarios.size() <= size_t(kPackedMax);
-                    int const target = g.packed ? packed_target() : std::clamp(4096 / std::max(nblk, 1), 4, 256);
+                    int const target =
+                        g.packed ? packed_target() : std::clamp(plain_waves() / std::max(nblk, 1), 4, 256);
                     g.plan = build_plan(m, g.st, g.topo, all, sym, target);
                     if (g.packed) {
                         // lanes of a packed wave run different tasks: interleave them in memory
