@@ -1053,13 +1053,18 @@ __global__ __launch_bounds__(kWave) void backward_kernel(DevGroup const* __restr
 #pragma unroll
             for (int ph = 0; ph < P; ++ph) {
                 size_t const b = (size_t(p) * P + ph) * 2;
-                double th = g.xv[b * sp + s] + v[ph];
-                double vm = g.xv[(b + 1) * sp + s];
+                double const th0 = g.xv[b * sp + s], vm0 = g.xv[(b + 1) * sp + s];
+                double th = th0 + v[ph];
+                double vm = vm0;
                 vm += vm * v[ph + P];
-                double sn, cs;
+                double sn, cs, sn0, cs0;
                 sincos(th, &sn, &cs);
+                // the previous voltage from the previous polar state, as the previous update computed it
+                // (no read of the stored voltage; after the linear initial guess it differs from the stored
+                // one by rounding only)
+                sincos(th0, &sn0, &cs0);
                 double const nr = vm * cs, ni = vm * sn;
-                double const dr = nr - g.u[b * sp + s], di = ni - g.u[(b + 1) * sp + s];
+                double const dr = nr - vm0 * cs0, di = ni - vm0 * sn0;
                 double const d = sqrt(dr * dr + di * di);
                 dbus = ph == 0 ? d : (d > dbus ? d : dbus);
                 g.xv[b * sp + s] = th;
