@@ -395,8 +395,16 @@ __device__ __forceinline__ void extra_entries(DevGroup const& g, int p, int s, c
 // forward sweep: assemble row / column of each pivot, pull the Schur updates, factor the diagonal
 // block, finish the U row and L column, forward-substitute
 // ---------------------------------------------------------------------------------------------------
+// waves per SIMD the sweeps are compiled for where 80 VGPRs hold without spilling
+template <int P, class FT, int KIND> constexpr int forward_waves() {
+    return P == 1 && KIND == KIND_PLAIN && sizeof(FT) == 4 ? 6 : 1;
+}
+template <int P, int MODE, int KIND> constexpr int backward_waves() {
+    return P == 1 && KIND != KIND_PACKED ? (MODE == MODE_NR ? 8 : 6) : 1;
+}
+
 template <int P, class FT, int MODE, int KIND>
-__global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restrict__ groups,
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(forward_waves<P, FT, KIND>()))) void forward_kernel(DevGroup const* __restrict__ groups,
                                                         WorkItem const* __restrict__ items) {
     constexpr int NB = 2 * P;
     constexpr int NB2 = NB * NB;
@@ -972,7 +980,7 @@ __global__ __launch_bounds__(kWave) void forward_kernel(DevGroup const* __restri
 // backward sweep: x_p = Q U_pp^-1 (y_p - sum_j U_pj x_j), then the state update of bus p
 // ---------------------------------------------------------------------------------------------------
 template <int P, class FT, int MODE, int KIND>
-__global__ __launch_bounds__(kWave) void backward_kernel(DevGroup const* __restrict__ groups,
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(backward_waves<P, MODE, KIND>()))) void backward_kernel(DevGroup const* __restrict__ groups,
                                                          WorkItem const* __restrict__ items) {
     constexpr int NB = 2 * P;
     constexpr int NB2 = NB * NB;
