@@ -397,7 +397,6 @@ void upload_group(Batch& b, Group& g) {
     o.u = d.u;
     o.n_load = g.n_load;
     o.load_type = d.load_type;
-    o.load_si = dev_alloc<double>(A, size_t(g.n_load) * 2 * P * 2 * sp);
     o.n_src = g.n_src;
     o.src_ptr = d.src_ptr;
     o.load_ptr = d.load_ptr;
@@ -488,7 +487,7 @@ void upload_group(Batch& b, Group& g) {
             make(
                 lg_comps[kk], m.lg_count[kk], [&](Idx k) { return m.lg_id[off + k]; },
                 [&](Idx k) { return lg_math[off + k]; }, [&](Idx k) { return m.lg_node[off + k]; },
-                [&](Idx) { return int8_t(0); }, kk < 2 ? 1.0 : -1.0, g.load_status + size_t(off) * sp, o.load_si);
+                [&](Idx) { return int8_t(0); }, kk < 2 ? 1.0 : -1.0, g.load_status + size_t(off) * sp, nullptr);
         }
     }
 }
@@ -1635,10 +1634,11 @@ void Batch::run() {
         }
         return cs;
     };
-    // appliance injections and bus sums: needed by the node and appliance records
-    bool const appl = want[C_SHUNT] || want[C_SOURCE] || want[C_SYM_GEN] || want[C_ASYM_GEN] || want[C_SYM_LOAD] ||
-                      want[C_ASYM_LOAD];
-    if (appl || want[C_NODE]) launch_bus_results(P, d_outputs, d_bus_off, G, bus_total, st);
+    // source and shunt flows and the appliance sum of every bus: needed by source and shunt records and
+    // by the node's p and q (load_gen records form their own power from the injection and the voltage)
+    bool node_pq = want[C_NODE] && cols[C_NODE].empty();
+    for (auto const& c : cols[C_NODE]) node_pq = node_pq || c.attr->name == "p" || c.attr->name == "q";
+    if (node_pq || want[C_SOURCE] || want[C_SHUNT]) launch_bus_results(P, d_outputs, d_bus_off, G, bus_total, st);
     if (want[C_NODE])
         launch_node_records(P, d_outputs, d_node_recs, d_rec_off[C_NODE], G, rec_total[C_NODE], out[C_NODE],
                             column_set(C_NODE), st);

@@ -1663,8 +1663,6 @@ __device__ __forceinline__ void bus_result(DevOutput const& o, int b, int s) {
     if (s >= o.S || !o.ok[s] || ld(o.bus_mm, b) < 0) return;
     if (o.dead && o.dead[size_t(b) * sp + s]) {
         // not energised in this lane: the appliances of the bus carry nothing
-        for (int l = ld(o.load_ptr, b); l < ld(o.load_ptr, b + 1); ++l)
-            for (int q = 0; q < 4 * P; ++q) o.load_si[(size_t(l) * 4 * P + q) * sp + s] = 0.0;
         for (int q = ld(o.shunt_ptr, b); q < ld(o.shunt_ptr, b + 1); ++q)
             for (int c = 0; c < 4 * P; ++c) o.shunt_si[(size_t(q) * 4 * P + c) * sp + s] = 0.0;
         for (int c = 0; c < 2 * P; ++c) o.bus_inj[(size_t(b) * 2 * P + c) * sp + s] = 0.0;
@@ -1688,11 +1686,6 @@ __device__ __forceinline__ void bus_result(DevOutput const& o, int b, int s) {
             double const f = type == 0 ? 1.0 : (type == 1 ? a2 : sqrt(a2));
             sv = {sv.re * f, sv.im * f};
             cplx const iv = cconj(cdiv(sv, ub[ph]));
-            size_t const base = (size_t(l) * 2) * P;
-            o.load_si[((base + ph) * 2) * sp + s] = sv.re;
-            o.load_si[((base + ph) * 2 + 1) * sp + s] = sv.im;
-            o.load_si[((base + P + ph) * 2) * sp + s] = iv.re;
-            o.load_si[((base + P + ph) * 2 + 1) * sp + s] = iv.im;
             iload[ph] = cadd(iload[ph], iv);
             acc_s[ph] = cadd(acc_s[ph], sv);
         }
@@ -1927,13 +1920,26 @@ __device__ __forceinline__ void appliance_record(DevOutput const& o, ApplRec con
     double const base_power = P == 1 ? 1e6 : 1e6 / 3.0;
     int const bus = ld(r.bus, i);
     bool const dead = o.dead && bus >= 0 && ld(o.dead, size_t(bus) * sp + s);
-    if (m >= 0 && !dead) {
+    if (m >= 0 && !dead && (r.si || bus >= 0)) {
         rec.energized = r.status ? ld(r.status, size_t(i) * sp + s) : ld(r.status_const, i);
         size_t const base = (size_t(m) * 2) * P;
+        // load_gens: s = s_inj f(|u|) by type (1, |u|^2, |u| for constant power, impedance, current; the
+        // linear method treats every load as constant impedance), i = conj(s / u)
+        int const type = r.si ? 0 : (o.linear ? 1 : ld(o.load_type, m));
 #pragma unroll
         for (int ph = 0; ph < P; ++ph) {
-            cplx const sv = {r.si[((base + ph) * 2) * sp + s], r.si[((base + ph) * 2 + 1) * sp + s]};
-            cplx const iv = {r.si[((base + P + ph) * 2) * sp + s], r.si[((base + P + ph) * 2 + 1) * sp + s]};
+            cplx sv, iv;
+            if (r.si) {
+                sv = {r.si[((base + ph) * 2) * sp + s], r.si[((base + ph) * 2 + 1) * sp + s]};
+                iv = {r.si[((base + P + ph) * 2) * sp + s], r.si[((base + P + ph) * 2 + 1) * sp + s]};
+            } else {
+                cplx const ub = {o.u[((size_t(bus) * P + ph) * 2) * sp + s], o.u[((size_t(bus) * P + ph) * 2 + 1) * sp + s]};
+                sv = {o.s_inj[((size_t(m) * P + ph) * 2) * sp + s], o.s_inj[((size_t(m) * P + ph) * 2 + 1) * sp + s]};
+                double const a2 = ub.re * ub.re + ub.im * ub.im;
+                double const f = type == 0 ? 1.0 : (type == 1 ? a2 : sqrt(a2));
+                sv = {sv.re * f, sv.im * f};
+                iv = cconj(cdiv(sv, ub));
+            }
             rec.p[ph] = base_power * sv.re * r.direction;
             rec.q[ph] = base_power * sv.im * r.direction;
             rec.s[ph] = base_power * cabs_(sv);

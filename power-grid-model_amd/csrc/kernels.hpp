@@ -176,7 +176,6 @@ struct DevOutput {
     int const* load_bus = nullptr;
     int8_t const* load_type = nullptr;
     double const* s_inj = nullptr;
-    double* load_si = nullptr;  // [n_load][2 (s, i)][P][2][S_pad]
     // sources (math order)
     int n_src = 0;
     int const* src_bus = nullptr;
@@ -227,7 +226,8 @@ struct ApplRec {
     double direction;
     int8_t const* status; // per component per lane [n_comp][S_pad], or nullptr (use status_const)
     int8_t const* status_const;
-    double const* si;     // [n_math][2][P][2][S_pad]
+    double const* si;     // [n_math][2][P][2][S_pad]; nullptr for load_gens, whose power and current the
+                          // record writer forms from the injection and the bus voltage
 };
 // components per record tile (64 lanes x TN records staged in at most 24 KiB of LDS)
 constexpr int record_tile(size_t record_bytes) {
