@@ -1624,6 +1624,9 @@ void Batch::run() {
     }
     PGM_HIP_CHECK(hipMemcpy(d_ok, ok.data(), ok.size() * 4, hipMemcpyHostToDevice));
     int const G = int(groups.size());
+    bool node_pq = want[C_NODE] && cols[C_NODE].empty();
+    for (auto const& c : cols[C_NODE]) node_pq = node_pq || c.attr->name == "p" || c.attr->name == "q";
+    bool const bus_sums = node_pq || want[C_SOURCE] || want[C_SHUNT];
     auto column_set = [&](int c) {
         ColumnSet cs;
         for (auto const& col : cols[c]) {
@@ -1632,13 +1635,12 @@ void Batch::run() {
             cs.dst[cs.n] = static_cast<char*>(col.dev);
             ++cs.n;
         }
+        cs.bus_sums = bus_sums ? 1 : 0;
         return cs;
     };
     // source and shunt flows and the appliance sum of every bus: needed by source and shunt records and
     // by the node's p and q (load_gen records form their own power from the injection and the voltage)
-    bool node_pq = want[C_NODE] && cols[C_NODE].empty();
-    for (auto const& c : cols[C_NODE]) node_pq = node_pq || c.attr->name == "p" || c.attr->name == "q";
-    if (node_pq || want[C_SOURCE] || want[C_SHUNT]) launch_bus_results(P, d_outputs, d_bus_off, G, bus_total, st);
+    if (bus_sums) launch_bus_results(P, d_outputs, d_bus_off, G, bus_total, st);
     if (want[C_NODE])
         launch_node_records(P, d_outputs, d_node_recs, d_rec_off[C_NODE], G, rec_total[C_NODE], out[C_NODE],
                             column_set(C_NODE), st);

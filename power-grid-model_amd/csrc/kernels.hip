@@ -1832,7 +1832,8 @@ __global__ void bus_results_kernel(DevOutput const* __restrict__ outs, long long
 // ---------------------------------------------------------------------------------------------------
 // one record for component i of lane s (the lane is valid and converged)
 template <int P>
-__device__ __forceinline__ void node_record(DevOutput const& o, NodeRec const& r, int i, int s, NodeOutput<P>& rec) {
+__device__ __forceinline__ void node_record(DevOutput const& o, NodeRec const& r, int i, int s, NodeOutput<P>& rec,
+                                            bool bus_sums) {
     size_t const sp = o.S_pad;
     rec.id = ld(r.id, i);
     int const bus = ld(r.bus, i);
@@ -1847,8 +1848,10 @@ __device__ __forceinline__ void node_record(DevOutput const& o, NodeRec const& r
             rec.u_pu[ph] = sqrt(ur * ur + ui * ui);
             rec.u[ph] = u_scale * ld(r.u_rated, i) * rec.u_pu[ph];
             rec.u_angle[ph] = atan2(ui, ur);
-            rec.p[ph] = base_power * o.bus_inj[((size_t(bus) * P + ph) * 2) * sp + s];
-            rec.q[ph] = base_power * o.bus_inj[((size_t(bus) * P + ph) * 2 + 1) * sp + s];
+            if (bus_sums) {
+                rec.p[ph] = base_power * o.bus_inj[((size_t(bus) * P + ph) * 2) * sp + s];
+                rec.q[ph] = base_power * o.bus_inj[((size_t(bus) * P + ph) * 2 + 1) * sp + s];
+            }
         }
     }
 }
@@ -1950,16 +1953,18 @@ __device__ __forceinline__ void appliance_record(DevOutput const& o, ApplRec con
 }
 
 template <int P>
-__device__ __forceinline__ void make_record(DevOutput const& o, NodeRec const& r, int i, int s, NodeOutput<P>& rec) {
-    node_record<P>(o, r, i, s, rec);
+__device__ __forceinline__ void make_record(DevOutput const& o, NodeRec const& r, int i, int s, NodeOutput<P>& rec,
+                                            bool bus_sums) {
+    node_record<P>(o, r, i, s, rec, bus_sums);
 }
 template <int P>
-__device__ __forceinline__ void make_record(DevOutput const& o, BranchRec const& r, int i, int s, BranchOutput<P>& rec) {
+__device__ __forceinline__ void make_record(DevOutput const& o, BranchRec const& r, int i, int s, BranchOutput<P>& rec,
+                                            bool) {
     branch_record<P>(o, r, i, s, rec);
 }
 template <int P>
 __device__ __forceinline__ void make_record(DevOutput const& o, ApplRec const& r, int i, int s,
-                                            ApplianceOutput<P>& rec) {
+                                            ApplianceOutput<P>& rec, bool) {
     appliance_record<P>(o, r, i, s, rec);
 }
 
@@ -1995,7 +2000,7 @@ __global__ __launch_bounds__(256) void records_kernel(DevOutput const* __restric
             uint4* z = reinterpret_cast<uint4*>(&slot);
 #pragma unroll
             for (int k = 0; k < CH; ++k) z[k] = uint4{0u, 0u, 0u, 0u};
-            make_record<P>(o, r, i, s, slot);
+            make_record<P>(o, r, i, s, slot, cols.bus_sums != 0);
         }
     };
     if (W == kWaveLanes) {

@@ -310,6 +310,7 @@ void launch_load_scatter(double* s_inj, int8_t* lstat, int S, int S_pad, int con
 constexpr int kMaxColumns = 16;
 struct ColumnSet {
     int n = 0;
+    int bus_sums = 1;  // 0: the bus sums (node p, q) were not formed in this run; leave them out
     int off[kMaxColumns] = {};
     int size[kMaxColumns] = {};
     char* dst[kMaxColumns] = {};
