@@ -311,6 +311,13 @@ PGM_AMD_API int PGM_amd_block_lu_solve(PGM_Idx nb, PGM_Idx n, PGM_Idx const* row
  * WRITE_SIZE counters. Allocates and frees its own buffers. Returns 0, or -1 on a HIP error. */
 PGM_AMD_API int PGM_amd_copy_probe(PGM_Idx n_floats) PGM_AMD_NOEXCEPT;
 
+/* Test hook of the automatic tap changer (PGM_calculate with a tap changing strategy,
+ * optimizer/tap_position_optimizer.hpp:209-432 rank_transformers): the rank of every transformer_tap_regulator
+ * of the model in its input state, in input order; -1 for regulators that are off or cannot be reached
+ * from a source. Host work only (no device needed). ranks must hold one entry per regulator. */
+PGM_AMD_API void PGM_amd_tap_regulator_ranks(PGM_Handle* handle, PGM_PowerGridModel const* model,
+                                             PGM_Idx* ranks) PGM_AMD_NOEXCEPT;
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,8 +56,11 @@ template <int P> std::vector<PGM_MetaComponent> output_components() {
                            mk("p", &A::p, v), mk("q", &A::q, v), mk("i", &A::i, v), mk("s", &A::s, v),
                            mk("pf", &A::pf, v)});
     };
+    using R = TapRegulatorOutput;
+    auto reg = comp<R>(kTapRegulatorName, {mk("id", &R::id, PGM_int32), mk("energized", &R::energized, PGM_int8),
+                                           mk("tap_pos", &R::tap_pos, PGM_int8)});
     return {node,           branch("line"),  branch("transformer"), appl("shunt"),     appl("source"),
-            appl("sym_gen"), appl("asym_gen"), appl("sym_load"),     appl("asym_load")};
+            appl("sym_gen"), appl("asym_gen"), appl("sym_load"),     appl("asym_load"), reg};
 }
 
 std::vector<PGM_MetaDataset> build_meta() {
@@ -69,6 +72,7 @@ std::vector<PGM_MetaDataset> build_meta() {
     using SrcI = SourceInput;
     using LG = SymLoadGenInput;
     using ALG = AsymLoadGenInput;
+    using RI = TapRegulatorInput;
     auto lg_in = [&](char const* n) {
         return comp<LG>(n, {mk("id", &LG::id, PGM_int32), mk("node", &LG::node, PGM_int32),
                             mk("status", &LG::status, PGM_int8), mk("type", &LG::type, PGM_int8),
@@ -132,13 +136,20 @@ std::vector<PGM_MetaDataset> build_meta() {
                                mk("u_ref_angle", &SrcI::u_ref_angle, PGM_double), mk("sk", &SrcI::sk, PGM_double),
                                mk("rx_ratio", &SrcI::rx_ratio, PGM_double),
                                mk("z01_ratio", &SrcI::z01_ratio, PGM_double)}),
-         lg_in("sym_gen"), alg_in("asym_gen"), lg_in("sym_load"), alg_in("asym_load")}});
+         lg_in("sym_gen"), alg_in("asym_gen"), lg_in("sym_load"), alg_in("asym_load"),
+         comp<RI>(kTapRegulatorName,
+                  {mk("id", &RI::id, PGM_int32), mk("regulated_object", &RI::regulated_object, PGM_int32),
+                   mk("status", &RI::status, PGM_int8), mk("control_side", &RI::control_side, PGM_int8),
+                   mk("u_set", &RI::u_set, PGM_double), mk("u_band", &RI::u_band, PGM_double),
+                   mk("line_drop_compensation_r", &RI::line_drop_compensation_r, PGM_double),
+                   mk("line_drop_compensation_x", &RI::line_drop_compensation_x, PGM_double)})}});
     using BU = BranchUpdate;
     using TU = TransformerUpdate;
     using SU = ShuntUpdate;
     using SrcU = SourceUpdate;
     using LGU = SymLoadGenUpdate;
     using ALGU = AsymLoadGenUpdate;
+    using RU = TapRegulatorUpdate;
     auto lg_up = [&](char const* n) {
         return comp<LGU>(n, {mk("id", &LGU::id, PGM_int32), mk("status", &LGU::status, PGM_int8),
                              mk("p_specified", &LGU::p_specified, PGM_double),
@@ -163,7 +174,12 @@ std::vector<PGM_MetaDataset> build_meta() {
                                mk("u_ref_angle", &SrcU::u_ref_angle, PGM_double), mk("sk", &SrcU::sk, PGM_double),
                                mk("rx_ratio", &SrcU::rx_ratio, PGM_double),
                                mk("z01_ratio", &SrcU::z01_ratio, PGM_double)}),
-         lg_up("sym_gen"), alg_up("asym_gen"), lg_up("sym_load"), alg_up("asym_load")}});
+         lg_up("sym_gen"), alg_up("asym_gen"), lg_up("sym_load"), alg_up("asym_load"),
+         comp<RU>(kTapRegulatorName,
+                  {mk("id", &RU::id, PGM_int32), mk("status", &RU::status, PGM_int8),
+                   mk("u_set", &RU::u_set, PGM_double), mk("u_band", &RU::u_band, PGM_double),
+                   mk("line_drop_compensation_r", &RU::line_drop_compensation_r, PGM_double),
+                   mk("line_drop_compensation_x", &RU::line_drop_compensation_x, PGM_double)})}});
     ds.push_back(PGM_MetaDataset{"sym_output", output_components<1>()});
     ds.push_back(PGM_MetaDataset{"asym_output", output_components<3>()});
     return ds;

@@ -1,6 +1,7 @@
 // capi_model.cpp -- model and calculation entry points of the C ABI (model.h of the reference,
 // power_grid_model_c/power_grid_model_c/src/model.cpp:48-359), plus the PGM_amd_* extensions.
 #include "engine.hpp"
+#include "tap_changer.hpp"
 
 #include "../../include/power_grid_model_amd.h"
 
@@ -59,6 +60,16 @@ void PGM_update_model(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_ConstDa
             for (int c = 0; c < N_COMP; ++c)
                 if (buf.meta->name == comp_names[c]) comp = c;
             rec.resize(buf.meta->size);
+            if (buf.meta->name == kTapRegulatorName) {
+                for (Idx i = buf.begin(0); i < buf.end(0); ++i) {
+                    buf.read_record(i, rec.data());
+                    TapRegulatorUpdate u;
+                    std::memcpy(&u, rec.data(), sizeof u);
+                    Idx const idx = u.id == na_IntID ? i - buf.begin(0) : find_component(m, C_TAP_REGULATOR, u.id);
+                    apply_regulator_update(st, idx, u);
+                }
+                continue;
+            }
             for (Idx i = buf.begin(0); i < buf.end(0); ++i) {
                 buf.read_record(i, rec.data());
                 ID id;
@@ -96,6 +107,12 @@ void PGM_calculate(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_Options co
             throw PowerGridError(
                 "If batch_dataset is provided. Both batch_dataset and output_dataset should be a batch!\n");
         if (batch_dataset && batch_dataset->batch_size == 0) return;
+        // automatic tap changing (main_model_impl.hpp:318-346, optimizer/tap_position_optimizer.hpp)
+        if (opt->tap_changing_strategy != 0) {
+            calculate_with_tap_changer(model->model, *opt, *output_dataset, batch_dataset, handle);
+            return;
+        }
+        write_null_regulator_output(model->model, *output_dataset);
         auto batch = create_batch(model->model, *opt, batch_dataset);
         // only the requested components (and, for columnar buffers, attributes) are produced
         batch->set_output(*output_dataset);
@@ -105,6 +122,13 @@ void PGM_calculate(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_Options co
 }
 
 void PGM_destroy_model(PGM_PowerGridModel* model) noexcept { delete model; }
+
+void PGM_amd_tap_regulator_ranks(PGM_Handle* handle, PGM_PowerGridModel const* model, PGM_Idx* ranks) noexcept {
+    guarded(handle, [&] {
+        auto const r = tap_regulator_ranks(model->model, model->model.base);
+        std::copy(r.begin(), r.end(), ranks);
+    });
+}
 
 PGM_AmdBatch* PGM_amd_batch_create(PGM_Handle* handle, PGM_PowerGridModel const* model, PGM_Options const* opt,
                                    PGM_ConstDataset const* batch_dataset) noexcept {

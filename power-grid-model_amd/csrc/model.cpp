@@ -152,7 +152,7 @@ Model build_model(PGM_ConstDataset const& in, double f) {
         }
     };
     for (auto const& b : in.buffers) {
-        bool known = false;
+        bool known = b.meta->name == kTapRegulatorName;
         for (int c = 0; c < N_COMP; ++c) known |= b.meta->name == comp_names[c];
         if (!known) throw PowerGridError("component " + b.meta->name + " is not supported by this build");
     }
@@ -323,7 +323,49 @@ Model build_model(PGM_ConstDataset const& in, double f) {
         });
         m.lg_count[k] = cnt;
     }
+    // transformer tap regulators (main_core/input.hpp:168-213, 235-241): the regulated object is a
+    // transformer, the control side one of its two sides, each transformer regulated at most once
+    if (auto const* buf = in.find(kTapRegulatorName)) {
+        std::vector<char> regulated(m.trafo.size(), 0);
+        for (Idx i = 0; i < buf->total_elements; ++i) {
+            buf->read_record(i, rec.data());
+            TapRegulatorInput r;
+            std::memcpy(&r, rec.data(), sizeof r);
+            auto it = m.ids.find(r.regulated_object);
+            if (it == m.ids.end())
+                throw PowerGridError("The id cannot be found: " + std::to_string(r.regulated_object) + "\n");
+            if (it->second.first != C_TRANSFORMER)
+                throw PowerGridError(std::string(kTapRegulatorName) +
+                                     " regulator is not supported for object with ID " +
+                                     std::to_string(r.regulated_object));
+            if (r.control_side != 0 && r.control_side != 1)
+                throw PowerGridError("transformer_tap_regulator item retrieval is not implemented for " +
+                                     std::to_string(int(r.control_side)) + "!\n");
+            Idx const t = it->second.second;
+            reg(r.id, C_TAP_REGULATOR, Idx(m.regulator.size()));
+            if (regulated[t])
+                throw PowerGridError(
+                    "There are objects regulated by more than one regulator. Maximum one regulator is allowed.");
+            regulated[t] = 1;
+            m.regulator.push_back(r);
+            m.reg_trafo.push_back(t);
+            m.reg_u_rated.push_back(m.node[r.control_side == 0 ? m.tr_f[t] : m.tr_t[t]].u_rated);
+            m.base.reg_status.push_back(r.status != 0);
+            m.base.reg_u_set.push_back(r.u_set);
+            m.base.reg_u_band.push_back(r.u_band);
+            m.base.reg_ldc_r.push_back(r.line_drop_compensation_r);
+            m.base.reg_ldc_x.push_back(r.line_drop_compensation_x);
+        }
+    }
     return m;
+}
+
+void apply_regulator_update(ComponentState& st, Idx i, TapRegulatorUpdate const& u) {
+    if (u.status != na_IntS) st.reg_status[i] = u.status != 0;
+    if (!std::isnan(u.u_set)) st.reg_u_set[i] = u.u_set;
+    if (!std::isnan(u.u_band)) st.reg_u_band[i] = u.u_band;
+    if (!std::isnan(u.line_drop_compensation_r)) st.reg_ldc_r[i] = u.line_drop_compensation_r;
+    if (!std::isnan(u.line_drop_compensation_x)) st.reg_ldc_x[i] = u.line_drop_compensation_x;
 }
 
 BranchY branch_param(Model const& m, ComponentState const& st, Idx b, bool symmetric) {

@@ -58,6 +58,9 @@ struct ComponentState {
     // load_gens in container order: sym_gen, asym_gen, sym_load, asym_load
     std::vector<IntS> lg_status;
     std::vector<std::array<double, 3>> lg_p, lg_q;
+    // transformer tap regulators
+    std::vector<IntS> reg_status;
+    std::vector<double> reg_u_set, reg_u_band, reg_ldc_r, reg_ldc_x;
 };
 
 struct Model {
@@ -78,6 +81,10 @@ struct Model {
     std::vector<Idx> line_f, line_t, tr_f, tr_t, sh_node, src_node;
     std::vector<LineStatic> lstat;
     std::vector<TrafoStatic> tstat;
+    // transformer tap regulators: the regulated transformer and the rated voltage of its control-side node
+    std::vector<TapRegulatorInput> regulator;
+    std::vector<Idx> reg_trafo;
+    std::vector<double> reg_u_rated;
     ComponentState base;
 
     Idx n_node() const { return Idx(node.size()); }
@@ -101,5 +108,8 @@ double branch_phase_shift(Model const& m, Idx branch);
 
 // lookup of an update row: component index, or throws IDNotFound / IDWrongType
 Idx find_component(Model const& m, int comp, ID id);
+// apply one transformer_tap_regulator update record (transformer_tap_regulator.hpp:38-46: NaN / na keep
+// the current value)
+void apply_regulator_update(ComponentState& st, Idx i, TapRegulatorUpdate const& u);
 
 }  // namespace pgm_amd

@@ -56,6 +56,11 @@ struct AsymLoadGenInput {
     IntS status, type;
     double p_specified[3], q_specified[3];
 };
+struct TapRegulatorInput {
+    ID id, regulated_object;
+    IntS status, control_side;
+    double u_set, u_band, line_drop_compensation_r, line_drop_compensation_x;
+};
 
 // ---------- update ----------
 struct BranchUpdate {
@@ -86,6 +91,11 @@ struct AsymLoadGenUpdate {
     IntS status;
     double p_specified[3], q_specified[3];
 };
+struct TapRegulatorUpdate {
+    ID id;
+    IntS status;
+    double u_set, u_band, line_drop_compensation_r, line_drop_compensation_x;
+};
 
 // ---------- output ----------
 template <int P> struct NodeOutput {
@@ -104,6 +114,11 @@ template <int P> struct ApplianceOutput {
     IntS energized;
     double p[P], q[P], i[P], s[P], pf[P];
 };
+// the same record for symmetric and asymmetric output
+struct TapRegulatorOutput {
+    ID id;
+    IntS energized, tap_pos;
+};
 
 static_assert(sizeof(NodeInput) == 16);
 static_assert(sizeof(LineInput) == 88);
@@ -116,6 +131,7 @@ static_assert(sizeof(AsymLoadGenUpdate) == 56);
 static_assert(sizeof(NodeOutput<1>) == 48 && sizeof(NodeOutput<3>) == 128);
 static_assert(sizeof(BranchOutput<1>) == 80 && sizeof(BranchOutput<3>) == 208);
 static_assert(sizeof(ApplianceOutput<1>) == 48 && sizeof(ApplianceOutput<3>) == 128);
+static_assert(sizeof(TapRegulatorInput) == 48 && sizeof(TapRegulatorUpdate) == 40 && sizeof(TapRegulatorOutput) == 8);
 
 // component slots, in the reference container order (all_components.hpp:37-40)
 enum Comp : int {
@@ -132,5 +148,8 @@ enum Comp : int {
 };
 inline char const* const comp_names[N_COMP] = {"node",   "line",    "transformer", "shunt",    "source",
                                                "sym_gen", "asym_gen", "sym_load",    "asym_load"};
+// transformer_tap_regulator: not part of a math model (no slot above); it steers the tap changer
+inline char const* const kTapRegulatorName = "transformer_tap_regulator";
+constexpr int C_TAP_REGULATOR = N_COMP;  // slot of its ids in the id index
 
 }  // namespace pgm_amd

@@ -6,6 +6,7 @@ same PowerGridModel.calculate_power_flow(update_data=...) call, computed by hand
 
 from ._lib import LIB_PATH, lib
 from .errors import (
+    AutomaticTapInputError,
     ConflictID,
     ConflictVoltage,
     DatasetError,
@@ -14,6 +15,7 @@ from .errors import (
     InvalidCalculationMethod,
     InvalidTransformerClock,
     IterationDiverge,
+    MaxIterationReached,
     PowerGridBatchError,
     PowerGridError,
     SparseMatrixError,
@@ -28,11 +30,13 @@ from .model import (
     FactorPrecision,
     LoadGenType,
     PowerGridModel,
+    TapChangingStrategy,
     WindingType,
     block_lu_solve,
     inspect_topology,
     minimum_degree_ordering,
     plan_stats,
+    tap_regulator_ranks,
 )
 
 __all__ = [
@@ -41,4 +45,5 @@ __all__ = [
     "PowerGridError", "SparseMatrixError", "initialize_array", "power_grid_meta_data", "BranchSide",
     "CalculationMethod", "CalculationType", "ComponentAttributeFilterOptions", "DeviceBatch", "FactorPrecision", "LoadGenType", "PowerGridModel",
     "WindingType", "block_lu_solve", "inspect_topology", "minimum_degree_ordering", "plan_stats",
+    "AutomaticTapInputError", "MaxIterationReached", "TapChangingStrategy", "tap_regulator_ranks",
 ]

@@ -57,7 +57,27 @@ class DatasetError(PowerGridError):
     """Malformed dataset."""
 
 
+class MaxIterationReached(IterationDiverge):
+    """The tap changer did not settle within its iteration limit."""
+
+
+class AutomaticTapInputError(PowerGridError):
+    """A regulated transformer is controlled from its non-source side towards the source side."""
+
+
+class InvalidRegulatedObject(PowerGridError):
+    """A transformer_tap_regulator regulates an object that is not a transformer."""
+
+
+class DuplicativelyRegulatedObject(PowerGridError):
+    """An object has more than one regulator."""
+
+
 _PATTERNS = [
+    (r"Maximum number of iterations reached!", MaxIterationReached),
+    (r"Automatic tap changer has invalid configuration", AutomaticTapInputError),
+    (r"regulator is not supported for object", InvalidRegulatedObject),
+    (r"regulated by more than one regulator", DuplicativelyRegulatedObject),
     (r"Iteration failed to converge", IterationDiverge),
     (r"Sparse matrix error", SparseMatrixError),
     (r"The id cannot be found:", IDNotFound),

@@ -241,7 +241,7 @@ class PowerGridModel:
         return out
 
     def _options(self, symmetric, error_tolerance, max_iterations, calculation_method, threading,
-                 factor_precision):
+                 factor_precision, tap_changing_strategy=0):
         h = self._handle
         opt = lib.PGM_create_options(h.ptr)
         lib.PGM_set_calculation_type(h.ptr, opt, CalculationType.power_flow)
@@ -251,6 +251,7 @@ class PowerGridModel:
         lib.PGM_set_max_iter(h.ptr, opt, int(max_iterations))
         lib.PGM_set_threading(h.ptr, opt, int(threading))
         lib.PGM_amd_set_factor_precision(h.ptr, opt, _enum_value(factor_precision, FactorPrecision))
+        lib.PGM_set_tap_changing_strategy(h.ptr, opt, _enum_value(tap_changing_strategy, TapChangingStrategy))
         h.check()
         return _Dataset(opt, lib.PGM_destroy_options, [])
 
@@ -290,13 +291,15 @@ class PowerGridModel:
         tap_changing_strategy=0,
         factor_precision: FactorPrecision | str = FactorPrecision.fp64,
     ) -> dict:
-        """Newton-Raphson power flow of the model, or of every scenario of update_data (a batch)."""
-        if _enum_value(tap_changing_strategy, _TapChangingStrategy) != 0:
-            raise_regular_error("automatic tap changing is not supported by this build")
+        """Newton-Raphson power flow of the model, or of every scenario of update_data (a batch).
+
+        tap_changing_strategy other than disabled adjusts the tap position of every transformer with an
+        active transformer_tap_regulator until the regulated voltages lie in their bands
+        (power_grid_model.py:564-670, optimizer/tap_position_optimizer.hpp)."""
         h = self._handle
         self.batch_error = None
         opt = self._options(symmetric, error_tolerance, max_iterations, calculation_method, threading,
-                            factor_precision)
+                            factor_precision, tap_changing_strategy)
         is_batch = update_data is not None
         upd = None
         batch_size = 1
@@ -333,7 +336,7 @@ class PowerGridModel:
         raise err
 
 
-class _TapChangingStrategy(IntEnum):
+class TapChangingStrategy(IntEnum):
     disabled = 0
     any_valid_tap = 1
     min_voltage_tap = 2
@@ -435,6 +438,17 @@ class DeviceBatch:
         keys = ["kernel_ms", "iterations_sum", "buses", "lu_entries", "groups", "fwd_launch_ms", "fwd_launches",
                 "buses_per_scenario", "fwd_bytes", "fwd_total_ms", "fwd_timed"]
         return dict(zip(keys, out.tolist()))
+
+
+def tap_regulator_ranks(model: PowerGridModel) -> np.ndarray:
+    """Host-only ranking of the automatic tap changer: per transformer_tap_regulator (input order) its rank,
+    0 next to the sources, -1 when it is off or out of reach (tap_position_optimizer.hpp:209-432)."""
+    n = model.all_component_count.get("transformer_tap_regulator", 0)
+    ranks = np.zeros(max(n, 1), dtype=np.int64)
+    h = model._handle
+    lib.PGM_amd_tap_regulator_ranks(h.ptr, model._ptr, ranks.ctypes.data)
+    h.check()
+    return ranks[:n]
 
 
 def inspect_topology(model: PowerGridModel, n_node: int):

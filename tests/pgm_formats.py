@@ -63,6 +63,10 @@ INPUT = {
     "asym_gen": _struct(_APPL_IN + [("type", "i1"), ("p_specified", _D3), ("q_specified", _D3)]),
     "sym_load": _struct(_APPL_IN + [("type", "i1"), ("p_specified", _D), ("q_specified", _D)]),
     "asym_load": _struct(_APPL_IN + [("type", "i1"), ("p_specified", _D3), ("q_specified", _D3)]),
+    "transformer_tap_regulator": _struct(
+        _BASE + [("regulated_object", "i4"), ("status", "i1"), ("control_side", "i1")]
+        + [(n, _D) for n in ("u_set", "u_band", "line_drop_compensation_r", "line_drop_compensation_x")]
+    ),
 }
 
 UPDATE = {
@@ -75,6 +79,10 @@ UPDATE = {
     "asym_gen": _struct(_BASE + [("status", "i1"), ("p_specified", _D3), ("q_specified", _D3)]),
     "sym_load": _struct(_BASE + [("status", "i1"), ("p_specified", _D), ("q_specified", _D)]),
     "asym_load": _struct(_BASE + [("status", "i1"), ("p_specified", _D3), ("q_specified", _D3)]),
+    "transformer_tap_regulator": _struct(
+        _BASE + [("status", "i1")]
+        + [(n, _D) for n in ("u_set", "u_band", "line_drop_compensation_r", "line_drop_compensation_x")]
+    ),
 }
 
 
@@ -94,6 +102,7 @@ def _output(v):
         "asym_gen": appl,
         "sym_load": appl,
         "asym_load": appl,
+        "transformer_tap_regulator": _struct(base + [("tap_pos", "i1")]),
     }
 
 
@@ -104,6 +113,8 @@ DATASETS = {"input": INPUT, "update": UPDATE, "sym_output": SYM_OUTPUT, "asym_ou
 
 # component order of the reference's container (all_components.hpp:37-40)
 COMPONENT_ORDER = ["node", "line", "transformer", "shunt", "source", "sym_gen", "asym_gen", "sym_load", "asym_load"]
+# with the components outside the math model (the tap changer's regulators), in the same container order
+ALL_COMPONENTS = COMPONENT_ORDER + ["transformer_tap_regulator"]
 
 
 def null_value(fmt: np.dtype):
@@ -155,7 +166,7 @@ def unsupported_components(path: Path) -> set[str]:
     comps = set()
     for sc in data if isinstance(data, list) else [data]:
         comps |= set(sc.keys())
-    return comps - set(COMPONENT_ORDER)
+    return comps - set(ALL_COMPONENTS)
 
 
 def batch_to_uniform(batch: list[dict], input_data: dict) -> dict:
@@ -164,7 +175,7 @@ def batch_to_uniform(batch: list[dict], input_data: dict) -> dict:
     A padding row carries the id of a component that scenario does not touch and null attributes, which
     the reference's update semantics treat as "keep the current value".
     """
-    comps = sorted({c for sc in batch for c in sc}, key=COMPONENT_ORDER.index)
+    comps = sorted({c for sc in batch for c in sc}, key=ALL_COMPONENTS.index)
     out = {}
     for comp in comps:
         dtype = UPDATE[comp]

@@ -84,7 +84,8 @@ CASES = list_cases()
 def test_validation_case_gpu(name, sym, batch):
     params, input_data, update, n_scenarios, expected = load_case(name, sym, batch)
     model = PowerGridModel(input_data)
-    out = model.calculate_power_flow(symmetric=sym, update_data=update)
+    out = model.calculate_power_flow(symmetric=sym, update_data=update,
+                                     tap_changing_strategy=params.get("tap_changing_strategy", "disabled"))
     if not batch:
         out = {k: v.reshape(1, *v.shape) for k, v in out.items()}
     for s, exp in enumerate(expected):
@@ -100,11 +101,32 @@ METHOD_CASES = [(m, *c) for m in ("linear", "iterative_current", "linear_current
 def test_validation_case_gpu_other_methods(method, name, sym, batch):
     params, input_data, update, n_scenarios, expected = load_case(name, sym, batch)
     model = PowerGridModel(input_data)
-    out = model.calculate_power_flow(symmetric=sym, update_data=update, calculation_method=method)
+    out = model.calculate_power_flow(symmetric=sym, update_data=update, calculation_method=method,
+                                     tap_changing_strategy=params.get("tap_changing_strategy", "disabled"))
     if not batch:
         out = {k: v.reshape(1, *v.shape) for k, v in out.items()}
     for s, exp in enumerate(expected):
         compare_result(out, exp, params["rtol"], params["atol"], s)
+
+
+RAISE_CASES = list_cases(raises=True)
+
+
+@pytest.mark.parametrize("name,sym,batch", RAISE_CASES, ids=[c[0] for c in RAISE_CASES])
+def test_validation_case_gpu_raises(name, sym, batch):
+    """Tap changer cases whose regulators cannot settle: the reference raises MaxIterationReached (params.json
+    "raises"); a batch reports it for every scenario."""
+    params, input_data, update, n_scenarios, _ = load_case(name, sym, batch)
+    model = PowerGridModel(input_data)
+    expected = getattr(pgm, params.get("raises", params.get("xfail"))["raises"])
+    kwargs = dict(symmetric=sym, update_data=update, tap_changing_strategy=params["tap_changing_strategy"])
+    if not batch:
+        with pytest.raises(expected):
+            model.calculate_power_flow(**kwargs)
+        return
+    with pytest.raises(pgm.PowerGridBatchError) as info:
+        model.calculate_power_flow(**kwargs)
+    assert all(isinstance(e, expected) for e in info.value.errors)
 
 
 # ------------------------------------------------------------------------------------------------

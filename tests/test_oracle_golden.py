@@ -320,7 +320,7 @@ def test_linear_method_closed_form(load_type):
 # ------------------------------------------------------------------------------------------------
 # reference validation cases
 # ------------------------------------------------------------------------------------------------
-CASES = list_cases()
+CASES = list_cases(tap=False)
 
 
 @pytest.mark.parametrize("name,sym,batch", CASES, ids=[f"{c[0]}-{'sym' if c[1] else 'asym'}"
@@ -333,7 +333,7 @@ def test_validation_case(name, sym, batch):
         compare_result(out, exp, params["rtol"], params["atol"], s)
 
 
-METHOD_CASES = [(m, *c) for m in ("linear", "iterative_current", "linear_current") for c in list_cases(m)]
+METHOD_CASES = [(m, *c) for m in ("linear", "iterative_current", "linear_current") for c in list_cases(m, tap=False)]
 
 
 @pytest.mark.parametrize("method,name,sym,batch", METHOD_CASES,

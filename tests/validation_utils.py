@@ -19,14 +19,26 @@ from pgm_formats import batch_to_uniform, load_dataset
 GOLDEN = Path(__file__).resolve().parent / "golden" / "power_flow"
 
 
-def list_cases(method: str = "newton_raphson"):
-    """(case, symmetric, batch) of every fixture whose params.json lists the calculation method."""
+def list_cases(method: str = "newton_raphson", tap: bool | None = None, raises: bool = False):
+    """(case, symmetric, batch) of every fixture whose params.json lists the calculation method.
+
+    tap: None for every case, False without / True only with a tap changing strategy. raises: the cases
+    that expect an error instead of the ones with outputs (params "raises", or "xfail" naming the error
+    the reference's own run raises)."""
     cases = []
     for params in sorted(GOLDEN.rglob("params.json")):
         case = params.parent
-        methods = json.load(open(params)).get("calculation_method")
+        p = json.load(open(params))
+        methods = p.get("calculation_method")
         methods = methods if isinstance(methods, list) else [methods]
         if method not in methods:
+            continue
+        if tap is not None and tap != ("tap_changing_strategy" in p):
+            continue
+        if raises != ("raises" in p or "xfail" in p):
+            continue
+        if raises:
+            cases.append((str(case.relative_to(GOLDEN)), True, (case / "update_batch.json").exists()))
             continue
         for sym, stem in ((True, "sym_output"), (False, "asym_output")):
             for batch in (False, True):
@@ -41,7 +53,7 @@ def load_case(name: str, sym: bool, batch: bool):
     params = json.load(open(case / "params.json"))
     input_data = load_dataset(case / "input.json")
     stem = ("sym_output" if sym else "asym_output") + ("_batch" if batch else "")
-    expected = load_dataset(case / (stem + ".json"))
+    expected = load_dataset(case / (stem + ".json")) if (case / (stem + ".json")).exists() else None
     update = None
     n_scenarios = 1
     if batch:
