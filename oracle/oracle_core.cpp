@@ -15,6 +15,7 @@
 //   outputs               component/{node,branch,appliance}.hpp get_output, main_core/output.hpp,
 //                         main_core/topological_node_output.hpp:69-117
 //   batch semantics       job_dispatch.hpp:35-172 (threads stride scenarios; per-scenario errors)
+//   tap changer           optimizer/tap_position_optimizer.hpp:209-1110, transformer_tap_regulator.hpp
 //   other methods         math_solver.hpp:43-64 and 142-156, linear_pf_solver.hpp,
 //                         iterative_current_pf_solver.hpp (complex systems solved in their real
 //                         [[G, -B], [B, G]] form: equal to the reference's up to rounding)
@@ -2058,6 +2059,366 @@ void write_msg(char* messages, Idx stride, Idx scn, std::string const& s) {
     dst[n] = 0;
 }
 
+// ------------------------------------------------------------------------------------------------
+// automatic tap changing (optimizer/tap_position_optimizer.hpp, component/transformer_tap_regulator.hpp,
+// main_model_impl.hpp:318-346), restated per scenario in the reference's own sequence: rank the
+// regulated transformers, pilot run, iterate with fallback, exploit the neighbourhood
+// ------------------------------------------------------------------------------------------------
+struct MaxIterationReached : IterationDiverge {
+    explicit MaxIterationReached(std::string const& what)
+        : IterationDiverge("Maximum number of iterations reached! " + what + "\n") {}
+};
+
+struct RegulatorState {
+    std::vector<int8_t> status;
+    std::vector<double> u_set, u_band, ldc_r, ldc_x;
+};
+
+constexpr Idx kInfDist = std::numeric_limits<Idx>::max();
+
+// tap_position_optimizer.hpp:209-432. Returns the regulator indices rank by rank.
+std::vector<std::vector<Idx>> rank_regulators(Model const& m, State const& st,
+                                              std::vector<OrTapRegulatorInput> const& regs,
+                                              RegulatorState const& rs) {
+    struct Edge {
+        Idx from, to, weight, reg;
+    };
+    std::vector<Edge> edges;
+    // add_edge<Transformer> (:209-234), add_edge<Line> (:236-252)
+    for (Idx t = 0; t < Idx(m.trafo.size()); ++t) {
+        if (!st.tr_fs[t] || !st.tr_ts[t]) continue;
+        Idx reg = -1;
+        for (Idx r = 0; r < Idx(regs.size()); ++r)
+            if (rs.status[r] && regs[r].regulated_object == m.trafo[t].id) reg = r;
+        if (reg >= 0) {
+            Idx const ctrl = regs[reg].control_side == 0 ? m.tr_f[t] : m.tr_t[t];
+            Idx const other = regs[reg].control_side == 0 ? m.tr_t[t] : m.tr_f[t];
+            edges.push_back({other, ctrl, 1, reg});
+        } else {
+            edges.push_back({m.tr_f[t], m.tr_t[t], 0, -1});
+            edges.push_back({m.tr_t[t], m.tr_f[t], 0, -1});
+        }
+    }
+    for (Idx l = 0; l < Idx(m.line.size()); ++l) {
+        if (!st.line_fs[l] || !st.line_ts[l]) continue;
+        edges.push_back({m.line_f[l], m.line_t[l], 0, -1});
+        edges.push_back({m.line_t[l], m.line_f[l], 0, -1});
+    }
+    // compressed sparse row graph: out-edges of a vertex in the order they were added
+    std::stable_sort(edges.begin(), edges.end(), [](Edge const& a, Edge const& b) { return a.from < b.from; });
+    Idx const n = Idx(m.node.size());
+    std::vector<Idx> first(size_t(n) + 1, 0);
+    for (Edge const& e : edges) ++first[size_t(e.from) + 1];
+    for (Idx v = 0; v < n; ++v) first[size_t(v) + 1] += first[size_t(v)];
+    std::vector<char> is_source(size_t(n), 0);
+    for (Idx s = 0; s < Idx(m.source.size()); ++s) is_source[size_t(m.src_node[s])] = st.src_status[s] != 0;
+    // Dijkstra from every source vertex in turn over one distance vector (process_edges_dijkstra, :308-334)
+    std::vector<Idx> dist(size_t(n), kInfDist);
+    for (Idx v = 0; v < n; ++v) {
+        if (!is_source[size_t(v)]) continue;
+        using Item = std::pair<Idx, Idx>;
+        std::vector<Item> heap;
+        auto cmp = [](Item const& a, Item const& b) { return a > b; };
+        dist[size_t(v)] = 0;
+        heap.push_back({0, v});
+        while (!heap.empty()) {
+            std::pop_heap(heap.begin(), heap.end(), cmp);
+            auto const [d, u] = heap.back();
+            heap.pop_back();
+            if (d != dist[size_t(u)]) continue;
+            for (Idx k = first[size_t(u)]; k < first[size_t(u) + 1]; ++k) {
+                Edge const& e = edges[size_t(k)];
+                if (dist[size_t(e.from)] + e.weight < dist[size_t(e.to)]) {
+                    dist[size_t(e.to)] = dist[size_t(e.from)] + e.weight;
+                    heap.push_back({dist[size_t(e.to)], e.to});
+                    std::push_heap(heap.begin(), heap.end(), cmp);
+                }
+            }
+        }
+    }
+    // get_edge_weights (:338-407)
+    std::vector<std::pair<Idx, Idx>> weighted;  // (weight, regulator)
+    std::vector<int32_t> invalid;
+    for (Edge const& e : edges) {
+        if (e.reg < 0) continue;
+        Idx const a = dist[size_t(e.from)], b = dist[size_t(e.to)];
+        if (a == kInfDist && b == kInfDist) continue;
+        if (a == kInfDist || b == kInfDist)
+            invalid.push_back(regs[size_t(e.reg)].regulated_object);
+        else if (a != b - 1)
+            weighted.push_back({kInfDist - 1, e.reg});
+        else
+            weighted.push_back({b, e.reg});
+    }
+    if (!invalid.empty()) {
+        std::sort(invalid.begin(), invalid.end());
+        invalid.erase(std::unique(invalid.begin(), invalid.end()), invalid.end());
+        std::string msg = "Automatic tap changer has invalid configuration. The following transformer(s) are being "
+                          "controlled from non-source side towards source side:\n  Transformer IDs: ";
+        for (size_t i = 0; i < invalid.size(); ++i) {
+            if (i) msg += ", ";
+            msg += std::to_string(invalid[i]);
+        }
+        throw OtherError(msg);
+    }
+    // rank_transformers (:409-428)
+    std::stable_sort(weighted.begin(), weighted.end(),
+                     [](std::pair<Idx, Idx> const& a, std::pair<Idx, Idx> const& b) { return a.first < b.first; });
+    std::vector<std::vector<Idx>> groups;
+    Idx previous = std::numeric_limits<Idx>::lowest();
+    for (auto const& [w, r] : weighted) {
+        if (w > previous) {
+            groups.emplace_back();
+            previous = w;
+        }
+        if (std::find(groups.back().begin(), groups.back().end(), r) == groups.back().end()) groups.back().push_back(r);
+    }
+    return groups;
+}
+
+template <int P> struct TapOptimizer {
+    // BinarySearch of the reference (:809-944)
+    struct Bisection {
+        int8_t lower = 0, upper = 0, current = 0;
+        bool last_down = false, last_check = false, tap_reverse = false, inevitable_run = false, ctrl_at_tap = false;
+
+        void reset(int8_t tap, int8_t tmin, int8_t tmax, bool cat) {
+            last_down = last_check = inevitable_run = false;
+            current = tap;
+            lower = std::min(tmin, tmax);
+            upper = std::max(tmin, tmax);
+            tap_reverse = tmax < tmin;
+            ctrl_at_tap = cat;
+        }
+        bool end_of_search() const { return lower >= upper; }
+        int8_t search(bool prefer_higher_in) const {
+            bool const prefer_higher = ctrl_at_tap != prefer_higher_in;
+            int const primary = prefer_higher ? upper : lower, secondary = prefer_higher ? lower : upper;
+            // std::midpoint for integers: rounded towards the first argument
+            return int8_t(primary <= secondary ? primary + (secondary - primary) / 2
+                                               : primary - (primary - secondary) / 2);
+        }
+        void adjust(bool strategy_max) {
+            if (last_down)
+                upper = current;
+            else
+                lower = current;
+            if (lower < upper) current = search(strategy_max != tap_reverse);
+        }
+        void propose_new_pos(bool strategy_max, bool above_range) {
+            bool const is_down = (above_range == tap_reverse) != ctrl_at_tap;
+            if (last_check) {
+                current = is_down ? lower : upper;
+                inevitable_run = true;
+            } else {
+                last_down = is_down;
+                adjust(strategy_max);
+            }
+        }
+        void recalibrate(bool strategy_max) {
+            if (tap_reverse == (ctrl_at_tap != strategy_max)) {
+                lower = current;
+                last_down = false;
+            } else {
+                upper = current;
+                last_down = true;
+            }
+        }
+        int8_t repropose_tap(bool strategy_max, bool previous_down, bool& tap_changed) {
+            int8_t const tap = search((strategy_max != tap_reverse) != ctrl_at_tap);
+            int const diff = int(tap) - int(current);
+            if (diff == 0) {
+                tap_changed = !inevitable_run;
+                inevitable_run = true;
+                return tap;
+            }
+            if ((diff == 1 && previous_down) || (diff == -1 && !previous_down)) last_check = true;
+            tap_changed = true;
+            current = tap;
+            return tap;
+        }
+    };
+    struct Regulated {
+        Idx reg, trafo, ctrl_node;
+        bool ctrl_from, ctrl_at_tap;
+        int8_t tmin, tmax;
+        double u_set, u_band;
+        cd z;
+    };
+
+    Model const& m;
+    State& st;
+    TopoCache const& tc;
+    double err_tol;
+    Idx max_iter;
+    int method;  // the requested method (0 Newton-Raphson, 2 iterative current, ...)
+    int strategy;  // 1 any, 2 min, 3 max, 4 fast any
+    std::vector<std::vector<Regulated>> order;
+    std::vector<std::vector<Bisection>> bs;
+    std::vector<uint64_t> max_range;
+
+    std::vector<MathOutput<P>> calculate(int meth) const {
+        std::vector<MathOutput<P>> res;
+        for (Idx g = 0; g < Idx(tc.topo.bus_nodes.size()); ++g) {
+            MathModel<P> const mm = make_math_model<P>(m, st, tc, g);
+            MathInput<P> in;
+            for (Idx l : tc.mm_load[g]) in.s_inj.push_back(load_s<P>(m, st, l));
+            for (Idx s : tc.mm_source[g]) in.u_source.push_back(st.src_uref[s] * std::exp(cd(0.0, st.src_uang[s])));
+            NRSolver<P> solver(mm);
+            res.push_back(solver.run(in, err_tol, max_iter, meth));
+        }
+        return res;
+    }
+    int8_t& tap(Regulated const& r) const { return st.tr_tap[size_t(r.trafo)]; }
+    void set_tap(Regulated const& r, int8_t t) const {
+        if (t != kNaS) st.tr_tap[size_t(r.trafo)] = tap_limit(m.tstatic[size_t(r.trafo)], t);
+    }
+    static int8_t tap_up(int8_t t, Regulated const& r) {
+        if (t == r.tmax) return r.tmax;
+        return int8_t(r.tmin < r.tmax ? t + 1 : t - 1);
+    }
+    static int8_t tap_down(int8_t t, Regulated const& r) {
+        if (t == r.tmin) return r.tmin;
+        return int8_t(r.tmin < r.tmax ? t - 1 : t + 1);
+    }
+    static int8_t voltage_up(int8_t t, Regulated const& r) { return r.ctrl_at_tap ? tap_up(t, r) : tap_down(t, r); }
+    static int8_t voltage_down(int8_t t, Regulated const& r) { return r.ctrl_at_tap ? tap_down(t, r) : tap_up(t, r); }
+
+    // node state against the band (NodeState <=> TransformerTapRegulatorCalcParam, :702-729):
+    // -1 below, 0 inside, 1 above, 2 unordered
+    int compare(Regulated const& r, std::vector<MathOutput<P>> const& res) const {
+        Idx const g = tc.topo.node_group[size_t(r.ctrl_node)];
+        Val<P> const u = res[size_t(g)].u[size_t(tc.topo.node_pos[size_t(r.ctrl_node)])];
+        Idx const b = Idx(m.line.size()) + r.trafo;
+        Val<P> const i = r.ctrl_from ? res[size_t(tc.br_group[size_t(b)])].br_if[size_t(tc.br_pos[size_t(b)])]
+                                     : res[size_t(tc.br_group[size_t(b)])].br_it[size_t(tc.br_pos[size_t(b)])];
+        double v = 0.0;
+        for (int ph = 0; ph < P; ++ph) v += std::abs(u.v[ph] + r.z * i.v[ph]);
+        v /= P;
+        double const lower = r.u_set - 0.5 * r.u_band, upper = r.u_set + 0.5 * r.u_band;
+        auto three_way = [](double a, double b) { return a < b ? -1 : a > b ? 1 : a == b ? 0 : 2; };
+        int const lc = three_way(v, lower), uc = three_way(v, upper);
+        return lc == uc ? lc : 0;
+    }
+    bool connected(Regulated const& r) const { return tc.topo.node_group[size_t(r.ctrl_node)] >= 0; }
+
+    bool adjust_scan(Regulated const& r, std::vector<MathOutput<P>> const& res,
+                     std::vector<std::pair<Regulated const*, int8_t>>& upd) const {
+        if (!connected(r)) return false;
+        int const c = compare(r, res);
+        int8_t const t = c == 1 ? voltage_down(tap(r), r) : c == -1 ? voltage_up(tap(r), r) : tap(r);
+        if (t == tap(r)) return false;
+        upd.push_back({&r, t});
+        return true;
+    }
+    bool adjust_bs(Regulated const& r, Bisection& b, std::vector<MathOutput<P>> const& res,
+                   std::vector<std::pair<Regulated const*, int8_t>>& upd) const {
+        bool const strategy_max = strategy == 3;
+        if (!connected(r)) return false;
+        if (b.end_of_search() || b.inevitable_run) return false;
+        int const c = compare(r, res);
+        if (c != 0) b.propose_new_pos(strategy_max, c == 1);
+        if (b.current != tap(r)) {
+            upd.push_back({&r, b.current});
+            return true;
+        }
+        if (strategy == 4 && c == 0) return false;
+        bool const previous_down = b.last_down;
+        b.recalibrate(strategy_max);
+        bool changed = false;
+        int8_t const t = b.repropose_tap(strategy_max, previous_down, changed);
+        if (t == tap(r) && c != 0 && !b.end_of_search()) {
+            b.reset(t, r.tmin, r.tmax, b.ctrl_at_tap);
+            throw MaxIterationReached("TapPositionOptimizer::binary_search: no valid tap position found between tap " +
+                                      std::to_string(int(r.tmin)) + " and tap " + std::to_string(int(r.tmax)));
+        }
+        upd.push_back({&r, t});
+        return changed;
+    }
+
+    // iterate (:1064-1110)
+    std::vector<MathOutput<P>> iterate(int meth, bool binary) {
+        auto result = calculate(meth);
+        std::vector<uint64_t> per_rank(order.size(), 0);
+        bool tap_changed = true;
+        while (tap_changed) {
+            tap_changed = false;
+            std::vector<std::pair<Regulated const*, int8_t>> upd;
+            Idx rank = 0;
+            for (Idx i = 0; i < Idx(order.size()); ++i) {
+                for (Idx j = 0; j < Idx(order[size_t(i)].size()); ++j) {
+                    Regulated const& r = order[size_t(i)][size_t(j)];
+                    bool const c = binary ? adjust_bs(r, bs[size_t(i)][size_t(j)], result, upd)
+                                          : adjust_scan(r, result, upd);
+                    tap_changed = c || tap_changed;
+                }
+                if (tap_changed) {
+                    for (size_t k = size_t(rank) + 1; k < per_rank.size(); ++k) per_rank[k] = 0;
+                    ++per_rank[size_t(rank)];
+                    break;
+                }
+                ++rank;
+            }
+            if (tap_changed) {
+                if (per_rank[size_t(rank)] > 2 * max_range[size_t(rank)])
+                    throw MaxIterationReached("TapPositionOptimizer::iterate " + std::to_string(per_rank[size_t(rank)]) +
+                                              " iterations reached: " + std::to_string(max_range[size_t(rank)]) +
+                                              "x2 iterations in rank " + std::to_string(rank));
+                for (auto const& [r, t] : upd) set_tap(*r, t);
+                result = calculate(meth);
+            }
+        }
+        return result;
+    }
+    // iterate_with_fallback (:1046-1062): a diverging or singular run retries after a linear-method pass
+    std::vector<MathOutput<P>> iterate_with_fallback(bool binary) {
+        try {
+            return iterate(method, binary);
+        } catch (IterationDiverge const&) {
+        } catch (SparseMatrixError const&) {
+        }
+        (void)iterate(1, binary);
+        return iterate(method, binary);
+    }
+    // optimize (:965-1028, pilot_run :1257-1298, exploit_neighborhood :1300-1328)
+    std::vector<MathOutput<P>> optimize() {
+        bool const binary = strategy != 1;
+        max_range.clear();
+        bs.clear();
+        for (auto const& rank : order) {
+            uint64_t widest = 0;
+            std::vector<Bisection> row;
+            for (auto const& r : rank) {
+                widest = std::max<uint64_t>(widest, uint64_t(std::abs(int(r.tmax) - int(r.tmin))));
+                Bisection b;
+                b.reset(tap(r), r.tmin, r.tmax, r.ctrl_at_tap);
+                row.push_back(b);
+            }
+            max_range.push_back(widest);
+            bs.push_back(row);
+        }
+        if (strategy == 2 || strategy == 3) {
+            for (auto const& rank : order)
+                for (auto const& r : rank) {
+                    bool const max_voltage = strategy == 3;
+                    set_tap(r, (max_voltage == r.ctrl_at_tap) ? r.tmax : r.tmin);
+                }
+        }
+        if (binary)
+            for (size_t i = 0; i < order.size(); ++i)
+                for (size_t j = 0; j < order[i].size(); ++j) {
+                    bs[i][j].current = tap(order[i][j]);
+                    bs[i][j].last_check = false;
+                    bs[i][j].inevitable_run = false;
+                }
+        auto result = iterate_with_fallback(binary);
+        if (strategy == 1 || strategy == 4) return result;
+        for (auto const& rank : order)
+            for (auto const& r : rank) set_tap(r, strategy == 3 ? voltage_up(tap(r), r) : voltage_down(tap(r), r));
+        return iterate_with_fallback(false);
+    }
+};
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------
@@ -2131,6 +2492,152 @@ extern "C" int or_power_flow_method(OrConstDataset const* input, double system_f
                 write_msg(messages, msg_stride, s, e.what());
                 cache_valid = !topo_changed;
             }
+        }
+    };
+    if (n_threads == 1) {
+        worker(0);
+    } else {
+        std::vector<std::thread> th;
+        for (Idx t = 0; t < n_threads; ++t) th.emplace_back(worker, t);
+        for (auto& x : th) x.join();
+    }
+    return 0;
+}
+
+extern "C" int or_power_flow_tap(OrConstDataset const* input, double system_frequency, OrConstDataset const* update,
+                                 int64_t n_scenarios, int symmetric, double err_tol, int64_t max_iter,
+                                 int64_t threading, int method, int strategy, OrTapRegulatorInput const* regulators,
+                                 int64_t n_regulator, OrTapRegulatorUpdate const* reg_update,
+                                 int64_t reg_update_count, OrMutableDataset const* output, int8_t* tap_out,
+                                 int64_t* iterations, int32_t* status, char* messages, int64_t msg_stride) {
+    Model model;
+    std::vector<OrTapRegulatorInput> regs(regulators, regulators + n_regulator);
+    std::vector<Idx> reg_trafo(size_t(n_regulator), -1);
+    std::vector<double> reg_u_rated(size_t(n_regulator), 0.0);
+    try {
+        model = build_model(*input, system_frequency);
+        // main_core/input.hpp:168-241
+        std::set<Idx> regulated;
+        for (Idx r = 0; r < n_regulator; ++r) {
+            auto it = model.ids.find(regs[size_t(r)].regulated_object);
+            if (it == model.ids.end())
+                throw OtherError("The id cannot be found: " + std::to_string(regs[size_t(r)].regulated_object) + "\n");
+            if (it->second.first != OR_TRANSFORMER)
+                throw OtherError("transformer_tap_regulator regulator is not supported for object with ID " +
+                                 std::to_string(regs[size_t(r)].regulated_object));
+            if (!regulated.insert(it->second.second).second)
+                throw OtherError("There are objects regulated by more than one regulator. Maximum one regulator is "
+                                 "allowed.");
+            reg_trafo[size_t(r)] = it->second.second;
+            Idx const t = it->second.second;
+            reg_u_rated[size_t(r)] =
+                model.node[size_t(regs[size_t(r)].control_side == 0 ? model.tr_f[size_t(t)] : model.tr_t[size_t(t)])]
+                    .u_rated;
+        }
+    } catch (std::exception const& e) {
+        write_msg(messages, msg_stride, 0, e.what());
+        return -1;
+    }
+    if (!update) n_scenarios = 1;
+    TopoCache base_tc = build_topo_cache(model, model.base);
+    Idx n_threads = 1;
+    if (update && threading == 0) {
+        Idx const hw = Idx(std::thread::hardware_concurrency());
+        n_threads = hw < 2 ? 1 : std::min(hw, n_scenarios);
+    } else if (update && threading > 1) {
+        n_threads = std::min<Idx>(threading, n_scenarios);
+    }
+    double const base_power = symmetric ? kBaseP3 : kBaseP3 / 3.0;
+    auto run = [&](auto phases, Idx s) {
+        constexpr int P = decltype(phases)::value;
+        State st = model.base;
+        bool topo_changed = false, param_changed = false;
+        if (update) apply_update(model, st, *update, s, topo_changed, param_changed);
+        RegulatorState rs;
+        for (auto const& r : regs) {
+            rs.status.push_back(r.status != 0);
+            rs.u_set.push_back(r.u_set);
+            rs.u_band.push_back(r.u_band);
+            rs.ldc_r.push_back(r.line_drop_compensation_r);
+            rs.ldc_x.push_back(r.line_drop_compensation_x);
+        }
+        for (Idx k = 0; reg_update && k < reg_update_count; ++k) {
+            OrTapRegulatorUpdate const& u = reg_update[s * reg_update_count + k];
+            Idx r = k;
+            if (u.id != kNaID) {
+                r = -1;
+                for (Idx q = 0; q < n_regulator; ++q)
+                    if (regs[size_t(q)].id == u.id) r = q;
+                if (r < 0) throw OtherError("The id cannot be found: " + std::to_string(u.id) + "\n");
+            }
+            if (u.status != kNaS) rs.status[size_t(r)] = u.status != 0;
+            if (!is_nan(u.u_set)) rs.u_set[size_t(r)] = u.u_set;
+            if (!is_nan(u.u_band)) rs.u_band[size_t(r)] = u.u_band;
+            if (!is_nan(u.line_drop_compensation_r)) rs.ldc_r[size_t(r)] = u.line_drop_compensation_r;
+            if (!is_nan(u.line_drop_compensation_x)) rs.ldc_x[size_t(r)] = u.line_drop_compensation_x;
+        }
+        TopoCache own;
+        TopoCache const* tc = &base_tc;
+        if (topo_changed) {
+            own = build_topo_cache(model, st);
+            tc = &own;
+        }
+        TapOptimizer<P> opt{model, st, *tc, err_tol, max_iter, method, strategy, {}, {}, {}};
+        for (auto const& rank : rank_regulators(model, st, regs, rs)) {
+            opt.order.emplace_back();
+            for (Idx r : rank) {
+                typename TapOptimizer<P>::Regulated g{};
+                g.reg = r;
+                g.trafo = reg_trafo[size_t(r)];
+                g.ctrl_from = regs[size_t(r)].control_side == 0;
+                g.ctrl_node = g.ctrl_from ? model.tr_f[size_t(g.trafo)] : model.tr_t[size_t(g.trafo)];
+                g.ctrl_at_tap = regs[size_t(r)].control_side == model.trafo[size_t(g.trafo)].tap_side;
+                g.tmin = model.trafo[size_t(g.trafo)].tap_min;
+                g.tmax = model.trafo[size_t(g.trafo)].tap_max;
+                // transformer_tap_regulator.hpp:76-88
+                double const ur = reg_u_rated[size_t(r)];
+                g.u_set = rs.u_set[size_t(r)] / ur;
+                g.u_band = rs.u_band[size_t(r)] / ur;
+                g.z = cd(is_nan(rs.ldc_r[size_t(r)]) ? 0.0 : rs.ldc_r[size_t(r)],
+                         is_nan(rs.ldc_x[size_t(r)]) ? 0.0 : rs.ldc_x[size_t(r)]) /
+                      (ur * ur / base_power);
+                opt.order.back().push_back(g);
+            }
+        }
+        auto const res = opt.optimize();
+        if (output) write_outputs<P>(model, st, *tc, res, *output, s);
+        Idx it = 0;
+        for (auto const& r : res) it = std::max(it, r.iterations);
+        if (tap_out) {
+            for (Idx r = 0; r < n_regulator; ++r) tap_out[s * n_regulator + r] = kNaS;
+            for (auto const& rank : opt.order)
+                for (auto const& g : rank) tap_out[s * n_regulator + g.reg] = st.tr_tap[size_t(g.trafo)];
+        }
+        return it;
+    };
+    auto worker = [&](Idx t) {
+        for (Idx s = t; s < n_scenarios; s += n_threads) {
+            int32_t code = 0;
+            std::string msg;
+            Idx it = 0;
+            try {
+                it = symmetric ? run(std::integral_constant<int, 1>{}, s) : run(std::integral_constant<int, 3>{}, s);
+            } catch (MaxIterationReached const& e) {
+                code = 4;
+                msg = e.what();
+            } catch (SparseMatrixError const& e) {
+                code = 1;
+                msg = e.what();
+            } catch (IterationDiverge const& e) {
+                code = 2;
+                msg = e.what();
+            } catch (std::exception const& e) {
+                msg = e.what();
+                code = msg.rfind("Automatic tap changer", 0) == 0 ? 4 : 3;
+            }
+            if (iterations) iterations[s] = code ? 0 : it;
+            if (status) status[s] = code;
+            write_msg(messages, msg_stride, s, msg);
         }
     };
     if (n_threads == 1) {

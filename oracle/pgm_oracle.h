@@ -36,6 +36,10 @@ typedef struct { int32_t id, node; int8_t status; double g1, b1, g0, b0; } OrShu
 typedef struct { int32_t id, node; int8_t status; double u_ref, u_ref_angle, sk, rx_ratio, z01_ratio; } OrSourceInput;
 typedef struct { int32_t id, node; int8_t status, type; double p_specified, q_specified; } OrSymLoadGenInput;
 typedef struct { int32_t id, node; int8_t status, type; double p_specified[3], q_specified[3]; } OrAsymLoadGenInput;
+typedef struct {
+    int32_t id, regulated_object; int8_t status, control_side;
+    double u_set, u_band, line_drop_compensation_r, line_drop_compensation_x;
+} OrTapRegulatorInput;
 
 /* ---------- update records ---------- */
 typedef struct { int32_t id; int8_t from_status, to_status; } OrBranchUpdate;
@@ -44,6 +48,9 @@ typedef struct { int32_t id; int8_t status; double g1, b1, g0, b0; } OrShuntUpda
 typedef struct { int32_t id; int8_t status; double u_ref, u_ref_angle, sk, rx_ratio, z01_ratio; } OrSourceUpdate;
 typedef struct { int32_t id; int8_t status; double p_specified, q_specified; } OrSymLoadGenUpdate;
 typedef struct { int32_t id; int8_t status; double p_specified[3], q_specified[3]; } OrAsymLoadGenUpdate;
+typedef struct {
+    int32_t id; int8_t status; double u_set, u_band, line_drop_compensation_r, line_drop_compensation_x;
+} OrTapRegulatorUpdate;
 
 /* ---------- output records ---------- */
 typedef struct { int32_t id; int8_t energized; double u_pu, u, u_angle, p, q; } OrNodeOutputSym;
@@ -94,6 +101,18 @@ int or_power_flow_method(OrConstDataset const* input, double system_frequency, O
                          int method, OrMutableDataset const* output, int64_t* iterations, int32_t* status,
                          char* messages, int64_t msg_stride);
 
+/* or_power_flow_method with automatic tap changing (optimizer/tap_position_optimizer.hpp): strategy 1 any
+ * valid tap, 2 min voltage tap, 3 max voltage tap, 4 fast any tap. regulators: n_regulator
+ * transformer_tap_regulator input records; reg_update: NULL, or n_scenarios * reg_update_count update
+ * records. The output is that of the last power flow at the final taps; tap_out[n_scenarios * n_regulator]
+ * receives the final tap of every regulated transformer (-128 for regulators that did not regulate).
+ * status 4: the optimizer failed (MaxIterationReached, invalid regulator configuration). */
+int or_power_flow_tap(OrConstDataset const* input, double system_frequency, OrConstDataset const* update,
+                      int64_t n_scenarios, int symmetric, double err_tol, int64_t max_iter, int64_t threading,
+                      int method, int strategy, OrTapRegulatorInput const* regulators, int64_t n_regulator,
+                      OrTapRegulatorUpdate const* reg_update, int64_t reg_update_count,
+                      OrMutableDataset const* output, int8_t* tap_out, int64_t* iterations, int32_t* status,
+                      char* messages, int64_t msg_stride);
 /* Minimum-degree ordering (sparse_ordering.hpp:183-219). The adjacency map is given as sorted keys with
  * CSR neighbour lists. alpha_out has room for n_vertices (all vertices), fills_out for 2*max_fills.
  * Returns the number of fills, or -1 if max_fills is too small. n_alpha_out receives the order length. */

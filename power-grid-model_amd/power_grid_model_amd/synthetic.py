@@ -7,6 +7,9 @@ The five workloads (input + update datasets in this package's dataset format):
   feeders_64      64 independent feeders of 50..2000 nodes, 128 scenarios each (sym)
   year_20k        20 002-node MV grid: 110/10.5 kV transformer, 50 x 400-node feeders, 200 open ties,
                   12 000 loads, 8760 hourly scenarios, 10% of them switching 1-3 ties and 1-3 lines (sym)
+  year_20k_tap    year_20k with an on-load tap changer on the 110/10.5 kV transformer (taps -9..9 of
+                  1.5% at the 110 kV side) and a transformer_tap_regulator holding the 10.5 kV busbar
+                  at 10.5 kV +- 1.2% (the automatic tap changer's workload)
 
 As literally specified these grids are loaded beyond voltage collapse (the Newton-Raphson method of
 the reference would report IterationDiverge for most scenarios), so every load of a workload is
@@ -337,10 +340,34 @@ def year_20k(seed: int = 4, n_scenarios: int = 8760, kappa: float = 0.04, n_feed
                 info={"switching_scenarios": int(len(switching))})
 
 
+def year_20k_tap(seed: int = 4, n_scenarios: int = 8760, kappa: float = 0.04, u_band: float = 252.0,
+                 **kwargs) -> Case:
+    case = year_20k(seed=seed, n_scenarios=n_scenarios, kappa=kappa, **kwargs)
+    tr = case.input["transformer"]
+    tr["tap_min"] = -9
+    tr["tap_max"] = 9
+    tr["tap_nom"] = 0
+    tr["tap_pos"] = 0
+    tr["tap_size"] = 0.015 * 110e3
+    reg = initialize_array("input", "transformer_tap_regulator", 1)
+    reg["id"] = 50_000_000
+    reg["regulated_object"] = tr["id"][0]
+    reg["status"] = 1
+    reg["control_side"] = 1
+    reg["u_set"] = U_MV
+    reg["u_band"] = u_band
+    reg["line_drop_compensation_r"] = 0.0
+    reg["line_drop_compensation_x"] = 0.0
+    case.input["transformer_tap_regulator"] = reg
+    case.name = "year_20k_tap"
+    return case
+
+
 CONFIGS = {
     "radial_200": radial_200,
     "meshed_1000": meshed_1000,
     "asym_500": asym_500,
     "feeders_64": feeders_64,
     "year_20k": year_20k,
+    "year_20k_tap": year_20k_tap,
 }

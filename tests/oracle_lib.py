@@ -46,6 +46,10 @@ def lib():
         _lib.or_power_flow_method.restype = C.c_int
         _lib.or_power_flow_method.argtypes = [C.POINTER(_Const), C.c_double, C.POINTER(_Const), _i64, C.c_int,
                                               C.c_double, _i64, _i64, C.c_int, C.POINTER(_Mut), _p, _p, _p, _i64]
+        _lib.or_power_flow_tap.restype = C.c_int
+        _lib.or_power_flow_tap.argtypes = [C.POINTER(_Const), C.c_double, C.POINTER(_Const), _i64, C.c_int,
+                                           C.c_double, _i64, _i64, C.c_int, C.c_int, _p, _i64, _p, _i64,
+                                           C.POINTER(_Mut), _p, _p, _p, _p, _i64]
         _lib.or_minimum_degree_ordering.restype = _i64
         _lib.or_minimum_degree_ordering.argtypes = [_i64, _p, _p, _p, _p, _p, _p, _i64]
         _lib.or_build_topology.restype = _i64
@@ -73,6 +77,54 @@ def power_flow(input_data: dict, update: dict | None = None, n_scenarios: int = 
     """Run the oracle. update: {component: array of shape (n_scenarios, rows_per_scenario)}.
 
     Returns (output dict {component: array (n_scenarios, n)}, iterations, status, messages)."""
+    keep, cin, cup, cout, out, n_scenarios = _datasets(input_data, update, n_scenarios, symmetric)
+    iters = np.zeros(n_scenarios, dtype=np.int64)
+    status = np.zeros(n_scenarios, dtype=np.int32)
+    stride = 512
+    msgs = np.zeros(n_scenarios * stride, dtype=np.uint8)
+    rc = lib().or_power_flow_method(C.byref(cin), system_frequency, C.byref(cup) if cup is not None else None,
+                                    n_scenarios, int(symmetric), err_tol, max_iter, threading, METHODS[method],
+                                    C.byref(cout), _ptr(iters), _ptr(status), _ptr(msgs), stride)
+    messages = [bytes(msgs[i * stride:(i + 1) * stride]).split(b"\0")[0].decode() for i in range(n_scenarios)]
+    if rc != 0:
+        raise RuntimeError(messages[0])
+    return out, iters, status, messages
+
+
+TAP_STRATEGIES = {"any_valid_tap": 1, "min_voltage_tap": 2, "max_voltage_tap": 3, "fast_any_tap": 4}
+
+
+def power_flow_tap(input_data: dict, update: dict | None = None, n_scenarios: int = 1, symmetric: bool = True,
+                   strategy: str = "any_valid_tap", err_tol: float = 1e-8, max_iter: int = 20, threading: int = -1,
+                   system_frequency: float = 50.0, method: str = "newton_raphson"):
+    """Power flow with automatic tap changing. input_data / update may hold transformer_tap_regulator.
+
+    Returns (output dict, tap positions (n_scenarios, n_regulator), iterations, status, messages); status
+    4 marks a scenario whose optimizer failed (MaxIterationReached, invalid regulator configuration)."""
+    keep, cin, cup, cout, out, n_scenarios = _datasets(input_data, update, n_scenarios, symmetric)
+    reg = input_data.get("transformer_tap_regulator")
+    reg = np.ascontiguousarray(reg if reg is not None else np.zeros(0, INPUT["transformer_tap_regulator"]),
+                               dtype=INPUT["transformer_tap_regulator"])
+    ru = None if update is None else update.get("transformer_tap_regulator")
+    if ru is not None:
+        ru = np.ascontiguousarray(ru, dtype=UPDATE["transformer_tap_regulator"])
+    taps = np.zeros((n_scenarios, max(len(reg), 1)), dtype=np.int8)
+    iters = np.zeros(n_scenarios, dtype=np.int64)
+    status = np.zeros(n_scenarios, dtype=np.int32)
+    stride = 512
+    msgs = np.zeros(n_scenarios * stride, dtype=np.uint8)
+    rc = lib().or_power_flow_tap(C.byref(cin), system_frequency, C.byref(cup) if cup is not None else None,
+                                 n_scenarios, int(symmetric), err_tol, max_iter, threading, METHODS[method],
+                                 TAP_STRATEGIES[strategy], _ptr(reg), len(reg), _ptr(ru),
+                                 0 if ru is None else ru.shape[1], C.byref(cout), _ptr(taps), _ptr(iters),
+                                 _ptr(status), _ptr(msgs), stride)
+    messages = [bytes(msgs[i * stride:(i + 1) * stride]).split(b"\0")[0].decode() for i in range(n_scenarios)]
+    if rc != 0:
+        raise RuntimeError(messages[0])
+    return out, taps[:, :len(reg)], iters, status, messages
+
+
+def _datasets(input_data: dict, update: dict | None, n_scenarios: int, symmetric: bool):
     keep = []
     cin = _Const()
     for k, comp in enumerate(COMPONENT_ORDER):
@@ -104,17 +156,7 @@ def power_flow(input_data: dict, update: dict | None = None, n_scenarios: int = 
             out[comp] = o
             cout.data[k] = o.ctypes.data
             cout.count[k] = len(arr)
-    iters = np.zeros(n_scenarios, dtype=np.int64)
-    status = np.zeros(n_scenarios, dtype=np.int32)
-    stride = 512
-    msgs = np.zeros(n_scenarios * stride, dtype=np.uint8)
-    rc = lib().or_power_flow_method(C.byref(cin), system_frequency, C.byref(cup) if cup is not None else None,
-                                    n_scenarios, int(symmetric), err_tol, max_iter, threading, METHODS[method],
-                                    C.byref(cout), _ptr(iters), _ptr(status), _ptr(msgs), stride)
-    messages = [bytes(msgs[i * stride:(i + 1) * stride]).split(b"\0")[0].decode() for i in range(n_scenarios)]
-    if rc != 0:
-        raise RuntimeError(messages[0])
-    return out, iters, status, messages
+    return keep, cin, cup, cout, out, n_scenarios
 
 
 def minimum_degree_ordering(graph: dict[int, list[int]]):

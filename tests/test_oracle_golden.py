@@ -18,6 +18,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as ol
+from pgm_formats import SYM_OUTPUT
 from validation_utils import compare_result, list_cases, load_case
 
 
@@ -345,6 +346,45 @@ def test_validation_case_other_methods(method, name, sym, batch):
     assert (status == 0).all(), msgs
     for s, exp in enumerate(expected):
         compare_result(out, exp, params["rtol"], params["atol"], s)
+
+
+TAP_CASES = [(m, *c) for m in ("newton_raphson", "iterative_current") for c in list_cases(m, tap=True)]
+
+
+def _with_regulator_output(out, taps, input_data):
+    reg = input_data["transformer_tap_regulator"]
+    rec = np.zeros(taps.shape, dtype=SYM_OUTPUT["transformer_tap_regulator"])
+    rec["id"] = reg["id"][None, :]
+    rec["tap_pos"] = taps
+    rec["energized"] = (taps != -128).astype(np.int8)
+    return dict(out, transformer_tap_regulator=rec)
+
+
+@pytest.mark.parametrize("method,name,sym,batch", TAP_CASES,
+                         ids=[f"{c[0]}-{c[1]}-{'sym' if c[2] else 'asym'}{'-batch' if c[3] else ''}"
+                              for c in TAP_CASES])
+def test_validation_case_tap_changer(method, name, sym, batch):
+    """The oracle's automatic tap changer against the reference's tap regulator cases: every output and the
+    regulated tap positions."""
+    params, input_data, update, n_scenarios, expected = load_case(name, sym, batch)
+    out, taps, iters, status, msgs = ol.power_flow_tap(input_data, update, n_scenarios, symmetric=sym,
+                                                       strategy=params["tap_changing_strategy"], method=method)
+    assert (status == 0).all(), msgs
+    out = _with_regulator_output(out, taps, input_data)
+    for s, exp in enumerate(expected):
+        compare_result(out, exp, params["rtol"], params["atol"], s)
+
+
+TAP_RAISE_CASES = list_cases(raises=True)
+
+
+@pytest.mark.parametrize("name,sym,batch", TAP_RAISE_CASES, ids=[c[0] for c in TAP_RAISE_CASES])
+def test_validation_case_tap_changer_raises(name, sym, batch):
+    """Cases the reference reports as MaxIterationReached."""
+    params, input_data, update, n_scenarios, _ = load_case(name, sym, batch)
+    _, _, _, status, msgs = ol.power_flow_tap(input_data, update, n_scenarios, symmetric=sym,
+                                              strategy=params["tap_changing_strategy"])
+    assert (status == 4).all() and all(m.startswith("Maximum number of iterations reached!") for m in msgs), msgs
 
 
 def test_iterative_current_reaches_newton_raphson_fixed_point():
