@@ -10,6 +10,8 @@
 #include "../../include/power_grid_model_amd.h"
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -301,6 +303,57 @@ std::string structural_key(Model const& m, ComponentState const& st, bool branch
     return k.str();
 }
 
+// structural_key of a state that differs from the base at most in the listed components (sorted here)
+std::string touched_key(Model const& m, ComponentState const& st, std::vector<std::pair<int, Idx>>& touched) {
+    std::sort(touched.begin(), touched.end(), [](auto const& a, auto const& b) {
+        // key order: lines, transformers, shunts, sources
+        auto rank = [](int c) { return c == C_LINE ? 0 : c == C_TRANSFORMER ? 1 : c == C_SHUNT ? 2 : 3; };
+        return std::make_pair(rank(a.first), a.second) < std::make_pair(rank(b.first), b.second);
+    });
+    touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+    std::string k;
+    auto const& b = m.base;
+    auto num = [&k](long long v) { k += std::to_string(v); };
+    for (auto const& [comp, i] : touched) {
+        if (comp == C_LINE) {
+            if (st.line_fs[i] == b.line_fs[i] && st.line_ts[i] == b.line_ts[i]) continue;
+            k += 'l';
+            num(i);
+            k += ':';
+            num(st.line_fs[i]);
+            num(st.line_ts[i]);
+            k += ';';
+        } else if (comp == C_TRANSFORMER) {
+            if (st.tr_fs[i] == b.tr_fs[i] && st.tr_ts[i] == b.tr_ts[i] && st.tr_tap[i] == b.tr_tap[i]) continue;
+            k += 't';
+            num(i);
+            k += ':';
+            num(st.tr_fs[i]);
+            num(st.tr_ts[i]);
+            k += ',';
+            num(st.tr_tap[i]);
+            k += ';';
+        } else if (comp == C_SHUNT) {
+            double const a[4] = {st.sh_g1[i], st.sh_b1[i], st.sh_g0[i], st.sh_b0[i]};
+            double const c[4] = {b.sh_g1[i], b.sh_b1[i], b.sh_g0[i], b.sh_b0[i]};
+            if (st.sh_status[i] == b.sh_status[i] && std::memcmp(a, c, sizeof a) == 0) continue;
+            k += 's';
+            num(i);
+            k += ':';
+            num(st.sh_status[i]);
+            for (double x : a) k += ',' + format_double(x);
+            k += ';';
+        } else {
+            if (st.src_status[i] == b.src_status[i]) continue;
+            k += 'c';
+            num(i);
+            k += ':';
+            num(st.src_status[i]);
+            k += ';';
+        }
+    }
+    return k;
+}
 
 // structure and per-scenario device arrays of one group (records maps included)
 void upload_group(Batch& b, Group& g) {
@@ -899,18 +952,6 @@ struct UpdateReader {
     LoadUpdateSource dsrc;
 };
 
-// id of row i of an update buffer (row-based or columnar; na when the buffer has no id column)
-ID update_row_id(ComponentBuffer const& buf, Idx i) {
-    ID id = na_IntID;
-    if (buf.data) {
-        std::memcpy(&id, static_cast<char const*>(buf.data) + size_t(i) * buf.meta->size, sizeof(ID));
-    } else {
-        for (auto const& [a, col] : buf.columns)
-            if (a->name == "id") std::memcpy(&id, static_cast<char const*>(col) + size_t(i) * sizeof(ID), sizeof(ID));
-    }
-    return id;
-}
-
 }  // namespace
 
 void Batch::release() {
@@ -926,7 +967,23 @@ void Batch::release() {
     if (stream) (void)hipStreamDestroy(stream);
 }
 
+namespace {
+// phase timings of batch creation on stderr when PGM_AMD_TRACE is set (diagnostics only)
+struct PhaseTrace {
+    bool on = std::getenv("PGM_AMD_TRACE") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void mark(char const* phase) {
+        if (!on) return;
+        auto const now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[pgm_amd] create_batch %-22s %8.1f ms\n", phase,
+                     std::chrono::duration<double, std::milli>(now - t).count());
+        t = now;
+    }
+};
+}  // namespace
+
 std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, PGM_ConstDataset const* update) {
+    PhaseTrace trace;
     if (opt.calculation_type != PGM_power_flow)
         throw PowerGridError("The calculation type is not supported by this build: only power flow is implemented\n");
     if (opt.calculation_method != PGM_default_method && opt.calculation_method != PGM_newton_raphson &&
@@ -981,30 +1038,71 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             UpdateReader r;
             r.buf = &buf;
             r.comp = comp;
-            bool all_na = true, any_na = false;
-            for (Idx i = 0; i < buf.total_elements; ++i) {
-                ID const id = update_row_id(buf, i);
-                all_na = all_na && id == na_IntID;
-                any_na = any_na || id == na_IntID;
+            Idx const eps = buf.elements_per_scenario;
+            // one pass over the id column (several host threads): NA ids, and whether every scenario
+            // repeats scenario 0's ids
+            char const* idcol = nullptr;
+            size_t idstride = 0;
+            if (buf.data) {
+                idcol = static_cast<char const*>(buf.data);
+                idstride = buf.meta->size;
+            } else {
+                for (auto const& [a, col] : buf.columns)
+                    if (a->name == "id") {
+                        idcol = static_cast<char const*>(col);
+                        idstride = sizeof(ID);
+                    }
             }
-            if (buf.total_elements > 0 && any_na && !all_na)
+            Idx const total = buf.total_elements;
+            auto id_at = [&](Idx i) {
+                ID id = na_IntID;
+                if (idcol) std::memcpy(&id, idcol + size_t(i) * idstride, sizeof id);
+                return id;
+            };
+            unsigned const nth =
+                unsigned(std::max<Idx>(1, std::min<Idx>(16, std::min<Idx>(Idx(std::thread::hardware_concurrency()),
+                                                                            total / (1 << 20) + 1))));
+            std::vector<char> t_na(nth, 0), t_num(nth, 0), t_diff(nth, 0);
+            auto scan = [&](unsigned t) {
+                Idx const lo = total * Idx(t) / Idx(nth), hi = total * Idx(t + 1) / Idx(nth);
+                bool na = false, num = false, diff = false;
+                for (Idx i = lo; i < hi; ++i) {
+                    ID const id = id_at(i);
+                    na = na || id == na_IntID;
+                    num = num || id != na_IntID;
+                    if (eps > 0 && i >= eps && !diff) diff = id != id_at(i % eps);
+                }
+                t_na[t] = na;
+                t_num[t] = num;
+                t_diff[t] = diff;
+            };
+            if (!idcol) {
+                t_na[0] = total > 0;
+            } else if (nth == 1) {
+                scan(0);
+            } else {
+                std::vector<std::thread> th;
+                for (unsigned t = 1; t < nth; ++t) th.emplace_back(scan, t);
+                scan(0);
+                for (auto& x : th) x.join();
+            }
+            bool const any_na = std::any_of(t_na.begin(), t_na.end(), [](char c) { return c != 0; });
+            bool const any_num = std::any_of(t_num.begin(), t_num.end(), [](char c) { return c != 0; });
+            bool const ids_differ = std::any_of(t_diff.begin(), t_diff.end(), [](char c) { return c != 0; });
+            if (total > 0 && any_na && any_num)
                 throw PowerGridError("IDs contain both numbers and NANs for component " + buf.meta->name +
                                      " in update data!");
-            r.all_na_ids = all_na && buf.total_elements > 0;
+            r.all_na_ids = any_na && !any_num && total > 0;
             if (r.all_na_ids && (buf.elements_per_scenario < 0 || buf.elements_per_scenario != m.count(comp)))
                 throw PowerGridError("Update data without IDs for component " + buf.meta->name +
                                      " has a different number of elements per scenario then input data!");
-            Idx const eps = buf.elements_per_scenario;
             if (!r.all_na_ids && eps > 0 && update->batch_size > 0) {
-                bool ok = true;
+                bool ok = !ids_differ;
                 try {
-                    for (Idx i = 0; i < eps; ++i) r.idx0.push_back(find_component(m, comp, update_row_id(buf, i)));
+                    for (Idx i = 0; ok && i < eps; ++i) r.idx0.push_back(find_component(m, comp, id_at(i)));
                 } catch (PowerGridError const&) {
                     ok = false;  // scenario 0 has an unknown id: rows are looked up one by one
                 }
-                for (Idx sc = 1; ok && sc < update->batch_size; ++sc)
-                    for (Idx i = 0; ok && i < eps; ++i)
-                        ok = update_row_id(buf, sc * eps + i) == update_row_id(buf, i);
                 r.same_ids = ok;
                 if (!ok) r.idx0.clear();
             }
@@ -1036,6 +1134,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     };
     auto for_each_row = [&](Idx s, std::vector<char>& scratch, auto&& fn) { rows_of(s, scratch, false, fn); };
     auto for_each_host_row = [&](Idx s, std::vector<char>& scratch, auto&& fn) { rows_of(s, scratch, true, fn); };
+    trace.mark("update readers");
     // load_gen updates on the device: every load_gen reader lists the same components in every scenario,
     // each once
     bool device_loads = false;
@@ -1124,6 +1223,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             device_srcs.push_back(d);
         }
     }
+    trace.mark("device load upload");
     // rows whose ids need no per-scenario check (resolved once, or positional)
     bool const ids_resolved =
         std::all_of(readers.begin(), readers.end(), [](UpdateReader const& r) { return r.all_na_ids || r.same_ids; });
@@ -1146,6 +1246,37 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         return gi;
     };
     std::vector<char> scratch;
+    // one working state: each scenario's structural rows are applied to it and undone afterwards, and its
+    // key lists only the components those rows touched (the others equal the base)
+    ComponentState work = m.base;
+    std::vector<std::pair<int, Idx>> touched;
+    auto restore = [&] {
+        ComponentState const& bs = m.base;
+        for (auto const& [comp, i] : touched) {
+            if (comp == C_LINE) {
+                work.line_fs[i] = bs.line_fs[i];
+                work.line_ts[i] = bs.line_ts[i];
+            } else if (comp == C_TRANSFORMER) {
+                work.tr_fs[i] = bs.tr_fs[i];
+                work.tr_ts[i] = bs.tr_ts[i];
+                work.tr_tap[i] = bs.tr_tap[i];
+            } else if (comp == C_SHUNT) {
+                work.sh_status[i] = bs.sh_status[i];
+                work.sh_g1[i] = bs.sh_g1[i];
+                work.sh_b1[i] = bs.sh_b1[i];
+                work.sh_g0[i] = bs.sh_g0[i];
+                work.sh_b0[i] = bs.sh_b0[i];
+            } else {
+                work.src_status[i] = bs.src_status[i];
+                work.src_uref[i] = bs.src_uref[i];
+                work.src_uang[i] = bs.src_uang[i];
+                work.src_sk[i] = bs.src_sk[i];
+                work.src_rx[i] = bs.src_rx[i];
+                work.src_z01[i] = bs.src_z01[i];
+            }
+        }
+        touched.clear();
+    };
     for (Idx s = 0; s < S; ++s) {
         try {
             if (!has_structural_update) {
@@ -1153,15 +1284,19 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
                 if (!ids_resolved) for_each_row(s, scratch, [](int, Idx, void const*) {});
                 scn_group[s] = group_of(std::string{}, m.base);
             } else {
-                ComponentState st = m.base;
                 bool structural = false;
                 for_each_host_row(s, scratch, [&](int comp, Idx idx, void const* rec) {
-                    if (comp == C_LINE || comp == C_TRANSFORMER || comp == C_SHUNT || comp == C_SOURCE)
-                        apply_update_record(m, st, comp, idx, rec, structural);
+                    if (comp == C_LINE || comp == C_TRANSFORMER || comp == C_SHUNT || comp == C_SOURCE) {
+                        touched.emplace_back(comp, idx);
+                        apply_update_record(m, work, comp, idx, rec, structural);
+                    }
                 });
-                scn_group[s] = group_of(structural ? structural_key(m, st) : std::string{}, structural ? st : m.base);
+                scn_group[s] = group_of(structural ? touched_key(m, work, touched) : std::string{},
+                                        structural ? work : m.base);
+                restore();
             }
         } catch (PowerGridError const& e) {
+            restore();
             if (!update) throw;
             scn_error[s] = e.what();
             scn_group[s] = -1;
@@ -1170,6 +1305,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         b.groups[scn_group[s]]->scenarios.push_back(int(s));
     }
 
+    trace.mark("grouping");
     // ---- small groups that differ from the base state only in branch statuses: one union group on
     // the base structure per state of the other components. Members that only open branches run on the
     // shared structure with per-lane admittances. Members that also close branches the base has open
@@ -1268,6 +1404,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         b.groups = std::move(kept);
     }
 
+    trace.mark("union groups");
     // ---- per group: topology and plan, built in parallel on the host ----
     {
         size_t const G = b.groups.size();
@@ -1302,6 +1439,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             if (!e.empty()) throw PowerGridError(e);
     }
 
+    trace.mark("plans");
     // ---- batch-wide iteration state and lanes ----
     long long pairs = 0, lanes = 0;
     for (auto& gp : b.groups) {
@@ -1357,6 +1495,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         fill_scenario_data(b, *gp, update != nullptr, for_each_host_row, device_loads, device_srcs, table);
     }
 
+    trace.mark("upload + scenario data");
     // ---- device tables and work lists ----
     size_t const G = b.groups.size();
     {
@@ -1458,15 +1597,14 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         }
     }
 
+    trace.mark("tables + work lists");
     // ---- output record buffers ----
     for (int c = 0; c < N_COMP; ++c) {
-        Idx const cnt = m.count(c);
         size_t const rs = c == C_NODE ? (sym ? sizeof(NodeOutput<1>) : sizeof(NodeOutput<3>))
                           : (c == C_LINE || c == C_TRANSFORMER)
                               ? (sym ? sizeof(BranchOutput<1>) : sizeof(BranchOutput<3>))
                               : (sym ? sizeof(ApplianceOutput<1>) : sizeof(ApplianceOutput<3>));
-        b.rec_size[c] = rs;
-        if (cnt > 0) b.out[c] = b.arena.take(size_t(S) * size_t(cnt) * rs);
+        b.rec_size[c] = rs;  // the record buffer itself is taken by the first run that writes records
     }
     b.status.assign(S, 0);
     b.iterations.assign(S, 0);
@@ -1477,6 +1615,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             b.messages[s] = scn_error[s];
         }
     PGM_HIP_CHECK(hipDeviceSynchronize());
+    trace.mark("outputs + sync");
     return bp;
 }
 
@@ -1640,6 +1779,9 @@ void Batch::run() {
     };
     // source and shunt flows and the appliance sum of every bus: needed by source and shunt records and
     // by the node's p and q (load_gen records form their own power from the injection and the voltage)
+    for (int c = 0; c < N_COMP; ++c)
+        if (want[c] && cols[c].empty() && !out[c] && model.count(c) > 0)
+            out[c] = arena.take(size_t(n_scenarios) * size_t(model.count(c)) * rec_size[c]);
     if (bus_sums) launch_bus_results(P, d_outputs, d_bus_off, G, bus_total, st);
     if (want[C_NODE])
         launch_node_records(P, d_outputs, d_node_recs, d_rec_off[C_NODE], G, rec_total[C_NODE], out[C_NODE],

@@ -11,6 +11,8 @@
 #include "../../include/power_grid_model_amd.h"
 
 #include <algorithm>
+#include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -649,29 +651,72 @@ struct TapChanger {
         sc.pending = -1;
     }
 
-    // the update data of one round: the caller's buffers (less the regulators) followed by the taps of
-    // every scenario's regulated transformers, which therefore take precedence
+    // the update data of one round over the scenarios `scns` (global indices, ascending): the caller's
+    // buffers (less the regulators) followed by the taps of every scenario's regulated transformers, which
+    // therefore take precedence. Over every scenario the caller's buffers are passed on as they are;
+    // over a subset their rows are copied, scenario by scenario, into row-based buffers.
     struct RoundData {
         PGM_ConstDataset ds;
         std::vector<TransformerUpdate> rows;
         std::vector<Idx> indptr;
+        std::vector<std::vector<char>> copies;
+        std::vector<std::vector<Idx>> copy_indptr;
     };
-    void build_round(RoundData& rd) const {
+    void build_round(RoundData& rd, std::vector<Idx> const& scns) const {
+        Idx const k = Idx(scns.size());
+        bool const all = k == S;
         rd.ds.meta = &meta_dataset("update");
         rd.ds.is_batch = true;
-        rd.ds.batch_size = S;
-        if (update)
-            for (auto const& buf : update->buffers)
-                if (buf.meta->name != kTapRegulatorName) rd.ds.buffers.push_back(buf);
+        rd.ds.batch_size = k;
+        if (update) {
+            rd.copies.reserve(update->buffers.size());
+            rd.copy_indptr.reserve(update->buffers.size());
+            for (auto const& buf : update->buffers) {
+                if (buf.meta->name == kTapRegulatorName) continue;
+                if (all) {
+                    rd.ds.buffers.push_back(buf);
+                    continue;
+                }
+                size_t const rs = buf.meta->size;
+                Idx n = 0;
+                for (Idx s : scns) n += buf.end(s) - buf.begin(s);
+                auto& data = rd.copies.emplace_back(size_t(n) * rs);
+                auto& ind = rd.copy_indptr.emplace_back(1, 0);
+                Idx row = 0;
+                for (Idx s : scns) {
+                    if (buf.columnar()) {
+                        for (Idx i = buf.begin(s); i < buf.end(s); ++i, ++row)
+                            buf.read_record(i, data.data() + size_t(row) * rs);
+                    } else {
+                        size_t const cnt = size_t(buf.end(s) - buf.begin(s));
+                        std::memcpy(data.data() + size_t(row) * rs,
+                                    static_cast<char const*>(buf.data) + size_t(buf.begin(s)) * rs, cnt * rs);
+                        row += Idx(cnt);
+                    }
+                    ind.push_back(row);
+                }
+                ComponentBuffer cb;
+                cb.meta = buf.meta;
+                cb.data = data.data();
+                cb.total_elements = n;
+                if (buf.elements_per_scenario >= 0) {
+                    cb.elements_per_scenario = buf.elements_per_scenario;
+                } else {
+                    cb.elements_per_scenario = -1;
+                    cb.indptr = ind.data();
+                }
+                rd.ds.buffers.push_back(cb);
+            }
+        }
         rd.rows.clear();
         rd.indptr.assign(1, 0);
         bool uniform = true;
-        for (Idx s = 0; s < S; ++s) {
-            ScenarioRun const& sc = runs[size_t(s)];
-            for (size_t k = 0; k < sc.slot.size(); ++k)
-                rd.rows.push_back(TransformerUpdate{m.trafo[sc.slot[k].trafo].id, na_IntS, na_IntS, sc.tap[k]});
+        for (Idx j = 0; j < k; ++j) {
+            ScenarioRun const& sc = runs[size_t(scns[size_t(j)])];
+            for (size_t q = 0; q < sc.slot.size(); ++q)
+                rd.rows.push_back(TransformerUpdate{m.trafo[sc.slot[q].trafo].id, na_IntS, na_IntS, sc.tap[q]});
             rd.indptr.push_back(Idx(rd.rows.size()));
-            uniform = uniform && rd.indptr[size_t(s) + 1] - rd.indptr[size_t(s)] == rd.indptr[1];
+            uniform = uniform && rd.indptr[size_t(j) + 1] - rd.indptr[size_t(j)] == rd.indptr[1];
         }
         if (rd.rows.empty()) return;
         ComponentBuffer tb;
@@ -686,6 +731,11 @@ struct TapChanger {
         }
         rd.ds.buffers.push_back(tb);
     }
+    std::vector<Idx> every_scenario() const {
+        std::vector<Idx> v(static_cast<size_t>(S));
+        std::iota(v.begin(), v.end(), Idx(0));
+        return v;
+    }
     PGM_Options round_options(int method) const {
         PGM_Options o = opt;
         o.tap_changing_strategy = 0;
@@ -693,12 +743,15 @@ struct TapChanger {
         return o;
     }
 
-    // one power-flow batch over every scenario at its current taps; the control-node voltages and the
-    // control-side transformer flows are gathered for the optimizer
-    void run_round(int method, RoundView& rv) {
+    // one power-flow batch over the scenarios `scns` at their current taps; the control-node voltages
+    // and the control-side transformer flows are gathered for the optimizer, indexed by position in scns
+    void run_round(int method, std::vector<Idx> const& scns, RoundView& rv) {
+        auto const t0 = std::chrono::steady_clock::now();
         RoundData rd;
-        build_round(rd);
+        build_round(rd, scns);
+        Idx const S = Idx(scns.size());
         auto b = create_batch(m, round_options(method), &rd.ds);
+        auto const t1 = std::chrono::steady_clock::now();
         PGM_MutableDataset out;
         out.meta = &meta_dataset(sym ? "sym_output" : "asym_output");
         out.is_batch = true;
@@ -720,18 +773,25 @@ struct TapChanger {
         rv.base_power = base_power;
         rv.status = b->status;
         rv.message = b->messages;
-        for (auto const& sc : runs)
-            for (auto const& r : sc.slot) {
+        for (Idx s : scns)
+            for (auto const& r : runs[size_t(s)].slot) {
                 rv.node_slot.emplace(r.node, rv.node_slot.size());
                 rv.trafo_slot.emplace(r.trafo, rv.trafo_slot.size());
             }
-        auto gather = [&](int comp, char const* name, Idx item, void* dst) {
+        // every needed (column, item) is packed on the device by one strided copy each, then the packed
+        // block comes back in one transfer
+        struct Pick {
+            void const* src;
+            size_t size, pitch;
+            void* dst;
+        };
+        std::vector<Pick> picks;
+        auto pick = [&](int comp, char const* name, Idx item, void* dst) {
             for (auto const& dc : b->cols[comp]) {
                 if (dc.attr->name != name) continue;
                 size_t const as = dc.attr->size;
-                size_t const pitch = size_t(m.count(comp)) * as;
-                PGM_TAP_HIP_CHECK(hipMemcpy2D(dst, as, static_cast<char const*>(dc.dev) + size_t(item) * as, pitch,
-                                              as, size_t(S), hipMemcpyDeviceToHost));
+                picks.push_back({static_cast<char const*>(dc.dev) + size_t(item) * as, as,
+                                 size_t(m.count(comp)) * as, dst});
                 return;
             }
             throw PowerGridError(std::string("missing output column ") + name);
@@ -743,9 +803,9 @@ struct TapChanger {
             rv.energized[k].resize(size_t(S));
             rv.u_pu[k].resize(size_t(S) * P);
             rv.u_angle[k].resize(size_t(S) * P);
-            gather(C_NODE, "energized", node, rv.energized[k].data());
-            gather(C_NODE, "u_pu", node, rv.u_pu[k].data());
-            gather(C_NODE, "u_angle", node, rv.u_angle[k].data());
+            pick(C_NODE, "energized", node, rv.energized[k].data());
+            pick(C_NODE, "u_pu", node, rv.u_pu[k].data());
+            pick(C_NODE, "u_angle", node, rv.u_angle[k].data());
         }
         rv.p.resize(rv.trafo_slot.size());
         rv.q.resize(rv.trafo_slot.size());
@@ -755,13 +815,44 @@ struct TapChanger {
                 if (m.reg_trafo[r] == t) from = m.regulator[r].control_side == 0;
             rv.p[k].resize(size_t(S) * P);
             rv.q[k].resize(size_t(S) * P);
-            gather(C_TRANSFORMER, from ? "p_from" : "p_to", t, rv.p[k].data());
-            gather(C_TRANSFORMER, from ? "q_from" : "q_to", t, rv.q[k].data());
+            pick(C_TRANSFORMER, from ? "p_from" : "p_to", t, rv.p[k].data());
+            pick(C_TRANSFORMER, from ? "q_from" : "q_to", t, rv.q[k].data());
+        }
+        size_t packed = 0;
+        for (auto const& pk : picks) packed += pk.size * size_t(S);
+        if (packed > 0) {
+            void* dev = nullptr;
+            PGM_TAP_HIP_CHECK(hipMalloc(&dev, packed));
+            std::vector<char> host(packed);
+            try {
+                size_t off = 0;
+                for (auto const& pk : picks) {
+                    PGM_TAP_HIP_CHECK(hipMemcpy2D(static_cast<char*>(dev) + off, pk.size, pk.src, pk.pitch, pk.size,
+                                                  size_t(S), hipMemcpyDeviceToDevice));
+                    off += pk.size * size_t(S);
+                }
+                PGM_TAP_HIP_CHECK(hipMemcpy(host.data(), dev, packed, hipMemcpyDeviceToHost));
+            } catch (...) {
+                (void)hipFree(dev);
+                throw;
+            }
+            PGM_TAP_HIP_CHECK(hipFree(dev));
+            size_t off = 0;
+            for (auto const& pk : picks) {
+                std::memcpy(pk.dst, host.data() + off, pk.size * size_t(S));
+                off += pk.size * size_t(S);
+            }
         }
         // a scenario that failed to power-flow leaves its columns unwritten: mark it not energised
         for (Idx s = 0; s < S; ++s)
             if (rv.status[size_t(s)] != 0)
                 for (auto& e : rv.energized) e[size_t(s)] = 0;
+        if (std::getenv("PGM_AMD_TRACE")) {
+            auto const t2 = std::chrono::steady_clock::now();
+            std::fprintf(stderr, "[pgm_amd] tap round (method %d, %lld scenarios): batch %8.1f ms, run + gather %8.1f ms\n",
+                         method, (long long)S, std::chrono::duration<double, std::milli>(t1 - t0).count(),
+                         std::chrono::duration<double, std::milli>(t2 - t1).count());
+        }
     }
 
     void optimize() {
@@ -778,9 +869,16 @@ struct TapChanger {
                     for (Idx s : waiting) fail(runs[size_t(s)], max_iteration_message("TapPositionOptimizer::rounds"));
                     continue;
                 }
+                // a minority of waiting scenarios runs alone; otherwise every scenario runs (the caller's
+                // buffers need no copy then)
+                std::vector<Idx> const scns = 2 * waiting.size() <= size_t(S) ? waiting : every_scenario();
                 RoundView rv;
-                run_round(method, rv);
-                for (Idx s : waiting) advance(runs[size_t(s)], rv, s);
+                run_round(method, scns, rv);
+                for (size_t j = 0, w = 0; j < scns.size() && w < waiting.size(); ++j)
+                    if (scns[j] == waiting[w]) {
+                        advance(runs[size_t(waiting[w])], rv, Idx(j));
+                        ++w;
+                    }
             }
             if (!any) break;
         }
@@ -789,7 +887,7 @@ struct TapChanger {
     // the caller's output at the final taps, the failures of the optimizer and the regulators' taps
     void finish(PGM_MutableDataset const& output, PGM_Handle* handle) {
         RoundData rd;
-        build_round(rd);
+        build_round(rd, every_scenario());
         auto b = create_batch(m, round_options(kRequested), &rd.ds);
         b->is_batch = update != nullptr;
         b->set_output(output);
