@@ -10,7 +10,10 @@ restatement, threaded like the reference's batch dispatch over all host cores, o
 are measured in the same run.
 
     python bench.py [--steps K] [--warmup W] [--workload year_20k] [--precision fp32|fp64]
-                    [--output timing|records]
+                    [--output timing|records] [--no-tap] [--no-adjoint] [--no-cpu]
+
+The line also carries the automatic tap changer on year_20k_tap (scenarios/s of calculate_power_flow with
+a tap changing strategy, from host buffers), the adjoint on feeders_64 and the PCIe-inclusive rate.
 """
 
 
@@ -104,6 +107,7 @@ def main():
     ap.add_argument("--no-fp64-leg", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-tap", action="store_true", help="skip the automatic tap changer leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--e2e-fraction", type=float, default=0.2,
                     help="share of the scenarios timed through calculate_power_flow from host buffers (0: skip)")
@@ -225,6 +229,38 @@ def main():
         log(f"[bench] adjoint on {fcase.name}: solve {adjoint['solve_ms']:.1f} ms, adjoint {adjoint['adjoint_ms']:.1f} ms")
         del fb
 
+    # automatic tap changer on year_20k_tap (the 110/10.5 kV transformer regulated): calculate_power_flow
+    # from host buffers with a tap changing strategy; every round is one device batch over the scenarios
+    # still searching. The oracle's tap changer runs beside it on a bounded sample.
+    tap = None
+    if not args.no_tap:
+        tcase = synthetic.year_20k_tap()
+        tmodel = PowerGridModel(tcase.input)
+        tsel = dict(selection, transformer_tap_regulator=["tap_pos"])
+        tap = {"workload": tcase.name, "scenarios": tcase.n_scenarios, "unit": "scenarios/s", "strategies": {}}
+        for strategy in ("any_valid_tap", "min_voltage_tap"):
+            t0 = time.perf_counter()
+            tout = tmodel.calculate_power_flow(update_data=tcase.update, tap_changing_strategy=strategy,
+                                               factor_precision="fp64", output_component_types=tsel)
+            secs = time.perf_counter() - t0
+            taps = tout["transformer_tap_regulator"]["tap_pos"][:, 0]
+            tap["strategies"][strategy] = {"value": tcase.n_scenarios / secs, "seconds": secs,
+                                           "distinct_taps": int(len(np.unique(taps)))}
+            log(f"[bench] tap changer {strategy}: {tcase.n_scenarios} scenarios in {secs:.2f}s")
+        if not args.no_cpu:
+            import oracle_lib as ol
+            threads = len(os.sched_getaffinity(0))
+            n_tap = 256
+            upd_u, k = ol.uniform_update(tcase.update, tcase.input, list(range(n_tap)))
+            t0 = time.perf_counter()
+            ol.power_flow_tap(tcase.input, upd_u, k, strategy="any_valid_tap", threading=threads)
+            secs = time.perf_counter() - t0
+            tap["cpu_baseline"] = {"value": k / secs, "unit": "scenarios/s", "cores": threads, "kind": "port",
+                                   "sample": f"first {k} scenarios of {tcase.name}, any_valid_tap (oracle tap "
+                                             f"changer, {threads} threads, {secs:.1f} s)"}
+            log(f"[bench] tap changer cpu baseline {k / secs:.1f} scen/s on {threads} threads")
+        del tmodel
+
     cpu = None
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
@@ -270,6 +306,7 @@ def main():
                           "frac": step_bytes / t_step / 1e9 / HBM_PEAK_GBS},
         "pcie_inclusive": e2e,
         "adjoint": adjoint,
+        "tap_changer": tap,
         "cpu_baseline": cpu,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
     }
