@@ -21,6 +21,7 @@
 #include <limits>
 #include <sstream>
 #include <thread>
+#include <tuple>
 
 namespace pgm_amd {
 
@@ -31,17 +32,39 @@ namespace pgm_amd {
             throw PowerGridError(std::string("HIP error in ") + #expr + ": " + hipGetErrorString(e_) + "\n");    \
     } while (0)
 
+ChunkPool::~ChunkPool() {
+    for (auto const& c : free_) (void)hipFree(c.first);
+}
+std::pair<void*, size_t> ChunkPool::get(size_t bytes) {
+    size_t best = free_.size();
+    for (size_t i = 0; i < free_.size(); ++i)
+        if (free_[i].second >= bytes && (best == free_.size() || free_[i].second < free_[best].second)) best = i;
+    if (best == free_.size()) return {nullptr, 0};
+    auto const c = free_[best];
+    free_.erase(free_.begin() + long(best));
+    return c;
+}
+
 Arena::~Arena() {
-    for (void* c : chunks_) (void)hipFree(c);
+    for (auto const& c : chunks_) {
+        if (pool_)
+            pool_->put(c.first, c.second);
+        else
+            (void)hipFree(c.first);
+    }
 }
 void* Arena::take(size_t bytes) {
     bytes = (bytes + 255) / 256 * 256;
     if (bytes == 0) return nullptr;
     if (bytes > left_) {
-        size_t const chunk = std::max<size_t>(bytes, size_t(256) << 20);
+        size_t chunk = std::max<size_t>(bytes, size_t(256) << 20);
         void* p = nullptr;
-        PGM_HIP_CHECK(hipMalloc(&p, chunk));
-        chunks_.push_back(p);
+        if (pool_) std::tie(p, chunk) = pool_->get(chunk);
+        if (!p) {
+            chunk = std::max<size_t>(bytes, size_t(256) << 20);
+            PGM_HIP_CHECK(hipMalloc(&p, chunk));
+        }
+        chunks_.emplace_back(p, chunk);
         cur_ = static_cast<char*>(p);
         left_ = chunk;
         reserved_ += chunk;
@@ -624,109 +647,149 @@ void upload_union_lanes(Batch& b, Group& g) {
     std::vector<uint8_t> dead(size_t(n) * sp, 0), lstat(size_t(nbr) * sp, 0);
     std::vector<int> patch_lane, patch_entry;
     std::vector<double> patch_val;
-    std::vector<uint8_t> bits(nbr), reach(n);
-    std::vector<int> stack;
-    for (auto const& mem : g.members) {
-        for (Idx mb = 0; mb < nbr; ++mb)
-            bits[mb] = uint8_t(side(mem.line_fs, mem.line_ts, mem.tr_fs, mem.tr_ts, p.branch_comp[mb]));
-        // buses reached from a source over branches closed on both sides
-        std::fill(reach.begin(), reach.end(), 0);
-        stack.clear();
-        for (Idx r = 0; r < n; ++r)
-            if (p.src_ptr[r + 1] > p.src_ptr[r]) {
-                reach[r] = 1;
-                stack.push_back(int(r));
-            }
-        while (!stack.empty()) {
-            int const r = stack.back();
-            stack.pop_back();
-            for (int mb : inc[r]) {
-                if (bits[mb] != 3) continue;
-                int const f = nfrom[mb], t = nto[mb];
-                if (f < 0 || t < 0) continue;
-                int const o = f == r ? t : f;
-                if (!reach[o]) {
-                    reach[o] = 1;
-                    stack.push_back(o);
+    // members are independent: several host threads each take members in turn and write the lanes of
+    // their own members; patches and extra entries are merged afterwards in member order
+    struct MemberOut {
+        std::vector<int> pe;
+        std::vector<double> pv;
+        std::vector<std::pair<std::pair<int, int>, cd>> extras;
+        std::vector<int> mbus;
+    };
+    size_t const n_mem = g.members.size();
+    std::vector<MemberOut> mouts(n_mem);
+    unsigned const n_thr =
+        unsigned(std::max<size_t>(1, std::min<size_t>({size_t(16), size_t(std::thread::hardware_concurrency()), n_mem})));
+    std::vector<std::string> thr_err(n_thr);
+    auto member_work = [&](unsigned t) {
+        try {
+            std::vector<uint8_t> bits(nbr), reach(n);
+            std::vector<int> stack;
+            std::vector<cd> val(PP);
+            std::vector<std::pair<int, int>> touched;
+            for (size_t mi = t; mi < n_mem; mi += n_thr) {
+                auto const& mem = g.members[mi];
+                MemberOut& mo = mouts[mi];
+                for (Idx mb = 0; mb < nbr; ++mb)
+                    bits[mb] = uint8_t(side(mem.line_fs, mem.line_ts, mem.tr_fs, mem.tr_ts, p.branch_comp[mb]));
+                // buses reached from a source over branches closed on both sides
+                std::fill(reach.begin(), reach.end(), 0);
+                stack.clear();
+                for (Idx r = 0; r < n; ++r)
+                    if (p.src_ptr[r + 1] > p.src_ptr[r]) {
+                        reach[r] = 1;
+                        stack.push_back(int(r));
+                    }
+                while (!stack.empty()) {
+                    int const r = stack.back();
+                    stack.pop_back();
+                    for (int mb : inc[r]) {
+                        if (bits[mb] != 3) continue;
+                        int const f = nfrom[mb], tt = nto[mb];
+                        if (f < 0 || tt < 0) continue;
+                        int const o = f == r ? tt : f;
+                        if (!reach[o]) {
+                            reach[o] = 1;
+                            stack.push_back(o);
+                        }
+                    }
+                }
+                // entries touched by branches whose statuses differ from the union state
+                touched.clear();
+                for (Idx mb = 0; mb < nbr; ++mb) {
+                    Idx const c = p.branch_comp[mb];
+                    if (bits[mb] == side(g.st.line_fs, g.st.line_ts, g.st.tr_fs, g.st.tr_ts, c)) continue;
+                    int const f = nfrom[mb], tt = nto[mb];
+                    if (f >= 0) touched.emplace_back(f, f);
+                    if (tt >= 0) touched.emplace_back(tt, tt);
+                    if (f >= 0 && tt >= 0 && f != tt) {
+                        touched.emplace_back(f, tt);
+                        touched.emplace_back(tt, f);
+                    }
+                }
+                std::sort(touched.begin(), touched.end());
+                touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+                for (auto const& [r, c] : touched) {
+                    std::fill(val.begin(), val.end(), cd{});
+                    for (int mb : inc[r]) {
+                        int const f = nfrom[mb], tt = nto[mb];
+                        int w = -1;
+                        if (r == c) {
+                            if (f == r) w = 0;
+                            else if (tt == r) w = 3;
+                        } else if (f == r && tt == c) {
+                            w = 1;
+                        } else if (tt == r && f == c) {
+                            w = 2;
+                        }
+                        if (w < 0) continue;
+                        double const* y = yvar.data() + ((size_t(mb) * 4 + bits[mb]) * 4 * PP + size_t(w) * PP) * 2;
+                        for (int q = 0; q < PP; ++q) val[q] += cd(y[2 * q], y[2 * q + 1]);
+                    }
+                    if (r == c)
+                        for (int q = p.shunt_ptr[r]; q < p.shunt_ptr[r + 1]; ++q)
+                            for (int k = 0; k < PP; ++k) val[k] += p.shunt_y[size_t(q) * PP + k];
+                    int const e = find_entry(r, c);
+                    if (e < 0) {
+                        // off the pattern: a branch closed in this member that the union structure has
+                        // open; between live buses it becomes an extra entry of the member's lanes
+                        if (!extras || PP != 1) throw PowerGridError("union group: LU pattern entry missing\n");
+                        if (reach[r] && reach[c] && val[0] != cd{}) mo.extras.push_back({{r, c}, val[0]});
+                        continue;
+                    }
+                    mo.pe.push_back(e);
+                    for (int k = 0; k < PP; ++k) {
+                        mo.pv.push_back(val[k].real());
+                        mo.pv.push_back(val[k].imag());
+                    }
+                }
+                if (int(mo.extras.size()) > kExtraMax)
+                    throw PowerGridError("union group: too many extra entries\n");
+                for (auto const& [rc, v] : mo.extras) {
+                    mo.mbus.push_back(rc.first);
+                    mo.mbus.push_back(rc.second);
+                }
+                std::sort(mo.mbus.begin(), mo.mbus.end());
+                mo.mbus.erase(std::unique(mo.mbus.begin(), mo.mbus.end()), mo.mbus.end());
+                for (int lane = mem.lane_begin; lane < mem.lane_end; ++lane) {
+                    for (Idx r = 0; r < n; ++r) dead[size_t(r) * sp + lane] = reach[r] ? 0 : 1;
+                    for (Idx mb = 0; mb < nbr; ++mb) lstat[size_t(mb) * sp + lane] = bits[mb];
                 }
             }
+        } catch (std::exception const& e) {
+            thr_err[t] = e.what();
         }
-        // entries touched by branches whose statuses differ from the union state
-        std::vector<std::pair<int, int>> touched;
-        for (Idx mb = 0; mb < nbr; ++mb) {
-            Idx const c = p.branch_comp[mb];
-            if (bits[mb] == side(g.st.line_fs, g.st.line_ts, g.st.tr_fs, g.st.tr_ts, c)) continue;
-            int const f = nfrom[mb], t = nto[mb];
-            if (f >= 0) touched.emplace_back(f, f);
-            if (t >= 0) touched.emplace_back(t, t);
-            if (f >= 0 && t >= 0 && f != t) {
-                touched.emplace_back(f, t);
-                touched.emplace_back(t, f);
-            }
-        }
-        std::sort(touched.begin(), touched.end());
-        touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
-        std::vector<cd> val(PP);
-        std::vector<std::pair<std::pair<int, int>, cd>> member_extras;
-        for (auto const& [r, c] : touched) {
-            std::fill(val.begin(), val.end(), cd{});
-            for (int mb : inc[r]) {
-                int const f = nfrom[mb], t = nto[mb];
-                int w = -1;
-                if (r == c) {
-                    if (f == r) w = 0;
-                    else if (t == r) w = 3;
-                } else if (f == r && t == c) {
-                    w = 1;
-                } else if (t == r && f == c) {
-                    w = 2;
-                }
-                if (w < 0) continue;
-                double const* y = yvar.data() + ((size_t(mb) * 4 + bits[mb]) * 4 * PP + size_t(w) * PP) * 2;
-                for (int q = 0; q < PP; ++q) val[q] += cd(y[2 * q], y[2 * q + 1]);
-            }
-            if (r == c)
-                for (int q = p.shunt_ptr[r]; q < p.shunt_ptr[r + 1]; ++q)
-                    for (int k = 0; k < PP; ++k) val[k] += p.shunt_y[size_t(q) * PP + k];
-            int const e = find_entry(r, c);
-            if (e < 0) {
-                // off the pattern: a branch closed in this member that the union structure has open;
-                // between live buses it becomes an extra entry of the member's lanes
-                if (!extras || PP != 1) throw PowerGridError("union group: LU pattern entry missing\n");
-                if (reach[r] && reach[c] && val[0] != cd{}) member_extras.push_back({{r, c}, val[0]});
-                continue;
-            }
+    };
+    {
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < n_thr; ++t) th.emplace_back(member_work, t);
+        member_work(0);
+        for (auto& x : th) x.join();
+        for (auto const& e : thr_err)
+            if (!e.empty()) throw PowerGridError(e);
+    }
+    for (size_t mi = 0; mi < n_mem; ++mi) {
+        auto const& mem = g.members[mi];
+        MemberOut const& mo = mouts[mi];
+        size_t const ne = mo.pe.size();
+        for (size_t k = 0; k < ne; ++k)
             for (int lane = mem.lane_begin; lane < mem.lane_end; ++lane) {
                 patch_lane.push_back(lane);
-                patch_entry.push_back(e);
-                for (int k = 0; k < PP; ++k) {
-                    patch_val.push_back(val[k].real());
-                    patch_val.push_back(val[k].imag());
-                }
+                patch_entry.push_back(mo.pe[k]);
+                patch_val.insert(patch_val.end(), mo.pv.begin() + long(k) * 2 * PP,
+                                 mo.pv.begin() + long(k + 1) * 2 * PP);
             }
-        }
-        if (int(member_extras.size()) > kExtraMax) throw PowerGridError("union group: too many extra entries\n");
-        std::vector<int> mbus;
-        for (auto const& [rc, v] : member_extras) {
-            mbus.push_back(rc.first);
-            mbus.push_back(rc.second);
-            xbus[rc.first] = 1;
-        }
-        std::sort(mbus.begin(), mbus.end());
-        mbus.erase(std::unique(mbus.begin(), mbus.end()), mbus.end());
+        for (auto const& [rc, v] : mo.extras) xbus[rc.first] = 1;
+        if (mo.extras.empty()) continue;
         for (int lane = mem.lane_begin; lane < mem.lane_end; ++lane) {
-            for (Idx r = 0; r < n; ++r) dead[size_t(r) * sp + lane] = reach[r] ? 0 : 1;
-            for (Idx mb = 0; mb < nbr; ++mb) lstat[size_t(mb) * sp + lane] = bits[mb];
-            if (member_extras.empty()) continue;
-            xcnt[lane] = int(member_extras.size());
-            for (size_t e = 0; e < member_extras.size(); ++e) {
-                xrow[size_t(lane) * kExtraMax + e] = member_extras[e].first.first;
-                xcol[size_t(lane) * kExtraMax + e] = member_extras[e].first.second;
-                xy[(size_t(lane) * kExtraMax + e) * 2] = member_extras[e].second.real();
-                xy[(size_t(lane) * kExtraMax + e) * 2 + 1] = member_extras[e].second.imag();
+            xcnt[lane] = int(mo.extras.size());
+            for (size_t e = 0; e < mo.extras.size(); ++e) {
+                xrow[size_t(lane) * kExtraMax + e] = mo.extras[e].first.first;
+                xcol[size_t(lane) * kExtraMax + e] = mo.extras[e].first.second;
+                xy[(size_t(lane) * kExtraMax + e) * 2] = mo.extras[e].second.real();
+                xy[(size_t(lane) * kExtraMax + e) * 2 + 1] = mo.extras[e].second.imag();
             }
-            xm[lane] = int(mbus.size());
-            for (size_t k = 0; k < mbus.size(); ++k) xb[size_t(lane) * kExtraMax + k] = mbus[k];
+            xm[lane] = int(mo.mbus.size());
+            for (size_t k = 0; k < mo.mbus.size(); ++k) xb[size_t(lane) * kExtraMax + k] = mo.mbus[k];
             g.xlanes.push_back(lane);
         }
     }
@@ -972,6 +1035,7 @@ namespace {
 struct PhaseTrace {
     bool on = std::getenv("PGM_AMD_TRACE") != nullptr;
     std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    double part[4] = {0, 0, 0, 0};
     void mark(char const* phase) {
         if (!on) return;
         auto const now = std::chrono::steady_clock::now();
@@ -979,10 +1043,23 @@ struct PhaseTrace {
                      std::chrono::duration<double, std::milli>(now - t).count());
         t = now;
     }
+    // time since the last mark or add, accumulated into part k
+    void add(int k) {
+        if (!on) return;
+        auto const now = std::chrono::steady_clock::now();
+        part[k] += std::chrono::duration<double, std::milli>(now - t).count();
+        t = now;
+    }
+    void report_parts(std::initializer_list<char const*> names) {
+        if (!on) return;
+        int k = 0;
+        for (char const* n : names) std::fprintf(stderr, "[pgm_amd] create_batch   %-20s %8.1f ms\n", n, part[k++]);
+    }
 };
 }  // namespace
 
-std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, PGM_ConstDataset const* update) {
+std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, PGM_ConstDataset const* update,
+                                    ChunkPool* pool) {
     PhaseTrace trace;
     if (opt.calculation_type != PGM_power_flow)
         throw PowerGridError("The calculation type is not supported by this build: only power flow is implemented\n");
@@ -994,6 +1071,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     ensure_device();
     auto bp = std::make_unique<Batch>();
     Batch& b = *bp;
+    b.arena.set_pool(pool);
     b.model = model;
     Model const& m = b.model;
     b.symmetric = opt.symmetric != 0;
@@ -1315,11 +1393,29 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     {
         bool const closing_ok = sym && !b.current;
         std::map<std::string, std::vector<int>> by_key;
-        for (size_t gi = 0; gi < b.groups.size(); ++gi) {
-            Group const& g = *b.groups[gi];
-            if (g.scenarios.size() >= size_t(kUnionBelow)) continue;
-            if (opens_only(m, g.st) || (closing_ok && closed_against_base(m, g.st, kExtraMax / 2) > 0))
-                by_key[structural_key(m, g.st, false)].push_back(int(gi));
+        {
+            // candidates and their keys, classified on several host threads (each a scan of every branch)
+            size_t const ng = b.groups.size();
+            std::vector<std::string> keys(ng);
+            std::vector<char> cand(ng, 0);
+            unsigned const nth = unsigned(std::max<size_t>(
+                1, std::min<size_t>({size_t(16), size_t(std::thread::hardware_concurrency()), ng / 8 + 1})));
+            auto classify = [&](unsigned t) {
+                for (size_t gi = t; gi < ng; gi += nth) {
+                    Group const& g = *b.groups[gi];
+                    if (g.scenarios.size() >= size_t(kUnionBelow)) continue;
+                    if (opens_only(m, g.st) || (closing_ok && closed_against_base(m, g.st, kExtraMax / 2) > 0)) {
+                        cand[gi] = 1;
+                        keys[gi] = structural_key(m, g.st, false);
+                    }
+                }
+            };
+            std::vector<std::thread> th;
+            for (unsigned t = 1; t < nth; ++t) th.emplace_back(classify, t);
+            classify(0);
+            for (auto& x : th) x.join();
+            for (size_t gi = 0; gi < ng; ++gi)
+                if (cand[gi]) by_key[keys[gi]].push_back(int(gi));
         }
         std::vector<char> merged(b.groups.size(), 0);
         std::vector<std::unique_ptr<Group>> unions;
@@ -1489,13 +1585,18 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             off += (long long)(gp->S_pad) * m.n_lg();
         }
     }
+    trace.mark("lane state");
     for (auto& gp : b.groups) {
         upload_group(b, *gp);
+        trace.add(0);
         if (gp->is_union) upload_union_lanes(b, *gp);
+        trace.add(1);
         fill_scenario_data(b, *gp, update != nullptr, for_each_host_row, device_loads, device_srcs, table);
+        trace.add(2);
     }
+    trace.report_parts({"upload_group", "union lanes", "scenario data"});
 
-    trace.mark("upload + scenario data");
+    trace.mark("device tables");
     // ---- device tables and work lists ----
     size_t const G = b.groups.size();
     {

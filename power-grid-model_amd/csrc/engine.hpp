@@ -16,8 +16,24 @@ struct PGM_PowerGridModel {
 
 namespace pgm_amd {
 
+// device chunks kept for reuse: callers that create one batch after another (the tap changer's rounds)
+// hand the chunks of a finished batch to the next instead of freeing and allocating tens of GB again
+class ChunkPool {
+  public:
+    ChunkPool() = default;
+    ChunkPool(ChunkPool const&) = delete;
+    ChunkPool& operator=(ChunkPool const&) = delete;
+    ~ChunkPool();
+    // a free chunk of at least `bytes` (the smallest such), or nullptr
+    std::pair<void*, size_t> get(size_t bytes);
+    void put(void* p, size_t bytes) { free_.emplace_back(p, bytes); }
+
+  private:
+    std::vector<std::pair<void*, size_t>> free_;
+};
+
 // bump allocator over large device chunks: one batch makes thousands of small arrays (one set per
-// topology group) and frees them together
+// topology group) and frees them together (or returns them to its pool)
 class Arena {
   public:
     Arena() = default;
@@ -26,12 +42,14 @@ class Arena {
     ~Arena();
     void* take(size_t bytes);
     size_t reserved() const { return reserved_; }
+    void set_pool(ChunkPool* pool) { pool_ = pool; }
 
   private:
-    std::vector<void*> chunks_;
+    std::vector<std::pair<void*, size_t>> chunks_;
     char* cur_ = nullptr;
     size_t left_ = 0;
     size_t reserved_ = 0;
+    ChunkPool* pool_ = nullptr;
 };
 
 // apply one update record of component `comp` to a state; `structural` is set when it changes a branch
@@ -127,7 +145,9 @@ struct Batch {
     void fetch(PGM_MutableDataset const& output, PGM_Handle* handle);
 };
 
-std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, PGM_ConstDataset const* update);
+// pool: where the batch takes its device chunks from and returns them to (nullptr: allocate and free)
+std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, PGM_ConstDataset const* update,
+                                    ChunkPool* pool = nullptr);
 
 // test hook: batched block LU on the device
 int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices, Idx const* diag_lu, Idx batch,

@@ -404,6 +404,9 @@ struct TapChanger {
     bool strategy_max, strategy_any;
     std::vector<Ranking> rankings;
     std::vector<ScenarioRun> runs;
+    // device chunks handed from each round's batch to the next (declared first among the members that
+    // hold device memory, so that it is released last)
+    ChunkPool pool;
 
     TapChanger(Model const& model, PGM_Options const& o, PGM_ConstDataset const* upd)
         : m(model), opt(o), update(upd), S(upd ? upd->batch_size : 1), sym(o.symmetric != 0), P(sym ? 1 : 3),
@@ -750,7 +753,7 @@ struct TapChanger {
         RoundData rd;
         build_round(rd, scns);
         Idx const S = Idx(scns.size());
-        auto b = create_batch(m, round_options(method), &rd.ds);
+        auto b = create_batch(m, round_options(method), &rd.ds, &pool);
         auto const t1 = std::chrono::steady_clock::now();
         PGM_MutableDataset out;
         out.meta = &meta_dataset(sym ? "sym_output" : "asym_output");
@@ -888,7 +891,7 @@ struct TapChanger {
     void finish(PGM_MutableDataset const& output, PGM_Handle* handle) {
         RoundData rd;
         build_round(rd, every_scenario());
-        auto b = create_batch(m, round_options(kRequested), &rd.ds);
+        auto b = create_batch(m, round_options(kRequested), &rd.ds, &pool);
         b->is_batch = update != nullptr;
         b->set_output(output);
         b->run();
