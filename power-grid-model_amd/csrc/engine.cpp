@@ -45,6 +45,24 @@ std::pair<void*, size_t> ChunkPool::get(size_t bytes) {
     return c;
 }
 
+UploadCache::~UploadCache() {
+    for (auto const& [k, p] : dev_) (void)hipFree(p);
+}
+void const* UploadCache::get(void const* host, size_t bytes) {
+    auto const key = std::make_pair(host, bytes);
+    auto it = dev_.find(key);
+    if (it != dev_.end()) return it->second;
+    void* p = nullptr;
+    PGM_HIP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 1)));
+    hipError_t const e = hipMemcpy(p, host, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        throw PowerGridError(std::string("HIP error in upload: ") + hipGetErrorString(e) + "\n");
+    }
+    dev_.emplace(key, p);
+    return p;
+}
+
 Arena::~Arena() {
     for (auto const& c : chunks_) {
         if (pool_)
@@ -1059,7 +1077,7 @@ struct PhaseTrace {
 }  // namespace
 
 std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, PGM_ConstDataset const* update,
-                                    ChunkPool* pool) {
+                                    ChunkPool* pool, UploadCache* uploads) {
     PhaseTrace trace;
     if (opt.calculation_type != PGM_power_flow)
         throw PowerGridError("The calculation type is not supported by this build: only power flow is implemented\n");
@@ -1273,12 +1291,15 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             d.comp = put(b.arena, comp);
             d.asym = (type == 1 || type == 3) ? 1 : 0;
             // row-based: the whole row buffer is uploaded once and the fields address into it
-            char* drows = nullptr;
-            if (buf.data) {
-                drows = static_cast<char*>(b.arena.take(size_t(buf.total_elements) * buf.meta->size));
-                PGM_HIP_CHECK(hipMemcpy(drows, buf.data, size_t(buf.total_elements) * buf.meta->size,
-                                        hipMemcpyHostToDevice));
-            }
+            // device copy of a caller buffer, shared through `uploads` when given
+            auto upload = [&](void const* host, size_t bytes) -> char const* {
+                if (uploads) return static_cast<char const*>(uploads->get(host, bytes));
+                char* d = static_cast<char*>(b.arena.take(bytes));
+                PGM_HIP_CHECK(hipMemcpy(d, host, bytes, hipMemcpyHostToDevice));
+                return d;
+            };
+            char const* drows = nullptr;
+            if (buf.data) drows = upload(buf.data, size_t(buf.total_elements) * buf.meta->size);
             auto field = [&](char const* name, char const*& ptr, long long& stride) {
                 PGM_MetaAttribute const* a = buf.meta->find(name);
                 if (!a) return;
@@ -1289,9 +1310,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
                 }
                 for (auto const& [ca, col] : buf.columns)
                     if (ca == a) {
-                        char* dcol = static_cast<char*>(b.arena.take(size_t(buf.total_elements) * a->size));
-                        PGM_HIP_CHECK(hipMemcpy(dcol, col, size_t(buf.total_elements) * a->size, hipMemcpyHostToDevice));
-                        ptr = dcol;
+                        ptr = upload(col, size_t(buf.total_elements) * a->size);
                         stride = (long long)(a->size);
                     }
             };

@@ -6,6 +6,7 @@
 #include "model.hpp"
 #include "topology.hpp"
 
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -30,6 +31,20 @@ class ChunkPool {
 
   private:
     std::vector<std::pair<void*, size_t>> free_;
+};
+
+// device copies of caller buffers that several batches upload unchanged (the tap changer's rounds all read
+// the caller's load updates): keyed by host address and size, freed with the cache
+class UploadCache {
+  public:
+    UploadCache() = default;
+    UploadCache(UploadCache const&) = delete;
+    UploadCache& operator=(UploadCache const&) = delete;
+    ~UploadCache();
+    void const* get(void const* host, size_t bytes);
+
+  private:
+    std::map<std::pair<void const*, size_t>, void*> dev_;
 };
 
 // bump allocator over large device chunks: one batch makes thousands of small arrays (one set per
@@ -145,9 +160,10 @@ struct Batch {
     void fetch(PGM_MutableDataset const& output, PGM_Handle* handle);
 };
 
-// pool: where the batch takes its device chunks from and returns them to (nullptr: allocate and free)
+// pool: where the batch takes its device chunks from and returns them to (nullptr: allocate and free);
+// uploads: device copies of update buffers shared with other batches (nullptr: the batch uploads its own)
 std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, PGM_ConstDataset const* update,
-                                    ChunkPool* pool = nullptr);
+                                    ChunkPool* pool = nullptr, UploadCache* uploads = nullptr);
 
 // test hook: batched block LU on the device
 int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices, Idx const* diag_lu, Idx batch,

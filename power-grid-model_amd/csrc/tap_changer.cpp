@@ -404,9 +404,10 @@ struct TapChanger {
     bool strategy_max, strategy_any;
     std::vector<Ranking> rankings;
     std::vector<ScenarioRun> runs;
-    // device chunks handed from each round's batch to the next (declared first among the members that
-    // hold device memory, so that it is released last)
+    // device chunks handed from each round's batch to the next, and the device copies of the caller's
+    // load updates that every round over all scenarios reads
     ChunkPool pool;
+    UploadCache uploads;
 
     TapChanger(Model const& model, PGM_Options const& o, PGM_ConstDataset const* upd)
         : m(model), opt(o), update(upd), S(upd ? upd->batch_size : 1), sym(o.symmetric != 0), P(sym ? 1 : 3),
@@ -753,7 +754,9 @@ struct TapChanger {
         RoundData rd;
         build_round(rd, scns);
         Idx const S = Idx(scns.size());
-        auto b = create_batch(m, round_options(method), &rd.ds, &pool);
+        // only the caller's own buffers (rounds over every scenario) may go through the upload cache: the
+        // copies of a subset round are temporaries whose addresses can recur with other contents
+        auto b = create_batch(m, round_options(method), &rd.ds, &pool, S == this->S ? &uploads : nullptr);
         auto const t1 = std::chrono::steady_clock::now();
         PGM_MutableDataset out;
         out.meta = &meta_dataset(sym ? "sym_output" : "asym_output");
@@ -891,7 +894,7 @@ struct TapChanger {
     void finish(PGM_MutableDataset const& output, PGM_Handle* handle) {
         RoundData rd;
         build_round(rd, every_scenario());
-        auto b = create_batch(m, round_options(kRequested), &rd.ds, &pool);
+        auto b = create_batch(m, round_options(kRequested), &rd.ds, &pool, &uploads);
         b->is_batch = update != nullptr;
         b->set_output(output);
         b->run();
