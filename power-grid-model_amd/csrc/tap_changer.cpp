@@ -21,6 +21,7 @@
 #include <map>
 #include <numeric>
 #include <string>
+#include <thread>
 
 namespace pgm_amd {
 
@@ -60,14 +61,49 @@ struct TopoState {
         return {st.line_fs,    st.line_ts,   st.tr_fs,      st.tr_ts,     st.tr_tap,   st.src_status,
                 st.reg_status, st.reg_u_set, st.reg_u_band, st.reg_ldc_r, st.reg_ldc_x};
     }
-    std::string key() const {
+    // the statuses that shape the ranking, as differences from `base` at the touched components
+    std::string key(TopoState const& base, std::vector<std::pair<int, Idx>>& touched) const {
+        std::sort(touched.begin(), touched.end());
+        touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
         std::string k;
-        k.reserve(line_fs.size() * 2 + tr_fs.size() * 2 + src_status.size() + reg_status.size() + 4);
-        for (auto const* v : {&line_fs, &line_ts, &tr_fs, &tr_ts, &src_status, &reg_status}) {
-            for (IntS x : *v) k.push_back(char('0' + (x != 0)));
-            k.push_back('|');
+        for (auto const& [comp, i] : touched) {
+            int a = 0, b = 0;
+            if (comp == C_LINE) {
+                a = (line_fs[i] != 0) | (line_ts[i] != 0) << 1;
+                b = (base.line_fs[i] != 0) | (base.line_ts[i] != 0) << 1;
+            } else if (comp == C_TRANSFORMER) {
+                a = (tr_fs[i] != 0) | (tr_ts[i] != 0) << 1;
+                b = (base.tr_fs[i] != 0) | (base.tr_ts[i] != 0) << 1;
+            } else if (comp == C_SOURCE) {
+                a = src_status[i] != 0;
+                b = base.src_status[i] != 0;
+            } else {
+                a = reg_status[i] != 0;
+                b = base.reg_status[i] != 0;
+            }
+            if (a != b) k += std::to_string(comp) + ':' + std::to_string(i) + '=' + std::to_string(a) + ';';
         }
         return k;
+    }
+    void restore(TopoState const& base, std::vector<std::pair<int, Idx>> const& touched) {
+        for (auto const& [comp, i] : touched) {
+            if (comp == C_LINE) {
+                line_fs[i] = base.line_fs[i];
+                line_ts[i] = base.line_ts[i];
+            } else if (comp == C_TRANSFORMER) {
+                tr_fs[i] = base.tr_fs[i];
+                tr_ts[i] = base.tr_ts[i];
+                tr_tap[i] = base.tr_tap[i];
+            } else if (comp == C_SOURCE) {
+                src_status[i] = base.src_status[i];
+            } else {
+                reg_status[i] = base.reg_status[i];
+                reg_u_set[i] = base.reg_u_set[i];
+                reg_u_band[i] = base.reg_u_band[i];
+                reg_ldc_r[i] = base.reg_ldc_r[i];
+                reg_ldc_x[i] = base.reg_ldc_x[i];
+            }
+        }
     }
 };
 
@@ -344,7 +380,8 @@ struct RowSource {
     bool positional;
 };
 
-void apply_rows(Model const& m, std::vector<RowSource> const& src, Idx s, TopoState& st, std::vector<char>& rec) {
+void apply_rows(Model const& m, std::vector<RowSource> const& src, Idx s, TopoState& st, std::vector<char>& rec,
+                std::vector<std::pair<int, Idx>>& touched) {
     auto set_status = [](IntS& tgt, IntS v) {
         if (v != na_IntS) tgt = v != 0;
     };
@@ -356,6 +393,7 @@ void apply_rows(Model const& m, std::vector<RowSource> const& src, Idx s, TopoSt
             ID id;
             std::memcpy(&id, rec.data(), sizeof id);
             Idx const idx = r.positional ? i - b0 : find_component(m, r.comp, id);
+            touched.emplace_back(r.comp, idx);
             switch (r.comp) {
             case C_LINE: {
                 BranchUpdate u;
@@ -443,25 +481,61 @@ struct TapChanger {
         }
         TopoState const base = TopoState::of(m.base);
         std::map<std::string, int> by_key;
+        std::vector<TopoState> key_state;  // one state per distinct ranking key
         runs.resize(size_t(S));
-        std::vector<char> rec;
+        Idx const nreg = Idx(m.regulator.size());
+        // per scenario and regulator: the transformer's tap and the regulator's settings after the update
+        std::vector<IntS> scn_tap(size_t(S * nreg));
+        std::vector<double> scn_set(size_t(S * nreg) * 4);
+        {
+            TopoState work = base;
+            std::vector<char> rec;
+            std::vector<std::pair<int, Idx>> touched;
+            for (Idx s = 0; s < S; ++s) {
+                ScenarioRun& sc = runs[size_t(s)];
+                touched.clear();
+                try {
+                    apply_rows(m, src, s, work, rec, touched);
+                } catch (PowerGridError const& e) {
+                    sc.failed = true;
+                    sc.error = e.what();
+                    work.restore(base, touched);
+                    continue;
+                }
+                std::string const key = work.key(base, touched);
+                auto it = by_key.find(key);
+                if (it == by_key.end()) {
+                    it = by_key.emplace(key, int(key_state.size())).first;
+                    key_state.push_back(work);
+                }
+                sc.ranking = it->second;
+                for (Idx r = 0; r < nreg; ++r) {
+                    scn_tap[size_t(s * nreg + r)] = work.tr_tap[size_t(m.reg_trafo[r])];
+                    double* v = scn_set.data() + size_t(s * nreg + r) * 4;
+                    v[0] = work.reg_u_set[size_t(r)];
+                    v[1] = work.reg_u_band[size_t(r)];
+                    v[2] = work.reg_ldc_r[size_t(r)];
+                    v[3] = work.reg_ldc_x[size_t(r)];
+                }
+                work.restore(base, touched);
+            }
+        }
+        // rank every distinct state on several host threads
+        rankings.resize(key_state.size());
+        {
+            unsigned const nth = unsigned(std::max<size_t>(
+                1, std::min<size_t>({size_t(16), size_t(std::thread::hardware_concurrency()), key_state.size()})));
+            auto work = [&](unsigned t) {
+                for (size_t k = t; k < key_state.size(); k += nth) rankings[k] = rank_state(m, key_state[k]);
+            };
+            std::vector<std::thread> th;
+            for (unsigned t = 1; t < nth; ++t) th.emplace_back(work, t);
+            work(0);
+            for (auto& x : th) x.join();
+        }
         for (Idx s = 0; s < S; ++s) {
             ScenarioRun& sc = runs[size_t(s)];
-            TopoState st = base;
-            try {
-                apply_rows(m, src, s, st, rec);
-            } catch (PowerGridError const& e) {
-                sc.failed = true;
-                sc.error = e.what();
-                continue;
-            }
-            std::string const key = st.key();
-            auto it = by_key.find(key);
-            if (it == by_key.end()) {
-                it = by_key.emplace(key, int(rankings.size())).first;
-                rankings.push_back(rank_state(m, st));
-            }
-            sc.ranking = it->second;
+            if (sc.failed) continue;
             Ranking const& rk = rankings[size_t(sc.ranking)];
             if (!rk.error.empty()) {
                 sc.failed = true;
@@ -482,14 +556,13 @@ struct TapChanger {
                     slot.range = {ts.tap_min, ts.tap_max};
                     // transformer_tap_regulator.hpp:76-88
                     double const u_rated = m.reg_u_rated[r];
-                    slot.u_set = st.reg_u_set[r] / u_rated;
-                    slot.u_band = st.reg_u_band[r] / u_rated;
+                    double const* v = scn_set.data() + size_t(s * nreg + r) * 4;
+                    slot.u_set = v[0] / u_rated;
+                    slot.u_band = v[1] / u_rated;
                     double const z_base = u_rated * u_rated / base_power;
-                    slot.z = std::complex<double>(std::isnan(st.reg_ldc_r[r]) ? 0.0 : st.reg_ldc_r[r],
-                                                  std::isnan(st.reg_ldc_x[r]) ? 0.0 : st.reg_ldc_x[r]) /
-                             z_base;
+                    slot.z = std::complex<double>(std::isnan(v[2]) ? 0.0 : v[2], std::isnan(v[3]) ? 0.0 : v[3]) / z_base;
                     sc.slot.push_back(slot);
-                    sc.tap.push_back(st.tr_tap[slot.trafo]);
+                    sc.tap.push_back(scn_tap[size_t(s * nreg + r)]);
                     widest = std::max<uint64_t>(
                         widest, uint64_t(std::abs(int64_t(ts.tap_max) - int64_t(ts.tap_min))));
                 }
