@@ -292,14 +292,14 @@ int plain_waves() {
 }
 
 // largest number of tasks per level of a union group's schedule; PGM_AMD_UNION_TASKS overrides. Union
-// groups run on the second stream beside the plain groups: shorter chains let them finish their share of
-// the machine sooner (year_20k: the same step time at 256, 1024 and 2048, the plain forward sweep 1.39 ms
-// per launch at 256 and 1.30 ms at 1024)
+// groups run on the second stream beside the plain groups. year_20k: the same step time at 256, 1024 and
+// 2048; above 256 the plain forward sweep runs faster (1.30 against 1.39 ms per launch) only because the
+// union group's sweeps, then longer in total than the plain ones, overlap it less
 int union_tasks() {
     static int const t = [] {
         char const* v = std::getenv("PGM_AMD_UNION_TASKS");
         int const x = v ? std::atoi(v) : 0;
-        return x > 0 ? x : 1024;
+        return x > 0 ? x : 256;
     }();
     return t;
 }
