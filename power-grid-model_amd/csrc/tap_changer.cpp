@@ -971,9 +971,10 @@ struct TapChanger {
         b->is_batch = update != nullptr;
         b->set_output(output);
         b->run();
+        // a scenario the optimizer gave up on reports the optimizer's error (whatever the last batch did)
         for (Idx s = 0; s < S; ++s) {
             ScenarioRun const& sc = runs[size_t(s)];
-            if (sc.failed && b->status[size_t(s)] == 0) {
+            if (sc.failed) {
                 b->status[size_t(s)] = 4;
                 b->messages[size_t(s)] = sc.error;
             }
