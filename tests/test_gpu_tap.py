@@ -163,3 +163,25 @@ def test_cascade_ranks(strategy):
     off = upd["transformer_tap_regulator"]["status"][:, 1] == 0
     ok = np.array([s not in failed for s in range(n)])
     assert (taps[off & ok, 3] == -128).all() and (taps[~off & ok, 3] != -128).all()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_year_20k_tap_full_year_in_band(precision):
+    """All 8760 scenarios of year_20k_tap (size-independent property): with any_valid_tap every scenario
+    ends with the busbar voltage inside 10.5 kV +- 126 V (the band is wider than one tap step and every
+    scenario can reach it), the mixed-precision path included, and a second run gives the same taps and
+    voltages bit for bit."""
+    case = synthetic.year_20k_tap()
+    model = PowerGridModel(case.input)
+    sel = {"node": ["u_pu"], "transformer_tap_regulator": ["tap_pos"]}
+    out = model.calculate_power_flow(update_data=case.update, tap_changing_strategy="any_valid_tap",
+                                     factor_precision=precision, output_component_types=sel)
+    u = out["node"]["u_pu"][:, 1]
+    band = 126.0 / 10.5e3
+    assert np.all(np.abs(u - 1.0) <= band + 1e-12), (u.min(), u.max())
+    taps = out["transformer_tap_regulator"]["tap_pos"][:, 0]
+    assert len(np.unique(taps)) > 1
+    again = model.calculate_power_flow(update_data=case.update, tap_changing_strategy="any_valid_tap",
+                                       factor_precision=precision, output_component_types=sel)
+    np.testing.assert_array_equal(again["transformer_tap_regulator"]["tap_pos"][:, 0], taps)
+    assert again["node"]["u_pu"].tobytes() == out["node"]["u_pu"].tobytes()
