@@ -1425,18 +1425,28 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     {
         bool const closing_ok = sym && !b.current;
         std::map<std::string, std::vector<int>> by_key;
+        // per group: only opens branches of the base state; the branches it closes (when at most
+        // kExtraMax / 2, else none listed)
+        std::vector<char> opens;
+        std::vector<std::vector<Idx>> closed;
         {
             // candidates and their keys, classified on several host threads (each a scan of every branch)
             size_t const ng = b.groups.size();
             std::vector<std::string> keys(ng);
             std::vector<char> cand(ng, 0);
+            opens.assign(ng, 0);
+            closed.assign(ng, {});
             unsigned const nth = unsigned(std::max<size_t>(
                 1, std::min<size_t>({size_t(16), size_t(std::thread::hardware_concurrency()), ng / 8 + 1})));
             auto classify = [&](unsigned t) {
                 for (size_t gi = t; gi < ng; gi += nth) {
                     Group const& g = *b.groups[gi];
                     if (g.scenarios.size() >= size_t(kUnionBelow)) continue;
-                    if (opens_only(m, g.st) || (closing_ok && closed_against_base(m, g.st, kExtraMax / 2) > 0)) {
+                    opens[gi] = opens_only(m, g.st);
+                    // over the limit the scan stops with a partial list: no candidate then
+                    if (!opens[gi] && closing_ok && closed_against_base(m, g.st, kExtraMax / 2, &closed[gi]) <= 0)
+                        closed[gi].clear();
+                    if (opens[gi] || !closed[gi].empty()) {
                         cand[gi] = 1;
                         keys[gi] = structural_key(m, g.st, false);
                     }
@@ -1462,7 +1472,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             std::vector<char> couple(size_t(m.n_branch()), 0);
             std::vector<int> closing;
             for (int gi : cand) {
-                if (opens_only(m, b.groups[gi]->st))
+                if (opens[size_t(gi)])
                     list.push_back(gi);
                 else
                     closing.push_back(gi);
@@ -1472,11 +1482,8 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
                 // must join two buses of one math model without a source, and the structure must be a
                 // forest (one U entry per row: the correction walks single-parent chains)
                 std::vector<char> all(size_t(m.n_branch()), 0);
-                for (int gi : closing) {
-                    std::vector<Idx> cl;
-                    closed_against_base(m, b.groups[gi]->st, kExtraMax / 2, &cl);
-                    for (Idx c : cl) all[c] = 1;
-                }
+                for (int gi : closing)
+                    for (Idx c : closed[size_t(gi)]) all[c] = 1;
                 Topology const t = build_topology(m, ust, &all);
                 std::vector<char> src_node(size_t(m.n_node()), 0);
                 for (Idx k = 0; k < Idx(m.source.size()); ++k) src_node[m.src_node[k]] = 1;
@@ -1488,8 +1495,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
                 Idx const nl = m.n_line();
                 for (int gi : closing) {
                     if (!radial) break;
-                    std::vector<Idx> cl;
-                    closed_against_base(m, b.groups[gi]->st, kExtraMax / 2, &cl);
+                    std::vector<Idx> const& cl = closed[size_t(gi)];
                     bool ok = !cl.empty();
                     Idx const g0 = ok ? t.node_group[cl[0] < nl ? m.line_f[cl[0]] : m.tr_f[cl[0] - nl]] : -1;
                     for (Idx c : cl) {
