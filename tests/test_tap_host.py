@@ -108,3 +108,26 @@ def test_regulator_input_validation():
         _grid(good, [(2, 0, 1, FROM)], [(4, 1, TO)], [(3, 0)])
     with pytest.raises(pgm.PowerGridError, match="cannot be found"):
         _grid(good, [(2, 0, 1, FROM)], [(4, 99, TO)], [(3, 0)])
+
+
+def test_model_update_of_regulators():
+    """PGM_update_model applies transformer_tap_regulator updates (transformer_tap_regulator.hpp:38-46: a null
+    attribute keeps its value): switching a regulator off removes it from the ranking, and switching it
+    back on restores the ranks; an unknown id is rejected."""
+    model = _grid([(0, 10e3), (1, 10e3), (2, 10e3), (3, 10e3)],
+                  [(11, 0, 1, TO), (12, 1, 2, FROM), (13, 2, 3, FROM)],
+                  [(41, 11, TO), (42, 12, TO), (43, 13, TO)], [(31, 0)], lines=[(21, 0, 2)])
+    np.testing.assert_array_equal(tap_regulator_ranks(model), [0, 1, 0])
+    upd = initialize_array("update", "transformer_tap_regulator", 1)
+    upd["id"] = 41
+    upd["status"] = 0
+    model.update(update_data={"transformer_tap_regulator": upd})
+    # without 41, transformer 11 joins nodes 0 and 1 at distance 0: 12 runs between two buses at distance
+    # 0 (last rank), 13 from distance 0 to 1 (first rank)
+    np.testing.assert_array_equal(tap_regulator_ranks(model), [-1, 1, 0])
+    upd["status"] = 1
+    model.update(update_data={"transformer_tap_regulator": upd})
+    np.testing.assert_array_equal(tap_regulator_ranks(model), [0, 1, 0])
+    upd["id"] = 99
+    with pytest.raises(pgm.PowerGridError, match="cannot be found"):
+        model.update(update_data={"transformer_tap_regulator": upd})
