@@ -613,11 +613,14 @@ void upload_union_lanes(Batch& b, Group& g) {
                     std::vector<IntS> const& tt, Idx c) {
         return (c < nl ? (lf[c] != 0) : (tf[c - nl] != 0)) | ((c < nl ? (lt[c] != 0) : (tt[c - nl] != 0)) << 1);
     };
-    // admittances of the four status combinations of every math branch
+    // admittances of the four status combinations of every math branch (branches split over host threads,
+    // each with its own working state)
     std::vector<double> yvar(size_t(nbr) * 16 * PP * 2);
-    {
+    unsigned const n_ythr =
+        unsigned(std::max<Idx>(1, std::min<Idx>({Idx(16), Idx(std::thread::hardware_concurrency()), nbr / 512 + 1})));
+    auto yvar_work = [&](unsigned th) {
         ComponentState work = g.st;
-        for (Idx mb = 0; mb < nbr; ++mb) {
+        for (Idx mb = th; mb < nbr; mb += n_ythr) {
             Idx const c = p.branch_comp[mb];
             for (int v = 0; v < 4; ++v) {
                 IntS const fs = v & 1, ts = (v >> 1) & 1;
@@ -644,6 +647,12 @@ void upload_union_lanes(Batch& b, Group& g) {
                 work.tr_ts[c - nl] = g.st.tr_ts[c - nl];
             }
         }
+    };
+    {
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < n_ythr; ++t) th.emplace_back(yvar_work, t);
+        yvar_work(0);
+        for (auto& x : th) x.join();
     }
     // buses of the nodes of every math branch, whatever its side statuses in the union state
     Topology const& tp = g.topo;
