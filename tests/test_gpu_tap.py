@@ -185,3 +185,11 @@ def test_year_20k_tap_full_year_in_band(precision):
                                        factor_precision=precision, output_component_types=sel)
     np.testing.assert_array_equal(again["transformer_tap_regulator"]["tap_pos"][:, 0], taps)
     assert again["node"]["u_pu"].tobytes() == out["node"]["u_pu"].tobytes()
+
+
+@pytest.mark.parametrize("strategy", ["any_valid_tap", "min_voltage_tap"])
+def test_cascade_ranks_asymmetric(strategy):
+    """The cascade as an asymmetric calculation: per-phase voltages and flows, the regulated voltage is the
+    mean of the three phases' compensated magnitudes (tap_position_optimizer.hpp:720-729)."""
+    inp, upd, n = _cascade(n_scenarios=16, seed=11)
+    _compare(inp, upd, n, strategy, sym=False)
