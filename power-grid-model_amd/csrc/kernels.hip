@@ -1974,6 +1974,15 @@ __device__ __forceinline__ void make_record(DevOutput const& o, ApplRec const& r
 // for single-scenario ones. Tile t of group g: offsets[g] <= t.
 // With columns (cols.n > 0) only the listed attributes leave the tile, each into its own column: the
 // lanes of a tile row write consecutive elements of one scenario's column.
+// Workgroups are handed to the chip's 8 XCDs in turn. Tile b of the grid takes tile xcd_tile(b) instead,
+// which gives every XCD one contiguous run of tiles: neighbouring tiles write neighbouring pieces of the
+// same rows (columns and records), and those partial lines then meet in one L2. A bijection of [0, nb).
+__device__ __forceinline__ long long xcd_tile(long long b, long long nb) {
+    constexpr long long kXcd = 8;
+    long long const x = b % kXcd, q = b / kXcd, base = nb / kXcd, extra = nb % kXcd;
+    return x * base + (x < extra ? x : extra) + q;
+}
+
 template <int P, class Rec, class Out>
 __global__ __launch_bounds__(256) void records_kernel(DevOutput const* __restrict__ outs, Rec const* __restrict__ recs,
                                                       long long const* __restrict__ tile_off, int n_groups,
@@ -1982,7 +1991,7 @@ __global__ __launch_bounds__(256) void records_kernel(DevOutput const* __restric
     constexpr int CH = int(sizeof(Out) / 16);
     static_assert(sizeof(Out) % 16 == 0, "records are copied in 16-byte chunks");
     __shared__ __align__(16) Out tile[SLOTS];
-    long long const t = blockIdx.x;
+    long long const t = xcd_tile(blockIdx.x, gridDim.x);
     int const gr = find_group(tile_off, n_groups, t);
     DevOutput const o = ld(outs, gr);
     Rec const r = ld(recs, gr);
