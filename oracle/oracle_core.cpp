@@ -170,7 +170,7 @@ BranchParam<3> sym_to_asym(BranchParamSym const& p1, BranchParamSym const& p0) {
 
 struct LineParam {
     cd y1s, y1sh, y0s, y0sh;
-    double base_i, i_n;
+    double base_i, i_n, base_i_to;
 };
 // line.hpp:26-36
 LineParam line_param(OrLineInput const& in, double f, double u) {
@@ -182,6 +182,19 @@ LineParam line_param(OrLineInput const& in, double f, double u) {
     p.y0s = 1.0 / cd(in.r0, in.x0) / base_y;
     p.y0sh = 2.0 * kPi * f * in.c0 / base_y * cd(in.tan0, 1.0);
     p.i_n = in.i_n;
+    p.base_i_to = p.base_i;
+    return p;
+}
+// link.hpp:19-36 and common.hpp:97-101: y_series = y_link = g_link (1 + 1j) with g_link = 1e6 S at 10 kV in
+// per unit, no shunt, tap 1; base currents from the rated voltages of both nodes; loading 0 (i_n infinite)
+LineParam link_param(double u1, double u2) {
+    LineParam p;
+    double const g_link = 1e6 / (kBaseP3 / 10e3 / 10e3);
+    p.y1s = p.y0s = cd(g_link, g_link);
+    p.y1sh = p.y0sh = cd(0.0, 0.0);
+    p.base_i = kBaseP3 / u1 / kSqrt3;
+    p.base_i_to = kBaseP3 / u2 / kSqrt3;
+    p.i_n = std::numeric_limits<double>::infinity();
     return p;
 }
 
@@ -1408,6 +1421,7 @@ struct Model {
     double freq = 50.0;
     std::vector<OrNodeInput> node;
     std::vector<OrLineInput> line;
+    std::vector<OrLinkInput> link;
     std::vector<OrTransformerInput> trafo;
     std::vector<OrShuntInput> shunt;
     std::vector<OrSourceInput> source;
@@ -1423,7 +1437,9 @@ struct Model {
     std::vector<Idx> line_f, line_t, tr_f, tr_t, sh_node, src_node;
     State base;
 
-    Idx n_branch() const { return Idx(line.size() + trafo.size()); }
+    // line slots: the lines, then the links (a link is a line of admittance y_link)
+    Idx n_lslot() const { return Idx(line_f.size()); }
+    Idx n_branch() const { return Idx(line_f.size() + trafo.size()); }
 };
 
 Idx node_idx(Model const& m, int32_t id) {
@@ -1456,6 +1472,18 @@ Model build_model(OrConstDataset const& in, double f) {
         m.line_f.push_back(f_);
         m.line_t.push_back(t_);
         m.lparam.push_back(line_param(l, f, u1));
+        m.base.line_fs.push_back(l.from_status);
+        m.base.line_ts.push_back(l.to_status);
+    }
+    auto const* links = static_cast<OrLinkInput const*>(in.data[OR_LINK]);
+    for (Idx i = 0; i < in.count[OR_LINK]; ++i) {
+        auto const& l = links[i];
+        m.link.push_back(l);
+        reg(l.id, OR_LINK, i);
+        Idx const f_ = node_idx(m, l.from_node), t_ = node_idx(m, l.to_node);
+        m.line_f.push_back(f_);
+        m.line_t.push_back(t_);
+        m.lparam.push_back(link_param(m.node[f_].u_rated, m.node[t_].u_rated));
         m.base.line_fs.push_back(l.from_status);
         m.base.line_ts.push_back(l.to_status);
     }
@@ -1571,15 +1599,19 @@ void apply_update(Model const& m, State& st, OrConstDataset const& upd, Idx scen
         tgt = v != 0;
         return ch;
     };
-    Idx cnt = upd.count[OR_LINE];
-    auto const* lu = static_cast<OrBranchUpdate const*>(upd.data[OR_LINE]);
-    for (Idx r = 0; r < cnt; ++r) {
-        auto const& u = lu[scenario * cnt + r];
-        Idx const i = lookup(u.id, OR_LINE, r, cnt);
-        bool ch = set_s(st.line_fs[i], u.from_status);
-        ch = set_s(st.line_ts[i], u.to_status) || ch;
-        topo_changed |= ch;
-        param_changed |= ch;
+    Idx cnt = 0;
+    for (int comp : {int(OR_LINE), int(OR_LINK)}) {
+        cnt = upd.count[comp];
+        auto const* lu = static_cast<OrBranchUpdate const*>(upd.data[comp]);
+        Idx const slot0 = comp == OR_LINE ? 0 : Idx(m.line.size());
+        for (Idx r = 0; r < cnt; ++r) {
+            auto const& u = lu[scenario * cnt + r];
+            Idx const i = slot0 + lookup(u.id, comp, r, cnt);
+            bool ch = set_s(st.line_fs[i], u.from_status);
+            ch = set_s(st.line_ts[i], u.to_status) || ch;
+            topo_changed |= ch;
+            param_changed |= ch;
+        }
     }
     cnt = upd.count[OR_TRANSFORMER];
     auto const* tu = static_cast<OrTransformerUpdate const*>(upd.data[OR_TRANSFORMER]);
@@ -1667,7 +1699,7 @@ struct TopoCache {
 TopoCache build_topo_cache(Model const& m, State const& st) {
     TopoCache tc;
     std::vector<CompBranch> br;
-    Idx const nl = Idx(m.line.size());
+    Idx const nl = m.n_lslot();
     for (Idx i = 0; i < nl; ++i) br.push_back({m.line_f[i], m.line_t[i], st.line_fs[i] != 0, st.line_ts[i] != 0, 0.0});
     for (Idx i = 0; i < Idx(m.trafo.size()); ++i)
         br.push_back({m.tr_f[i], m.tr_t[i], st.tr_fs[i] != 0, st.tr_ts[i] != 0, m.tstatic[i].clock * kDeg30});
@@ -1734,7 +1766,7 @@ template <int P> MathModel<P> make_math_model(Model const& m, State const& st, T
     for (Idx node : tc.topo.bus_nodes[g]) mm.phase_shift.push_back(tc.topo.phase[node]);
     mm.branch_bus = tc.mm_branch_bus[g];
     mm.fill_in = tc.topo.fill_in[g];
-    Idx const nl = Idx(m.line.size());
+    Idx const nl = m.n_lslot();
     for (Idx b : tc.mm_branch[g]) {
         BranchParam<P> bp;
         if (b < nl) {
@@ -1883,15 +1915,16 @@ template <int P> void write_outputs(Model const& m, State const& st, TopoCache c
         }
     }
     // branches (branch.hpp get_output)
-    Idx const nl = Idx(m.line.size());
+    Idx const nl = m.n_lslot();
     auto branch_out = [&](int comp, Idx first, Idx count) {
         if (!out.data[comp]) return;
         for (Idx k = 0; k < count; ++k) {
             Idx const b = first + k;
-            int32_t const id = comp == OR_LINE ? m.line[k].id : m.trafo[k].id;
+            int32_t const id = comp == OR_LINE ? m.line[k].id : comp == OR_LINK ? m.link[k].id : m.trafo[k].id;
             double bif, bit;
-            if (comp == OR_LINE) {
-                bif = bit = m.lparam[k].base_i;
+            if (comp != OR_TRANSFORMER) {
+                bif = m.lparam[b].base_i;
+                bit = m.lparam[b].base_i_to;
             } else {
                 bif = m.tstatic[k].base_i_from;
                 bit = m.tstatic[k].base_i_to;
@@ -1924,7 +1957,7 @@ template <int P> void write_outputs(Model const& m, State const& st, TopoCache c
                 }
                 max_i = std::max(mi_f, mi_t);
                 double const max_s = std::max(sum_sf, sum_st);
-                loading = comp == OR_LINE ? max_i / m.lparam[k].i_n : max_s / m.tstatic[k].sn;
+                loading = comp != OR_TRANSFORMER ? max_i / m.lparam[b].i_n : max_s / m.tstatic[k].sn;
             }
             if (P == 1) {
                 auto& o = static_cast<OrBranchOutputSym*>(out.data[comp])[scn * count + k];
@@ -1959,7 +1992,8 @@ template <int P> void write_outputs(Model const& m, State const& st, TopoCache c
             }
         }
     };
-    branch_out(OR_LINE, 0, nl);
+    branch_out(OR_LINE, 0, Idx(m.line.size()));
+    branch_out(OR_LINK, Idx(m.line.size()), Idx(m.link.size()));
     branch_out(OR_TRANSFORMER, nl, Idx(m.trafo.size()));
     // appliances (appliance.hpp get_output)
     auto app_out = [&](int comp, Idx k, Idx count, int32_t id, double base_i, double dir, bool status, Idx g,
@@ -2099,7 +2133,7 @@ std::vector<std::vector<Idx>> rank_regulators(Model const& m, State const& st,
             edges.push_back({m.tr_t[t], m.tr_f[t], 0, -1});
         }
     }
-    for (Idx l = 0; l < Idx(m.line.size()); ++l) {
+    for (Idx l = 0; l < m.n_lslot(); ++l) {
         if (!st.line_fs[l] || !st.line_ts[l]) continue;
         edges.push_back({m.line_f[l], m.line_t[l], 0, -1});
         edges.push_back({m.line_t[l], m.line_f[l], 0, -1});
@@ -2289,7 +2323,7 @@ template <int P> struct TapOptimizer {
     int compare(Regulated const& r, std::vector<MathOutput<P>> const& res) const {
         Idx const g = tc.topo.node_group[size_t(r.ctrl_node)];
         Val<P> const u = res[size_t(g)].u[size_t(tc.topo.node_pos[size_t(r.ctrl_node)])];
-        Idx const b = Idx(m.line.size()) + r.trafo;
+        Idx const b = m.n_lslot() + r.trafo;
         Val<P> const i = r.ctrl_from ? res[size_t(tc.br_group[size_t(b)])].br_if[size_t(tc.br_pos[size_t(b)])]
                                      : res[size_t(tc.br_group[size_t(b)])].br_it[size_t(tc.br_pos[size_t(b)])];
         double v = 0.0;

@@ -25,6 +25,7 @@ typedef struct {
     int32_t id, from_node, to_node; int8_t from_status, to_status;
     double r1, x1, c1, tan1, r0, x0, c0, tan0, i_n;
 } OrLineInput;
+typedef struct { int32_t id, from_node, to_node; int8_t from_status, to_status; } OrLinkInput;
 typedef struct {
     int32_t id, from_node, to_node; int8_t from_status, to_status;
     double u1, u2, sn, uk, pk, i0, p0, i0_zero_sequence, p0_zero_sequence;
@@ -66,10 +67,10 @@ typedef struct {
 typedef struct { int32_t id; int8_t energized; double p, q, i, s, pf; } OrApplianceOutputSym;
 typedef struct { int32_t id; int8_t energized; double p[3], q[3], i[3], s[3], pf[3]; } OrApplianceOutputAsym;
 
-/* component slots, in the reference's container order (all_components.hpp:37-40) */
+/* component slots, in the reference's container order (all_components.hpp:37-40), the link last */
 enum OrComp {
     OR_NODE = 0, OR_LINE, OR_TRANSFORMER, OR_SHUNT, OR_SOURCE, OR_SYM_GEN, OR_ASYM_GEN, OR_SYM_LOAD, OR_ASYM_LOAD,
-    OR_N_COMP
+    OR_LINK, OR_N_COMP
 };
 
 /* one dataset: per component a row buffer and a count (per scenario for update/output batches) */

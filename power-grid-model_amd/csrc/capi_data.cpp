@@ -59,14 +59,16 @@ template <int P> std::vector<PGM_MetaComponent> output_components() {
     using R = TapRegulatorOutput;
     auto reg = comp<R>(kTapRegulatorName, {mk("id", &R::id, PGM_int32), mk("energized", &R::energized, PGM_int8),
                                            mk("tap_pos", &R::tap_pos, PGM_int8)});
-    return {node,           branch("line"),  branch("transformer"), appl("shunt"),     appl("source"),
-            appl("sym_gen"), appl("asym_gen"), appl("sym_load"),     appl("asym_load"), reg};
+    return {node,           branch("line"),  branch("link"),    branch("transformer"), appl("shunt"),
+            appl("source"), appl("sym_gen"), appl("asym_gen"), appl("sym_load"),     appl("asym_load"),
+            reg};
 }
 
 std::vector<PGM_MetaDataset> build_meta() {
     std::vector<PGM_MetaDataset> ds;
     using NI = NodeInput;
     using LI = LineInput;
+    using LkI = LinkInput;
     using TI = TransformerInput;
     using SI = ShuntInput;
     using SrcI = SourceInput;
@@ -95,6 +97,9 @@ std::vector<PGM_MetaDataset> build_meta() {
                            mk("tan1", &LI::tan1, PGM_double), mk("r0", &LI::r0, PGM_double),
                            mk("x0", &LI::x0, PGM_double), mk("c0", &LI::c0, PGM_double),
                            mk("tan0", &LI::tan0, PGM_double), mk("i_n", &LI::i_n, PGM_double)}),
+         comp<LkI>("link", {mk("id", &LkI::id, PGM_int32), mk("from_node", &LkI::from_node, PGM_int32),
+                            mk("to_node", &LkI::to_node, PGM_int32), mk("from_status", &LkI::from_status, PGM_int8),
+                            mk("to_status", &LkI::to_status, PGM_int8)}),
          comp<TI>("transformer",
                   {mk("id", &TI::id, PGM_int32),
                    mk("from_node", &TI::from_node, PGM_int32),
@@ -163,6 +168,8 @@ std::vector<PGM_MetaDataset> build_meta() {
     ds.push_back(PGM_MetaDataset{
         "update",
         {comp<BU>("line", {mk("id", &BU::id, PGM_int32), mk("from_status", &BU::from_status, PGM_int8),
+                           mk("to_status", &BU::to_status, PGM_int8)}),
+         comp<BU>("link", {mk("id", &BU::id, PGM_int32), mk("from_status", &BU::from_status, PGM_int8),
                            mk("to_status", &BU::to_status, PGM_int8)}),
          comp<TU>("transformer", {mk("id", &TU::id, PGM_int32), mk("from_status", &TU::from_status, PGM_int8),
                                   mk("to_status", &TU::to_status, PGM_int8), mk("tap_pos", &TU::tap_pos, PGM_int8)}),

@@ -74,9 +74,11 @@ void PGM_update_model(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_ConstDa
                 buf.read_record(i, rec.data());
                 ID id;
                 std::memcpy(&id, rec.data(), 4);
-                Idx const idx = id == na_IntID ? i - buf.begin(0) : find_component(m, comp, id);
+                Idx idx = id == na_IntID ? i - buf.begin(0) : find_component(m, comp, id);
+                int slot = comp;
+                line_slot(m, slot, idx);
                 bool structural = false;
-                apply_update_record(m, st, comp, idx, rec.data(), structural);
+                apply_update_record(m, st, slot, idx, rec.data(), structural);
             }
         }
         m.base = st;

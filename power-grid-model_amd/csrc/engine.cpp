@@ -241,7 +241,7 @@ struct Group {
     DevGroup dg;
     DevOutput dout;
     NodeRec node_rec{};
-    BranchRec branch_rec[2]{};
+    BranchRec branch_rec[3]{};  // lines, transformers, links
     ApplRec appl_rec[N_COMP]{};
     int8_t* load_status = nullptr;
     std::vector<double> level_entries, level_pivots;  // LU entries of the rows / pivots per level
@@ -534,20 +534,22 @@ void upload_group(Batch& b, Group& g) {
         int* br_from = put(A, p.branch_from);
         int* br_to = put(A, p.branch_to);
         double* br_y = put(A, complex_to_doubles(p.branch_y));
-        for (int kind = 0; kind < 2; ++kind) {
-            Idx const first = kind == 0 ? 0 : m.n_line();
-            Idx const cnt = kind == 0 ? m.n_line() : Idx(m.trafo.size());
+        // kind 0: lines, 1: transformers, 2: links (the line slots behind the lines)
+        Idx const n_real_line = Idx(m.line.size());
+        for (int kind = 0; kind < 3; ++kind) {
+            Idx const first = kind == 0 ? 0 : kind == 1 ? m.n_line() : n_real_line;
+            Idx const cnt = kind == 0 ? n_real_line : kind == 1 ? Idx(m.trafo.size()) : Idx(m.link.size());
             std::vector<int> ids, math;
             std::vector<double> bif, bit, rating;
             for (Idx k = 0; k < cnt; ++k) {
                 Idx const bidx = first + k;
-                ids.push_back(kind == 0 ? m.line[k].id : m.trafo[k].id);
+                ids.push_back(kind == 0 ? m.line[k].id : kind == 1 ? m.trafo[k].id : m.link[k].id);
                 Idx const gg = t.branch_group[bidx];
                 math.push_back(gg < 0 ? -1 : mm_branch_begin[gg] + int(t.branch_pos[bidx]));
-                if (kind == 0) {
-                    bif.push_back(m.lstat[k].base_i);
-                    bit.push_back(m.lstat[k].base_i);
-                    rating.push_back(m.lstat[k].i_n);
+                if (kind != 1) {
+                    bif.push_back(m.lstat[bidx].base_i);
+                    bit.push_back(m.lstat[bidx].base_i_to);
+                    rating.push_back(m.lstat[bidx].i_n);
                 } else {
                     bif.push_back(m.tstat[k].base_i_from);
                     bit.push_back(m.tstat[k].base_i_to);
@@ -556,7 +558,7 @@ void upload_group(Batch& b, Group& g) {
             }
             g.branch_rec[kind] = BranchRec{int(cnt),    put(A, ids), put(A, math), br_from,        br_to,
                                            br_y,        nullptr,     nullptr,      put(A, bif),    put(A, bit),
-                                           put(A, rating), kind == 0 ? 1 : 0};
+                                           put(A, rating), kind != 1 ? 1 : 0};
         }
     }
     {
@@ -850,7 +852,7 @@ void upload_union_lanes(Batch& b, Group& g) {
     g.dout.dead = d_dead;
     int* d_nfrom = put(A, nfrom);
     int* d_nto = put(A, nto);
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < 3; ++k) {
         g.branch_rec[k].lane_status = d_lstat;
         g.branch_rec[k].branch_y_var = d_yvar;
         // sides are masked per lane by the side statuses
@@ -1126,7 +1128,9 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         b.max_iter = 1;
     }
     // a single linear solve has no correction step to absorb FP32 factor error: FP64 factors
-    b.fp32 = opt.factor_precision == 1 && !b.linear;
+    // links (y_link = 1e8 (1 + 1j) pu against branch admittances of order 1-100 pu) leave an FP32 factor no
+    // significant digits for the rest of the grid: models with links are factored in FP64
+    b.fp32 = opt.factor_precision == 1 && !b.linear && m.link.empty();
     b.is_batch = update != nullptr;
     b.n_scenarios = update ? update->batch_size : 1;
     PGM_HIP_CHECK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
@@ -1246,7 +1250,9 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
                     std::memcpy(&id, scratch.data(), 4);
                     idx = find_component(m, r.comp, id);
                 }
-                fn(r.comp, idx, scratch.data());
+                int comp = r.comp;
+                line_slot(m, comp, idx);
+                fn(comp, idx, scratch.data());
             }
         }
     };
@@ -1347,7 +1353,8 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     bool const ids_resolved =
         std::all_of(readers.begin(), readers.end(), [](UpdateReader const& r) { return r.all_na_ids || r.same_ids; });
     bool const has_structural_update = std::any_of(readers.begin(), readers.end(), [](UpdateReader const& r) {
-        return r.comp == C_LINE || r.comp == C_TRANSFORMER || r.comp == C_SHUNT || r.comp == C_SOURCE;
+        return r.comp == C_LINE || r.comp == C_LINK || r.comp == C_TRANSFORMER || r.comp == C_SHUNT ||
+               r.comp == C_SOURCE;
     });
 
     // ---- group scenarios by their structural state ----
@@ -1651,13 +1658,13 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         std::vector<DevGroup> dg;
         std::vector<DevOutput> dout;
         std::vector<NodeRec> nr;
-        std::vector<BranchRec> br[2];
+        std::vector<BranchRec> br[3];
         std::vector<ApplRec> ar[N_COMP];
         for (auto& gp : b.groups) {
             dg.push_back(gp->dg);
             dout.push_back(gp->dout);
             nr.push_back(gp->node_rec);
-            for (int k = 0; k < 2; ++k) br[k].push_back(gp->branch_rec[k]);
+            for (int k = 0; k < 3; ++k) br[k].push_back(gp->branch_rec[k]);
             for (int c = 0; c < N_COMP; ++c) ar[c].push_back(gp->appl_rec[c]);
         }
         b.d_groups = put(b.arena, dg);
@@ -1669,7 +1676,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         if (!xw.empty()) b.d_xwork = put(b.arena, xw);
         b.d_outputs = put(b.arena, dout);
         b.d_node_recs = put(b.arena, nr);
-        for (int k = 0; k < 2; ++k) b.d_branch_recs[k] = put(b.arena, br[k]);
+        for (int k = 0; k < 3; ++k) b.d_branch_recs[k] = put(b.arena, br[k]);
         for (int c = 0; c < N_COMP; ++c)
             if (c == C_SHUNT || c == C_SOURCE || c >= C_SYM_GEN) b.d_appl_recs[c] = put(b.arena, ar[c]);
         int max_levels = 0, max_levels_u = 0, max_levels_p = 0;
@@ -1729,7 +1736,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         b.d_bus_off = put(b.arena, bus_off);
         for (int c = 0; c < N_COMP; ++c) {
             size_t const rs = c == C_NODE ? (b.symmetric ? sizeof(NodeOutput<1>) : sizeof(NodeOutput<3>))
-                              : (c == C_LINE || c == C_TRANSFORMER)
+                              : (c == C_LINE || c == C_TRANSFORMER || c == C_LINK)
                                   ? (b.symmetric ? sizeof(BranchOutput<1>) : sizeof(BranchOutput<3>))
                                   : (b.symmetric ? sizeof(ApplianceOutput<1>) : sizeof(ApplianceOutput<3>));
             long long const slots = 64LL * record_tile(rs);
@@ -1750,7 +1757,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     // ---- output record buffers ----
     for (int c = 0; c < N_COMP; ++c) {
         size_t const rs = c == C_NODE ? (sym ? sizeof(NodeOutput<1>) : sizeof(NodeOutput<3>))
-                          : (c == C_LINE || c == C_TRANSFORMER)
+                          : (c == C_LINE || c == C_TRANSFORMER || c == C_LINK)
                               ? (sym ? sizeof(BranchOutput<1>) : sizeof(BranchOutput<3>))
                               : (sym ? sizeof(ApplianceOutput<1>) : sizeof(ApplianceOutput<3>));
         b.rec_size[c] = rs;  // the record buffer itself is taken by the first run that writes records
@@ -1941,6 +1948,9 @@ void Batch::run() {
     if (want[C_TRANSFORMER])
         launch_branch_records(P, d_outputs, d_branch_recs[1], d_rec_off[C_TRANSFORMER], G, rec_total[C_TRANSFORMER],
                               out[C_TRANSFORMER], column_set(C_TRANSFORMER), st);
+    if (want[C_LINK])
+        launch_branch_records(P, d_outputs, d_branch_recs[2], d_rec_off[C_LINK], G, rec_total[C_LINK], out[C_LINK],
+                              column_set(C_LINK), st);
     for (int c : {int(C_SHUNT), int(C_SOURCE), int(C_SYM_GEN), int(C_ASYM_GEN), int(C_SYM_LOAD), int(C_ASYM_LOAD)})
         if (want[c])
             launch_appliance_records(P, d_outputs, d_appl_recs[c], d_rec_off[c], G, rec_total[c], out[c],
@@ -1971,6 +1981,7 @@ void Batch::adjoint(double* grad_p, double* grad_q, double* grad_r, double* grad
     if (!ran) throw PowerGridError("the batch has not been run\n");
     if (!symmetric || linear || current)
         throw PowerGridError("the adjoint is implemented for symmetric Newton-Raphson batches\n");
+    if (!model.link.empty()) throw PowerGridError("the adjoint is implemented for grids without links\n");
     for (auto const& gp : groups)
         if (gp->is_union || gp->packed)
             throw PowerGridError("the adjoint needs every topology group to hold more than 32 scenarios\n");

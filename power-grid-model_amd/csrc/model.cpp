@@ -96,7 +96,8 @@ Tensor3 matmul3(Tensor3 const& a, Tensor3 const& b) {
 Idx Model::count(int comp) const {
     switch (comp) {
     case C_NODE: return n_node();
-    case C_LINE: return n_line();
+    case C_LINE: return Idx(line.size());
+    case C_LINK: return Idx(link.size());
     case C_TRANSFORMER: return Idx(trafo.size());
     case C_SHUNT: return Idx(shunt.size());
     case C_SOURCE: return Idx(source.size());
@@ -111,6 +112,7 @@ ID Model::comp_id(int comp, Idx i) const {
     switch (comp) {
     case C_NODE: return node[i].id;
     case C_LINE: return line[i].id;
+    case C_LINK: return link[i].id;
     case C_TRANSFORMER: return trafo[i].id;
     case C_SHUNT: return shunt[i].id;
     case C_SOURCE: return source[i].id;
@@ -191,7 +193,29 @@ Model build_model(PGM_ConstDataset const& in, double f) {
         s.y0s = 1.0 / cd(l.r0, l.x0) / base_y;
         s.y0sh = 2.0 * kPi * f * l.c0 / base_y * cd(l.tan0, 1.0);
         s.i_n = l.i_n;
+        s.base_i_to = s.base_i;
         m.line.push_back(l);
+        m.line_f.push_back(fi);
+        m.line_t.push_back(ti);
+        m.lstat.push_back(s);
+        m.base.line_fs.push_back(l.from_status != 0);
+        m.base.line_ts.push_back(l.to_status != 0);
+    });
+    // link.hpp:19-36: y_series = y_link = (1 + 1j) * 1e6 S at 10 kV in per unit, no shunt, tap 1, phase
+    // shift 0, loading 0; base currents from the rated voltages of its two nodes (no voltage check)
+    each(C_LINK, [&](char const* r) {
+        LinkInput l;
+        std::memcpy(&l, r, sizeof l);
+        reg(l.id, C_LINK, Idx(m.link.size()));
+        Idx const fi = nidx(l.from_node), ti = nidx(l.to_node);
+        double const g_link = 1e6 / (kBaseP3 / 10e3 / 10e3);
+        LineStatic s;
+        s.y1s = s.y0s = cd(g_link, g_link);
+        s.y1sh = s.y0sh = cd{};
+        s.base_i = kBaseP3 / m.node[fi].u_rated / kSqrt3;
+        s.base_i_to = kBaseP3 / m.node[ti].u_rated / kSqrt3;
+        s.i_n = std::numeric_limits<double>::infinity();
+        m.link.push_back(l);
         m.line_f.push_back(fi);
         m.line_t.push_back(ti);
         m.lstat.push_back(s);

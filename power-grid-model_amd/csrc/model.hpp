@@ -36,9 +36,11 @@ struct BranchY {
     std::array<Tensor3, 4> v{};
 };
 
+// lines, and behind them the links (y1 = y0 = y_link, no shunt; base currents of both ends, i_n infinite so
+// that the loading is 0)
 struct LineStatic {
     cd y1s, y1sh, y0s, y0sh;
-    double base_i, i_n;
+    double base_i, i_n, base_i_to;
 };
 struct TrafoStatic {
     double u1, u2, sn, tap_size, uk, pk, i0, p0, i0_zs, p0_zs, uk_min, uk_max, pk_min, pk_max;
@@ -67,6 +69,7 @@ struct Model {
     double freq = 50.0;
     std::vector<NodeInput> node;
     std::vector<LineInput> line;
+    std::vector<LinkInput> link;
     std::vector<TransformerInput> trafo;
     std::vector<ShuntInput> shunt;
     std::vector<SourceInput> source;
@@ -88,8 +91,9 @@ struct Model {
     ComponentState base;
 
     Idx n_node() const { return Idx(node.size()); }
-    Idx n_line() const { return Idx(line.size()); }
-    Idx n_branch() const { return Idx(line.size() + trafo.size()); }
+    // branch slots: lines, then links (line slots too), then transformers
+    Idx n_line() const { return Idx(line_f.size()); }
+    Idx n_branch() const { return Idx(line_f.size() + trafo.size()); }
     Idx n_lg() const { return Idx(lg_id.size()); }
     Idx count(int comp) const;
     ID comp_id(int comp, Idx i) const;
@@ -108,6 +112,13 @@ double branch_phase_shift(Model const& m, Idx branch);
 
 // lookup of an update row: component index, or throws IDNotFound / IDWrongType
 Idx find_component(Model const& m, int comp, ID id);
+// the state slot an update row acts on: a link's rows act on its line slot behind the lines
+inline void line_slot(Model const& m, int& comp, Idx& i) {
+    if (comp == C_LINK) {
+        comp = C_LINE;
+        i += Idx(m.line.size());
+    }
+}
 // apply one transformer_tap_regulator update record (transformer_tap_regulator.hpp:38-46: NaN / na keep
 // the current value)
 void apply_regulator_update(ComponentState& st, Idx i, TapRegulatorUpdate const& u);

@@ -28,6 +28,11 @@ struct LineInput {
     IntS from_status, to_status;
     double r1, x1, c1, tan1, r0, x0, c0, tan0, i_n;
 };
+// link: a branch of very high admittance between two nodes (component/link.hpp)
+struct LinkInput {
+    ID id, from_node, to_node;
+    IntS from_status, to_status;
+};
 struct TransformerInput {
     ID id, from_node, to_node;
     IntS from_status, to_status;
@@ -122,6 +127,7 @@ struct TapRegulatorOutput {
 
 static_assert(sizeof(NodeInput) == 16);
 static_assert(sizeof(LineInput) == 88);
+static_assert(sizeof(LinkInput) == 16);
 static_assert(sizeof(TransformerInput) == 168);
 static_assert(sizeof(SourceInput) == 56);
 static_assert(sizeof(SymLoadGenInput) == 32);
@@ -133,7 +139,8 @@ static_assert(sizeof(BranchOutput<1>) == 80 && sizeof(BranchOutput<3>) == 208);
 static_assert(sizeof(ApplianceOutput<1>) == 48 && sizeof(ApplianceOutput<3>) == 128);
 static_assert(sizeof(TapRegulatorInput) == 48 && sizeof(TapRegulatorUpdate) == 40 && sizeof(TapRegulatorOutput) == 8);
 
-// component slots, in the reference container order (all_components.hpp:37-40)
+// component slots, in the reference container order (all_components.hpp:37-40), except that the link
+// comes last: its rows live behind the lines in the branch arrays, so no slot before it moves
 enum Comp : int {
     C_NODE = 0,
     C_LINE,
@@ -144,10 +151,11 @@ enum Comp : int {
     C_ASYM_GEN,
     C_SYM_LOAD,
     C_ASYM_LOAD,
+    C_LINK,
     N_COMP
 };
-inline char const* const comp_names[N_COMP] = {"node",   "line",    "transformer", "shunt",    "source",
-                                               "sym_gen", "asym_gen", "sym_load",    "asym_load"};
+inline char const* const comp_names[N_COMP] = {"node",    "line",     "transformer", "shunt",     "source",
+                                               "sym_gen", "asym_gen", "sym_load",    "asym_load", "link"};
 // transformer_tap_regulator: not part of a math model (no slot above); it steers the tap changer
 inline char const* const kTapRegulatorName = "transformer_tap_regulator";
 constexpr int C_TAP_REGULATOR = N_COMP;  // slot of its ids in the id index

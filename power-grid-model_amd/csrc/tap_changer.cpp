@@ -392,9 +392,11 @@ void apply_rows(Model const& m, std::vector<RowSource> const& src, Idx s, TopoSt
             r.buf->read_record(i, rec.data());
             ID id;
             std::memcpy(&id, rec.data(), sizeof id);
-            Idx const idx = r.positional ? i - b0 : find_component(m, r.comp, id);
-            touched.emplace_back(r.comp, idx);
-            switch (r.comp) {
+            Idx idx = r.positional ? i - b0 : find_component(m, r.comp, id);
+            int comp = r.comp;
+            line_slot(m, comp, idx);
+            touched.emplace_back(comp, idx);
+            switch (comp) {
             case C_LINE: {
                 BranchUpdate u;
                 std::memcpy(&u, rec.data(), sizeof u);
@@ -463,6 +465,8 @@ struct TapChanger {
                     comp = C_TAP_REGULATOR;
                 else if (buf.meta->name == "line")
                     comp = C_LINE;
+                else if (buf.meta->name == "link")
+                    comp = C_LINK;
                 else if (buf.meta->name == "transformer")
                     comp = C_TRANSFORMER;
                 else if (buf.meta->name == "source")

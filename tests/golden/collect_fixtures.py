@@ -13,8 +13,8 @@ import shutil
 import sys
 from pathlib import Path
 
-SUPPORTED = {"node", "line", "transformer", "shunt", "source", "sym_gen", "asym_gen", "sym_load", "asym_load",
-             "transformer_tap_regulator"}
+SUPPORTED = {"node", "line", "link", "transformer", "shunt", "source", "sym_gen", "asym_gen", "sym_load",
+             "asym_load", "transformer_tap_regulator"}
 FILES = ["input.json", "update_batch.json", "sym_output.json", "asym_output.json", "sym_output_batch.json",
          "asym_output_batch.json", "params.json"]
 
@@ -40,6 +40,9 @@ def main(ref_root):
             continue
         # expected failures are kept for the tap changer only (its iteration limits)
         if "raises" in p and "tap_changing_strategy" not in p:
+            continue
+        # cases the reference itself does not match (xfail with an AssertionError) pin nothing
+        if p.get("xfail", {}).get("raises") == "AssertionError":
             continue
         comps = set()
         for f in FILES[:-1]:

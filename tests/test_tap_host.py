@@ -17,7 +17,7 @@ from power_grid_model_amd import PowerGridModel, initialize_array, tap_regulator
 FROM, TO = 0, 1
 
 
-def _grid(nodes, transformers, regulators, sources, lines=()):
+def _grid(nodes, transformers, regulators, sources, lines=(), links=()):
     """transformers: (id, from, to, tap_side); regulators: (id, transformer, control_side); sources: (id, node)."""
     node = initialize_array("input", "node", len(nodes))
     node["id"] = [n for n, _ in nodes]
@@ -43,6 +43,12 @@ def _grid(nodes, transformers, regulators, sources, lines=()):
             ln[k]["id"], ln[k]["from_node"], ln[k]["to_node"] = i, f, t
             ln[k]["from_status"] = ln[k]["to_status"] = 1
         data["line"] = ln
+    if links:
+        lk = initialize_array("input", "link", len(links))
+        for k, (i, f, t) in enumerate(links):
+            lk[k]["id"], lk[k]["from_node"], lk[k]["to_node"] = i, f, t
+            lk[k]["from_status"] = lk[k]["to_status"] = 1
+        data["link"] = lk
     return PowerGridModel(data)
 
 
@@ -53,6 +59,24 @@ def test_meshed_grid_low_priority_rank():
                   [(11, 0, 1, TO), (12, 1, 2, FROM), (13, 2, 3, FROM)],
                   [(41, 11, TO), (42, 12, TO), (43, 13, TO)], [(31, 0)], lines=[(21, 0, 2)])
     np.testing.assert_array_equal(tap_regulator_ranks(model), [0, 1, 0])
+
+
+def test_meshed_grid_through_a_link():
+    """The meshed grid above with the line 0 - 2 a link: every branch other than a regulated transformer is
+    an edge of weight 0 (tap_position_optimizer.hpp:240-251, 289), so the ranks are the same."""
+    model = _grid([(0, 10e3), (1, 10e3), (2, 10e3), (3, 10e3)],
+                  [(11, 0, 1, TO), (12, 1, 2, FROM), (13, 2, 3, FROM)],
+                  [(41, 11, TO), (42, 12, TO), (43, 13, TO)], [(31, 0)], links=[(21, 0, 2)])
+    np.testing.assert_array_equal(tap_regulator_ranks(model), [0, 1, 0])
+    upd = initialize_array("update", "link", 1)
+    upd["id"] = 21
+    upd["from_status"] = 0
+    model.update(update_data={"link": upd})
+    # without the link the chain is 11, 12, 13 in turn
+    np.testing.assert_array_equal(tap_regulator_ranks(model), [0, 1, 2])
+    upd["id"] = 12
+    with pytest.raises(pgm.PowerGridError, match="Wrong type"):
+        model.update(update_data={"link": upd})
 
 
 def test_transformer_into_itself():
