@@ -1218,15 +1218,12 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             if (r.all_na_ids && (buf.elements_per_scenario < 0 || buf.elements_per_scenario != m.count(comp)))
                 throw PowerGridError("Update data without IDs for component " + buf.meta->name +
                                      " has a different number of elements per scenario then input data!");
-            if (!r.all_na_ids && eps > 0 && update->batch_size > 0) {
-                bool ok = !ids_differ;
-                try {
-                    for (Idx i = 0; ok && i < eps; ++i) r.idx0.push_back(find_component(m, comp, id_at(i)));
-                } catch (PowerGridError const&) {
-                    ok = false;  // scenario 0 has an unknown id: rows are looked up one by one
-                }
-                r.same_ids = ok;
-                if (!ok) r.idx0.clear();
+            if (!r.all_na_ids && eps > 0 && update->batch_size > 0 && !ids_differ) {
+                // every scenario lists the same ids: they are resolved once, before any scenario runs, and
+                // an unknown id fails the whole call, not each scenario (job_adapter.hpp:113-124 with
+                // main_core/update.hpp:236-255: the sequence of an independent component)
+                for (Idx i = 0; i < eps; ++i) r.idx0.push_back(find_component(m, comp, id_at(i)));
+                r.same_ids = true;
             }
             readers.push_back(std::move(r));
         }

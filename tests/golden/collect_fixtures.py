@@ -4,7 +4,7 @@ Run once in the build container (the reference tree is not present on the GPU bo
     python tests/golden/collect_fixtures.py /root/reference
 A case is taken when its params.json lists a method this build runs (newton_raphson, linear,
 iterative_current, linear_current) and every component it uses is one this build supports; cases that
-expect an error are taken when they exercise the tap changer. The data files (input, update_batch, *_output*.json, params.json) are copied unchanged
+expect an error are taken when they exercise the tap changer or an unknown update id. The data files (input, update_batch, *_output*.json, params.json) are copied unchanged
 together with their .license sidecars (MPL-2.0).
 """
 
@@ -38,8 +38,9 @@ def main(ref_root):
         methods = methods if isinstance(methods, list) else [methods]
         if not ({"newton_raphson", "linear", "iterative_current", "linear_current"} & set(methods)):
             continue
-        # expected failures are kept for the tap changer only (its iteration limits)
-        if "raises" in p and "tap_changing_strategy" not in p:
+        # expected failures are kept for the tap changer (its iteration limits) and for update ids that do not
+        # exist
+        if "raises" in p and "tap_changing_strategy" not in p and p["raises"].get("raises") != "IDNotFound":
             continue
         # cases the reference itself does not match (xfail with an AssertionError) pin nothing
         if p.get("xfail", {}).get("raises") == "AssertionError":

@@ -375,7 +375,7 @@ def test_validation_case_tap_changer(method, name, sym, batch):
         compare_result(out, exp, params["rtol"], params["atol"], s)
 
 
-TAP_RAISE_CASES = list_cases(raises=True)
+TAP_RAISE_CASES = list_cases(tap=True, raises=True)
 
 
 @pytest.mark.parametrize("name,sym,batch", TAP_RAISE_CASES, ids=[c[0] for c in TAP_RAISE_CASES])
