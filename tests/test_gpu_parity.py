@@ -136,7 +136,7 @@ def test_validation_case_gpu_other_methods(method, name, sym, batch):
         compare_result(out, exp, params["rtol"], params["atol"], s)
 
 
-RAISE_CASES = list_cases(raises=True)
+RAISE_CASES = list_cases(raises=True) + [c for c in list_cases("linear", raises=True) if c not in list_cases(raises=True)]
 
 
 @pytest.mark.parametrize("name,sym,batch", RAISE_CASES, ids=[c[0] for c in RAISE_CASES])
