@@ -97,6 +97,7 @@ void PGM_get_indexer(PGM_Handle* handle, PGM_PowerGridModel const* model, char c
         int comp = -1;
         for (int c = 0; c < N_COMP; ++c)
             if (std::strcmp(component, comp_names[c]) == 0) comp = c;
+        if (std::strcmp(component, kTapRegulatorName) == 0) comp = C_TAP_REGULATOR;
         if (comp < 0) throw PowerGridError(std::string("unknown component ") + component);
         for (PGM_Idx i = 0; i < size; ++i) indexer[i] = find_component(model->model, comp, ids[i]);
     });

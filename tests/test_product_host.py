@@ -37,6 +37,7 @@ from power_grid_model_amd import (  # noqa: E402
     synthetic,
 )
 from power_grid_model_amd._lib import lib  # noqa: E402
+import power_grid_model_amd as pgm_errors  # noqa: E402
 
 
 def declared_symbols():
@@ -234,3 +235,28 @@ def test_interleaved_layout_of_switching_topologies():
         assert st["valid"] == 1
         checked += 1
     assert checked == 4
+
+
+# ------------------------------------------------------------------------------------------------
+# get_indexer over the reference's validation inputs (tests/unit/test_0Z_model_validation.py:150-157:
+# shuffled ids of every input component map back to their rows)
+# ------------------------------------------------------------------------------------------------
+def _golden_inputs():
+    from validation_utils import GOLDEN
+
+    return sorted(str(p.parent.relative_to(GOLDEN)) for p in GOLDEN.rglob("input.json"))
+
+
+@pytest.mark.parametrize("name", _golden_inputs())
+def test_get_indexer_on_reference_inputs(name):
+    from validation_utils import GOLDEN
+
+    input_data = pf.load_dataset(GOLDEN / name / "input.json")
+    model = PowerGridModel(input_data)
+    for comp, arr in input_data.items():
+        ids = arr["id"].copy()
+        np.random.default_rng(3).shuffle(ids)
+        idx = model.get_indexer(comp, ids)
+        assert (arr["id"][idx] == ids).all(), comp
+    with pytest.raises(pgm_errors.IDNotFound):
+        model.get_indexer("node", np.array([-12345], dtype=np.int32))
