@@ -114,6 +114,15 @@ def test_validation_case_gpu(name, sym, batch):
         out = {k: v.reshape(1, *v.shape) for k, v in out.items()}
     for s, exp in enumerate(expected):
         compare_result(out, exp, params["rtol"], params["atol"], s)
+    if not batch:
+        # test_0Z_model_validation.py:159-164: a node and source selection, as a list and as a set, returns
+        # exactly those components
+        for sel in (["node", "source"], {"node", "source"}):
+            part = model.calculate_power_flow(symmetric=sym, output_component_types=sel,
+                                              tap_changing_strategy=params.get("tap_changing_strategy", "disabled"))
+            assert set(part) == {"node", "source"}
+            for comp in part:
+                assert part[comp].tobytes() == out[comp][0].tobytes()
 
 
 METHOD_CASES = [(m, *c) for m in ("linear", "iterative_current", "linear_current") for c in list_cases(m)]
