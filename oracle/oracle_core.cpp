@@ -171,6 +171,10 @@ BranchParam<3> sym_to_asym(BranchParamSym const& p1, BranchParamSym const& p0) {
 struct LineParam {
     cd y1s, y1sh, y0s, y0sh;
     double base_i, i_n, base_i_to;
+    cd ratio{1.0, 0.0};  // generic branch: k e^(j theta)
+    double shift = 0.0;
+    double sn = 0.0;     // generic branch: rated power, NaN when not given
+    bool generic = false;
 };
 // line.hpp:26-36
 LineParam line_param(OrLineInput const& in, double f, double u) {
@@ -195,6 +199,23 @@ LineParam link_param(double u1, double u2) {
     p.base_i = kBaseP3 / u1 / kSqrt3;
     p.base_i_to = kBaseP3 / u2 / kSqrt3;
     p.i_n = std::numeric_limits<double>::infinity();
+    return p;
+}
+// generic_branch.hpp:24-60: y1 in per unit of the to side, ratio k e^(j theta) with k = 1 and theta = 0 when
+// not given (theta modulo 2 pi), loading max s / sn or 0 without sn
+LineParam generic_param(OrGenericBranchInput const& in, double u1, double u2) {
+    LineParam p;
+    p.base_i = kBaseP3 / u1 / kSqrt3;
+    p.base_i_to = kBaseP3 / u2 / kSqrt3;
+    double const base_y = p.base_i_to / (u2 / kSqrt3);
+    p.y1s = p.y0s = 1.0 / cd(in.r1, in.x1) / base_y;
+    p.y1sh = p.y0sh = cd(in.g1, in.b1) / base_y;
+    double const k = std::isnan(in.k) ? 1.0 : in.k;
+    p.shift = std::isnan(in.theta) ? 0.0 : std::fmod(in.theta, 2.0 * kPi);
+    p.ratio = k * std::exp(cd(0.0, p.shift));
+    p.i_n = kNan;
+    p.sn = in.sn;
+    p.generic = true;
     return p;
 }
 
@@ -1422,6 +1443,7 @@ struct Model {
     std::vector<OrNodeInput> node;
     std::vector<OrLineInput> line;
     std::vector<OrLinkInput> link;
+    std::vector<OrGenericBranchInput> gbranch;
     std::vector<OrTransformerInput> trafo;
     std::vector<OrShuntInput> shunt;
     std::vector<OrSourceInput> source;
@@ -1486,6 +1508,18 @@ Model build_model(OrConstDataset const& in, double f) {
         m.lparam.push_back(link_param(m.node[f_].u_rated, m.node[t_].u_rated));
         m.base.line_fs.push_back(l.from_status);
         m.base.line_ts.push_back(l.to_status);
+    }
+    auto const* gbs = static_cast<OrGenericBranchInput const*>(in.data[OR_GENERIC_BRANCH]);
+    for (Idx i = 0; i < in.count[OR_GENERIC_BRANCH]; ++i) {
+        auto const& g = gbs[i];
+        m.gbranch.push_back(g);
+        reg(g.id, OR_GENERIC_BRANCH, i);
+        Idx const f_ = node_idx(m, g.from_node), t_ = node_idx(m, g.to_node);
+        m.line_f.push_back(f_);
+        m.line_t.push_back(t_);
+        m.lparam.push_back(generic_param(g, m.node[f_].u_rated, m.node[t_].u_rated));
+        m.base.line_fs.push_back(g.from_status);
+        m.base.line_ts.push_back(g.to_status);
     }
     auto const* trs = static_cast<OrTransformerInput const*>(in.data[OR_TRANSFORMER]);
     for (Idx i = 0; i < in.count[OR_TRANSFORMER]; ++i) {
@@ -1600,10 +1634,10 @@ void apply_update(Model const& m, State& st, OrConstDataset const& upd, Idx scen
         return ch;
     };
     Idx cnt = 0;
-    for (int comp : {int(OR_LINE), int(OR_LINK)}) {
+    for (int comp : {int(OR_LINE), int(OR_LINK), int(OR_GENERIC_BRANCH)}) {
         cnt = upd.count[comp];
         auto const* lu = static_cast<OrBranchUpdate const*>(upd.data[comp]);
-        Idx const slot0 = comp == OR_LINE ? 0 : Idx(m.line.size());
+        Idx const slot0 = comp == OR_LINE ? 0 : comp == OR_LINK ? Idx(m.line.size()) : Idx(m.line.size() + m.link.size());
         for (Idx r = 0; r < cnt; ++r) {
             auto const& u = lu[scenario * cnt + r];
             Idx const i = slot0 + lookup(u.id, comp, r, cnt);
@@ -1700,7 +1734,8 @@ TopoCache build_topo_cache(Model const& m, State const& st) {
     TopoCache tc;
     std::vector<CompBranch> br;
     Idx const nl = m.n_lslot();
-    for (Idx i = 0; i < nl; ++i) br.push_back({m.line_f[i], m.line_t[i], st.line_fs[i] != 0, st.line_ts[i] != 0, 0.0});
+    for (Idx i = 0; i < nl; ++i)
+        br.push_back({m.line_f[i], m.line_t[i], st.line_fs[i] != 0, st.line_ts[i] != 0, m.lparam[i].shift});
     for (Idx i = 0; i < Idx(m.trafo.size()); ++i)
         br.push_back({m.tr_f[i], m.tr_t[i], st.tr_fs[i] != 0, st.tr_ts[i] != 0, m.tstatic[i].clock * kDeg30});
     std::vector<bool> son;
@@ -1773,9 +1808,10 @@ template <int P> MathModel<P> make_math_model(Model const& m, State const& st, T
             bool const fs = st.line_fs[b] != 0, ts = st.line_ts[b] != 0;
             auto const& lp = m.lparam[b];
             if (P == 1) {
-                auto const p = calc_param_y_sym(lp.y1s, lp.y1sh, 1.0, fs, ts);
+                auto const p = calc_param_y_sym(lp.y1s, lp.y1sh, lp.ratio, fs, ts);
                 for (int i = 0; i < 4; ++i) bp.v[i] = ten_sm<P>(p.v[i], p.v[i]);
             } else {
+                if (lp.generic) throw OtherError("The functionality is not implemented. asymmetric generic_branch\n");
                 auto const p1 = calc_param_y_sym(lp.y1s, lp.y1sh, 1.0, fs, ts);
                 auto const p0 = calc_param_y_sym(lp.y0s, lp.y0sh, 1.0, fs, ts);
                 auto const a = sym_to_asym(p1, p0);
@@ -1920,7 +1956,10 @@ template <int P> void write_outputs(Model const& m, State const& st, TopoCache c
         if (!out.data[comp]) return;
         for (Idx k = 0; k < count; ++k) {
             Idx const b = first + k;
-            int32_t const id = comp == OR_LINE ? m.line[k].id : comp == OR_LINK ? m.link[k].id : m.trafo[k].id;
+            int32_t const id = comp == OR_LINE            ? m.line[k].id
+                               : comp == OR_LINK           ? m.link[k].id
+                               : comp == OR_GENERIC_BRANCH ? m.gbranch[k].id
+                                                           : m.trafo[k].id;
             double bif, bit;
             if (comp != OR_TRANSFORMER) {
                 bif = m.lparam[b].base_i;
@@ -1957,7 +1996,10 @@ template <int P> void write_outputs(Model const& m, State const& st, TopoCache c
                 }
                 max_i = std::max(mi_f, mi_t);
                 double const max_s = std::max(sum_sf, sum_st);
-                loading = comp != OR_TRANSFORMER ? max_i / m.lparam[b].i_n : max_s / m.tstatic[k].sn;
+                if (comp == OR_GENERIC_BRANCH)
+                    loading = std::isnan(m.lparam[b].sn) ? 0.0 : max_s / m.lparam[b].sn;
+                else
+                    loading = comp != OR_TRANSFORMER ? max_i / m.lparam[b].i_n : max_s / m.tstatic[k].sn;
             }
             if (P == 1) {
                 auto& o = static_cast<OrBranchOutputSym*>(out.data[comp])[scn * count + k];
@@ -1994,6 +2036,7 @@ template <int P> void write_outputs(Model const& m, State const& st, TopoCache c
     };
     branch_out(OR_LINE, 0, Idx(m.line.size()));
     branch_out(OR_LINK, Idx(m.line.size()), Idx(m.link.size()));
+    branch_out(OR_GENERIC_BRANCH, Idx(m.line.size() + m.link.size()), Idx(m.gbranch.size()));
     branch_out(OR_TRANSFORMER, nl, Idx(m.trafo.size()));
     // appliances (appliance.hpp get_output)
     auto app_out = [&](int comp, Idx k, Idx count, int32_t id, double base_i, double dir, bool status, Idx g,
@@ -2133,7 +2176,8 @@ std::vector<std::vector<Idx>> rank_regulators(Model const& m, State const& st,
             edges.push_back({m.tr_t[t], m.tr_f[t], 0, -1});
         }
     }
-    for (Idx l = 0; l < m.n_lslot(); ++l) {
+    // lines and links; generic branches are no edges (tap_position_optimizer.hpp:289)
+    for (Idx l = 0; l < Idx(m.line.size() + m.link.size()); ++l) {
         if (!st.line_fs[l] || !st.line_ts[l]) continue;
         edges.push_back({m.line_f[l], m.line_t[l], 0, -1});
         edges.push_back({m.line_t[l], m.line_f[l], 0, -1});

@@ -28,6 +28,10 @@ typedef struct {
 typedef struct { int32_t id, from_node, to_node; int8_t from_status, to_status; } OrLinkInput;
 typedef struct {
     int32_t id, from_node, to_node; int8_t from_status, to_status;
+    double r1, x1, g1, b1, k, theta, sn;
+} OrGenericBranchInput;
+typedef struct {
+    int32_t id, from_node, to_node; int8_t from_status, to_status;
     double u1, u2, sn, uk, pk, i0, p0, i0_zero_sequence, p0_zero_sequence;
     int8_t winding_from, winding_to, clock, tap_side, tap_pos, tap_min, tap_max, tap_nom;
     double tap_size, uk_min, uk_max, pk_min, pk_max;
@@ -67,10 +71,11 @@ typedef struct {
 typedef struct { int32_t id; int8_t energized; double p, q, i, s, pf; } OrApplianceOutputSym;
 typedef struct { int32_t id; int8_t energized; double p[3], q[3], i[3], s[3], pf[3]; } OrApplianceOutputAsym;
 
-/* component slots, in the reference's container order (all_components.hpp:37-40), the link last */
+/* component slots, in the reference's container order (all_components.hpp:37-40), the link and the generic
+ * branch last */
 enum OrComp {
     OR_NODE = 0, OR_LINE, OR_TRANSFORMER, OR_SHUNT, OR_SOURCE, OR_SYM_GEN, OR_ASYM_GEN, OR_SYM_LOAD, OR_ASYM_LOAD,
-    OR_LINK, OR_N_COMP
+    OR_LINK, OR_GENERIC_BRANCH, OR_N_COMP
 };
 
 /* one dataset: per component a row buffer and a count (per scenario for update/output batches) */

@@ -59,7 +59,7 @@ template <int P> std::vector<PGM_MetaComponent> output_components() {
     using R = TapRegulatorOutput;
     auto reg = comp<R>(kTapRegulatorName, {mk("id", &R::id, PGM_int32), mk("energized", &R::energized, PGM_int8),
                                            mk("tap_pos", &R::tap_pos, PGM_int8)});
-    return {node,           branch("line"),  branch("link"),    branch("transformer"), appl("shunt"),
+    return {node,           branch("line"),  branch("link"),    branch("generic_branch"), branch("transformer"), appl("shunt"),
             appl("source"), appl("sym_gen"), appl("asym_gen"), appl("sym_load"),     appl("asym_load"),
             reg};
 }
@@ -69,6 +69,7 @@ std::vector<PGM_MetaDataset> build_meta() {
     using NI = NodeInput;
     using LI = LineInput;
     using LkI = LinkInput;
+    using GBI = GenericBranchInput;
     using TI = TransformerInput;
     using SI = ShuntInput;
     using SrcI = SourceInput;
@@ -100,6 +101,13 @@ std::vector<PGM_MetaDataset> build_meta() {
          comp<LkI>("link", {mk("id", &LkI::id, PGM_int32), mk("from_node", &LkI::from_node, PGM_int32),
                             mk("to_node", &LkI::to_node, PGM_int32), mk("from_status", &LkI::from_status, PGM_int8),
                             mk("to_status", &LkI::to_status, PGM_int8)}),
+         comp<GBI>("generic_branch",
+                   {mk("id", &GBI::id, PGM_int32), mk("from_node", &GBI::from_node, PGM_int32),
+                    mk("to_node", &GBI::to_node, PGM_int32), mk("from_status", &GBI::from_status, PGM_int8),
+                    mk("to_status", &GBI::to_status, PGM_int8), mk("r1", &GBI::r1, PGM_double),
+                    mk("x1", &GBI::x1, PGM_double), mk("g1", &GBI::g1, PGM_double), mk("b1", &GBI::b1, PGM_double),
+                    mk("k", &GBI::k, PGM_double), mk("theta", &GBI::theta, PGM_double),
+                    mk("sn", &GBI::sn, PGM_double)}),
          comp<TI>("transformer",
                   {mk("id", &TI::id, PGM_int32),
                    mk("from_node", &TI::from_node, PGM_int32),
@@ -171,6 +179,8 @@ std::vector<PGM_MetaDataset> build_meta() {
                            mk("to_status", &BU::to_status, PGM_int8)}),
          comp<BU>("link", {mk("id", &BU::id, PGM_int32), mk("from_status", &BU::from_status, PGM_int8),
                            mk("to_status", &BU::to_status, PGM_int8)}),
+         comp<BU>("generic_branch", {mk("id", &BU::id, PGM_int32), mk("from_status", &BU::from_status, PGM_int8),
+                                     mk("to_status", &BU::to_status, PGM_int8)}),
          comp<TU>("transformer", {mk("id", &TU::id, PGM_int32), mk("from_status", &TU::from_status, PGM_int8),
                                   mk("to_status", &TU::to_status, PGM_int8), mk("tap_pos", &TU::tap_pos, PGM_int8)}),
          comp<SU>("shunt", {mk("id", &SU::id, PGM_int32), mk("status", &SU::status, PGM_int8),

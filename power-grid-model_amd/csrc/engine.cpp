@@ -241,7 +241,7 @@ struct Group {
     DevGroup dg;
     DevOutput dout;
     NodeRec node_rec{};
-    BranchRec branch_rec[3]{};  // lines, transformers, links
+    BranchRec branch_rec[4]{};  // lines, transformers, links, generic branches
     ApplRec appl_rec[N_COMP]{};
     int8_t* load_status = nullptr;
     std::vector<double> level_entries, level_pivots;  // LU entries of the rows / pivots per level
@@ -534,22 +534,28 @@ void upload_group(Batch& b, Group& g) {
         int* br_from = put(A, p.branch_from);
         int* br_to = put(A, p.branch_to);
         double* br_y = put(A, complex_to_doubles(p.branch_y));
-        // kind 0: lines, 1: transformers, 2: links (the line slots behind the lines)
+        // kind 0: lines, 1: transformers, 2: links, 3: generic branches (2 and 3 are line slots behind the lines)
         Idx const n_real_line = Idx(m.line.size());
-        for (int kind = 0; kind < 3; ++kind) {
-            Idx const first = kind == 0 ? 0 : kind == 1 ? m.n_line() : n_real_line;
-            Idx const cnt = kind == 0 ? n_real_line : kind == 1 ? Idx(m.trafo.size()) : Idx(m.link.size());
+        for (int kind = 0; kind < 4; ++kind) {
+            Idx const first = kind == 0 ? 0 : kind == 1 ? m.n_line() : kind == 2 ? n_real_line : generic_slot0(m);
+            Idx const cnt = kind == 0   ? n_real_line
+                            : kind == 1 ? Idx(m.trafo.size())
+                            : kind == 2 ? Idx(m.link.size())
+                                        : Idx(m.gbranch.size());
             std::vector<int> ids, math;
             std::vector<double> bif, bit, rating;
             for (Idx k = 0; k < cnt; ++k) {
                 Idx const bidx = first + k;
-                ids.push_back(kind == 0 ? m.line[k].id : kind == 1 ? m.trafo[k].id : m.link[k].id);
+                ids.push_back(kind == 0 ? m.line[k].id
+                              : kind == 1 ? m.trafo[k].id
+                              : kind == 2 ? m.link[k].id
+                                          : m.gbranch[k].id);
                 Idx const gg = t.branch_group[bidx];
                 math.push_back(gg < 0 ? -1 : mm_branch_begin[gg] + int(t.branch_pos[bidx]));
                 if (kind != 1) {
                     bif.push_back(m.lstat[bidx].base_i);
                     bit.push_back(m.lstat[bidx].base_i_to);
-                    rating.push_back(m.lstat[bidx].i_n);
+                    rating.push_back(kind == 3 ? m.lstat[bidx].sn : m.lstat[bidx].i_n);
                 } else {
                     bif.push_back(m.tstat[k].base_i_from);
                     bit.push_back(m.tstat[k].base_i_to);
@@ -558,7 +564,7 @@ void upload_group(Batch& b, Group& g) {
             }
             g.branch_rec[kind] = BranchRec{int(cnt),    put(A, ids), put(A, math), br_from,        br_to,
                                            br_y,        nullptr,     nullptr,      put(A, bif),    put(A, bit),
-                                           put(A, rating), kind != 1 ? 1 : 0};
+                                           put(A, rating), kind == 0 || kind == 2 ? 1 : 0};
         }
     }
     {
@@ -852,7 +858,7 @@ void upload_union_lanes(Batch& b, Group& g) {
     g.dout.dead = d_dead;
     int* d_nfrom = put(A, nfrom);
     int* d_nto = put(A, nto);
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
         g.branch_rec[k].lane_status = d_lstat;
         g.branch_rec[k].branch_y_var = d_yvar;
         // sides are masked per lane by the side statuses
@@ -1350,8 +1356,8 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     bool const ids_resolved =
         std::all_of(readers.begin(), readers.end(), [](UpdateReader const& r) { return r.all_na_ids || r.same_ids; });
     bool const has_structural_update = std::any_of(readers.begin(), readers.end(), [](UpdateReader const& r) {
-        return r.comp == C_LINE || r.comp == C_LINK || r.comp == C_TRANSFORMER || r.comp == C_SHUNT ||
-               r.comp == C_SOURCE;
+        return r.comp == C_LINE || r.comp == C_LINK || r.comp == C_GENERIC_BRANCH || r.comp == C_TRANSFORMER ||
+               r.comp == C_SHUNT || r.comp == C_SOURCE;
     });
 
     // ---- group scenarios by their structural state ----
@@ -1655,13 +1661,13 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         std::vector<DevGroup> dg;
         std::vector<DevOutput> dout;
         std::vector<NodeRec> nr;
-        std::vector<BranchRec> br[3];
+        std::vector<BranchRec> br[4];
         std::vector<ApplRec> ar[N_COMP];
         for (auto& gp : b.groups) {
             dg.push_back(gp->dg);
             dout.push_back(gp->dout);
             nr.push_back(gp->node_rec);
-            for (int k = 0; k < 3; ++k) br[k].push_back(gp->branch_rec[k]);
+            for (int k = 0; k < 4; ++k) br[k].push_back(gp->branch_rec[k]);
             for (int c = 0; c < N_COMP; ++c) ar[c].push_back(gp->appl_rec[c]);
         }
         b.d_groups = put(b.arena, dg);
@@ -1673,7 +1679,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         if (!xw.empty()) b.d_xwork = put(b.arena, xw);
         b.d_outputs = put(b.arena, dout);
         b.d_node_recs = put(b.arena, nr);
-        for (int k = 0; k < 3; ++k) b.d_branch_recs[k] = put(b.arena, br[k]);
+        for (int k = 0; k < 4; ++k) b.d_branch_recs[k] = put(b.arena, br[k]);
         for (int c = 0; c < N_COMP; ++c)
             if (c == C_SHUNT || c == C_SOURCE || c >= C_SYM_GEN) b.d_appl_recs[c] = put(b.arena, ar[c]);
         int max_levels = 0, max_levels_u = 0, max_levels_p = 0;
@@ -1733,7 +1739,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         b.d_bus_off = put(b.arena, bus_off);
         for (int c = 0; c < N_COMP; ++c) {
             size_t const rs = c == C_NODE ? (b.symmetric ? sizeof(NodeOutput<1>) : sizeof(NodeOutput<3>))
-                              : (c == C_LINE || c == C_TRANSFORMER || c == C_LINK)
+                              : (c == C_LINE || c == C_TRANSFORMER || c == C_LINK || c == C_GENERIC_BRANCH)
                                   ? (b.symmetric ? sizeof(BranchOutput<1>) : sizeof(BranchOutput<3>))
                                   : (b.symmetric ? sizeof(ApplianceOutput<1>) : sizeof(ApplianceOutput<3>));
             long long const slots = 64LL * record_tile(rs);
@@ -1754,7 +1760,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     // ---- output record buffers ----
     for (int c = 0; c < N_COMP; ++c) {
         size_t const rs = c == C_NODE ? (sym ? sizeof(NodeOutput<1>) : sizeof(NodeOutput<3>))
-                          : (c == C_LINE || c == C_TRANSFORMER || c == C_LINK)
+                          : (c == C_LINE || c == C_TRANSFORMER || c == C_LINK || c == C_GENERIC_BRANCH)
                               ? (sym ? sizeof(BranchOutput<1>) : sizeof(BranchOutput<3>))
                               : (sym ? sizeof(ApplianceOutput<1>) : sizeof(ApplianceOutput<3>));
         b.rec_size[c] = rs;  // the record buffer itself is taken by the first run that writes records
@@ -1948,6 +1954,9 @@ void Batch::run() {
     if (want[C_LINK])
         launch_branch_records(P, d_outputs, d_branch_recs[2], d_rec_off[C_LINK], G, rec_total[C_LINK], out[C_LINK],
                               column_set(C_LINK), st);
+    if (want[C_GENERIC_BRANCH])
+        launch_branch_records(P, d_outputs, d_branch_recs[3], d_rec_off[C_GENERIC_BRANCH], G,
+                              rec_total[C_GENERIC_BRANCH], out[C_GENERIC_BRANCH], column_set(C_GENERIC_BRANCH), st);
     for (int c : {int(C_SHUNT), int(C_SOURCE), int(C_SYM_GEN), int(C_ASYM_GEN), int(C_SYM_LOAD), int(C_ASYM_LOAD)})
         if (want[c])
             launch_appliance_records(P, d_outputs, d_appl_recs[c], d_rec_off[c], G, rec_total[c], out[c],
@@ -1978,7 +1987,8 @@ void Batch::adjoint(double* grad_p, double* grad_q, double* grad_r, double* grad
     if (!ran) throw PowerGridError("the batch has not been run\n");
     if (!symmetric || linear || current)
         throw PowerGridError("the adjoint is implemented for symmetric Newton-Raphson batches\n");
-    if (!model.link.empty()) throw PowerGridError("the adjoint is implemented for grids without links\n");
+    if (!model.link.empty() || !model.gbranch.empty())
+        throw PowerGridError("the adjoint is implemented for grids without links and generic branches\n");
     for (auto const& gp : groups)
         if (gp->is_union || gp->packed)
             throw PowerGridError("the adjoint needs every topology group to hold more than 32 scenarios\n");

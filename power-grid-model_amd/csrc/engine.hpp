@@ -95,7 +95,7 @@ struct Batch {
     DevGroup* d_groups = nullptr;
     DevOutput* d_outputs = nullptr;
     NodeRec* d_node_recs = nullptr;
-    BranchRec* d_branch_recs[3] = {nullptr, nullptr, nullptr};  // lines, transformers, links
+    BranchRec* d_branch_recs[4] = {nullptr, nullptr, nullptr, nullptr};  // lines, transformers, links, generic
     ApplRec* d_appl_recs[N_COMP] = {};
     // work items per level (forward in increasing, backward in decreasing level order)
     std::vector<WorkItem*> d_items;
@@ -123,7 +123,7 @@ struct Batch {
     size_t rec_size[N_COMP] = {};
     // output selection (set_output): the components a run produces; a component given as attribute
     // columns writes only those columns, each [n_scenarios][count] on the device
-    bool want[N_COMP] = {true, true, true, true, true, true, true, true, true, true};
+    bool want[N_COMP] = {true, true, true, true, true, true, true, true, true, true, true};
     struct DevColumn {
         PGM_MetaAttribute const* attr;
         void* dev;

@@ -98,6 +98,7 @@ Idx Model::count(int comp) const {
     case C_NODE: return n_node();
     case C_LINE: return Idx(line.size());
     case C_LINK: return Idx(link.size());
+    case C_GENERIC_BRANCH: return Idx(gbranch.size());
     case C_TRANSFORMER: return Idx(trafo.size());
     case C_SHUNT: return Idx(shunt.size());
     case C_SOURCE: return Idx(source.size());
@@ -113,6 +114,7 @@ ID Model::comp_id(int comp, Idx i) const {
     case C_NODE: return node[i].id;
     case C_LINE: return line[i].id;
     case C_LINK: return link[i].id;
+    case C_GENERIC_BRANCH: return gbranch[i].id;
     case C_TRANSFORMER: return trafo[i].id;
     case C_SHUNT: return shunt[i].id;
     case C_SOURCE: return source[i].id;
@@ -138,7 +140,7 @@ IntS tap_limit(TrafoStatic const& t, IntS tap) {
 }
 
 double branch_phase_shift(Model const& m, Idx b) {
-    return b < m.n_line() ? 0.0 : m.tstat[b - m.n_line()].clock * kDeg30;
+    return b < m.n_line() ? m.lstat[b].shift : m.tstat[b - m.n_line()].clock * kDeg30;
 }
 
 Model build_model(PGM_ConstDataset const& in, double f) {
@@ -221,6 +223,34 @@ Model build_model(PGM_ConstDataset const& in, double f) {
         m.lstat.push_back(s);
         m.base.line_fs.push_back(l.from_status != 0);
         m.base.line_ts.push_back(l.to_status != 0);
+    });
+    // generic_branch.hpp:24-60: y1 in per unit of the to side, ratio k e^(j theta) (k NaN: 1, theta NaN: 0,
+    // theta taken modulo 2 pi), loading max s / sn (0 without sn); no zero-sequence model
+    each(C_GENERIC_BRANCH, [&](char const* r) {
+        GenericBranchInput g;
+        std::memcpy(&g, r, sizeof g);
+        reg(g.id, C_GENERIC_BRANCH, Idx(m.gbranch.size()));
+        Idx const fi = nidx(g.from_node), ti = nidx(g.to_node);
+        double const u1 = m.node[fi].u_rated, u2 = m.node[ti].u_rated;
+        LineStatic s;
+        s.base_i = kBaseP3 / u1 / kSqrt3;
+        s.base_i_to = kBaseP3 / u2 / kSqrt3;
+        double const base_y = s.base_i_to / (u2 / kSqrt3);
+        s.y1s = 1.0 / cd(g.r1, g.x1) / base_y;
+        s.y1sh = cd(g.g1, g.b1) / base_y;
+        s.y0s = s.y1s;
+        s.y0sh = s.y1sh;
+        double const k = std::isnan(g.k) ? 1.0 : g.k;
+        s.shift = std::isnan(g.theta) ? 0.0 : std::fmod(g.theta, 2.0 * kPi);
+        s.tap = k * std::exp(cd(0.0, s.shift));
+        s.i_n = std::numeric_limits<double>::infinity();
+        s.sn = std::isnan(g.sn) ? std::numeric_limits<double>::infinity() : g.sn;
+        m.gbranch.push_back(g);
+        m.line_f.push_back(fi);
+        m.line_t.push_back(ti);
+        m.lstat.push_back(s);
+        m.base.line_fs.push_back(g.from_status != 0);
+        m.base.line_ts.push_back(g.to_status != 0);
     });
     each(C_TRANSFORMER, [&](char const* r) {
         TransformerInput t;
@@ -398,10 +428,12 @@ BranchY branch_param(Model const& m, ComponentState const& st, Idx b, bool symme
     if (b < nl) {
         auto const& s = m.lstat[b];
         bool const fs = st.line_fs[b], ts = st.line_ts[b];
-        auto const p1 = pi_model(s.y1s, s.y1sh, 1.0, fs, ts);
+        auto const p1 = pi_model(s.y1s, s.y1sh, s.tap, fs, ts);
         if (symmetric) {
             for (int i = 0; i < 4; ++i) r.v[i].m[0] = p1[i];
         } else {
+            if (b >= generic_slot0(m))
+                throw PowerGridError("The functionality is not implemented. asymmetric generic_branch\n");
             auto const p0 = pi_model(s.y0s, s.y0sh, 1.0, fs, ts);
             for (int i = 0; i < 4; ++i) r.v[i] = sm_tensor((2.0 * p1[i] + p0[i]) / 3.0, (p0[i] - p1[i]) / 3.0);
         }

@@ -36,11 +36,14 @@ struct BranchY {
     std::array<Tensor3, 4> v{};
 };
 
-// lines, and behind them the links (y1 = y0 = y_link, no shunt; base currents of both ends, i_n infinite so
-// that the loading is 0)
+// lines, behind them the links (y1 = y0 = y_link, no shunt; base currents of both ends, i_n infinite so
+// that the loading is 0), then the generic branches (ratio k e^(j theta), loading max s / sn; symmetric only)
 struct LineStatic {
     cd y1s, y1sh, y0s, y0sh;
     double base_i, i_n, base_i_to;
+    cd tap{1.0, 0.0};
+    double shift = 0.0;
+    double sn = 0.0;  // generic branch: rated power (infinite when not given)
 };
 struct TrafoStatic {
     double u1, u2, sn, tap_size, uk, pk, i0, p0, i0_zs, p0_zs, uk_min, uk_max, pk_min, pk_max;
@@ -70,6 +73,7 @@ struct Model {
     std::vector<NodeInput> node;
     std::vector<LineInput> line;
     std::vector<LinkInput> link;
+    std::vector<GenericBranchInput> gbranch;
     std::vector<TransformerInput> trafo;
     std::vector<ShuntInput> shunt;
     std::vector<SourceInput> source;
@@ -91,7 +95,7 @@ struct Model {
     ComponentState base;
 
     Idx n_node() const { return Idx(node.size()); }
-    // branch slots: lines, then links (line slots too), then transformers
+    // branch slots: lines, then links and generic branches (line slots too), then transformers
     Idx n_line() const { return Idx(line_f.size()); }
     Idx n_branch() const { return Idx(line_f.size() + trafo.size()); }
     Idx n_lg() const { return Idx(lg_id.size()); }
@@ -117,8 +121,13 @@ inline void line_slot(Model const& m, int& comp, Idx& i) {
     if (comp == C_LINK) {
         comp = C_LINE;
         i += Idx(m.line.size());
+    } else if (comp == C_GENERIC_BRANCH) {
+        comp = C_LINE;
+        i += Idx(m.line.size() + m.link.size());
     }
 }
+// first line slot of the generic branches (they are no edges of the tap changer's ranking graph)
+inline Idx generic_slot0(Model const& m) { return Idx(m.line.size() + m.link.size()); }
 // apply one transformer_tap_regulator update record (transformer_tap_regulator.hpp:38-46: NaN / na keep
 // the current value)
 void apply_regulator_update(ComponentState& st, Idx i, TapRegulatorUpdate const& u);

@@ -33,6 +33,12 @@ struct LinkInput {
     ID id, from_node, to_node;
     IntS from_status, to_status;
 };
+// generic branch: a pi model with off-nominal ratio k and angle shift theta (component/generic_branch.hpp)
+struct GenericBranchInput {
+    ID id, from_node, to_node;
+    IntS from_status, to_status;
+    double r1, x1, g1, b1, k, theta, sn;
+};
 struct TransformerInput {
     ID id, from_node, to_node;
     IntS from_status, to_status;
@@ -128,6 +134,7 @@ struct TapRegulatorOutput {
 static_assert(sizeof(NodeInput) == 16);
 static_assert(sizeof(LineInput) == 88);
 static_assert(sizeof(LinkInput) == 16);
+static_assert(sizeof(GenericBranchInput) == 72);
 static_assert(sizeof(TransformerInput) == 168);
 static_assert(sizeof(SourceInput) == 56);
 static_assert(sizeof(SymLoadGenInput) == 32);
@@ -139,8 +146,9 @@ static_assert(sizeof(BranchOutput<1>) == 80 && sizeof(BranchOutput<3>) == 208);
 static_assert(sizeof(ApplianceOutput<1>) == 48 && sizeof(ApplianceOutput<3>) == 128);
 static_assert(sizeof(TapRegulatorInput) == 48 && sizeof(TapRegulatorUpdate) == 40 && sizeof(TapRegulatorOutput) == 8);
 
-// component slots, in the reference container order (all_components.hpp:37-40), except that the link
-// comes last: its rows live behind the lines in the branch arrays, so no slot before it moves
+// component slots, in the reference container order (all_components.hpp:37-40), except that the link and
+// the generic branch come last: their rows live behind the lines in the branch arrays, so no slot before
+// them moves
 enum Comp : int {
     C_NODE = 0,
     C_LINE,
@@ -152,10 +160,12 @@ enum Comp : int {
     C_SYM_LOAD,
     C_ASYM_LOAD,
     C_LINK,
+    C_GENERIC_BRANCH,
     N_COMP
 };
-inline char const* const comp_names[N_COMP] = {"node",    "line",     "transformer", "shunt",     "source",
-                                               "sym_gen", "asym_gen", "sym_load",    "asym_load", "link"};
+inline char const* const comp_names[N_COMP] = {"node",     "line",      "transformer", "shunt",
+                                               "source",   "sym_gen",   "asym_gen",    "sym_load",
+                                               "asym_load", "link",     "generic_branch"};
 // transformer_tap_regulator: not part of a math model (no slot above); it steers the tap changer
 inline char const* const kTapRegulatorName = "transformer_tap_regulator";
 constexpr int C_TAP_REGULATOR = N_COMP;  // slot of its ids in the id index

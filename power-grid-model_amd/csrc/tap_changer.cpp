@@ -135,7 +135,8 @@ Ranking rank_state(Model const& m, TopoState const& st) {
             adj[to].push_back({f, 0});
         }
     }
-    for (Idx l = 0; l < m.n_line(); ++l) {
+    // lines and links; generic branches are no edges (tap_position_optimizer.hpp:289)
+    for (Idx l = 0; l < generic_slot0(m); ++l) {
         if (!st.line_fs[l] || !st.line_ts[l]) continue;
         adj[m.line_f[l]].push_back({m.line_t[l], 0});
         adj[m.line_t[l]].push_back({m.line_f[l], 0});

@@ -13,7 +13,7 @@ import shutil
 import sys
 from pathlib import Path
 
-SUPPORTED = {"node", "line", "link", "transformer", "shunt", "source", "sym_gen", "asym_gen", "sym_load",
+SUPPORTED = {"node", "line", "link", "generic_branch", "transformer", "shunt", "source", "sym_gen", "asym_gen", "sym_load",
              "asym_load", "transformer_tap_regulator"}
 FILES = ["input.json", "update_batch.json", "sym_output.json", "asym_output.json", "sym_output_batch.json",
          "asym_output_batch.json", "params.json"]

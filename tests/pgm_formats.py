@@ -50,6 +50,7 @@ INPUT = {
         + [(n, _D) for n in ("r1", "x1", "c1", "tan1", "r0", "x0", "c0", "tan0", "i_n")]
     ),
     "link": _struct(_BRANCH_IN),
+    "generic_branch": _struct(_BRANCH_IN + [(n, _D) for n in ("r1", "x1", "g1", "b1", "k", "theta", "sn")]),
     "transformer": _struct(
         _BRANCH_IN
         + [(n, _D) for n in ("u1", "u2", "sn", "uk", "pk", "i0", "p0", "i0_zero_sequence", "p0_zero_sequence")]
@@ -73,6 +74,7 @@ INPUT = {
 UPDATE = {
     "line": _struct(_BASE + [("from_status", "i1"), ("to_status", "i1")]),
     "link": _struct(_BASE + [("from_status", "i1"), ("to_status", "i1")]),
+    "generic_branch": _struct(_BASE + [("from_status", "i1"), ("to_status", "i1")]),
     "transformer": _struct(_BASE + [("from_status", "i1"), ("to_status", "i1"), ("tap_pos", "i1")]),
     "shunt": _struct(_BASE + [("status", "i1")] + [(n, _D) for n in ("g1", "b1", "g0", "b0")]),
     "source": _struct(_BASE + [("status", "i1")] + [(n, _D) for n in ("u_ref", "u_ref_angle", "sk", "rx_ratio",
@@ -98,6 +100,7 @@ def _output(v):
         "node": node,
         "line": branch,
         "link": branch,
+        "generic_branch": branch,
         "transformer": branch,
         "shunt": appl,
         "source": appl,
@@ -114,9 +117,10 @@ ASYM_OUTPUT = _output(_D3)
 
 DATASETS = {"input": INPUT, "update": UPDATE, "sym_output": SYM_OUTPUT, "asym_output": ASYM_OUTPUT}
 
-# component order of the reference's container (all_components.hpp:37-40), the link last (the oracle's slots)
+# component order of the reference's container (all_components.hpp:37-40), the link and the generic branch last
+# (the oracle's slots)
 COMPONENT_ORDER = ["node", "line", "transformer", "shunt", "source", "sym_gen", "asym_gen", "sym_load", "asym_load",
-                   "link"]
+                   "link", "generic_branch"]
 # with the components outside the math model (the tap changer's regulators), in the same container order
 ALL_COMPONENTS = COMPONENT_ORDER + ["transformer_tap_regulator"]
 
