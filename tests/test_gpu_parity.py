@@ -303,10 +303,11 @@ def test_union_group_switching(precision):
         np.testing.assert_allclose(out["node"][f], ref["node"][f], rtol=PQ_RTOL, atol=floor, err_msg=f"node.{f}")
 
 
-def _link_case(n_scenarios=64, seed=10):
+def _link_case(n_scenarios=64, seed=10, generic=False):
     """meshed_1000 with 40 of its lines replaced by links (same nodes and statuses), and per scenario the
     side statuses of two links and two lines updated: links switch like lines, so the batch runs as a
-    union group of the base structure."""
+    union group of the base structure. generic: 20 more lines become generic branches (the line's r1, x1,
+    ratio 0.98-1.02, angle shift up to 0.01 rad, sn on half of them), two of them switched per scenario."""
     from power_grid_model_amd import initialize_array
 
     case = synthetic.meshed_1000(n_scenarios=n_scenarios)
@@ -320,8 +321,24 @@ def _link_case(n_scenarios=64, seed=10):
         link[f] = line[f][mask]
     inp["line"] = line[~mask]
     inp["link"] = link
+    comps = [("link", inp["link"]["id"]), ("line", inp["line"]["id"])]
+    if generic:
+        rest = inp["line"]
+        gmask = np.zeros(len(rest), dtype=bool)
+        gmask[rng.choice(len(rest), 20, replace=False)] = True
+        gb = initialize_array("input", "generic_branch", 20)
+        for f in ("id", "from_node", "to_node", "from_status", "to_status", "r1", "x1"):
+            gb[f] = rest[f][gmask]
+        gb["g1"] = 0.0
+        gb["b1"] = 1e-6
+        gb["k"] = rng.uniform(0.98, 1.02, 20)
+        gb["theta"] = rng.uniform(-0.01, 0.01, 20)
+        gb["sn"] = np.where(np.arange(20) % 2 == 0, 5e6, np.nan)
+        inp["line"] = rest[~gmask]
+        inp["generic_branch"] = gb
+        comps = [("link", inp["link"]["id"]), ("line", inp["line"]["id"]), ("generic_branch", gb["id"])]
     upd = {"sym_load": case.update["sym_load"]}
-    for comp, pool in (("link", inp["link"]["id"]), ("line", inp["line"]["id"])):
+    for comp, pool in comps:
         u = initialize_array("update", comp, (n_scenarios, 2))
         for s in range(n_scenarios):
             u["id"][s] = rng.choice(pool, 2, replace=False)
@@ -332,7 +349,8 @@ def _link_case(n_scenarios=64, seed=10):
 
 
 def test_links_with_switching():
-    """Links (link.hpp: y_link = 1e8 (1 + 1j) pu, no shunt, loading 0) against the oracle. The link
+    """Links (link.hpp: y_link = 1e8 (1 + 1j) pu, no shunt, loading 0) and generic branches
+    (generic_branch.hpp: ratio k e^(j theta), loading by sn) against the oracle. The link
     admittance raises the condition number of the Jacobian about 1e6-fold over a grid of lines, so an FP64
     solve carries about 1e-9 pu of rounding in the voltages (still inside the 1e-8 bar), and every flow
     carries that rounding times its branch admittance: about 1e-8 of the batch's largest flow. Bars here:
@@ -340,7 +358,7 @@ def test_links_with_switching():
     of that attribute; iteration counts within one of the oracle's (the convergence test compares a
     deviation with 1e-8, the size of that rounding). A mixed-precision request on a model with links is
     factored in FP64 and gives the FP64 results bit for bit."""
-    inp, upd = _link_case()
+    inp, upd = _link_case(generic=True)
     model = PowerGridModel(inp)
     batch = DeviceBatch(model, upd)
     batch.run()
@@ -356,6 +374,8 @@ def test_links_with_switching():
     assert np.max(np.abs(np.where(rnode["energized"] == 1, da, 0))) < ANGLE_TOL
     for comp, fields in (("line", ("p_from", "q_from", "p_to", "q_to", "i_from", "i_to", "s_from", "s_to")),
                          ("link", ("p_from", "q_from", "p_to", "q_to", "i_from", "i_to", "s_from", "s_to")),
+                         ("generic_branch", ("p_from", "q_from", "p_to", "q_to", "i_from", "i_to", "s_from", "s_to",
+                                             "loading")),
                          ("sym_load", ("p", "q")), ("source", ("p", "q"))):
         a, r = out[comp], ref[comp]
         assert (a["id"] == r["id"]).all() and (a["energized"] == r["energized"]).all()
@@ -367,7 +387,7 @@ def test_links_with_switching():
     mixed.run()
     out32 = mixed.fetch()
     assert (mixed.iterations() == batch.iterations()).all()
-    for comp in ("node", "line", "link"):
+    for comp in ("node", "line", "link", "generic_branch"):
         assert out32[comp].tobytes() == out[comp].tobytes()
 
 
