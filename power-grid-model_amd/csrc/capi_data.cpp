@@ -59,9 +59,11 @@ template <int P> std::vector<PGM_MetaComponent> output_components() {
     using R = TapRegulatorOutput;
     auto reg = comp<R>(kTapRegulatorName, {mk("id", &R::id, PGM_int32), mk("energized", &R::energized, PGM_int8),
                                            mk("tap_pos", &R::tap_pos, PGM_int8)});
+    using F = FaultOutput;
+    auto fault = comp<F>(kFaultName, {mk("id", &F::id, PGM_int32), mk("energized", &F::energized, PGM_int8)});
     return {node,           branch("line"),  branch("link"),    branch("generic_branch"), branch("transformer"), appl("shunt"),
             appl("source"), appl("sym_gen"), appl("asym_gen"), appl("sym_load"),     appl("asym_load"),
-            reg};
+            reg,            fault};
 }
 
 std::vector<PGM_MetaDataset> build_meta() {
@@ -70,6 +72,7 @@ std::vector<PGM_MetaDataset> build_meta() {
     using LI = LineInput;
     using LkI = LinkInput;
     using GBI = GenericBranchInput;
+    using FI = FaultInput;
     using TI = TransformerInput;
     using SI = ShuntInput;
     using SrcI = SourceInput;
@@ -150,6 +153,10 @@ std::vector<PGM_MetaDataset> build_meta() {
                                mk("rx_ratio", &SrcI::rx_ratio, PGM_double),
                                mk("z01_ratio", &SrcI::z01_ratio, PGM_double)}),
          lg_in("sym_gen"), alg_in("asym_gen"), lg_in("sym_load"), alg_in("asym_load"),
+         comp<FI>(kFaultName, {mk("id", &FI::id, PGM_int32), mk("status", &FI::status, PGM_int8),
+                               mk("fault_type", &FI::fault_type, PGM_int8), mk("fault_phase", &FI::fault_phase, PGM_int8),
+                               mk("fault_object", &FI::fault_object, PGM_int32), mk("r_f", &FI::r_f, PGM_double),
+                               mk("x_f", &FI::x_f, PGM_double)}),
          comp<RI>(kTapRegulatorName,
                   {mk("id", &RI::id, PGM_int32), mk("regulated_object", &RI::regulated_object, PGM_int32),
                    mk("status", &RI::status, PGM_int8), mk("control_side", &RI::control_side, PGM_int8),

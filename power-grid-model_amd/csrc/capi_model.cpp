@@ -98,6 +98,7 @@ void PGM_get_indexer(PGM_Handle* handle, PGM_PowerGridModel const* model, char c
         for (int c = 0; c < N_COMP; ++c)
             if (std::strcmp(component, comp_names[c]) == 0) comp = c;
         if (std::strcmp(component, kTapRegulatorName) == 0) comp = C_TAP_REGULATOR;
+        if (std::strcmp(component, kFaultName) == 0) comp = C_FAULT;
         if (comp < 0) throw PowerGridError(std::string("unknown component ") + component);
         for (PGM_Idx i = 0; i < size; ++i) indexer[i] = find_component(model->model, comp, ids[i]);
     });
@@ -110,6 +111,7 @@ void PGM_calculate(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_Options co
             throw PowerGridError(
                 "If batch_dataset is provided. Both batch_dataset and output_dataset should be a batch!\n");
         if (batch_dataset && batch_dataset->batch_size == 0) return;
+        write_null_fault_output(model->model, *output_dataset);
         // automatic tap changing (main_model_impl.hpp:318-346, optimizer/tap_position_optimizer.hpp)
         if (opt->tap_changing_strategy != 0) {
             calculate_with_tap_changer(model->model, *opt, *output_dataset, batch_dataset, handle);

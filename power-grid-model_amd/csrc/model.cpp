@@ -156,7 +156,7 @@ Model build_model(PGM_ConstDataset const& in, double f) {
         }
     };
     for (auto const& b : in.buffers) {
-        bool known = b.meta->name == kTapRegulatorName;
+        bool known = b.meta->name == kTapRegulatorName || b.meta->name == kFaultName;
         for (int c = 0; c < N_COMP; ++c) known |= b.meta->name == comp_names[c];
         if (!known) throw PowerGridError("component " + b.meta->name + " is not supported by this build");
     }
@@ -376,6 +376,17 @@ Model build_model(PGM_ConstDataset const& in, double f) {
             ++cnt;
         });
         m.lg_count[k] = cnt;
+    }
+    // faults (main_core/input.hpp:164-167: the faulted object must exist and be a node)
+    if (auto const* buf = in.find(kFaultName)) {
+        for (Idx i = 0; i < buf->total_elements; ++i) {
+            FaultInput fl;
+            buf->read_record(i, rec.data());
+            std::memcpy(&fl, rec.data(), sizeof fl);
+            reg(fl.id, C_FAULT, Idx(m.fault.size()));
+            (void)find_component(m, C_NODE, fl.fault_object);
+            m.fault.push_back(fl);
+        }
     }
     // transformer tap regulators (main_core/input.hpp:168-213, 235-241): the regulated object is a
     // transformer, the control side one of its two sides, each transformer regulated at most once

@@ -74,6 +74,7 @@ struct Model {
     std::vector<LineInput> line;
     std::vector<LinkInput> link;
     std::vector<GenericBranchInput> gbranch;
+    std::vector<FaultInput> fault;  // kept for the output; a power flow does not use it
     std::vector<TransformerInput> trafo;
     std::vector<ShuntInput> shunt;
     std::vector<SourceInput> source;

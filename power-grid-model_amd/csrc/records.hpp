@@ -73,6 +73,14 @@ struct TapRegulatorInput {
     double u_set, u_band, line_drop_compensation_r, line_drop_compensation_x;
 };
 
+// fault: input of short-circuit calculations; a power flow only checks it and reports it not energised
+struct FaultInput {
+    ID id;
+    IntS status, fault_type, fault_phase;
+    ID fault_object;
+    double r_f, x_f;
+};
+
 // ---------- update ----------
 struct BranchUpdate {
     ID id;
@@ -125,6 +133,11 @@ template <int P> struct ApplianceOutput {
     IntS energized;
     double p[P], q[P], i[P], s[P], pf[P];
 };
+// fault output of a power flow (fault.hpp:70-78): id and energized 0
+struct FaultOutput {
+    ID id;
+    IntS energized;
+};
 // the same record for symmetric and asymmetric output
 struct TapRegulatorOutput {
     ID id;
@@ -134,6 +147,7 @@ struct TapRegulatorOutput {
 static_assert(sizeof(NodeInput) == 16);
 static_assert(sizeof(LineInput) == 88);
 static_assert(sizeof(LinkInput) == 16);
+static_assert(sizeof(FaultInput) == 32 && sizeof(FaultOutput) == 8);
 static_assert(sizeof(GenericBranchInput) == 72);
 static_assert(sizeof(TransformerInput) == 168);
 static_assert(sizeof(SourceInput) == 56);
@@ -169,5 +183,7 @@ inline char const* const comp_names[N_COMP] = {"node",     "line",      "transfo
 // transformer_tap_regulator: not part of a math model (no slot above); it steers the tap changer
 inline char const* const kTapRegulatorName = "transformer_tap_regulator";
 constexpr int C_TAP_REGULATOR = N_COMP;  // slot of its ids in the id index
+inline char const* const kFaultName = "fault";
+constexpr int C_FAULT = N_COMP + 1;  // slot of fault ids in the id index
 
 }  // namespace pgm_amd

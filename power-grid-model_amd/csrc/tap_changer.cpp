@@ -1020,6 +1020,17 @@ void calculate_with_tap_changer(Model const& model, PGM_Options const& opt, PGM_
     tc.finish(output, handle);
 }
 
+void write_null_fault_output(Model const& m, PGM_MutableDataset const& output) {
+    ComponentBuffer const* fb = output.find(kFaultName);
+    if (!fb) return;
+    Idx const n = Idx(m.fault.size());
+    for (Idx s = 0; s < output.batch_size; ++s)
+        for (Idx i = fb->begin(s); i < fb->end(s) && i - fb->begin(s) < n; ++i) {
+            FaultOutput const o{m.fault[i - fb->begin(s)].id, 0};
+            fb->write_record(i, &o);
+        }
+}
+
 void write_null_regulator_output(Model const& m, PGM_MutableDataset const& output) {
     ComponentBuffer const* rb = output.find(kTapRegulatorName);
     if (!rb) return;

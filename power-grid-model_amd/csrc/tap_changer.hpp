@@ -21,6 +21,8 @@ void calculate_with_tap_changer(Model const& model, PGM_Options const& opt, PGM_
 // transformer_tap_regulator output without tap changing: every regulator reports no tap position
 // (main_core/output.hpp:381-396 with an empty optimizer output)
 void write_null_regulator_output(Model const& model, PGM_MutableDataset const& output);
+// fault output of a power flow: every fault reported not energised (fault.hpp:70-78)
+void write_null_fault_output(Model const& model, PGM_MutableDataset const& output);
 
 // rank of each regulator in the base state: 0 for transformers next to a source, growing with the number
 // of regulated transformers in between; a regulator whose control side is also reached through a closer

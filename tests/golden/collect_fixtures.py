@@ -14,7 +14,7 @@ import sys
 from pathlib import Path
 
 SUPPORTED = {"node", "line", "link", "generic_branch", "transformer", "shunt", "source", "sym_gen", "asym_gen", "sym_load",
-             "asym_load", "transformer_tap_regulator"}
+             "asym_load", "transformer_tap_regulator", "fault"}
 FILES = ["input.json", "update_batch.json", "sym_output.json", "asym_output.json", "sym_output_batch.json",
          "asym_output_batch.json", "params.json"]
 

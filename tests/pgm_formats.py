@@ -65,6 +65,8 @@ INPUT = {
     "asym_gen": _struct(_APPL_IN + [("type", "i1"), ("p_specified", _D3), ("q_specified", _D3)]),
     "sym_load": _struct(_APPL_IN + [("type", "i1"), ("p_specified", _D), ("q_specified", _D)]),
     "asym_load": _struct(_APPL_IN + [("type", "i1"), ("p_specified", _D3), ("q_specified", _D3)]),
+    "fault": _struct(_BASE + [("status", "i1"), ("fault_type", "i1"), ("fault_phase", "i1"), ("fault_object", "i4"),
+                              ("r_f", _D), ("x_f", _D)]),
     "transformer_tap_regulator": _struct(
         _BASE + [("regulated_object", "i4"), ("status", "i1"), ("control_side", "i1")]
         + [(n, _D) for n in ("u_set", "u_band", "line_drop_compensation_r", "line_drop_compensation_x")]
@@ -109,6 +111,7 @@ def _output(v):
         "sym_load": appl,
         "asym_load": appl,
         "transformer_tap_regulator": _struct(base + [("tap_pos", "i1")]),
+        "fault": _struct(base),
     }
 
 
@@ -122,7 +125,7 @@ DATASETS = {"input": INPUT, "update": UPDATE, "sym_output": SYM_OUTPUT, "asym_ou
 COMPONENT_ORDER = ["node", "line", "transformer", "shunt", "source", "sym_gen", "asym_gen", "sym_load", "asym_load",
                    "link", "generic_branch"]
 # with the components outside the math model (the tap changer's regulators), in the same container order
-ALL_COMPONENTS = COMPONENT_ORDER + ["transformer_tap_regulator"]
+ALL_COMPONENTS = COMPONENT_ORDER + ["transformer_tap_regulator", "fault"]
 
 
 def null_value(fmt: np.dtype):
