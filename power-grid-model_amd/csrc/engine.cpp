@@ -1166,7 +1166,19 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             UpdateReader r;
             r.buf = &buf;
             r.comp = comp;
-            Idx const eps = buf.elements_per_scenario;
+            // elements per scenario of a uniform buffer: a dense one, or an indptr one whose scenarios all hold
+            // the same number of rows (dataset.hpp:334-360 is_uniform / uniform_elements_per_scenario); -1
+            // otherwise
+            Idx eps = buf.elements_per_scenario;
+            Idx row0 = 0;  // first row of scenario 0
+            if (eps < 0 && buf.indptr && update->batch_size > 0) {
+                Idx const k = buf.indptr[1] - buf.indptr[0];
+                bool uniform = true;
+                for (Idx s = 1; s < update->batch_size && uniform; ++s)
+                    uniform = buf.indptr[s + 1] - buf.indptr[s] == k;
+                eps = uniform ? k : -1;
+                row0 = buf.indptr[0];
+            }
             // one pass over the id column (several host threads): NA ids, and whether every scenario
             // repeats scenario 0's ids
             char const* idcol = nullptr;
@@ -1198,7 +1210,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
                     ID const id = id_at(i);
                     na = na || id == na_IntID;
                     num = num || id != na_IntID;
-                    if (eps > 0 && i >= eps && !diff) diff = id != id_at(i % eps);
+                    if (eps > 0 && i >= row0 + eps && !diff) diff = id != id_at(row0 + (i - row0) % eps);
                 }
                 t_na[t] = na;
                 t_num[t] = num;
@@ -1221,14 +1233,14 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
                 throw PowerGridError("IDs contain both numbers and NANs for component " + buf.meta->name +
                                      " in update data!");
             r.all_na_ids = any_na && !any_num && total > 0;
-            if (r.all_na_ids && (buf.elements_per_scenario < 0 || buf.elements_per_scenario != m.count(comp)))
+            if (r.all_na_ids && (eps < 0 || eps != m.count(comp)))
                 throw PowerGridError("Update data without IDs for component " + buf.meta->name +
                                      " has a different number of elements per scenario then input data!");
             if (!r.all_na_ids && eps > 0 && update->batch_size > 0 && !ids_differ) {
                 // every scenario lists the same ids: they are resolved once, before any scenario runs, and
                 // an unknown id fails the whole call, not each scenario (job_adapter.hpp:113-124 with
                 // main_core/update.hpp:236-255: the sequence of an independent component)
-                for (Idx i = 0; i < eps; ++i) r.idx0.push_back(find_component(m, comp, id_at(i)));
+                for (Idx i = 0; i < eps; ++i) r.idx0.push_back(find_component(m, comp, id_at(row0 + i)));
                 r.same_ids = true;
             }
             readers.push_back(std::move(r));

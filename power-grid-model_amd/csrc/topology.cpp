@@ -176,7 +176,8 @@ Topology build_topology(Model const& m, ComponentState const& st, std::vector<ch
     std::vector<Idx> target(deg[nn]);
     // phase shift along an edge: theta_to = theta_from - shift for the branch's from->to direction
     std::vector<double> edge_shift(deg[nn]);
-    auto shift = [&](Idx b) { return b < nl ? 0.0 : double(m.tstat[b - nl].clock) * kDeg30; };
+    // line slots carry the generic branches' angle theta (0 for lines and links), transformer slots their clock
+    auto shift = [&](Idx b) { return branch_phase_shift(m, b); };
     {
         std::vector<Idx> fillp(deg.begin(), deg.end() - 1);
         for (Idx b = 0; b < nb; ++b)

@@ -54,17 +54,19 @@ class PowerFlowLoss(torch.autograd.Function):
                 max_iterations=20):
         args = [t.detach().cpu().to(torch.float64).numpy() for t in (load_p, load_q, line_r1, line_x1)]
         loss, batch = _solve(input_data, *args, factor_precision, error_tolerance, max_iterations)
-        grads = batch.adjoint()
-        lg, ln = grads["sym_load"], grads["line"]
-        saved = [torch.from_numpy(np.ascontiguousarray(a)) for a in
-                 (lg["p_specified"], lg["q_specified"], ln["r1"], ln["x1"])]
-        ctx.save_for_backward(*saved)
-        ctx.dtype = load_p.dtype
-        return torch.from_numpy(loss).to(load_p.dtype)
+        # the adjoint runs in backward only (a forward without backward costs one solve); the batch keeps the
+        # converged state on the device until then
+        ctx.batch = batch
+        ctx.inputs = [(t.dtype, t.device) for t in (load_p, load_q, line_r1, line_x1)]
+        return torch.from_numpy(loss).to(device=load_p.device, dtype=load_p.dtype)
 
     @staticmethod
     def backward(ctx, grad_loss):
-        gp, gq, gr, gx = ctx.saved_tensors
-        w = grad_loss.to(torch.float64)[:, None]
+        grads = ctx.batch.adjoint()
+        ctx.batch = None
+        lg, ln = grads["sym_load"], grads["line"]
+        gp, gq, gr, gx = (torch.from_numpy(np.ascontiguousarray(a)) for a in
+                          (lg["p_specified"], lg["q_specified"], ln["r1"], ln["x1"]))
+        w = grad_loss.detach().cpu().to(torch.float64)[:, None]
         out = [w * gp, w * gq, (w * gr).sum(0), (w * gx).sum(0)]
-        return tuple(o.to(ctx.dtype) for o in out) + (None, None, None, None)
+        return tuple(o.to(device=dev, dtype=dt) for o, (dt, dev) in zip(out, ctx.inputs)) + (None, None, None, None)

@@ -195,3 +195,141 @@ def test_adjoint_on_meshed_grid_with_fill_ins():
             vals.append(val[0])
         fd.append(_richardson(*vals, h))
     _close(grads["line"]["x1"][scn, lines], np.array(fd), "x1")
+
+
+# ------------------------------------------------------------------------------------------------
+# the uneven-feeder workload at full width: 64 feeders of 50-2000 nodes, gradients against the oracle
+# ------------------------------------------------------------------------------------------------
+FEEDER_OUTPUT = {"node": ["u_pu", "u_angle", "energized"], "line": ["p_from", "p_to"]}
+
+
+def _feeder_maps(case):
+    sizes = case.info["feeder_sizes"]
+    feeder_of_node = np.repeat(np.arange(len(sizes)), sizes)
+    inp = case.input
+    return feeder_of_node, feeder_of_node[inp["line"]["from_node"]], feeder_of_node[inp["sym_load"]["node"]]
+
+
+def _feeder_losses(node_out, line_tab, feeder_of_node, feeder_of_line, n_feeder):
+    """Per scenario and feeder, L_f = sum over its energised nodes (u_pu - 1)^2 + its line losses in MW.
+    A line's p_from + p_to equals g_s |u_from - u_to|^2 when its shunt is lossless (tan1 = 0 here); the
+    losses are formed that way, from the node voltages, because the sum of the two large end flows loses
+    about 1e-13 MW per line to cancellation, which swamps the finite differences."""
+    u, en = node_out["u_pu"], node_out["energized"]
+    uc = u * np.exp(1j * node_out["u_angle"])
+    z_base = (10.5e3 ** 2) / 1e6
+    g = (1.0 / ((line_tab["r1"] + 1j * line_tab["x1"]) / z_base)).real
+    loss = g * np.abs(uc[:, line_tab["from_node"]] - uc[:, line_tab["to_node"]]) ** 2
+    nodes = np.where(en == 1, (u - 1.0) ** 2, 0.0)
+    return np.stack([np.bincount(feeder_of_node, weights=nodes[s], minlength=n_feeder) +
+                     np.bincount(feeder_of_line, weights=loss[s], minlength=n_feeder) for s in range(u.shape[0])])
+
+
+def _pick(owner, n_feeder, k):
+    """The first k elements of every feeder: array (k, n_feeder) of element indices."""
+    return np.stack([np.flatnonzero(owner == f)[:k] for f in range(n_feeder)], axis=1)
+
+
+FD_STEPS = (1e-1, 5e-2)
+MULTS = (1.0, -1.0, 2.0, -2.0)
+
+
+def _fd_from(vals, steps):
+    """Richardson-extrapolated central differences from L at (+h, -h, +2h, -2h)."""
+    return _richardson(vals[0], vals[1], vals[2], vals[3], steps)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_adjoint_feeders_64_against_oracle_finite_differences(precision):
+    """feeders_64 at full width (64 feeders, 31.8k buses, 40 scenarios, solved to 1e-8 pu): the adjoint's
+    gradients by 8 load p, 8 load q, 8 line r1 and 8 line x1 of every feeder (32 per feeder, 2048 in all) in
+    scenario 2, against finite differences of the CPU oracle solved to 1e-12 pu in FP64 (an FP64 reference
+    gradient independent of the GPU solver), and the load gradients also against finite differences of the
+    GPU solver.
+
+    Finite differences: a parameter of one feeder changes that feeder's L_f only (the feeders are separate
+    math models), so the j-th parameter of every feeder is perturbed in the same solve. L_f is known to about
+    1e-12 at best, so central differences at h = 1e-6 relative carry up to 1e-4 relative noise on these
+    gradients; each reference gradient is instead a Richardson extrapolation (O(h^4)) at h = 1e-1 relative,
+    and its error (truncation and noise) is estimated per parameter by the same extrapolation at h = 5e-2.
+    Bar: |adjoint - FD| <= 1e-6 |FD| + 2 x that estimate. The line gradients span eight orders of magnitude
+    (lines at feeder ends carry almost nothing), so the estimate resolves 1e-6 relative for most load and
+    many line gradients, not all; the test asserts the resolved share it reaches on this workload."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle_lib as ol
+
+    case = synthetic.feeders_64(n_scenarios=40)
+    inp, upd = case.input, case.update
+    n_feeder = len(case.info["feeder_sizes"])
+    fon, fol, fold = _feeder_maps(case)
+    model = PowerGridModel(inp)
+    batch = DeviceBatch(model, upd, factor_precision=precision)
+    batch.set_output({"node": ["u_pu", "energized"], "line": ["p_from", "p_to"]})
+    batch.run()
+    grads = batch.adjoint()
+    scn = 2
+    base = upd["sym_load"][scn]
+    loads, lines = _pick(fold, n_feeder, 8), _pick(fol, n_feeder, 8)
+
+    def oracle_losses(inp_mod, rows):
+        out, it, st, _ = ol.power_flow(inp_mod, {"sym_load": np.stack(rows)}, len(rows), err_tol=1e-12,
+                                       max_iter=30, threading=0)
+        assert (st == 0).all()
+        return _feeder_losses(out["node"], inp_mod["line"], fon, fol, n_feeder)
+
+    def gpu_losses(inp_mod, rows):
+        b = DeviceBatch(PowerGridModel(inp_mod), {"sym_load": np.stack(rows)}, error_tolerance=1e-12,
+                        max_iterations=30)
+        b.set_output(FEEDER_OUTPUT)
+        b.run()
+        assert (b.iterations() > 0).all()
+        return _feeder_losses(b.fetch()["node"], inp_mod["line"], fon, fol, n_feeder)
+
+    def fd_of(L, steps_abs):
+        L = L.reshape(8, len(FD_STEPS), 4, n_feeder)
+        return np.stack([[_fd_from(L[j, k], h * steps_abs[j]) for k, h in enumerate(FD_STEPS)] for j in range(8)])
+
+    results = {}
+    for attr in ("p_specified", "q_specified"):
+        rows = []
+        for j in range(8):
+            for h_rel in FD_STEPS:
+                for mult in MULTS:
+                    r = base.copy()
+                    r[attr][loads[j]] += mult * h_rel * np.abs(base[attr][loads[j]])
+                    rows.append(r)
+        steps = np.abs(base[attr][loads])
+        results[(attr, "oracle")] = fd_of(oracle_losses(inp, rows), steps)
+        results[(attr, "gpu")] = fd_of(gpu_losses(inp, rows), steps)
+        results[(attr, "adjoint")] = grads["sym_load"][attr][scn][loads]
+    for attr in ("r1", "x1"):
+        mods = []
+        for j in range(8):
+            for h_rel in FD_STEPS:
+                for mult in MULTS:
+                    mod = {c: v.copy() for c, v in inp.items()}
+                    mod["line"][attr][lines[j]] += mult * h_rel * np.abs(inp["line"][attr][lines[j]])
+                    mods.append(mod)
+        with ThreadPoolExecutor(8) as ex:
+            lo = np.stack(list(ex.map(lambda m: oracle_losses(m, [base])[0], mods)))
+        results[(attr, "oracle")] = fd_of(lo, np.abs(inp["line"][attr][lines]))
+        results[(attr, "adjoint")] = grads["line"][attr][scn][lines]
+
+    min_resolved = {"p_specified": 0.95, "q_specified": 0.9, "r1": 0.5, "x1": 0.3}
+    for attr in ("p_specified", "q_specified", "r1", "x1"):
+        adj = results[(attr, "adjoint")]
+        for who in ("oracle", "gpu"):
+            if (attr, who) not in results:
+                continue
+            fd = results[(attr, who)]
+            est = np.abs(fd[:, 0] - fd[:, 1])
+            resolved = est <= 1e-6 * np.abs(fd[:, 0])
+            err = np.abs(adj - fd[:, 0])
+            rel = err / np.abs(fd[:, 0])
+            print(f"{precision} {attr} vs {who} FD: resolved at 1e-6 {np.mean(resolved):.0%}, max |adj - fd| / |fd| "
+                  f"there {np.max(rel[resolved]):.1e}; largest |adj - fd| / (1e-6 |fd| + 2 est) "
+                  f"{np.max(err / (1e-6 * np.abs(fd[:, 0]) + 2 * est)):.2f}")
+            assert np.mean(resolved) >= min_resolved[attr], (attr, who, np.mean(resolved))
+            bad = err > 1e-6 * np.abs(fd[:, 0]) + 2 * est
+            assert not bad.any(), (attr, who, np.argwhere(bad)[:5], rel[bad][:5])

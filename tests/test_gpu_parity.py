@@ -173,6 +173,38 @@ def test_validation_case_gpu_raises(name, sym, batch):
     assert all(isinstance(e, expected) for e in info.value.errors)
 
 
+def test_indptr_update_ids_resolved_once():
+    """An indptr (sparse) update buffer whose scenarios all hold the same number of rows is uniform
+    (dataset.hpp:334-343), and when they list the same ids the reference resolves them once before any
+    scenario runs (main_core/update.hpp:96-143 with job_adapter.hpp:113-124): an unknown id fails the
+    whole call with IDNotFound. Scenarios of different sizes are resolved one by one, so the unknown id
+    fails only the scenarios that list it. Known ids give the dense buffer's results."""
+    from power_grid_model_amd import initialize_array
+
+    case = synthetic.radial_200(n_scenarios=6)
+    model = PowerGridModel(case.input)
+    dense = case.update["sym_load"]
+    n = dense.shape[1]
+    flat = dense.reshape(-1)
+    indptr = np.arange(0, 6 * n + 1, n, dtype=np.int64)
+    out_sparse = model.calculate_power_flow(update_data={"sym_load": {"indptr": indptr, "data": flat}})
+    out_dense = model.calculate_power_flow(update_data=case.update)
+    for comp in out_dense:
+        assert out_sparse[comp].tobytes() == out_dense[comp].tobytes(), comp
+    bad = initialize_array("update", "sym_load", 6)
+    bad["id"] = -12345
+    bad["p_specified"] = 1e3
+    with pytest.raises(pgm.IDNotFound):
+        model.calculate_power_flow(update_data={"sym_load": {"indptr": np.arange(7, dtype=np.int64), "data": bad}})
+    # scenario 0 holds two rows, the others one: not uniform, each scenario resolves its own ids
+    ragged = np.concatenate([dense[0, :1], bad])
+    indptr = np.array([0, 2, 3, 4, 5, 6, 7], dtype=np.int64)
+    with pytest.raises(pgm.PowerGridBatchError) as info:
+        model.calculate_power_flow(update_data={"sym_load": {"indptr": indptr, "data": ragged}})
+    assert sorted(info.value.failed_scenarios.tolist()) == [0, 1, 2, 3, 4, 5]
+    assert all(isinstance(e, pgm.IDNotFound) for e in info.value.errors)
+
+
 # ------------------------------------------------------------------------------------------------
 # synthetic workloads against the oracle
 # ------------------------------------------------------------------------------------------------
@@ -212,7 +244,10 @@ WORKLOADS = [
     ("meshed_1000", lambda: synthetic.meshed_1000(n_scenarios=300), list(range(300))),
     ("asym_500", lambda: synthetic.asym_500(n_scenarios=130), list(range(130))),
     ("feeders_64", lambda: synthetic.feeders_64(n_scenarios=6, n_feeder=16), list(range(6))),
+    # the uneven-feeder workload at its full width: 64 feeders of 50-2000 nodes in one model (64 math models)
+    ("feeders_64_full", lambda: synthetic.feeders_64(n_scenarios=8), list(range(8))),
 ]
+MIXED_WORKLOADS = [w for w in WORKLOADS if w[0] != "feeders_64"]
 
 
 @pytest.mark.parametrize("name,make,scen", WORKLOADS, ids=[w[0] for w in WORKLOADS])
@@ -231,20 +266,69 @@ def test_workload_fp64_parity(name, make, scen):
     _compare(out, ref, case.symmetric, slice(None))
 
 
-def test_year_20k_sample_with_switching():
+def test_dense_newton_brute_force_counts_on_gpu():
+    """FP64 iteration counts and voltages of the HIP path against the dense numpy Newton brute force
+    (tests/dense_newton.py) on the oracle's 100 random grids (600 scenarios, counts 1-9 and divergent
+    scenarios): counts and the divergence set exact, voltages within 1e-10 pu."""
+    from dense_newton import dense_nr, from_input
+    from test_oracle_golden import _brute_force_grids
+
+    for g, inp, upd, scales, sym, tol in _brute_force_grids():
+        batch = DeviceBatch(PowerGridModel(inp), {"sym_load": upd}, symmetric=sym, error_tolerance=tol)
+        batch.run()
+        out = batch.fetch()
+        it = batch.iterations()
+        for k, sc in enumerate(scales):
+            mod = {kk: v.copy() for kk, v in inp.items()}
+            mod["sym_load"]["p_specified"] *= sc
+            mod["sym_load"]["q_specified"] *= sc
+            it_dense, u_dense = dense_nr(1 if sym else 3, **from_input(mod, sym), err_tol=tol)
+            if it_dense < 0:
+                assert it[k] == 0, (g, k, it[k])
+                continue
+            assert it[k] == it_dense, (g, k, it[k], it_dense)
+            u = out["node"]["u_pu"][k] * np.exp(1j * out["node"]["u_angle"][k])
+            np.testing.assert_allclose(u.reshape(u_dense.shape), u_dense, atol=1e-10, rtol=0)
+
+
+def _max_errors(out, ref):
+    """Largest error of each output against the oracle: |u_pu| in pu, angle in rad, flows relative to the
+    largest |value| of their attribute."""
+    err = {"u_pu": float(np.max(np.abs(out["node"]["u_pu"] - ref["node"]["u_pu"]))),
+           "u_angle": float(np.max(np.abs(np.angle(np.exp(1j * (out["node"]["u_angle"] - ref["node"]["u_angle"]))))))}
+    for comp, fields in (("line", ("p_from", "q_from", "p_to", "q_to")),
+                         ("transformer", ("p_from", "q_from", "p_to", "q_to")), ("sym_load", ("p", "q"))):
+        if comp not in ref:
+            continue
+        for f in fields:
+            r = ref[comp][f]
+            err[f"{comp}.{f}"] = float(np.max(np.abs(out[comp][f] - r)) / max(np.max(np.abs(r)), 1.0))
+    return err
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_year_20k_sample_with_switching(precision):
+    """The year-long workload (20k buses, 10% switching scenarios) on 40 plain scenarios and 24 that close
+    ties and open feeder lines, FP64 and mixed precision (the bench path), against the oracle. The FP64
+    path's iteration counts are bit-exact; the mixed path's largest error per output is printed."""
     case = synthetic.year_20k()
     ind = case.update["line"]["indptr"]
     sw = np.flatnonzero(np.diff(ind) > 0)
     scen = list(range(0, 40)) + list(sw[:24])
     upd = _subset(case, scen)
     model = PowerGridModel(case.input)
-    batch = DeviceBatch(model, upd)
+    batch = DeviceBatch(model, upd, factor_precision=precision)
     batch.run()
     out = batch.fetch()
     oupd, k = ol.uniform_update(upd, case.input)
     ref, it_ref, st_ref, _ = ol.power_flow(case.input, oupd, k, threading=0)
     assert (st_ref == 0).all()
-    assert (batch.iterations() == it_ref).all()
+    if precision == "fp64":
+        assert (batch.iterations() == it_ref).all()
+    else:
+        assert (batch.iterations() <= it_ref + 2).all()
+        print("year_20k mixed max error vs oracle:", _max_errors(out, ref),
+              "extra iterations:", int(np.sum(batch.iterations() - it_ref)))
     _compare(out, ref, True, slice(None))
 
 
@@ -389,6 +473,54 @@ def test_links_with_switching():
     assert (mixed.iterations() == batch.iterations()).all()
     for comp in ("node", "line", "link", "generic_branch"):
         assert out32[comp].tobytes() == out[comp].tobytes()
+
+
+def _phase_shift_case(n_scenarios=24, seed=12):
+    """radial_200 with 12 of its lines replaced by generic branches of angle +-pi/6 (ratio 0.98-1.02): every
+    bus behind one of them carries that phase shift in the topology (topology.hpp:196-210), which the
+    current methods' flat start rotates the source voltage by."""
+    from power_grid_model_amd import initialize_array
+
+    case = synthetic.radial_200(n_scenarios=n_scenarios)
+    inp = {k: v.copy() for k, v in case.input.items()}
+    rng = np.random.default_rng(seed)
+    line = inp["line"]
+    mask = np.zeros(len(line), dtype=bool)
+    mask[rng.choice(len(line), 12, replace=False)] = True
+    gb = initialize_array("input", "generic_branch", 12)
+    for f in ("id", "from_node", "to_node", "from_status", "to_status", "r1", "x1"):
+        gb[f] = line[f][mask]
+    gb["g1"] = 0.0
+    gb["b1"] = 1e-6
+    gb["k"] = rng.uniform(0.98, 1.02, 12)
+    gb["theta"] = np.where(np.arange(12) % 2 == 0, np.pi / 6, -np.pi / 6)
+    gb["sn"] = 5e6
+    inp["line"] = line[~mask]
+    inp["generic_branch"] = gb
+    return inp, case.update
+
+
+@pytest.mark.parametrize("method", ["newton_raphson", "iterative_current", "linear_current", "linear"])
+def test_generic_branch_phase_shift_methods(method):
+    """Generic branches with a large angle shift under every power-flow method against the oracle: the
+    iterative-current flat start and the single linear-current step depend on each bus's phase shift."""
+    inp, upd = _phase_shift_case()
+    model = PowerGridModel(inp)
+    batch = DeviceBatch(model, upd, calculation_method=method, max_iterations=100)
+    batch.run()
+    out = batch.fetch()
+    ref, it_ref, st_ref, _ = ol.power_flow(inp, upd, len(upd["sym_load"]), max_iter=100, method=method)
+    assert (st_ref == 0).all()
+    it = batch.iterations()
+    if method in ("newton_raphson", "linear", "linear_current"):
+        assert (it == it_ref).all(), (it, it_ref)
+    else:
+        assert (np.abs(it - it_ref) <= 1).all(), (it, it_ref)
+    _compare(out, ref, True, slice(None))
+    for f in ("p_from", "q_from", "p_to", "q_to"):
+        r = ref["generic_branch"][f]
+        floor = 1e-9 * max(np.max(np.abs(r)), 1.0)
+        np.testing.assert_allclose(out["generic_branch"][f], r, rtol=PQ_RTOL, atol=floor, err_msg=f)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
@@ -572,7 +704,7 @@ def test_single_stream_schedule_gives_identical_results(monkeypatch):
     assert (one.iterations() == two.iterations()).all()
 
 
-@pytest.mark.parametrize("name,make,scen", WORKLOADS[:3], ids=[w[0] for w in WORKLOADS[:3]])
+@pytest.mark.parametrize("name,make,scen", MIXED_WORKLOADS, ids=[w[0] for w in MIXED_WORKLOADS])
 def test_workload_mixed_precision(name, make, scen):
     case = make()
     upd = _subset(case, scen)

@@ -8,6 +8,9 @@ Sources of the vectors (reference tests, restated here as data):
   * closed-form 3-bus NR         tests/cpp_unit_tests/math_solver/test_math_solver_common.hpp:62-262,
                                  test_math_solver_pf.hpp:36-153 (run at tol 1e-12)
   * validation cases             tests/data/power_flow/** (copied under tests/golden/power_flow)
+  * NR iteration counts          test_math_solver_pf.hpp:118-130 (single iteration at err_tol = inf),
+                                 iterative_pf_solver.hpp:53-58 (the max_iter boundary), and a dense numpy
+                                 Newton brute force (tests/dense_newton.py) on 100 random grids
 Each check is chosen so a dropped term / wrong sign / transposed operand fails it.
 """
 
@@ -18,6 +21,8 @@ import numpy as np
 import pytest
 
 import oracle_lib as ol
+from dense_newton import dense_nr, random_grid
+from dense_newton import from_input as dense_from_input
 from pgm_formats import SYM_OUTPUT
 from validation_utils import compare_result, list_cases, load_case
 
@@ -208,7 +213,13 @@ def closed_form_grid(p):
 def test_closed_form_nr(p):
     grid, ref = closed_form_grid(p)
     it, r = ol.nr_math_model(p, **grid, err_tol=1e-12, max_iter=20)
-    assert it >= 1
+    # the iteration count is pinned by the dense brute force (same initial guess and stopping rule)
+    it_dense, u_dense = dense_nr(p, **grid, err_tol=1e-12, max_iter=20)
+    assert it == it_dense == 4
+    np.testing.assert_allclose(u_dense, ref["u"], atol=1e-12)
+    for tol, expect in ((1e-4, 2), (1e-6, 3), (1e-8, 3), (1e-10, 4)):
+        it_t, _ = ol.nr_math_model(p, **grid, err_tol=tol, max_iter=20)
+        assert it_t == dense_nr(p, **grid, err_tol=tol, max_iter=20)[0] == expect, tol
     np.testing.assert_allclose(r["u"], ref["u"], atol=1e-12)
     np.testing.assert_allclose(r["source"][0, 1], ref["source_i"], atol=1e-12)
     np.testing.assert_allclose(r["branch"][:, 2], ref["branch_if"], atol=1e-12)
@@ -241,6 +252,76 @@ def test_closed_form_not_converge_and_singular(p):
     sp = np.zeros_like(np.asarray(grid["shunt_param"]))
     it, _ = ol.nr_math_model(p, **{**grid, "branch_param": bp, "shunt_param": sp}, err_tol=1e-12, max_iter=20)
     assert it == -1
+
+
+@pytest.mark.parametrize("p", [1, 3])
+def test_single_iteration_infinite_tolerance(p):
+    """test_math_solver_pf.hpp:118-130: err_tol = inf and max_iter = 1 run exactly one iteration without
+    IterationDiverge, and the result lies within 0.15 of the exact solution."""
+    grid, ref = closed_form_grid(p)
+    it, r = ol.nr_math_model(p, **grid, err_tol=math.inf, max_iter=1)
+    assert it == 1
+    np.testing.assert_allclose(r["u"], ref["u"], atol=0.15)
+    it_dense, u_dense = dense_nr(p, **grid, err_tol=math.inf, max_iter=1)
+    assert it_dense == 1
+    np.testing.assert_allclose(r["u"], u_dense, atol=1e-12)
+
+
+@pytest.mark.parametrize("p", [1, 3])
+def test_max_iter_boundary(p):
+    """iterative_pf_solver.hpp:53-58: IterationDiverge fires on entering iteration max_iter + 1. A grid that
+    converges in k iterations converges with max_iter = k and diverges with max_iter = k - 1."""
+    grid, _ = closed_form_grid(p)
+    k, _ = ol.nr_math_model(p, **grid, err_tol=1e-12, max_iter=20)
+    assert k == 4
+    it, _ = ol.nr_math_model(p, **grid, err_tol=1e-12, max_iter=k)
+    assert it == k
+    it, _ = ol.nr_math_model(p, **grid, err_tol=1e-12, max_iter=k - 1)
+    assert it == -2
+    assert dense_nr(p, **grid, err_tol=1e-12, max_iter=k)[0] == k
+    assert dense_nr(p, **grid, err_tol=1e-12, max_iter=k - 1)[0] == -2
+
+
+def _brute_force_grids():
+    """100 random grids of 3-50 nodes (sym, asym and asym loads in sym calculations), each under six load
+    scales from light to beyond collapse, at three tolerances: iteration counts 1-9 and divergent cases."""
+    from power_grid_model_amd import initialize_array
+
+    rng = np.random.default_rng(0)
+    for g in range(100):
+        sym = g % 4 != 3
+        n = int(rng.integers(3, 51))
+        inp = random_grid(rng, n, int(rng.integers(0, 4)), sym, n_asym=0 if sym and g % 2 else 2)
+        scales = [0.05, 1.0, 4.0, 10.0, 20.0, 40.0]
+        tol = [1e-8, 1e-12, 1e-5][g % 3]
+        ld = inp["sym_load"]
+        upd = initialize_array("update", "sym_load", (len(scales), len(ld)))
+        upd["id"] = ld["id"]
+        for k, sc in enumerate(scales):
+            upd["p_specified"][k] = ld["p_specified"] * sc
+            upd["q_specified"][k] = ld["q_specified"] * sc
+        yield g, inp, upd, scales, sym, tol
+
+
+def test_dense_newton_brute_force_iteration_counts():
+    """The oracle's per-scenario iteration counts, divergence set and voltages against the dense numpy Newton
+    brute force (tests/dense_newton.py) on 600 scenarios of 100 random grids: counts exact, voltages within
+    1e-12 pu."""
+    seen = set()
+    for g, inp, upd, scales, sym, tol in _brute_force_grids():
+        out, it, st, _ = ol.power_flow(inp, {"sym_load": upd}, len(scales), symmetric=sym, err_tol=tol)
+        for k, sc in enumerate(scales):
+            mod = {kk: v.copy() for kk, v in inp.items()}
+            mod["sym_load"]["p_specified"] *= sc
+            mod["sym_load"]["q_specified"] *= sc
+            it_dense, u_dense = dense_nr(1 if sym else 3, **dense_from_input(mod, sym), err_tol=tol)
+            it_oracle = int(it[k]) if st[k] == 0 else (-2 if st[k] == 2 else -1)
+            assert it_oracle == it_dense, (g, k, it_oracle, it_dense)
+            seen.add(it_dense)
+            if it_dense > 0:
+                u = out["node"]["u_pu"][k] * np.exp(1j * out["node"]["u_angle"][k])
+                np.testing.assert_allclose(u.reshape(u_dense.shape), u_dense, atol=1e-12, rtol=0)
+    assert {-2, 1, 2, 3, 4, 5, 6}.issubset(seen), seen
 
 
 def test_two_bus_quadratic_closed_form():
