@@ -229,6 +229,11 @@ PGM_AMD_API void PGM_destroy_model(PGM_PowerGridModel* model) PGM_AMD_NOEXCEPT;
 PGM_AMD_API void PGM_amd_set_factor_precision(PGM_Handle* handle, PGM_Options* opt,
                                               PGM_Idx precision) PGM_AMD_NOEXCEPT;
 
+/* 1: batches built with these options keep every topology group as a plain group (no union groups that
+ * share one structure between switching states, no packed groups of a few scenarios), the layout
+ * PGM_amd_batch_adjoint runs on; 0 (default): the fastest layout. Results are the same to FP64 rounding. */
+PGM_AMD_API void PGM_amd_set_adjoint_layout(PGM_Handle* handle, PGM_Options* opt, PGM_Idx value) PGM_AMD_NOEXCEPT;
+
 /* Device-resident batch: build the batch (topology grouping, plans, device upload of every
  * scenario's update data) once, then run it any number of times with the inputs already in HBM.
  * PGM_calculate is build + run + fetch + destroy. Returns NULL on error (message in handle). */
@@ -236,6 +241,28 @@ typedef struct PGM_AmdBatch PGM_AmdBatch;
 PGM_AMD_API PGM_AmdBatch* PGM_amd_batch_create(PGM_Handle* handle, PGM_PowerGridModel const* model,
                                                PGM_Options const* opt,
                                                PGM_ConstDataset const* batch_dataset) PGM_AMD_NOEXCEPT;
+/* Make the load_gen update buffers of an update dataset resident in HBM for this model: every
+ * sym_gen / asym_gen / sym_load / asym_load buffer (the row buffer, or its status, p_specified and
+ * q_specified columns) is copied to the device once. Later PGM_calculate and PGM_amd_batch_create calls
+ * on this model whose update datasets hold the same host buffers (same address and size) read the device
+ * copies instead of uploading them again; the host buffers must not change meanwhile. Returns the bytes
+ * uploaded. For callers that solve one scenario set again and again (studies, serving). */
+PGM_AMD_API PGM_Idx PGM_amd_model_make_resident(PGM_Handle* handle, PGM_PowerGridModel* model,
+                                                PGM_ConstDataset const* update_dataset) PGM_AMD_NOEXCEPT;
+/* Free the device memory the model keeps between calculations: the resident update buffers and the
+ * device chunks of finished batches (PGM_calculate and destroyed PGM_AmdBatch objects hand their chunks
+ * back to the model, and the next calculation reuses them instead of allocating). */
+PGM_AMD_API void PGM_amd_model_release_device_memory(PGM_PowerGridModel* model) PGM_AMD_NOEXCEPT;
+/* Autotuner of the schedule (no counterpart in the reference): builds and runs this batch with candidate
+ * schedule parameters, coordinate by coordinate from the defaults, and keeps the fastest for the model.
+ * Later PGM_calculate and PGM_amd_batch_create calls on the model with the same shape (number of
+ * scenarios, symmetry, factor precision, method) use it. Parameters: [0] waves a plain group's schedule
+ * aims at per level, [1] largest number of tasks per level of a union group, [2] target number of parallel
+ * tasks of a packed group. trials_out receives up to max_trials rows of (plain waves, union tasks, packed
+ * target, device ms of one run). Returns the number of trials, or 0 on error (message in handle). */
+PGM_AMD_API PGM_Idx PGM_amd_autotune(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_Options const* opt,
+                                     PGM_ConstDataset const* batch_dataset, PGM_Idx* params, double* trials_out,
+                                     PGM_Idx max_trials) PGM_AMD_NOEXCEPT;
 /* Run every scenario to convergence on the device; returns when the device work is complete. */
 PGM_AMD_API void PGM_amd_batch_run(PGM_Handle* handle, PGM_AmdBatch* batch) PGM_AMD_NOEXCEPT;
 /* Select what the following runs produce on the device, from the layout of an output dataset: only
@@ -271,8 +298,9 @@ PGM_AMD_API void PGM_amd_batch_iterations(PGM_AmdBatch const* batch, PGM_Idx* it
  * out[5] = average duration in ms of one forward-sweep launch, out[6] = number of forward-sweep
  * launches, out[7] = buses per scenario, out[8] = algorithmic bytes of the timed forward-sweep
  * launches (factor written and read once, 2 E b, plus 32 P bytes per bus, for the scenarios still
- * iterating), out[9] = their summed duration in ms (HIP events), out[10] = number of timed launches.
- * out must hold 11 entries. */
+ * iterating), out[9] = their summed duration in ms (HIP events), out[10] = number of timed launches,
+ * out[11] = Newton-Raphson iterations launched as graphs (0: one launch per level). out must hold 12
+ * entries. */
 PGM_AMD_API void PGM_amd_batch_stats(PGM_AmdBatch const* batch, double* out) PGM_AMD_NOEXCEPT;
 
 /* Host-only inspection of the product's topology for a model (no GPU needed): per node the math

@@ -82,6 +82,9 @@ void PGM_update_model(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_ConstDa
             }
         }
         m.base = st;
+        // plans and tuned schedules were made for the previous base state
+        model->plans->clear();
+        model->tuned.clear();
     });
 }
 
@@ -118,7 +121,7 @@ void PGM_calculate(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_Options co
             return;
         }
         write_null_regulator_output(model->model, *output_dataset);
-        auto batch = create_batch(model->model, *opt, batch_dataset);
+        auto batch = create_batch(model->model, *opt, batch_dataset, model_context(*model, *opt, batch_dataset));
         // only the requested components (and, for columnar buffers, attributes) are produced
         batch->set_output(*output_dataset);
         batch->run();
@@ -139,10 +142,43 @@ PGM_AmdBatch* PGM_amd_batch_create(PGM_Handle* handle, PGM_PowerGridModel const*
                                    PGM_ConstDataset const* batch_dataset) noexcept {
     PGM_AmdBatch* out = nullptr;
     guarded(handle, [&] {
-        auto b = create_batch(model->model, *opt, batch_dataset);
+        auto b = create_batch(model->model, *opt, batch_dataset, model_context(*model, *opt, batch_dataset));
         out = new PGM_AmdBatch{std::move(b)};
     });
     return out;
+}
+PGM_Idx PGM_amd_autotune(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_Options const* opt,
+                         PGM_ConstDataset const* batch_dataset, PGM_Idx* params, double* trials_out,
+                         PGM_Idx max_trials) noexcept {
+    PGM_Idx n = 0;
+    guarded(handle, [&] {
+        std::vector<TuneTrial> trials;
+        Tuning const best = autotune(*model, *opt, batch_dataset, trials);
+        params[0] = best.plain_waves;
+        params[1] = best.union_tasks;
+        params[2] = best.packed_target;
+        n = PGM_Idx(trials.size());
+        for (PGM_Idx i = 0; i < std::min(n, max_trials); ++i) {
+            trials_out[4 * i] = trials[size_t(i)].t.plain_waves;
+            trials_out[4 * i + 1] = trials[size_t(i)].t.union_tasks;
+            trials_out[4 * i + 2] = trials[size_t(i)].t.packed_target;
+            trials_out[4 * i + 3] = trials[size_t(i)].ms;
+        }
+    });
+    return n;
+}
+PGM_Idx PGM_amd_model_make_resident(PGM_Handle* handle, PGM_PowerGridModel* model,
+                                    PGM_ConstDataset const* update_dataset) noexcept {
+    PGM_Idx bytes = 0;
+    guarded(handle, [&] {
+        if (update_dataset->meta->name != "update") throw PowerGridError("update dataset must be of type update\n");
+        bytes = PGM_Idx(make_resident(model->model, *update_dataset, *model->resident));
+    });
+    return bytes;
+}
+void PGM_amd_model_release_device_memory(PGM_PowerGridModel* model) noexcept {
+    model->resident = std::make_unique<pgm_amd::UploadCache>();
+    model->pool = std::make_shared<pgm_amd::ChunkPool>();
 }
 void PGM_amd_batch_run(PGM_Handle* handle, PGM_AmdBatch* batch) noexcept {
     guarded(handle, [&] { batch->b->run(); });
@@ -179,6 +215,7 @@ void PGM_amd_batch_stats(PGM_AmdBatch const* batch, double* out) noexcept {
     out[8] = b.fwd_bytes;
     out[9] = b.fwd_total_ms;
     out[10] = double(b.fwd_timed);
+    out[11] = double(b.graph_launches);
 }
 
 PGM_Idx PGM_amd_inspect_topology(PGM_Handle* handle, PGM_PowerGridModel const* model, PGM_Idx* node_group,
@@ -299,7 +336,6 @@ int PGM_amd_block_lu_solve(PGM_Idx nb, PGM_Idx n, PGM_Idx const* row_indptr, PGM
         return -1;
     }
 }
-
 }  // extern "C"
 
 int PGM_amd_copy_probe(PGM_Idx n_floats) noexcept {

@@ -147,4 +147,7 @@ struct PGM_Options {
     pgm_amd::Idx tap_changing_strategy = 0;
     pgm_amd::Idx experimental_features = 0;
     pgm_amd::Idx factor_precision = 0;
+    // 1: every topology group of a batch is a plain group (no union or packed groups), the layout the
+    // adjoint runs on
+    pgm_amd::Idx adjoint_layout = 0;
 };

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <exception>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <cstdlib>
 #include <limits>
@@ -61,6 +62,67 @@ void const* UploadCache::get(void const* host, size_t bytes) {
     }
     dev_.emplace(key, p);
     return p;
+}
+
+// device-to-host copy of a large result into pageable caller memory: T host threads, each with its own
+// pinned staging buffer and stream, take the chunks in turn (DMA into the staging buffer, then a host
+// copy out of it), so the DMA of one chunk overlaps the host copies of the others. The staging buffers
+// and streams are made on first use and kept for the process.
+namespace {
+constexpr int kStageThreads = 8;
+constexpr size_t kStageBytes = size_t(32) << 20;
+struct Staging {
+    std::mutex mu;
+    char* buf[kStageThreads] = {};
+    hipStream_t st[kStageThreads] = {};
+    bool ready = false;
+};
+Staging& staging() {
+    static Staging s;
+    return s;
+}
+}  // namespace
+
+void copy_to_host(void* dst, void const* src, size_t bytes) {
+    if (bytes < 4 * kStageBytes) {
+        PGM_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return;
+    }
+    Staging& S = staging();
+    std::lock_guard<std::mutex> lock(S.mu);
+    if (!S.ready) {
+        for (int t = 0; t < kStageThreads; ++t) {
+            PGM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&S.buf[t]), kStageBytes));
+            PGM_HIP_CHECK(hipStreamCreateWithFlags(&S.st[t], hipStreamNonBlocking));
+        }
+        S.ready = true;
+    }
+    size_t const n_chunks = (bytes + kStageBytes - 1) / kStageBytes;
+    std::vector<hipError_t> errs(kStageThreads, hipSuccess);
+    auto work = [&](int t) {
+        for (size_t c = size_t(t); c < n_chunks; c += kStageThreads) {
+            size_t const off = c * kStageBytes, len = std::min(kStageBytes, bytes - off);
+            hipError_t e = hipMemcpyAsync(S.buf[t], static_cast<char const*>(src) + off, len, hipMemcpyDeviceToHost,
+                                          S.st[t]);
+            if (e == hipSuccess) e = hipStreamSynchronize(S.st[t]);
+            if (e != hipSuccess) {
+                errs[size_t(t)] = e;
+                return;
+            }
+            std::memcpy(static_cast<char*>(dst) + off, S.buf[t], len);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < kStageThreads; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto& x : th) x.join();
+    for (auto e : errs)
+        if (e != hipSuccess) throw PowerGridError(std::string("HIP error in device-to-host copy: ") + hipGetErrorString(e) + "\n");
+}
+
+void const* UploadCache::find(void const* host, size_t bytes) const {
+    auto it = dev_.find(std::make_pair(host, bytes));
+    return it == dev_.end() ? nullptr : it->second;
 }
 
 Arena::~Arena() {
@@ -119,9 +181,9 @@ char const* const SparseMatrixErrorMessage =
     "See https://github.com/PowerGridModel/power-grid-model/issues/864.";
 
 
-template <class T> T* dev_alloc(Arena& a, size_t count) { return static_cast<T*>(a.take(count * sizeof(T))); }
+template <class T, class A> T* dev_alloc(A& a, size_t count) { return static_cast<T*>(a.take(count * sizeof(T))); }
 // blocking copy into the arena (the host vectors are often temporaries); nullptr for an empty vector
-template <class T> T* put(Arena& a, std::vector<T> const& h) {
+template <class T, class A> T* put(A& a, std::vector<T> const& h) {
     if (h.empty()) return nullptr;
     T* d = dev_alloc<T>(a, h.size());
     PGM_HIP_CHECK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
@@ -692,7 +754,9 @@ void upload_union_lanes(Batch& b, Group& g) {
     std::vector<int> xm(xcnt.size(), 0), xb(xrow.size(), 0);
     std::vector<double> xy(xrow.size() * 2, 0.0);
     std::vector<int8_t> xbus(extras ? size_t(n) : 0, 0);
-    std::vector<uint8_t> dead(size_t(n) * sp, 0), lstat(size_t(nbr) * sp, 0);
+    // buses without a source and branch side statuses, one row per member (its lanes share them); the
+    // device expands them to the lane-interleaved layout
+    std::vector<uint8_t> dead_m(g.members.size() * size_t(n), 0), lstat_m(g.members.size() * size_t(nbr), 0);
     std::vector<int> patch_lane, patch_entry;
     std::vector<double> patch_val;
     // members are independent: several host threads each take members in turn and write the lanes of
@@ -798,10 +862,9 @@ void upload_union_lanes(Batch& b, Group& g) {
                 }
                 std::sort(mo.mbus.begin(), mo.mbus.end());
                 mo.mbus.erase(std::unique(mo.mbus.begin(), mo.mbus.end()), mo.mbus.end());
-                for (int lane = mem.lane_begin; lane < mem.lane_end; ++lane) {
-                    for (Idx r = 0; r < n; ++r) dead[size_t(r) * sp + lane] = reach[r] ? 0 : 1;
-                    for (Idx mb = 0; mb < nbr; ++mb) lstat[size_t(mb) * sp + lane] = bits[mb];
-                }
+                uint8_t* const dm = dead_m.data() + mi * size_t(n);
+                for (Idx r = 0; r < n; ++r) dm[r] = reach[r] ? 0 : 1;
+                std::copy(bits.begin(), bits.end(), lstat_m.begin() + long(mi * size_t(nbr)));
             }
         } catch (std::exception const& e) {
             thr_err[t] = e.what();
@@ -849,8 +912,15 @@ void upload_union_lanes(Batch& b, Group& g) {
                         b.stream);
     PGM_HIP_CHECK(hipGetLastError());
     PGM_HIP_CHECK(hipStreamSynchronize(b.stream));
-    uint8_t* d_dead = put(A, dead);
-    uint8_t* d_lstat = put(A, lstat);
+    std::vector<int> lane_member(sp, -1);
+    for (size_t mi = 0; mi < n_mem; ++mi)
+        for (int lane = g.members[mi].lane_begin; lane < g.members[mi].lane_end; ++lane) lane_member[size_t(lane)] = int(mi);
+    int* d_lane_member = put(A, lane_member);
+    uint8_t* d_dead = dev_alloc<uint8_t>(A, size_t(n) * sp);
+    uint8_t* d_lstat = dev_alloc<uint8_t>(A, size_t(nbr) * sp);
+    launch_expand_member_rows(put(A, dead_m), d_lane_member, int(n), int(sp), d_dead, b.stream);
+    launch_expand_member_rows(put(A, lstat_m), d_lane_member, int(nbr), int(sp), d_lstat, b.stream);
+    PGM_HIP_CHECK(hipGetLastError());
     double* d_yvar = put(A, yvar);
     g.dg.ylane = ylane;
     g.dg.dead = d_dead;
@@ -1066,6 +1136,10 @@ struct UpdateReader {
 }  // namespace
 
 void Batch::release() {
+    destroy_graphs();
+    for (int k = 0; k < 2; ++k)
+        if (ev_iter[k]) (void)hipEventDestroy(ev_iter[k]);
+    if (h_act) (void)hipHostFree(h_act);
     groups.clear();
     for (auto e : event_pool) (void)hipEventDestroy(e);
     if (h_active) (void)hipHostFree(h_active);
@@ -1107,7 +1181,9 @@ struct PhaseTrace {
 }  // namespace
 
 std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, PGM_ConstDataset const* update,
-                                    ChunkPool* pool, UploadCache* uploads) {
+                                    BatchContext const& ctx) {
+    UploadCache* const uploads = ctx.uploads;
+    UploadCache const* const resident = ctx.resident;
     PhaseTrace trace;
     if (opt.calculation_type != PGM_power_flow)
         throw PowerGridError("The calculation type is not supported by this build: only power flow is implemented\n");
@@ -1119,7 +1195,8 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     ensure_device();
     auto bp = std::make_unique<Batch>();
     Batch& b = *bp;
-    b.arena.set_pool(pool);
+    if (ctx.tuning) b.tuning = *ctx.tuning;
+    b.arena.set_pool(ctx.pool);
     b.model = model;
     Model const& m = b.model;
     b.symmetric = opt.symmetric != 0;
@@ -1216,8 +1293,25 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
                 t_num[t] = num;
                 t_diff[t] = diff;
             };
+            // a resident buffer is scanned where it lies, in HBM
+            char const* d_idcol = nullptr;
+            if (idcol && resident && total > 0)
+                d_idcol = static_cast<char const*>(
+                    buf.data ? resident->find(buf.data, size_t(total) * buf.meta->size)
+                             : resident->find(idcol, size_t(total) * sizeof(ID)));
             if (!idcol) {
                 t_na[0] = total > 0;
+            } else if (d_idcol) {
+                int* d_flags = static_cast<int*>(b.arena.take(sizeof(int)));
+                launch_set_int(d_flags, 0, b.stream);
+                launch_id_scan(d_idcol, (long long)(idstride), (long long)(total), (long long)(row0),
+                               (long long)(eps), d_flags, b.stream);
+                int fl = 0;
+                PGM_HIP_CHECK(hipMemcpyAsync(&fl, d_flags, sizeof(int), hipMemcpyDeviceToHost, b.stream));
+                PGM_HIP_CHECK(hipStreamSynchronize(b.stream));
+                t_na[0] = (fl & 1) != 0;
+                t_num[0] = (fl & 2) != 0;
+                t_diff[0] = (fl & 4) != 0;
             } else if (nth == 1) {
                 scan(0);
             } else {
@@ -1336,6 +1430,8 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             // row-based: the whole row buffer is uploaded once and the fields address into it
             // device copy of a caller buffer, shared through `uploads` when given
             auto upload = [&](void const* host, size_t bytes) -> char const* {
+                if (resident)
+                    if (void const* d = resident->find(host, bytes)) return static_cast<char const*>(d);
                 if (uploads) return static_cast<char const*>(uploads->get(host, bytes));
                 char* d = static_cast<char*>(b.arena.take(bytes));
                 PGM_HIP_CHECK(hipMemcpy(d, host, bytes, hipMemcpyHostToDevice));
@@ -1453,7 +1549,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     // (symmetric Newton-Raphson or linear method, radial base structure, at most kExtraMax / 2 such
     // branches, no source at their buses) add off-pattern entries that a low-rank correction takes
     // into account ----
-    {
+    if (opt.adjoint_layout == 0) {
         bool const closing_ok = sym && !b.current;
         std::map<std::string, std::vector<int>> by_key;
         // per group: only opens branches of the base state; the branches it closes (when at most
@@ -1579,19 +1675,38 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             for (size_t gi = t; gi < G; gi += hw) {
                 try {
                     Group& g = *b.groups[gi];
-                    g.topo = build_topology(m, g.st, g.couple.empty() ? nullptr : &g.couple);
-                    std::vector<Idx> all(g.topo.models.size());
-                    std::iota(all.begin(), all.end(), Idx{0});
                     int const nblk = int((g.scenarios.size() + 63) / 64);
-                    g.packed = !g.is_union && g.scenarios.size() <= size_t(kPackedMax);
-                    int const target = g.packed     ? packed_target()
-                                       : g.is_union ? std::clamp(plain_waves() / std::max(nblk, 1), 4, union_tasks())
-                                                    : std::clamp(plain_waves() / std::max(nblk, 1), 4, 256);
-                    g.plan = build_plan(m, g.st, g.topo, all, sym, target);
-                    if (g.packed) {
-                        // lanes of a packed wave run different tasks: interleave them in memory
-                        interleave_plan(g.plan, kWaveLanes / int(g.scenarios.size()));
-                        check_plan(g.plan);
+                    g.packed = !g.is_union && g.scenarios.size() <= size_t(kPackedMax) && opt.adjoint_layout == 0;
+                    int const pw = b.tuning.plain_waves > 0 ? b.tuning.plain_waves : plain_waves();
+                    int const ut = b.tuning.union_tasks > 0 ? b.tuning.union_tasks : union_tasks();
+                    int const pt = b.tuning.packed_target > 0 ? b.tuning.packed_target : packed_target();
+                    int const target = g.packed     ? pt
+                                       : g.is_union ? std::clamp(pw / std::max(nblk, 1), 4, ut)
+                                                    : std::clamp(pw / std::max(nblk, 1), 4, 256);
+                    int const ilv = g.packed ? kWaveLanes / int(g.scenarios.size()) : 0;
+                    // the state's plan from an earlier batch of the model, or built now
+                    std::string key;
+                    if (ctx.plans) {
+                        key = structural_key(m, g.st) + "|" + (sym ? "s" : "a") + std::to_string(target) + "/" +
+                              std::to_string(ilv) + "|";
+                        for (size_t c = 0; c < g.couple.size(); ++c)
+                            if (g.couple[c]) key += std::to_string(c) + ",";
+                    }
+                    std::shared_ptr<PlanEntry const> hit = ctx.plans ? ctx.plans->find(key) : nullptr;
+                    if (hit) {
+                        g.topo = hit->topo;
+                        g.plan = hit->plan;
+                    } else {
+                        g.topo = build_topology(m, g.st, g.couple.empty() ? nullptr : &g.couple);
+                        std::vector<Idx> all(g.topo.models.size());
+                        std::iota(all.begin(), all.end(), Idx{0});
+                        g.plan = build_plan(m, g.st, g.topo, all, sym, target);
+                        if (g.packed) {
+                            // lanes of a packed wave run different tasks: interleave them in memory
+                            interleave_plan(g.plan, ilv);
+                            check_plan(g.plan);
+                        }
+                        if (ctx.plans) ctx.plans->put(key, std::make_shared<PlanEntry const>(PlanEntry{g.topo, g.plan}));
                     }
                 } catch (std::exception const& e) {
                     errs[gi] = e.what();
@@ -1795,6 +1910,8 @@ void Batch::run() {
     fwd_launches = 0;
     fwd_bytes = 0.0;
     std::vector<hipEvent_t> fwd_events;
+    double fwd_graph_ms = 0.0;  // forward-level durations of the graph launches (their own events)
+    Idx fwd_graph_timed = 0;
     // event timing of the forward launches (stats); PGM_AMD_EVENTS=0 leaves it out, for tools that
     // intercept the queue and take only kernel dispatches
     bool const timing = [] {
@@ -1862,7 +1979,101 @@ void Batch::run() {
         if (current) launch_ic_start(P, d_groups, d_bus_off, int(groups.size()), bus_total, st);
         fork();
     }
-    for (int it = 1; it <= (linear ? 0 : int(max_iter)); ++it) {
+    // Newton-Raphson iterations as graph launches (PGM_AMD_GRAPH=0: one launch per level, as below)
+    bool const graph_wanted = !linear && !current && max_iter > 0 && [] {
+        char const* v = std::getenv("PGM_AMD_GRAPH");
+        return !(v && std::atoi(v) == 0);
+    }();
+    // one iteration on the streams: the body that is captured
+    auto nr_iteration = [&](int parity, bool events) {
+        launch_begin_iteration(d_iter, d_active, st);
+        fork();
+        union_sweeps(MODE_NR);
+        for (int l = 0; l < L; ++l) {
+            if (events) PGM_HIP_CHECK(hipEventRecordWithFlags(nr_ev[parity][2 * l], st, hipEventRecordExternal));
+            launch_forward(d_groups, d_items[l], n_items[l], P, fp32, MODE_NR, st);
+            if (events) PGM_HIP_CHECK(hipEventRecordWithFlags(nr_ev[parity][2 * l + 1], st, hipEventRecordExternal));
+        }
+        for (int l = L - 1; l >= 0; --l) launch_backward(d_groups, d_items[l], n_items[l], P, fp32, MODE_NR, st);
+        join();
+        launch_converge(pairs, 0, int(max_iter), err_tol, d_active, st, d_iter);
+    };
+    if (graph_wanted && !graph_tried) {
+        graph_tried = true;
+        if (!d_iter) d_iter = static_cast<int*>(arena.take(sizeof(int)));
+        if (!h_act) PGM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_act), 2 * sizeof(int)));
+        for (int k = 0; k < 2; ++k)
+            if (!ev_iter[k]) PGM_HIP_CHECK(hipEventCreateWithFlags(&ev_iter[k], hipEventDisableTiming));
+        // the prologue's work on the second stream joins the main stream before the capture starts
+        join();
+        PGM_HIP_CHECK(hipStreamSynchronize(st));
+        graph_ok = true;
+        for (int k = 0; k < 2 && graph_ok; ++k) {
+            nr_ev[k].clear();
+            if (timing) {
+                for (int e = 0; e < 2 * L; ++e) {
+                    hipEvent_t ev;
+                    PGM_HIP_CHECK(hipEventCreate(&ev));
+                    nr_ev[k].push_back(ev);
+                }
+            }
+            hipGraph_t graph = nullptr;
+            if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                graph_ok = false;
+                break;
+            }
+            try {
+                nr_iteration(k, timing);
+            } catch (PowerGridError const&) {
+                graph_ok = false;
+            }
+            hipError_t const ec = hipStreamEndCapture(st, &graph);
+            graph_ok = graph_ok && ec == hipSuccess && graph &&
+                       hipGraphInstantiate(&nr_exec[k], graph, nullptr, nullptr, 0) == hipSuccess;
+            if (graph) (void)hipGraphDestroy(graph);
+        }
+        (void)hipGetLastError();
+        if (!graph_ok) destroy_graphs();
+    }
+    graph_launches = 0;
+    if (graph_wanted && graph_ok) {
+        // the prologue's second-stream work joins before the first launch; every launch forks and joins
+        join();
+        launch_set_int(d_iter, 0, st);
+        int launched = 0, done_it = 0;
+        auto launch_next = [&] {
+            int const k = launched % 2;
+            PGM_HIP_CHECK(hipGraphLaunch(nr_exec[k], st));
+            PGM_HIP_CHECK(hipMemcpyAsync(&h_act[k], d_active, sizeof(int), hipMemcpyDeviceToHost, st));
+            PGM_HIP_CHECK(hipEventRecord(ev_iter[k], st));
+            ++launched;
+        };
+        launch_next();
+        // one iteration ahead: the next iteration is queued before the host waits for the present one, so
+        // the device never waits for the host; the iteration queued after the last one finds every lane
+        // done and returns at once
+        while (true) {
+            if (launched < int(max_iter)) launch_next();
+            int const k = done_it % 2;
+            PGM_HIP_CHECK(hipEventSynchronize(ev_iter[k]));
+            ++done_it;
+            if (timing) {
+                for (int l = 0; l < L; ++l) {
+                    float e = 0;
+                    if (hipEventElapsedTime(&e, nr_ev[k][2 * l], nr_ev[k][2 * l + 1]) != hipSuccess) continue;
+                    fwd_graph_ms += e;
+                    fwd_bytes += level_bytes_full[l] * active_frac;
+                    ++fwd_graph_timed;
+                }
+            }
+            fwd_launches += L;
+            int const act = h_act[k];
+            active_frac = real_pairs > 0 ? double(act) / double(real_pairs) : 0.0;
+            if (act == 0 || done_it >= int(max_iter)) break;
+        }
+        graph_launches = launched;
+    }
+    for (int it = 1; it <= ((linear || (graph_wanted && graph_ok)) ? 0 : int(max_iter)); ++it) {
         int const mode = current ? (it == 1 ? MODE_ICF : MODE_RESID) : MODE_NR;
         union_sweeps(mode);
         for (int l = 0; l < L; ++l) {
@@ -1989,10 +2200,20 @@ void Batch::run() {
         PGM_HIP_CHECK(hipEventElapsedTime(&e, fwd_events[i], fwd_events[i + 1]));
         fwd_ms += e;
     }
-    fwd_timed = Idx(fwd_events.size() / 2);
+    fwd_ms += fwd_graph_ms;
+    fwd_timed = Idx(fwd_events.size() / 2) + fwd_graph_timed;
     fwd_launch_ms = fwd_timed > 0 ? fwd_ms / double(fwd_timed) : 0.0;
     fwd_total_ms = fwd_ms;
     ran = true;
+}
+
+void Batch::destroy_graphs() {
+    for (int k = 0; k < 2; ++k) {
+        if (nr_exec[k]) (void)hipGraphExecDestroy(nr_exec[k]);
+        nr_exec[k] = nullptr;
+        for (auto e : nr_ev[k]) (void)hipEventDestroy(e);
+        nr_ev[k].clear();
+    }
 }
 
 void Batch::adjoint(double* grad_p, double* grad_q, double* grad_r, double* grad_x) {
@@ -2003,7 +2224,8 @@ void Batch::adjoint(double* grad_p, double* grad_q, double* grad_r, double* grad
         throw PowerGridError("the adjoint is implemented for grids without links and generic branches\n");
     for (auto const& gp : groups)
         if (gp->is_union || gp->packed)
-            throw PowerGridError("the adjoint needs every topology group to hold more than 32 scenarios\n");
+            throw PowerGridError("the adjoint needs plain topology groups: build the batch with the adjoint layout "
+                                 "(PGM_amd_set_adjoint_layout), or with more than 32 scenarios in every topology\n");
     Model const& m = model;
     Idx const nlg = m.n_lg(), nl = m.n_line();
     size_t const n_grad = size_t(n_scenarios) * size_t(2 * nlg + 2 * nl);
@@ -2209,11 +2431,11 @@ void Batch::fetch(PGM_MutableDataset const& output, PGM_Handle* handle) {
                                          " was not selected for output of this batch\n");
                 size_t const as = attr->size;
                 if (all_ok && buf.elements_per_scenario == cnt) {
-                    PGM_HIP_CHECK(hipMemcpy(hcol, dcol, size_t(n_scenarios) * cnt * as, hipMemcpyDeviceToHost));
+                    copy_to_host(hcol, dcol, size_t(n_scenarios) * cnt * as);
                     continue;
                 }
                 host.resize(size_t(n_scenarios) * cnt * as);
-                PGM_HIP_CHECK(hipMemcpy(host.data(), dcol, host.size(), hipMemcpyDeviceToHost));
+                copy_to_host(host.data(), dcol, host.size());
                 for (Idx s = 0; s < n_scenarios; ++s) {
                     if (status[s] != 0) continue;
                     Idx const b0 = buf.begin(s), b1 = buf.end(s);
@@ -2229,11 +2451,11 @@ void Batch::fetch(PGM_MutableDataset const& output, PGM_Handle* handle) {
         size_t const rs = rec_size[comp];
         bool const direct = !buf.columnar() && buf.elements_per_scenario == cnt;
         if (direct && all_ok) {
-            PGM_HIP_CHECK(hipMemcpy(buf.data, out[comp], size_t(n_scenarios) * cnt * rs, hipMemcpyDeviceToHost));
+            copy_to_host(buf.data, out[comp], size_t(n_scenarios) * cnt * rs);
             continue;
         }
         host.resize(size_t(n_scenarios) * cnt * rs);
-        PGM_HIP_CHECK(hipMemcpy(host.data(), out[comp], host.size(), hipMemcpyDeviceToHost));
+        copy_to_host(host.data(), out[comp], host.size());
         for (Idx s = 0; s < n_scenarios; ++s) {
             if (status[s] != 0) continue;
             Idx const b0 = buf.begin(s), b1 = buf.end(s);
@@ -2267,6 +2489,99 @@ void Batch::fetch(PGM_MutableDataset const& output, PGM_Handle* handle) {
     }
 }
 
+std::shared_ptr<PlanEntry const> PlanCache::find(std::string const& key) {
+    std::lock_guard<std::mutex> lock(mu_);
+    auto it = map_.find(key);
+    return it == map_.end() ? nullptr : it->second;
+}
+void PlanCache::put(std::string const& key, std::shared_ptr<PlanEntry const> e) {
+    std::lock_guard<std::mutex> lock(mu_);
+    if (map_.size() >= 512) map_.clear();  // bounded: a long run of distinct states starts over
+    map_[key] = std::move(e);
+}
+void PlanCache::clear() {
+    std::lock_guard<std::mutex> lock(mu_);
+    map_.clear();
+}
+
+BatchContext model_context(PGM_PowerGridModel const& model, PGM_Options const& opt, PGM_ConstDataset const* update) {
+    BatchContext ctx;
+    ctx.pool = model.pool;
+    ctx.resident = model.resident.get();
+    ctx.plans = model.plans.get();
+    auto const tu = model.tuned.find(tune_key(opt, update));
+    if (tu != model.tuned.end()) ctx.tuning = &tu->second;
+    return ctx;
+}
+
+std::string tune_key(PGM_Options const& opt, PGM_ConstDataset const* update) {
+    return std::to_string(update ? update->batch_size : 1) + (opt.symmetric ? "/sym" : "/asym") +
+           (opt.factor_precision == 1 ? "/fp32" : "/fp64") + "/m" + std::to_string(int(opt.calculation_method)) +
+           (opt.adjoint_layout ? "/adjoint" : "");
+}
+
+Tuning autotune(PGM_PowerGridModel& model, PGM_Options const& opt, PGM_ConstDataset const* update,
+                std::vector<TuneTrial>& trials) {
+    auto measure = [&](Tuning const& t) {
+        BatchContext ctx = model_context(model, opt, update);
+        ctx.tuning = &t;
+        auto b = create_batch(model.model, opt, update, ctx);
+        b->run();
+        b->run();
+        double const ms = b->kernel_ms;
+        bool const has_union = std::any_of(b->groups.begin(), b->groups.end(), [](auto const& g) { return g->is_union; });
+        bool const has_packed = std::any_of(b->groups.begin(), b->groups.end(), [](auto const& g) { return g->packed; });
+        trials.push_back({t, ms});
+        return std::make_tuple(ms, has_union, has_packed);
+    };
+    Tuning best{plain_waves(), union_tasks(), packed_target()};
+    auto [best_ms, has_union, has_packed] = measure(best);
+    auto try_values = [&](int Tuning::*field, std::initializer_list<int> values) {
+        for (int v : values) {
+            if (v == best.*field) continue;
+            Tuning t = best;
+            t.*field = v;
+            double const ms = std::get<0>(measure(t));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        }
+    };
+    try_values(&Tuning::plain_waves, {3072, 6144, 12288, 24576, 49152});
+    if (has_union) try_values(&Tuning::union_tasks, {64, 256, 1024});
+    if (has_packed) try_values(&Tuning::packed_target, {64, 256, 1024});
+    model.tuned[tune_key(opt, update)] = best;
+    return best;
+}
+
+size_t make_resident(Model const& model, PGM_ConstDataset const& update, UploadCache& cache) {
+    (void)model;
+    ensure_device();
+    size_t bytes = 0;
+    for (auto const& buf : update.buffers) {
+        std::string const& name = buf.meta->name;
+        if (name != comp_names[C_SYM_GEN] && name != comp_names[C_ASYM_GEN] && name != comp_names[C_SYM_LOAD] &&
+            name != comp_names[C_ASYM_LOAD])
+            continue;
+        // the same host ranges the batch's device load updates read (create_batch)
+        if (buf.data) {
+            size_t const n = size_t(buf.total_elements) * buf.meta->size;
+            if (!cache.find(buf.data, n)) bytes += n;
+            cache.get(buf.data, n);
+            continue;
+        }
+        for (auto const& [a, col] : buf.columns) {
+            if (a->name != "id" && a->name != "status" && a->name != "p_specified" && a->name != "q_specified")
+                continue;
+            size_t const n = size_t(buf.total_elements) * a->size;
+            if (!cache.find(col, n)) bytes += n;
+            cache.get(col, n);
+        }
+    }
+    return bytes;
+}
+
 void batch_plan_sizes(Batch const& b, double& buses, double& entries) {
     double sb = 0, se = 0, ss = 0;
     for (auto const& g : b.groups) {
@@ -2278,10 +2593,32 @@ void batch_plan_sizes(Batch const& b, double& buses, double& entries) {
     entries = ss > 0 ? se / ss : 0;
 }
 
+namespace {
+template <class Ar>
+int block_lu_impl(Ar& A, hipStream_t st, Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices,
+                  Idx const* diag_lu, Idx batch, double const* data, double const* rhs, double* x, int32_t* status,
+                  bool fp32);
+}  // namespace
+
 int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices, Idx const* diag_lu, Idx batch,
                     double const* data, double const* rhs, double* x, int32_t* status, bool fp32) {
     if (nb != 2 && nb != 6) return -1;
     ensure_device();
+    hipStream_t st;
+    PGM_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } guard{st};
+    Arena A;
+    return block_lu_impl(A, st, nb, n, row_indptr, col_indices, diag_lu, batch, data, rhs, x, status, fp32);
+}
+
+namespace {
+template <class Ar>
+int block_lu_impl(Ar& A, hipStream_t st, Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices,
+                  Idx const* diag_lu, Idx batch, double const* data, double const* rhs, double* x, int32_t* status,
+                  bool fp32) {
     int const P = int(nb / 2), NB = int(nb), NB2 = NB * NB;
     Plan p;
     p.P = P;
@@ -2304,13 +2641,6 @@ int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices
         }
     p.row_end.assign(p.row_ptr.begin() + 1, p.row_ptr.end());
     build_schedule(p, 16);
-    hipStream_t st;
-    PGM_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    struct StreamGuard {
-        hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
-    } guard{st};
-    Arena A;
     int const S = int(batch), S_pad = S > 32 ? (S + 63) / 64 * 64 : S;
     size_t const sp = size_t(S_pad);
     DevGroup d;
@@ -2388,5 +2718,17 @@ int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices
     }
     return 0;
 }
+}  // namespace
 
 }  // namespace pgm_amd
+
+PGM_PowerGridModel::PGM_PowerGridModel()
+    : pool(std::make_shared<pgm_amd::ChunkPool>()),
+      resident(std::make_unique<pgm_amd::UploadCache>()),
+      plans(std::make_shared<pgm_amd::PlanCache>()) {}
+PGM_PowerGridModel::PGM_PowerGridModel(PGM_PowerGridModel const& other)
+    : model(other.model),
+      pool(std::make_shared<pgm_amd::ChunkPool>()),
+      resident(std::make_unique<pgm_amd::UploadCache>()),
+      plans(std::make_shared<pgm_amd::PlanCache>()) {}
+PGM_PowerGridModel::~PGM_PowerGridModel() = default;

@@ -8,11 +8,40 @@
 
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
+namespace pgm_amd {
+class ChunkPool;
+class UploadCache;
+class PlanCache;
+// schedule parameters of a batch (0: the default, or the PGM_AMD_* environment override):
+//   plain_waves   waves a plain group's schedule aims at per level (tasks x 64-scenario blocks)
+//   union_tasks   largest number of tasks per level of a union group's schedule
+//   packed_target target number of parallel tasks of a packed group's schedule
+struct Tuning {
+    int plain_waves = 0;
+    int union_tasks = 0;
+    int packed_target = 0;
+};
+}  // namespace pgm_amd
+
+// the model, and the device memory that calculations on it keep between calls: released chunks of
+// earlier batches (reused by the next calculation instead of new hipMallocs) and update buffers the caller
+// made resident in HBM. A copy of the model starts with neither.
 struct PGM_PowerGridModel {
     pgm_amd::Model model;
+    std::shared_ptr<pgm_amd::ChunkPool> pool;
+    std::unique_ptr<pgm_amd::UploadCache> resident;
+    // schedule parameters picked by the autotuner, per batch shape (tune_key)
+    std::map<std::string, pgm_amd::Tuning> tuned;
+    // host plans (topology, ordering, Y-bus, schedule) of the structural states met so far; emptied when
+    // the model is updated
+    std::shared_ptr<pgm_amd::PlanCache> plans;
+    PGM_PowerGridModel();
+    PGM_PowerGridModel(PGM_PowerGridModel const& other);
+    ~PGM_PowerGridModel();
 };
 
 namespace pgm_amd {
@@ -42,9 +71,30 @@ class UploadCache {
     UploadCache& operator=(UploadCache const&) = delete;
     ~UploadCache();
     void const* get(void const* host, size_t bytes);
+    // the device copy of exactly this host range, or nullptr (nothing is uploaded)
+    void const* find(void const* host, size_t bytes) const;
+    size_t size() const { return dev_.size(); }
 
   private:
     std::map<std::pair<void const*, size_t>, void*> dev_;
+};
+
+// topology and plan of one structural state (and schedule target), shared between batches of one model:
+// the reference keeps its topology and solvers between calculations while the topology is unchanged
+// (main_model_impl.hpp is_topology_up_to_date_); here they are kept per structural state
+struct PlanEntry {
+    Topology topo;
+    Plan plan;
+};
+class PlanCache {
+  public:
+    std::shared_ptr<PlanEntry const> find(std::string const& key);
+    void put(std::string const& key, std::shared_ptr<PlanEntry const> e);
+    void clear();
+
+  private:
+    std::mutex mu_;
+    std::map<std::string, std::shared_ptr<PlanEntry const>> map_;
 };
 
 // bump allocator over large device chunks: one batch makes thousands of small arrays (one set per
@@ -57,14 +107,14 @@ class Arena {
     ~Arena();
     void* take(size_t bytes);
     size_t reserved() const { return reserved_; }
-    void set_pool(ChunkPool* pool) { pool_ = pool; }
+    void set_pool(std::shared_ptr<ChunkPool> pool) { pool_ = std::move(pool); }
 
   private:
     std::vector<std::pair<void*, size_t>> chunks_;
     char* cur_ = nullptr;
     size_t left_ = 0;
     size_t reserved_ = 0;
-    ChunkPool* pool_ = nullptr;
+    std::shared_ptr<ChunkPool> pool_;  // shared: a batch may outlive the model that lent it
 };
 
 // apply one update record of component `comp` to a state; `structural` is set when it changes a branch
@@ -76,6 +126,7 @@ struct Group;
 
 struct Batch {
     Model model;
+    Tuning tuning;  // the schedule parameters the batch was built with (0: default)
     bool symmetric = true;
     int P = 1;
     double err_tol = 1e-8;
@@ -142,6 +193,19 @@ struct Batch {
     double fwd_total_ms = 0.0;  // their summed durations
     std::vector<hipEvent_t> event_pool;
     bool ran = false;
+    // Newton-Raphson iterations as hipGraph launches: one captured iteration (every level of every kind
+    // of group, the low-rank correction, the convergence check), instantiated twice so that the host can
+    // read one iteration's active count and event times while the next one runs. d_iter is the device
+    // iteration counter the captured check reads; h_act[2] the pinned active counts of the two parities.
+    bool graph_tried = false;
+    bool graph_ok = false;
+    hipGraphExec_t nr_exec[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> nr_ev[2];  // per parity: start and end event of every plain forward level
+    hipEvent_t ev_iter[2] = {nullptr, nullptr};
+    int* d_iter = nullptr;
+    int* h_act = nullptr;  // pinned, 2 entries
+    int graph_launches = 0;  // iterations launched by the last run (the last may have found every lane done)
+    void destroy_graphs();
 
     Batch() = default;
     ~Batch();
@@ -162,8 +226,35 @@ struct Batch {
 
 // pool: where the batch takes its device chunks from and returns them to (nullptr: allocate and free);
 // uploads: device copies of update buffers shared with other batches (nullptr: the batch uploads its own)
+// what a batch may borrow from its caller: pool, where the batch takes its device chunks from and returns
+// them to (nullptr: allocate and free); uploads, device copies of update buffers shared with other
+// batches (nullptr: the batch uploads its own); resident, update buffers the caller made resident in HBM
+// (looked up only, never filled); tuning, schedule parameters; plans, host plans of structural states
+struct BatchContext {
+    std::shared_ptr<ChunkPool> pool;
+    UploadCache* uploads = nullptr;
+    UploadCache const* resident = nullptr;
+    Tuning const* tuning = nullptr;
+    PlanCache* plans = nullptr;
+};
 std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, PGM_ConstDataset const* update,
-                                    ChunkPool* pool = nullptr, UploadCache* uploads = nullptr);
+                                    BatchContext const& ctx = {});
+// the context of a calculation on a model: its pool, resident buffers, plans and tuned parameters
+BatchContext model_context(PGM_PowerGridModel const& model, PGM_Options const& opt, PGM_ConstDataset const* update);
+// the shape a tuning applies to: scenarios, symmetry, factor precision, method
+std::string tune_key(PGM_Options const& opt, PGM_ConstDataset const* update);
+// one measured candidate of the autotuner
+struct TuneTrial {
+    Tuning t;
+    double ms;  // device time of one run (HIP events over the whole run)
+};
+// try schedule parameters on this batch (each candidate: build the batch, run it twice, keep the second
+// run's device time), coordinate by coordinate from the defaults; the best is stored in model.tuned
+Tuning autotune(PGM_PowerGridModel& model, PGM_Options const& opt, PGM_ConstDataset const* update,
+                std::vector<TuneTrial>& trials);
+// device copies of every load_gen update buffer of `update` (rows, or the status / p / q columns) in
+// `cache`; returns the bytes uploaded
+size_t make_resident(Model const& model, PGM_ConstDataset const& update, UploadCache& cache);
 
 // test hook: batched block LU on the device
 int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices, Idx const* diag_lu, Idx batch,

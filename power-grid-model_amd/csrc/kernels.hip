@@ -1601,10 +1601,13 @@ __global__ void retire_linear_kernel(PairState ps, uint8_t const* __restrict__ p
 
 // pairs of padding lanes are never touched by the sweeps: their deviation stays 0 and they retire
 // at the first check
-__global__ void converge_kernel(PairState ps, int it, int max_iter, double tol, int* active) {
+__global__ void converge_kernel(PairState ps, int it_host, int const* __restrict__ it_dev, int max_iter, double tol,
+                                int* active) {
     size_t const i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= size_t(ps.n)) return;
     if (ps.done[i] != ACTIVE) return;
+    // the iteration number from the host, or from the device counter (graph launches)
+    int const it = it_dev ? *it_dev : it_host;
     double const md = __longlong_as_double(static_cast<long long>(ps.devmax[i]));
     ps.devmax[i] = 0ull;
     ps.iters[i] = it;
@@ -1619,6 +1622,17 @@ __global__ void converge_kernel(PairState ps, int it, int max_iter, double tol, 
     }
 }
 
+// start of a captured Newton-Raphson iteration: the iteration counter advances and the active count of
+// this iteration's check starts at zero
+__global__ void begin_iteration_kernel(int* it, int* active) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *it += 1;
+        *active = 0;
+    }
+}
+__global__ void set_int_kernel(int* p, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
 
 template <int P>
 __global__ void ic_start_kernel(DevGroup const* __restrict__ groups, long long const* __restrict__ offsets,
@@ -2119,6 +2133,32 @@ __global__ void broadcast_y_kernel(double const* __restrict__ ydata, long long n
     if (i >= n_val * S_pad) return;
     ylane[i] = ydata[i / S_pad];
 }
+// ids of an update buffer in HBM: bit 0 when any id is NA, bit 1 when any is a number, bit 2 when some
+// scenario's id differs from scenario 0's at the same position (eps > 0: rows per scenario, row0: first row)
+__global__ void id_scan_kernel(char const* __restrict__ base, long long stride, long long total, long long row0,
+                               long long eps, int* __restrict__ flags) {
+    long long const i = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+    int f = 0;
+    if (i < total) {
+        int const id = *reinterpret_cast<int const*>(base + i * stride);
+        f = id == na_IntID ? 1 : 2;
+        if (eps > 0 && i >= row0 + eps) {
+            int const id0 = *reinterpret_cast<int const*>(base + (row0 + (i - row0) % eps) * stride);
+            if (id != id0) f |= 4;
+        }
+    }
+    // one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) f |= __shfl_xor(f, off);
+    if ((threadIdx.x & 63) == 0 && f) atomicOr(flags, f);
+}
+__global__ void expand_member_rows_kernel(uint8_t const* __restrict__ src, int const* __restrict__ lane_member,
+                                          int cols, int S_pad, uint8_t* __restrict__ dst) {
+    long long const i = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= (long long)(cols) * S_pad) return;
+    int const c = int(i / S_pad), lane = int(i % S_pad);
+    int const m = lane_member[lane];
+    dst[i] = m < 0 ? uint8_t(0) : src[(long long)(m) * cols + c];
+}
 __global__ void patch_y_kernel(int n_patch, int Q, int S_pad, int const* __restrict__ lane,
                                int const* __restrict__ entry, double const* __restrict__ val,
                                double* __restrict__ ylane) {
@@ -2163,6 +2203,19 @@ void launch_backward(DevGroup const* groups, WorkItem const* items, int n_items,
                      hipStream_t st, int kind) {
     launch_kind<false>(groups, items, n_items, P, fp32, mode, st, kind);
 }
+void launch_id_scan(char const* base, long long stride, long long total, long long row0, long long eps, int* flags,
+                    hipStream_t st) {
+    if (total > 0)
+        hipLaunchKernelGGL(id_scan_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, base, stride, total,
+                           row0, eps, flags);
+}
+void launch_expand_member_rows(uint8_t const* src, int const* lane_member, int cols, int S_pad, uint8_t* dst,
+                               hipStream_t st) {
+    long long const total = (long long)(cols) * S_pad;
+    if (total > 0)
+        hipLaunchKernelGGL(expand_member_rows_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, src,
+                           lane_member, cols, S_pad, dst);
+}
 void launch_expand_ylane(double const* ydata, int E, int PP, int S_pad, double* ylane, int n_patch,
                          int const* patch_lane, int const* patch_entry, double const* patch_val, hipStream_t st) {
     long long const n_val = (long long)(E) * PP * 2;
@@ -2175,11 +2228,16 @@ void launch_expand_ylane(double const* ydata, int E, int PP, int S_pad, double* 
         hipLaunchKernelGGL(patch_y_kernel, dim3(unsigned((np + 255) / 256)), dim3(256), 0, st, n_patch, PP * 2,
                            S_pad, patch_lane, patch_entry, patch_val, ylane);
 }
-void launch_converge(PairState const& ps, int it, int max_iter, double err_tol, int* active, hipStream_t st) {
+void launch_converge(PairState const& ps, int it, int max_iter, double err_tol, int* active, hipStream_t st,
+                     int const* it_dev) {
     if (ps.n == 0) return;
-    hipLaunchKernelGGL(converge_kernel, dim3(unsigned((ps.n + 255) / 256)), dim3(256), 0, st, ps, it, max_iter,
-                       err_tol, active);
+    hipLaunchKernelGGL(converge_kernel, dim3(unsigned((ps.n + 255) / 256)), dim3(256), 0, st, ps, it, it_dev,
+                       max_iter, err_tol, active);
 }
+void launch_begin_iteration(int* it_dev, int* active, hipStream_t st) {
+    hipLaunchKernelGGL(begin_iteration_kernel, dim3(1), dim3(64), 0, st, it_dev, active);
+}
+void launch_set_int(int* p, int v, hipStream_t st) { hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(64), 0, st, p, v); }
 void launch_ic_start(int P, DevGroup const* groups, long long const* bus_offsets, int n_groups, long long total,
                      hipStream_t st) {
     if (total == 0) return;

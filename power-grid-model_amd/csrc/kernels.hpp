@@ -112,6 +112,15 @@ struct ExtraWork {
 // (MODE_INIT: admittance blocks, MODE_NR: Jacobian blocks)
 void launch_lowrank(DevGroup const* groups, ExtraWork const* work, int n_work, bool fp32, hipStream_t st);
 
+// ids of an update buffer in HBM (4-byte ids every `stride` bytes): *flags |= 1 when any id is NA, 2 when any
+// is a number, 4 when some scenario's id differs from scenario 0's at the same position (eps rows per
+// scenario from row row0; eps <= 0: no comparison). *flags must start at 0.
+void launch_id_scan(char const* base, long long stride, long long total, long long row0, long long eps, int* flags,
+                    hipStream_t st);
+// union groups: per-member rows src[member][cols] to the lane-interleaved dst[cols][S_pad] (lane_member:
+// the member of every lane, -1 for padding lanes, which get 0)
+void launch_expand_member_rows(uint8_t const* src, int const* lane_member, int cols, int S_pad, uint8_t* dst,
+                               hipStream_t st);
 // union groups: Y values of every lane from the shared values plus per-lane patches of the entries
 // that switched branches touch (patch q: lane, entry and P*P*2 values)
 void launch_expand_ylane(double const* ydata, int E, int PP, int S_pad, double* ylane, int n_patch,
@@ -141,7 +150,13 @@ void launch_forward(DevGroup const* groups, WorkItem const* items, int n_items, 
 void launch_backward(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
                      hipStream_t st, int kind = KIND_PLAIN);
 // fold the deviation of iteration `it`, mark converged / diverged, count the still active pairs
-void launch_converge(PairState const& ps, int it, int max_iter, double err_tol, int* active, hipStream_t st);
+// it_dev: read the iteration number from this device counter instead of `it` (captured iterations)
+void launch_converge(PairState const& ps, int it, int max_iter, double err_tol, int* active, hipStream_t st,
+                     int const* it_dev = nullptr);
+// one thread: ++*it_dev, *active = 0 (the start of a captured iteration)
+void launch_begin_iteration(int* it_dev, int* active, hipStream_t st);
+// one thread: *p = v (a fill that graph capture records as a kernel node)
+void launch_set_int(int* p, int v, hipStream_t st);
 void launch_init_state(PairState const& ps, hipStream_t st);
 // one plain copy kernel over n floats (HBM counter calibration); 0 or -1
 int copy_probe(long long n);

@@ -447,8 +447,11 @@ struct TapChanger {
     std::vector<ScenarioRun> runs;
     // device chunks handed from each round's batch to the next, and the device copies of the caller's
     // load updates that every round over all scenarios reads
-    ChunkPool pool;
+    std::shared_ptr<ChunkPool> pool = std::make_shared<ChunkPool>();
     UploadCache uploads;
+    // plans of the tap states met in earlier rounds
+    PlanCache plan_cache;
+    PlanCache* plans = &plan_cache;
 
     TapChanger(Model const& model, PGM_Options const& o, PGM_ConstDataset const* upd)
         : m(model), opt(o), update(upd), S(upd ? upd->batch_size : 1), sym(o.symmetric != 0), P(sym ? 1 : 3),
@@ -834,7 +837,11 @@ struct TapChanger {
         Idx const S = Idx(scns.size());
         // only the caller's own buffers (rounds over every scenario) may go through the upload cache: the
         // copies of a subset round are temporaries whose addresses can recur with other contents
-        auto b = create_batch(m, round_options(method), &rd.ds, &pool, S == this->S ? &uploads : nullptr);
+        BatchContext ctx;
+        ctx.pool = pool;
+        ctx.uploads = S == this->S ? &uploads : nullptr;
+        ctx.plans = plans;
+        auto b = create_batch(m, round_options(method), &rd.ds, ctx);
         auto const t1 = std::chrono::steady_clock::now();
         PGM_MutableDataset out;
         out.meta = &meta_dataset(sym ? "sym_output" : "asym_output");
@@ -972,7 +979,11 @@ struct TapChanger {
     void finish(PGM_MutableDataset const& output, PGM_Handle* handle) {
         RoundData rd;
         build_round(rd, every_scenario());
-        auto b = create_batch(m, round_options(kRequested), &rd.ds, &pool, &uploads);
+        BatchContext ctx;
+        ctx.pool = pool;
+        ctx.uploads = &uploads;
+        ctx.plans = plans;
+        auto b = create_batch(m, round_options(kRequested), &rd.ds, ctx);
         b->is_batch = update != nullptr;
         b->set_output(output);
         b->run();

@@ -11,8 +11,9 @@ line_x1 [ohm]. The forward pass solves the batch on the GPU; the backward pass i
 costs about one extra Newton iteration. The line parameters are shared by all scenarios: their gradient
 is summed over the scenarios, weighted by the incoming gradient of each L_s.
 
-The adjoint needs every scenario in one topology group of more than 32 scenarios (a plain group).
-Scenarios that fail to converge give NaN losses and zero gradients.
+The batch is built with the adjoint layout (every topology group a plain group), so scenarios may differ
+in their load statuses as well as their powers. Scenarios that fail to converge give NaN losses and zero
+gradients.
 """
 
 from __future__ import annotations
@@ -37,7 +38,7 @@ def _solve(input_data, load_p, load_q, line_r1, line_x1, factor_precision, error
     upd["p_specified"] = load_p
     upd["q_specified"] = load_q
     batch = DeviceBatch(PowerGridModel(inp), {"sym_load": upd}, factor_precision=factor_precision,
-                        error_tolerance=error_tolerance, max_iterations=max_iterations)
+                        error_tolerance=error_tolerance, max_iterations=max_iterations, adjoint_layout=True)
     batch.set_output(OUTPUT)
     batch.run()
     out = batch.fetch()

@@ -223,6 +223,41 @@ class PowerGridModel:
     def copy(self) -> "PowerGridModel":
         return PowerGridModel(self)
 
+    def make_resident(self, update_data: dict) -> int:
+        """Copy the load_gen update buffers of a batch update dataset to HBM once: later calculations on
+        this model over the same arrays (same memory, unchanged) read the device copies instead of uploading
+        them again. The model keeps a reference to the arrays so that their memory is not reused while the
+        device copies exist. Returns the bytes uploaded. Extension of this build (no reference counterpart)."""
+        ds = _make_const_dataset(self._handle, update_data, "update", True)
+        n = lib.PGM_amd_model_make_resident(self._handle.ptr, self._ptr, ds.ptr)
+        self._handle.check()
+        self.__dict__.setdefault("_resident_keep", []).append(ds._keep)
+        return int(n)
+
+    def autotune(self, update_data: dict, *, symmetric: bool = True, error_tolerance: float = 1e-8,
+                 max_iterations: int = 20,
+                 calculation_method: CalculationMethod | str = CalculationMethod.newton_raphson,
+                 factor_precision: FactorPrecision | str = FactorPrecision.fp64) -> dict:
+        """Pick the schedule parameters of this batch shape by measurement: the batch is built and run with
+        candidate parameters, and the fastest are kept for later calculations on this model with the same
+        shape (number of scenarios, symmetry, factor precision, method). Extension of this build.
+        Returns {"params": {...}, "trials": [(params, device ms), ...]}."""
+        h = self._handle
+        opt = self._options(symmetric, error_tolerance, max_iterations, calculation_method, -1, factor_precision)
+        upd = _make_const_dataset(h, update_data, "update", True)
+        params = np.zeros(3, dtype=np.int64)
+        trials = np.zeros((64, 4), dtype=np.float64)
+        n = lib.PGM_amd_autotune(h.ptr, self._ptr, opt.ptr, upd.ptr, params.ctypes.data, trials.ctypes.data, 64)
+        h.check()
+        names = ("plain_waves", "union_tasks", "packed_target")
+        return {"params": dict(zip(names, params.tolist())),
+                "trials": [(dict(zip(names, (int(x) for x in t[:3]))), float(t[3])) for t in trials[:min(n, 64)]]}
+
+    def release_device_memory(self) -> None:
+        """Free the resident update buffers and the device memory kept from earlier calculations."""
+        lib.PGM_amd_model_release_device_memory(self._ptr)
+        self.__dict__.pop("_resident_keep", None)
+
     @property
     def all_component_count(self) -> dict[str, int]:
         return {k: v for k, v in self._all_count.items() if v > 0}
@@ -241,7 +276,7 @@ class PowerGridModel:
         return out
 
     def _options(self, symmetric, error_tolerance, max_iterations, calculation_method, threading,
-                 factor_precision, tap_changing_strategy=0):
+                 factor_precision, tap_changing_strategy=0, adjoint_layout=False):
         h = self._handle
         opt = lib.PGM_create_options(h.ptr)
         lib.PGM_set_calculation_type(h.ptr, opt, CalculationType.power_flow)
@@ -252,6 +287,7 @@ class PowerGridModel:
         lib.PGM_set_threading(h.ptr, opt, int(threading))
         lib.PGM_amd_set_factor_precision(h.ptr, opt, _enum_value(factor_precision, FactorPrecision))
         lib.PGM_set_tap_changing_strategy(h.ptr, opt, _enum_value(tap_changing_strategy, TapChangingStrategy))
+        lib.PGM_amd_set_adjoint_layout(h.ptr, opt, int(bool(adjoint_layout)))
         h.check()
         return _Dataset(opt, lib.PGM_destroy_options, [])
 
@@ -352,12 +388,15 @@ class DeviceBatch:
     def __init__(self, model: PowerGridModel, update_data: dict | None, *, symmetric: bool = True,
                  error_tolerance: float = 1e-8, max_iterations: int = 20,
                  factor_precision: FactorPrecision | str = FactorPrecision.fp64,
-                 calculation_method: CalculationMethod | str = CalculationMethod.newton_raphson):
+                 calculation_method: CalculationMethod | str = CalculationMethod.newton_raphson,
+                 adjoint_layout: bool = False):
+        """adjoint_layout: keep every topology group a plain group (switching scenarios are solved on their
+        own structure instead of sharing one), so that adjoint() accepts batches with switching scenarios."""
         self._model = model
         self._handle = Handle()
         self.symmetric = symmetric
         self._opt = model._options(symmetric, error_tolerance, max_iterations, calculation_method, -1,
-                                   factor_precision)
+                                   factor_precision, adjoint_layout=adjoint_layout)
         self._upd = _make_const_dataset(self._handle, update_data, "update", True) if update_data else None
         self._ptr = lib.PGM_amd_batch_create(self._handle.ptr, model._ptr, self._opt.ptr,
                                              None if self._upd is None else self._upd.ptr)
@@ -433,10 +472,10 @@ class DeviceBatch:
         return out
 
     def stats(self) -> dict:
-        out = np.zeros(11, dtype=np.float64)
+        out = np.zeros(12, dtype=np.float64)
         lib.PGM_amd_batch_stats(self._ptr, out.ctypes.data)
         keys = ["kernel_ms", "iterations_sum", "buses", "lu_entries", "groups", "fwd_launch_ms", "fwd_launches",
-                "buses_per_scenario", "fwd_bytes", "fwd_total_ms", "fwd_timed"]
+                "buses_per_scenario", "fwd_bytes", "fwd_total_ms", "fwd_timed", "graph_launches"]
         return dict(zip(keys, out.tolist()))
 
 
