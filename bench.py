@@ -274,6 +274,7 @@ def measure_workload(name, steps, warmup, threads, cpu_seconds, do_cpu):
         res[key] = {"value": case.n_scenarios / float(np.median(times)), "ms_per_step": 1e3 * float(np.median(times)),
                     "device_resident_value": case.n_scenarios / float(np.median(dr)),
                     "mean_iterations": float(np.mean(it[it > 0])) if np.any(it > 0) else 0.0,
+                    "pivot_perturbations_refined": int(st.get("refinements", 0)),
                     "failed_scenarios": int(np.sum(it == 0)), "roofline": roofline_of(st, P, prec == "fp32")}
         del b
     res["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
@@ -499,6 +500,7 @@ def main():
                    "buses": int(st["buses_per_scenario"]), "lu_entries": int(st["lu_entries"]),
                    "symmetric": case.symmetric, "load_scale_kappa": case.kappa, "topology_groups": int(st["groups"]),
                    "err_tol": 1e-8, "failed_scenarios": failed, "mean_iterations": float(np.mean(it[it > 0])),
+                   "pivot_perturbations_refined": int(st.get("refinements", 0)),
                    "step": "whole job with the update data resident in HBM: batch creation (host grouping, "
                            "topology and plans; device scatter of the load updates), solve, output columns in HBM",
                    "output": ("columnar " + ", ".join(f"{c}{{{','.join(a)}}}" for c, a in selection.items()))

@@ -234,6 +234,12 @@ PGM_AMD_API void PGM_amd_set_factor_precision(PGM_Handle* handle, PGM_Options* o
  * PGM_amd_batch_adjoint runs on; 0 (default): the fastest layout. Results are the same to FP64 rounding. */
 PGM_AMD_API void PGM_amd_set_adjoint_layout(PGM_Handle* handle, PGM_Options* opt, PGM_Idx value) PGM_AMD_NOEXCEPT;
 
+/* 1: every Newton-Raphson step of an FP32 factorisation (factor_precision 1) is refined in FP64 against the
+ * FP64 Jacobian, as the reference refines solves after pivot perturbation (sparse_lu_solver.hpp:514-649), so
+ * the iterates follow the FP64 path; 0 (default): only steps whose FP32 factorisation perturbed a pivot are
+ * refined. No counterpart in the reference, which has no FP32 factorisation. */
+PGM_AMD_API void PGM_amd_set_step_refinement(PGM_Handle* handle, PGM_Options* opt, PGM_Idx value) PGM_AMD_NOEXCEPT;
+
 /* Device-resident batch: build the batch (topology grouping, plans, device upload of every
  * scenario's update data) once, then run it any number of times with the inputs already in HBM.
  * PGM_calculate is build + run + fetch + destroy. Returns NULL on error (message in handle). */
@@ -299,8 +305,9 @@ PGM_AMD_API void PGM_amd_batch_iterations(PGM_AmdBatch const* batch, PGM_Idx* it
  * launches, out[7] = buses per scenario, out[8] = algorithmic bytes of the timed forward-sweep
  * launches (factor written and read once, 2 E b, plus 32 P bytes per bus, for the scenarios still
  * iterating), out[9] = their summed duration in ms (HIP events), out[10] = number of timed launches,
- * out[11] = Newton-Raphson iterations launched as graphs (0: one launch per level). out must hold 12
- * entries. */
+ * out[11] = Newton-Raphson iterations launched as graphs (0: one launch per level), out[12] = times the
+ * FP32 factorisation perturbed pivots and the held steps were refined in FP64, out[13] = residuals
+ * formed by those refinements. out must hold 14 entries. */
 PGM_AMD_API void PGM_amd_batch_stats(PGM_AmdBatch const* batch, double* out) PGM_AMD_NOEXCEPT;
 
 /* Host-only inspection of the product's topology for a model (no GPU needed): per node the math

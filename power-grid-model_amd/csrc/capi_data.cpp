@@ -417,6 +417,13 @@ void PGM_amd_set_adjoint_layout(PGM_Handle* handle, PGM_Options* opt, PGM_Idx va
     });
 }
 
+void PGM_amd_set_step_refinement(PGM_Handle* handle, PGM_Options* opt, PGM_Idx value) noexcept {
+    call_with_catch_void(handle, [&] {
+        if (value != 0 && value != 1) throw PowerGridError("step refinement must be 0 or 1");
+        opt->refine_steps = value;
+    });
+}
+
 PGM_Idx PGM_meta_n_datasets(PGM_Handle*) noexcept { return PGM_Idx(meta_datasets().size()); }
 PGM_MetaDataset const* PGM_meta_get_dataset_by_idx(PGM_Handle* handle, PGM_Idx idx) noexcept {
     return call_with_catch(

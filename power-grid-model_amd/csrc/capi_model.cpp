@@ -6,6 +6,7 @@
 #include "../../include/power_grid_model_amd.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <new>
 
@@ -216,6 +217,8 @@ void PGM_amd_batch_stats(PGM_AmdBatch const* batch, double* out) noexcept {
     out[9] = b.fwd_total_ms;
     out[10] = double(b.fwd_timed);
     out[11] = double(b.graph_launches);
+    out[12] = double(b.refinements);
+    out[13] = double(b.refine_passes);
 }
 
 PGM_Idx PGM_amd_inspect_topology(PGM_Handle* handle, PGM_PowerGridModel const* model, PGM_Idx* node_group,

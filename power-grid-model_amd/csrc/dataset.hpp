@@ -150,4 +150,7 @@ struct PGM_Options {
     // 1: every topology group of a batch is a plain group (no union or packed groups), the layout the
     // adjoint runs on
     pgm_amd::Idx adjoint_layout = 0;
+    // 1: every Newton-Raphson step of an FP32 factorisation is refined in FP64 against the FP64 Jacobian (not
+    // only the steps whose factorisation perturbed a pivot): the iterates are then the FP64 path's
+    pgm_amd::Idx refine_steps = 0;
 };

@@ -532,6 +532,12 @@ void upload_group(Batch& b, Group& g) {
     d.iters = b.pairs.iters + g.pair_off;
     d.devmax = b.pairs.devmax + g.pair_off;
     d.lastdev = b.pairs.lastdev + g.pair_off;
+    if (b.pairs.pert) {
+        d.pert = b.pairs.pert + g.pair_off;
+        d.rmax = b.pairs.rmax + g.pair_off;
+        d.berr = b.pairs.berr + g.pair_off;
+        d.refine_all = b.refine_steps ? 1 : 0;
+    }
     for (size_t l = 0; l + 1 < p.level_ptr.size(); ++l) {
         double ent = 0, piv = 0;
         for (int q = p.level_ptr[l]; q < p.level_ptr[l + 1]; ++q) {
@@ -1214,6 +1220,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     // links (y_link = 1e8 (1 + 1j) pu against branch admittances of order 1-100 pu) leave an FP32 factor no
     // significant digits for the rest of the grid: models with links are factored in FP64
     b.fp32 = opt.factor_precision == 1 && !b.linear && m.link.empty();
+    b.refine_steps = b.fp32 && opt.refine_steps != 0 && !b.current;
     b.is_batch = update != nullptr;
     b.n_scenarios = update ? update->batch_size : 1;
     PGM_HIP_CHECK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
@@ -1228,7 +1235,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     PGM_HIP_CHECK(hipEventCreateWithFlags(&b.ev_lead, hipEventDisableTiming));
     PGM_HIP_CHECK(hipEventCreate(&b.ev_start));
     PGM_HIP_CHECK(hipEventCreate(&b.ev_stop));
-    PGM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b.h_active), sizeof(int)));
+    PGM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b.h_active), 2 * sizeof(int)));
     Idx const S = b.n_scenarios;
     bool const sym = b.symmetric;
 
@@ -1680,9 +1687,12 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
                     int const pw = b.tuning.plain_waves > 0 ? b.tuning.plain_waves : plain_waves();
                     int const ut = b.tuning.union_tasks > 0 ? b.tuning.union_tasks : union_tasks();
                     int const pt = b.tuning.packed_target > 0 ? b.tuning.packed_target : packed_target();
+                    // a plain group of few 64-scenario blocks takes more, shorter tasks per level: its
+                    // levels would otherwise hold too few waves to hide the latency of the pivot chains
+                    // (feeders_64, 2 blocks: critical path 281 pivots at 256 tasks, 105 at 4096)
                     int const target = g.packed     ? pt
                                        : g.is_union ? std::clamp(pw / std::max(nblk, 1), 4, ut)
-                                                    : std::clamp(pw / std::max(nblk, 1), 4, 256);
+                                                    : std::clamp(pw / std::max(nblk, 1), 4, 4096);
                     int const ilv = g.packed ? kWaveLanes / int(g.scenarios.size()) : 0;
                     // the state's plan from an earlier batch of the model, or built now
                     std::string key;
@@ -1740,6 +1750,11 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     b.pairs.iters = dev_alloc<int>(b.arena, size_t(pairs));
     b.pairs.devmax = dev_alloc<unsigned long long>(b.arena, size_t(pairs));
     b.pairs.lastdev = dev_alloc<double>(b.arena, size_t(pairs));
+    if (b.fp32) {
+        b.pairs.pert = dev_alloc<int>(b.arena, size_t(pairs));
+        b.pairs.rmax = dev_alloc<unsigned long long>(b.arena, size_t(pairs));
+        b.pairs.berr = dev_alloc<unsigned long long>(b.arena, size_t(pairs));
+    }
     // pairs solved by the linear method: every pair for calculation_method = linear, else the math
     // models whose load_gens are all const_y (math_solver.hpp:43-64)
     {
@@ -1760,7 +1775,9 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             }
         if (any) b.d_pair_linear = put(b.arena, lin);
     }
-    b.d_active = dev_alloc<int>(b.arena, 1);
+    // [0] pairs still iterating after a check, [1] pairs whose step waits for a refinement
+    b.d_active = dev_alloc<int>(b.arena, 2);
+    b.d_gmax = dev_alloc<unsigned long long>(b.arena, 1);
     b.d_ok = dev_alloc<int>(b.arena, size_t(std::max<long long>(lanes, 1)));
     int8_t* lstat_all = dev_alloc<int8_t>(b.arena, size_t(std::max<long long>(lanes * m.n_lg(), 1)));
     {
@@ -2001,7 +2018,7 @@ void Batch::run() {
     if (graph_wanted && !graph_tried) {
         graph_tried = true;
         if (!d_iter) d_iter = static_cast<int*>(arena.take(sizeof(int)));
-        if (!h_act) PGM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_act), 2 * sizeof(int)));
+        if (!h_act) PGM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_act), 4 * sizeof(int)));
         for (int k = 0; k < 2; ++k)
             if (!ev_iter[k]) PGM_HIP_CHECK(hipEventCreateWithFlags(&ev_iter[k], hipEventDisableTiming));
         // the prologue's work on the second stream joins the main stream before the capture starts
@@ -2036,15 +2053,17 @@ void Batch::run() {
         if (!graph_ok) destroy_graphs();
     }
     graph_launches = 0;
+    refinements = 0;
+    refine_passes = 0;
     if (graph_wanted && graph_ok) {
         // the prologue's second-stream work joins before the first launch; every launch forks and joins
         join();
         launch_set_int(d_iter, 0, st);
-        int launched = 0, done_it = 0;
+        int launched = 0, done_it = 0, limit = int(max_iter), rejoined = 0;
         auto launch_next = [&] {
             int const k = launched % 2;
             PGM_HIP_CHECK(hipGraphLaunch(nr_exec[k], st));
-            PGM_HIP_CHECK(hipMemcpyAsync(&h_act[k], d_active, sizeof(int), hipMemcpyDeviceToHost, st));
+            PGM_HIP_CHECK(hipMemcpyAsync(&h_act[2 * k], d_active, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
             PGM_HIP_CHECK(hipEventRecord(ev_iter[k], st));
             ++launched;
         };
@@ -2053,7 +2072,7 @@ void Batch::run() {
         // the device never waits for the host; the iteration queued after the last one finds every lane
         // done and returns at once
         while (true) {
-            if (launched < int(max_iter)) launch_next();
+            if (launched < limit) launch_next();
             int const k = done_it % 2;
             PGM_HIP_CHECK(hipEventSynchronize(ev_iter[k]));
             ++done_it;
@@ -2067,13 +2086,23 @@ void Batch::run() {
                 }
             }
             fwd_launches += L;
-            int const act = h_act[k];
+            int act = h_act[2 * k] + rejoined;
+            rejoined = 0;
+            if (h_act[2 * k + 1] > 0) {
+                // steps from perturbed pivots: the iteration queued after this one runs without those pairs;
+                // once it is done their steps are refined and applied, and they rejoin one iteration behind
+                PGM_HIP_CHECK(hipStreamSynchronize(st));
+                rejoined = refine_held_steps();
+                act += rejoined;
+                ++limit;
+            }
             active_frac = real_pairs > 0 ? double(act) / double(real_pairs) : 0.0;
-            if (act == 0 || done_it >= int(max_iter)) break;
+            if (act == 0 || done_it >= limit) break;
         }
         graph_launches = launched;
     }
-    for (int it = 1; it <= ((linear || (graph_wanted && graph_ok)) ? 0 : int(max_iter)); ++it) {
+    int extra_iterations = 0;  // refinements: the pairs that waited are one iteration behind
+    for (int it = 1; it <= ((linear || (graph_wanted && graph_ok)) ? 0 : int(max_iter) + extra_iterations); ++it) {
         int const mode = current ? (it == 1 ? MODE_ICF : MODE_RESID) : MODE_NR;
         union_sweeps(mode);
         for (int l = 0; l < L; ++l) {
@@ -2091,15 +2120,20 @@ void Batch::run() {
         }
         for (int l = L - 1; l >= 0; --l) launch_backward(d_groups, d_items[l], n_items[l], P, fp32, mode, st);
         join();
-        *h_active = 0;
-        PGM_HIP_CHECK(hipMemcpyAsync(d_active, h_active, sizeof(int), hipMemcpyHostToDevice, st));
+        h_active[0] = h_active[1] = 0;
+        PGM_HIP_CHECK(hipMemcpyAsync(d_active, h_active, 2 * sizeof(int), hipMemcpyHostToDevice, st));
         launch_converge(pairs, it, int(max_iter), err_tol, d_active, st);
         fork();
-        PGM_HIP_CHECK(hipMemcpyAsync(h_active, d_active, sizeof(int), hipMemcpyDeviceToHost, st));
+        PGM_HIP_CHECK(hipMemcpyAsync(h_active, d_active, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         PGM_HIP_CHECK(hipStreamSynchronize(st));
+        int act = h_active[0];
+        if (h_active[1] > 0) {
+            act += refine_held_steps();
+            ++extra_iterations;
+        }
         // lanes still iterating, for the byte count of the next iteration's launches
-        active_frac = real_pairs > 0 ? double(*h_active) / double(real_pairs) : 0.0;
-        if (*h_active == 0) break;
+        active_frac = real_pairs > 0 ? double(act) / double(real_pairs) : 0.0;
+        if (act == 0) break;
     }
     PGM_HIP_CHECK(hipGetLastError());
     // per-scenario status
@@ -2205,6 +2239,54 @@ void Batch::run() {
     fwd_launch_ms = fwd_timed > 0 ? fwd_ms / double(fwd_timed) : 0.0;
     fwd_total_ms = fwd_ms;
     ran = true;
+}
+
+int Batch::refine_held_steps() {
+    hipStream_t const st = stream;
+    // the accumulated steps and the residual denominators of the plain and packed groups (union groups
+    // perturb no pivots), made at the first refinement of the batch
+    if (!refine_ready) {
+        for (auto& gp : groups) {
+            Group& g = *gp;
+            if (g.is_union || g.dg.xacc) continue;
+            size_t const cnt = size_t(g.plan.n) * 2 * size_t(P) * size_t(g.S_pad);
+            g.dg.xacc = dev_alloc<double>(arena, cnt);
+            g.dg.rden = dev_alloc<double>(arena, cnt);
+        }
+        std::vector<DevGroup> dg;
+        for (auto& gp : groups) dg.push_back(gp->dg);
+        PGM_HIP_CHECK(hipMemcpy(d_groups, dg.data(), dg.size() * sizeof(DevGroup), hipMemcpyHostToDevice));
+        refine_ready = true;
+    }
+    int const G = int(groups.size());
+    int const L = int(n_items.size()), LP = int(n_items_p.size());
+    launch_refine_start(pairs, d_groups, d_bus_off, G, bus_total, P, st);
+    for (int pass = 0;; ++pass) {
+        PGM_HIP_CHECK(hipMemsetAsync(d_gmax, 0, sizeof(unsigned long long), st));
+        launch_refine_residual(pairs, d_groups, d_bus_off, G, bus_total, P, d_gmax, st);
+        unsigned long long bits = 0;
+        PGM_HIP_CHECK(hipMemcpyAsync(&bits, d_gmax, sizeof bits, hipMemcpyDeviceToHost, st));
+        PGM_HIP_CHECK(hipStreamSynchronize(st));
+        double berr = 0.0;
+        std::memcpy(&berr, &bits, sizeof berr);
+        ++refine_passes;
+        if (berr <= (refine_steps ? kRefineTolSteps : kRefineTol) || pass == kRefineMaxPasses) break;
+        // the correction solved with the stored (perturbed) factors, added to the accumulated step
+        for (int l = 0; l < L; ++l) launch_forward(d_groups, d_items[l], n_items[l], P, fp32, MODE_REFINE, st);
+        for (int l = 0; l < LP; ++l)
+            launch_forward(d_groups, d_items_p[l], n_items_p[l], P, fp32, MODE_REFINE, st, KIND_PACKED);
+        for (int l = LP - 1; l >= 0; --l)
+            launch_backward(d_groups, d_items_p[l], n_items_p[l], P, fp32, MODE_REFINE, st, KIND_PACKED);
+        for (int l = L - 1; l >= 0; --l) launch_backward(d_groups, d_items[l], n_items[l], P, fp32, MODE_REFINE, st);
+    }
+    h_active[0] = h_active[1] = 0;
+    PGM_HIP_CHECK(hipMemcpyAsync(d_active, h_active, 2 * sizeof(int), hipMemcpyHostToDevice, st));
+    launch_refine_finish(pairs, d_groups, d_bus_off, G, bus_total, P, int(max_iter), err_tol, d_active, st);
+    PGM_HIP_CHECK(hipMemcpyAsync(h_active, d_active, sizeof(int), hipMemcpyDeviceToHost, st));
+    PGM_HIP_CHECK(hipStreamSynchronize(st));
+    PGM_HIP_CHECK(hipGetLastError());
+    ++refinements;
+    return h_active[0];
 }
 
 void Batch::destroy_graphs() {
@@ -2517,7 +2599,7 @@ BatchContext model_context(PGM_PowerGridModel const& model, PGM_Options const& o
 std::string tune_key(PGM_Options const& opt, PGM_ConstDataset const* update) {
     return std::to_string(update ? update->batch_size : 1) + (opt.symmetric ? "/sym" : "/asym") +
            (opt.factor_precision == 1 ? "/fp32" : "/fp64") + "/m" + std::to_string(int(opt.calculation_method)) +
-           (opt.adjoint_layout ? "/adjoint" : "");
+           (opt.adjoint_layout ? "/adjoint" : "") + (opt.refine_steps ? "/refine" : "");
 }
 
 Tuning autotune(PGM_PowerGridModel& model, PGM_Options const& opt, PGM_ConstDataset const* update,

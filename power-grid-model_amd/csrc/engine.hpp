@@ -132,6 +132,7 @@ struct Batch {
     double err_tol = 1e-8;
     Idx max_iter = 20;
     bool fp32 = false;
+    bool refine_steps = false;  // every FP32 Newton-Raphson step refined in FP64
     bool linear = false;   // calculation_method = linear
     bool current = false;  // iterative current or linear current
     bool is_batch = false;
@@ -205,6 +206,13 @@ struct Batch {
     int* d_iter = nullptr;
     int* h_act = nullptr;  // pinned, 2 entries
     int graph_launches = 0;  // iterations launched by the last run (the last may have found every lane done)
+    // pivot perturbation (FP32 factors): steps held by the sweeps are refined in FP64 and applied; returns the
+    // pairs that go on iterating. refinements / refine_passes: counts of the last run
+    int refine_held_steps();
+    bool refine_ready = false;
+    unsigned long long* d_gmax = nullptr;
+    int refinements = 0;
+    int refine_passes = 0;
     void destroy_graphs();
 
     Batch() = default;

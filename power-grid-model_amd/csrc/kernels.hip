@@ -112,9 +112,15 @@ __device__ __forceinline__ int perm_row(uint32_t code, int t) { return int((code
 __device__ __forceinline__ int perm_col(uint32_t code, int t) { return int((code >> (15 + 3 * t)) & 7u); }
 
 // full-pivot LU in place: P a Q = L U; returns false when the reference would raise SparseMatrixError
-template <int NB, class FT> __device__ __forceinline__ bool factor_block(FT (&a)[NB * NB], uint32_t& code) {
+// perturb_thr > 0 (perturb_pivot_if_needed, sparse_lu_solver.hpp:39-48 and :120-124): a chosen pivot whose
+// magnitude is below the threshold takes the threshold's magnitude with its own sign (+ for a zero pivot) and
+// `perturbed` is set; the pivot test then checks only for non-normal values, as the reference's does once it
+// has perturbed (:157-164)
+template <int NB, class FT>
+__device__ __forceinline__ bool factor_block(FT (&a)[NB * NB], uint32_t& code, FT perturb_thr, bool& perturbed) {
     FT max_pivot = FT(0);
     bool zero = false;
+    bool pert = false;
     code = 0;
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
@@ -133,13 +139,21 @@ template <int NB, class FT> __device__ __forceinline__ bool factor_block(FT (&a)
                 cb = gt ? c : cb;
             }
         }
-        zero = zero || best == FT(0);
-        max_pivot = sqrt(best) > max_pivot ? sqrt(best) : max_pivot;
         if (t < NB - 1) {
             code |= uint32_t(rb) << (3 * t);
             code |= uint32_t(cb) << (15 + 3 * t);
             swap_rows<NB, FT>(a, t, rb);
             swap_cols<NB, FT>(a, t, cb);
+        }
+        // the chosen pivot now sits at (t, t)
+        if (perturb_thr > FT(0) && best < perturb_thr * perturb_thr) {
+            a[t * NB + t] = a[t * NB + t] < FT(0) ? -perturb_thr : perturb_thr;
+            best = perturb_thr * perturb_thr;
+            pert = true;
+        }
+        zero = zero || best == FT(0);
+        max_pivot = sqrt(best) > max_pivot ? sqrt(best) : max_pivot;
+        if (t < NB - 1) {
             FT const piv = a[t * NB + t];
 #pragma unroll
             for (int r = 0; r < NB; ++r)
@@ -154,7 +168,8 @@ template <int NB, class FT> __device__ __forceinline__ bool factor_block(FT (&a)
         }
     }
     FT const eps = sizeof(FT) == 8 ? FT(DBL_EPSILON) : FT(FLT_EPSILON);
-    FT const thr = eps * max_pivot;
+    FT const thr = pert ? FT(0) : eps * max_pivot;
+    perturbed = pert;
     bool ok = !zero;
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
@@ -163,6 +178,10 @@ template <int NB, class FT> __device__ __forceinline__ bool factor_block(FT (&a)
         ok = ok && !(ad < thr) && is_normal_(d);
     }
     return ok;
+}
+template <int NB, class FT> __device__ __forceinline__ bool factor_block(FT (&a)[NB * NB], uint32_t& code) {
+    bool unused = false;
+    return factor_block<NB, FT>(a, code, FT(0), unused);
 }
 template <int NB, class T> __device__ __forceinline__ void apply_p_rows(T (&m)[NB * NB], uint32_t code) {
 #pragma unroll
@@ -396,8 +415,74 @@ __device__ __forceinline__ void extra_entries(DevGroup const& g, int p, int s, c
 // block, finish the U row and L column, forward-substitute
 // ---------------------------------------------------------------------------------------------------
 // waves per SIMD the sweeps are compiled for where 80 VGPRs hold without spilling
+// the bus's own terms of the Newton-Raphson system once the branch sums pcal, qcal of row p are formed: the
+// mismatch (rhs = -P_cal, -Q_cal, plus the loads) and the diagonal block's power, load (ZIP) and source terms
+// (newton_raphson_pf_solver.hpp:473-547, 764-852)
+template <int P>
+__device__ __forceinline__ void nr_bus_terms(DevGroup const& g, int p, int s, cplx const (&up)[P], double const (&pcal)[P],
+                                             double const (&qcal)[P], double (&D)[4 * P * P], double (&rhs)[2 * P]) {
+    constexpr int NB = 2 * P;
+    constexpr int NB2 = NB * NB;
+    constexpr int PP = P * P;
+    size_t const sp = g.S_pad;
+#pragma unroll
+    for (int ph = 0; ph < P; ++ph) {
+        rhs[ph] = -pcal[ph];
+        rhs[ph + P] = -qcal[ph];
+        D[ph * NB + ph] += -qcal[ph];
+        D[(ph + P) * NB + ph] += pcal[ph];
+        D[ph * NB + ph + P] += pcal[ph];
+        D[(ph + P) * NB + ph + P] += qcal[ph];
+    }
+    // loads: constant power / impedance / current (newton_raphson_pf_solver.hpp:764-822)
+    for (int l = ld(g.load_ptr, p); l < ld(g.load_ptr, p + 1); ++l) {
+        int const type = ld(g.load_type, l);
+#pragma unroll
+        for (int ph = 0; ph < P; ++ph) {
+            double const sr = g.s_inj[((size_t(l) * P + ph) * 2) * sp + s];
+            double const si = g.s_inj[((size_t(l) * P + ph) * 2 + 1) * sp + s];
+            double const v = g.xv[((size_t(p) * P + ph) * 2 + 1) * sp + s];
+            double const f = type == 0 ? 1.0 : (type == 1 ? v * v : v);
+            rhs[ph] += sr * f;
+            rhs[ph + P] += si * f;
+            double const df = type == 0 ? 0.0 : (type == 1 ? 2.0 * v * v : v);
+            D[(ph + P) * NB + ph] += -sr * df;
+            D[(ph + P) * NB + ph + P] += -si * df;
+        }
+    }
+    // sources: two-bus equivalent with y_ref (newton_raphson_pf_solver.hpp:824-852)
+    for (int q = ld(g.src_ptr, p); q < ld(g.src_ptr, p + 1); ++q) {
+        cplx const y1 = {g.y_src[(size_t(q) * 4) * sp + s], g.y_src[(size_t(q) * 4 + 1) * sp + s]};
+        cplx const y0 = {g.y_src[(size_t(q) * 4 + 2) * sp + s], g.y_src[(size_t(q) * 4 + 3) * sp + s]};
+        cplx const ur = {g.u_src[(size_t(q) * 2) * sp + s], g.u_src[(size_t(q) * 2 + 1) * sp + s]};
+        double ys[PP * 2], yn[PP * 2];
+        source_y<P>(y1, y0, ys);
+#pragma unroll
+        for (int k = 0; k < PP * 2; ++k) yn[k] = -ys[k];
+        cplx ub[P];
+        balanced<P>(ur, ub);
+        double bmm[NB2], bms[NB2];
+        double p1[P], q1[P], p2[P], q2[P];
+        hnml<P, double>(ys, up, up, bmm, p1, q1);
+        hnml<P, double>(yn, up, ub, bms, p2, q2);
+#pragma unroll
+        for (int r = 0; r < P; ++r) {
+            double const pc = p1[r] + p2[r];
+            double const qc = q1[r] + q2[r];
+            bmm[r * NB + r] += -qc;
+            bmm[(r + P) * NB + r] += pc;
+            bmm[r * NB + r + P] += pc;
+            bmm[(r + P) * NB + r + P] += qc;
+            rhs[r] -= pc;
+            rhs[r + P] -= qc;
+        }
+#pragma unroll
+        for (int k = 0; k < NB2; ++k) D[k] += bmm[k];
+    }
+}
+
 template <int P, class FT, int KIND> constexpr int forward_waves() {
-    return P == 1 && KIND == KIND_PLAIN && sizeof(FT) == 4 ? 6 : 1;
+    return P == 1 && KIND == KIND_PLAIN && sizeof(FT) == 4 ? 5 : 1;
 }
 template <int P, int MODE, int KIND> constexpr int backward_waves() {
     return P == 1 && KIND != KIND_PACKED ? (MODE == MODE_NR ? 8 : 6) : 1;
@@ -416,7 +501,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(forward_w
     if (!lane_work<KIND>(g, wi, task, s)) return;
     size_t const sp = g.S_pad;
     int const mm = ld(g.task_mm, task);
-    if (g.done[size_t(mm) * sp + s] != (MODE == MODE_NRA ? CONVERGED : ACTIVE)) return;
+    if (g.done[size_t(mm) * sp + s] != (MODE == MODE_NRA ? CONVERGED : MODE == MODE_REFINE ? REFINE : ACTIVE)) return;
     Blocks<NB, FT> const lu{static_cast<FT*>(g.lu), sp, s};
     // a keep row whose parent is the next pivot of the task hands its finished U_p,parent,
     // L_parent,p and y_p to that pivot through LDS (fw_le: label of the forwarded L entry); LDS keeps
@@ -432,11 +517,69 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(forward_w
     // L_parent,p of a handing row is needed by nobody but the parent, which takes it from LDS
     // (NR and the linear system; union groups keep it for the low-rank correction's chains)
     constexpr bool kSkipL = kFwd && (MODE == MODE_NR || MODE == MODE_INIT) && KIND != KIND_UNION;
+    // FP32 factors of a Newton-Raphson sweep in plain and packed groups may perturb small pivots
+    constexpr bool kPerturb = sizeof(FT) == 4 && MODE == MODE_NR && KIND != KIND_UNION;
+    __shared__ FT thr_s[kPerturb ? kWave : 1];
+    int handed = -1;  // MODE_REFINE: the previous pivot when it handed its L block over (not stored)
 
     for (int ti = ld(g.task_ptr, task); ti < ld(g.task_ptr, task + 1); ++ti) {
         int const p = ld(g.task_pivot, ti);
         int const rb = ld(g.row_ptr, p), re = ld(g.row_end, p), dg = ld(g.diag, p);
         int const es = g.row_stride ? ld(g.row_stride, p) : 1;
+        if constexpr (MODE == MODE_REFINE) {
+            // forward substitution of the residual (in y) with the stored factors: y_p = L_pp^-1 P (r_p -
+            // sum_k L_pk y_k). A child that handed its results to p through LDS in the Newton-Raphson sweep did
+            // not store its L block: it is formed again as that sweep formed it, J_pj Q_j U_jj^-1 (a keep row
+            // takes no Schur update in that block)
+            double v[NB];
+#pragma unroll
+            for (int r = 0; r < NB; ++r) v[r] = g.y[(size_t(p) * NB + r) * sp + s];
+            for (int k = rb; k < dg; k += es) {
+                int const j = ld(g.col, k);
+                FT L[NB2];
+                if (kFwd && KIND != KIND_UNION && j == handed) {
+                    if (ld(g.fill, k)) {
+#pragma unroll
+                        for (int q = 0; q < NB2; ++q) L[q] = FT(0);
+                    } else {
+                        cplx up[P], uj[P];
+                        load_u<P>(g, p, s, up);
+                        load_u<P>(g, j, s, uj);
+                        double yk[PP * 2], pr[P], qr[P];
+                        load_y<P, false>(g, k, s, yk);
+                        hnml<P, FT>(yk, up, uj, L, pr, qr);
+                        FT Dj[NB2];
+                        lu.load(ld(g.diag, j), Dj);
+                        apply_q_cols<NB, FT>(L, g.perm[size_t(j) * sp + s]);
+                        upper_right<NB, FT>(Dj, L);
+                    }
+                } else {
+                    lu.load(k, L);
+                }
+                double yj[NB];
+#pragma unroll
+                for (int r = 0; r < NB; ++r) yj[r] = g.y[(size_t(j) * NB + r) * sp + s];
+#pragma unroll
+                for (int r = 0; r < NB; ++r) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) acc += double(L[c * NB + r]) * yj[c];
+                    v[r] -= acc;
+                }
+            }
+            FT Dst[NB2];
+            lu.load(dg, Dst);
+            apply_p_vec<NB>(v, g.perm[size_t(p) * sp + s]);
+            lower_vec<NB, FT>(Dst, v);
+#pragma unroll
+            for (int r = 0; r < NB; ++r) g.y[(size_t(p) * NB + r) * sp + s] = v[r];
+            // the hand-over condition of the Newton-Raphson sweep (keep row whose parent is the next pivot)
+            bool const keep = P == 1 && ld(g.keep, p) != 0;
+            handed = keep && ti + 1 < ld(g.task_ptr, task + 1) && ld(g.task_pivot, ti + 1) == ld(g.col, dg + es)
+                         ? p
+                         : -1;
+            continue;
+        }
         // a bus without a source in this lane (union groups): identity row, zero row and column
         bool const dead_p = UNI && g.dead[size_t(p) * sp + s] != 0;
         // rows with one U entry and no off-diagonal Schur target (every bus of a radial feeder) keep
@@ -793,60 +936,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(forward_w
                     fw_u[lane][1] = up[0].im;
                 }
             }
-#pragma unroll
-            for (int ph = 0; ph < P; ++ph) {
-                rhs[ph] = -pcal[ph];
-                rhs[ph + P] = -qcal[ph];
-                D[ph * NB + ph] += -qcal[ph];
-                D[(ph + P) * NB + ph] += pcal[ph];
-                D[ph * NB + ph + P] += pcal[ph];
-                D[(ph + P) * NB + ph + P] += qcal[ph];
-            }
-            // loads: constant power / impedance / current (newton_raphson_pf_solver.hpp:764-822)
-            for (int l = ld(g.load_ptr, p); l < ld(g.load_ptr, p + 1); ++l) {
-                int const type = ld(g.load_type, l);
-#pragma unroll
-                for (int ph = 0; ph < P; ++ph) {
-                    double const sr = g.s_inj[((size_t(l) * P + ph) * 2) * sp + s];
-                    double const si = g.s_inj[((size_t(l) * P + ph) * 2 + 1) * sp + s];
-                    double const v = g.xv[((size_t(p) * P + ph) * 2 + 1) * sp + s];
-                    double const f = type == 0 ? 1.0 : (type == 1 ? v * v : v);
-                    rhs[ph] += sr * f;
-                    rhs[ph + P] += si * f;
-                    double const df = type == 0 ? 0.0 : (type == 1 ? 2.0 * v * v : v);
-                    D[(ph + P) * NB + ph] += -sr * df;
-                    D[(ph + P) * NB + ph + P] += -si * df;
-                }
-            }
-            // sources: two-bus equivalent with y_ref (newton_raphson_pf_solver.hpp:824-852)
-            for (int q = ld(g.src_ptr, p); q < ld(g.src_ptr, p + 1); ++q) {
-                cplx const y1 = {g.y_src[(size_t(q) * 4) * sp + s], g.y_src[(size_t(q) * 4 + 1) * sp + s]};
-                cplx const y0 = {g.y_src[(size_t(q) * 4 + 2) * sp + s], g.y_src[(size_t(q) * 4 + 3) * sp + s]};
-                cplx const ur = {g.u_src[(size_t(q) * 2) * sp + s], g.u_src[(size_t(q) * 2 + 1) * sp + s]};
-                double ys[PP * 2], yn[PP * 2];
-                source_y<P>(y1, y0, ys);
-#pragma unroll
-                for (int k = 0; k < PP * 2; ++k) yn[k] = -ys[k];
-                cplx ub[P];
-                balanced<P>(ur, ub);
-                double bmm[NB2], bms[NB2];
-                double p1[P], q1[P], p2[P], q2[P];
-                hnml<P, double>(ys, up, up, bmm, p1, q1);
-                hnml<P, double>(yn, up, ub, bms, p2, q2);
-#pragma unroll
-                for (int r = 0; r < P; ++r) {
-                    double const pc = p1[r] + p2[r];
-                    double const qc = q1[r] + q2[r];
-                    bmm[r * NB + r] += -qc;
-                    bmm[(r + P) * NB + r] += pc;
-                    bmm[r * NB + r + P] += pc;
-                    bmm[(r + P) * NB + r + P] += qc;
-                    rhs[r] -= pc;
-                    rhs[r + P] -= qc;
-                }
-#pragma unroll
-                for (int k = 0; k < NB2; ++k) D[k] += bmm[k];
-            }
+            nr_bus_terms<P>(g, p, s, up, pcal, qcal, D, rhs);
         }
 
         // Schur updates owned by pivot p, pulled from the already factorised descendants. The diagonal
@@ -855,6 +945,16 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(forward_w
         FT Df[NB2];
 #pragma unroll
         for (int k = 0; k < NB2; ++k) Df[k] = FT(D[k]);
+        // FP32 factors of a Newton-Raphson sweep (plain and packed groups): the perturbation threshold from the
+        // row's scale, the largest entry of its assembled diagonal block (the Jacobian's rows are close to
+        // diagonally dominant, so this is about the row's norm)
+        // (kept in LDS through the Schur updates, which hold every register they can get)
+        if constexpr (kPerturb) {
+            double scale = 0.0;
+#pragma unroll
+            for (int q = 0; q < NB2; ++q) scale = fabs(D[q]) > scale ? fabs(D[q]) : scale;
+            thr_s[lane] = FT(kPivotPerturbation * scale);
+        }
         for (int q = ld(g.schur_ptr, p); q < ld(g.schur_ptr, p + 1); ++q) {
             int const tgt = ld(g.schur, 3 * q), le = ld(g.schur, 3 * q + 1), ue = ld(g.schur, 3 * q + 2);
             FT L[NB2], U[NB2];
@@ -912,7 +1012,14 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(forward_w
         }
 
         uint32_t code;
-        bool const ok = factor_block<NB, FT>(Df, code);
+        // a pivot the FP32 factor cannot carry is perturbed (the step is then refined in FP64, see
+        // refine_residual_kernel); without perturbation the pivot test is the reference's
+        bool perturbed = false;
+        bool const ok = factor_block<NB, FT>(Df, code, kPerturb ? thr_s[lane] : FT(0), perturbed);
+        if constexpr (kPerturb) {
+            // 2: a pivot was perturbed, the refinement must reach its bound
+            if (perturbed) atomicMax(&g.pert[size_t(mm) * sp + s], 2);
+        }
         if (!ok) {
             if constexpr (MODE != MODE_NRA) {
                 g.done[size_t(mm) * sp + s] = FAILED;
@@ -974,6 +1081,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(forward_w
             }
         }
     }
+    if constexpr (kPerturb) {
+        // 1: refined on request only
+        if (g.refine_all) atomicMax(&g.pert[size_t(mm) * sp + s], 1);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -990,7 +1101,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(backward_
     if (!lane_work<KIND>(g, wi, task, s)) return;
     size_t const sp = g.S_pad;
     int const mm = ld(g.task_mm, task);
-    if (g.done[size_t(mm) * sp + s] != ACTIVE) return;
+    if (g.done[size_t(mm) * sp + s] != (MODE == MODE_REFINE ? REFINE : ACTIVE)) return;
+    // Newton-Raphson: a step from perturbed pivots is not applied here; it waits for its refinement
+    bool const held = MODE == MODE_NR && sizeof(FT) == 4 && KIND != KIND_UNION && g.pert &&
+                      g.pert[size_t(mm) * sp + s] != 0;
     Blocks<NB, FT> const lu{static_cast<FT*>(g.lu), sp, s};
     double dev = 0.0;
     // x of the pivot solved last, handed to the next pivot through LDS (its parent's value when the
@@ -1056,7 +1170,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(backward_
                 dbus = d > dbus ? d : dbus;
             }
             dev = dbus > dev ? dbus : dev;
+        } else if constexpr (MODE == MODE_REFINE) {
+#pragma unroll
+            for (int r = 0; r < NB; ++r) g.xacc[(size_t(p) * NB + r) * sp + s] += v[r];
         } else if constexpr (MODE == MODE_NR) {
+            if (held) continue;
             double dbus = 0.0;
 #pragma unroll
             for (int ph = 0; ph < P; ++ph) {
@@ -1084,7 +1202,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(backward_
         }
     }
     if constexpr (MODE == MODE_NR || MODE == MODE_ICF || MODE == MODE_RESID) {
-        atomicMax(&g.devmax[size_t(mm) * sp + s], static_cast<unsigned long long>(__double_as_longlong(dev)));
+        if (!held)
+            atomicMax(&g.devmax[size_t(mm) * sp + s], static_cast<unsigned long long>(__double_as_longlong(dev)));
     }
 }
 
@@ -1519,66 +1638,79 @@ __global__ void load_broadcast_kernel(double* __restrict__ s_inj, int8_t* __rest
     }
 }
 
-__global__ void load_scatter_kernel(double* __restrict__ s_inj, int8_t* __restrict__ lstat, int S, int S_pad,
-                                    int const* __restrict__ scenario_index, LoadUpdateSource const src,
-                                    LoadTable const tab) {
-    long long const i = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= (long long)(S) * src.n_elem) return;
-    int const e = int(i / S), lane = int(i % S);
-    long long const row = (long long)(ld(scenario_index, lane)) * src.n_elem + e;
-    int const l = ld(src.comp, e);
-    bool st = ld(tab.status, l) != 0;
-    double p[3], q[3];
-#pragma unroll
-    for (int ph = 0; ph < 3; ++ph) {
-        p[ph] = ld(tab.p, 3 * l + ph);
-        q[ph] = ld(tab.q, 3 * l + ph);
-    }
-    if (src.status) {
-        int8_t const v = *reinterpret_cast<int8_t const*>(src.status + row * src.status_stride);
-        if (v != int8_t(-128)) st = v != 0;
-    }
+// one tile of 64 lanes x 16 update elements per block: the update rows are read row-wise (each scenario's 16
+// records in one piece), staged in LDS, and the lane-interleaved injections and statuses written lane-wise
+constexpr int kScatterLanes = 64, kScatterElems = 16;
+__global__ __launch_bounds__(256) void load_scatter_kernel(double* __restrict__ s_inj, int8_t* __restrict__ lstat,
+                                                            int S, int S_pad, int const* __restrict__ scenario_index,
+                                                            LoadUpdateSource const src, LoadTable const tab) {
+    __shared__ double ps[kScatterLanes][kScatterElems][3], qs[kScatterLanes][kScatterElems][3];
+    __shared__ int8_t ss[kScatterLanes][kScatterElems];
+    int const lane0 = int(blockIdx.x) * kScatterLanes, e0 = int(blockIdx.y) * kScatterElems;
     int const nph = src.asym ? 3 : 1;
-    for (int ph = 0; ph < nph; ++ph) {
-        if (src.p) {
-            double const v = reinterpret_cast<double const*>(src.p + row * src.p_stride)[ph];
-            if (!isnan(v)) p[ph] = v;
-        }
-        if (src.q) {
-            double const v = reinterpret_cast<double const*>(src.q + row * src.q_stride)[ph];
-            if (!isnan(v)) q[ph] = v;
+    // rows in: element fastest
+    for (int t = int(threadIdx.x); t < kScatterLanes * kScatterElems; t += int(blockDim.x)) {
+        int const le = t % kScatterElems, ll = t / kScatterElems;
+        int const lane = lane0 + ll, e = e0 + le;
+        if (lane >= S || e >= src.n_elem) continue;
+        long long const row = (long long)(ld(scenario_index, lane)) * src.n_elem + e;
+        ss[ll][le] = src.status ? *reinterpret_cast<int8_t const*>(src.status + row * src.status_stride) : int8_t(-128);
+        for (int ph = 0; ph < nph; ++ph) {
+            ps[ll][le][ph] = src.p ? reinterpret_cast<double const*>(src.p + row * src.p_stride)[ph] : __builtin_nan("");
+            qs[ll][le][ph] = src.q ? reinterpret_cast<double const*>(src.q + row * src.q_stride)[ph] : __builtin_nan("");
         }
     }
+    __syncthreads();
+    // lanes out: lane fastest
     size_t const sp = S_pad;
-    lstat[size_t(l) * sp + lane] = st ? 1 : 0;
-    int const k = ld(tab.math, l);
-    if (k < 0) return;
-    double re[3] = {0.0, 0.0, 0.0}, im[3] = {0.0, 0.0, 0.0};
-    if (st) {
-        double const dir = ld(tab.gen, l) ? 1.0 : -1.0;
-        if (!ld(tab.asym, l)) {
-            double const r0 = dir * p[0] / 1e6, i0 = dir * q[0] / 1e6;
-            bool const bad = isnan(r0) || isnan(i0);
+    for (int t = int(threadIdx.x); t < kScatterLanes * kScatterElems; t += int(blockDim.x)) {
+        int const ll = t % kScatterLanes, le = t / kScatterLanes;
+        int const lane = lane0 + ll, e = e0 + le;
+        if (lane >= S || e >= src.n_elem) continue;
+        int const l = ld(src.comp, e);
+        bool st = ld(tab.status, l) != 0;
+        double p[3], q[3];
 #pragma unroll
-            for (int ph = 0; ph < 3; ++ph) {
-                re[ph] = bad ? __builtin_nan("") : r0;
-                im[ph] = bad ? __builtin_nan("") : i0;
-            }
-        } else {
+        for (int ph = 0; ph < 3; ++ph) {
+            p[ph] = ld(tab.p, 3 * l + ph);
+            q[ph] = ld(tab.q, 3 * l + ph);
+        }
+        int8_t const v = ss[ll][le];
+        if (v != int8_t(-128)) st = v != 0;
+        for (int ph = 0; ph < nph; ++ph) {
+            if (!isnan(ps[ll][le][ph])) p[ph] = ps[ll][le][ph];
+            if (!isnan(qs[ll][le][ph])) q[ph] = qs[ll][le][ph];
+        }
+        lstat[size_t(l) * sp + lane] = st ? 1 : 0;
+        int const k = ld(tab.math, l);
+        if (k < 0) continue;
+        double re[3] = {0.0, 0.0, 0.0}, im[3] = {0.0, 0.0, 0.0};
+        if (st) {
+            double const dir = ld(tab.gen, l) ? 1.0 : -1.0;
+            if (!ld(tab.asym, l)) {
+                double const r0 = dir * p[0] / 1e6, i0 = dir * q[0] / 1e6;
+                bool const bad = isnan(r0) || isnan(i0);
 #pragma unroll
-            for (int ph = 0; ph < 3; ++ph) {
-                re[ph] = dir * p[ph] / (1e6 / 3.0);
-                im[ph] = dir * q[ph] / (1e6 / 3.0);
-            }
-            if (tab.symmetric) {
-                re[0] = (re[0] + re[1] + re[2]) / 3.0;
-                im[0] = (im[0] + im[1] + im[2]) / 3.0;
+                for (int ph = 0; ph < 3; ++ph) {
+                    re[ph] = bad ? __builtin_nan("") : r0;
+                    im[ph] = bad ? __builtin_nan("") : i0;
+                }
+            } else {
+#pragma unroll
+                for (int ph = 0; ph < 3; ++ph) {
+                    re[ph] = dir * p[ph] / (1e6 / 3.0);
+                    im[ph] = dir * q[ph] / (1e6 / 3.0);
+                }
+                if (tab.symmetric) {
+                    re[0] = (re[0] + re[1] + re[2]) / 3.0;
+                    im[0] = (im[0] + im[1] + im[2]) / 3.0;
+                }
             }
         }
-    }
-    for (int ph = 0; ph < tab.P; ++ph) {
-        s_inj[((size_t(k) * tab.P + ph) * 2) * sp + lane] = re[ph];
-        s_inj[((size_t(k) * tab.P + ph) * 2 + 1) * sp + lane] = im[ph];
+        for (int ph = 0; ph < tab.P; ++ph) {
+            s_inj[((size_t(k) * tab.P + ph) * 2) * sp + lane] = re[ph];
+            s_inj[((size_t(k) * tab.P + ph) * 2 + 1) * sp + lane] = im[ph];
+        }
     }
 }
 
@@ -1590,6 +1722,11 @@ __global__ void init_state_kernel(PairState ps) {
     ps.iters[i] = 0;
     ps.devmax[i] = 0ull;
     ps.lastdev[i] = 0.0;
+    if (ps.pert) {
+        ps.pert[i] = 0;
+        ps.rmax[i] = 0ull;
+        ps.berr[i] = 0ull;
+    }
 }
 
 __global__ void retire_linear_kernel(PairState ps, uint8_t const* __restrict__ pair_linear) {
@@ -1606,8 +1743,17 @@ __global__ void converge_kernel(PairState ps, int it_host, int const* __restrict
     size_t const i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= size_t(ps.n)) return;
     if (ps.done[i] != ACTIVE) return;
-    // the iteration number from the host, or from the device counter (graph launches)
-    int const it = it_dev ? *it_dev : it_host;
+    // a step from perturbed pivots waits for its refinement; the pair takes no part in the sweeps meanwhile
+    if (ps.pert && ps.pert[i]) {
+        ps.done[i] = REFINE;
+        atomicAdd(active + 1, 1);
+        return;
+    }
+    // every active pair iterates once per check: its own count (a pair that waited for a refinement is one
+    // behind the launches)
+    (void)it_host;
+    (void)it_dev;
+    int const it = ps.iters[i] + 1;
     double const md = __longlong_as_double(static_cast<long long>(ps.devmax[i]));
     ps.devmax[i] = 0ull;
     ps.iters[i] = it;
@@ -1627,7 +1773,211 @@ __global__ void converge_kernel(PairState ps, int it_host, int const* __restrict
 __global__ void begin_iteration_kernel(int* it, int* active) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         *it += 1;
-        *active = 0;
+        active[0] = 0;
+        active[1] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// iterative refinement of Newton-Raphson steps whose FP32 factorisation perturbed pivots
+// (sparse_lu_solver.hpp:514-649: residual with the original matrix, componentwise backward error with the
+// denominator capped below at 1e-4 of its largest value, correction solved with the perturbed factors)
+// ---------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
+    atomicMax(p, static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+__device__ __forceinline__ double from_bits(unsigned long long b) { return __longlong_as_double(static_cast<long long>(b)); }
+
+// the (group, bus, lane) of a flat index over every group's buses and lanes, when the lane's pair is in REFINE
+struct RefineLane {
+    int gr, bus, s, mm;
+};
+__device__ __forceinline__ bool refine_lane(DevGroup const* groups, long long const* offsets, int n_groups, RefineLane& r,
+                                            DevGroup& g) {
+    long long const gidx = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gidx >= ld(offsets, n_groups)) return false;
+    r.gr = find_group(offsets, n_groups, gidx);
+    g = ld(groups, r.gr);
+    size_t const sp = size_t(g.S_pad);
+    size_t const idx = size_t(gidx - ld(offsets, r.gr));
+    r.bus = int(idx / sp);
+    r.s = int(idx % sp);
+    if (r.s >= g.S || !g.xacc) return false;
+    r.mm = ld(g.bus_mm, r.bus);
+    return r.mm >= 0 && g.done[size_t(r.mm) * sp + r.s] == REFINE;
+}
+
+__global__ void refine_reset_kernel(PairState ps) {
+    size_t const i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(ps.n) || ps.done[i] != REFINE) return;
+    ps.rmax[i] = 0ull;
+    ps.berr[i] = 0ull;
+}
+
+template <int P>
+__global__ void refine_start_kernel(DevGroup const* __restrict__ groups, long long const* __restrict__ offsets,
+                                    int n_groups) {
+    RefineLane r;
+    DevGroup g;
+    if (!refine_lane(groups, offsets, n_groups, r, g)) return;
+    size_t const sp = g.S_pad;
+    // the step of the Newton-Raphson sweep, solved with the perturbed factors, starts the accumulation
+#pragma unroll
+    for (int c = 0; c < 2 * P; ++c) g.xacc[(size_t(r.bus) * 2 * P + c) * sp + r.s] = g.y[(size_t(r.bus) * 2 * P + c) * sp + r.s];
+}
+
+// residual r = rhs - J x of row `bus` at the present voltages (J applied in FP64, as assembled by the sweep
+// before its FP32 rounding), into the forward vector; |rhs| + |J| |x| into rden and its largest value per pair
+template <int P>
+__global__ void refine_residual_kernel(DevGroup const* __restrict__ groups, long long const* __restrict__ offsets,
+                                       int n_groups) {
+    constexpr int NB = 2 * P;
+    constexpr int NB2 = NB * NB;
+    constexpr int PP = P * P;
+    RefineLane r;
+    DevGroup g;
+    if (!refine_lane(groups, offsets, n_groups, r, g)) return;
+    size_t const sp = g.S_pad;
+    int const p = r.bus, s = r.s;
+    int const rb = ld(g.row_ptr, p), re = ld(g.row_end, p);
+    int const es = g.row_stride ? ld(g.row_stride, p) : 1;
+    auto X = [&](int bus, int c) { return g.xacc[(size_t(bus) * NB + c) * sp + s]; };
+    cplx up[P];
+    load_u<P>(g, p, s, up);
+    double D[NB2], rhs[NB], jx[NB], ajx[NB], pcal[P], qcal[P];
+#pragma unroll
+    for (int q = 0; q < NB2; ++q) D[q] = 0.0;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) jx[c] = ajx[c] = 0.0;
+#pragma unroll
+    for (int ph = 0; ph < P; ++ph) pcal[ph] = qcal[ph] = 0.0;
+    for (int k = rb; k < re; k += es) {
+        int const j = ld(g.col, k);
+        if (ld(g.fill, k)) continue;
+        double yk[PP * 2], pr[P], qr[P];
+        load_y<P, false>(g, k, s, yk);
+        if (j == p) {
+            hnml<P, double>(yk, up, up, D, pr, qr);
+        } else {
+            cplx uj[P];
+            load_u<P>(g, j, s, uj);
+            double a[NB2];
+            hnml<P, double>(yk, up, uj, a, pr, qr);
+#pragma unroll
+            for (int rr = 0; rr < NB; ++rr)
+#pragma unroll
+                for (int c = 0; c < NB; ++c) {
+                    double const xj = X(j, c);
+                    jx[rr] += a[c * NB + rr] * xj;
+                    ajx[rr] += fabs(a[c * NB + rr]) * fabs(xj);
+                }
+        }
+#pragma unroll
+        for (int ph = 0; ph < P; ++ph) {
+            pcal[ph] += pr[ph];
+            qcal[ph] += qr[ph];
+        }
+    }
+    nr_bus_terms<P>(g, p, s, up, pcal, qcal, D, rhs);
+    double den_max = 0.0;
+#pragma unroll
+    for (int rr = 0; rr < NB; ++rr) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            double const xp = X(p, c);
+            jx[rr] += D[c * NB + rr] * xp;
+            ajx[rr] += fabs(D[c * NB + rr]) * fabs(xp);
+        }
+        double const den = fabs(rhs[rr]) + ajx[rr];
+        g.y[(size_t(p) * NB + rr) * sp + s] = rhs[rr] - jx[rr];
+        g.rden[(size_t(p) * NB + rr) * sp + s] = den;
+        den_max = den > den_max ? den : den_max;
+    }
+    atomic_max_nonneg(&g.rmax[size_t(r.mm) * sp + s], den_max);
+}
+
+// componentwise backward error of the residual, the denominators capped below at 1e-4 of the pair's largest
+// (iterate_and_backward_error, sparse_lu_solver.hpp:582-626); per pair and over all pairs
+template <int P>
+__global__ void refine_berr_kernel(DevGroup const* __restrict__ groups, long long const* __restrict__ offsets,
+                                   int n_groups, unsigned long long* __restrict__ gmax) {
+    constexpr int NB = 2 * P;
+    RefineLane r;
+    DevGroup g;
+    if (!refine_lane(groups, offsets, n_groups, r, g)) return;
+    size_t const sp = g.S_pad;
+    double const cap = 1e-4 * from_bits(g.rmax[size_t(r.mm) * sp + r.s]);
+    double be = 0.0;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        size_t const i = (size_t(r.bus) * NB + c) * sp + r.s;
+        double const den = g.rden[i] > cap ? g.rden[i] : cap;
+        double const e = den > 0.0 ? fabs(g.y[i]) / den : (g.y[i] == 0.0 ? 0.0 : __builtin_inf());
+        be = e > be ? e : be;
+    }
+    atomic_max_nonneg(&g.berr[size_t(r.mm) * sp + r.s], be);
+    atomic_max_nonneg(gmax, be);
+}
+
+// the refined step applied as the Newton-Raphson sweep applies a step (newton_raphson_pf_solver.hpp:325-349)
+template <int P>
+__global__ void refine_apply_kernel(DevGroup const* __restrict__ groups, long long const* __restrict__ offsets,
+                                    int n_groups) {
+    RefineLane r;
+    DevGroup g;
+    if (!refine_lane(groups, offsets, n_groups, r, g)) return;
+    size_t const sp = g.S_pad;
+    int const p = r.bus, s = r.s;
+    double dbus = 0.0;
+#pragma unroll
+    for (int ph = 0; ph < P; ++ph) {
+        size_t const b = (size_t(p) * P + ph) * 2;
+        double const dth = g.xacc[(size_t(p) * 2 * P + ph) * sp + s];
+        double const dv = g.xacc[(size_t(p) * 2 * P + ph + P) * sp + s];
+        double const th0 = g.xv[b * sp + s], vm0 = g.xv[(b + 1) * sp + s];
+        double const th = th0 + dth;
+        double vm = vm0;
+        vm += vm * dv;
+        double sn, cs, sn0, cs0;
+        sincos(th, &sn, &cs);
+        sincos(th0, &sn0, &cs0);
+        double const nr = vm * cs, ni = vm * sn;
+        double const dr = nr - vm0 * cs0, di = ni - vm0 * sn0;
+        double const d = sqrt(dr * dr + di * di);
+        dbus = ph == 0 ? d : (d > dbus ? d : dbus);
+        g.xv[b * sp + s] = th;
+        g.xv[(b + 1) * sp + s] = vm;
+        g.u[b * sp + s] = nr;
+        g.u[(b + 1) * sp + s] = ni;
+    }
+    atomic_max_nonneg(&g.devmax[size_t(r.mm) * sp + s], dbus);
+}
+
+// per REFINE pair: a backward error still above the tolerance after the last pass fails the pair as the
+// reference's refinement does (SparseMatrixError); otherwise the convergence decision of converge_kernel
+__global__ void refine_finish_kernel(PairState ps, int max_iter, double tol, int* active) {
+    size_t const i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(ps.n) || ps.done[i] != REFINE) return;
+    bool const perturbed = ps.pert[i] >= 2;
+    ps.pert[i] = 0;
+    if (perturbed && !(from_bits(ps.berr[i]) <= kRefineTol)) {
+        ps.done[i] = FAILED;
+        ps.err[i] = ERR_SPARSE;
+        return;
+    }
+    int const it = ps.iters[i] + 1;
+    double const md = from_bits(ps.devmax[i]);
+    ps.devmax[i] = 0ull;
+    ps.iters[i] = it;
+    if (!(md > tol)) {
+        ps.done[i] = CONVERGED;
+    } else if (it >= max_iter) {
+        ps.done[i] = FAILED;
+        ps.err[i] = ERR_DIVERGE;
+        ps.lastdev[i] = md;
+    } else {
+        ps.done[i] = ACTIVE;
+        atomicAdd(active, 1);
     }
 }
 __global__ void set_int_kernel(int* p, int v) {
@@ -2077,6 +2427,10 @@ void launch_fwd(DevGroup const* groups, WorkItem const* items, int n_items, int 
         hipLaunchKernelGGL((forward_kernel<P, FT, MODE_ICF, KIND>), grid, dim3(kWave), 0, st, groups, items);
     else if (mode == MODE_RESID)
         hipLaunchKernelGGL((forward_kernel<P, FT, MODE_RESID, KIND>), grid, dim3(kWave), 0, st, groups, items);
+    else if (mode == MODE_REFINE) {
+        if constexpr (sizeof(FT) == 4 && KIND != KIND_UNION)
+            hipLaunchKernelGGL((forward_kernel<P, FT, MODE_REFINE, KIND>), grid, dim3(kWave), 0, st, groups, items);
+    }
     else if constexpr (KIND == KIND_PLAIN) {
         if (mode == MODE_NRA)
             hipLaunchKernelGGL((forward_kernel<P, FT, MODE_NRA, KIND_PLAIN>), grid, dim3(kWave), 0, st, groups, items);
@@ -2093,6 +2447,10 @@ void launch_bwd(DevGroup const* groups, WorkItem const* items, int n_items, int 
         hipLaunchKernelGGL((backward_kernel<P, FT, MODE_NR, KIND>), grid, dim3(kWave), 0, st, groups, items);
     else if (mode == MODE_ICF || mode == MODE_RESID)
         hipLaunchKernelGGL((backward_kernel<P, FT, MODE_RESID, KIND>), grid, dim3(kWave), 0, st, groups, items);
+    else if (mode == MODE_REFINE) {
+        if constexpr (sizeof(FT) == 4 && KIND != KIND_UNION)
+            hipLaunchKernelGGL((backward_kernel<P, FT, MODE_REFINE, KIND>), grid, dim3(kWave), 0, st, groups, items);
+    }
     else if constexpr (KIND == KIND_PLAIN)
         hipLaunchKernelGGL((backward_kernel<P, FT, MODE_RAW, KIND_PLAIN>), grid, dim3(kWave), 0, st, groups, items);
 }
@@ -2134,18 +2492,26 @@ __global__ void broadcast_y_kernel(double const* __restrict__ ydata, long long n
     ylane[i] = ydata[i / S_pad];
 }
 // ids of an update buffer in HBM: bit 0 when any id is NA, bit 1 when any is a number, bit 2 when some
-// scenario's id differs from scenario 0's at the same position (eps > 0: rows per scenario, row0: first row)
+// scenario's id differs from scenario 0's at the same position. eps > 0: rows row0 + s eps + e of n_scn
+// scenarios, a thread per element e walking scenarios blockIdx.y, + gridDim.y, ... (scenario 0's id read
+// once); eps <= 0: every row once, no comparison
 __global__ void id_scan_kernel(char const* __restrict__ base, long long stride, long long total, long long row0,
-                               long long eps, int* __restrict__ flags) {
-    long long const i = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+                               long long eps, long long n_scn, int* __restrict__ flags) {
+    auto id_at = [&](long long i) { return *reinterpret_cast<int const*>(base + i * stride); };
     int f = 0;
-    if (i < total) {
-        int const id = *reinterpret_cast<int const*>(base + i * stride);
-        f = id == na_IntID ? 1 : 2;
-        if (eps > 0 && i >= row0 + eps) {
-            int const id0 = *reinterpret_cast<int const*>(base + (row0 + (i - row0) % eps) * stride);
-            if (id != id0) f |= 4;
+    if (eps > 0) {
+        long long const e = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+        if (e < eps) {
+            int const id0 = id_at(row0 + e);
+            for (long long sc = blockIdx.y; sc < n_scn; sc += gridDim.y) {
+                int const id = id_at(row0 + sc * eps + e);
+                f |= id == na_IntID ? 1 : 2;
+                if (id != id0) f |= 4;
+            }
         }
+    } else {
+        long long const i = (long long)(blockIdx.x) * blockDim.x + threadIdx.x;
+        if (i < total) f = id_at(i) == na_IntID ? 1 : 2;
     }
     // one atomic per wave
     for (int off = 32; off > 0; off >>= 1) f |= __shfl_xor(f, off);
@@ -2205,9 +2571,23 @@ void launch_backward(DevGroup const* groups, WorkItem const* items, int n_items,
 }
 void launch_id_scan(char const* base, long long stride, long long total, long long row0, long long eps, int* flags,
                     hipStream_t st) {
-    if (total > 0)
+    if (total <= 0) return;
+    if (eps > 0) {
+        long long const n_scn = (total - row0) / eps;
+        dim3 const grid(unsigned((eps + 255) / 256), unsigned(std::min<long long>(std::max<long long>(n_scn, 1), 1024)));
+        hipLaunchKernelGGL(id_scan_kernel, grid, dim3(256), 0, st, base, stride, total, row0, eps, n_scn, flags);
+        // rows outside the scenarios' range (before row0 or after the last scenario) are scanned alone
+        if (row0 > 0)
+            hipLaunchKernelGGL(id_scan_kernel, dim3(unsigned((row0 + 255) / 256)), dim3(256), 0, st, base, stride, row0,
+                               0LL, 0LL, 0LL, flags);
+        long long const end = row0 + n_scn * eps;
+        if (end < total)
+            hipLaunchKernelGGL(id_scan_kernel, dim3(unsigned((total - end + 255) / 256)), dim3(256), 0, st,
+                               base + end * stride, stride, total - end, 0LL, 0LL, 0LL, flags);
+    } else {
         hipLaunchKernelGGL(id_scan_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, base, stride, total,
-                           row0, eps, flags);
+                           0LL, 0LL, 0LL, flags);
+    }
 }
 void launch_expand_member_rows(uint8_t const* src, int const* lane_member, int cols, int S_pad, uint8_t* dst,
                                hipStream_t st) {
@@ -2233,6 +2613,42 @@ void launch_converge(PairState const& ps, int it, int max_iter, double err_tol, 
     if (ps.n == 0) return;
     hipLaunchKernelGGL(converge_kernel, dim3(unsigned((ps.n + 255) / 256)), dim3(256), 0, st, ps, it, it_dev,
                        max_iter, err_tol, active);
+}
+void launch_refine_start(PairState const& ps, DevGroup const* groups, long long const* bus_offsets, int n_groups,
+                         long long total, int P, hipStream_t st) {
+    if (ps.n == 0 || total == 0) return;
+    hipLaunchKernelGGL(refine_reset_kernel, dim3(unsigned((ps.n + 255) / 256)), dim3(256), 0, st, ps);
+    dim3 const grid(unsigned((total + 255) / 256));
+    if (P == 1)
+        hipLaunchKernelGGL(refine_start_kernel<1>, grid, dim3(256), 0, st, groups, bus_offsets, n_groups);
+    else
+        hipLaunchKernelGGL(refine_start_kernel<3>, grid, dim3(256), 0, st, groups, bus_offsets, n_groups);
+}
+void launch_refine_residual(PairState const& ps, DevGroup const* groups, long long const* bus_offsets, int n_groups,
+                            long long total, int P, unsigned long long* gmax, hipStream_t st) {
+    if (ps.n == 0 || total == 0) return;
+    hipLaunchKernelGGL(refine_reset_kernel, dim3(unsigned((ps.n + 255) / 256)), dim3(256), 0, st, ps);
+    dim3 const grid(unsigned((total + 255) / 256));
+    if (P == 1) {
+        hipLaunchKernelGGL(refine_residual_kernel<1>, grid, dim3(256), 0, st, groups, bus_offsets, n_groups);
+        hipLaunchKernelGGL(refine_berr_kernel<1>, grid, dim3(256), 0, st, groups, bus_offsets, n_groups, gmax);
+    } else {
+        hipLaunchKernelGGL(refine_residual_kernel<3>, grid, dim3(256), 0, st, groups, bus_offsets, n_groups);
+        hipLaunchKernelGGL(refine_berr_kernel<3>, grid, dim3(256), 0, st, groups, bus_offsets, n_groups, gmax);
+    }
+}
+void launch_refine_finish(PairState const& ps, DevGroup const* groups, long long const* bus_offsets, int n_groups,
+                          long long total, int P, int max_iter, double err_tol, int* active, hipStream_t st) {
+    if (ps.n == 0) return;
+    if (total > 0) {
+        dim3 const grid(unsigned((total + 255) / 256));
+        if (P == 1)
+            hipLaunchKernelGGL(refine_apply_kernel<1>, grid, dim3(256), 0, st, groups, bus_offsets, n_groups);
+        else
+            hipLaunchKernelGGL(refine_apply_kernel<3>, grid, dim3(256), 0, st, groups, bus_offsets, n_groups);
+    }
+    hipLaunchKernelGGL(refine_finish_kernel, dim3(unsigned((ps.n + 255) / 256)), dim3(256), 0, st, ps, max_iter,
+                       err_tol, active);
 }
 void launch_begin_iteration(int* it_dev, int* active, hipStream_t st) {
     hipLaunchKernelGGL(begin_iteration_kernel, dim3(1), dim3(64), 0, st, it_dev, active);
@@ -2299,10 +2715,11 @@ void launch_load_broadcast(double* s_inj, int8_t* lstat, double const* base_s, i
 }
 void launch_load_scatter(double* s_inj, int8_t* lstat, int S, int S_pad, int const* scenario_index,
                          LoadUpdateSource const& src, LoadTable const& tab, hipStream_t st) {
-    long long const total = (long long)(S) * src.n_elem;
-    if (total == 0) return;
-    hipLaunchKernelGGL(load_scatter_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, s_inj, lstat, S,
-                       S_pad, scenario_index, src, tab);
+    if (S == 0 || src.n_elem == 0) return;
+    dim3 const grid(unsigned((S + kScatterLanes - 1) / kScatterLanes),
+                    unsigned((src.n_elem + kScatterElems - 1) / kScatterElems));
+    hipLaunchKernelGGL(load_scatter_kernel, grid, dim3(256), 0, st, s_inj, lstat, S, S_pad, scenario_index, src,
+                       tab);
 }
 void launch_lowrank(DevGroup const* groups, ExtraWork const* work, int n_work, bool fp32, hipStream_t st) {
     if (n_work == 0) return;

@@ -18,8 +18,28 @@ constexpr int kWaveLanes = 64;  // lanes of a gfx950 wavefront: one scenario (or
 // current with the stored factors (u += x after both)
 // MODE_NRA: the Newton-Raphson Jacobian assembled and factorised at the converged state of converged
 // lanes, every block kept (the adjoint solves with it)
-enum SweepMode : int { MODE_INIT = 0, MODE_NR = 1, MODE_RAW = 2, MODE_ICF = 3, MODE_RESID = 4, MODE_NRA = 5 };
-enum DoneState : int { ACTIVE = 0, CONVERGED = 1, FAILED = 2 };
+// MODE_REFINE: a pass of iterative refinement with the stored factors (forward: substitution of the residual
+// held in the forward vectors; backward: back substitution, the correction added to the accumulated step)
+enum SweepMode : int { MODE_INIT = 0, MODE_NR = 1, MODE_RAW = 2, MODE_ICF = 3, MODE_RESID = 4, MODE_NRA = 5,
+                       MODE_REFINE = 6 };
+// REFINE: the pair's Newton-Raphson step came from a factorisation with perturbed pivots and waits for its
+// iterative refinement (it takes no part in the sweeps meanwhile)
+enum DoneState : int { ACTIVE = 0, CONVERGED = 1, FAILED = 2, REFINE = 3 };
+// pivot perturbation of FP32 factors: in the full-pivot factorisation of a diagonal block, a chosen pivot below
+// this fraction of the row's scale (the largest entry of its assembled diagonal Jacobian block, before
+// elimination; the rows are close to diagonally dominant) takes that magnitude with its own sign, and the step
+// is refined in FP64. 2^-14, about 512 FP32 epsilons, as the reference's threshold (1e-13 of its matrix norm)
+// is about 450 FP64 epsilons: a pivot that small relative to its row is a cancellation of much larger eliminated
+// terms, which the stored FP32 blocks carry only to about that relative level
+constexpr double kPivotPerturbation = 6.103515625e-05;  // 2^-14
+// the refinement stops once the backward error of the step is at most this (the reference's convergence
+// criterion of its refinement) and fails after this many passes (one initial solve and five refinements)
+constexpr double kRefineTol = 1e-13;
+// refinement of every step on request (no perturbed pivot): down to this backward error, near the FP64
+// rounding level, so that the steps are those of an FP64 factorisation; the step is applied after the last
+// pass whatever its backward error
+constexpr double kRefineTolSteps = 1e-15;
+constexpr int kRefineMaxPasses = 6;
 enum ErrCode : int { ERR_NONE = 0, ERR_SPARSE = 1, ERR_DIVERGE = 2 };
 
 struct DevGroup {
@@ -76,6 +96,16 @@ struct DevGroup {
     int* iters = nullptr;           // [n_mm][S_pad]
     unsigned long long* devmax = nullptr;  // [n_mm][S_pad], bits of a non-negative double
     double* lastdev = nullptr;      // [n_mm][S_pad]
+    // pivot perturbation and iterative refinement (FP32 factors): a flag per pair that the factorisation of
+    // this iteration perturbed a pivot; per pair the largest denominator and the backward error of the last
+    // residual (bits of non-negative doubles); the accumulated step and the residual's denominators (|rhs| +
+    // |J| |x|), [n][2P][S_pad], made when a refinement first happens
+    int* pert = nullptr;
+    unsigned long long* rmax = nullptr;
+    unsigned long long* berr = nullptr;
+    double* xacc = nullptr;
+    double* rden = nullptr;
+    int refine_all = 0;  // 1: every FP32 step is held and refined, perturbed or not
     // raw block LU (test hook): input blocks and rhs
     double const* raw_a = nullptr;  // [E][4P^2][S_pad]
     double const* raw_b = nullptr;  // [n][2P][S_pad]
@@ -139,6 +169,9 @@ struct PairState {
     int* iters = nullptr;
     unsigned long long* devmax = nullptr;  // bits of a non-negative double
     double* lastdev = nullptr;
+    int* pert = nullptr;                   // as in DevGroup
+    unsigned long long* rmax = nullptr;
+    unsigned long long* berr = nullptr;
 };
 
 // one sweep over the work items of one level (groups: device table of DevGroup)
@@ -149,6 +182,21 @@ void launch_forward(DevGroup const* groups, WorkItem const* items, int n_items, 
                     hipStream_t st, int kind = KIND_PLAIN);
 void launch_backward(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
                      hipStream_t st, int kind = KIND_PLAIN);
+// iterative refinement of the pairs in REFINE (FP32 factors with perturbed pivots), as the reference refines a
+// solve with perturbed pivots (sparse_lu_solver.hpp:514-649): with x the accumulated step, the residual
+// r = rhs - J x of the Newton-Raphson system at the present voltages in FP64 (J applied without being stored)
+// goes into the forward vectors and its componentwise backward error |r| / max(|rhs| + |J| |x|, 1e-4 max) into
+// each pair's berr and the largest over all pairs into *gmax (bits); MODE_REFINE sweeps then solve the
+// correction with the stored factors. bus_offsets as for launch_bus_results.
+void launch_refine_start(PairState const& ps, DevGroup const* groups, long long const* bus_offsets, int n_groups,
+                         long long total, int P, hipStream_t st);
+void launch_refine_residual(PairState const& ps, DevGroup const* groups, long long const* bus_offsets, int n_groups,
+                            long long total, int P, unsigned long long* gmax, hipStream_t st);
+// the refined step applied to the state of the REFINE pairs (theta += dtheta, V += V dV), the deviation
+// folded; then per pair: a pair with a perturbed pivot whose backward error is still above kRefineTol fails
+// (SparseMatrixError), otherwise the convergence decision of converge_kernel, active pairs counted in *active
+void launch_refine_finish(PairState const& ps, DevGroup const* groups, long long const* bus_offsets, int n_groups,
+                          long long total, int P, int max_iter, double err_tol, int* active, hipStream_t st);
 // fold the deviation of iteration `it`, mark converged / diverged, count the still active pairs
 // it_dev: read the iteration number from this device counter instead of `it` (captured iterations)
 void launch_converge(PairState const& ps, int it, int max_iter, double err_tol, int* active, hipStream_t st,

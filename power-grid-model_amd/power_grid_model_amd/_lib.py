@@ -77,6 +77,7 @@ def _load():
         "PGM_amd_model_release_device_memory": (None, [_p]),
         "PGM_amd_autotune": (Idx, [_p, _p, _p, _p, _p, _p, Idx]),
         "PGM_amd_set_adjoint_layout": (None, [_p, _p, Idx]),
+        "PGM_amd_set_step_refinement": (None, [_p, _p, Idx]),
         "PGM_amd_batch_fetch": (None, [_p, _p, _p]),
         "PGM_amd_batch_set_output": (None, [_p, _p, _p]),
         "PGM_amd_batch_adjoint": (None, [_p, _p, _p, _p, _p, _p]),

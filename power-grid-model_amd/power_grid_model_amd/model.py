@@ -276,7 +276,7 @@ class PowerGridModel:
         return out
 
     def _options(self, symmetric, error_tolerance, max_iterations, calculation_method, threading,
-                 factor_precision, tap_changing_strategy=0, adjoint_layout=False):
+                 factor_precision, tap_changing_strategy=0, adjoint_layout=False, refine_steps=False):
         h = self._handle
         opt = lib.PGM_create_options(h.ptr)
         lib.PGM_set_calculation_type(h.ptr, opt, CalculationType.power_flow)
@@ -288,6 +288,7 @@ class PowerGridModel:
         lib.PGM_amd_set_factor_precision(h.ptr, opt, _enum_value(factor_precision, FactorPrecision))
         lib.PGM_set_tap_changing_strategy(h.ptr, opt, _enum_value(tap_changing_strategy, TapChangingStrategy))
         lib.PGM_amd_set_adjoint_layout(h.ptr, opt, int(bool(adjoint_layout)))
+        lib.PGM_amd_set_step_refinement(h.ptr, opt, int(bool(refine_steps)))
         h.check()
         return _Dataset(opt, lib.PGM_destroy_options, [])
 
@@ -389,14 +390,16 @@ class DeviceBatch:
                  error_tolerance: float = 1e-8, max_iterations: int = 20,
                  factor_precision: FactorPrecision | str = FactorPrecision.fp64,
                  calculation_method: CalculationMethod | str = CalculationMethod.newton_raphson,
-                 adjoint_layout: bool = False):
+                 adjoint_layout: bool = False, refine_steps: bool = False):
         """adjoint_layout: keep every topology group a plain group (switching scenarios are solved on their
-        own structure instead of sharing one), so that adjoint() accepts batches with switching scenarios."""
+        own structure instead of sharing one), so that adjoint() accepts batches with switching scenarios.
+        refine_steps: with factor_precision fp32, refine every Newton-Raphson step in FP64 (the iterates then
+        follow the FP64 path); by default only steps whose factorisation perturbed a pivot are refined."""
         self._model = model
         self._handle = Handle()
         self.symmetric = symmetric
         self._opt = model._options(symmetric, error_tolerance, max_iterations, calculation_method, -1,
-                                   factor_precision, adjoint_layout=adjoint_layout)
+                                   factor_precision, adjoint_layout=adjoint_layout, refine_steps=refine_steps)
         self._upd = _make_const_dataset(self._handle, update_data, "update", True) if update_data else None
         self._ptr = lib.PGM_amd_batch_create(self._handle.ptr, model._ptr, self._opt.ptr,
                                              None if self._upd is None else self._upd.ptr)
@@ -472,10 +475,11 @@ class DeviceBatch:
         return out
 
     def stats(self) -> dict:
-        out = np.zeros(12, dtype=np.float64)
+        out = np.zeros(14, dtype=np.float64)
         lib.PGM_amd_batch_stats(self._ptr, out.ctypes.data)
         keys = ["kernel_ms", "iterations_sum", "buses", "lu_entries", "groups", "fwd_launch_ms", "fwd_launches",
-                "buses_per_scenario", "fwd_bytes", "fwd_total_ms", "fwd_timed", "graph_launches"]
+                "buses_per_scenario", "fwd_bytes", "fwd_total_ms", "fwd_timed", "graph_launches", "refinements",
+                "refine_passes"]
         return dict(zip(keys, out.tolist()))
 
 

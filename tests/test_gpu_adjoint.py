@@ -380,7 +380,7 @@ def _year_subset(case, scenarios):
 
 
 def _year_losses(model, upd):
-    b = DeviceBatch(model, upd, error_tolerance=1e-12, max_iterations=30, adjoint_layout=True)
+    b = DeviceBatch(model, upd, error_tolerance=1e-11, max_iterations=50, adjoint_layout=True)
     b.set_output(OUTPUT)
     b.run()
     assert (b.iterations() > 0).all()
@@ -393,8 +393,8 @@ def _year_losses(model, upd):
 def test_adjoint_on_switching_scenarios():
     """The year-long workload with switching scenarios (ties closed, feeder lines opened, islands) built with
     the adjoint layout: the gradients by load p and q of two switching scenarios and one plain one, against
-    Richardson-extrapolated finite differences of the GPU solver (each side solved to 1e-12 pu in the same
-    switching state). Bar as for the uneven feeders: |adjoint - FD| <= 1e-6 |FD| + 2 x the larger difference
+    Richardson-extrapolated finite differences of the GPU solver (each side solved to 1e-11 pu in the same
+    switching state; 20k buses carry about 1e-12 pu of rounding in their iterates). Bar as for the uneven feeders: |adjoint - FD| <= 1e-6 |FD| + 2 x the larger difference
     between the extrapolations at h, h/2 and h/4 (h = 1e-1 relative)."""
     from power_grid_model_amd import synthetic
 

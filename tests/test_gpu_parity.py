@@ -887,3 +887,93 @@ def test_singular_single_calculation_raises():
         model.calculate_power_flow()
     ref, _, st, msg = ol.power_flow(inp)
     assert st[0] == 1
+
+
+# ------------------------------------------------------------------------------------------------
+# FP32 factors: FP64 diagonal-block factorisation, pivot perturbation and FP64 iterative refinement
+# ------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("name,make", [("year_20k_sample", None), ("asym_500", lambda: synthetic.asym_500(n_scenarios=130)),
+                                       ("meshed_1000", lambda: synthetic.meshed_1000(n_scenarios=300))])
+def test_refined_fp32_steps_follow_the_fp64_path(name, make):
+    """With every FP32 Newton-Raphson step refined in FP64 against the FP64 Jacobian (residual with the
+    Jacobian applied in FP64, corrections solved with the FP32 factors until the componentwise backward error
+    is at most 1e-13, as the reference's refinement after pivot perturbation), the iterates are the FP64
+    path's: the same per-scenario iteration counts as the FP64 path (which equal the oracle's) and outputs
+    within 1e-10 pu / 1e-9 relative of it. Without refinement the FP32 path may take extra iterations.
+    Switching scenarios that share a union structure (per-lane admittances, low-rank correction) are not
+    refined: their FP32 steps stay inexact, at most one extra iteration."""
+    refined = slice(None)
+    if make is None:
+        case = synthetic.year_20k()
+        sw = np.flatnonzero(np.diff(case.update["line"]["indptr"]) > 0)
+        upd = _subset(case, list(range(0, 40)) + list(sw[:24]))
+        # the switching scenarios (6 of the first 40 and the 24 appended) form a union group
+        refined = np.array([i for i in range(40) if i not in set(sw.tolist())])
+    else:
+        case = make()
+        upd = case.update
+    model = PowerGridModel(case.input)
+    runs = {}
+    for key, prec, refine in (("fp64", "fp64", False), ("fp32", "fp32", False), ("refined", "fp32", True)):
+        b = DeviceBatch(model, upd, symmetric=case.symmetric, factor_precision=prec, refine_steps=refine)
+        b.run()
+        runs[key] = (b.fetch(), b.iterations(), b.stats())
+    out64, it64, _ = runs["fp64"]
+    outr, itr, str_ = runs["refined"]
+    _, it32, _ = runs["fp32"]
+    d = itr[refined] != it64[refined]
+    assert not d.any(), (np.flatnonzero(d)[:10], itr[refined][d][:10], it64[refined][d][:10])
+    assert (np.abs(itr - it64) <= 1).all()
+    assert str_["refinements"] >= it64.max()  # one refinement per iteration of the batch
+    print(f"{name}: extra iterations FP32 {int(np.sum(it32 - it64))}, refined 0; refinement residuals "
+          f"{int(str_['refine_passes'])} over {int(str_['refinements'])} refinements")
+    nu = out64["node"]["u_pu"]
+    np.testing.assert_allclose(outr["node"]["u_pu"], nu, rtol=0, atol=1e-10)
+    np.testing.assert_allclose(outr["node"]["u_angle"], out64["node"]["u_angle"], rtol=0, atol=1e-10)
+    for f in ("p_from", "q_from", "p_to", "q_to"):
+        r = out64["line"][f]
+        np.testing.assert_allclose(outr["line"][f], r, rtol=1e-9, atol=1e-9 * np.max(np.abs(r)), err_msg=f)
+
+
+def _stiff_line_case(z_scale, n_scenarios=64):
+    """meshed_1000 with one feeder line's impedance scaled by z_scale (a nearly ideal connection): the bus it
+    feeds is eliminated against a pivot that is a small difference of large terms."""
+    case = synthetic.meshed_1000(n_scenarios=n_scenarios)
+    inp = {k: v.copy() for k, v in case.input.items()}
+    line = inp["line"]
+    k = 10
+    line["r1"][k] *= z_scale
+    line["x1"][k] *= z_scale
+    return inp, case.update
+
+
+def test_pivot_perturbation_refines_or_fails_loudly():
+    """A line of 1e-5 of the usual impedance (an admittance near a link's) puts the pivot of the bus behind it
+    at about 1e-5 of its row's norm: the FP32 factorisation perturbs it (threshold 2^-14 of the row norm) and
+    the held steps are refined in FP64 against the FP64 Jacobian. A scenario then either converges to the
+    oracle's answer at the FP64 bars, or, when the refinement does not reach the backward error bound within
+    the reference's number of passes, fails with SparseMatrixError as the reference's refinement does: never a
+    silently wrong answer. The FP64 path solves every scenario with the oracle's iteration counts."""
+    inp, upd = _stiff_line_case(1e-5)
+    model = PowerGridModel(inp)
+    ref, it_ref, st_ref, _ = ol.power_flow(inp, upd, len(upd["sym_load"]))
+    assert (st_ref == 0).all()
+    b64 = DeviceBatch(model, upd)
+    b64.run()
+    assert (b64.iterations() == it_ref).all()
+    b = DeviceBatch(model, upd, factor_precision="fp32")
+    b.run()
+    out = b.fetch(continue_on_batch_error=True)
+    st = b.stats()
+    it = b.iterations()
+    print(f"stiff line: refinements {st['refinements']}, residuals {st['refine_passes']}, failed {int(np.sum(it == 0))}")
+    assert st["refinements"] > 0
+    ok = it > 0
+    if ok.any():
+        np.testing.assert_allclose(out["node"]["u_pu"][ok], ref["node"]["u_pu"][ok], rtol=0, atol=U_TOL)
+        np.testing.assert_allclose(out["node"]["u_angle"][ok], ref["node"]["u_angle"][ok], rtol=0, atol=ANGLE_TOL)
+    if not ok.all():
+        with pytest.raises(pgm.PowerGridBatchError) as info:
+            model.calculate_power_flow(update_data=upd, factor_precision="fp32")
+        assert sorted(info.value.failed_scenarios.tolist()) == np.flatnonzero(~ok).tolist()
+        assert all(isinstance(e, pgm.SparseMatrixError) for e in info.value.errors)
