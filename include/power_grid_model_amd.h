@@ -341,6 +341,18 @@ PGM_AMD_API int PGM_amd_block_lu_solve(PGM_Idx nb, PGM_Idx n, PGM_Idx const* row
                                        double const* data, double const* rhs, double* x, int32_t* status,
                                        PGM_Idx precision) PGM_AMD_NOEXCEPT;
 
+/* The same block LU factorise + solve with no device allocation: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Everything on the device (structure, input blocks, factors, vectors, per-system state) lives in a
+ * fixed file-scope device region of 64 MiB per GPU, held until the process ends, and the work runs on the
+ * default stream. Returns -2 (and prints why on stderr) when the batch does not fit the region, before any
+ * kernel runs; otherwise as PGM_amd_block_lu_solve. Calls on one GPU share its region: calls that overlap in
+ * time on one GPU race and must be serialised by the caller. */
+PGM_AMD_API int PGM_amd_block_lu_solve_noalloc(PGM_Idx nb, PGM_Idx n, PGM_Idx const* row_indptr,
+                                               PGM_Idx const* col_indices, PGM_Idx const* diag_lu, PGM_Idx batch,
+                                               double const* data, double const* rhs, double* x, int32_t* status,
+                                               PGM_Idx precision) PGM_AMD_NOEXCEPT;
+
 /* Test hook with no counterpart in the reference: one plain copy kernel over n_floats floats (reads and
  * writes 4 * n_floats bytes of HBM), the reference kernel that calibrates rocprofv3 FETCH_SIZE /
  * WRITE_SIZE counters. Allocates and frees its own buffers. Returns 0, or -1 on a HIP error. */

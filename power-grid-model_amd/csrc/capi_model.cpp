@@ -339,6 +339,22 @@ int PGM_amd_block_lu_solve(PGM_Idx nb, PGM_Idx n, PGM_Idx const* row_indptr, PGM
         return -1;
     }
 }
+int PGM_amd_block_lu_solve_noalloc(PGM_Idx nb, PGM_Idx n, PGM_Idx const* row_indptr, PGM_Idx const* col_indices,
+                                   PGM_Idx const* diag_lu, PGM_Idx batch, double const* data, double const* rhs,
+                                   double* x, int32_t* status, PGM_Idx precision) noexcept {
+    try {
+        return device_block_lu_noalloc(nb, n, row_indptr, col_indices, diag_lu, batch, data, rhs, x, status,
+                                       precision == 1);
+    } catch (pgm_amd::ScratchTooSmall const& e) {
+        std::fprintf(stderr, "PGM_amd_block_lu_solve_noalloc: %s", e.what());
+        return -2;
+    } catch (std::exception const& e) {
+        std::fprintf(stderr, "PGM_amd_block_lu_solve_noalloc: %s", e.what());
+        return -1;
+    } catch (...) {
+        return -1;
+    }
+}
 }  // extern "C"
 
 int PGM_amd_copy_probe(PGM_Idx n_floats) noexcept {

@@ -45,6 +45,10 @@ namespace pgm_amd {
 struct PowerGridError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
+// a request larger than a fixed device region
+struct ScratchTooSmall : PowerGridError {
+    using PowerGridError::PowerGridError;
+};
 struct DatasetError : PowerGridError {
     explicit DatasetError(std::string const& m) : PowerGridError("Dataset error: " + m) {}
 };

@@ -2676,6 +2676,24 @@ void batch_plan_sizes(Batch const& b, double& buses, double& entries) {
 }
 
 namespace {
+// bump allocation inside a fixed device region (the allocation-free entry point's scratch); a request
+// beyond the region throws before any kernel runs
+struct ScratchArena {
+    char* base;
+    size_t cap;
+    size_t used = 0;
+    void* take(size_t bytes) {
+        bytes = (bytes + 255) / 256 * 256;
+        if (bytes == 0) return nullptr;
+        if (used + bytes > cap)
+            throw ScratchTooSmall("the block LU batch needs more than the " + std::to_string(cap >> 20) +
+                                  " MiB device scratch of the allocation-free entry point\n");
+        void* p = base + used;
+        used += bytes;
+        return p;
+    }
+};
+
 template <class Ar>
 int block_lu_impl(Ar& A, hipStream_t st, Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices,
                   Idx const* diag_lu, Idx batch, double const* data, double const* rhs, double* x, int32_t* status,
@@ -2694,6 +2712,19 @@ int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices
     } guard{st};
     Arena A;
     return block_lu_impl(A, st, nb, n, row_indptr, col_indices, diag_lu, batch, data, rhs, x, status, fp32);
+}
+
+int device_block_lu_noalloc(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices, Idx const* diag_lu,
+                            Idx batch, double const* data, double const* rhs, double* x, int32_t* status, bool fp32) {
+    if (nb != 2 && nb != 6) return -1;
+    ensure_device();
+    void* base = nullptr;
+    size_t cap = 0;
+    if (scratch_region(&base, &cap) != 0) throw PowerGridError("the device scratch could not be located\n");
+    ScratchArena A{static_cast<char*>(base), cap};
+    // the default stream: no stream is created either
+    return block_lu_impl(A, hipStream_t(nullptr), nb, n, row_indptr, col_indices, diag_lu, batch, data, rhs, x,
+                         status, fp32);
 }
 
 namespace {

@@ -267,6 +267,10 @@ size_t make_resident(Model const& model, PGM_ConstDataset const& update, UploadC
 // test hook: batched block LU on the device
 int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices, Idx const* diag_lu, Idx batch,
                     double const* data, double const* rhs, double* x, int32_t* status, bool fp32);
+// the same in the fixed device scratch, with no device allocation and no stream creation (default stream);
+// throws ScratchTooSmall before any kernel runs when the batch does not fit
+int device_block_lu_noalloc(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices, Idx const* diag_lu,
+                            Idx batch, double const* data, double const* rhs, double* x, int32_t* status, bool fp32);
 
 std::string format_double(double x);
 // throws PowerGridError when no HIP device is visible

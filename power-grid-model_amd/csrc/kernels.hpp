@@ -208,6 +208,9 @@ void launch_set_int(int* p, int v, hipStream_t st);
 void launch_init_state(PairState const& ps, hipStream_t st);
 // one plain copy kernel over n floats (HBM counter calibration); 0 or -1
 int copy_probe(long long n);
+// the device address and size of this GPU's copy of the allocation-free entry point's fixed scratch region
+// (located once per device); 0, or -1 when it cannot be located
+int scratch_region(void** ptr, size_t* bytes);
 // iterative current flat start: every bus of an active pair at the averaged source voltage of its math
 // model, rotated by the bus's phase shift (iterative_current_pf_solver.hpp make_flat_start)
 void launch_ic_start(int P, DevGroup const* groups, long long const* bus_offsets, int n_groups, long long total,

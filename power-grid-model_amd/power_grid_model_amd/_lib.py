@@ -89,6 +89,7 @@ def _load():
         "PGM_amd_plan_stats": (None, [_p, _p, Idx, Idx, Idx, _p]),
         "PGM_amd_minimum_degree_ordering": (Idx, [Idx, _p, _p, _p, _p, _p, _p, Idx]),
         "PGM_amd_block_lu_solve": (C.c_int, [Idx, Idx, _p, _p, _p, Idx, _p, _p, _p, _p, Idx]),
+        "PGM_amd_block_lu_solve_noalloc": (C.c_int, [Idx, Idx, _p, _p, _p, Idx, _p, _p, _p, _p, Idx]),
         "PGM_amd_copy_probe": (C.c_int, [Idx]),
         "PGM_amd_tap_regulator_ranks": (None, [_p, _p, _p]),
     }

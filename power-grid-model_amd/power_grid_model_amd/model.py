@@ -537,9 +537,11 @@ def minimum_degree_ordering(graph: dict[int, list[int]]):
     return alpha[: n_alpha[0]].tolist(), [tuple(fills[2 * i:2 * i + 2].tolist()) for i in range(nf)]
 
 
-def block_lu_solve(nb: int, row_indptr, col_indices, diag_lu, blocks, rhs, precision: int = 0):
+def block_lu_solve(nb: int, row_indptr, col_indices, diag_lu, blocks, rhs, precision: int = 0,
+                   no_alloc: bool = False):
     """Device block-sparse LU of a batch of systems sharing one pattern. blocks: (batch, nnz, nb, nb)
-    row-major, rhs: (batch, n, nb). Returns (status per system, x)."""
+    row-major, rhs: (batch, n, nb). Returns (status per system, x). no_alloc: the allocation-free entry point
+    (the library's fixed 64 MiB device region; ValueError when the batch does not fit it)."""
     ip = np.ascontiguousarray(row_indptr, dtype=np.int64)
     ci = np.ascontiguousarray(col_indices, dtype=np.int64)
     dg = np.ascontiguousarray(diag_lu, dtype=np.int64)
@@ -550,8 +552,11 @@ def block_lu_solve(nb: int, row_indptr, col_indices, diag_lu, blocks, rhs, preci
     n = len(ip) - 1
     x = np.zeros((batch, n, nb))
     status = np.zeros(batch, dtype=np.int32)
-    rc = lib.PGM_amd_block_lu_solve(nb, n, ip.ctypes.data, ci.ctypes.data, dg.ctypes.data, batch, data.ctypes.data,
-                                    b.ctypes.data, x.ctypes.data, status.ctypes.data, precision)
+    fn = lib.PGM_amd_block_lu_solve_noalloc if no_alloc else lib.PGM_amd_block_lu_solve
+    rc = fn(nb, n, ip.ctypes.data, ci.ctypes.data, dg.ctypes.data, batch, data.ctypes.data, b.ctypes.data,
+            x.ctypes.data, status.ctypes.data, precision)
+    if rc == -2:
+        raise ValueError("the batch does not fit the device scratch of the allocation-free entry point")
     if rc != 0:
         raise RuntimeError("device block LU failed")
     return status, x
