@@ -310,6 +310,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--output", default="timing", choices=["timing", "records"],
                     help="timing: the columnar timing output set; records: whole records of every component")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default schedule (a profile then sees one launch shape per kernel)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain 64 MiB copy kernel (the calibration kernel for HBM counters)")
     args = ap.parse_args()
@@ -343,7 +345,7 @@ def main():
 
     # schedule parameters picked by measurement for this batch shape (kept by the model for the legs below)
     tuning = {}
-    for prec in [args.precision] + (["fp64"] if fp32 and not args.no_fp64_leg else []):
+    for prec in [] if args.no_autotune else [args.precision] + (["fp64"] if fp32 and not args.no_fp64_leg else []):
         t0 = time.perf_counter()
         tuning[prec] = model.autotune(case.update, symmetric=case.symmetric, factor_precision=prec)
         tuning[prec]["seconds"] = time.perf_counter() - t0
@@ -363,8 +365,10 @@ def main():
     log(f"[bench] {args.precision} solve only: {['%.1f ms' % (1e3 * t) for t in dr]} stats {st}")
     os.environ["PGM_AMD_GRAPH"] = "0"
     dr_levels = device_resident(batch, args.steps)
-    if st["fwd_timed"] == 0:
-        st = batch.stats()  # graph launches carried no kernel timings: those of the level launches
+    # the dominant kernel's duration from the events around its direct launches on its stream: events
+    # captured as graph nodes also time the node hand-over around the kernel (about 0.2 ms per launch)
+    graph_launch_ms = st["fwd_launch_ms"]
+    st = batch.stats()
     del os.environ["PGM_AMD_GRAPH"]
     log(f"[bench] {args.precision} solve only, one launch per level: {['%.1f ms' % (1e3 * t) for t in dr_levels]}")
     failed = int(np.sum(it == 0))
@@ -373,6 +377,7 @@ def main():
                                             _out_bytes(case, selection or {}, records))
     roof = roofline_of(st, P, fp32)
     roof["traffic"] = measured_traffic(case.name, args.precision)
+    roof["graph_event_launch_ms"] = graph_launch_ms
     mixed_out = fetch_columns(batch, selection) if (not args.no_cpu and not records) else None
     mixed_iters = it.copy()
     del batch

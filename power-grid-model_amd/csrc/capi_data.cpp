@@ -61,9 +61,16 @@ template <int P> std::vector<PGM_MetaComponent> output_components() {
                                            mk("tap_pos", &R::tap_pos, PGM_int8)});
     using F = FaultOutput;
     auto fault = comp<F>(kFaultName, {mk("id", &F::id, PGM_int32), mk("energized", &F::energized, PGM_int8)});
+    using VS = VoltageSensorOutput<P>;
+    auto vs = comp<VS>(kVoltageSensorName, {mk("id", &VS::id, PGM_int32), mk("energized", &VS::energized, PGM_int8),
+                                            mk("u_residual", &VS::u_residual, v),
+                                            mk("u_angle_residual", &VS::u_angle_residual, v)});
+    using PS = PowerSensorOutput<P>;
+    auto ps = comp<PS>(kPowerSensorName, {mk("id", &PS::id, PGM_int32), mk("energized", &PS::energized, PGM_int8),
+                                          mk("p_residual", &PS::p_residual, v), mk("q_residual", &PS::q_residual, v)});
     return {node,           branch("line"),  branch("link"),    branch("generic_branch"), branch("transformer"), appl("shunt"),
             appl("source"), appl("sym_gen"), appl("asym_gen"), appl("sym_load"),     appl("asym_load"),
-            reg,            fault};
+            reg,            fault,           vs,                ps};
 }
 
 std::vector<PGM_MetaDataset> build_meta() {
@@ -79,6 +86,8 @@ std::vector<PGM_MetaDataset> build_meta() {
     using LG = SymLoadGenInput;
     using ALG = AsymLoadGenInput;
     using RI = TapRegulatorInput;
+    using VSI = SymVoltageSensorInput;
+    using PSI = SymPowerSensorInput;
     auto lg_in = [&](char const* n) {
         return comp<LG>(n, {mk("id", &LG::id, PGM_int32), mk("node", &LG::node, PGM_int32),
                             mk("status", &LG::status, PGM_int8), mk("type", &LG::type, PGM_int8),
@@ -157,6 +166,16 @@ std::vector<PGM_MetaDataset> build_meta() {
                                mk("fault_type", &FI::fault_type, PGM_int8), mk("fault_phase", &FI::fault_phase, PGM_int8),
                                mk("fault_object", &FI::fault_object, PGM_int32), mk("r_f", &FI::r_f, PGM_double),
                                mk("x_f", &FI::x_f, PGM_double)}),
+         comp<VSI>(kVoltageSensorName,
+                   {mk("id", &VSI::id, PGM_int32), mk("measured_object", &VSI::measured_object, PGM_int32),
+                    mk("u_sigma", &VSI::u_sigma, PGM_double), mk("u_measured", &VSI::u_measured, PGM_double),
+                    mk("u_angle_measured", &VSI::u_angle_measured, PGM_double)}),
+         comp<PSI>(kPowerSensorName,
+                   {mk("id", &PSI::id, PGM_int32), mk("measured_object", &PSI::measured_object, PGM_int32),
+                    mk("measured_terminal_type", &PSI::measured_terminal_type, PGM_int8),
+                    mk("power_sigma", &PSI::power_sigma, PGM_double), mk("p_measured", &PSI::p_measured, PGM_double),
+                    mk("q_measured", &PSI::q_measured, PGM_double), mk("p_sigma", &PSI::p_sigma, PGM_double),
+                    mk("q_sigma", &PSI::q_sigma, PGM_double)}),
          comp<RI>(kTapRegulatorName,
                   {mk("id", &RI::id, PGM_int32), mk("regulated_object", &RI::regulated_object, PGM_int32),
                    mk("status", &RI::status, PGM_int8), mk("control_side", &RI::control_side, PGM_int8),
@@ -170,6 +189,8 @@ std::vector<PGM_MetaDataset> build_meta() {
     using LGU = SymLoadGenUpdate;
     using ALGU = AsymLoadGenUpdate;
     using RU = TapRegulatorUpdate;
+    using VSU = SymVoltageSensorUpdate;
+    using PSU = SymPowerSensorUpdate;
     auto lg_up = [&](char const* n) {
         return comp<LGU>(n, {mk("id", &LGU::id, PGM_int32), mk("status", &LGU::status, PGM_int8),
                              mk("p_specified", &LGU::p_specified, PGM_double),
@@ -199,6 +220,14 @@ std::vector<PGM_MetaDataset> build_meta() {
                                mk("rx_ratio", &SrcU::rx_ratio, PGM_double),
                                mk("z01_ratio", &SrcU::z01_ratio, PGM_double)}),
          lg_up("sym_gen"), alg_up("asym_gen"), lg_up("sym_load"), alg_up("asym_load"),
+         comp<VSU>(kVoltageSensorName,
+                   {mk("id", &VSU::id, PGM_int32), mk("u_sigma", &VSU::u_sigma, PGM_double),
+                    mk("u_measured", &VSU::u_measured, PGM_double),
+                    mk("u_angle_measured", &VSU::u_angle_measured, PGM_double)}),
+         comp<PSU>(kPowerSensorName,
+                   {mk("id", &PSU::id, PGM_int32), mk("power_sigma", &PSU::power_sigma, PGM_double),
+                    mk("p_measured", &PSU::p_measured, PGM_double), mk("q_measured", &PSU::q_measured, PGM_double),
+                    mk("p_sigma", &PSU::p_sigma, PGM_double), mk("q_sigma", &PSU::q_sigma, PGM_double)}),
          comp<RU>(kTapRegulatorName,
                   {mk("id", &RU::id, PGM_int32), mk("status", &RU::status, PGM_int8),
                    mk("u_set", &RU::u_set, PGM_double), mk("u_band", &RU::u_band, PGM_double),

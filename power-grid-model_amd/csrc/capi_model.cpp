@@ -1,6 +1,7 @@
 // capi_model.cpp -- model and calculation entry points of the C ABI (model.h of the reference,
 // power_grid_model_c/power_grid_model_c/src/model.cpp:48-359), plus the PGM_amd_* extensions.
 #include "engine.hpp"
+#include "se.hpp"
 #include "tap_changer.hpp"
 
 #include "../../include/power_grid_model_amd.h"
@@ -115,6 +116,10 @@ void PGM_calculate(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_Options co
             throw PowerGridError(
                 "If batch_dataset is provided. Both batch_dataset and output_dataset should be a batch!\n");
         if (batch_dataset && batch_dataset->batch_size == 0) return;
+        if (opt->calculation_type == PGM_state_estimation) {
+            run_state_estimation(model->model, *opt, *output_dataset, batch_dataset, handle);
+            return;
+        }
         write_null_fault_output(model->model, *output_dataset);
         // automatic tap changing (main_model_impl.hpp:318-346, optimizer/tap_position_optimizer.hpp)
         if (opt->tap_changing_strategy != 0) {

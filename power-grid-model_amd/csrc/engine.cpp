@@ -366,6 +366,35 @@ int union_tasks() {
     return t;
 }
 
+// the state of a topology group: every component's state but the load_gens' (each scenario's loads are
+// applied from the base and its own update rows, never from its group's state), so that a batch of many
+// groups does not copy the load arrays once per group
+ComponentState structural_state(ComponentState const& st) {
+    ComponentState r;
+    r.line_fs = st.line_fs;
+    r.line_ts = st.line_ts;
+    r.tr_fs = st.tr_fs;
+    r.tr_ts = st.tr_ts;
+    r.tr_tap = st.tr_tap;
+    r.sh_status = st.sh_status;
+    r.sh_g1 = st.sh_g1;
+    r.sh_b1 = st.sh_b1;
+    r.sh_g0 = st.sh_g0;
+    r.sh_b0 = st.sh_b0;
+    r.src_status = st.src_status;
+    r.src_uref = st.src_uref;
+    r.src_uang = st.src_uang;
+    r.src_sk = st.src_sk;
+    r.src_rx = st.src_rx;
+    r.src_z01 = st.src_z01;
+    r.reg_status = st.reg_status;
+    r.reg_u_set = st.reg_u_set;
+    r.reg_u_band = st.reg_u_band;
+    r.reg_ldc_r = st.reg_ldc_r;
+    r.reg_ldc_x = st.reg_ldc_x;
+    return r;
+}
+
 // every branch side of st is open where the base state has it open
 bool opens_only(Model const& m, ComponentState const& st) {
     auto le = [](std::vector<IntS> const& a, std::vector<IntS> const& b) {
@@ -675,10 +704,26 @@ void upload_group(Batch& b, Group& g) {
     }
 }
 
+// timings of the steps inside one phase of batch creation on stderr when PGM_AMD_TRACE is set
+// (diagnostics only)
+struct StepTrace {
+    char const* phase;
+    bool on = std::getenv("PGM_AMD_TRACE") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void mark(char const* step) {
+        if (!on) return;
+        auto const now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[pgm_amd]     %s: %-26s %8.2f ms\n", phase, step,
+                     std::chrono::duration<double, std::milli>(now - t).count());
+        t = now;
+    }
+};
+
 // per-lane data of a union group: side statuses of every math branch, buses without a source, the
 // admittances of every status combination and the Y values of the entries a member's switched
 // branches touch, summed in the same order as the shared ones (branches in math order, then shunts)
 void upload_union_lanes(Batch& b, Group& g) {
+    StepTrace tr{"union lanes"};
     Model const& m = b.model;
     Arena& A = b.arena;
     Plan const& p = g.plan;
@@ -730,6 +775,7 @@ void upload_union_lanes(Batch& b, Group& g) {
         yvar_work(0);
         for (auto& x : th) x.join();
     }
+    tr.mark("branch admittances");
     // buses of the nodes of every math branch, whatever its side statuses in the union state
     Topology const& tp = g.topo;
     std::vector<int> nfrom(nbr, -1), nto(nbr, -1);
@@ -762,7 +808,10 @@ void upload_union_lanes(Batch& b, Group& g) {
     std::vector<int8_t> xbus(extras ? size_t(n) : 0, 0);
     // buses without a source and branch side statuses, one row per member (its lanes share them); the
     // device expands them to the lane-interleaved layout
-    std::vector<uint8_t> dead_m(g.members.size() * size_t(n), 0), lstat_m(g.members.size() * size_t(nbr), 0);
+    // (not value-initialised: every member's thread writes its whole rows, and the first touch of the pages is
+    // spread over those threads)
+    std::unique_ptr<uint8_t[]> dead_m(new uint8_t[std::max<size_t>(g.members.size() * size_t(n), 1)]);
+    std::unique_ptr<uint8_t[]> lstat_m(new uint8_t[std::max<size_t>(g.members.size() * size_t(nbr), 1)]);
     std::vector<int> patch_lane, patch_entry;
     std::vector<double> patch_val;
     // members are independent: several host threads each take members in turn and write the lanes of
@@ -775,6 +824,7 @@ void upload_union_lanes(Batch& b, Group& g) {
     };
     size_t const n_mem = g.members.size();
     std::vector<MemberOut> mouts(n_mem);
+    tr.mark("incidence + buffers");
     unsigned const n_thr =
         unsigned(std::max<size_t>(1, std::min<size_t>({size_t(16), size_t(std::thread::hardware_concurrency()), n_mem})));
     std::vector<std::string> thr_err(n_thr);
@@ -868,9 +918,9 @@ void upload_union_lanes(Batch& b, Group& g) {
                 }
                 std::sort(mo.mbus.begin(), mo.mbus.end());
                 mo.mbus.erase(std::unique(mo.mbus.begin(), mo.mbus.end()), mo.mbus.end());
-                uint8_t* const dm = dead_m.data() + mi * size_t(n);
+                uint8_t* const dm = dead_m.get() + mi * size_t(n);
                 for (Idx r = 0; r < n; ++r) dm[r] = reach[r] ? 0 : 1;
-                std::copy(bits.begin(), bits.end(), lstat_m.begin() + long(mi * size_t(nbr)));
+                std::copy(bits.begin(), bits.end(), lstat_m.get() + mi * size_t(nbr));
             }
         } catch (std::exception const& e) {
             thr_err[t] = e.what();
@@ -884,6 +934,7 @@ void upload_union_lanes(Batch& b, Group& g) {
         for (auto const& e : thr_err)
             if (!e.empty()) throw PowerGridError(e);
     }
+    tr.mark("members");
     for (size_t mi = 0; mi < n_mem; ++mi) {
         auto const& mem = g.members[mi];
         MemberOut const& mo = mouts[mi];
@@ -910,6 +961,7 @@ void upload_union_lanes(Batch& b, Group& g) {
             g.xlanes.push_back(lane);
         }
     }
+    tr.mark("patch lists");
     double* ylane = dev_alloc<double>(A, size_t(p.E) * Q * sp);
     int* d_pl = put(A, patch_lane);
     int* d_pe = put(A, patch_entry);
@@ -918,14 +970,22 @@ void upload_union_lanes(Batch& b, Group& g) {
                         b.stream);
     PGM_HIP_CHECK(hipGetLastError());
     PGM_HIP_CHECK(hipStreamSynchronize(b.stream));
+    tr.mark("lane admittances");
     std::vector<int> lane_member(sp, -1);
     for (size_t mi = 0; mi < n_mem; ++mi)
         for (int lane = g.members[mi].lane_begin; lane < g.members[mi].lane_end; ++lane) lane_member[size_t(lane)] = int(mi);
     int* d_lane_member = put(A, lane_member);
     uint8_t* d_dead = dev_alloc<uint8_t>(A, size_t(n) * sp);
     uint8_t* d_lstat = dev_alloc<uint8_t>(A, size_t(nbr) * sp);
-    launch_expand_member_rows(put(A, dead_m), d_lane_member, int(n), int(sp), d_dead, b.stream);
-    launch_expand_member_rows(put(A, lstat_m), d_lane_member, int(nbr), int(sp), d_lstat, b.stream);
+    auto put_rows = [&](uint8_t const* h, size_t bytes) {
+        uint8_t* d = dev_alloc<uint8_t>(A, bytes);
+        PGM_HIP_CHECK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+        return d;
+    };
+    launch_expand_member_rows(put_rows(dead_m.get(), n_mem * size_t(n)), d_lane_member, int(n), int(sp), d_dead,
+                              b.stream);
+    launch_expand_member_rows(put_rows(lstat_m.get(), n_mem * size_t(nbr)), d_lane_member, int(nbr), int(sp), d_lstat,
+                              b.stream);
     PGM_HIP_CHECK(hipGetLastError());
     double* d_yvar = put(A, yvar);
     g.dg.ylane = ylane;
@@ -934,6 +994,7 @@ void upload_union_lanes(Batch& b, Group& g) {
     g.dout.dead = d_dead;
     int* d_nfrom = put(A, nfrom);
     int* d_nto = put(A, nto);
+    tr.mark("member rows");
     for (int k = 0; k < 4; ++k) {
         g.branch_rec[k].lane_status = d_lstat;
         g.branch_rec[k].branch_y_var = d_yvar;
@@ -1485,7 +1546,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         int const gi = int(b.groups.size());
         key_to_group.emplace(key, gi);
         auto g = std::make_unique<Group>();
-        g->st = st;
+        g->st = structural_state(st);
         b.groups.push_back(std::move(g));
         return gi;
     };
@@ -1557,6 +1618,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
     // branches, no source at their buses) add off-pattern entries that a low-rank correction takes
     // into account ----
     if (opt.adjoint_layout == 0) {
+        StepTrace utr{"union groups"};
         bool const closing_ok = sym && !b.current;
         std::map<std::string, std::vector<int>> by_key;
         // per group: only opens branches of the base state; the branches it closes (when at most
@@ -1593,6 +1655,7 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             for (size_t gi = 0; gi < ng; ++gi)
                 if (cand[gi]) by_key[keys[gi]].push_back(int(gi));
         }
+        utr.mark("classify");
         std::vector<char> merged(b.groups.size(), 0);
         std::vector<std::unique_ptr<Group>> unions;
         for (auto& [key, cand] : by_key) {
@@ -1618,12 +1681,28 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
                 std::vector<char> all(size_t(m.n_branch()), 0);
                 for (int gi : closing)
                     for (Idx c : closed[size_t(gi)]) all[c] = 1;
-                Topology const t = build_topology(m, ust, &all);
+                // the coupled structure's plan, kept by the model like the groups' plans
+                std::string ckey;
+                if (ctx.plans) {
+                    ckey = structural_key(m, ust) + "|closing|" + (sym ? "s|" : "a|");
+                    for (size_t c = 0; c < all.size(); ++c)
+                        if (all[c]) ckey += std::to_string(c) + ",";
+                }
+                std::shared_ptr<PlanEntry const> cp = ctx.plans ? ctx.plans->find(ckey) : nullptr;
+                if (!cp) {
+                    PlanEntry e;
+                    e.topo = build_topology(m, ust, &all);
+                    std::vector<Idx> models(e.topo.models.size());
+                    std::iota(models.begin(), models.end(), Idx{0});
+                    e.plan = build_plan(m, ust, e.topo, models, sym, 16);
+                    cp = std::make_shared<PlanEntry const>(std::move(e));
+                    if (ctx.plans) ctx.plans->put(ckey, cp);
+                }
+                Topology const& t = cp->topo;
+                Plan const& pl = cp->plan;
                 std::vector<char> src_node(size_t(m.n_node()), 0);
                 for (Idx k = 0; k < Idx(m.source.size()); ++k) src_node[m.src_node[k]] = 1;
-                std::vector<Idx> models(t.models.size());
-                std::iota(models.begin(), models.end(), Idx{0});
-                Plan const pl = build_plan(m, ust, t, models, sym, 16);
+                utr.mark("closing structure");
                 bool radial = true;
                 for (Idx r = 0; r < pl.n && radial; ++r) radial = pl.row_end[r] - pl.diag[r] <= 2;
                 Idx const nl = m.n_line();
@@ -1665,11 +1744,13 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
             }
             unions.push_back(std::move(u));
         }
+        utr.mark("members");
         std::vector<std::unique_ptr<Group>> kept;
         for (size_t gi = 0; gi < b.groups.size(); ++gi)
             if (!merged[gi]) kept.push_back(std::move(b.groups[gi]));
         for (auto& u : unions) kept.push_back(std::move(u));
         b.groups = std::move(kept);
+        utr.mark("regroup");
     }
 
     trace.mark("union groups");

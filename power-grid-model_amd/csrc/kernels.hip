@@ -1015,10 +1015,19 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(forward_w
         // a pivot the FP32 factor cannot carry is perturbed (the step is then refined in FP64, see
         // refine_residual_kernel); without perturbation the pivot test is the reference's
         bool perturbed = false;
-        bool const ok = factor_block<NB, FT>(Df, code, kPerturb ? thr_s[lane] : FT(0), perturbed);
+        // FP64 raw systems (the state-estimation gain matrix) perturb below a threshold per pair when one is
+        // given (sparse_lu_solver.hpp:39-48, 352)
+        constexpr bool kPerturbRaw = MODE == MODE_RAW && sizeof(FT) == 8;
+        FT raw_thr = FT(0);
+        if constexpr (kPerturbRaw) raw_thr = g.pthr ? FT(g.pthr[size_t(mm) * sp + s]) : FT(0);
+        bool const ok =
+            factor_block<NB, FT>(Df, code, kPerturb ? thr_s[lane] : (kPerturbRaw ? raw_thr : FT(0)), perturbed);
         if constexpr (kPerturb) {
             // 2: a pivot was perturbed, the refinement must reach its bound
             if (perturbed) atomicMax(&g.pert[size_t(mm) * sp + s], 2);
+        }
+        if constexpr (kPerturbRaw) {
+            if (perturbed && g.pert) atomicMax(&g.pert[size_t(mm) * sp + s], 1);
         }
         if (!ok) {
             if constexpr (MODE != MODE_NRA) {
@@ -2559,6 +2568,29 @@ int copy_probe(long long n) {
     (void)hipFree(a);
     (void)hipFree(b);
     return rc;
+}
+
+// 4x4 blocks of the state-estimation gain matrix (P = 2 gives NB = 4): factorise and solve raw systems,
+// and the refinement sweeps with the stored factors
+void launch_gain_sweep(DevGroup const* groups, WorkItem const* items, int n_items, bool refine, bool fwd,
+                       hipStream_t st) {
+    if (n_items == 0) return;
+    dim3 const grid{unsigned(n_items)};
+    if (fwd) {
+        if (refine)
+            hipLaunchKernelGGL((dev::forward_kernel<2, double, MODE_REFINE, KIND_PLAIN>), grid, dim3(dev::kWave), 0, st, groups,
+                               items);
+        else
+            hipLaunchKernelGGL((dev::forward_kernel<2, double, MODE_RAW, KIND_PLAIN>), grid, dim3(dev::kWave), 0, st, groups,
+                               items);
+    } else {
+        if (refine)
+            hipLaunchKernelGGL((dev::backward_kernel<2, double, MODE_REFINE, KIND_PLAIN>), grid, dim3(dev::kWave), 0, st, groups,
+                               items);
+        else
+            hipLaunchKernelGGL((dev::backward_kernel<2, double, MODE_RAW, KIND_PLAIN>), grid, dim3(dev::kWave), 0, st, groups,
+                               items);
+    }
 }
 
 void launch_forward(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,

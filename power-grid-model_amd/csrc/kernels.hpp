@@ -106,9 +106,11 @@ struct DevGroup {
     double* xacc = nullptr;
     double* rden = nullptr;
     int refine_all = 0;  // 1: every FP32 step is held and refined, perturbed or not
-    // raw block LU (test hook): input blocks and rhs
+    // raw block LU (test hook and the state-estimation gain matrix): input blocks and rhs; FP64 raw systems
+    // perturb pivots below pthr[pair] when pthr is given, and flag the pair in pert
     double const* raw_a = nullptr;  // [E][4P^2][S_pad]
     double const* raw_b = nullptr;  // [n][2P][S_pad]
+    double const* pthr = nullptr;   // [n_mm][S_pad]
     // union groups whose lanes close branches that the shared structure has open (symmetric, radial
     // structure): every such branch adds two off-pattern admittances (a, b) and (b, a) in its lane. The
     // sweeps factorise the matrix without them; a low-rank correction between the forward and the
@@ -180,6 +182,11 @@ struct PairState {
 enum : int { KIND_PLAIN = 0, KIND_UNION = 1, KIND_PACKED = 2 };
 void launch_forward(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
                     hipStream_t st, int kind = KIND_PLAIN);
+// 4x4-block raw systems (state-estimation gain matrix, FP64, plain groups): forward (factorise and forward
+// substitution, or with refine the forward substitution of the residual in y) or backward sweep (solution
+// in y, or with refine added to xacc)
+void launch_gain_sweep(DevGroup const* groups, WorkItem const* items, int n_items, bool refine, bool fwd,
+                       hipStream_t st);
 void launch_backward(DevGroup const* groups, WorkItem const* items, int n_items, int P, bool fp32, int mode,
                      hipStream_t st, int kind = KIND_PLAIN);
 // iterative refinement of the pairs in REFINE (FP32 factors with perturbed pivots), as the reference refines a
@@ -258,6 +265,75 @@ struct DevOutput {
     double const* shunt_y = nullptr;  // P*P*2 per shunt
     double* shunt_si = nullptr;       // [n_shunt][2][P][2][S_pad]
 };
+
+// Newton-Raphson state estimation (newton_raphson_se_solver.hpp) of one group of scenarios sharing a
+// topology: unknowns per bus theta, V and the Lagrange multipliers phi_p, phi_q; the gain matrix has one
+// 4x4 block (column-major) per LU entry of the Y-bus pattern. Per-lane arrays are [item][...][S_pad].
+struct SeDev {
+    int n = 0, n_mm = 0;
+    long long S_pad = 0;
+    int const* row_ptr = nullptr;
+    int const* col = nullptr;
+    int const* diag = nullptr;
+    int const* transpose = nullptr;
+    int8_t const* fill = nullptr;
+    int const* bus_mm = nullptr;
+    double const* phase = nullptr;   // phase shift of every bus
+    // Y-bus elements of every LU entry: el[el_ptr[k] ...], kind in the low 2 bits (0 shunt, 1 branch whose
+    // from side is the entry's row, 2 branch whose to side is the entry's row), index above
+    int const* el_ptr = nullptr;
+    int const* el = nullptr;
+    double const* yent = nullptr;    // admittance of every LU entry (re, im), zero for fill-ins
+    double const* ybr = nullptr;     // per math branch yff, yft, ytf, ytt (re, im)
+    double const* ysh = nullptr;     // per shunt (re, im)
+    // aggregated measurements, variances normalised per math model (measured_values.hpp)
+    int8_t const* v_flag = nullptr;  // [n][S_pad] 0 none, 1 magnitude only, 2 phasor
+    double const* v_val = nullptr;   // [n][3][S_pad] re, im, variance
+    int8_t const* inj_flag = nullptr;  // [n][S_pad]
+    double const* inj_val = nullptr;   // [n][4][S_pad] p, q, variance p, variance q
+    int8_t const* bf_flag = nullptr;   // [n_branch][S_pad] branch from-side power
+    double const* bf_val = nullptr;    // [n_branch][4][S_pad]
+    int8_t const* bt_flag = nullptr;   // to side
+    double const* bt_val = nullptr;
+    int8_t const* sh_flag = nullptr;   // [n_shunt][S_pad]
+    double const* sh_val = nullptr;    // [n_shunt][4][S_pad]
+    // per pair (math model, lane)
+    int const* slack = nullptr;        // [n_mm] slack bus
+    int const* vref = nullptr;         // [n_mm][S_pad] bus of the virtual angle measurement
+    int8_t const* has_angle = nullptr; // [n_mm][S_pad]
+    int8_t const* use_pert = nullptr;  // [n_mm][S_pad] observable and possibly ill-conditioned
+    int* done = nullptr;
+    unsigned long long* devmax = nullptr;
+    unsigned long long* norm = nullptr;  // bits: block-wise off-diagonal infinity norm of the gain matrix
+    double* pthr = nullptr;              // perturbation threshold of the factorisation
+    int* pert = nullptr;                 // the factorisation perturbed a pivot: the solve is refined
+    unsigned long long* rmax = nullptr;  // refinement: largest denominator, backward error (bits)
+    unsigned long long* berr = nullptr;
+    double* offset = nullptr;            // angle offset of the update
+    // state and the linear system
+    double* x = nullptr;      // [n][4][S_pad] theta, V, phi_p, phi_q
+    double* u = nullptr;      // [n][2][S_pad]
+    double* raw_a = nullptr;  // [E][16][S_pad]
+    double* raw_b = nullptr;  // [n][4][S_pad]
+    double* y = nullptr;      // [n][4][S_pad] solution of the factor sweep, residual during a refinement
+    double* xacc = nullptr;   // [n][4][S_pad] refined solution
+    double* rden = nullptr;   // [n][4][S_pad] refinement denominators
+};
+// per (bus, lane): initial state (newton_raphson_se_solver.hpp:219-236) from the per-pair mean angle shift
+void launch_se_init(SeDev const& g, double const* mean_shift, hipStream_t st);
+// pair flags of an iteration: norm, pert cleared
+void launch_se_pair_reset(SeDev const& g, hipStream_t st);
+// gain matrix and right-hand side of every row (prepare_matrix_and_rhs), then Q^T blocks, the Lagrange
+// terms and the per-pair matrix norm (fill_qt, process_lagrange_multiplier), then the thresholds
+void launch_se_gain(SeDev const& g, hipStream_t st);
+// refinement of pairs in REFINE: xacc = y (begin); residual r = b - A xacc into y with denominators (the
+// pairs' largest denominator and backward error restart at zero); backward error per pair
+void launch_se_refine_begin(SeDev const& g, hipStream_t st);
+void launch_se_residual(SeDev const& g, hipStream_t st);
+void launch_se_berr(SeDev const& g, hipStream_t st);
+// after the solve: angle offset per pair, then the update of every bus and the deviation per pair
+// (iterate_unknown)
+void launch_se_update(SeDev const& g, hipStream_t st);
 
 // record writers: one thread per (lane, component); map gives the math index of every component (-1
 // when it is not energised in this group)

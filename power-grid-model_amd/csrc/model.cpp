@@ -106,6 +106,8 @@ Idx Model::count(int comp) const {
     case C_ASYM_GEN: return lg_count[1];
     case C_SYM_LOAD: return lg_count[2];
     case C_ASYM_LOAD: return lg_count[3];
+    case C_VOLTAGE_SENSOR: return Idx(vs_id.size());
+    case C_POWER_SENSOR: return Idx(ps_id.size());
     default: return 0;
     }
 }
@@ -122,6 +124,8 @@ ID Model::comp_id(int comp, Idx i) const {
     case C_ASYM_GEN: return lg_id[lg_offset[1] + i];
     case C_SYM_LOAD: return lg_id[lg_offset[2] + i];
     case C_ASYM_LOAD: return lg_id[lg_offset[3] + i];
+    case C_VOLTAGE_SENSOR: return vs_id[i];
+    case C_POWER_SENSOR: return ps_id[i];
     default: return na_IntID;
     }
 }
@@ -156,7 +160,8 @@ Model build_model(PGM_ConstDataset const& in, double f) {
         }
     };
     for (auto const& b : in.buffers) {
-        bool known = b.meta->name == kTapRegulatorName || b.meta->name == kFaultName;
+        bool known = b.meta->name == kTapRegulatorName || b.meta->name == kFaultName ||
+                     b.meta->name == kVoltageSensorName || b.meta->name == kPowerSensorName;
         for (int c = 0; c < N_COMP; ++c) known |= b.meta->name == comp_names[c];
         if (!known) throw PowerGridError("component " + b.meta->name + " is not supported by this build");
     }
@@ -422,7 +427,93 @@ Model build_model(PGM_ConstDataset const& in, double f) {
             m.base.reg_ldc_x.push_back(r.line_drop_compensation_x);
         }
     }
+    // voltage sensors: the measured object is a node (main_core/input.hpp:84-86)
+    if (auto const* buf = in.find(kVoltageSensorName)) {
+        for (Idx i = 0; i < buf->total_elements; ++i) {
+            buf->read_record(i, rec.data());
+            SymVoltageSensorInput v;
+            std::memcpy(&v, rec.data(), sizeof v);
+            Idx const node = find_component(m, C_NODE, v.measured_object);
+            reg(v.id, C_VOLTAGE_SENSOR, Idx(m.vs_id.size()));
+            m.vs_id.push_back(v.id);
+            m.vs_node.push_back(node);
+            m.base.vs_sigma.push_back(v.u_sigma);
+            m.base.vs_u.push_back(v.u_measured);
+            m.base.vs_angle.push_back(v.u_angle_measured);
+        }
+    }
+    // power sensors: the measured object must be of the kind the terminal type names, never a link
+    // (main_core/input.hpp:87-129)
+    if (auto const* buf = in.find(kPowerSensorName)) {
+        for (Idx i = 0; i < buf->total_elements; ++i) {
+            buf->read_record(i, rec.data());
+            SymPowerSensorInput ps;
+            std::memcpy(&ps, rec.data(), sizeof ps);
+            auto it = m.ids.find(ps.measured_object);
+            if (it == m.ids.end())
+                throw PowerGridError("The id cannot be found: " + std::to_string(ps.measured_object) + "\n");
+            int const comp = it->second.first;
+            Idx obj = it->second.second;
+            if (comp == C_LINK) throw PowerGridError("PowerSensor measurement is not supported for object of type Link");
+            auto wrong = [&] { return PowerGridError("Wrong type for object with id " + std::to_string(ps.measured_object) + "\n"); };
+            switch (ps.measured_terminal_type) {
+            case T_BRANCH_FROM:
+            case T_BRANCH_TO:
+                if (comp == C_TRANSFORMER) {
+                    obj += m.n_line();
+                } else if (comp == C_LINE || comp == C_GENERIC_BRANCH) {
+                    int slot = comp;
+                    line_slot(m, slot, obj);
+                } else {
+                    throw wrong();
+                }
+                break;
+            case T_SHUNT:
+                if (comp != C_SHUNT) throw wrong();
+                break;
+            case T_SOURCE:
+                if (comp != C_SOURCE) throw wrong();
+                break;
+            case T_LOAD:
+                if (comp != C_SYM_LOAD && comp != C_ASYM_LOAD) throw wrong();
+                obj += m.lg_offset[comp == C_SYM_LOAD ? 2 : 3];
+                break;
+            case T_GENERATOR:
+                if (comp != C_SYM_GEN && comp != C_ASYM_GEN) throw wrong();
+                obj += m.lg_offset[comp == C_SYM_GEN ? 0 : 1];
+                break;
+            case T_NODE:
+                if (comp != C_NODE) throw wrong();
+                break;
+            default:
+                throw PowerGridError("measured terminal type " + std::to_string(int(ps.measured_terminal_type)) +
+                                     " of sym_power_sensor is not supported by this build\n");
+            }
+            reg(ps.id, C_POWER_SENSOR, Idx(m.ps_id.size()));
+            m.ps_id.push_back(ps.id);
+            m.ps_term.push_back(ps.measured_terminal_type);
+            m.ps_obj.push_back(obj);
+            m.base.ps_sigma.push_back(ps.power_sigma);
+            m.base.ps_p.push_back(ps.p_measured);
+            m.base.ps_q.push_back(ps.q_measured);
+            m.base.ps_psig.push_back(ps.p_sigma);
+            m.base.ps_qsig.push_back(ps.q_sigma);
+        }
+    }
     return m;
+}
+
+void apply_voltage_sensor_update(ComponentState& st, Idx i, SymVoltageSensorUpdate const& u) {
+    if (!std::isnan(u.u_measured)) st.vs_u[i] = u.u_measured;
+    if (!std::isnan(u.u_angle_measured)) st.vs_angle[i] = u.u_angle_measured;
+    if (!std::isnan(u.u_sigma)) st.vs_sigma[i] = u.u_sigma;
+}
+void apply_power_sensor_update(ComponentState& st, Idx i, SymPowerSensorUpdate const& u) {
+    if (!std::isnan(u.p_measured)) st.ps_p[i] = u.p_measured;
+    if (!std::isnan(u.q_measured)) st.ps_q[i] = u.q_measured;
+    if (!std::isnan(u.power_sigma)) st.ps_sigma[i] = u.power_sigma;
+    if (!std::isnan(u.p_sigma)) st.ps_psig[i] = u.p_sigma;
+    if (!std::isnan(u.q_sigma)) st.ps_qsig[i] = u.q_sigma;
 }
 
 void apply_regulator_update(ComponentState& st, Idx i, TapRegulatorUpdate const& u) {

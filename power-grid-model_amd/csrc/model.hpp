@@ -66,6 +66,10 @@ struct ComponentState {
     // transformer tap regulators
     std::vector<IntS> reg_status;
     std::vector<double> reg_u_set, reg_u_band, reg_ldc_r, reg_ldc_x;
+    // sym_voltage_sensor: sigma and magnitude in V, angle in rad, as given (NaN: not given)
+    std::vector<double> vs_sigma, vs_u, vs_angle;
+    // sym_power_sensor: sigmas and powers in W / var, as given
+    std::vector<double> ps_sigma, ps_p, ps_q, ps_psig, ps_qsig;
 };
 
 struct Model {
@@ -93,6 +97,11 @@ struct Model {
     std::vector<TapRegulatorInput> regulator;
     std::vector<Idx> reg_trafo;
     std::vector<double> reg_u_rated;
+    // sensors (state estimation): the measured node of every voltage sensor; the terminal type and the
+    // measured object of every power sensor (node, branch slot, shunt, source or load_gen index)
+    std::vector<ID> vs_id, ps_id;
+    std::vector<Idx> vs_node, ps_obj;
+    std::vector<int> ps_term;
     ComponentState base;
 
     Idx n_node() const { return Idx(node.size()); }
@@ -132,5 +141,8 @@ inline Idx generic_slot0(Model const& m) { return Idx(m.line.size() + m.link.siz
 // apply one transformer_tap_regulator update record (transformer_tap_regulator.hpp:38-46: NaN / na keep
 // the current value)
 void apply_regulator_update(ComponentState& st, Idx i, TapRegulatorUpdate const& u);
+// apply one sensor update record (voltage_sensor.hpp:88-100, power_sensor.hpp:94-105: NaN keeps the value)
+void apply_voltage_sensor_update(ComponentState& st, Idx i, SymVoltageSensorUpdate const& u);
+void apply_power_sensor_update(ComponentState& st, Idx i, SymPowerSensorUpdate const& u);
 
 }  // namespace pgm_amd

@@ -81,6 +81,18 @@ struct FaultInput {
     double r_f, x_f;
 };
 
+// symmetric voltage and power sensors, inputs of state estimation (auxiliary/input.hpp VoltageSensorInput,
+// PowerSensorInput); measured_terminal_type follows MeasuredTerminalType (common/enum.hpp:43-54)
+struct SymVoltageSensorInput {
+    ID id, measured_object;
+    double u_sigma, u_measured, u_angle_measured;
+};
+struct SymPowerSensorInput {
+    ID id, measured_object;
+    IntS measured_terminal_type;
+    double power_sigma, p_measured, q_measured, p_sigma, q_sigma;
+};
+
 // ---------- update ----------
 struct BranchUpdate {
     ID id;
@@ -116,6 +128,15 @@ struct TapRegulatorUpdate {
     double u_set, u_band, line_drop_compensation_r, line_drop_compensation_x;
 };
 
+struct SymVoltageSensorUpdate {
+    ID id;
+    double u_sigma, u_measured, u_angle_measured;
+};
+struct SymPowerSensorUpdate {
+    ID id;
+    double power_sigma, p_measured, q_measured, p_sigma, q_sigma;
+};
+
 // ---------- output ----------
 template <int P> struct NodeOutput {
     ID id;
@@ -144,6 +165,21 @@ struct TapRegulatorOutput {
     IntS energized, tap_pos;
 };
 
+// state-estimation outputs of the sensors (auxiliary/output.hpp VoltageSensorOutput, PowerSensorOutput)
+template <int P> struct VoltageSensorOutput {
+    ID id;
+    IntS energized;
+    double u_residual[P], u_angle_residual[P];
+};
+template <int P> struct PowerSensorOutput {
+    ID id;
+    IntS energized;
+    double p_residual[P], q_residual[P];
+};
+
+static_assert(sizeof(SymVoltageSensorInput) == 32 && sizeof(SymVoltageSensorUpdate) == 32);
+static_assert(sizeof(SymPowerSensorInput) == 56 && sizeof(SymPowerSensorUpdate) == 48);
+static_assert(sizeof(VoltageSensorOutput<1>) == 24 && sizeof(PowerSensorOutput<1>) == 24);
 static_assert(sizeof(NodeInput) == 16);
 static_assert(sizeof(LineInput) == 88);
 static_assert(sizeof(LinkInput) == 16);
@@ -185,5 +221,23 @@ inline char const* const kTapRegulatorName = "transformer_tap_regulator";
 constexpr int C_TAP_REGULATOR = N_COMP;  // slot of its ids in the id index
 inline char const* const kFaultName = "fault";
 constexpr int C_FAULT = N_COMP + 1;  // slot of fault ids in the id index
+// sensors: inputs of state estimation only (no slot in the power-flow math model)
+inline char const* const kVoltageSensorName = "sym_voltage_sensor";
+constexpr int C_VOLTAGE_SENSOR = N_COMP + 2;
+inline char const* const kPowerSensorName = "sym_power_sensor";
+constexpr int C_POWER_SENSOR = N_COMP + 3;
+// MeasuredTerminalType
+enum Terminal : int {
+    T_BRANCH_FROM = 0,
+    T_BRANCH_TO = 1,
+    T_SOURCE = 2,
+    T_SHUNT = 3,
+    T_LOAD = 4,
+    T_GENERATOR = 5,
+    T_BRANCH3_1 = 6,
+    T_BRANCH3_2 = 7,
+    T_BRANCH3_3 = 8,
+    T_NODE = 9
+};
 
 }  // namespace pgm_amd

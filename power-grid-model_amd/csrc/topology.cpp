@@ -231,6 +231,7 @@ Topology build_topology(Model const& m, ComponentState const& st, std::vector<ch
             }
         }
         MathModel mm;
+        mm.radial = back.empty();
         if (back.empty()) {
             mm.bus_node.assign(order.rbegin(), order.rend());
         } else {

@@ -17,6 +17,7 @@ namespace pgm_amd {
 struct MathModel {
     Idx n_bus = 0;
     Idx slack = 0;
+    bool radial = true;                            // no cycle met by the depth-first visit
     std::vector<Idx> bus_node;                     // topological node of each bus
     std::vector<Idx> branch;                       // component branch index of each math branch
     std::vector<std::array<Idx, 2>> branch_bus;    // -1 for an open side

@@ -13,9 +13,11 @@ from .errors import (
     IDNotFound,
     IDWrongType,
     InvalidCalculationMethod,
+    InvalidMeasuredObject,
     InvalidTransformerClock,
     IterationDiverge,
     MaxIterationReached,
+    NotObservableError,
     PowerGridBatchError,
     PowerGridError,
     SparseMatrixError,
@@ -42,7 +44,7 @@ from .model import (
 __all__ = [
     "LIB_PATH", "lib", "ConflictID", "ConflictVoltage", "DatasetError", "IDNotFound", "IDWrongType",
     "InvalidCalculationMethod", "InvalidTransformerClock", "IterationDiverge", "PowerGridBatchError",
-    "PowerGridError", "SparseMatrixError", "initialize_array", "power_grid_meta_data", "BranchSide",
+    "PowerGridError", "SparseMatrixError", "NotObservableError", "InvalidMeasuredObject", "initialize_array", "power_grid_meta_data", "BranchSide",
     "CalculationMethod", "CalculationType", "ComponentAttributeFilterOptions", "DeviceBatch", "FactorPrecision", "LoadGenType", "PowerGridModel",
     "WindingType", "block_lu_solve", "inspect_topology", "minimum_degree_ordering", "plan_stats",
     "AutomaticTapInputError", "MaxIterationReached", "TapChangingStrategy", "tap_regulator_ranks",

@@ -73,7 +73,17 @@ class DuplicativelyRegulatedObject(PowerGridError):
     """An object has more than one regulator."""
 
 
+class NotObservableError(PowerGridError):
+    pass
+
+
+class InvalidMeasuredObject(PowerGridError):
+    pass
+
+
 _PATTERNS = [
+    (r"Not enough measurements available for state estimation", NotObservableError),
+    (r"measurement is not supported for object of type", InvalidMeasuredObject),
     (r"Maximum number of iterations reached!", MaxIterationReached),
     (r"Automatic tap changer has invalid configuration", AutomaticTapInputError),
     (r"regulator is not supported for object", InvalidRegulatedObject),

@@ -276,10 +276,12 @@ class PowerGridModel:
         return out
 
     def _options(self, symmetric, error_tolerance, max_iterations, calculation_method, threading,
-                 factor_precision, tap_changing_strategy=0, adjoint_layout=False, refine_steps=False):
+                 factor_precision, tap_changing_strategy=0, adjoint_layout=False, refine_steps=False,
+                 calculation_type=None):
         h = self._handle
         opt = lib.PGM_create_options(h.ptr)
-        lib.PGM_set_calculation_type(h.ptr, opt, CalculationType.power_flow)
+        lib.PGM_set_calculation_type(h.ptr, opt, CalculationType.power_flow if calculation_type is None
+                                     else calculation_type)
         lib.PGM_set_calculation_method(h.ptr, opt, _enum_value(calculation_method, CalculationMethod))
         lib.PGM_set_symmetric(h.ptr, opt, int(bool(symmetric)))
         lib.PGM_set_err_tol(h.ptr, opt, float(error_tolerance))
@@ -337,6 +339,39 @@ class PowerGridModel:
         self.batch_error = None
         opt = self._options(symmetric, error_tolerance, max_iterations, calculation_method, threading,
                             factor_precision, tap_changing_strategy)
+        is_batch = update_data is not None
+        upd = None
+        batch_size = 1
+        if is_batch:
+            upd = _make_const_dataset(h, update_data, "update", True)
+            batch_size = upd.batch_size
+        out_type, output = self._output_arrays(output_component_types, symmetric, is_batch, batch_size)
+        out_ds = _make_mutable_dataset(h, output, out_type, is_batch, batch_size)
+        lib.PGM_calculate(h.ptr, self._ptr, opt.ptr, out_ds.ptr, None if upd is None else upd.ptr)
+        self._handle_errors(continue_on_batch_error, batch_size, decode_error)
+        return output
+
+    def calculate_state_estimation(
+        self,
+        *,
+        symmetric: bool = True,
+        error_tolerance: float = 1e-8,
+        max_iterations: int = 20,
+        calculation_method: CalculationMethod | str = CalculationMethod.newton_raphson,
+        update_data: dict | None = None,
+        threading: int = -1,
+        output_component_types=None,
+        continue_on_batch_error: bool = False,
+        decode_error: bool = True,
+    ) -> dict:
+        """Newton-Raphson state estimation of the model, or of every scenario of update_data (a batch), from
+        its sym_voltage_sensor and sym_power_sensor measurements (power_grid_model.py calculate_state_estimation,
+        math_solver/newton_raphson_se_solver.hpp). Symmetric, method newton_raphson; the reference's default
+        method (iterative_linear) is not part of this build and is refused."""
+        h = self._handle
+        self.batch_error = None
+        opt = self._options(symmetric, error_tolerance, max_iterations, calculation_method, threading,
+                            FactorPrecision.fp64, calculation_type=CalculationType.state_estimation)
         is_batch = update_data is not None
         upd = None
         batch_size = 1

@@ -71,6 +71,11 @@ INPUT = {
         _BASE + [("regulated_object", "i4"), ("status", "i1"), ("control_side", "i1")]
         + [(n, _D) for n in ("u_set", "u_band", "line_drop_compensation_r", "line_drop_compensation_x")]
     ),
+    "sym_voltage_sensor": _struct(_BASE + [("measured_object", "i4")]
+                                  + [(n, _D) for n in ("u_sigma", "u_measured", "u_angle_measured")]),
+    "sym_power_sensor": _struct(_BASE + [("measured_object", "i4"), ("measured_terminal_type", "i1")]
+                                + [(n, _D) for n in ("power_sigma", "p_measured", "q_measured", "p_sigma",
+                                                     "q_sigma")]),
 }
 
 UPDATE = {
@@ -89,6 +94,9 @@ UPDATE = {
         _BASE + [("status", "i1")]
         + [(n, _D) for n in ("u_set", "u_band", "line_drop_compensation_r", "line_drop_compensation_x")]
     ),
+    "sym_voltage_sensor": _struct(_BASE + [(n, _D) for n in ("u_sigma", "u_measured", "u_angle_measured")]),
+    "sym_power_sensor": _struct(_BASE + [(n, _D) for n in ("power_sigma", "p_measured", "q_measured", "p_sigma",
+                                                           "q_sigma")]),
 }
 
 
@@ -112,6 +120,8 @@ def _output(v):
         "asym_load": appl,
         "transformer_tap_regulator": _struct(base + [("tap_pos", "i1")]),
         "fault": _struct(base),
+        "sym_voltage_sensor": _struct(base + [("u_residual", v), ("u_angle_residual", v)]),
+        "sym_power_sensor": _struct(base + [("p_residual", v), ("q_residual", v)]),
     }
 
 
@@ -125,7 +135,7 @@ DATASETS = {"input": INPUT, "update": UPDATE, "sym_output": SYM_OUTPUT, "asym_ou
 COMPONENT_ORDER = ["node", "line", "transformer", "shunt", "source", "sym_gen", "asym_gen", "sym_load", "asym_load",
                    "link", "generic_branch"]
 # with the components outside the math model (the tap changer's regulators), in the same container order
-ALL_COMPONENTS = COMPONENT_ORDER + ["transformer_tap_regulator", "fault"]
+ALL_COMPONENTS = COMPONENT_ORDER + ["transformer_tap_regulator", "fault", "sym_voltage_sensor", "sym_power_sensor"]
 
 
 def null_value(fmt: np.dtype):

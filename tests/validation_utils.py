@@ -17,16 +17,17 @@ import numpy as np
 from pgm_formats import batch_to_uniform, load_dataset
 
 GOLDEN = Path(__file__).resolve().parent / "golden" / "power_flow"
+GOLDEN_SE = Path(__file__).resolve().parent / "golden" / "state_estimation"
 
 
-def list_cases(method: str = "newton_raphson", tap: bool | None = None, raises: bool = False):
+def list_cases(method: str = "newton_raphson", tap: bool | None = None, raises: bool = False, root: Path = GOLDEN):
     """(case, symmetric, batch) of every fixture whose params.json lists the calculation method.
 
     tap: None for every case, False without / True only with a tap changing strategy. raises: the cases
     that expect an error instead of the ones with outputs (params "raises", or "xfail" naming the error
     the reference's own run raises)."""
     cases = []
-    for params in sorted(GOLDEN.rglob("params.json")):
+    for params in sorted(root.rglob("params.json")):
         case = params.parent
         p = json.load(open(params))
         methods = p.get("calculation_method")
@@ -38,18 +39,18 @@ def list_cases(method: str = "newton_raphson", tap: bool | None = None, raises: 
         if raises != ("raises" in p or "xfail" in p):
             continue
         if raises:
-            cases.append((str(case.relative_to(GOLDEN)), True, (case / "update_batch.json").exists()))
+            cases.append((str(case.relative_to(root)), True, (case / "update_batch.json").exists()))
             continue
         for sym, stem in ((True, "sym_output"), (False, "asym_output")):
             for batch in (False, True):
                 f = case / (stem + ("_batch" if batch else "") + ".json")
                 if f.exists():
-                    cases.append((str(case.relative_to(GOLDEN)), sym, batch))
+                    cases.append((str(case.relative_to(root)), sym, batch))
     return cases
 
 
-def load_case(name: str, sym: bool, batch: bool):
-    case = GOLDEN / name
+def load_case(name: str, sym: bool, batch: bool, root: Path = GOLDEN):
+    case = root / name
     params = json.load(open(case / "params.json"))
     input_data = load_dataset(case / "input.json")
     stem = ("sym_output" if sym else "asym_output") + ("_batch" if batch else "")
