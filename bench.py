@@ -304,6 +304,7 @@ def main():
     ap.add_argument("--no-fp64-leg", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-se", action="store_true", help="skip the state-estimation leg")
     ap.add_argument("--no-tap", action="store_true", help="skip the automatic tap changer leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the other workloads")
     ap.add_argument("--no-pcie", action="store_true", help="skip the PCIe-inclusive calls")
@@ -407,6 +408,35 @@ def main():
             pcie[("seconds" if prec == args.precision else "fp64_seconds")] = float(np.median(tp))
             log(f"[bench] calculate_power_flow from host buffers ({prec}): "
                 f"{['%.3f s' % t for t in tp]}")
+
+    # Newton-Raphson state estimation on meshed_1000 (1024 scenarios): sensors placed on the grid, measurements
+    # from the power flow of each scenario with 0.1% noise; the whole call from host buffers
+    state_est = None
+    if not args.no_se:
+        se_n = 1024
+        pcase = synthetic.meshed_1000(n_scenarios=se_n)
+        ptrue = PowerGridModel(pcase.input).calculate_power_flow(
+            update_data=pcase.update, output_component_types=["node", "line", "source", "sym_load"])
+        scase = synthetic.state_estimation_case(pcase, ptrue)
+        smodel = PowerGridModel(scase.input)
+        sel = {"node": ["u_pu", "u_angle"], "line": ["p_from", "q_from"]}
+        st_times = []
+        for k in range(args.steps + 1):
+            t0 = time.perf_counter()
+            sout = smodel.calculate_state_estimation(update_data=scase.update, calculation_method="newton_raphson",
+                                                     output_component_types=sel)
+            if k > 0:
+                st_times.append(time.perf_counter() - t0)
+        t_se = float(np.median(st_times))
+        err_u = float(np.max(np.abs(sout["node"]["u_pu"] - ptrue["node"]["u_pu"])))
+        state_est = {"workload": scase.name, "scenarios": se_n, "unit": "scenarios/s", "value": se_n / t_se,
+                     "seconds": t_se, "step": "calculate_state_estimation from host buffers (newton_raphson, sym)",
+                     "sensors": {"voltage": int(len(scase.input["sym_voltage_sensor"])),
+                                 "power": int(len(scase.input["sym_power_sensor"]))},
+                     "max_u_pu_deviation_from_power_flow": err_u,
+                     "cpu_baseline": None,
+                     "cpu_baseline_note": "no CPU restatement of state estimation in the oracle"}
+        log(f"[bench] state estimation {scase.name}: {['%.3f s' % t for t in st_times]}, max |u_pu - pf| {err_u:.2e}")
 
     # adjoint gradients on the uneven-feeder batch (64 feeders, 128 scenarios): the solve, then one
     # adjoint pass (re-factorisation in FP64 at the converged state and the transposed sweeps)
@@ -524,6 +554,7 @@ def main():
                           "frac": step_bytes / t_step / 1e9 / HBM_PEAK_GBS},
         "pcie_inclusive": pcie,
         "adjoint": adjoint,
+        "state_estimation": state_est,
         "tap_changer": tap,
         "configs": configs,
         "cpu_baseline": cpu,

@@ -1259,6 +1259,10 @@ std::unique_ptr<Batch> create_batch(Model const& model, PGM_Options const& opt, 
         opt.calculation_method != PGM_linear_current)
         throw PowerGridError("The calculation method is invalid for this calculation!");
     if (update && update->next) throw PowerGridError("Cartesian product batches are not supported by this build\n");
+    // a power flow needs every load_gen's type (a state estimation does not use it)
+    for (Idx l = 0; l < model.n_lg(); ++l)
+        if (model.lg_type[size_t(l)] < 0 || model.lg_type[size_t(l)] > 2)
+            throw PowerGridError("invalid load_gen type for id " + std::to_string(model.lg_id[size_t(l)]));
     ensure_device();
     auto bp = std::make_unique<Batch>();
     Batch& b = *bp;

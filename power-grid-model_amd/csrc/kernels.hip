@@ -116,8 +116,11 @@ __device__ __forceinline__ int perm_col(uint32_t code, int t) { return int((code
 // magnitude is below the threshold takes the threshold's magnitude with its own sign (+ for a zero pivot) and
 // `perturbed` is set; the pivot test then checks only for non-normal values, as the reference's does once it
 // has perturbed (:157-164)
+// skip_relative: an earlier pivot of the same system was perturbed (the reference's flag is per solver, so from
+// then on only non-normal pivots fail)
 template <int NB, class FT>
-__device__ __forceinline__ bool factor_block(FT (&a)[NB * NB], uint32_t& code, FT perturb_thr, bool& perturbed) {
+__device__ __forceinline__ bool factor_block(FT (&a)[NB * NB], uint32_t& code, FT perturb_thr, bool& perturbed,
+                                             bool skip_relative = false) {
     FT max_pivot = FT(0);
     bool zero = false;
     bool pert = false;
@@ -168,7 +171,7 @@ __device__ __forceinline__ bool factor_block(FT (&a)[NB * NB], uint32_t& code, F
         }
     }
     FT const eps = sizeof(FT) == 8 ? FT(DBL_EPSILON) : FT(FLT_EPSILON);
-    FT const thr = pert ? FT(0) : eps * max_pivot;
+    FT const thr = (pert || skip_relative) ? FT(0) : eps * max_pivot;
     perturbed = pert;
     bool ok = !zero;
 #pragma unroll
@@ -521,6 +524,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(forward_w
     constexpr bool kPerturb = sizeof(FT) == 4 && MODE == MODE_NR && KIND != KIND_UNION;
     __shared__ FT thr_s[kPerturb ? kWave : 1];
     int handed = -1;  // MODE_REFINE: the previous pivot when it handed its L block over (not stored)
+    // raw FP64 systems with pivot perturbation: a pivot of this pair was perturbed in an earlier launch (read once;
+    // launches are ordered) or earlier in this task
+    bool raw_pert = false;
+    if constexpr (MODE == MODE_RAW && sizeof(FT) == 8) raw_pert = g.pthr && g.pert && g.pert[size_t(mm) * sp + s] != 0;
 
     for (int ti = ld(g.task_ptr, task); ti < ld(g.task_ptr, task + 1); ++ti) {
         int const p = ld(g.task_pivot, ti);
@@ -1020,14 +1027,15 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(forward_w
         constexpr bool kPerturbRaw = MODE == MODE_RAW && sizeof(FT) == 8;
         FT raw_thr = FT(0);
         if constexpr (kPerturbRaw) raw_thr = g.pthr ? FT(g.pthr[size_t(mm) * sp + s]) : FT(0);
-        bool const ok =
-            factor_block<NB, FT>(Df, code, kPerturb ? thr_s[lane] : (kPerturbRaw ? raw_thr : FT(0)), perturbed);
+        bool const ok = factor_block<NB, FT>(Df, code, kPerturb ? thr_s[lane] : (kPerturbRaw ? raw_thr : FT(0)),
+                                             perturbed, kPerturbRaw && raw_pert);
         if constexpr (kPerturb) {
             // 2: a pivot was perturbed, the refinement must reach its bound
             if (perturbed) atomicMax(&g.pert[size_t(mm) * sp + s], 2);
         }
         if constexpr (kPerturbRaw) {
             if (perturbed && g.pert) atomicMax(&g.pert[size_t(mm) * sp + s], 1);
+            raw_pert = raw_pert || perturbed;
         }
         if (!ok) {
             if constexpr (MODE != MODE_NRA) {

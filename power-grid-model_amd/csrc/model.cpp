@@ -368,7 +368,6 @@ Model build_model(PGM_ConstDataset const& in, double f) {
                 p[0] = a.p_specified;
                 q[0] = a.q_specified;
             }
-            if (type < 0 || type > 2) throw PowerGridError("invalid load_gen type for id " + std::to_string(id));
             reg(id, comp, cnt);
             m.lg_id.push_back(id);
             m.lg_type.push_back(type);

@@ -26,6 +26,8 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -261,6 +263,9 @@ struct SeGroup {
     std::vector<int> ps_math;   // per power sensor: plan object (bus, branch, shunt, load_gen, source) or -1
     std::vector<int> branch_of; // per component branch slot: plan branch or -1
     std::vector<int> mm_branch_begin;
+    // Y-bus elements of every LU entry (kind in the low 2 bits: 0 shunt, 1 / 2 branch seen from its from / to
+    // side), branches in math order, then shunts
+    std::vector<int> el_ptr, el;
 };
 
 // per lane results needed after the solve: the measured values of the appliances
@@ -460,12 +465,12 @@ void measure_lane(Model const& m, SeGroup const& g, ComponentState const& st, in
                 }
                 for (int e = dg + 1; e < p.row_ptr[size_t(b) + 1]; ++e) {
                     if (p.fill[size_t(e)]) continue;
-                    int const c = p.col[size_t(e)];
-                    // the entry's branches, in math order: the first measured, fully connected one counts
-                    for (int br = br0; br < br1; ++br) {
-                        int const f = p.branch_from[size_t(br)], t = p.branch_to[size_t(br)];
-                        if (!((f == b && t == c) || (f == c && t == b))) continue;
-                        if ((bf_ok[size_t(br)] || bt_ok[size_t(br)]) && f >= 0 && t >= 0) {
+                    // the entry's branches (both sides connected): one measured branch makes a flow sensor
+                    for (int q = g.el_ptr[size_t(e)]; q < g.el_ptr[size_t(e) + 1]; ++q) {
+                        int const code = g.el[size_t(q)];
+                        if ((code & 3) == 0) continue;
+                        int const br = code >> 2;
+                        if (bf_ok[size_t(br)] || bt_ok[size_t(br)]) {
                             flow[size_t(e)] = 1;
                             any = true;
                             break;
@@ -684,11 +689,14 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
         for (int b = 0; b < n; ++b)
             for (int q = p.shunt_ptr[size_t(b)]; q < p.shunt_ptr[size_t(b) + 1]; ++q)
                 el_of[size_t(p.diag[size_t(b)])].push_back(q << 2);
-        std::vector<int> el_ptr(static_cast<size_t>(E) + 1, 0), el;
+        g.el_ptr.assign(size_t(E) + 1, 0);
+        g.el.clear();
         for (int k = 0; k < E; ++k) {
-            el.insert(el.end(), el_of[size_t(k)].begin(), el_of[size_t(k)].end());
-            el_ptr[size_t(k) + 1] = int(el.size());
+            g.el.insert(g.el.end(), el_of[size_t(k)].begin(), el_of[size_t(k)].end());
+            g.el_ptr[size_t(k) + 1] = int(g.el.size());
         }
+        std::vector<int> const& el_ptr = g.el_ptr;
+        std::vector<int> const& el = g.el;
         // ---- measured values of every lane ----
         HostLanes h;
         h.sp = sp;
@@ -731,6 +739,15 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
             for (auto const& e : errs)
                 if (!e.empty()) throw PowerGridError(e);
         }
+        bool const trace = std::getenv("PGM_AMD_TRACE") != nullptr;
+        if (trace)
+            for (int k = 0; k < nmm; ++k)
+                for (int lane = 0; lane < std::min(g.S, 2); ++lane) {
+                    size_t const i = size_t(k) * sp + size_t(lane);
+                    std::fprintf(stderr, "[pgm_amd se] group lane %d model %d: buses %d use_pert %d has_angle %d vref %d error '%s'\n",
+                                 lane, k, int(p.mm_bus_begin[size_t(k) + 1] - p.mm_bus_begin[size_t(k)]), int(h.use_pert[i]),
+                                 int(h.has_angle[i]), h.vref[i], h.lane_error[size_t(lane)].c_str());
+                }
         // ---- device ----
         Arena A;
         SeDev d;
@@ -869,6 +886,17 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
             for (int l = 0; l < L; ++l) launch_gain_sweep(d_lu, items[size_t(l)], counts[size_t(l)], false, true, st);
             for (int l = L - 1; l >= 0; --l) launch_gain_sweep(d_lu, items[size_t(l)], counts[size_t(l)], false, false, st);
             SE_HIP_CHECK(hipGetLastError());
+            if (trace) {
+                std::vector<int> td(npairs), te(npairs), tp(npairs);
+                std::vector<double> tthr(npairs);
+                SE_HIP_CHECK(hipMemcpyAsync(td.data(), d.done, npairs * 4, hipMemcpyDeviceToHost, st));
+                SE_HIP_CHECK(hipMemcpyAsync(te.data(), d_err, npairs * 4, hipMemcpyDeviceToHost, st));
+                SE_HIP_CHECK(hipMemcpyAsync(tp.data(), d.pert, npairs * 4, hipMemcpyDeviceToHost, st));
+                SE_HIP_CHECK(hipMemcpyAsync(tthr.data(), d.pthr, npairs * 8, hipMemcpyDeviceToHost, st));
+                SE_HIP_CHECK(hipStreamSynchronize(st));
+                std::fprintf(stderr, "[pgm_amd se] iteration %lld pair 0: done %d err %d pert %d threshold %.3e\n",
+                             (long long)it, td[0], te[0], tp[0], tthr[0]);
+            }
             if (any_pert) {
                 // pairs whose factorisation perturbed a pivot: iterative refinement against the gain matrix
                 SE_HIP_CHECK(hipMemcpyAsync(hpert.data(), d.pert, npairs * 4, hipMemcpyDeviceToHost, st));
@@ -892,6 +920,12 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
                             launch_gain_sweep(d_lu, items[size_t(l)], counts[size_t(l)], true, false, st);
                         SE_HIP_CHECK(hipMemcpyAsync(hberr.data(), d.berr, npairs * 8, hipMemcpyDeviceToHost, st));
                         SE_HIP_CHECK(hipStreamSynchronize(st));
+                        if (trace) {
+                            double be0;
+                            std::memcpy(&be0, &hberr[0], 8);
+                            std::fprintf(stderr, "[pgm_amd se]   refinement solve %d pair 0: state %d backward error %.3e\n",
+                                         pass, hdone[0], be0);
+                        }
                         refine = false;
                         for (size_t i = 0; i < npairs; ++i) {
                             if (hdone[i] != REFINE) continue;

@@ -371,3 +371,63 @@ CONFIGS = {
     "year_20k": year_20k,
     "year_20k_tap": year_20k_tap,
 }
+
+
+def state_estimation_case(case: Case, pf_output: dict, seed: int = 7, n_scenarios: int | None = None,
+                          voltage_share: float = 0.05, flow_share: float = 0.1, noise: float = 1e-3) -> Case:
+    """A state-estimation workload on a power-flow case: sensors placed from its structure and measurements taken
+    from its power-flow results (pf_output, a batch over the case's scenarios) with relative Gaussian noise.
+
+    Sensors: a voltage phasor sensor at the source node, magnitude-only voltage sensors at a share of the other
+    nodes, a power sensor on every load and on the source (every injection measured, so the grid is observable),
+    and from-side power sensors on a share of the lines. One scenario per power-flow scenario (the first
+    n_scenarios): the update batch carries every sensor's measured values."""
+    rng = np.random.default_rng(seed)
+    inp = {k: v.copy() for k, v in case.input.items()}
+    node = inp["node"]
+    n_node = len(node)
+    src = inp["source"]
+    load = inp["sym_load"]
+    line = inp["line"]
+    s_total = pf_output["node"].shape[0] if n_scenarios is None else n_scenarios
+    src_node = int(src["node"][0])
+    others = np.setdiff1d(node["id"], [src_node])
+    vnodes = np.concatenate([[src_node], rng.choice(others, size=max(1, int(voltage_share * len(others))), replace=False)])
+    flines = rng.choice(len(line), size=max(1, int(flow_share * len(line))), replace=False)
+    next_id = int(max(int(node["id"].max()), int(line["id"].max()), int(load["id"].max()), int(src["id"].max()))) + 1
+    vs = initialize_array("input", "sym_voltage_sensor", len(vnodes))
+    vs["id"] = next_id + np.arange(len(vnodes))
+    vs["measured_object"] = vnodes
+    vs["u_sigma"] = 0.001 * U_MV
+    next_id += len(vnodes)
+    n_ps = len(load) + 1 + len(flines)
+    ps = initialize_array("input", "sym_power_sensor", n_ps)
+    ps["id"] = next_id + np.arange(n_ps)
+    ps["measured_object"][: len(load)] = load["id"]
+    ps["measured_terminal_type"][: len(load)] = 4  # load
+    ps["measured_object"][len(load)] = src["id"][0]
+    ps["measured_terminal_type"][len(load)] = 2  # source
+    ps["measured_object"][len(load) + 1:] = line["id"][flines]
+    ps["measured_terminal_type"][len(load) + 1:] = 0  # branch from side
+    ps["power_sigma"] = 1e3
+    # measurements of every scenario from the power-flow results
+    node_pos = {int(i): k for k, i in enumerate(pf_output["node"]["id"][0])}
+    vpos = np.array([node_pos[int(i)] for i in vnodes])
+    u = pf_output["node"]["u"][:s_total][:, vpos]
+    ang = pf_output["node"]["u_angle"][:s_total][:, vpos]
+    vs_upd = initialize_array("update", "sym_voltage_sensor", (s_total, len(vnodes)))
+    vs_upd["id"] = vs["id"][None, :]
+    vs_upd["u_measured"] = u * (1.0 + noise * rng.standard_normal(u.shape))
+    vs_upd["u_angle_measured"][:, 0] = ang[:, 0]
+    p = np.concatenate([pf_output["sym_load"]["p"][:s_total], pf_output["source"]["p"][:s_total],
+                        pf_output["line"]["p_from"][:s_total][:, flines]], axis=1)
+    q = np.concatenate([pf_output["sym_load"]["q"][:s_total], pf_output["source"]["q"][:s_total],
+                        pf_output["line"]["q_from"][:s_total][:, flines]], axis=1)
+    ps_upd = initialize_array("update", "sym_power_sensor", (s_total, n_ps))
+    ps_upd["id"] = ps["id"][None, :]
+    ps_upd["p_measured"] = p + noise * np.abs(p).mean() * rng.standard_normal(p.shape)
+    ps_upd["q_measured"] = q + noise * np.abs(q).mean() * rng.standard_normal(q.shape)
+    inp["sym_voltage_sensor"] = vs
+    inp["sym_power_sensor"] = ps
+    upd = {"sym_voltage_sensor": vs_upd, "sym_power_sensor": ps_upd}
+    return Case("se_" + case.name, inp, upd, s_total, True, case.kappa, seed)
