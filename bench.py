@@ -433,10 +433,30 @@ def main():
                      "seconds": t_se, "step": "calculate_state_estimation from host buffers (newton_raphson, sym)",
                      "sensors": {"voltage": int(len(scase.input["sym_voltage_sensor"])),
                                  "power": int(len(scase.input["sym_power_sensor"]))},
-                     "max_u_pu_deviation_from_power_flow": err_u,
-                     "cpu_baseline": None,
-                     "cpu_baseline_note": "no CPU restatement of state estimation in the oracle"}
+                     "max_u_pu_deviation_from_power_flow": err_u}
         log(f"[bench] state estimation {scase.name}: {['%.3f s' % t for t in st_times]}, max |u_pu - pf| {err_u:.2e}")
+        if not args.no_cpu:
+            # the oracle's state estimation (block sparse gain system, pivot perturbation and refinement as the
+            # reference) on the first scenarios, sized to a few seconds; its voltages check the GPU's
+            import oracle_lib as ol
+
+            def oracle_se(n):
+                upd = {c: v[:n] for c, v in scase.update.items()}
+                t0 = time.perf_counter()
+                ref, _, st_ref, _ = ol.state_estimation(scase.input, upd, n, threading=threads)
+                return ref, st_ref, time.perf_counter() - t0
+
+            _, _, secs = oracle_se(16)
+            n_cpu = int(min(se_n, max(16, 16 / secs * args.cpu_seconds / 2)))
+            ref, st_ref, secs = oracle_se(n_cpu)
+            ok = st_ref == 0
+            err_o = float(np.max(np.abs(sout["node"]["u_pu"][:n_cpu][ok] - ref["node"]["u_pu"][ok])))
+            state_est["cpu_baseline"] = {"value": n_cpu / secs, "unit": "scenarios/s", "cores": threads, "kind": "port",
+                                         "sample": f"first {n_cpu} scenarios of {scase.name} (oracle state "
+                                                   f"estimation, {threads} threads, {secs:.1f} s)"}
+            state_est["max_u_pu_deviation_from_oracle"] = err_o
+            log(f"[bench] state estimation cpu baseline {n_cpu / secs:.1f} scen/s on {threads} threads; "
+                f"max |u_pu - oracle| {err_o:.2e} over {int(ok.sum())} scenarios")
 
     # adjoint gradients on the uneven-feeder batch (64 feeders, 128 scenarios): the solve, then one
     # adjoint pass (re-factorisation in FP64 at the converged state and the transposed sweeps)

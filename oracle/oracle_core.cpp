@@ -28,6 +28,7 @@
 #include "pgm_oracle.h"
 
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <cmath>
 #include <complex>
@@ -40,6 +41,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -783,8 +785,15 @@ template <int NB> using Vec = std::array<double, NB>;
 template <int NB> inline double& at(Blk<NB>& b, int r, int c) { return b[c * NB + r]; }
 template <int NB> inline double at(Blk<NB> const& b, int r, int c) { return b[c * NB + r]; }
 
-template <int NB> void factorize_block(Blk<NB>& a, BlockPerm<NB>& perm) {
+// pivot perturbation (sparse_lu_solver.hpp:39-48, 85-165): with `use_pert`, a chosen pivot whose magnitude is
+// below `pert_thr` takes that magnitude with its own sign (+ for a zero pivot) and `has_pert` is set; `has_pert`
+// belongs to the whole factorisation, and once set the pivot test checks only for non-normal pivots
+template <int NB>
+void factorize_block(Blk<NB>& a, BlockPerm<NB>& perm, double pert_thr = 0.0, bool use_pert = false,
+                     bool* has_pert = nullptr) {
     double max_pivot = 0.0;
+    bool dummy = false;
+    bool& pert = has_pert ? *has_pert : dummy;
     for (int t = 0; t < NB; ++t) {
         double best = -1.0;
         int rb = t, cb = t;
@@ -797,11 +806,18 @@ template <int NB> void factorize_block(Blk<NB>& a, BlockPerm<NB>& perm) {
                     cb = c;
                 }
             }
-        if (best == 0.0) {
+        if (best == 0.0 && !use_pert) {
             for (int k = t; k < NB; ++k) perm.r[k] = perm.c[k] = int8_t(k);
             break;
         }
-        max_pivot = std::max(max_pivot, std::sqrt(best));
+        double abs_pivot = std::sqrt(best);
+        if (abs_pivot < pert_thr) {
+            double& v = at<NB>(a, rb, cb);
+            v = (abs_pivot == 0.0 ? 1.0 : v / abs_pivot) * pert_thr;
+            abs_pivot = pert_thr;
+            pert = true;
+        }
+        max_pivot = std::max(max_pivot, abs_pivot);
         perm.r[t] = int8_t(rb);
         perm.c[t] = int8_t(cb);
         if (rb != t)
@@ -815,7 +831,7 @@ template <int NB> void factorize_block(Blk<NB>& a, BlockPerm<NB>& perm) {
                 for (int r = t + 1; r < NB; ++r) at<NB>(a, r, c) -= at<NB>(a, r, t) * at<NB>(a, t, c);
         }
     }
-    double const thr = kEps * max_pivot;
+    double const thr = pert ? 0.0 : kEps * max_pivot;
     for (int t = 0; t < NB; ++t) {
         double const d = at<NB>(a, t, t);
         if (std::abs(d) < thr || !std::isnormal(d)) throw SparseMatrixError{};
@@ -899,6 +915,10 @@ template <int NB> struct SparseLU {
     std::vector<Idx> const& diag;
     Idx n;
     std::vector<BlockPerm<NB>> perm;
+    // pivot perturbation of the whole factorisation (sparse_lu_solver.hpp:346-360)
+    double pert_thr = 0.0;
+    bool use_pert = false;
+    bool has_pert = false;
     SparseLU(std::vector<Idx> const& ip, std::vector<Idx> const& ci, std::vector<Idx> const& dg)
         : indptr(ip), cols(ci), diag(dg), n(Idx(ip.size()) - 1), perm(n) {}
 
@@ -911,7 +931,7 @@ template <int NB> struct SparseLU {
         std::vector<Idx> colpos(indptr.begin(), indptr.end() - 1);
         for (Idx p = 0; p < n; ++p) {
             Idx const pidx = diag[p];
-            factorize_block<NB>(lu[pidx], perm[p]);
+            factorize_block<NB>(lu[pidx], perm[p], pert_thr, use_pert, &has_pert);
             auto const& bp = perm[p];
             Blk<NB> const piv = lu[pidx];
             for (Idx l_idx = indptr[p]; l_idx < pidx; ++l_idx) {
@@ -2497,6 +2517,868 @@ template <int P> struct TapOptimizer {
     }
 };
 
+// ------------------------------------------------------------------------------------------------
+// state estimation: symmetric Newton-Raphson with the gain system solved densely
+// (math_solver/newton_raphson_se_solver.hpp, measured_values.hpp, observability.hpp)
+// ------------------------------------------------------------------------------------------------
+struct NotObservableError : std::runtime_error {
+    explicit NotObservableError(std::string const& msg)
+        : std::runtime_error("Not enough measurements available for state estimation.\n" + msg + "\n") {}
+};
+
+constexpr double kInfVar = std::numeric_limits<double>::infinity();
+
+// sensors of the model: the measured object of each, and the values of one scenario
+struct SeSensorSet {
+    std::vector<int32_t> vs_id, ps_id;
+    std::vector<Idx> vs_node, ps_obj;
+    std::vector<int> ps_term;
+};
+struct SeSensorValues {
+    std::vector<double> vs_sigma, vs_u, vs_ang;
+    std::vector<double> ps_sigma, ps_p, ps_q, ps_psig, ps_qsig;
+};
+
+// a real random variable and its inverse-variance combination (statistics.hpp:316-346)
+struct RealVar {
+    double value = 0.0, var = kInfVar;
+};
+RealVar combine_real(std::vector<RealVar> const& xs) {
+    double inv = 0.0, weighted = 0.0;
+    for (auto const& x : xs) {
+        inv += 1.0 / x.var;
+        weighted += x.value / x.var;
+    }
+    if (!std::isnormal(inv)) return {weighted, kInfVar};
+    return {weighted / inv, 1.0 / inv};
+}
+// power measurement: real and imaginary parts with their own variances (statistics.hpp:348-366)
+struct PowerVar {
+    RealVar p, q;
+    bool valid() const { return !std::isinf(p.var) && !std::isinf(q.var); }
+};
+PowerVar combine_power(std::vector<PowerVar> const& xs) {
+    std::vector<RealVar> ps, qs;
+    for (auto const& x : xs) {
+        ps.push_back(x.p);
+        qs.push_back(x.q);
+    }
+    PowerVar r{combine_real(ps), combine_real(qs)};
+    if (!(std::isnormal(r.p.var) && std::isnormal(r.q.var))) r.p.var = r.q.var = kInfVar;
+    return r;
+}
+// voltage measurement: a phasor, or a magnitude (angle NaN), with one variance (statistics.hpp:377-396)
+struct VoltVar {
+    cd value{0.0, 0.0};
+    bool has_angle = false;
+    double var = kInfVar;
+};
+VoltVar combine_voltage(std::vector<VoltVar> const& xs) {
+    bool all_angle = true;
+    for (auto const& x : xs) all_angle = all_angle && x.has_angle;
+    VoltVar r;
+    if (!all_angle) {
+        std::vector<RealVar> mags;
+        for (auto const& x : xs) mags.push_back({x.has_angle ? std::abs(x.value) : x.value.real(), x.var});
+        RealVar const c = combine_real(mags);
+        r.value = c.value;
+        r.var = c.var;
+        r.has_angle = false;
+        return r;
+    }
+    double inv = 0.0;
+    cd weighted{0.0, 0.0};
+    for (auto const& x : xs) {
+        inv += 1.0 / x.var;
+        weighted += x.value / x.var;
+    }
+    r.has_angle = true;
+    if (!std::isnormal(inv)) {
+        r.value = weighted;
+        return r;
+    }
+    r.value = weighted / inv;
+    r.var = 1.0 / inv;
+    return r;
+}
+
+// calculation parameters of one sensor in per unit (voltage_sensor.hpp sym_calc_param, power_sensor.hpp)
+VoltVar voltage_calc(Model const& m, SeSensorSet const& S, SeSensorValues const& v, Idx i) {
+    double const ur = m.node[S.vs_node[i]].u_rated;
+    double const sigma = v.vs_sigma[i] / ur;
+    VoltVar r;
+    r.var = sigma * sigma;
+    double const mag = v.vs_u[i] / ur;
+    if (!std::isnan(v.vs_ang[i])) {
+        r.value = std::polar(mag, v.vs_ang[i]);
+        r.has_angle = true;
+    } else {
+        r.value = cd(mag, 0.0);
+        r.has_angle = false;
+    }
+    return r;
+}
+double sensor_direction(int term) { return (term == 4 || term == 3) ? -1.0 : 1.0; }  // load, shunt
+PowerVar power_calc(SeSensorValues const& v, SeSensorSet const& S, Idx i) {
+    double const dir = sensor_direction(S.ps_term[i]);
+    double const sp = v.ps_psig[i] / kBaseP3, sq = v.ps_qsig[i] / kBaseP3, ss = v.ps_sigma[i] / kBaseP3;
+    double vp, vq;
+    if (std::isnormal(sp) && std::isnormal(sq)) {
+        vp = sp * sp;
+        vq = sq * sq;
+    } else {
+        vp = vq = std::isnan(sp) ? ss * ss / 2.0 : kInfVar;
+    }
+    return PowerVar{{dir * v.ps_p[i] / kBaseP3, vp}, {dir * v.ps_q[i] / kBaseP3, vq}};
+}
+
+// the measurements of one math model, aggregated
+struct SeMeasured {
+    std::vector<VoltVar> volt;           // per bus (var inf: none)
+    std::vector<PowerVar> inj;           // per bus, valid() when the bus has an injection measurement
+    std::vector<bool> has_inj;
+    std::vector<PowerVar> br_from, br_to;  // per math branch
+    std::vector<bool> has_from, has_to;
+    std::vector<PowerVar> shunt;         // per math shunt
+    std::vector<bool> has_shunt;
+    // appliances for the results: 0 disconnected, 1 unmeasured, 2 measured; unnormalised values
+    std::vector<int> lg_state, src_state;
+    std::vector<PowerVar> lg_val, src_val;
+    std::vector<PowerVar> bus_appl;
+    std::vector<int> bus_unmeasured;
+    int n_angle = 0, first_volt = -1;
+    double mean_shift = 0.0;
+};
+
+SeMeasured measure(Model const& m, State const& st, TopoCache const& tc, MathModel<1> const& mm, Idx g,
+                   SeSensorSet const& S, SeSensorValues const& val) {
+    Idx const n = mm.n_bus;
+    SeMeasured r;
+    Idx const nbr = Idx(mm.branch_bus.size()), nsh = Idx(mm.shunt_bus.size());
+    Idx const nlg = Idx(tc.mm_load[g].size()), nsrc = Idx(tc.mm_source[g].size());
+    // sensors per object of this math model, in sensor order
+    std::vector<std::vector<Idx>> vs_at(n), inj_at(n), bf_at(nbr), bt_at(nbr), sh_at(nsh), lg_at(nlg), src_at(nsrc);
+    for (Idx i = 0; i < Idx(S.vs_node.size()); ++i) {
+        Idx const node = S.vs_node[i];
+        if (tc.topo.node_group[node] == g) vs_at[tc.topo.node_pos[node]].push_back(i);
+    }
+    for (Idx i = 0; i < Idx(S.ps_term.size()); ++i) {
+        Idx const o = S.ps_obj[i];
+        switch (S.ps_term[i]) {
+        case 0:
+        case 1:
+            if (tc.br_group[o] == g) (S.ps_term[i] == 0 ? bf_at : bt_at)[tc.br_pos[o]].push_back(i);
+            break;
+        case 2:
+            if (tc.src_group[o] == g) src_at[tc.src_pos[o]].push_back(i);
+            break;
+        case 3:
+            if (tc.sh_group[o] == g) sh_at[tc.sh_pos[o]].push_back(i);
+            break;
+        case 4:
+        case 5:
+            if (tc.lg_group[o] == g) lg_at[tc.lg_pos[o]].push_back(i);
+            break;
+        case 9:
+            if (tc.topo.node_group[o] == g) inj_at[tc.topo.node_pos[o]].push_back(i);
+            break;
+        default:
+            break;
+        }
+    }
+    // voltages (measured_values.hpp:240-299)
+    r.volt.assign(n, VoltVar{});
+    double angle_sum = 0.0;
+    for (Idx b = 0; b < n; ++b) {
+        if (vs_at[b].empty()) continue;
+        std::vector<VoltVar> xs;
+        for (Idx i : vs_at[b]) xs.push_back(voltage_calc(m, S, val, i));
+        VoltVar const c = combine_voltage(xs);
+        if (std::isinf(c.var)) continue;
+        r.volt[b] = c;
+        if (r.first_volt < 0) r.first_volt = int(b);
+        if (c.has_angle) {
+            ++r.n_angle;
+            angle_sum += std::arg(c.value * std::exp(cd(0.0, -mm.phase_shift[b])));
+        }
+    }
+    r.mean_shift = r.n_angle > 0 ? angle_sum / double(r.n_angle) : 0.0;
+    // one object: disconnected, unmeasured or measured (measured_values.hpp:510-540)
+    auto object = [&](bool connected, std::vector<Idx> const& sensors, PowerVar& out) {
+        if (!connected) return 0;
+        if (sensors.empty()) return 1;
+        std::vector<PowerVar> xs;
+        for (Idx i : sensors) xs.push_back(power_calc(val, S, i));
+        PowerVar const c = combine_power(xs);
+        if (!c.valid()) return 1;
+        out = c;
+        return 2;
+    };
+    r.shunt.assign(nsh, PowerVar{});
+    r.has_shunt.assign(nsh, false);
+    r.lg_state.assign(nlg, 0);
+    r.src_state.assign(nsrc, 0);
+    r.lg_val.assign(nlg, PowerVar{});
+    r.src_val.assign(nsrc, PowerVar{});
+    r.inj.assign(n, PowerVar{});
+    r.has_inj.assign(n, false);
+    r.bus_appl.assign(n, PowerVar{{0.0, 0.0}, {0.0, 0.0}});
+    r.bus_unmeasured.assign(n, 0);
+    for (Idx k = 0; k < nsh; ++k)
+        r.has_shunt[k] = object(st.sh_status[tc.mm_shunt[g][k]] != 0, sh_at[k], r.shunt[k]) == 2;
+    for (Idx k = 0; k < nlg; ++k) r.lg_state[k] = object(st.lg_status[tc.mm_load[g][k]] != 0, lg_at[k], r.lg_val[k]);
+    for (Idx k = 0; k < nsrc; ++k)
+        r.src_state[k] = object(st.src_status[tc.mm_source[g][k]] != 0, src_at[k], r.src_val[k]);
+    // injections (measured_values.hpp:318-378)
+    auto add = [&](Idx bus, int state, PowerVar const& x) {
+        if (state == 1) ++r.bus_unmeasured[bus];
+        if (state != 2) return;
+        PowerVar& a = r.bus_appl[bus];
+        a.p.value += x.p.value;
+        a.q.value += x.q.value;
+        a.p.var += x.p.var;
+        a.q.var += x.q.var;
+    };
+    for (Idx k = 0; k < nlg; ++k) add(mm.load_bus[k], r.lg_state[k], r.lg_val[k]);
+    for (Idx k = 0; k < nsrc; ++k) add(mm.source_bus[k], r.src_state[k], r.src_val[k]);
+    for (Idx b = 0; b < n; ++b) {
+        std::vector<PowerVar> xs;
+        for (Idx i : inj_at[b]) xs.push_back(power_calc(val, S, i));
+        PowerVar const direct = combine_power(xs);
+        PowerVar const& appl = r.bus_appl[b];
+        if (r.bus_unmeasured[b] > 0) {
+            if (direct.valid()) {
+                r.inj[b] = direct;
+                r.has_inj[b] = true;
+            }
+        } else if (!direct.valid() || appl.p.var == 0.0 || appl.q.var == 0.0) {
+            r.inj[b] = appl;
+            r.has_inj[b] = true;
+        } else {
+            r.inj[b] = combine_power({direct, appl});
+            r.has_inj[b] = true;
+        }
+    }
+    // branch sides (measured_values.hpp:380-445)
+    r.br_from.assign(nbr, PowerVar{});
+    r.br_to.assign(nbr, PowerVar{});
+    r.has_from.assign(nbr, false);
+    r.has_to.assign(nbr, false);
+    for (Idx k = 0; k < nbr; ++k) {
+        r.has_from[k] = object(mm.branch_bus[k][0] != -1, bf_at[k], r.br_from[k]) == 2;
+        r.has_to[k] = object(mm.branch_bus[k][1] != -1, bt_at[k], r.br_to[k]) == 2;
+    }
+    // normalize_variance (measured_values.hpp:545-600)
+    double min_var = kInfVar;
+    auto consider = [&](double v) {
+        if (v != 0.0) min_var = std::min(min_var, v);
+    };
+    for (auto const& v : r.volt)
+        if (!std::isinf(v.var)) consider(v.var);
+    for (Idx k = 0; k < nsh; ++k)
+        if (r.has_shunt[k]) consider(r.shunt[k].p.var + r.shunt[k].q.var);
+    for (Idx b = 0; b < n; ++b)
+        if (r.has_inj[b]) consider(r.inj[b].p.var + r.inj[b].q.var);
+    for (Idx k = 0; k < nbr; ++k) {
+        if (r.has_from[k]) consider(r.br_from[k].p.var + r.br_from[k].q.var);
+        if (r.has_to[k]) consider(r.br_to[k].p.var + r.br_to[k].q.var);
+    }
+    double const scale = 1.0 / min_var;
+    auto norm = [&](PowerVar& x) {
+        x.p.var *= scale;
+        x.q.var *= scale;
+    };
+    for (auto& v : r.volt)
+        if (!std::isinf(v.var)) v.var *= scale;
+    for (Idx k = 0; k < nsh; ++k)
+        if (r.has_shunt[k]) norm(r.shunt[k]);
+    for (Idx b = 0; b < n; ++b)
+        if (r.has_inj[b]) norm(r.inj[b]);
+    for (Idx k = 0; k < nbr; ++k) {
+        if (r.has_from[k]) norm(r.br_from[k]);
+        if (r.has_to[k]) norm(r.br_to[k]);
+    }
+    return r;
+}
+
+// observability (observability.hpp:60-140, 644-698): throws NotObservableError; the meshed sufficient condition
+// is taken as met. Returns whether the gain system may be ill-conditioned (a bus other than the last with no
+// injection, flow or voltage phasor sensor), which turns on the pivot perturbation
+bool check_observable(MathModel<1> const& mm, YStruct const& ys, SeMeasured const& me, bool radial) {
+    Idx const n = mm.n_bus;
+    if (me.first_volt < 0) throw NotObservableError("No voltage sensor found!\n");
+    Idx const nnz = ys.row_indptr[n];
+    std::vector<int8_t> flow(nnz, 0), vph(n, 0);
+    Idx total_inj = 0;
+    bool ill = false;
+    for (Idx b = 0; b < n; ++b) {
+        Idx const d = ys.bus_entry[b];
+        if (me.has_inj[b]) {
+            flow[d] = 1;
+            ++total_inj;
+        }
+        for (Idx e = d + 1; e < ys.row_indptr[b + 1]; ++e) {
+            for (Idx q = ys.y_bus_entry_indptr[e]; q < ys.y_bus_entry_indptr[e + 1]; ++q) {
+                auto const& el = ys.y_bus_element[q];
+                if (el.type == 4) continue;  // shunt
+                Idx const br = el.idx;
+                bool const connected = mm.branch_bus[br][0] != -1 && mm.branch_bus[br][1] != -1;
+                if ((me.has_from[br] || me.has_to[br]) && connected) {
+                    flow[e] = 1;
+                    break;
+                }
+            }
+        }
+        if (!std::isinf(me.volt[b].var) && me.volt[b].has_angle) vph[b] = 1;
+        if (!flow[d] && vph[b] == 0 && b != n - 1) {
+            bool any = false;
+            for (Idx e = d + 1; e < ys.row_indptr[b + 1] && !any; ++e) any = flow[e] != 0;
+            ill = ill || !any;
+        }
+    }
+    Idx n_flow = 0, n_vph = 0;
+    for (auto f : flow) n_flow += f;
+    for (auto v : vph) n_vph += v;
+    if (n_vph == 0 && n_flow < n - 1)
+        throw NotObservableError(
+            "The total number of independent power sensors is not enough to make the grid observable.");
+    if (n_vph > 0 && n_flow + n_vph < n)
+        throw NotObservableError("The total number of independent power sensors and voltage phasor sensors is not "
+                                 "enough to make the grid observable.");
+    if ((n_vph > 1 && !radial) || total_inj > n - 2 || !radial) return ill;
+    for (Idx b = 0; b + 1 < n; ++b) {
+        Idx const cur = ys.bus_entry[b], up = cur + 1;
+        Idx const up_bus = ys.col_indices[up];
+        Idx const up_entry = ys.bus_entry[up_bus];
+        if (flow[up] == 0) {
+            if (flow[cur] == 1)
+                std::swap(flow[up], flow[cur]);
+            else if (flow[up_entry] == 1)
+                std::swap(flow[up], flow[up_entry]);
+            else if (vph[b] == 1)
+                std::swap(flow[up], vph[b]);
+            else if (vph[b + 1] == 1)
+                std::swap(flow[up], vph[b + 1]);
+        }
+        flow[cur] = 0;
+    }
+    flow[ys.bus_entry[n - 1]] = 0;
+    Idx n_ind = 0, n_rem = 0;
+    for (auto f : flow) n_ind += f;
+    for (auto v : vph) n_rem += v;
+    if (n_ind < n - 1 || (n_vph > 0 && n_rem < 1))
+        throw NotObservableError("The number of power, current, and voltage phasor sensors appears sufficient, but "
+                                 "they are not independent enough. The system is still not observable.\n");
+    return ill;
+}
+
+// dense LU with partial pivoting: solves a x = b in place (b becomes x); throws SparseMatrixError when a pivot
+// is exactly zero
+void dense_solve(std::vector<double>& a, std::vector<double>& b, Idx n) {
+    std::vector<Idx> piv(n);
+    for (Idx k = 0; k < n; ++k) {
+        Idx best = k;
+        for (Idx r = k + 1; r < n; ++r)
+            if (std::abs(a[r * n + k]) > std::abs(a[best * n + k])) best = r;
+        if (a[best * n + k] == 0.0) throw SparseMatrixError();
+        if (best != k) {
+            for (Idx c = 0; c < n; ++c) std::swap(a[k * n + c], a[best * n + c]);
+            std::swap(b[k], b[best]);
+        }
+        double const d = a[k * n + k];
+        for (Idx r = k + 1; r < n; ++r) {
+            double const f = a[r * n + k] / d;
+            if (f == 0.0) continue;
+            for (Idx c = k; c < n; ++c) a[r * n + c] -= f * a[k * n + c];
+            b[r] -= f * b[k];
+        }
+    }
+    for (Idx k = n - 1; k >= 0; --k) {
+        double s = b[k];
+        for (Idx c = k + 1; c < n; ++c) s -= a[k * n + c] * b[c];
+        b[k] = s / a[k * n + k];
+    }
+}
+
+// the gain system of one Newton-Raphson state-estimation iteration, 4 unknowns per bus (theta, V, phi_p, phi_q):
+// a dense matrix solved with partial pivoting, or 4x4 blocks on the LU pattern of the Y-bus solved with the
+// block sparse LU, its pivot perturbation and iterative refinement (sparse_lu_solver.hpp:297-318, 514-650)
+// sparse gain factorisations that perturbed a pivot, and the refinement solves they took (for the tests)
+std::atomic<long long> g_se_perturbed{0}, g_se_refine_solves{0};
+
+struct DenseGain {
+    Idx n, N;
+    std::vector<double> A;
+    explicit DenseGain(Idx n_bus) : n(n_bus), N(4 * n_bus), A(size_t(N * N), 0.0) {}
+    double& ref(Idx i, int r, Idx j, int c) { return A[size_t((4 * i + r) * N + 4 * j + c)]; }
+    template <class F> void for_each_entry(F&& f) const {
+        for (Idx i = 0; i < n; ++i)
+            for (Idx j = 0; j < n; ++j) f(i, j);
+    }
+    void solve(std::vector<double>& rhs, bool) { dense_solve(A, rhs, N); }
+};
+struct SparseGain {
+    YStruct const& ys;
+    Idx n;
+    std::vector<Blk<4>> lu;
+    SparseGain(YStruct const& y, Idx n_bus) : ys(y), n(n_bus), lu(size_t(y.row_indptr_lu[n_bus]), Blk<4>{}) {}
+    Idx entry(Idx i, Idx j) const {
+        auto const b = ys.col_indices_lu.begin() + ys.row_indptr_lu[i];
+        auto const e = ys.col_indices_lu.begin() + ys.row_indptr_lu[i + 1];
+        auto const it = std::lower_bound(b, e, j);
+        if (it == e || *it != j) throw OtherError("state estimation: gain entry outside the LU pattern");
+        return Idx(it - ys.col_indices_lu.begin());
+    }
+    double& ref(Idx i, int r, Idx j, int c) { return at<4>(lu[size_t(entry(i, j))], r, c); }
+    template <class F> void for_each_entry(F&& f) const {
+        for (Idx i = 0; i < n; ++i)
+            for (Idx k = ys.row_indptr_lu[i]; k < ys.row_indptr_lu[i + 1]; ++k) f(i, ys.col_indices_lu[k]);
+    }
+    // rhs becomes the solution
+    void solve(std::vector<double>& rhs, bool use_pert) {
+        Idx const nnz = Idx(lu.size());
+        std::vector<Vec<4>> b(static_cast<size_t>(n)), x(static_cast<size_t>(n));
+        for (Idx i = 0; i < n; ++i)
+            for (int r = 0; r < 4; ++r) b[size_t(i)][size_t(r)] = rhs[size_t(4 * i + r)];
+        SparseLU<4> solver(ys.row_indptr_lu, ys.col_indices_lu, ys.diag_lu);
+        std::vector<Blk<4>> original;
+        if (use_pert) {
+            // the block-wise off-diagonal infinity norm sets the threshold (:624-650)
+            original = lu;
+            double norm = 0.0;
+            for (Idx i = 0; i < n; ++i) {
+                double row = 0.0;
+                for (Idx k = ys.row_indptr_lu[i]; k < ys.row_indptr_lu[i + 1]; ++k) {
+                    if (ys.col_indices_lu[k] == i) continue;
+                    double bn = 0.0;
+                    for (int r = 0; r < 4; ++r) {
+                        double sr = 0.0;
+                        for (int c = 0; c < 4; ++c) sr += std::abs(at<4>(lu[size_t(k)], r, c));
+                        bn = std::max(bn, sr);
+                    }
+                    row += bn;
+                }
+                norm = std::max(norm, row);
+            }
+            solver.use_pert = true;
+            solver.pert_thr = 1e-13 * norm;
+        }
+        solver.factorize(lu);
+        if (solver.has_pert) ++g_se_perturbed;
+        if (!solver.has_pert) {
+            solver.solve(lu, b, x);
+        } else {
+            // iterative refinement against the unperturbed matrix: x from 0, at most six solves, until the
+            // componentwise backward error (denominators capped below at 1e-4 of the largest) is at most 1e-13
+            for (auto& v : x) v.fill(0.0);
+            std::vector<Vec<4>> res = b, dx(static_cast<size_t>(n));
+            double berr = std::numeric_limits<double>::max();
+            Idx it = 0;
+            while (berr > 1e-13) {
+                if (it++ == 6) throw SparseMatrixError{};
+                solver.solve(lu, res, dx);
+                ++g_se_refine_solves;
+                std::vector<Vec<4>> den(static_cast<size_t>(n));
+                double max_den = 0.0;
+                for (Idx i = 0; i < n; ++i) {
+                    for (int r = 0; r < 4; ++r) den[size_t(i)][size_t(r)] = std::abs(b[size_t(i)][size_t(r)]);
+                    for (Idx k = ys.row_indptr_lu[i]; k < ys.row_indptr_lu[i + 1]; ++k) {
+                        Idx const j = ys.col_indices_lu[k];
+                        for (int r = 0; r < 4; ++r)
+                            for (int c = 0; c < 4; ++c)
+                                den[size_t(i)][size_t(r)] +=
+                                    std::abs(at<4>(original[size_t(k)], r, c)) * std::abs(x[size_t(j)][size_t(c)]);
+                    }
+                    for (int r = 0; r < 4; ++r) max_den = std::max(max_den, den[size_t(i)][size_t(r)]);
+                }
+                double const cap = 1e-4 * max_den;
+                berr = 0.0;
+                for (Idx i = 0; i < n; ++i)
+                    for (int r = 0; r < 4; ++r) {
+                        double const d = std::max(den[size_t(i)][size_t(r)], cap);
+                        berr = std::max(berr, std::abs(res[size_t(i)][size_t(r)]) / d);
+                        x[size_t(i)][size_t(r)] += dx[size_t(i)][size_t(r)];
+                    }
+                for (Idx i = 0; i < n; ++i) {
+                    res[size_t(i)] = b[size_t(i)];
+                    for (Idx k = ys.row_indptr_lu[i]; k < ys.row_indptr_lu[i + 1]; ++k) {
+                        Idx const j = ys.col_indices_lu[k];
+                        for (int r = 0; r < 4; ++r)
+                            for (int c = 0; c < 4; ++c)
+                                res[size_t(i)][size_t(r)] -= at<4>(original[size_t(k)], r, c) * x[size_t(j)][size_t(c)];
+                    }
+                }
+            }
+        }
+        (void)nnz;
+        for (Idx i = 0; i < n; ++i)
+            for (int r = 0; r < 4; ++r) rhs[size_t(4 * i + r)] = x[size_t(i)][size_t(r)];
+    }
+};
+
+// the dense gain system instead of the sparse one (a second restatement, to check the first)
+bool g_se_dense = false;
+
+// Newton-Raphson state estimation of one math model; returns the iteration count, u in per unit
+// (newton_raphson_se_solver.hpp:170-214)
+template <class Gain>
+Idx se_solve_on(MathModel<1> const& mm, YStruct const& ys, SeMeasured const& me, Idx slack, bool use_pert,
+                double err_tol, Idx max_iter, std::vector<cd>& u) {
+    (void)ys;
+    Idx const n = mm.n_bus;
+    // dense Y-bus: branches then shunts (y_bus.hpp)
+    std::vector<cd> Y(n * n, cd{0.0, 0.0});
+    std::vector<std::array<cd, 4>> by(mm.branch_bus.size());
+    for (size_t b = 0; b < mm.branch_bus.size(); ++b) {
+        for (int w = 0; w < 4; ++w) by[b][w] = mm.branch_param[b].v[w].m[0];
+        Idx const f = mm.branch_bus[b][0], t = mm.branch_bus[b][1];
+        if (f >= 0) Y[f * n + f] += by[b][0];
+        if (f >= 0 && t >= 0) {
+            Y[f * n + t] += by[b][1];
+            Y[t * n + f] += by[b][2];
+        }
+        if (t >= 0) Y[t * n + t] += by[b][3];
+    }
+    for (size_t k = 0; k < mm.shunt_bus.size(); ++k) Y[mm.shunt_bus[k] * n + mm.shunt_bus[k]] += mm.shunt_param[k].m[0];
+    Idx const vref = !std::isinf(me.volt[slack].var) ? slack : me.first_volt;
+    bool const has_angle = me.n_angle > 0;
+    // state: theta, V, phi_p, phi_q per bus (initialize_unknown :219-236)
+    std::vector<double> th(n), v(n, 1.0), php(n, 0.0), phq(n, 0.0);
+    u.assign(n, cd{});
+    for (Idx b = 0; b < n; ++b) {
+        th[b] = me.mean_shift + mm.phase_shift[b];
+        if (!std::isinf(me.volt[b].var)) {
+            if (me.volt[b].has_angle) th[b] = std::arg(me.volt[b].value);
+            v[b] = me.volt[b].has_angle ? std::abs(me.volt[b].value) : me.volt[b].value.real();
+        }
+        u[b] = std::polar(v[b], th[b]);
+    }
+    Idx const N = 4 * n;
+    // measurement Jacobian of one bus: dP/dtheta, dP/dV, dQ/dtheta, dQ/dV
+    struct J {
+        double pt, pv, qt, qv;
+    };
+    auto jac = [](cd hm, cd nl) { return J{hm.imag(), nl.real(), -hm.real(), nl.imag()}; };
+    auto jac_diag = [](cd fv, cd f) { return J{-f.imag(), fv.real(), f.real(), fv.imag()}; };
+    auto plus = [](J a, J b) { return J{a.pt + b.pt, a.pv + b.pv, a.qt + b.qt, a.qv + b.qv}; };
+    double max_dev = std::numeric_limits<double>::max();
+    Idx num_iter = 0;
+    while (max_dev > err_tol || num_iter == 0) {
+        if (num_iter++ == max_iter)
+            throw IterationDiverge("Iteration failed to converge after " + std::to_string(max_iter) +
+                                   " iterations! Max deviation: " + fmt_double(max_dev) +
+                                   ", error tolerance: " + fmt_double(err_tol) + ".\n");
+        Gain G = [&] {
+            if constexpr (std::is_same_v<Gain, DenseGain>)
+                return DenseGain(n);
+            else
+                return SparseGain(ys, n);
+        }();
+        std::vector<double> rhs(N, 0.0);
+        auto a = [&](Idx i, int r, Idx j, int c) -> double& { return G.ref(i, r, j, c); };
+        // weighted measurement term: G(x, y) += H_x^T W H_y, eta(x) += H_x^T W (z - f)
+        auto wls = [&](std::vector<std::pair<Idx, J>> const& hs, cd f, PowerVar const& z) {
+            double const wp = 1.0 / z.p.var, wq = 1.0 / z.q.var;
+            double const dp = z.p.value - f.real(), dq = z.q.value - f.imag();
+            for (auto const& [x, hx] : hs) {
+                for (auto const& [y, hy] : hs) {
+                    a(x, 0, y, 0) += wp * hx.pt * hy.pt + wq * hx.qt * hy.qt;
+                    a(x, 0, y, 1) += wp * hx.pt * hy.pv + wq * hx.qt * hy.qv;
+                    a(x, 1, y, 0) += wp * hx.pv * hy.pt + wq * hx.qv * hy.qt;
+                    a(x, 1, y, 1) += wp * hx.pv * hy.pv + wq * hx.qv * hy.qv;
+                }
+                rhs[4 * x] += wp * hx.pt * dp + wq * hx.qt * dq;
+                rhs[4 * x + 1] += wp * hx.pv * dp + wq * hx.qv * dq;
+            }
+        };
+        // voltages (:647-679)
+        for (Idx b = 0; b < n; ++b) {
+            if (std::isinf(me.volt[b].var)) continue;
+            double const w = 1.0 / me.volt[b].var;
+            double const mag = me.volt[b].has_angle ? std::abs(me.volt[b].value) : me.volt[b].value.real();
+            a(b, 1, b, 1) += w;
+            rhs[4 * b + 1] += w * (mag - v[b]);
+            if (me.volt[b].has_angle) {
+                a(b, 0, b, 0) += w;
+                rhs[4 * b] += w * (std::arg(me.volt[b].value) - th[b]);
+            } else if (b == vref && !has_angle) {
+                a(b, 0, b, 0) += w;
+                rhs[4 * b] += w * (0.0 - th[b]);
+            }
+        }
+        // branch power at one side: the measured side's bus chi, the other psi (:473-523)
+        for (size_t b = 0; b < mm.branch_bus.size(); ++b) {
+            Idx const f = mm.branch_bus[b][0], t = mm.branch_bus[b][1];
+            if (f < 0 || t < 0 || f == t) continue;
+            for (int side = 0; side < 2; ++side) {
+                if (!(side == 0 ? me.has_from[b] : me.has_to[b])) continue;
+                Idx const chi = side == 0 ? f : t, psi = side == 0 ? t : f;
+                cd const yxx = side == 0 ? by[b][0] : by[b][3], yxm = side == 0 ? by[b][1] : by[b][2];
+                cd const hm1 = std::conj(yxx) * u[chi] * std::conj(u[chi]);
+                cd const hm2 = std::conj(yxm) * u[chi] * std::conj(u[psi]);
+                cd const fx = hm1 + hm2;
+                J const h_chi = plus(jac(hm1, hm1 / v[chi]), jac_diag(fx / v[chi], fx));
+                J const h_psi = jac(hm2, hm2 / v[psi]);
+                wls({{chi, h_chi}, {psi, h_psi}}, fx, side == 0 ? me.br_from[b] : me.br_to[b]);
+            }
+        }
+        // shunt power (:450-471)
+        for (size_t k = 0; k < mm.shunt_bus.size(); ++k) {
+            if (!me.has_shunt[k]) continue;
+            Idx const b = mm.shunt_bus[k];
+            cd const hm = std::conj(-mm.shunt_param[k].m[0]) * u[b] * std::conj(u[b]);
+            J const h = plus(jac(hm, hm / v[b]), jac_diag(hm / v[b], hm));
+            wls({{b, h}}, hm, me.shunt[k]);
+        }
+        // Y-bus rows without structural zeros
+        std::vector<std::vector<std::pair<Idx, cd>>> yrow(static_cast<size_t>(n));
+        for (Idx i = 0; i < n; ++i)
+            for (Idx j = 0; j < n; ++j)
+                if (Y[size_t(i * n + j)] != cd{0.0, 0.0}) yrow[size_t(i)].emplace_back(j, Y[size_t(i * n + j)]);
+        // injection constraints with Lagrange multipliers (:397-448, 583-617); rows 2, 3 of bus i: Q, R
+        for (Idx i = 0; i < n; ++i) {
+            if (!me.has_inj[i]) {
+                a(i, 2, i, 2) = -1.0;
+                a(i, 3, i, 3) = -1.0;
+                continue;
+            }
+            cd f_inj{0.0, 0.0};
+            J diag{0, 0, 0, 0};
+            for (auto const& [j, y] : yrow[size_t(i)]) {
+                cd const hm = std::conj(y) * u[i] * std::conj(u[j]);
+                J const q = jac(hm, hm / v[j]);
+                a(i, 2, j, 0) += q.pt;
+                a(i, 2, j, 1) += q.pv;
+                a(i, 3, j, 0) += q.qt;
+                a(i, 3, j, 1) += q.qv;
+                J const qd = jac_diag(hm / v[i], hm);
+                diag = plus(diag, qd);
+                f_inj += hm;
+            }
+            a(i, 2, i, 0) += diag.pt;
+            a(i, 2, i, 1) += diag.pv;
+            a(i, 3, i, 0) += diag.qt;
+            a(i, 3, i, 1) += diag.qv;
+            rhs[4 * i + 2] += me.inj[i].p.value - f_inj.real();
+            rhs[4 * i + 3] += me.inj[i].q.value - f_inj.imag();
+            a(i, 2, i, 2) = -me.inj[i].p.var;
+            a(i, 3, i, 3) = -me.inj[i].q.var;
+        }
+        // Q^T and the Lagrange terms
+        G.for_each_entry([&](Idx i, Idx j) {
+            a(i, 0, j, 2) = a(j, 2, i, 0);
+            a(i, 0, j, 3) = a(j, 3, i, 0);
+            a(i, 1, j, 2) = a(j, 2, i, 1);
+            a(i, 1, j, 3) = a(j, 3, i, 1);
+            rhs[4 * i] += a(i, 0, j, 2) * php[j] + a(i, 0, j, 3) * phq[j];
+            rhs[4 * i + 1] += a(i, 1, j, 2) * php[j] + a(i, 1, j, 3) * phq[j];
+        });
+        G.solve(rhs, use_pert);
+        // iterate_unknown (:777-821)
+        double const offset = has_angle ? 0.0 : th[slack] + rhs[4 * slack];
+        max_dev = 0.0;
+        for (Idx b = 0; b < n; ++b) {
+            th[b] += rhs[4 * b] - offset;
+            v[b] += rhs[4 * b + 1];
+            if (me.has_inj[b]) {
+                if (me.inj[b].p.var == 0.0) php[b] += rhs[4 * b + 2];
+                if (me.inj[b].q.var == 0.0) phq[b] += rhs[4 * b + 3];
+            }
+            cd const nu = std::polar(v[b], th[b]);
+            max_dev = std::max(max_dev, std::abs(nu - u[b]));
+            u[b] = nu;
+        }
+    }
+    return num_iter;
+}
+
+Idx se_solve(MathModel<1> const& mm, YStruct const& ys, SeMeasured const& me, Idx slack, bool use_pert,
+             double err_tol, Idx max_iter, std::vector<cd>& u) {
+    return g_se_dense ? se_solve_on<DenseGain>(mm, ys, me, slack, use_pert, err_tol, max_iter, u)
+                      : se_solve_on<SparseGain>(mm, ys, me, slack, use_pert, err_tol, max_iter, u);
+}
+
+// one scenario: every math model estimated, outputs written (main_core/output.hpp:189-280)
+Idx se_scenario(Model const& m, State const& st, TopoCache const& tc, SeSensorSet const& S,
+                SeSensorValues const& val, double err_tol, Idx max_iter, OrMutableDataset const& out,
+                OrVoltageSensorOutputSym* vs_out, OrPowerSensorOutputSym* ps_out, Idx scn) {
+    Idx const nmm = Idx(tc.topo.bus_nodes.size());
+    std::vector<MathOutput<1>> res(nmm);
+    std::vector<SeMeasured> meas(nmm);
+    std::vector<MathModel<1>> models(nmm);
+    std::vector<std::vector<cd>> bus_s(nmm);
+    Idx iters = 0;
+    for (Idx g = 0; g < nmm; ++g) {
+        MathModel<1>& mm = models[g];
+        mm = make_math_model<1>(m, st, tc, g);
+        Idx const n = mm.n_bus;
+        SeMeasured& me = meas[g];
+        me = measure(m, st, tc, mm, g, S, val);
+        std::vector<std::vector<Idx>> sh_per_bus(n);
+        for (size_t k = 0; k < mm.shunt_bus.size(); ++k) sh_per_bus[mm.shunt_bus[k]].push_back(Idx(k));
+        YStruct const ys = build_ystruct(n, mm.branch_bus, sh_per_bus, mm.fill_in);
+        bool const use_pert = check_observable(mm, ys, me, tc.topo.radial[g]);
+        Idx const slack = tc.topo.slack[g];
+        std::vector<cd> u;
+        iters = std::max(iters, se_solve(mm, ys, me, slack, use_pert, err_tol, max_iter, u));
+        // results (common_solver_functions.hpp:449-456, y_bus.hpp:482-546, measured_values.hpp:124-150)
+        MathOutput<1>& r = res[g];
+        r.u.resize(n);
+        for (Idx b = 0; b < n; ++b) r.u[b].v[0] = u[b];
+        std::vector<cd> s(n, cd{});
+        {
+            std::vector<cd> Y(n * n, cd{});
+            for (size_t b = 0; b < mm.branch_bus.size(); ++b) {
+                Idx const f = mm.branch_bus[b][0], t = mm.branch_bus[b][1];
+                auto const& bp = mm.branch_param[b];
+                if (f >= 0) Y[f * n + f] += bp.v[0].m[0];
+                if (f >= 0 && t >= 0) {
+                    Y[f * n + t] += bp.v[1].m[0];
+                    Y[t * n + f] += bp.v[2].m[0];
+                }
+                if (t >= 0) Y[t * n + t] += bp.v[3].m[0];
+            }
+            for (size_t k = 0; k < mm.shunt_bus.size(); ++k)
+                Y[mm.shunt_bus[k] * n + mm.shunt_bus[k]] += mm.shunt_param[k].m[0];
+            for (Idx i = 0; i < n; ++i) {
+                cd cur{0.0, 0.0};
+                for (Idx j = 0; j < n; ++j) cur += Y[i * n + j] * u[j];
+                s[i] = std::conj(cur) * u[i];
+            }
+        }
+        bus_s[g] = s;
+        r.bus_injection.resize(n);
+        for (Idx b = 0; b < n; ++b) r.bus_injection[b].v[0] = s[b];
+        Idx const nbr = Idx(mm.branch_bus.size());
+        r.br_sf.resize(nbr);
+        r.br_st.resize(nbr);
+        r.br_if.resize(nbr);
+        r.br_it.resize(nbr);
+        for (Idx b = 0; b < nbr; ++b) {
+            Idx const f = mm.branch_bus[b][0], t = mm.branch_bus[b][1];
+            cd const uf = f >= 0 ? u[f] : cd{}, ut = t >= 0 ? u[t] : cd{};
+            auto const& bp = mm.branch_param[b];
+            cd const i_f = bp.v[0].m[0] * uf + bp.v[1].m[0] * ut, i_t = bp.v[2].m[0] * uf + bp.v[3].m[0] * ut;
+            r.br_if[b].v[0] = i_f;
+            r.br_it[b].v[0] = i_t;
+            r.br_sf[b].v[0] = uf * std::conj(i_f);
+            r.br_st[b].v[0] = ut * std::conj(i_t);
+        }
+        Idx const nsh = Idx(mm.shunt_bus.size());
+        r.sh_s.resize(nsh);
+        r.sh_i.resize(nsh);
+        for (Idx k = 0; k < nsh; ++k) {
+            cd const ub = u[mm.shunt_bus[k]];
+            cd const i = -mm.shunt_param[k].m[0] * ub;
+            r.sh_i[k].v[0] = i;
+            r.sh_s[k].v[0] = ub * std::conj(i);
+        }
+        Idx const nlg = Idx(mm.load_bus.size()), nsrc = Idx(mm.source_bus.size());
+        r.ld_s.assign(nlg, Val<1>{});
+        r.ld_i.assign(nlg, Val<1>{});
+        r.src_s.assign(nsrc, Val<1>{});
+        r.src_i.assign(nsrc, Val<1>{});
+        for (Idx b = 0; b < n; ++b) {
+            PowerVar const& ap = me.bus_appl[b];
+            if (me.bus_unmeasured[b] > 0) {
+                cd const each = (s[b] - cd(ap.p.value, ap.q.value)) / double(me.bus_unmeasured[b]);
+                for (Idx k = 0; k < nlg; ++k)
+                    if (mm.load_bus[k] == b)
+                        r.ld_s[k].v[0] = me.lg_state[k] == 2 ? cd(me.lg_val[k].p.value, me.lg_val[k].q.value)
+                                         : me.lg_state[k] == 1 ? each
+                                                               : cd{};
+                for (Idx k = 0; k < nsrc; ++k)
+                    if (mm.source_bus[k] == b)
+                        r.src_s[k].v[0] = me.src_state[k] == 2 ? cd(me.src_val[k].p.value, me.src_val[k].q.value)
+                                          : me.src_state[k] == 1 ? each
+                                                                 : cd{};
+            } else {
+                cd const delta = cd(ap.p.value, ap.q.value) - s[b];
+                double const mu_p = delta.real() / ap.p.var, mu_q = delta.imag() / ap.q.var;
+                for (Idx k = 0; k < nlg; ++k)
+                    if (mm.load_bus[k] == b && me.lg_state[k] == 2)
+                        r.ld_s[k].v[0] = cd(me.lg_val[k].p.value - me.lg_val[k].p.var * mu_p,
+                                            me.lg_val[k].q.value - me.lg_val[k].q.var * mu_q);
+                for (Idx k = 0; k < nsrc; ++k)
+                    if (mm.source_bus[k] == b && me.src_state[k] == 2)
+                        r.src_s[k].v[0] = cd(me.src_val[k].p.value - me.src_val[k].p.var * mu_p,
+                                             me.src_val[k].q.value - me.src_val[k].q.var * mu_q);
+            }
+        }
+        for (Idx k = 0; k < nlg; ++k) r.ld_i[k].v[0] = std::conj(r.ld_s[k].v[0] / u[mm.load_bus[k]]);
+        for (Idx k = 0; k < nsrc; ++k) r.src_i[k].v[0] = std::conj(r.src_s[k].v[0] / u[mm.source_bus[k]]);
+    }
+    write_outputs<1>(m, st, tc, res, out, scn);
+    // node p, q: the bus injection (node.hpp get_output with the solver's bus_injection)
+    if (out.data[OR_NODE]) {
+        Idx const nn = Idx(m.node.size());
+        for (Idx i = 0; i < nn; ++i) {
+            Idx const g = tc.topo.node_group[i];
+            if (g < 0) continue;
+            auto& o = static_cast<OrNodeOutputSym*>(out.data[OR_NODE])[scn * nn + i];
+            cd const sv = bus_s[g][tc.topo.node_pos[i]];
+            o.p = kBaseP3 * sv.real();
+            o.q = kBaseP3 * sv.imag();
+        }
+    }
+    // sensors (voltage_sensor.hpp get_sym_output, power_sensor.hpp get_generic_output)
+    Idx const nvs = Idx(S.vs_id.size()), nps = Idx(S.ps_id.size());
+    for (Idx i = 0; vs_out && i < nvs; ++i) {
+        auto& o = vs_out[scn * nvs + i];
+        o = OrVoltageSensorOutputSym{};
+        o.id = S.vs_id[i];
+        Idx const node = S.vs_node[i], g = tc.topo.node_group[node];
+        if (g < 0) continue;
+        o.energized = 1;
+        cd const uc = res[g].u[tc.topo.node_pos[node]].v[0];
+        VoltVar const mv = voltage_calc(m, S, val, i);
+        double const ur = m.node[node].u_rated;
+        if (mv.has_angle) {
+            o.u_residual = (std::abs(mv.value) - std::abs(uc)) * ur;
+            o.u_angle_residual = std::arg(std::exp(cd(0.0, std::arg(mv.value) - std::arg(uc))));
+        } else {
+            o.u_residual = (mv.value.real() - std::abs(uc)) * ur;
+            o.u_angle_residual = kNan;
+        }
+    }
+    for (Idx i = 0; ps_out && i < nps; ++i) {
+        auto& o = ps_out[scn * nps + i];
+        o = OrPowerSensorOutputSym{};
+        o.id = S.ps_id[i];
+        Idx const obj = S.ps_obj[i];
+        Idx g = -1, pos = -1;
+        switch (S.ps_term[i]) {
+        case 0:
+        case 1: g = tc.br_group[obj]; pos = tc.br_pos[obj]; break;
+        case 2: g = tc.src_group[obj]; pos = tc.src_pos[obj]; break;
+        case 3: g = tc.sh_group[obj]; pos = tc.sh_pos[obj]; break;
+        case 4:
+        case 5: g = tc.lg_group[obj]; pos = tc.lg_pos[obj]; break;
+        case 9: g = tc.topo.node_group[obj]; pos = g >= 0 ? tc.topo.node_pos[obj] : -1; break;
+        default: break;
+        }
+        if (g < 0) continue;
+        o.energized = 1;
+        cd sc{};
+        switch (S.ps_term[i]) {
+        case 0: sc = res[g].br_sf[pos].v[0]; break;
+        case 1: sc = res[g].br_st[pos].v[0]; break;
+        case 2: sc = res[g].src_s[pos].v[0]; break;
+        case 3: sc = res[g].sh_s[pos].v[0]; break;
+        case 4:
+        case 5: sc = res[g].ld_s[pos].v[0]; break;
+        case 9: sc = bus_s[g][pos]; break;
+        default: break;
+        }
+        PowerVar const pm = power_calc(val, S, i);
+        double const dir = sensor_direction(S.ps_term[i]);
+        cd const resid = (cd(pm.p.value, pm.q.value) - sc) * dir * kBaseP3;
+        o.p_residual = resid.real();
+        o.q_residual = resid.imag();
+    }
+    return iters;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------
@@ -2927,4 +3809,148 @@ extern "C" int64_t or_nr_math_model(int p, int64_t n_bus, int64_t n_branch, int6
     return nr_math_model_impl<3>(n_bus, n_branch, branch_bus, branch_param, n_shunt, shunt_bus, shunt_param, n_source,
                                  source_bus, source_y, u_source, n_load, load_bus, load_type, s_inj, n_fill, fill_in,
                                  err_tol, max_iter, u, branch_flow, shunt_flow, load_flow, source_flow, bus_injection);
+}
+
+extern "C" void or_set_se_dense(int dense) { g_se_dense = dense != 0; }
+extern "C" void or_se_counters(long long* perturbed, long long* refine_solves, int reset) {
+    if (perturbed) *perturbed = g_se_perturbed.load();
+    if (refine_solves) *refine_solves = g_se_refine_solves.load();
+    if (reset) {
+        g_se_perturbed = 0;
+        g_se_refine_solves = 0;
+    }
+}
+
+extern "C" int or_state_estimation(OrConstDataset const* input, double system_frequency, OrConstDataset const* update,
+                                   int64_t n_scenarios, double err_tol, int64_t max_iter, int64_t threading,
+                                   OrSymVoltageSensorInput const* vs, int64_t n_vs,
+                                   OrSymVoltageSensorUpdate const* vs_upd, int64_t vs_upd_count,
+                                   OrSymPowerSensorInput const* ps, int64_t n_ps, OrSymPowerSensorUpdate const* ps_upd,
+                                   int64_t ps_upd_count, OrMutableDataset const* output,
+                                   OrVoltageSensorOutputSym* vs_out, OrPowerSensorOutputSym* ps_out,
+                                   int64_t* iterations, int32_t* status, char* messages, int64_t msg_stride) {
+    Model model;
+    SeSensorSet S;
+    SeSensorValues base;
+    try {
+        model = build_model(*input, system_frequency);
+        // sensors: the measured object must exist and be of the terminal type's kind (main_core/input.hpp:84-129)
+        for (Idx i = 0; i < n_vs; ++i) {
+            S.vs_id.push_back(vs[i].id);
+            S.vs_node.push_back(node_idx(model, vs[i].measured_object));
+            base.vs_sigma.push_back(vs[i].u_sigma);
+            base.vs_u.push_back(vs[i].u_measured);
+            base.vs_ang.push_back(vs[i].u_angle_measured);
+        }
+        for (Idx i = 0; i < n_ps; ++i) {
+            auto it = model.ids.find(ps[i].measured_object);
+            if (it == model.ids.end())
+                throw OtherError("The id cannot be found: " + std::to_string(ps[i].measured_object) + "\n");
+            int const comp = it->second.first;
+            Idx obj = it->second.second;
+            int const term = ps[i].measured_terminal_type;
+            bool ok = false;
+            if (term == 0 || term == 1) {
+                ok = comp == OR_LINE || comp == OR_GENERIC_BRANCH || comp == OR_TRANSFORMER;
+                if (comp == OR_GENERIC_BRANCH) obj += Idx(model.line.size() + model.link.size());
+                if (comp == OR_TRANSFORMER) obj += model.n_lslot();
+            } else if (term == 2) {
+                ok = comp == OR_SOURCE;
+            } else if (term == 3) {
+                ok = comp == OR_SHUNT;
+            } else if (term == 4) {
+                ok = comp == OR_SYM_LOAD || comp == OR_ASYM_LOAD;
+                obj += lg_offset(model, comp);
+            } else if (term == 5) {
+                ok = comp == OR_SYM_GEN || comp == OR_ASYM_GEN;
+                obj += lg_offset(model, comp);
+            } else if (term == 9) {
+                ok = comp == OR_NODE;
+            }
+            if (!ok) throw OtherError("Wrong type for object with id " + std::to_string(ps[i].measured_object) + "\n");
+            S.ps_id.push_back(ps[i].id);
+            S.ps_term.push_back(term);
+            S.ps_obj.push_back(obj);
+            base.ps_sigma.push_back(ps[i].power_sigma);
+            base.ps_p.push_back(ps[i].p_measured);
+            base.ps_q.push_back(ps[i].q_measured);
+            base.ps_psig.push_back(ps[i].p_sigma);
+            base.ps_qsig.push_back(ps[i].q_sigma);
+        }
+    } catch (std::exception const& e) {
+        write_msg(messages, msg_stride, 0, e.what());
+        return -1;
+    }
+    if (!update) n_scenarios = 1;
+    std::map<int32_t, Idx> vs_index, ps_index;
+    for (Idx i = 0; i < n_vs; ++i) vs_index[vs[i].id] = i;
+    for (Idx i = 0; i < n_ps; ++i) ps_index[ps[i].id] = i;
+    // job_dispatch.hpp:166-172
+    Idx n_threads;
+    if (threading < 0 || threading == 1 || !update)
+        n_threads = 1;
+    else if (threading == 0) {
+        Idx const hw = Idx(std::thread::hardware_concurrency());
+        n_threads = hw < 2 ? 1 : std::min(hw, n_scenarios);
+    } else
+        n_threads = std::min<Idx>(threading, n_scenarios);
+    n_threads = std::max<Idx>(n_threads, 1);
+    auto worker = [&](Idx t) {
+    for (Idx s = t; s < n_scenarios; s += n_threads) {
+        State st = model.base;
+        SeSensorValues val = base;
+        try {
+            bool topo_changed = false, param_changed = false;
+            if (update) apply_update(model, st, *update, s, topo_changed, param_changed);
+            // sensor updates: NaN keeps the value (voltage_sensor.hpp:88-100, power_sensor.hpp:94-105)
+            auto find = [](std::map<int32_t, Idx> const& idx, int32_t id, Idx row) -> Idx {
+                if (id == kNaID) return row;
+                auto it = idx.find(id);
+                if (it == idx.end()) throw OtherError("The id cannot be found: " + std::to_string(id) + "\n");
+                return it->second;
+            };
+            for (Idx r = 0; vs_upd && r < vs_upd_count; ++r) {
+                auto const& u = vs_upd[s * vs_upd_count + r];
+                Idx const i = find(vs_index, u.id, r);
+                if (!std::isnan(u.u_measured)) val.vs_u[i] = u.u_measured;
+                if (!std::isnan(u.u_angle_measured)) val.vs_ang[i] = u.u_angle_measured;
+                if (!std::isnan(u.u_sigma)) val.vs_sigma[i] = u.u_sigma;
+            }
+            for (Idx r = 0; ps_upd && r < ps_upd_count; ++r) {
+                auto const& u = ps_upd[s * ps_upd_count + r];
+                Idx const i = find(ps_index, u.id, r);
+                if (!std::isnan(u.p_measured)) val.ps_p[i] = u.p_measured;
+                if (!std::isnan(u.q_measured)) val.ps_q[i] = u.q_measured;
+                if (!std::isnan(u.power_sigma)) val.ps_sigma[i] = u.power_sigma;
+                if (!std::isnan(u.p_sigma)) val.ps_psig[i] = u.p_sigma;
+                if (!std::isnan(u.q_sigma)) val.ps_qsig[i] = u.q_sigma;
+            }
+            TopoCache const tc = build_topo_cache(model, st);
+            Idx const it = se_scenario(model, st, tc, S, val, err_tol, max_iter, *output, vs_out, ps_out, s);
+            if (iterations) iterations[s] = it;
+            if (status) status[s] = 0;
+            write_msg(messages, msg_stride, s, "");
+        } catch (SparseMatrixError const& e) {
+            if (iterations) iterations[s] = 0;
+            if (status) status[s] = 1;
+            write_msg(messages, msg_stride, s, e.what());
+        } catch (IterationDiverge const& e) {
+            if (iterations) iterations[s] = 0;
+            if (status) status[s] = 2;
+            write_msg(messages, msg_stride, s, e.what());
+        } catch (std::exception const& e) {
+            if (iterations) iterations[s] = 0;
+            if (status) status[s] = 3;
+            write_msg(messages, msg_stride, s, e.what());
+        }
+    }
+    };
+    if (n_threads == 1) {
+        worker(0);
+    } else {
+        std::vector<std::thread> th;
+        for (Idx t = 0; t < n_threads; ++t) th.emplace_back(worker, t);
+        for (auto& x : th) x.join();
+    }
+    return 0;
 }

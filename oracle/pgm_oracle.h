@@ -46,6 +46,12 @@ typedef struct {
     double u_set, u_band, line_drop_compensation_r, line_drop_compensation_x;
 } OrTapRegulatorInput;
 
+typedef struct { int32_t id, measured_object; double u_sigma, u_measured, u_angle_measured; } OrSymVoltageSensorInput;
+typedef struct {
+    int32_t id, measured_object; int8_t measured_terminal_type;
+    double power_sigma, p_measured, q_measured, p_sigma, q_sigma;
+} OrSymPowerSensorInput;
+
 /* ---------- update records ---------- */
 typedef struct { int32_t id; int8_t from_status, to_status; } OrBranchUpdate;
 typedef struct { int32_t id; int8_t from_status, to_status, tap_pos; } OrTransformerUpdate;
@@ -56,6 +62,9 @@ typedef struct { int32_t id; int8_t status; double p_specified[3], q_specified[3
 typedef struct {
     int32_t id; int8_t status; double u_set, u_band, line_drop_compensation_r, line_drop_compensation_x;
 } OrTapRegulatorUpdate;
+
+typedef struct { int32_t id; double u_sigma, u_measured, u_angle_measured; } OrSymVoltageSensorUpdate;
+typedef struct { int32_t id; double power_sigma, p_measured, q_measured, p_sigma, q_sigma; } OrSymPowerSensorUpdate;
 
 /* ---------- output records ---------- */
 typedef struct { int32_t id; int8_t energized; double u_pu, u, u_angle, p, q; } OrNodeOutputSym;
@@ -70,6 +79,9 @@ typedef struct {
 } OrBranchOutputAsym;
 typedef struct { int32_t id; int8_t energized; double p, q, i, s, pf; } OrApplianceOutputSym;
 typedef struct { int32_t id; int8_t energized; double p[3], q[3], i[3], s[3], pf[3]; } OrApplianceOutputAsym;
+
+typedef struct { int32_t id; int8_t energized; double u_residual, u_angle_residual; } OrVoltageSensorOutputSym;
+typedef struct { int32_t id; int8_t energized; double p_residual, q_residual; } OrPowerSensorOutputSym;
 
 /* component slots, in the reference's container order (all_components.hpp:37-40), the link and the generic
  * branch last */
@@ -119,6 +131,25 @@ int or_power_flow_tap(OrConstDataset const* input, double system_frequency, OrCo
                       OrTapRegulatorUpdate const* reg_update, int64_t reg_update_count,
                       OrMutableDataset const* output, int8_t* tap_out, int64_t* iterations, int32_t* status,
                       char* messages, int64_t msg_stride);
+/* Batch Newton-Raphson state estimation, symmetric, FP64 (math_solver/newton_raphson_se_solver.hpp with
+ * measured_values.hpp and observability.hpp; the gain system is solved densely). The sensors come beside the
+ * datasets: vs / ps input records, vs_upd / ps_upd NULL or n_scenarios * *_upd_count update records; outputs as
+ * or_power_flow (symmetric records) plus vs_out / ps_out, n_scenarios * n_vs / n_ps records.
+ * status: 0 ok, 1 SparseMatrixError, 2 IterationDiverge, 3 other error (NotObservableError included). */
+int or_state_estimation(OrConstDataset const* input, double system_frequency, OrConstDataset const* update,
+                        int64_t n_scenarios, double err_tol, int64_t max_iter, int64_t threading,
+                        OrSymVoltageSensorInput const* vs, int64_t n_vs, OrSymVoltageSensorUpdate const* vs_upd,
+                        int64_t vs_upd_count, OrSymPowerSensorInput const* ps, int64_t n_ps,
+                        OrSymPowerSensorUpdate const* ps_upd, int64_t ps_upd_count, OrMutableDataset const* output,
+                        OrVoltageSensorOutputSym* vs_out, OrPowerSensorOutputSym* ps_out, int64_t* iterations,
+                        int32_t* status, char* messages, int64_t msg_stride);
+
+/* The state estimation's gain system: 1 solves it densely (partial pivoting), 0 (the default) with the block
+ * sparse LU, pivot perturbation and iterative refinement as the reference (sparse_lu_solver.hpp). */
+void or_set_se_dense(int dense);
+/* Sparse gain factorisations that perturbed a pivot and the refinement solves they took since the last reset. */
+void or_se_counters(long long* perturbed, long long* refine_solves, int reset);
+
 /* Minimum-degree ordering (sparse_ordering.hpp:183-219). The adjacency map is given as sorted keys with
  * CSR neighbour lists. alpha_out has room for n_vertices (all vertices), fills_out for 2*max_fills.
  * Returns the number of fills, or -1 if max_fills is too small. n_alpha_out receives the order length. */

@@ -50,6 +50,14 @@ def lib():
         _lib.or_power_flow_tap.argtypes = [C.POINTER(_Const), C.c_double, C.POINTER(_Const), _i64, C.c_int,
                                            C.c_double, _i64, _i64, C.c_int, C.c_int, _p, _i64, _p, _i64,
                                            C.POINTER(_Mut), _p, _p, _p, _p, _i64]
+        _lib.or_state_estimation.restype = C.c_int
+        _lib.or_state_estimation.argtypes = [C.POINTER(_Const), C.c_double, C.POINTER(_Const), _i64, C.c_double,
+                                             _i64, _i64, _p, _i64, _p, _i64, _p, _i64, _p, _i64, C.POINTER(_Mut), _p, _p,
+                                             _p, _p, _p, _i64]
+        _lib.or_set_se_dense.restype = None
+        _lib.or_set_se_dense.argtypes = [C.c_int]
+        _lib.or_se_counters.restype = None
+        _lib.or_se_counters.argtypes = [_p, _p, C.c_int]
         _lib.or_minimum_degree_ordering.restype = _i64
         _lib.or_minimum_degree_ordering.argtypes = [_i64, _p, _p, _p, _p, _p, _p, _i64]
         _lib.or_build_topology.restype = _i64
@@ -122,6 +130,60 @@ def power_flow_tap(input_data: dict, update: dict | None = None, n_scenarios: in
     if rc != 0:
         raise RuntimeError(messages[0])
     return out, taps[:, :len(reg)], iters, status, messages
+
+
+def state_estimation(input_data: dict, update: dict | None = None, n_scenarios: int = 1, err_tol: float = 1e-8,
+                     max_iter: int = 20, system_frequency: float = 50.0, threading: int = -1):
+    """Symmetric Newton-Raphson state estimation on the oracle. input_data / update may hold sym_voltage_sensor
+    and sym_power_sensor (update arrays of shape (n_scenarios, rows)).
+
+    Returns (output dict with the sensors' residuals, iterations, status, messages); status 1 marks a
+    SparseMatrixError, 2 a diverged iteration, 3 any other failure (NotObservableError and the like)."""
+    keep, cin, cup, cout, out, n_scenarios = _datasets(input_data, update, n_scenarios, True)
+
+    def sensor(name, table):
+        arr = input_data.get(name)
+        return np.ascontiguousarray(arr if arr is not None else np.zeros(0, table[name]), dtype=table[name])
+
+    def sensor_update(name):
+        arr = None if update is None else update.get(name)
+        return None if arr is None else np.ascontiguousarray(arr, dtype=UPDATE[name])
+
+    vs, ps = sensor("sym_voltage_sensor", INPUT), sensor("sym_power_sensor", INPUT)
+    vu, pu = sensor_update("sym_voltage_sensor"), sensor_update("sym_power_sensor")
+    vs_out = np.zeros((n_scenarios, len(vs)), dtype=SYM_OUTPUT["sym_voltage_sensor"])
+    ps_out = np.zeros((n_scenarios, len(ps)), dtype=SYM_OUTPUT["sym_power_sensor"])
+    iters = np.zeros(n_scenarios, dtype=np.int64)
+    status = np.zeros(n_scenarios, dtype=np.int32)
+    stride = 512
+    msgs = np.zeros(n_scenarios * stride, dtype=np.uint8)
+    rc = lib().or_state_estimation(C.byref(cin), system_frequency, C.byref(cup) if cup is not None else None,
+                                   n_scenarios, err_tol, max_iter, threading, _ptr(vs), len(vs), _ptr(vu),
+                                   0 if vu is None else vu.shape[1], _ptr(ps), len(ps), _ptr(pu),
+                                   0 if pu is None else pu.shape[1], C.byref(cout), _ptr(vs_out), _ptr(ps_out),
+                                   _ptr(iters), _ptr(status), _ptr(msgs), stride)
+    messages = [bytes(msgs[i * stride:(i + 1) * stride]).split(b"\0")[0].decode() for i in range(n_scenarios)]
+    if rc != 0:
+        raise RuntimeError(messages[0])
+    if len(vs):
+        out["sym_voltage_sensor"] = vs_out
+    if len(ps):
+        out["sym_power_sensor"] = ps_out
+    return out, iters, status, messages
+
+
+def set_se_dense(dense: bool) -> None:
+    """Solve the state estimation's gain system densely (a second restatement) instead of with the block sparse
+    LU, pivot perturbation and refinement."""
+    lib().or_set_se_dense(int(dense))
+
+
+def se_counters(reset: bool = True) -> tuple[int, int]:
+    """(sparse gain factorisations that perturbed a pivot, refinement solves) since the last reset."""
+    a = C.c_longlong(0)
+    b = C.c_longlong(0)
+    lib().or_se_counters(C.addressof(a), C.addressof(b), int(reset))
+    return a.value, b.value
 
 
 def _datasets(input_data: dict, update: dict | None, n_scenarios: int, symmetric: bool):

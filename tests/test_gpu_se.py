@@ -7,7 +7,7 @@ import pytest
 from validation_utils import GOLDEN_SE, compare_result, list_cases, load_case
 
 pgm = pytest.importorskip("power_grid_model_amd")
-from power_grid_model_amd import PowerGridModel  # noqa: E402
+from power_grid_model_amd import PowerGridModel, synthetic  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -58,3 +58,44 @@ def test_state_estimation_batch_lanes_independent():
             for attr in single[comp].dtype.names:
                 np.testing.assert_allclose(batch[comp][attr][s], single[comp][attr], rtol=1e-12, atol=1e-12,
                                            err_msg=f"{comp}.{attr} scenario {s}")
+
+
+SE_ORACLE_CASES = [
+    ("radial_feeder_60", lambda: synthetic.radial_feeder_case(n_node=60, n_scenarios=70, seed=5)),
+    ("radial_200", lambda: synthetic.radial_200(n_scenarios=6, seed=6)),
+]
+
+
+@pytest.mark.parametrize("name,make", SE_ORACLE_CASES, ids=[c[0] for c in SE_ORACLE_CASES])
+def test_state_estimation_against_oracle(name, make):
+    """Synthetic grids with noisy measurements taken from their power flows (voltage magnitudes on a share of
+    the nodes, the source's voltage phasor, every load and source power, from-side flows on a share of the
+    lines), a batch of scenarios: the GPU estimate equals the oracle's restatement of the reference's
+    Newton-Raphson state estimation scenario by scenario (FP64 both; the same iteration, so agreement far below
+    the 1e-8 convergence tolerance)."""
+    import oracle_lib
+
+    case = make()
+    pf, _, st, msg = oracle_lib.power_flow(case.input, case.update, case.n_scenarios, err_tol=1e-12)
+    assert (st == 0).all(), msg
+    se_case = synthetic.state_estimation_case(case, pf, voltage_share=0.1, flow_share=0.2)
+    want, iters, st, msg = oracle_lib.state_estimation(se_case.input, se_case.update, se_case.n_scenarios)
+    assert (st == 0).all(), msg
+    got = PowerGridModel(se_case.input).calculate_state_estimation(update_data=se_case.update,
+                                                                   calculation_method="newton_raphson")
+    upd = se_case.update
+    measured = {"sym_voltage_sensor": upd["sym_voltage_sensor"]["u_measured"],
+                "sym_power_sensor": np.concatenate([upd["sym_power_sensor"]["p_measured"],
+                                                    upd["sym_power_sensor"]["q_measured"]])}
+    np.testing.assert_allclose(got["node"]["u_pu"], want["node"]["u_pu"], rtol=1e-9, atol=0)
+    np.testing.assert_allclose(got["node"]["u_angle"], want["node"]["u_angle"], rtol=0, atol=1e-9)
+    for comp, attrs in (("node", ("p", "q")), ("line", ("p_from", "q_from", "p_to", "q_to")),
+                        ("sym_load", ("p", "q")), ("source", ("p", "q")),
+                        ("sym_voltage_sensor", ("u_residual",)), ("sym_power_sensor", ("p_residual", "q_residual"))):
+        for attr in attrs:
+            # a residual is a difference of two values of the measured quantity's size: its absolute bound
+            # follows that size
+            ref = measured[comp] if comp in measured else want[comp][attr]
+            scale = max(float(np.abs(ref).max()), 1.0)
+            np.testing.assert_allclose(got[comp][attr], want[comp][attr], rtol=1e-7, atol=1e-9 * scale,
+                                       err_msg=f"{comp}.{attr}")
