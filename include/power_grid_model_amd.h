@@ -161,6 +161,9 @@ PGM_AMD_API void PGM_dataset_const_add_attribute_buffer(PGM_Handle* handle, PGM_
                                                         void const* data) PGM_AMD_NOEXCEPT;
 PGM_AMD_API PGM_DatasetInfo const* PGM_dataset_const_get_info(PGM_Handle* handle,
                                                               PGM_ConstDataset const* dataset) PGM_AMD_NOEXCEPT;
+/* dataset.h PGM_dataset_const_set_next_cartesian_product_dimension: a batch over the cartesian product of the
+ * chained datasets; PGM_calculate runs it as model.cpp:285-334 (first dimension outermost, each of its scenarios
+ * a permanent update of a model copy, the rest one batch on that copy) */
 PGM_AMD_API void PGM_dataset_const_set_next_cartesian_product_dimension(
     PGM_Handle* handle, PGM_ConstDataset* dataset, PGM_ConstDataset const* next_dataset) PGM_AMD_NOEXCEPT;
 PGM_AMD_API PGM_MutableDataset* PGM_create_dataset_mutable(PGM_Handle* handle, char const* dataset,

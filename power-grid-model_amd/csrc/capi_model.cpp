@@ -14,6 +14,102 @@
 using namespace pgm_amd;
 
 namespace {
+// a permanent update of the model's base state (main_core/update.hpp permanent_update_t): every component the
+// power flow and the state estimation read, the sensors included
+void update_model_permanent(PGM_PowerGridModel& model, PGM_ConstDataset const& update) {
+    if (update.meta->name != "update") throw PowerGridError("update dataset must be of type update\n");
+    Model& m = model.model;
+    ComponentState st = m.base;
+    std::vector<char> rec;
+    for (auto const& buf : update.buffers) {
+        int comp = -1;
+        for (int c = 0; c < N_COMP; ++c)
+            if (buf.meta->name == comp_names[c]) comp = c;
+        rec.resize(buf.meta->size);
+        if (buf.meta->name == kTapRegulatorName) {
+            for (Idx i = buf.begin(0); i < buf.end(0); ++i) {
+                buf.read_record(i, rec.data());
+                TapRegulatorUpdate u;
+                std::memcpy(&u, rec.data(), sizeof u);
+                Idx const idx = u.id == na_IntID ? i - buf.begin(0) : find_component(m, C_TAP_REGULATOR, u.id);
+                apply_regulator_update(st, idx, u);
+            }
+            continue;
+        }
+        if (buf.meta->name == kVoltageSensorName || buf.meta->name == kPowerSensorName) {
+            bool const vs = buf.meta->name == kVoltageSensorName;
+            for (Idx i = buf.begin(0); i < buf.end(0); ++i) {
+                buf.read_record(i, rec.data());
+                ID id;
+                std::memcpy(&id, rec.data(), 4);
+                Idx const idx =
+                    id == na_IntID ? i - buf.begin(0) : find_component(m, vs ? C_VOLTAGE_SENSOR : C_POWER_SENSOR, id);
+                if (vs) {
+                    SymVoltageSensorUpdate u;
+                    std::memcpy(&u, rec.data(), sizeof u);
+                    apply_voltage_sensor_update(st, idx, u);
+                } else {
+                    SymPowerSensorUpdate u;
+                    std::memcpy(&u, rec.data(), sizeof u);
+                    apply_power_sensor_update(st, idx, u);
+                }
+            }
+            continue;
+        }
+        for (Idx i = buf.begin(0); i < buf.end(0); ++i) {
+            buf.read_record(i, rec.data());
+            ID id;
+            std::memcpy(&id, rec.data(), 4);
+            Idx idx = id == na_IntID ? i - buf.begin(0) : find_component(m, comp, id);
+            int slot = comp;
+            line_slot(m, slot, idx);
+            bool structural = false;
+            apply_update_record(m, st, slot, idx, rec.data(), structural);
+        }
+    }
+    m.base = st;
+    // plans and tuned schedules were made for the previous base state
+    model.plans->clear();
+    model.tuned.clear();
+}
+
+// scenario i of a batch dataset as a single dataset (auxiliary/dataset.hpp get_individual_scenario)
+PGM_ConstDataset scenario_of(PGM_ConstDataset const& d, Idx i) {
+    PGM_ConstDataset r;
+    r.meta = d.meta;
+    for (auto const& b : d.buffers) {
+        ComponentBuffer c;
+        c.meta = b.meta;
+        Idx const b0 = b.begin(i), n = b.end(i) - b0;
+        c.elements_per_scenario = n;
+        c.total_elements = n;
+        if (b.data) c.data = static_cast<char*>(b.data) + size_t(b0) * b.meta->size;
+        for (auto const& [a, ptr] : b.columns) c.columns.emplace_back(a, static_cast<char*>(ptr) + size_t(b0) * a->size);
+        r.buffers.push_back(std::move(c));
+    }
+    return r;
+}
+
+// scenarios [s0, s0 + n) of a batch output dataset (auxiliary/dataset.hpp get_slice_scenario)
+PGM_MutableDataset output_rows(PGM_MutableDataset const& d, Idx s0, Idx n) {
+    PGM_MutableDataset r;
+    r.meta = d.meta;
+    r.is_batch = true;
+    r.batch_size = n;
+    for (auto const& b : d.buffers) {
+        if (b.elements_per_scenario < 0) throw PowerGridError("output buffers of a batch must be uniform\n");
+        ComponentBuffer c;
+        c.meta = b.meta;
+        c.elements_per_scenario = b.elements_per_scenario;
+        c.total_elements = n * b.elements_per_scenario;
+        Idx const b0 = s0 * b.elements_per_scenario;
+        if (b.data) c.data = static_cast<char*>(b.data) + size_t(b0) * b.meta->size;
+        for (auto const& [a, ptr] : b.columns) c.columns.emplace_back(a, static_cast<char*>(ptr) + size_t(b0) * a->size);
+        r.buffers.push_back(std::move(c));
+    }
+    return r;
+}
+
 void record_error(PGM_Handle* h, char const* msg) {
     if (!h) return;
     h->err_code = PGM_regular_error;
@@ -27,6 +123,76 @@ template <class F> void guarded(PGM_Handle* h, F&& f) {
         record_error(h, e.what());
     } catch (...) {
         record_error(h, "Unknown error!\n");
+    }
+}
+
+// one calculation, or one batch dimension (model.cpp:336-359 of the reference C API)
+void calculate_one(PGM_Handle* handle, PGM_PowerGridModel& model, PGM_Options const& opt,
+                   PGM_MutableDataset const& output, PGM_ConstDataset const* batch) {
+    if (opt.calculation_type == PGM_state_estimation) {
+        run_state_estimation(model.model, opt, output, batch, handle);
+        return;
+    }
+    write_null_fault_output(model.model, output);
+    // automatic tap changing (main_model_impl.hpp:318-346, optimizer/tap_position_optimizer.hpp)
+    if (opt.tap_changing_strategy != 0) {
+        calculate_with_tap_changer(model.model, opt, output, batch, handle);
+        return;
+    }
+    write_null_regulator_output(model.model, output);
+    auto b = create_batch(model.model, opt, batch, model_context(model, opt, batch));
+    // only the requested components (and, for columnar buffers, attributes) are produced
+    b->set_output(output);
+    b->run();
+    b->fetch(output, handle);
+}
+
+// batches over the cartesian product of several update datasets (model.cpp:285-334 of the reference C API):
+// every scenario of the first dimension updates a copy of the model permanently, and the remaining dimensions
+// run on that copy as a batch into the output rows of its stride. Failures of a stride count for its own
+// scenarios (MDBatchExceptionHandler, :230-264).
+void calculate_dimensions(PGM_Handle* handle, PGM_PowerGridModel& model, PGM_Options const& opt,
+                          PGM_MutableDataset const& output, PGM_ConstDataset const* batch) {
+    if (!batch || !batch->next) {
+        calculate_one(handle, model, opt, output, batch);
+        return;
+    }
+    Idx stride = 1;
+    for (auto const* d = batch->next; d; d = d->next) stride *= d->batch_size;
+    std::vector<Idx> failed;
+    std::vector<std::string> msgs;
+    std::string last;
+    for (Idx i = 0; i < batch->batch_size; ++i) {
+        PGM_Handle local;
+        PGM_MutableDataset const rows = output_rows(output, i * stride, stride);
+        try {
+            PGM_PowerGridModel copy{model};
+            copy.pool = model.pool;  // the strides run one after another: device chunks pass between them
+            update_model_permanent(copy, scenario_of(*batch, i));
+            calculate_dimensions(&local, copy, opt, rows, batch->next);
+        } catch (std::exception const& e) {
+            local.err_code = PGM_regular_error;
+            local.err_msg = e.what();
+        }
+        if (local.err_code == PGM_batch_error) {
+            for (size_t k = 0; k < local.failed_scenarios.size(); ++k) {
+                failed.push_back(local.failed_scenarios[k] + i * stride);
+                msgs.push_back(local.batch_errs[k]);
+            }
+            last = local.err_msg;
+        } else if (local.err_code != PGM_no_error) {
+            for (Idx k = 0; k < stride; ++k) {
+                failed.push_back(i * stride + k);
+                msgs.push_back(local.err_msg);
+            }
+            last = local.err_msg;
+        }
+    }
+    if (!failed.empty() && handle) {
+        handle->err_code = PGM_batch_error;
+        handle->err_msg = last;
+        handle->failed_scenarios = std::move(failed);
+        handle->batch_errs = std::move(msgs);
     }
 }
 }  // namespace
@@ -52,42 +218,7 @@ PGM_PowerGridModel* PGM_create_model(PGM_Handle* handle, double system_frequency
 }
 
 void PGM_update_model(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_ConstDataset const* update) noexcept {
-    guarded(handle, [&] {
-        if (update->meta->name != "update") throw PowerGridError("update dataset must be of type update\n");
-        Model& m = model->model;
-        ComponentState st = m.base;
-        std::vector<char> rec;
-        for (auto const& buf : update->buffers) {
-            int comp = -1;
-            for (int c = 0; c < N_COMP; ++c)
-                if (buf.meta->name == comp_names[c]) comp = c;
-            rec.resize(buf.meta->size);
-            if (buf.meta->name == kTapRegulatorName) {
-                for (Idx i = buf.begin(0); i < buf.end(0); ++i) {
-                    buf.read_record(i, rec.data());
-                    TapRegulatorUpdate u;
-                    std::memcpy(&u, rec.data(), sizeof u);
-                    Idx const idx = u.id == na_IntID ? i - buf.begin(0) : find_component(m, C_TAP_REGULATOR, u.id);
-                    apply_regulator_update(st, idx, u);
-                }
-                continue;
-            }
-            for (Idx i = buf.begin(0); i < buf.end(0); ++i) {
-                buf.read_record(i, rec.data());
-                ID id;
-                std::memcpy(&id, rec.data(), 4);
-                Idx idx = id == na_IntID ? i - buf.begin(0) : find_component(m, comp, id);
-                int slot = comp;
-                line_slot(m, slot, idx);
-                bool structural = false;
-                apply_update_record(m, st, slot, idx, rec.data(), structural);
-            }
-        }
-        m.base = st;
-        // plans and tuned schedules were made for the previous base state
-        model->plans->clear();
-        model->tuned.clear();
-    });
+    guarded(handle, [&] { update_model_permanent(*model, *update); });
 }
 
 PGM_PowerGridModel* PGM_copy_model(PGM_Handle* handle, PGM_PowerGridModel const* model) noexcept {
@@ -115,23 +246,15 @@ void PGM_calculate(PGM_Handle* handle, PGM_PowerGridModel* model, PGM_Options co
         if (batch_dataset && (!batch_dataset->is_batch || !output_dataset->is_batch))
             throw PowerGridError(
                 "If batch_dataset is provided. Both batch_dataset and output_dataset should be a batch!\n");
-        if (batch_dataset && batch_dataset->batch_size == 0) return;
-        if (opt->calculation_type == PGM_state_estimation) {
-            run_state_estimation(model->model, *opt, *output_dataset, batch_dataset, handle);
-            return;
+        if (batch_dataset) {
+            Idx total = 1;
+            for (auto const* d = batch_dataset; d; d = d->next) total *= d->batch_size;
+            if (total == 0) return;
+            if (batch_dataset->next && output_dataset->batch_size != total)
+                throw PowerGridError("The output dataset must hold every scenario of the cartesian product of the "
+                                     "update datasets\n");
         }
-        write_null_fault_output(model->model, *output_dataset);
-        // automatic tap changing (main_model_impl.hpp:318-346, optimizer/tap_position_optimizer.hpp)
-        if (opt->tap_changing_strategy != 0) {
-            calculate_with_tap_changer(model->model, *opt, *output_dataset, batch_dataset, handle);
-            return;
-        }
-        write_null_regulator_output(model->model, *output_dataset);
-        auto batch = create_batch(model->model, *opt, batch_dataset, model_context(*model, *opt, batch_dataset));
-        // only the requested components (and, for columnar buffers, attributes) are produced
-        batch->set_output(*output_dataset);
-        batch->run();
-        batch->fetch(*output_dataset, handle);
+        calculate_dimensions(handle, *model, *opt, *output_dataset, batch_dataset);
     });
 }
 

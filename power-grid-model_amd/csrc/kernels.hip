@@ -524,10 +524,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(forward_w
     constexpr bool kPerturb = sizeof(FT) == 4 && MODE == MODE_NR && KIND != KIND_UNION;
     __shared__ FT thr_s[kPerturb ? kWave : 1];
     int handed = -1;  // MODE_REFINE: the previous pivot when it handed its L block over (not stored)
-    // raw FP64 systems with pivot perturbation: a pivot of this pair was perturbed in an earlier launch (read once;
-    // launches are ordered) or earlier in this task
+    // the state estimation's gain systems (MODE_RAWP): a pivot of this pair was perturbed in an earlier launch
+    // (read once; launches are ordered) or earlier in this task
     bool raw_pert = false;
-    if constexpr (MODE == MODE_RAW && sizeof(FT) == 8) raw_pert = g.pthr && g.pert && g.pert[size_t(mm) * sp + s] != 0;
+    if constexpr (MODE == MODE_RAWP) raw_pert = g.pthr && g.pert && g.pert[size_t(mm) * sp + s] != 0;
 
     for (int ti = ld(g.task_ptr, task); ti < ld(g.task_ptr, task + 1); ++ti) {
         int const p = ld(g.task_pivot, ti);
@@ -634,7 +634,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(forward_w
 #pragma unroll
         for (int k = 0; k < NB; ++k) rhs[k] = 0.0;
 
-        if constexpr (MODE == MODE_RAW) {
+        if constexpr (MODE == MODE_RAW || MODE == MODE_RAWP) {
 #pragma unroll
             for (int k = 0; k < NB2; ++k) D[k] = g.raw_a[(size_t(dg) * NB2 + k) * sp + s];
 #pragma unroll
@@ -1024,19 +1024,22 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(forward_w
         bool perturbed = false;
         // FP64 raw systems (the state-estimation gain matrix) perturb below a threshold per pair when one is
         // given (sparse_lu_solver.hpp:39-48, 352)
-        constexpr bool kPerturbRaw = MODE == MODE_RAW && sizeof(FT) == 8;
+        constexpr bool kPerturbRaw = (MODE == MODE_RAW || MODE == MODE_RAWP) && sizeof(FT) == 8;
         FT raw_thr = FT(0);
         if constexpr (kPerturbRaw) raw_thr = g.pthr ? FT(g.pthr[size_t(mm) * sp + s]) : FT(0);
-        bool const ok = factor_block<NB, FT>(Df, code, kPerturb ? thr_s[lane] : (kPerturbRaw ? raw_thr : FT(0)),
-                                             perturbed, kPerturbRaw && raw_pert);
+        bool ok;
+        if constexpr (MODE == MODE_RAWP)
+            ok = factor_block<NB, FT>(Df, code, raw_thr, perturbed, raw_pert);
+        else
+            ok = factor_block<NB, FT>(Df, code, kPerturb ? thr_s[lane] : (kPerturbRaw ? raw_thr : FT(0)), perturbed);
         if constexpr (kPerturb) {
             // 2: a pivot was perturbed, the refinement must reach its bound
             if (perturbed) atomicMax(&g.pert[size_t(mm) * sp + s], 2);
         }
         if constexpr (kPerturbRaw) {
             if (perturbed && g.pert) atomicMax(&g.pert[size_t(mm) * sp + s], 1);
-            raw_pert = raw_pert || perturbed;
         }
+        if constexpr (MODE == MODE_RAWP) raw_pert = raw_pert || perturbed;
         if (!ok) {
             if constexpr (MODE != MODE_NRA) {
                 g.done[size_t(mm) * sp + s] = FAILED;
@@ -2589,7 +2592,7 @@ void launch_gain_sweep(DevGroup const* groups, WorkItem const* items, int n_item
             hipLaunchKernelGGL((dev::forward_kernel<2, double, MODE_REFINE, KIND_PLAIN>), grid, dim3(dev::kWave), 0, st, groups,
                                items);
         else
-            hipLaunchKernelGGL((dev::forward_kernel<2, double, MODE_RAW, KIND_PLAIN>), grid, dim3(dev::kWave), 0, st, groups,
+            hipLaunchKernelGGL((dev::forward_kernel<2, double, MODE_RAWP, KIND_PLAIN>), grid, dim3(dev::kWave), 0, st, groups,
                                items);
     } else {
         if (refine)

@@ -21,7 +21,9 @@ constexpr int kWaveLanes = 64;  // lanes of a gfx950 wavefront: one scenario (or
 // MODE_REFINE: a pass of iterative refinement with the stored factors (forward: substitution of the residual
 // held in the forward vectors; backward: back substitution, the correction added to the accumulated step)
 enum SweepMode : int { MODE_INIT = 0, MODE_NR = 1, MODE_RAW = 2, MODE_ICF = 3, MODE_RESID = 4, MODE_NRA = 5,
-                       MODE_REFINE = 6 };
+                       MODE_REFINE = 6, MODE_RAWP = 7 };
+// RAWP: a raw system whose pivot perturbation, once it happened in a pair, holds for the pair's later pivots
+// (the reference's per-solver flag; the state estimation's gain systems)
 // REFINE: the pair's Newton-Raphson step came from a factorisation with perturbed pivots and waits for its
 // iterative refinement (it takes no part in the sweeps meanwhile)
 enum DoneState : int { ACTIVE = 0, CONVERGED = 1, FAILED = 2, REFINE = 3 };

@@ -60,6 +60,7 @@ def _load():
         "PGM_create_dataset_const": (_p, [_p, _cp, Idx, Idx]),
         "PGM_destroy_dataset_const": (None, [_p]),
         "PGM_dataset_const_add_buffer": (None, [_p, _p, _cp, Idx, Idx, _p, _p]),
+        "PGM_dataset_const_set_next_cartesian_product_dimension": (None, [_p, _p, _p]),
         "PGM_dataset_const_add_attribute_buffer": (None, [_p, _p, _cp, _cp, _p]),
         "PGM_create_dataset_mutable": (_p, [_p, _cp, Idx, Idx]),
         "PGM_destroy_dataset_mutable": (None, [_p]),

@@ -147,6 +147,27 @@ def _make_const_dataset(h: Handle, data: dict, dataset_type: str, is_batch: bool
     return ds
 
 
+def _make_update(h: Handle, update_data) -> tuple[_Dataset, int]:
+    """The batch update dataset of a calculation. A list of update datasets is a cartesian product of batches
+    (power_grid_model.py calculate_power_flow update_data list; model.cpp:285-334 of the reference C API): the
+    datasets are chained as dimensions, the first dimension outermost, and the batch size is the product of
+    their sizes. Returns the first dataset (which keeps the others alive) and the total batch size."""
+    if not isinstance(update_data, (list, tuple)):
+        ds = _make_const_dataset(h, update_data, "update", True)
+        return ds, ds.batch_size
+    if not update_data:
+        raise_regular_error("update_data is an empty list")
+    dims = [_make_const_dataset(h, u, "update", True) for u in update_data]
+    for a, b in zip(dims[:-1], dims[1:]):
+        lib.PGM_dataset_const_set_next_cartesian_product_dimension(h.ptr, a.ptr, b.ptr)
+        h.check()
+    dims[0]._keep.append(dims[1:])
+    total = 1
+    for d in dims:
+        total *= d.batch_size
+    return dims[0], total
+
+
 def _make_mutable_dataset(h: Handle, data: dict, dataset_type: str, is_batch: bool, batch_size: int) -> _Dataset:
     ptr = lib.PGM_create_dataset_mutable(h.ptr, dataset_type.encode(), int(is_batch), batch_size)
     h.check()
@@ -322,7 +343,7 @@ class PowerGridModel:
         error_tolerance: float = 1e-8,
         max_iterations: int = 20,
         calculation_method: CalculationMethod | str = CalculationMethod.newton_raphson,
-        update_data: dict | None = None,
+        update_data: dict | list[dict] | None = None,
         threading: int = -1,
         output_component_types=None,
         continue_on_batch_error: bool = False,
@@ -343,8 +364,7 @@ class PowerGridModel:
         upd = None
         batch_size = 1
         if is_batch:
-            upd = _make_const_dataset(h, update_data, "update", True)
-            batch_size = upd.batch_size
+            upd, batch_size = _make_update(h, update_data)
         out_type, output = self._output_arrays(output_component_types, symmetric, is_batch, batch_size)
         out_ds = _make_mutable_dataset(h, output, out_type, is_batch, batch_size)
         lib.PGM_calculate(h.ptr, self._ptr, opt.ptr, out_ds.ptr, None if upd is None else upd.ptr)
@@ -376,8 +396,7 @@ class PowerGridModel:
         upd = None
         batch_size = 1
         if is_batch:
-            upd = _make_const_dataset(h, update_data, "update", True)
-            batch_size = upd.batch_size
+            upd, batch_size = _make_update(h, update_data)
         out_type, output = self._output_arrays(output_component_types, symmetric, is_batch, batch_size)
         out_ds = _make_mutable_dataset(h, output, out_type, is_batch, batch_size)
         lib.PGM_calculate(h.ptr, self._ptr, opt.ptr, out_ds.ptr, None if upd is None else upd.ptr)

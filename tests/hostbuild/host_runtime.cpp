@@ -1,0 +1,354 @@
+// Host runtime of the host build (tests/hostbuild): the HIP API subset the library uses, with kernel launches run
+// by host threads, synchronous streams, graphs replayed in capture order, and optional checks that device and
+// host memory are not confused (EMU_MMAP=1).
+#include "hip/hip_runtime.h"
+
+#include <barrier>
+#include <execinfo.h>
+#include <map>
+#include <sys/mman.h>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+thread_local dim3 threadIdx, blockIdx, blockDim, gridDim;
+
+struct EmuStream { int id; };
+struct EmuEvent { int id; };
+struct EmuGraph { std::vector<std::function<void()>> nodes; };
+
+namespace {
+int fill_byte() {
+    static int v = [] {
+        char const* e = std::getenv("EMU_FILL");
+        return e ? int(std::strtol(e, nullptr, 0)) : 0xA5;
+    }();
+    return v;
+}
+std::mutex cap_mu;
+EmuGraph* capturing = nullptr;
+std::thread::id capture_thread;
+bool record(std::function<void()> f) {
+    std::lock_guard<std::mutex> l(cap_mu);
+    if (!capturing || std::this_thread::get_id() != capture_thread) return false;
+    capturing->nodes.push_back(std::move(f));
+    return true;
+}
+
+// a team of host threads that runs one block at a time
+struct Team {
+    std::vector<std::thread> th;
+    std::mutex mu;
+    std::condition_variable cv, done_cv;
+    unsigned long long gen = 0;
+    int active = 0, remaining = 0;
+    std::function<void()> const* fn = nullptr;
+    dim3 bidx, bdim, gdim;
+    std::barrier<>* bar = nullptr;
+    bool stop = false;
+    void worker(int i) {
+        unsigned long long seen = 0;
+        while (true) {
+            std::unique_lock<std::mutex> l(mu);
+            cv.wait(l, [&] { return stop || gen != seen; });
+            if (stop) return;
+            seen = gen;
+            if (i >= active) continue;
+            l.unlock();
+            unsigned const nx = bdim.x, ny = bdim.y;
+            threadIdx = dim3(unsigned(i) % nx, (unsigned(i) / nx) % ny, unsigned(i) / (nx * ny));
+            blockIdx = bidx;
+            blockDim = bdim;
+            gridDim = gdim;
+            (*fn)();
+            bar->arrive_and_drop();
+            l.lock();
+            if (--remaining == 0) done_cv.notify_all();
+        }
+    }
+    void ensure(int n) {
+        while (int(th.size()) < n) {
+            int const i = int(th.size());
+            th.emplace_back([this, i] { worker(i); });
+        }
+    }
+    ~Team() {
+        {
+            std::lock_guard<std::mutex> l(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        for (auto& t : th) t.join();
+    }
+};
+Team& team() {
+    static Team t;
+    return t;
+}
+std::barrier<>* cur_bar = nullptr;
+std::mutex launch_mu;
+
+// __shfl_xor exchange buffer of the running block
+std::vector<int> shfl_buf;
+
+void run_grid(dim3 grid, dim3 block, std::function<void()> const& body) {
+    std::lock_guard<std::mutex> lg(launch_mu);
+    int const nt = int(block.x * block.y * block.z);
+    Team& T = team();
+    T.ensure(nt);
+    shfl_buf.assign(size_t(nt), 0);
+    for (unsigned bz = 0; bz < grid.z; ++bz)
+        for (unsigned by = 0; by < grid.y; ++by)
+            for (unsigned bx = 0; bx < grid.x; ++bx) {
+                std::barrier<> bar(nt);
+                cur_bar = &bar;
+                std::unique_lock<std::mutex> l(T.mu);
+                T.fn = &body;
+                T.bidx = dim3(bx, by, bz);
+                T.bdim = block;
+                T.gdim = grid;
+                T.bar = &bar;
+                T.active = nt;
+                T.remaining = nt;
+                ++T.gen;
+                T.cv.notify_all();
+                T.done_cv.wait(l, [&] { return T.remaining == 0; });
+            }
+}
+}  // namespace
+
+void emu_bad_pointer(char const* what, void const* p) {
+    std::fprintf(stderr, "[emu] %s %p is not device memory\n", what, p);
+    void* bt[32];
+    int const n = backtrace(bt, 32);
+    backtrace_symbols_fd(bt, n, 2);
+    std::abort();
+}
+void __syncthreads() { cur_bar->arrive_and_wait(); }
+int __shfl_xor(int v, int mask) {
+    int const i = int(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
+    shfl_buf[size_t(i)] = v;
+    cur_bar->arrive_and_wait();
+    int const j = (i & ~63) | ((i & 63) ^ mask);
+    int const r = shfl_buf[size_t(j)];
+    cur_bar->arrive_and_wait();
+    return r;
+}
+
+void emu_launch(dim3 grid, dim3 block, hipStream_t, std::function<void()> body) {
+    if (grid.x == 0 || grid.y == 0 || grid.z == 0) {
+        std::fprintf(stderr, "[emu] launch with an empty grid\n");
+        std::abort();
+    }
+    if (block.x * block.y * block.z > 1024) {
+        std::fprintf(stderr, "[emu] block too large\n");
+        std::abort();
+    }
+    auto f = [grid, block, body = std::move(body)] { run_grid(grid, block, body); };
+    if (record(f)) return;
+    f();
+}
+
+// EMU_MMAP=1: device allocations from a separate address region (page-padded, guard page after), so that a
+// host pointer used as a device pointer is recognised
+namespace {
+std::mutex alloc_mu;
+std::map<uintptr_t, size_t> dev_allocs;  // start -> size (device allocations)
+std::map<uintptr_t, size_t> host_allocs;  // pinned host allocations
+bool use_mmap() {
+    static bool v = std::getenv("EMU_MMAP") != nullptr;
+    return v;
+}
+uintptr_t next_region = 0x600000000000ull;
+}  // namespace
+bool emu_is_device(void const* p) {
+    uintptr_t const a = reinterpret_cast<uintptr_t>(p);
+    std::lock_guard<std::mutex> l(alloc_mu);
+    auto it = dev_allocs.upper_bound(a);
+    if (it == dev_allocs.begin()) return false;
+    --it;
+    return a >= it->first && a <= it->first + it->second;
+}
+static void* dev_alloc_raw(size_t n) {
+    if (!use_mmap()) return std::malloc(n ? n : 1);
+    size_t const pg = 4096, len = (n + pg - 1) / pg * pg + pg;
+    std::lock_guard<std::mutex> l(alloc_mu);
+    void* m = mmap(reinterpret_cast<void*>(next_region), len, PROT_READ | PROT_WRITE,
+                   MAP_PRIVATE | MAP_ANONYMOUS | MAP_FIXED_NOREPLACE, -1, 0);
+    if (m == MAP_FAILED) return nullptr;
+    next_region += len + pg;
+    mprotect(static_cast<char*>(m) + len - pg, pg, PROT_NONE);
+    return m;
+}
+bool emu_use_checks() { return use_mmap(); }
+hipError_t hipMalloc(void** p, size_t n) {
+    void* m = dev_alloc_raw(n);
+    if (!m) return hipErrorOutOfMemory;
+    std::memset(m, fill_byte(), n);
+    {
+        std::lock_guard<std::mutex> l(alloc_mu);
+        dev_allocs[reinterpret_cast<uintptr_t>(m)] = n;
+    }
+    *p = m;
+    return hipSuccess;
+}
+hipError_t hipFree(void* p) {
+    if (!p) return hipSuccess;
+    size_t n = 0;
+    {
+        std::lock_guard<std::mutex> l(alloc_mu);
+        auto it = dev_allocs.find(reinterpret_cast<uintptr_t>(p));
+        if (it == dev_allocs.end()) {
+            std::fprintf(stderr, "[emu] hipFree of an unknown pointer %p\n", p);
+            std::abort();
+        }
+        n = it->second;
+        dev_allocs.erase(it);
+    }
+    if (use_mmap()) {
+        size_t const pg = 4096, len = (n + pg - 1) / pg * pg + pg;
+        munmap(p, len);  // later use faults
+    } else {
+        std::free(p);
+    }
+    return hipSuccess;
+}
+hipError_t hipHostMalloc(void** p, size_t n, unsigned) {
+    *p = std::malloc(n ? n : 1);
+    std::lock_guard<std::mutex> l(alloc_mu);
+    host_allocs[reinterpret_cast<uintptr_t>(*p)] = n;
+    return hipSuccess;
+}
+hipError_t hipHostFree(void* p) {
+    {
+        std::lock_guard<std::mutex> l(alloc_mu);
+        host_allocs.erase(reinterpret_cast<uintptr_t>(p));
+    }
+    std::free(p);
+    return hipSuccess;
+}
+hipError_t hipMemcpy(void* d, void const* s, size_t n, hipMemcpyKind k) {
+    if (use_mmap() && n) {
+        bool const dd = emu_is_device(d), sd = emu_is_device(s);
+        bool const ok = (k == hipMemcpyHostToDevice && dd && !sd) || (k == hipMemcpyDeviceToHost && !dd && sd) ||
+                        (k == hipMemcpyDeviceToDevice && dd && sd) || k == hipMemcpyDefault ||
+                        (k == hipMemcpyHostToHost && !dd && !sd);
+        if (!ok) {
+            std::fprintf(stderr, "[emu] hipMemcpy kind %d with dst %s src %s\n", int(k), dd ? "device" : "host",
+                         sd ? "device" : "host");
+            std::abort();
+        }
+    }
+    if (n) std::memmove(d, s, n);
+    return hipSuccess;
+}
+hipError_t hipMemcpyAsync(void* d, void const* s, size_t n, hipMemcpyKind k, hipStream_t) {
+    if (record([=] { hipMemcpy(d, s, n, k); })) return hipSuccess;
+    return hipMemcpy(d, s, n, k);
+}
+hipError_t hipMemcpy2D(void* d, size_t dp, void const* s, size_t sp, size_t w, size_t h, hipMemcpyKind) {
+    for (size_t r = 0; r < h; ++r) std::memmove(static_cast<char*>(d) + r * dp, static_cast<char const*>(s) + r * sp, w);
+    return hipSuccess;
+}
+hipError_t hipMemset(void* d, int v, size_t n) {
+    if (use_mmap() && n && !emu_is_device(d)) {
+        std::fprintf(stderr, "[emu] hipMemset of host memory\n");
+        std::abort();
+    }
+    if (n) std::memset(d, v, n);
+    return hipSuccess;
+}
+hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) {
+    if (record([=] { hipMemset(d, v, n); })) {
+        std::fprintf(stderr, "[emu] memset recorded into a graph\n");
+        return hipSuccess;
+    }
+    return hipMemset(d, v, n);
+}
+hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
+    *s = new EmuStream{1};
+    return hipSuccess;
+}
+hipError_t hipStreamCreate(hipStream_t* s) { return hipStreamCreateWithFlags(s, 0); }
+hipError_t hipStreamDestroy(hipStream_t s) {
+    delete s;
+    return hipSuccess;
+}
+hipError_t hipStreamSynchronize(hipStream_t) {
+    std::lock_guard<std::mutex> l(cap_mu);
+    if (capturing && std::this_thread::get_id() == capture_thread) {
+        std::fprintf(stderr, "[emu] synchronize during capture\n");
+        return hipErrorUnknown;
+    }
+    return hipSuccess;
+}
+hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
+hipError_t hipEventCreate(hipEvent_t* e) {
+    *e = new EmuEvent{1};
+    return hipSuccess;
+}
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
+hipError_t hipEventDestroy(hipEvent_t e) {
+    delete e;
+    return hipSuccess;
+}
+hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventRecordWithFlags(hipEvent_t, hipStream_t, unsigned) { return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
+    *ms = 0.001f;
+    return hipSuccess;
+}
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+    std::lock_guard<std::mutex> l(cap_mu);
+    if (capturing) return hipErrorUnknown;
+    capturing = new EmuGraph;
+    capture_thread = std::this_thread::get_id();
+    return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+    std::lock_guard<std::mutex> l(cap_mu);
+    *g = capturing;
+    capturing = nullptr;
+    return *g ? hipSuccess : hipErrorUnknown;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, hipGraphNode_t*, char*, size_t) {
+    *e = new EmuGraph(*g);
+    return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+    for (auto const& f : e->nodes) f();
+    return hipSuccess;
+}
+hipError_t hipGraphDestroy(hipGraph_t g) {
+    delete g;
+    return hipSuccess;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
+    delete e;
+    return hipSuccess;
+}
+hipError_t hipGetLastError() { return hipSuccess; }
+hipError_t hipPeekAtLastError() { return hipSuccess; }
+const char* hipGetErrorString(hipError_t) { return "emulated error"; }
+hipError_t hipDeviceSynchronize() { return hipSuccess; }
+hipError_t hipGetDeviceCount(int* n) {
+    *n = 1;
+    return hipSuccess;
+}
+hipError_t hipGetDevice(int* d) {
+    *d = 0;
+    return hipSuccess;
+}
+hipError_t hipSetDevice(int) { return hipSuccess; }
+hipError_t hipGetSymbolAddress(void** p, const void* sym) {
+    *p = const_cast<void*>(sym);
+    {
+        std::lock_guard<std::mutex> l(alloc_mu);
+        dev_allocs[reinterpret_cast<uintptr_t>(sym)] = size_t(64) << 20;  // the device symbol's region
+    }
+    return hipSuccess;
+}
