@@ -176,8 +176,10 @@ def _make_mutable_dataset(h: Handle, data: dict, dataset_type: str, is_batch: bo
     for comp, arr in data.items():
         keep.append(arr)
         if isinstance(arr, dict):
-            # columnar: one buffer per requested attribute (dataset.h add_attribute_buffer)
-            n = next(iter(arr.values())).shape[-1] if arr else 0
+            # columnar: one buffer per requested attribute (dataset.h add_attribute_buffer); the element count is
+            # the axis after the batch axis (asymmetric attributes carry a trailing phase axis)
+            first = next(iter(arr.values())) if arr else None
+            n = 0 if first is None else (first.shape[1] if is_batch else first.shape[0])
             lib.PGM_dataset_mutable_add_buffer(h.ptr, ptr, comp.encode(), n, n * batch_size, None, None)
             h.check()
             for attr, col in arr.items():
@@ -185,7 +187,7 @@ def _make_mutable_dataset(h: Handle, data: dict, dataset_type: str, is_batch: bo
                                                              col.ctypes.data)
                 h.check()
             continue
-        n = arr.shape[-1]
+        n = arr.shape[1] if is_batch else arr.shape[0]
         lib.PGM_dataset_mutable_add_buffer(h.ptr, ptr, comp.encode(), n, n * batch_size, None, arr.ctypes.data)
         h.check()
     return ds

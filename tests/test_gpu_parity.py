@@ -865,6 +865,22 @@ def test_columnar_attribute_filtered_output():
     assert everything["line"]["i_from"].tobytes() == rows["line"]["i_from"].tobytes()
 
 
+@pytest.mark.parametrize("batch", [True, False])
+def test_columnar_output_asymmetric(batch):
+    """Columnar output of an asymmetric calculation: every attribute carries a phase axis, (scenarios, n, 3) for
+    a batch and (n, 3) for one calculation; its values equal the row-based output byte for byte."""
+    case = synthetic.asym_500(n_scenarios=6)
+    model = PowerGridModel(case.input)
+    upd = case.update if batch else None
+    rows = model.calculate_power_flow(update_data=upd, symmetric=False)
+    sel = {"node": ["u_pu", "u_angle", "p"], "line": ["i_from", "q_to"], "asym_load": ["p", "q"]}
+    cols = model.calculate_power_flow(update_data=upd, symmetric=False, output_component_types=sel)
+    for comp, attrs in sel.items():
+        for a in attrs:
+            assert cols[comp][a].shape == rows[comp][a].shape, (comp, a)
+            assert cols[comp][a].tobytes() == rows[comp][a].tobytes(), (comp, a)
+
+
 def test_device_output_selection():
     """set_output: only the selected components run, columnar ones write only their attributes on the
     device; the values equal the whole-record output byte for byte. Fetching a component that was not
