@@ -293,6 +293,60 @@ def measure_workload(name, steps, warmup, threads, cpu_seconds, do_cpu):
     return res
 
 
+def state_estimation_leg(args, threads):
+    """Newton-Raphson state estimation on meshed_1000 (1024 scenarios): sensors placed on the grid, measurements
+    from the power flow of each scenario with 0.1% noise; the whole call from host buffers, and the oracle's
+    state estimation on a sample of the same scenarios."""
+    from power_grid_model_amd import PowerGridModel, synthetic
+
+    se_n = 1024
+    pcase = synthetic.meshed_1000(n_scenarios=se_n)
+    ptrue = PowerGridModel(pcase.input).calculate_power_flow(
+        update_data=pcase.update, output_component_types=["node", "line", "source", "sym_load"])
+    scase = synthetic.state_estimation_case(pcase, ptrue)
+    smodel = PowerGridModel(scase.input)
+    sel = {"node": ["u_pu", "u_angle"], "line": ["p_from", "q_from"]}
+    st_times = []
+    for k in range(args.steps + 1):
+        t0 = time.perf_counter()
+        sout = smodel.calculate_state_estimation(update_data=scase.update, calculation_method="newton_raphson",
+                                                 output_component_types=sel)
+        if k > 0:
+            st_times.append(time.perf_counter() - t0)
+    t_se = float(np.median(st_times))
+    err_u = float(np.max(np.abs(sout["node"]["u_pu"] - ptrue["node"]["u_pu"])))
+    state_est = {"workload": scase.name, "scenarios": se_n, "unit": "scenarios/s", "value": se_n / t_se,
+                 "seconds": t_se, "step": "calculate_state_estimation from host buffers (newton_raphson, sym)",
+                 "sensors": {"voltage": int(len(scase.input["sym_voltage_sensor"])),
+                             "power": int(len(scase.input["sym_power_sensor"]))},
+                 "max_u_pu_deviation_from_power_flow": err_u}
+    log(f"[bench] state estimation {scase.name}: {['%.3f s' % t for t in st_times]}, max |u_pu - pf| {err_u:.2e}")
+    if not args.no_cpu:
+        # the oracle's state estimation (block sparse gain system, pivot perturbation and refinement as the
+        # reference) on the first scenarios, sized to a few seconds; its voltages check the GPU's
+        import oracle_lib as ol
+
+        def oracle_se(n):
+            upd = {c: v[:n] for c, v in scase.update.items()}
+            t0 = time.perf_counter()
+            ref, _, st_ref, _ = ol.state_estimation(scase.input, upd, n, threading=threads)
+            return ref, st_ref, time.perf_counter() - t0
+
+        _, _, secs = oracle_se(16)
+        n_cpu = int(min(se_n, max(16, 16 / secs * args.cpu_seconds / 2)))
+        ref, st_ref, secs = oracle_se(n_cpu)
+        ok = st_ref == 0
+        err_o = float(np.max(np.abs(sout["node"]["u_pu"][:n_cpu][ok] - ref["node"]["u_pu"][ok])))
+        state_est["cpu_baseline"] = {"value": n_cpu / secs, "unit": "scenarios/s", "cores": threads, "kind": "port",
+                                     "sample": f"first {n_cpu} scenarios of {scase.name} (oracle state "
+                                               f"estimation, {threads} threads, {secs:.1f} s)"}
+        state_est["max_u_pu_deviation_from_oracle"] = err_o
+        log(f"[bench] state estimation cpu baseline {n_cpu / secs:.1f} scen/s on {threads} threads; "
+            f"max |u_pu - oracle| {err_o:.2e} over {int(ok.sum())} scenarios")
+
+    return state_est
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -305,6 +359,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-se", action="store_true", help="skip the state-estimation leg")
+    ap.add_argument("--se-only", action="store_true", help="run only the state-estimation leg and print its JSON")
     ap.add_argument("--no-tap", action="store_true", help="skip the automatic tap changer leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the other workloads")
     ap.add_argument("--no-pcie", action="store_true", help="skip the PCIe-inclusive calls")
@@ -331,6 +386,9 @@ def main():
         log("[bench] copy probe: copy_probe_kernel read and wrote 67108864 bytes")
 
     threads = len(os.sched_getaffinity(0))
+    if args.se_only:
+        print(json.dumps(state_estimation_leg(args, threads)))
+        return
     make = synthetic.CONFIGS[args.workload]
     t0 = time.perf_counter()
     case = make(n_scenarios=args.scenarios) if args.scenarios else make()
@@ -409,54 +467,7 @@ def main():
             log(f"[bench] calculate_power_flow from host buffers ({prec}): "
                 f"{['%.3f s' % t for t in tp]}")
 
-    # Newton-Raphson state estimation on meshed_1000 (1024 scenarios): sensors placed on the grid, measurements
-    # from the power flow of each scenario with 0.1% noise; the whole call from host buffers
-    state_est = None
-    if not args.no_se:
-        se_n = 1024
-        pcase = synthetic.meshed_1000(n_scenarios=se_n)
-        ptrue = PowerGridModel(pcase.input).calculate_power_flow(
-            update_data=pcase.update, output_component_types=["node", "line", "source", "sym_load"])
-        scase = synthetic.state_estimation_case(pcase, ptrue)
-        smodel = PowerGridModel(scase.input)
-        sel = {"node": ["u_pu", "u_angle"], "line": ["p_from", "q_from"]}
-        st_times = []
-        for k in range(args.steps + 1):
-            t0 = time.perf_counter()
-            sout = smodel.calculate_state_estimation(update_data=scase.update, calculation_method="newton_raphson",
-                                                     output_component_types=sel)
-            if k > 0:
-                st_times.append(time.perf_counter() - t0)
-        t_se = float(np.median(st_times))
-        err_u = float(np.max(np.abs(sout["node"]["u_pu"] - ptrue["node"]["u_pu"])))
-        state_est = {"workload": scase.name, "scenarios": se_n, "unit": "scenarios/s", "value": se_n / t_se,
-                     "seconds": t_se, "step": "calculate_state_estimation from host buffers (newton_raphson, sym)",
-                     "sensors": {"voltage": int(len(scase.input["sym_voltage_sensor"])),
-                                 "power": int(len(scase.input["sym_power_sensor"]))},
-                     "max_u_pu_deviation_from_power_flow": err_u}
-        log(f"[bench] state estimation {scase.name}: {['%.3f s' % t for t in st_times]}, max |u_pu - pf| {err_u:.2e}")
-        if not args.no_cpu:
-            # the oracle's state estimation (block sparse gain system, pivot perturbation and refinement as the
-            # reference) on the first scenarios, sized to a few seconds; its voltages check the GPU's
-            import oracle_lib as ol
-
-            def oracle_se(n):
-                upd = {c: v[:n] for c, v in scase.update.items()}
-                t0 = time.perf_counter()
-                ref, _, st_ref, _ = ol.state_estimation(scase.input, upd, n, threading=threads)
-                return ref, st_ref, time.perf_counter() - t0
-
-            _, _, secs = oracle_se(16)
-            n_cpu = int(min(se_n, max(16, 16 / secs * args.cpu_seconds / 2)))
-            ref, st_ref, secs = oracle_se(n_cpu)
-            ok = st_ref == 0
-            err_o = float(np.max(np.abs(sout["node"]["u_pu"][:n_cpu][ok] - ref["node"]["u_pu"][ok])))
-            state_est["cpu_baseline"] = {"value": n_cpu / secs, "unit": "scenarios/s", "cores": threads, "kind": "port",
-                                         "sample": f"first {n_cpu} scenarios of {scase.name} (oracle state "
-                                                   f"estimation, {threads} threads, {secs:.1f} s)"}
-            state_est["max_u_pu_deviation_from_oracle"] = err_o
-            log(f"[bench] state estimation cpu baseline {n_cpu / secs:.1f} scen/s on {threads} threads; "
-                f"max |u_pu - oracle| {err_o:.2e} over {int(ok.sum())} scenarios")
+    state_est = None if args.no_se else state_estimation_leg(args, threads)
 
     # adjoint gradients on the uneven-feeder batch (64 feeders, 128 scenarios): the solve, then one
     # adjoint pass (re-factorisation in FP64 at the converged state and the transposed sweeps)

@@ -130,7 +130,7 @@ template <class F> void guarded(PGM_Handle* h, F&& f) {
 void calculate_one(PGM_Handle* handle, PGM_PowerGridModel& model, PGM_Options const& opt,
                    PGM_MutableDataset const& output, PGM_ConstDataset const* batch) {
     if (opt.calculation_type == PGM_state_estimation) {
-        run_state_estimation(model.model, opt, output, batch, handle);
+        run_state_estimation(model.model, opt, output, batch, handle, model.pool);
         return;
     }
     write_null_fault_output(model.model, output);

@@ -24,6 +24,7 @@
 #include "topology.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -37,6 +38,34 @@
 #include <vector>
 
 namespace pgm_amd {
+
+namespace {
+// target number of parallel tasks per level of a group's schedule (PGM_AMD_SE_TASKS overrides). The sweeps of
+// the 4x4 gain blocks are latency-bound on their pivot chains: se_meshed_1000 (1024 scenarios) takes the same
+// 48 ms of iterations at every target from 64 to 4096
+int se_tasks(int blocks) {
+    (void)blocks;
+    static int const t = [] {
+        char const* v = std::getenv("PGM_AMD_SE_TASKS");
+        int const x = v ? std::atoi(v) : 0;
+        return x > 0 ? x : 256;
+    }();
+    return t;
+}
+
+// phase timings of a state-estimation call on stderr when PGM_AMD_TRACE is set (diagnostics only)
+struct SeTrace {
+    bool on = std::getenv("PGM_AMD_TRACE") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void mark(char const* what) {
+        if (!on) return;
+        auto const now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[pgm_amd se] %-28s %8.2f ms\n", what,
+                     std::chrono::duration<double, std::milli>(now - t).count());
+        t = now;
+    }
+};
+}  // namespace
 
 #define SE_HIP_CHECK(expr)                                                                                       \
     do {                                                                                                         \
@@ -113,19 +142,35 @@ VoltMeas combine_voltage(std::vector<VoltMeas> const& xs) {
     return r;
 }
 
-// statistics::combine over decomposed complex variables (statistics.hpp:348-366)
-PowMeas combine_power(std::vector<PowMeas> const& xs) {
-    std::vector<std::pair<double, double>> re, im;
+// statistics::combine over decomposed complex variables (statistics.hpp:348-366); the real and imaginary parts
+// combine as combine_real does, in one pass without temporaries
+template <class Range> PowMeas combine_power(Range const& xs) {
+    double accp = 0.0, wp = 0.0, accq = 0.0, wq = 0.0;
     for (auto const& x : xs) {
-        re.push_back({x.p, x.vp});
-        im.push_back({x.q, x.vq});
+        accp += 1.0 / x.vp;
+        wp += x.p / x.vp;
+        accq += 1.0 / x.vq;
+        wq += x.q / x.vq;
     }
     PowMeas r;
-    combine_real(r.p, r.vp, re);
-    combine_real(r.q, r.vq, im);
+    if (std::isnormal(accp)) {
+        r.p = wp / accp;
+        r.vp = 1.0 / accp;
+    } else {
+        r.p = wp;
+        r.vp = kInf;
+    }
+    if (std::isnormal(accq)) {
+        r.q = wq / accq;
+        r.vq = 1.0 / accq;
+    } else {
+        r.q = wq;
+        r.vq = kInf;
+    }
     if (!(std::isnormal(r.vp) && std::isnormal(r.vq))) r.vp = r.vq = kInf;
     return r;
 }
+PowMeas combine_power(std::initializer_list<PowMeas> xs) { return combine_power<std::initializer_list<PowMeas>>(xs); }
 
 // sensor calculation parameters in per unit (voltage_sensor.hpp:126-135, power_sensor.hpp:110-152)
 VoltMeas voltage_param(Model const& m, ComponentState const& st, Idx i) {
@@ -296,6 +341,9 @@ void measure_lane(Model const& m, SeGroup const& g, ComponentState const& st, in
     int const n = int(p.n);
     int const nbr = int(p.branch_comp.size());
     auto& la = h.appl[size_t(lane)];
+    // per-thread scratch for the sensors of one object (no allocation per object)
+    static thread_local std::vector<PowMeas> obj_xs, bus_xs;
+    static thread_local std::vector<VoltMeas> volt_xs;
     la.lg_state.assign(p.load_comp.size(), 0);
     la.src_state.assign(p.src_comp.size(), 0);
     la.lg_meas.assign(p.load_comp.size(), PowMeas{});
@@ -317,9 +365,9 @@ void measure_lane(Model const& m, SeGroup const& g, ComponentState const& st, in
             for (int b = b0; b < b1; ++b) {
                 h.v_flag[lane_of(size_t(b))] = 0;
                 if (vs_at_bus[size_t(b)].empty()) continue;
-                std::vector<VoltMeas> xs;
-                for (int i : vs_at_bus[size_t(b)]) xs.push_back(voltage_param(m, st, i));
-                VoltMeas const a = combine_voltage(xs);
+                volt_xs.clear();
+                for (int i : vs_at_bus[size_t(b)]) volt_xs.push_back(voltage_param(m, st, i));
+                VoltMeas const a = combine_voltage(volt_xs);
                 if (std::isinf(a.var)) continue;
                 vm[size_t(b - b0)] = a;
                 ++n_volt;
@@ -344,9 +392,9 @@ void measure_lane(Model const& m, SeGroup const& g, ComponentState const& st, in
                 if (!connected) return 0;
                 auto const& sensors = ps_at[size_t(kind)][size_t(obj)];
                 if (sensors.empty()) return 1;
-                std::vector<PowMeas> xs;
-                for (int i : sensors) xs.push_back(power_param(m, st, i));
-                PowMeas const c = combine_power(xs);
+                obj_xs.clear();
+                for (int i : sensors) obj_xs.push_back(power_param(m, st, i));
+                PowMeas const c = combine_power(obj_xs);
                 if (std::isinf(c.vp) || std::isinf(c.vq)) return 1;
                 out = c;
                 return 2;
@@ -379,9 +427,9 @@ void measure_lane(Model const& m, SeGroup const& g, ComponentState const& st, in
                     add(la.src_state[size_t(q)], la.src_meas[size_t(q)]);
                 la.bus_appl[size_t(b)] = appl;
                 la.bus_unmeasured[size_t(b)] = n_unmeasured;
-                std::vector<PowMeas> direct_xs;
-                for (int i : ps_at[5][size_t(b)]) direct_xs.push_back(power_param(m, st, i));
-                PowMeas const direct = combine_power(direct_xs);
+                bus_xs.clear();
+                for (int i : ps_at[5][size_t(b)]) bus_xs.push_back(power_param(m, st, i));
+                PowMeas const direct = combine_power(bus_xs);
                 bool const uncertain = std::isinf(direct.vp) || std::isinf(direct.vq);
                 int const lb = b - b0;
                 if (n_unmeasured > 0) {
@@ -555,13 +603,14 @@ using cdd = std::complex<double>;
 }  // namespace
 
 void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDataset const& output,
-                          PGM_ConstDataset const* update, PGM_Handle* handle) {
+                          PGM_ConstDataset const* update, PGM_Handle* handle, std::shared_ptr<ChunkPool> pool) {
     if (!opt.symmetric)
         throw PowerGridError("asymmetric state estimation is not supported by this build: use symmetric\n");
     if (opt.calculation_method != PGM_newton_raphson)
         throw PowerGridError("The calculation method is not supported by this build for state estimation: only "
                              "newton_raphson is implemented\n");
     ensure_device();
+    SeTrace tr;
     Idx const S = update ? update->batch_size : 1;
     ComponentState base = m.base;
     // ---- scenarios grouped by topology ----
@@ -602,6 +651,7 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
             status[size_t(s)] = 1;
             messages[size_t(s)] = scn_error[size_t(s)];
         }
+    tr.mark("grouping");
     hipStream_t st = nullptr;
     SE_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     struct StreamGuard {
@@ -615,7 +665,7 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
         g.topo = build_topology(m, g.st);
         std::vector<Idx> all(g.topo.models.size());
         std::iota(all.begin(), all.end(), Idx{0});
-        g.plan = build_plan(m, g.st, g.topo, all, true, 256);
+        g.plan = build_plan(m, g.st, g.topo, all, true, se_tasks((int(g.scenarios.size()) + 63) / 64));
         Plan const& p = g.plan;
         g.S = int(g.scenarios.size());
         g.S_pad = (g.S + 63) / 64 * 64;
@@ -697,6 +747,7 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
         }
         std::vector<int> const& el_ptr = g.el_ptr;
         std::vector<int> const& el = g.el;
+        tr.mark("structure");
         // ---- measured values of every lane ----
         HostLanes h;
         h.sp = sp;
@@ -723,7 +774,11 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
             auto work = [&](unsigned t) {
                 try {
                     ScenarioReader rd{m, update, {}};
-                    for (int lane = int(t); lane < g.S; lane += int(nth)) {
+                    // contiguous runs of 64 lanes per thread: the host arrays interleave the lanes, and threads
+                    // writing neighbouring lanes would share cache lines
+                    int const runs = (g.S + 63) / 64;
+                    for (int r = int(t); r < runs; r += int(nth))
+                    for (int lane = r * 64; lane < std::min(g.S, r * 64 + 64); ++lane) {
                         ComponentState stl = base;
                         rd.apply(g.scenarios[size_t(lane)], stl);
                         measure_lane(m, g, stl, lane, h, vs_at_bus, ps_at);
@@ -739,6 +794,7 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
             for (auto const& e : errs)
                 if (!e.empty()) throw PowerGridError(e);
         }
+        tr.mark("measured values");
         bool const trace = std::getenv("PGM_AMD_TRACE") != nullptr;
         if (trace)
             for (int k = 0; k < nmm; ++k)
@@ -750,6 +806,7 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
                 }
         // ---- device ----
         Arena A;
+        A.set_pool(pool);
         SeDev d;
         d.n = n;
         d.n_mm = nmm;
@@ -880,6 +937,7 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
         launch_se_init(d, d_shift, st);
         std::vector<int> hpert(npairs), hdone(npairs);
         std::vector<unsigned long long> hberr(npairs);
+        tr.mark("device setup");
         for (Idx it = 1; it <= max_iter; ++it) {
             launch_se_pair_reset(d, st);
             launch_se_gain(d, st);
@@ -964,6 +1022,7 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
             SE_HIP_CHECK(hipGetLastError());
             if (act[0] == 0) break;
         }
+        tr.mark("iterations");
         // ---- per-lane status and voltages ----
         std::vector<int> herr(npairs), hits(npairs);
         std::vector<double> hldev(npairs), hu(size_t(n) * 2 * sp);
@@ -999,6 +1058,7 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
                 lane_ok[size_t(lane)] = 1;
             }
         }
+        tr.mark("status");
         // ---- results of every converged lane, written into the output dataset ----
         unsigned const nth = unsigned(std::max<size_t>(1, std::min<size_t>({size_t(16), size_t(std::thread::hardware_concurrency()),
                                                                              size_t(g.S)})));
@@ -1218,7 +1278,9 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
         auto out_work = [&](unsigned t) {
             try {
                 ScenarioReader rd{m, update, {}};
-                for (int lane = int(t); lane < g.S; lane += int(nth)) {
+                int const runs = (g.S + 63) / 64;
+                for (int r = int(t); r < runs; r += int(nth))
+                for (int lane = r * 64; lane < std::min(g.S, r * 64 + 64); ++lane) {
                     if (!lane_ok[size_t(lane)]) continue;
                     ComponentState stl = base;
                     rd.apply(g.scenarios[size_t(lane)], stl);
@@ -1235,6 +1297,7 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
         for (auto const& e : errs)
             if (!e.empty()) throw PowerGridError(e);
     }
+    tr.mark("results");
     // ---- errors (job_dispatch.hpp:208-224) ----
     if (handle) {
         std::vector<Idx> failed;
