@@ -15,6 +15,41 @@ _p = C.c_void_p
 _cp = C.c_char_p
 
 
+def _runtime_files() -> set[str]:
+    """Files of the HIP and HSA runtimes mapped into this process."""
+    found = set()
+    try:
+        with open("/proc/self/maps") as maps:
+            for line in maps:
+                path = line.split()[-1] if line.count(" ") >= 5 else ""
+                if "libamdhip64" in path or "libhsa-runtime64" in path:
+                    found.add(os.path.realpath(path))
+    except OSError:
+        pass
+    return found
+
+
+def _share_runtime() -> None:
+    """One HIP runtime per process. The library binds libamdhip64.so.7 and libhsa-runtime64.so.1 of the ROCm
+    installation. A PyTorch wheel imported afterwards asks for libamdhip64.so and libhsa-runtime64.so by their
+    unversioned names, which match no loaded object, and maps its own copies: a second HIP and HSA runtime that
+    manage the same GPU beside the library's, each with its own device memory and queues (the tests that use
+    torch then ended in GPU memory faults). Loading the installation's runtimes under the unversioned names
+    makes those requests resolve to the objects already loaded; when torch was imported first, the library's
+    own requests match torch's copies by their versioned names and nothing is loaded twice either."""
+    before = _runtime_files()
+    for name in ("libhsa-runtime64.so", "libamdhip64.so"):
+        try:
+            C.CDLL(name, mode=C.RTLD_GLOBAL)
+        except OSError:
+            pass
+    extra = _runtime_files() - before
+    if extra:
+        import sys
+
+        print(f"power_grid_model_amd: a second HIP runtime was mapped ({sorted(extra)})", file=sys.stderr)
+
+
 def _load():
     if not LIB_PATH.exists():
         raise ImportError(
@@ -22,6 +57,7 @@ def _load():
             "(the power flow runs only through the HIP library)"
         )
     lib = C.CDLL(str(LIB_PATH))
+    _share_runtime()
     sig = {
         "PGM_create_handle": (_p, []),
         "PGM_destroy_handle": (None, [_p]),

@@ -76,14 +76,8 @@ def test_device_block_lu_meshed_and_6x6():
 # ------------------------------------------------------------------------------------------------
 # reference validation cases through PowerGridModel on the GPU
 # ------------------------------------------------------------------------------------------------
-# Two runs of the GPU suite ended in a memory fault ("illegal memory access" at the wait for a graph-launched
-# Newton-Raphson iteration) in the first test after dummy-test-line-into-itself (asym batch): floating_zero_sequence
-# (asym), and with that case skipped, gaia-example (asym). Each passed in earlier runs of the suite whose power-flow
-# device code was the same instruction for instruction, and the kernels compiled as host code (address and
-# undefined-behaviour checks, one allocation per device array, device memory apart from host memory, fill patterns
-# that must not change any output) run every validation case in the suite's order without a finding. The cause is
-# not shown, so both cases stay off the GPU; the oracle still checks their fixtures on the CPU.
-GPU_FAULTED = {("floating_zero_sequence", False, False), ("gaia-example", False, False)}
+# Cases kept off the GPU (none at present)
+GPU_FAULTED = set()
 CASES = [c if c not in GPU_FAULTED else
          pytest.param(*c, marks=pytest.mark.skip(reason="memory fault on the GPU after dummy-test-line-into-itself (asym); cause not shown"))
          for c in list_cases()]
@@ -328,8 +322,6 @@ def test_workload_fp64_parity(name, make, scen):
     _compare(out, ref, case.symmetric, slice(None))
 
 
-@pytest.mark.skip(reason="a run of the GPU tests ended in a memory fault in this test (with a change to the batch's "
-                         "streams and graph capture that was then taken out); cause not shown")
 def test_dense_newton_brute_force_counts_on_gpu():
     """FP64 iteration counts and voltages of the HIP path against the dense numpy Newton brute force
     (tests/dense_newton.py) on the oracle's 100 random grids (600 scenarios, counts 1-9 and divergent
