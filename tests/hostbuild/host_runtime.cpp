@@ -1,6 +1,15 @@
 // Host runtime of the host build (tests/hostbuild): the HIP API subset the library uses, with kernel launches run
 // by host threads, synchronous streams, graphs replayed in capture order, and optional checks that device and
 // host memory are not confused (EMU_MMAP=1).
+//
+// EMU_DEFER=1: streams are queues that run only when something waits on them (a stream or event
+// synchronisation, a wait of another stream on one of their events, hipFree, hipHostFree,
+// hipDeviceSynchronize). Work on two streams then runs in the latest order their events allow, so a read of
+// the other stream's results without a wait, a write of data the other stream has still to read, and a host
+// read of device results before a synchronisation show up as wrong results (or, through EMU_MMAP and the
+// sanitizers, as faults). A graph's nodes of the capturing stream go to the stream it is launched on, those of
+// other streams to queues of their own, joined at the end of the launch. Host-to-device copies from pageable
+// memory take the source when they are issued, as HIP does.
 #include "hip/hip_runtime.h"
 
 #include <barrier>
@@ -8,6 +17,9 @@
 #include <map>
 #include <sys/mman.h>
 #include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
@@ -16,9 +28,35 @@
 
 thread_local dim3 threadIdx, blockIdx, blockDim, gridDim;
 
-struct EmuStream { int id; };
-struct EmuEvent { int id; };
-struct EmuGraph { std::vector<std::function<void()>> nodes; };
+// queued work of a stream: 0 run f, 1 event record (nothing to run), 2 wait for stream ws to reach wseq
+struct EmuOp {
+    int kind;
+    std::function<void()> f;
+    EmuStream* ws;
+    uint64_t wseq;
+};
+struct EmuStream {
+    int id;
+    std::deque<EmuOp> q;
+    uint64_t issued = 0, done = 0;
+};
+// the record that a wait refers to: stream s once it has done seq operations
+struct EmuEvent {
+    int id;
+    EmuStream* s = nullptr;
+    uint64_t seq = 0;
+};
+struct EmuNode {
+    int kind;
+    std::function<void()> f;
+    hipStream_t s;
+    hipEvent_t ev;
+};
+struct EmuGraph {
+    std::vector<EmuNode> nodes;
+    hipStream_t origin = nullptr;
+    std::map<hipStream_t, EmuStream*> branch;  // per launched graph: queues of the other streams' nodes
+};
 
 namespace {
 int fill_byte() {
@@ -31,11 +69,81 @@ int fill_byte() {
 std::mutex cap_mu;
 EmuGraph* capturing = nullptr;
 std::thread::id capture_thread;
-bool record(std::function<void()> f) {
+bool defer() {
+    static bool v = [] {
+        char const* e = std::getenv("EMU_DEFER");
+        return e && std::atoi(e) != 0;
+    }();
+    return v;
+}
+std::recursive_mutex q_mu;
+std::vector<EmuStream*> all_streams;
+void flush(EmuStream* s, uint64_t upto);
+void exec_op(EmuOp& op) {
+    if (op.kind == 0) op.f();
+    if (op.kind == 2 && op.ws) flush(op.ws, op.wseq);
+}
+// run stream s until it has done `upto` operations
+void flush(EmuStream* s, uint64_t upto) {
+    std::lock_guard<std::recursive_mutex> l(q_mu);
+    while (s->done < upto && !s->q.empty()) {
+        EmuOp op = std::move(s->q.front());
+        s->q.pop_front();
+        exec_op(op);
+        ++s->done;
+    }
+}
+void flush_all() {
+    std::lock_guard<std::recursive_mutex> l(q_mu);
+    for (bool more = true; more;) {
+        more = false;
+        for (size_t i = 0; i < all_streams.size(); ++i) {
+            EmuStream* s = all_streams[i];
+            if (s->done < s->issued) {
+                flush(s, s->issued);
+                more = true;
+            }
+        }
+    }
+}
+// issue an operation on stream s (not under capture): queued with EMU_DEFER, else run at once
+void issue(hipStream_t s, int kind, std::function<void()> f, hipEvent_t ev) {
+    std::lock_guard<std::recursive_mutex> l(q_mu);
+    EmuOp op{kind, std::move(f), nullptr, 0};
+    if (kind == 1 && ev) {
+        ev->s = s;
+        ev->seq = s ? s->issued + 1 : 0;
+    }
+    if (kind == 2 && ev) {
+        op.ws = ev->s;
+        op.wseq = ev->seq;
+    }
+    if (!s) {
+        exec_op(op);
+        return;
+    }
+    ++s->issued;
+    if (defer()) {
+        s->q.push_back(std::move(op));
+        return;
+    }
+    exec_op(op);
+    ++s->done;
+}
+// issue, or add to the graph being captured by this thread
+void submit(hipStream_t s, int kind, std::function<void()> f, hipEvent_t ev = nullptr) {
+    {
+        std::lock_guard<std::mutex> l(cap_mu);
+        if (capturing && std::this_thread::get_id() == capture_thread) {
+            capturing->nodes.push_back(EmuNode{kind, std::move(f), s, ev});
+            return;
+        }
+    }
+    issue(s, kind, std::move(f), ev);
+}
+bool capturing_here() {
     std::lock_guard<std::mutex> l(cap_mu);
-    if (!capturing || std::this_thread::get_id() != capture_thread) return false;
-    capturing->nodes.push_back(std::move(f));
-    return true;
+    return capturing && std::this_thread::get_id() == capture_thread;
 }
 
 // a team of host threads that runs one block at a time
@@ -138,7 +246,7 @@ int __shfl_xor(int v, int mask) {
     return r;
 }
 
-void emu_launch(dim3 grid, dim3 block, hipStream_t, std::function<void()> body) {
+void emu_launch(dim3 grid, dim3 block, hipStream_t st, std::function<void()> body) {
     if (grid.x == 0 || grid.y == 0 || grid.z == 0) {
         std::fprintf(stderr, "[emu] launch with an empty grid\n");
         std::abort();
@@ -147,9 +255,7 @@ void emu_launch(dim3 grid, dim3 block, hipStream_t, std::function<void()> body) 
         std::fprintf(stderr, "[emu] block too large\n");
         std::abort();
     }
-    auto f = [grid, block, body = std::move(body)] { run_grid(grid, block, body); };
-    if (record(f)) return;
-    f();
+    submit(st, 0, [grid, block, body = std::move(body)] { run_grid(grid, block, body); });
 }
 
 // EMU_MMAP=1: device allocations from a separate address region (page-padded, guard page after), so that a
@@ -197,6 +303,7 @@ hipError_t hipMalloc(void** p, size_t n) {
 }
 hipError_t hipFree(void* p) {
     if (!p) return hipSuccess;
+    flush_all();
     size_t n = 0;
     {
         std::lock_guard<std::mutex> l(alloc_mu);
@@ -223,6 +330,7 @@ hipError_t hipHostMalloc(void** p, size_t n, unsigned) {
     return hipSuccess;
 }
 hipError_t hipHostFree(void* p) {
+    flush_all();
     {
         std::lock_guard<std::mutex> l(alloc_mu);
         host_allocs.erase(reinterpret_cast<uintptr_t>(p));
@@ -245,9 +353,23 @@ hipError_t hipMemcpy(void* d, void const* s, size_t n, hipMemcpyKind k) {
     if (n) std::memmove(d, s, n);
     return hipSuccess;
 }
-hipError_t hipMemcpyAsync(void* d, void const* s, size_t n, hipMemcpyKind k, hipStream_t) {
-    if (record([=] { hipMemcpy(d, s, n, k); })) return hipSuccess;
-    return hipMemcpy(d, s, n, k);
+static bool emu_is_pinned(void const* p) {
+    uintptr_t const a = reinterpret_cast<uintptr_t>(p);
+    std::lock_guard<std::mutex> l(alloc_mu);
+    auto it = host_allocs.upper_bound(a);
+    if (it == host_allocs.begin()) return false;
+    --it;
+    return a >= it->first && a < it->first + std::max<size_t>(it->second, 1);
+}
+hipError_t hipMemcpyAsync(void* d, void const* s, size_t n, hipMemcpyKind k, hipStream_t st) {
+    if (defer() && !capturing_here() && n && !emu_is_device(s) && !emu_is_pinned(s)) {
+        // a pageable source is taken when the copy is issued
+        auto src = std::make_shared<std::vector<char>>(static_cast<char const*>(s), static_cast<char const*>(s) + n);
+        submit(st, 0, [=] { hipMemcpy(d, src->data(), n, k == hipMemcpyDefault ? k : hipMemcpyHostToDevice); });
+        return hipSuccess;
+    }
+    submit(st, 0, [=] { hipMemcpy(d, s, n, k); });
+    return hipSuccess;
 }
 hipError_t hipMemcpy2D(void* d, size_t dp, void const* s, size_t sp, size_t w, size_t h, hipMemcpyKind) {
     for (size_t r = 0; r < h; ++r) std::memmove(static_cast<char*>(d) + r * dp, static_cast<char const*>(s) + r * sp, w);
@@ -261,51 +383,65 @@ hipError_t hipMemset(void* d, int v, size_t n) {
     if (n) std::memset(d, v, n);
     return hipSuccess;
 }
-hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) {
-    if (record([=] { hipMemset(d, v, n); })) {
-        std::fprintf(stderr, "[emu] memset recorded into a graph\n");
-        return hipSuccess;
-    }
-    return hipMemset(d, v, n);
+hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t st) {
+    if (capturing_here()) std::fprintf(stderr, "[emu] memset recorded into a graph\n");
+    submit(st, 0, [=] { hipMemset(d, v, n); });
+    return hipSuccess;
 }
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
+    std::lock_guard<std::recursive_mutex> l(q_mu);
     *s = new EmuStream{1};
+    all_streams.push_back(*s);
     return hipSuccess;
 }
 hipError_t hipStreamCreate(hipStream_t* s) { return hipStreamCreateWithFlags(s, 0); }
 hipError_t hipStreamDestroy(hipStream_t s) {
-    delete s;
+    // the stream's work still runs; the object stays, since waits may refer to it
+    if (s) flush(s, s->issued);
     return hipSuccess;
 }
-hipError_t hipStreamSynchronize(hipStream_t) {
-    std::lock_guard<std::mutex> l(cap_mu);
-    if (capturing && std::this_thread::get_id() == capture_thread) {
+hipError_t hipStreamSynchronize(hipStream_t s) {
+    if (capturing_here()) {
         std::fprintf(stderr, "[emu] synchronize during capture\n");
         return hipErrorUnknown;
     }
+    if (s)
+        flush(s, s->issued);
+    else
+        flush_all();
     return hipSuccess;
 }
-hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned) {
+    submit(s, 2, nullptr, e);
+    return hipSuccess;
+}
 hipError_t hipEventCreate(hipEvent_t* e) {
     *e = new EmuEvent{1};
     return hipSuccess;
 }
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
-hipError_t hipEventDestroy(hipEvent_t e) {
-    delete e;
+hipError_t hipEventDestroy(hipEvent_t) {
+    // kept: nodes of graphs may still name it
     return hipSuccess;
 }
-hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
-hipError_t hipEventRecordWithFlags(hipEvent_t, hipStream_t, unsigned) { return hipSuccess; }
-hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t s) {
+    submit(s, 1, nullptr, e);
+    return hipSuccess;
+}
+hipError_t hipEventRecordWithFlags(hipEvent_t e, hipStream_t s, unsigned) { return hipEventRecord(e, s); }
+hipError_t hipEventSynchronize(hipEvent_t e) {
+    if (e && e->s) flush(e->s, e->seq);
+    return hipSuccess;
+}
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
     *ms = 0.001f;
     return hipSuccess;
 }
-hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
     std::lock_guard<std::mutex> l(cap_mu);
     if (capturing) return hipErrorUnknown;
     capturing = new EmuGraph;
+    capturing->origin = s;
     capture_thread = std::this_thread::get_id();
     return hipSuccess;
 }
@@ -319,8 +455,26 @@ hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, hipGraphNode_t*,
     *e = new EmuGraph(*g);
     return hipSuccess;
 }
-hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
-    for (auto const& f : e->nodes) f();
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t s) {
+    std::lock_guard<std::recursive_mutex> l(q_mu);
+    for (auto const& n : e->nodes) {
+        hipStream_t t = s;
+        if (defer() && n.s != e->origin) {
+            auto it = e->branch.find(n.s);
+            if (it == e->branch.end()) {
+                hipStream_t b;
+                hipStreamCreateWithFlags(&b, 0);
+                it = e->branch.emplace(n.s, b).first;
+            }
+            t = it->second;
+        }
+        issue(t, n.kind, n.f, n.ev);
+    }
+    // the launch is complete once every branch is
+    for (auto const& [k, b] : e->branch) {
+        EmuEvent end{0, b, b->issued};
+        issue(s, 2, nullptr, &end);
+    }
     return hipSuccess;
 }
 hipError_t hipGraphDestroy(hipGraph_t g) {
@@ -328,13 +482,17 @@ hipError_t hipGraphDestroy(hipGraph_t g) {
     return hipSuccess;
 }
 hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
+    flush_all();
     delete e;
     return hipSuccess;
 }
 hipError_t hipGetLastError() { return hipSuccess; }
 hipError_t hipPeekAtLastError() { return hipSuccess; }
 const char* hipGetErrorString(hipError_t) { return "emulated error"; }
-hipError_t hipDeviceSynchronize() { return hipSuccess; }
+hipError_t hipDeviceSynchronize() {
+    flush_all();
+    return hipSuccess;
+}
 hipError_t hipGetDeviceCount(int* n) {
     *n = 1;
     return hipSuccess;
