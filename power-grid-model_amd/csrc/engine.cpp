@@ -1203,6 +1203,10 @@ struct UpdateReader {
 }  // namespace
 
 void Batch::release() {
+    // no work of this batch may still run once its chunks go back to the model's pool (an error thrown
+    // between a launch and its synchronisation leaves work queued)
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (stream_u && stream_u != stream) (void)hipStreamSynchronize(stream_u);
     destroy_graphs();
     for (int k = 0; k < 2; ++k)
         if (ev_iter[k]) (void)hipEventDestroy(ev_iter[k]);
@@ -2793,7 +2797,10 @@ int device_block_lu(Idx nb, Idx n, Idx const* row_indptr, Idx const* col_indices
     PGM_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     struct StreamGuard {
         hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
+        ~StreamGuard() {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
     } guard{st};
     Arena A;
     return block_lu_impl(A, st, nb, n, row_indptr, col_indices, diag_lu, batch, data, rhs, x, status, fp32);

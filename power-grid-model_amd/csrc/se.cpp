@@ -656,7 +656,10 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
     SE_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     struct StreamGuard {
         hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
+        ~StreamGuard() {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
     } guard{st};
     double const err_tol = opt.err_tol;
     Idx const max_iter = opt.max_iter;
@@ -807,6 +810,12 @@ void run_state_estimation(Model const& m, PGM_Options const& opt, PGM_MutableDat
         // ---- device ----
         Arena A;
         A.set_pool(pool);
+        // destroyed before the arena: the work queued on the stream ends before its chunks go back to the
+        // model's pool (an error thrown between a launch and its synchronisation leaves work queued)
+        struct DrainGuard {
+            hipStream_t s;
+            ~DrainGuard() { (void)hipStreamSynchronize(s); }
+        } drain{st};
         SeDev d;
         d.n = n;
         d.n_mm = nmm;

@@ -141,6 +141,22 @@ void submit(hipStream_t s, int kind, std::function<void()> f, hipEvent_t ev = nu
     }
     issue(s, kind, std::move(f), ev);
 }
+// live objects of the HIP API, and their peaks (EMU_COUNT=1: reported at exit)
+struct Live {
+    long now = 0, peak = 0;
+    void up() { peak = std::max(peak, ++now); }
+    void down() { --now; }
+};
+Live live_stream, live_event, live_exec, live_dev, live_pinned;
+struct LiveReport {
+    ~LiveReport() {
+        if (!std::getenv("EMU_COUNT")) return;
+        std::fprintf(stderr, "[emu] live at exit (peak): streams %ld (%ld) events %ld (%ld) graph execs %ld (%ld) "
+                     "device allocations %ld (%ld) pinned allocations %ld (%ld)\n", live_stream.now, live_stream.peak,
+                     live_event.now, live_event.peak, live_exec.now, live_exec.peak, live_dev.now, live_dev.peak,
+                     live_pinned.now, live_pinned.peak);
+    }
+} live_report;
 bool capturing_here() {
     std::lock_guard<std::mutex> l(cap_mu);
     return capturing && std::this_thread::get_id() == capture_thread;
@@ -297,6 +313,7 @@ hipError_t hipMalloc(void** p, size_t n) {
     {
         std::lock_guard<std::mutex> l(alloc_mu);
         dev_allocs[reinterpret_cast<uintptr_t>(m)] = n;
+        live_dev.up();
     }
     *p = m;
     return hipSuccess;
@@ -314,6 +331,7 @@ hipError_t hipFree(void* p) {
         }
         n = it->second;
         dev_allocs.erase(it);
+        live_dev.down();
     }
     if (use_mmap()) {
         size_t const pg = 4096, len = (n + pg - 1) / pg * pg + pg;
@@ -327,13 +345,14 @@ hipError_t hipHostMalloc(void** p, size_t n, unsigned) {
     *p = std::malloc(n ? n : 1);
     std::lock_guard<std::mutex> l(alloc_mu);
     host_allocs[reinterpret_cast<uintptr_t>(*p)] = n;
+    live_pinned.up();
     return hipSuccess;
 }
 hipError_t hipHostFree(void* p) {
     flush_all();
     {
         std::lock_guard<std::mutex> l(alloc_mu);
-        host_allocs.erase(reinterpret_cast<uintptr_t>(p));
+        if (host_allocs.erase(reinterpret_cast<uintptr_t>(p))) live_pinned.down();
     }
     std::free(p);
     return hipSuccess;
@@ -392,12 +411,14 @@ hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
     std::lock_guard<std::recursive_mutex> l(q_mu);
     *s = new EmuStream{1};
     all_streams.push_back(*s);
+    live_stream.up();
     return hipSuccess;
 }
 hipError_t hipStreamCreate(hipStream_t* s) { return hipStreamCreateWithFlags(s, 0); }
 hipError_t hipStreamDestroy(hipStream_t s) {
     // the stream's work still runs; the object stays, since waits may refer to it
     if (s) flush(s, s->issued);
+    live_stream.down();
     return hipSuccess;
 }
 hipError_t hipStreamSynchronize(hipStream_t s) {
@@ -417,11 +438,13 @@ hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned) {
 }
 hipError_t hipEventCreate(hipEvent_t* e) {
     *e = new EmuEvent{1};
+    live_event.up();
     return hipSuccess;
 }
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
 hipError_t hipEventDestroy(hipEvent_t) {
     // kept: nodes of graphs may still name it
+    live_event.down();
     return hipSuccess;
 }
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s) {
@@ -453,6 +476,7 @@ hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
 }
 hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, hipGraphNode_t*, char*, size_t) {
     *e = new EmuGraph(*g);
+    live_exec.up();
     return hipSuccess;
 }
 hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t s) {
@@ -483,6 +507,11 @@ hipError_t hipGraphDestroy(hipGraph_t g) {
 }
 hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
     flush_all();
+    live_exec.down();
+    for (auto const& kv : e->branch) {
+        (void)kv;
+        live_stream.down();
+    }
     delete e;
     return hipSuccess;
 }
