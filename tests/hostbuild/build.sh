@@ -7,12 +7,14 @@
 # uses AddressSanitizer and UndefinedBehaviorSanitizer, and the arena takes one allocation per request so every
 # device array has its own bounds. Run the GPU tests against it with ./run.sh (EMU_MMAP=1: device memory in an
 # address range apart from host memory, with pointer and copy-direction checks; EMU_FILL=<byte>: fill of fresh
-# device memory; EMU_DEFER=1: streams run only when waited on, so missing waits between streams show). Output: tests/hostbuild/_build/libpower_grid_model_c.so.
+# device memory; EMU_DEFER=1: streams run only when waited on, so missing waits between streams show; EMU_DEFER=2:
+# a destroyed stream's work runs only at the next device-wide synchronisation). Output: tests/hostbuild/_build/libpower_grid_model_c.so.
 set -e
 cd "$(dirname "$0")"
 rm -rf _build && mkdir -p _build/src _build/obj
 cp ../../power-grid-model_amd/csrc/* _build/src/
-python3 - <<'PY'
+# POOL=1 keeps the library's own arena (256 MiB chunks) and chunk pool instead
+[ "${POOL:-0}" = 1 ] || python3 - <<'PY'
 p='_build/src/engine.cpp'
 s=open(p).read()
 a=s.index('void* Arena::take(size_t bytes) {')

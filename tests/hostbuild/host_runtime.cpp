@@ -416,8 +416,13 @@ hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
 }
 hipError_t hipStreamCreate(hipStream_t* s) { return hipStreamCreateWithFlags(s, 0); }
 hipError_t hipStreamDestroy(hipStream_t s) {
-    // the stream's work still runs; the object stays, since waits may refer to it
-    if (s) flush(s, s->issued);
+    // the object stays, since waits may refer to it; its queued work runs now, or (EMU_DEFER=2) at the next
+    // device-wide synchronisation, as a destroyed stream's work does on the device
+    static bool const late = [] {
+        char const* e = std::getenv("EMU_DEFER");
+        return e && std::atoi(e) >= 2;
+    }();
+    if (s && !late) flush(s, s->issued);
     live_stream.down();
     return hipSuccess;
 }
